@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark: Lagrangian f+grad evaluations per second on the metric instance (one MI355X).
+
+Step = one f+grad evaluation (f as an FP64 scalar and G = C(y) V, n x r) at fixed multipliers, rho = 0.02,
+rt = rho * tri_weight (tri_weight 1), on the resident instance: Chung-Lu, n ~ 1M, gamma 2.1, mean degree ~13,
+r = 16, T ~ 3.4M wedge triangles, one pair level and one VCS level (P = 64, 4 working-set entries per row), FP32 V.
+Inputs are resident in HBM before the timed region; the timed region is K back-to-back evaluations bracketed by
+device synchronisation.
+
+Also reported on the same instance: the HIP path with BF16-stored V and with FP64 V, the plain FP64 GPU form
+(torch FP64 CSR SpMM + gathers/index_add), and the FP64 C/OpenMP oracle on the host cores (cpu_baseline).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-extra]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Lagrangian f+grad evals/s (n=1M,r=16) vs FP64; batched bound p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def alg_bytes(inst, r, s_v, s_g, s_c):
+    """Compulsory bytes of one evaluation: every input read once, every output written once.
+    V + G + col + rowptr (+ weights) + d + labels + triangles (TI + lam) + VCS (ptr + col + sig)."""
+    n, nnz, T = inst.n, inst.nnz, inst.T
+    b = n * r * s_v + n * r * s_g + nnz * 4 + (n + 1) * 8 + n * s_c
+    if inst.w is not None:
+        b += nnz * s_c
+    b += (len(inst.levels) + len(inst.vcs)) * n * 4
+    b += T * (12 + s_c)
+    for v in inst.vcs:
+        b += (n + 1) * 8 + v.nnz * (4 + s_c)
+    return b
+
+
+def time_engine(eng, rho, steps, warmup):
+    import numpy as np  # noqa: F401
+    for _ in range(warmup):
+        eng.eval(rho, fetch=False)
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.eval(rho, fetch=False)
+    eng.sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--rho", type=float, default=0.02)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the host-CPU oracle baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the BF16 / FP64 / plain-FP64 comparison runs")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    from lagbound_amd import instances as I
+    from lagbound_amd.engine import Engine
+
+    t0 = time.time()
+    inst, V = I.metric_instance(n=a.n)
+    r = V.shape[1]
+    log(f"instance n={inst.n} nnz={inst.nnz} T={inst.T} ws={inst.vcs[0].nnz} built in {time.time() - t0:.1f}s")
+    rho = a.rho
+
+    eng = Engine(r=r, dtype="f32")
+    eng.set_instance(inst)
+    eng.set_V(V)
+    eng.eval(rho)
+    sec = time_engine(eng, rho, a.steps, a.warmup)
+    value = 1.0 / sec
+    log(f"f32: {sec * 1e3:.4f} ms/eval, {value:.1f} evals/s")
+    ms_ev, kms = eng.time_eval(rho, iters=max(20, a.steps // 4), per_kernel=True)
+    log("kernel ms:", json.dumps({k: round(v, 5) for k, v in kms.items()}))
+    B = alg_bytes(inst, r, 4, 4, 4)
+    ach = B / (ms_ev * 1e-3) / 1e9
+    dom = max(kms, key=kms.get)
+    rows_bytes = None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "eval_metric_f32_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_eval")
+        except Exception:
+            traffic = None
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "evals/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(sec * 1e3, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "f+grad eval, Chung-Lu gamma=2.1 mean-degree~13 (metric instance M)", "n": inst.n,
+                   "nnz": inst.nnz, "r": r, "T": inst.T, "pair_P": inst.levels[0].P, "vcs_P": inst.vcs[0].P,
+                   "vcs_ws": inst.vcs[0].nnz, "rho": rho, "V": "fp32, fp64 global sums"},
+        "roofline": {"bound": "hbm", "kernel": "one f+grad evaluation (lb_k_colsum .. lb_k_finalize chain)",
+                     "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "alg_bytes": B, "ms_per_eval_events": round(ms_ev, 5),
+                     "dominant_kernel": dom, "kernel_ms": {k: round(v, 5) for k, v in kms.items()}},
+    }
+    del rows_bytes
+    if not a.no_extra:
+        extra = {}
+        for dt, sv, sc in (("bf16", 2, 4), ("f64", 8, 8)):
+            e2 = Engine(r=r, dtype=dt)
+            e2.set_instance(inst)
+            e2.set_V(V)
+            s2 = time_engine(e2, rho, max(20, a.steps // 2), 5)
+            ms2, _ = e2.time_eval(rho, iters=20, per_kernel=False)
+            b2 = alg_bytes(inst, r, sv, 4 if dt == "bf16" else 8, sc)
+            extra[dt] = {"evals_per_s": round(1.0 / s2, 2), "ms_per_eval": round(s2 * 1e3, 5),
+                         "hbm_frac": round(b2 / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            log(f"{dt}: {s2 * 1e3:.4f} ms/eval")
+            e2.close()
+        try:
+            import torch
+            from lagbound_amd.baseline_fp64 import PlainFP64
+            pl = PlainFP64(inst, device="cuda")
+            Vt = torch.as_tensor(V, dtype=torch.float64, device="cuda")
+            for _ in range(3):
+                pl.eval(Vt, rho)
+            torch.cuda.synchronize()
+            k = 20
+            t1 = time.perf_counter()
+            for _ in range(k):
+                f, G = pl.eval(Vt, rho)
+            torch.cuda.synchronize()
+            s3 = (time.perf_counter() - t1) / k
+            extra["plain_fp64_gpu"] = {"evals_per_s": round(1.0 / s3, 2), "ms_per_eval": round(s3 * 1e3, 4),
+                                       "form": "torch FP64 CSR SpMM + gathers/index_add"}
+            out["speedup_vs_plain_fp64_gpu"] = round(value * s3, 2)
+            log(f"plain fp64 gpu: {s3 * 1e3:.3f} ms/eval")
+            del pl, Vt, f, G
+        except Exception as exc:  # the comparison leg is reported, never silently replaced
+            extra["plain_fp64_gpu"] = {"error": repr(exc)[:200]}
+        out["variants"] = extra
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        cores = os.cpu_count() or 1
+        inc = O.incidence_c(inst.n, inst.TI)
+        O.eval_c(inst, V, rho, threads=cores, incidence=inc)   # warm-up (page-in)
+        ts = []
+        t_start = time.perf_counter()
+        while len(ts) < 3 or (time.perf_counter() - t_start < a.cpu_seconds and len(ts) < 50):
+            t1 = time.perf_counter()
+            O.eval_c(inst, V, rho, threads=cores, incidence=inc)
+            ts.append(time.perf_counter() - t1)
+        med = float(np.median(ts))
+        out["cpu_baseline"] = {"value": round(1.0 / med, 4), "unit": "evals/s", "cores": cores, "kind": "port",
+                               "sample": f"{len(ts)} full FP64 evaluations of the same instance (C/OpenMP oracle, "
+                                         f"reference loop structure), median {med:.3f}s"}
+        out["speedup_vs_cpu"] = round(value * med, 1)
+        log(f"cpu oracle ({cores} threads): {med:.3f} s/eval")
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

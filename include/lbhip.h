@@ -1,0 +1,113 @@
+/* lbhip.h -- C ABI of the MI355X (gfx950) ALM search library (liblbhip.so).
+ *
+ * Replaces the numba kernels that lagbound's solver stage runs on the CPU (reference: src/lagbound/_core/alm_vcs.py,
+ * aggops.py, vcsops.py, est_dual_vcs.py). The reference has no C ABI for this path: its only FFI precedent is
+ * cholcert.c (src/lagbound/_core/cholcert.c:24-76, bound with ctypes at certify_lifted_vc.py:395-408), whose
+ * conventions this header keeps: plain C, 64-bit sizes, caller-owned host arrays borrowed for the call only, an int
+ * status (0 ok, < 0 usage / allocation / HIP error, > 0 numerical), no exceptions across the ABI.
+ *
+ * A context owns one device stream and every device buffer (graph, levels, working sets, triangle pool, V, G,
+ * multipliers, scratch). V and the multipliers stay resident on the device between calls. Calls on one context are
+ * serialised by the caller (one context per host thread); several contexts may share a device.
+ *
+ * Storage types of V (`vdtype`): LB_F32 (FP32 V, FP32 arithmetic, FP64 accumulation of every global sum),
+ * LB_BF16 (BF16-stored V, FP32 arithmetic, FP64 accumulation), LB_F64 (FP64 throughout).
+ */
+#ifndef LBHIP_H
+#define LBHIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LB_F32 0
+#define LB_BF16 1
+#define LB_F64 2
+
+#define LB_NPARTS 12
+/* parts[] written by lb_eval_fgrad (same layout as the FP64 oracle):
+ * [0] f  [1] quadA (rho > 0) or sum_i G_i.V_i (rho = 0)  [2] |d^T V|^2 / L  [3] sum of pair-level PHR terms
+ * [4] triangle PHR term  [5] VCS PHR term  [6] primal <B, V V^T> + C0  [7] C0  [8] #(h_QR < -1e-9)
+ * [9] min triangle slack 1 - tau  [10] #(g_iR < -1e-9)  [11] sum(lam) */
+
+typedef struct lb_ctx lb_ctx;
+
+/* Create a context on HIP device `device` for rank `r` (8, 16 or 32) embeddings stored as `vdtype`.
+ * (No reference counterpart: the reference keeps everything in one numba process, alm_vcs.py:1103-1117.) */
+int lb_create(lb_ctx **out, int device, int vdtype, int r);
+void lb_destroy(lb_ctx *ctx);
+/* Last error message of the context (static text owned by the context). */
+const char *lb_last_error(const lb_ctx *ctx);
+
+/* Graph: symmetric CSR with sorted rows (rowptr int64 n+1, col int32), optional FP64 edge weights (NULL =
+ * unweighted), vertex volumes d (FP64), L = original 2m, C0 = 2 * inside edges.
+ * Replaces the instance load of alm_vcs.py:1103-1117 (A, d, L, C0). */
+int lb_set_graph(lb_ctx *ctx, int64_t n, const int64_t *rowptr, const int32_t *col, const double *weights,
+                 const double *d, double L, double C0);
+
+/* Pair levels (aggops.PairLevel, aggops.py:51-68; rq, r: alm_vcs.py:1254-1259): labels (nlev x n, int32, values in
+ * [0, P_l)), P (nlev), sig (concatenated P_l x P_l FP64, symmetric), rq and dsq (concatenated P_l FP64),
+ * pairlvl_weight w (r_QR = rq_Q rq_R w). */
+int lb_set_levels(lb_ctx *ctx, int nlev, const int32_t *labs, const int32_t *P, const double *sig,
+                  const double *rq, const double *dsq, double pairlvl_weight);
+/* Replace the pair-level multipliers (concatenated P_l^2). */
+int lb_set_level_sig(lb_ctx *ctx, const double *sig);
+
+/* Sparse vertex-cluster levels (vcsops.VCSLevel, vcsops.py:31-70): labels (nvcs x n), P (nvcs), rq (concatenated),
+ * working sets as row CSR: ptr (nvcs x (n+1), each starting at 0), col (concatenated, sorted per row), sig
+ * (concatenated FP64); wv = vc_weight * pairlvl_weight (r_iR = wv rq_R / d_i^2). */
+int lb_set_vcs(lb_ctx *ctx, int nvcs, const int32_t *labs, const int32_t *P, const double *rq, const int64_t *ptr,
+               const int32_t *col, const double *sig, double wv);
+int lb_set_vcs_sig(lb_ctx *ctx, const double *sig);
+
+/* Triangle pool: TI (T x 3 int32, (a, centre, c)), multipliers lam (FP64). Builds the vertex -> (t, role)
+ * incidence in t order (alm_vcs.py:94-104) and the incidence-ordered records (inc_abc / inc_lam,
+ * alm_vcs.py:353-360) on the device side. */
+int lb_set_triangles(lb_ctx *ctx, int64_t T, const int32_t *TI, const double *lam);
+int lb_set_lam(lb_ctx *ctx, const double *lam);
+
+/* V (n x r row-major) from / to host FP64 or FP32 (`host_dtype` LB_F64 / LB_F32). With LB_BF16 storage lb_set_V
+ * rounds to nearest even and lb_get_V returns the stored (dequantised) values. */
+int lb_set_V(lb_ctx *ctx, const void *V, int host_dtype);
+int lb_get_V(lb_ctx *ctx, void *V, int host_dtype);
+
+/* One Lagrangian f+grad evaluation at the resident V and multipliers (the metric's unit of work):
+ *   rho > 0: f = augmented(V, rho) (alm_vcs.py:1496-1533) with rt = rho * tri_weight;
+ *   rho = 0: f = C0 + sum(lam) + sum_i (C V)_i.V_i (est_dual_vcs.value, est_dual_vcs.py:245-246);
+ *   G = C(y) V (CV, alm_vcs.py:1299-1312) with y the PHR effective multipliers (pp, pt, pv), kept on the device.
+ * parts (LB_NPARTS doubles, host) may be NULL to leave the scalars on the device (asynchronous call). */
+int lb_eval_fgrad(lb_ctx *ctx, double rho, double rt, double *parts);
+/* Copy G (n x r) to host as FP64 or FP32. */
+int lb_get_G(lb_ctx *ctx, void *G, int host_dtype);
+/* Copy the effective multipliers of the last evaluation: pp (concatenated P_l^2), pt (T), pv (concatenated nnz);
+ * any pointer may be NULL. */
+int lb_get_multipliers(lb_ctx *ctx, double *pp, double *pt, double *pv);
+
+/* PHR multiplier update on the device: sig <- pp, lam <- pt, VCS sig <- pv of the last rho > 0 evaluation
+ * (alm_vcs.py:1704-1711,1723). */
+int lb_phr_update(lb_ctx *ctx);
+
+/* Wait for the context's stream. */
+int lb_sync(lb_ctx *ctx);
+
+/* Timing of the evaluation: runs `iters` evaluations back to back on the context's stream and returns the mean
+ * milliseconds per evaluation (whole evaluation, HIP events) and per kernel (`kernel_ms`, LB_NKERNELS entries,
+ * events around each launch in a separate pass; NULL to skip). No reference counterpart (the reference logs
+ * wall-clock accumulators, alm_vcs.py:1350-1351,1779). */
+#define LB_NKERNELS 8
+int lb_time_eval(lb_ctx *ctx, double rho, double rt, int iters, double *ms_per_eval, double *kernel_ms);
+/* Name of kernel slot k of lb_time_eval (static string). */
+const char *lb_kernel_name(int k);
+
+/* Tuning knobs of the row kernels: rows with more than `heavy_edges` neighbours or `heavy_tris` triangle records
+ * are split into segments of `seg_len` items processed by whole wavefronts. Takes effect at the next
+ * lb_set_graph / lb_set_triangles. */
+int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len);
+
+/* Library version string. */
+const char *lb_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

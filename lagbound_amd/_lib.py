@@ -1,0 +1,75 @@
+"""ctypes binding of liblbhip.so (include/lbhip.h). The product path has no CPU fallback: if the library is missing
+or no HIP device is present, every entry point raises."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liblbhip.so")
+
+LB_F32, LB_BF16, LB_F64 = 0, 1, 2
+LB_NPARTS = 12
+LB_NKERNELS = 8
+
+# every symbol include/lbhip.h declares
+EXPORTS = ["lb_create", "lb_destroy", "lb_last_error", "lb_set_graph", "lb_set_levels", "lb_set_level_sig",
+           "lb_set_vcs", "lb_set_vcs_sig", "lb_set_triangles", "lb_set_lam", "lb_set_V", "lb_get_V",
+           "lb_eval_fgrad", "lb_get_G", "lb_get_multipliers", "lb_phr_update", "lb_sync", "lb_time_eval",
+           "lb_kernel_name", "lb_set_tuning", "lb_version"]
+
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class LBError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load liblbhip.so (raises LBError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise LBError(f"{LIBPATH} is missing: build it with `python -m lagbound_amd.build` (hipcc, gfx950)")
+    L = C.CDLL(LIBPATH)
+    ctx = C.c_void_p
+    sig = {
+        "lb_create": ([C.POINTER(ctx), C.c_int, C.c_int, C.c_int], C.c_int),
+        "lb_destroy": ([ctx], None),
+        "lb_last_error": ([ctx], C.c_char_p),
+        "lb_set_graph": ([ctx, C.c_int64, _i64p, _i32p, _f64p, _f64p, C.c_double, C.c_double], C.c_int),
+        "lb_set_levels": ([ctx, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double], C.c_int),
+        "lb_set_level_sig": ([ctx, _f64p], C.c_int),
+        "lb_set_vcs": ([ctx, C.c_int, _i32p, _i32p, _f64p, _i64p, _i32p, _f64p, C.c_double], C.c_int),
+        "lb_set_vcs_sig": ([ctx, _f64p], C.c_int),
+        "lb_set_triangles": ([ctx, C.c_int64, _i32p, _f64p], C.c_int),
+        "lb_set_lam": ([ctx, _f64p], C.c_int),
+        "lb_set_V": ([ctx, _vp, C.c_int], C.c_int),
+        "lb_get_V": ([ctx, _vp, C.c_int], C.c_int),
+        "lb_eval_fgrad": ([ctx, C.c_double, C.c_double, _f64p], C.c_int),
+        "lb_get_G": ([ctx, _vp, C.c_int], C.c_int),
+        "lb_get_multipliers": ([ctx, _f64p, _f64p, _f64p], C.c_int),
+        "lb_phr_update": ([ctx], C.c_int),
+        "lb_sync": ([ctx], C.c_int),
+        "lb_time_eval": ([ctx, C.c_double, C.c_double, C.c_int, _f64p, _f64p], C.c_int),
+        "lb_kernel_name": ([C.c_int], C.c_char_p),
+        "lb_set_tuning": ([ctx, C.c_int, C.c_int, C.c_int], C.c_int),
+        "lb_version": ([], C.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else C.cast(None, t)

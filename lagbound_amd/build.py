@@ -1,0 +1,32 @@
+"""Builds liblbhip.so (the HIP kernels + C ABI) in-tree for gfx950 with hipcc."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc", "lbhip.hip")
+OUT = os.path.join(HERE, "liblbhip.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-munsafe-fp-atomics",
+               # no packed FP32 instructions (v_pk_*_f32): plain scalar FP32 arithmetic only
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-Wno-unused-value", "-Wno-unused-result"]
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every HIP source of the package into liblbhip.so (always recompiles)."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    cmd = [hipcc] + HIPCC_FLAGS + ["-I", INCLUDE, "-o", OUT + ".tmp", SRC]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + res.stderr[-4000:])
+    os.replace(OUT + ".tmp", OUT)
+    if verbose:
+        print(" ".join(cmd))
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,1402 @@
+// liblbhip: MI355X (gfx950) kernels of lagbound's augmented-Lagrangian low-rank SDP search, behind the C ABI of
+// include/lbhip.h.
+//
+// One f+grad evaluation (alm_vcs.py:1496-1533 `augmented` + alm_vcs.py:1299-1312 `CV`) is a chain of kernels on the
+// context's stream:
+//   lb_k_colsum   z = d^T V and the cluster sums Y = W V of every label set (pair + VCS levels), FP64, one pass
+//                 over V (cluster_sums, alm_vcs.py:511-526); per-block partials in LDS, fixed-order reduction.
+//   lb_k_levels   per pair-level cluster row Q: h_QR, PHR multipliers pp, PHR value, M_Q = sum_R pp_QR Y_R
+//                 (level_aug alm_vcs.py:476-496; pair_all's M alm_vcs.py:378-415), FP64.
+//   lb_k_vcs_z    per vertex: VCS effective multipliers pv over the working set, PHR value, and the transposed
+//                 sums Z_R = sum_j pv_jR d_j V_j (_vcs_aug_rows vcsops.py:266-288; vcs_apply vcsops.py:326-330).
+//   lb_k_segs     wavefront-per-segment pass over power-law hubs: rows whose neighbour list or triangle
+//                 incidence is long are cut into segments; each wavefront sums one segment.
+//   lb_k_rows     the hot kernel: per vertex, one slot of S lanes (16 bytes of the V row per lane) computes
+//                 G_i = (A V)_i - d_i z / L + sum_l d_i (M_Q - pp_QQ d_i V_i) + VCS row terms + T(pt) V rows,
+//                 quadA_i = V_i.(A V)_i, and (for role-0 triangle records) pt_t and the triangle PHR value.
+//   lb_k_finalize fixed-order FP64 reduction of every per-block partial into the parts[] scalars.
+// Global sums are FP64; per-row arithmetic is FP32 (LB_F32, LB_BF16) or FP64 (LB_F64).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbhip.h"
+
+#define LB_MAXLEV 8
+#define LB_MAXVCS 4
+#define LB_RED_BLOCKS 512
+#define LB_BLOCK 256
+
+typedef uint16_t bf16raw;
+
+// ------------------------------------------------------------------------------------------------ storage traits
+template <typename TV> struct VT;
+template <> struct VT<float> {
+    typedef float TC;
+    static constexpr int CPL = 4;  // columns per lane: 16 bytes of the row
+    __device__ static inline void unpack(const uint4 &u, float *x) {
+        x[0] = __uint_as_float(u.x); x[1] = __uint_as_float(u.y); x[2] = __uint_as_float(u.z); x[3] = __uint_as_float(u.w);
+    }
+};
+template <> struct VT<bf16raw> {
+    typedef float TC;
+    static constexpr int CPL = 8;
+    __device__ static inline void unpack(const uint4 &u, float *x) {
+        x[0] = __uint_as_float(u.x << 16); x[1] = __uint_as_float(u.x & 0xffff0000u);
+        x[2] = __uint_as_float(u.y << 16); x[3] = __uint_as_float(u.y & 0xffff0000u);
+        x[4] = __uint_as_float(u.z << 16); x[5] = __uint_as_float(u.z & 0xffff0000u);
+        x[6] = __uint_as_float(u.w << 16); x[7] = __uint_as_float(u.w & 0xffff0000u);
+    }
+};
+template <> struct VT<double> {
+    typedef double TC;
+    static constexpr int CPL = 2;
+    __device__ static inline void unpack(const uint4 &u, double *x) {
+        x[0] = __hiloint2double((int)u.y, (int)u.x); x[1] = __hiloint2double((int)u.w, (int)u.z);
+    }
+};
+
+template <typename TC> struct alignas(16) TriRec { int32_t u, w, code; TC lam; };
+template <> struct alignas(8) TriRec<double> { int32_t u, w, code, pad; double lam; };
+
+struct Seg { int32_t row, kind; int64_t b, e; };
+
+template <int S, typename T> __device__ inline T slot_sum(T v) {
+#pragma unroll
+    for (int m = 1; m < S; m <<= 1) v += __shfl_xor(v, m);
+    return v;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+    return v;
+}
+__device__ inline double wave_min(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m));
+    return v;
+}
+// Block sum in a fixed order (wave trees, then waves in order); valid in thread 0.
+__device__ inline double block_sum(double v, double *sh) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += sh[k];
+    return s;
+}
+__device__ inline double block_min(double v, double *sh) {
+    v = wave_min(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double s = INFINITY;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) s = fmin(s, sh[k]);
+    return s;
+}
+
+// ------------------------------------------------------------------------------------------------ kernel args
+struct LabelSets {             // every label set whose cluster sums are needed (pair levels, then VCS levels)
+    int nsets;
+    int off[LB_MAXLEV + LB_MAXVCS + 1];   // cluster offsets into Y (rows of R doubles)
+    const int32_t *lab[LB_MAXLEV + LB_MAXVCS];
+};
+
+struct ColsumArgs {
+    const uint4 *V; int64_t n; const double *d;
+    LabelSets ls; int sumP; int use_lds;
+    double *part;              // gridDim x (R + sumP R) when use_lds, else gridDim x R
+    double *Yglob;             // sumP x R (atomic path)
+};
+
+struct LevelArgs {
+    int nlev; int R;
+    int Poff[LB_MAXLEV + 1];   // cluster offsets (global ids, rows of Y)
+    int64_t soff[LB_MAXLEV + 1];
+    const double *Y; const double *sig; double *pp; const double *rq; const double *dsq;
+    double w, rho;
+    double *M; double *pdiag; double *tot; double *viol;   // tot/viol per cluster row
+};
+
+struct VcsArgs {
+    const uint4 *V; int64_t n; const double *d;
+    int nvcs; int Yoff[LB_MAXVCS]; int Zoff[LB_MAXVCS + 1];
+    const int32_t *lab[LB_MAXVCS]; const double *rq[LB_MAXVCS];
+    const int64_t *ptr[LB_MAXVCS];   // absolute entry offsets
+    const int32_t *col; const void *sig; void *pv;
+    const double *Y; double wv, rho; int sumPv; int use_lds;
+    double *part;              // gridDim x (2 + sumPv R) when use_lds, else gridDim x 2
+    double *Zglob;
+};
+
+struct SegArgs {
+    const uint4 *V; const int64_t *rowptr; const int32_t *col; const void *w;
+    const void *rec; const Seg *segs; int64_t nseg;
+    void *segout; void *pt; double rho, rt;
+    double *tpart;             // per block: tri tot, tri gmin
+};
+
+struct RowArgs {
+    const uint4 *V; int64_t n; const int64_t *rowptr; const int32_t *col; const void *w; const double *d; double L;
+    const double *z;
+    int nlev; int Poff[LB_MAXLEV]; const int32_t *lab[LB_MAXLEV]; const double *M; const double *pdiag;
+    int nvcs; int Yoff[LB_MAXVCS]; int Zoff[LB_MAXVCS]; const int32_t *vlab[LB_MAXVCS]; const int64_t *vptr[LB_MAXVCS];
+    const int32_t *vcol; const void *pv; const double *Y; const double *Z;
+    const int64_t *iptr; const void *rec; void *pt; double rho, rt;
+    int64_t heavyE, heavyT;
+    const int32_t *hE_rows; const int32_t *hE_seg0; int nhE;
+    const int32_t *hT_rows; const int32_t *hT_seg0; int nhT;
+    const void *segout;
+    void *G;
+    double *part;              // per block: quad, gv, tri tot, tri gmin
+};
+
+struct FinArgs {
+    int R; double L, C0; const double *z; const double *lamsum;
+    const double *lev_tot; const double *lev_viol; int sumPpair;
+    const double *vcs_part; int nvcs_blk; int vcs_width;
+    const double *row_part; int nrow_blk;
+    const double *seg_part; int nseg_blk;
+    double rho; double *parts;
+};
+
+// ------------------------------------------------------------------------------------------------ lb_k_colsum
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_colsum(ColsumArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    extern __shared__ double shY[];
+    __shared__ double zred[LB_BLOCK / 64][R];
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    const int width = a.use_lds ? a.sumP * R : 0;
+    for (int e = threadIdx.x; e < width; e += LB_BLOCK) shY[e] = 0.0;
+    __syncthreads();
+    double zacc[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; c++) zacc[c] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * NSL + slot; i < a.n; i += (int64_t)gridDim.x * NSL) {
+        TC x[CPL];
+        VT<TV>::unpack(a.V[i * S + q], x);
+        const double di = a.d[i];
+        double v[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) { v[c] = di * (double)x[c]; zacc[c] += v[c]; }
+        for (int s = 0; s < a.ls.nsets; s++) {
+            const int Q = a.ls.off[s] + a.ls.lab[s][i];
+            double *dst = a.use_lds ? (shY + (int64_t)Q * R + q * CPL) : (a.Yglob + (int64_t)Q * R + q * CPL);
+#pragma unroll
+            for (int c = 0; c < CPL; c++) atomicAdd(dst + c, v[c]);
+        }
+    }
+    // z: reduce over the slots of each wave (lanes with equal q), then over waves
+#pragma unroll
+    for (int c = 0; c < CPL; c++) {
+        double t = zacc[c];
+        for (int m = S; m < 64; m <<= 1) t += __shfl_xor(t, m);
+        zacc[c] = t;
+    }
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln < S) {
+#pragma unroll
+        for (int c = 0; c < CPL; c++) zred[wv][ln * CPL + c] = zacc[c];
+    }
+    __syncthreads();
+    const int pw = R + width;
+    double *out = a.part + (int64_t)blockIdx.x * pw;
+    if (threadIdx.x < R) {
+        double s = 0.0;
+        for (int k = 0; k < LB_BLOCK / 64; k++) s += zred[k][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+    for (int e = threadIdx.x; e < width; e += LB_BLOCK) out[R + e] = shY[e];
+}
+
+// fixed-order sum of per-block partials: out[e] = sum_b part[b * width + e]
+__global__ void lb_k_reduce(const double *part, int nblk, int width, double *out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= width) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += part[(int64_t)b * width + e];
+    out[e] = s;
+}
+
+__global__ void lb_k_fill(double *p, int64_t cnt, double v) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * blockDim.x) p[e] = v;
+}
+
+// ------------------------------------------------------------------------------------------------ lb_k_levels
+// One wavefront per cluster row Q of a pair level (global row id = blockIdx.x).
+__global__ __launch_bounds__(64) void lb_k_levels(LevelArgs a) {
+    const int Qg = blockIdx.x;
+    int l = 0;
+    while (l + 1 < a.nlev && Qg >= a.Poff[l + 1]) l++;
+    const int P = a.Poff[l + 1] - a.Poff[l], Q = Qg - a.Poff[l], R = a.R;
+    const double *YQ = a.Y + (int64_t)Qg * R;
+    const double *sig = a.sig + a.soff[l] + (int64_t)Q * P;
+    double *pp = a.pp + a.soff[l] + (int64_t)Q * P;
+    const double rqQ = a.rq[Qg];
+    double Macc[32];
+    for (int c = 0; c < R; c++) Macc[c] = 0.0;
+    double tot = 0.0, viol = 0.0, pq = 0.0;
+    for (int Rr = threadIdx.x; Rr < P; Rr += 64) {
+        const double *YR = a.Y + (int64_t)(a.Poff[l] + Rr) * R;
+        double p = sig[Rr];
+        if (a.rho > 0.0) {
+            double h = 0.0;
+            for (int c = 0; c < R; c++) h += YQ[c] * YR[c];
+            if (Rr == Q) h -= a.dsq[Qg];
+            const double rr = a.rho * ((rqQ * a.rq[a.Poff[l] + Rr]) * a.w);
+            const double sg = p;
+            p = sg - rr * h;
+            if (p < 0.0) p = 0.0;
+            tot += (p * p - sg * sg) / (2 * rr);
+            if (h < -1e-9) viol += 1.0;
+        }
+        pp[Rr] = p;
+        if (Rr == Q) pq = p;
+        if (p != 0.0)
+            for (int c = 0; c < R; c++) Macc[c] += p * YR[c];
+    }
+    tot = wave_sum(tot); viol = wave_sum(viol); pq = wave_sum(pq);
+    for (int c = 0; c < R; c++) Macc[c] = wave_sum(Macc[c]);
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < R; c++) a.M[(int64_t)Qg * R + c] = Macc[c];
+        a.pdiag[Qg] = pq; a.tot[Qg] = tot; a.viol[Qg] = viol;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ lb_k_vcs_z
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    extern __shared__ double shZ[];
+    __shared__ double red[LB_BLOCK / 64];
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    const int width = a.use_lds ? a.sumPv * R : 0;
+    for (int e = threadIdx.x; e < width; e += LB_BLOCK) shZ[e] = 0.0;
+    __syncthreads();
+    const TC *sig = (const TC *)a.sig;
+    TC *pv = (TC *)a.pv;
+    double tot = 0.0, viol = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * NSL + slot; i < a.n; i += (int64_t)gridDim.x * NSL) {
+        TC x[CPL];
+        VT<TV>::unpack(a.V[i * S + q], x);
+        const double di = a.d[i];
+        for (int m = 0; m < a.nvcs; m++) {
+            const int Ri = a.lab[m][i];
+            const int64_t e0 = a.ptr[m][i], e1 = a.ptr[m][i + 1];
+            for (int64_t e = e0; e < e1; e++) {
+                const int Rc = a.col[e];
+                const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < CPL; c++) s += (double)x[c] * YR[c];
+                s = slot_sum<S>(s);
+                const double g = (Rc == Ri) ? di * (s - di) : di * s;
+                const double sg = (double)sig[e];
+                double p = sg;
+                if (a.rho > 0.0 && di > 0.0) {
+                    const double rr = a.rho * (a.wv * a.rq[m][Rc] / (di * di));
+                    p = sg - rr * g;
+                    if (p < 0.0) p = 0.0;
+                    if (q == 0) tot += (p * p - sg * sg) / (2 * rr);
+                }
+                if (q == 0) {
+                    pv[e] = (TC)p;
+                    if (g < -1e-9) viol += 1.0;
+                }
+                const TC pc = (TC)p;  // the stored multiplier is what G uses
+                if (pc != (TC)0) {
+                    double *dst = a.use_lds ? (shZ + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL)
+                                            : (a.Zglob + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL);
+#pragma unroll
+                    for (int c = 0; c < CPL; c++) atomicAdd(dst + c, (double)pc * (di * (double)x[c]));
+                }
+            }
+        }
+    }
+    tot = block_sum(tot, red);
+    viol = block_sum(viol, red);
+    __syncthreads();
+    const int pw = 2 + width;
+    double *out = a.part + (int64_t)blockIdx.x * pw;
+    if (threadIdx.x == 0) { out[0] = tot; out[1] = viol; }
+    for (int e = threadIdx.x; e < width; e += LB_BLOCK) out[2 + e] = shZ[e];
+}
+
+// ------------------------------------------------------------------------------------------------ triangle record
+// One incidence record of vertex i (x = its V chunk): tau of the triangle in the canonical (a, b, c) order (the
+// same arithmetic for all three owners, as batch_step6 does, alm_vcs.py:297-312), pt, and the row contribution of
+// T(pt) V (tri_apply, alm_vcs.py:106-120). Returns pt and tau.
+template <typename TV, int R>
+__device__ inline void tri_record(const uint4 *V, const TriRec<typename VT<TV>::TC> &rc, const typename VT<TV>::TC *x,
+                                  int q, double rho, double rt, typename VT<TV>::TC *g, typename VT<TV>::TC &pt_out,
+                                  typename VT<TV>::TC &tau_out) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL;
+    TC xu[CPL], xw[CPL];
+    VT<TV>::unpack(V[(int64_t)rc.u * S + q], xu);
+    VT<TV>::unpack(V[(int64_t)rc.w * S + q], xw);
+    const int role = rc.code % 3;
+    const TC *pa = role == 0 ? x : xu;
+    const TC *pb = role == 0 ? xu : (role == 1 ? x : xw);
+    const TC *pc = role == 2 ? x : xw;
+    TC tl = 0;
+#pragma unroll
+    for (int c = 0; c < CPL; c++) tl += pa[c] * pb[c] + pb[c] * pc[c] - pa[c] * pc[c];
+    const TC tau = slot_sum<S>(tl);
+    TC p = rc.lam;
+    if (rho > 0.0) {
+        p = rc.lam - (TC)rt * ((TC)1 - tau);
+        if (p < (TC)0) p = (TC)0;
+    }
+    if (p != (TC)0) {
+        const TC h = (TC)0.5 * p;
+        if (role == 0) {
+#pragma unroll
+            for (int c = 0; c < CPL; c++) g[c] += h * (xw[c] - xu[c]);
+        } else if (role == 1) {
+#pragma unroll
+            for (int c = 0; c < CPL; c++) g[c] -= h * (xu[c] + xw[c]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < CPL; c++) g[c] += h * (xu[c] - xw[c]);
+        }
+    }
+    pt_out = p;
+    tau_out = tau;
+}
+
+// ------------------------------------------------------------------------------------------------ lb_k_segs
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_segs(SegArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSW = 64 / S;
+    __shared__ double red[LB_BLOCK / 64];
+    const int64_t sg = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, ws = lane / S, q = lane % S;
+    double tot = 0.0, gmin = INFINITY;
+    if (sg < a.nseg) {
+        const Seg s = a.segs[sg];
+        TC acc[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) acc[c] = 0;
+        if (s.kind == 0) {
+            const TC *wt = (const TC *)a.w;
+            for (int64_t e = s.b + ws; e < s.e; e += NSW) {
+                const int64_t j = a.col[e];
+                TC xj[CPL];
+                VT<TV>::unpack(a.V[j * S + q], xj);
+                const TC w = wt ? wt[e] : (TC)1;
+#pragma unroll
+                for (int c = 0; c < CPL; c++) acc[c] += w * xj[c];
+            }
+        } else {
+            TC x[CPL];
+            VT<TV>::unpack(a.V[(int64_t)s.row * S + q], x);
+            const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
+            TC *pt = (TC *)a.pt;
+            for (int64_t k = s.b + ws; k < s.e; k += NSW) {
+                const TriRec<TC> rc = rec[k];
+                TC p, tau;
+                tri_record<TV, R>(a.V, rc, x, q, a.rho, a.rt, acc, p, tau);
+                if (rc.code % 3 == 0 && q == 0) {
+                    pt[rc.code / 3] = p;
+                    if (a.rho > 0.0) {
+                        tot += ((double)p * p - (double)rc.lam * rc.lam) / (2 * a.rt);
+                        gmin = fmin(gmin, (double)((TC)1 - tau));
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+            TC t = acc[c];
+            for (int m = S; m < 64; m <<= 1) t += __shfl_xor(t, m);
+            acc[c] = t;
+        }
+        if (ws == 0) {
+            TC *out = (TC *)a.segout + sg * R + q * CPL;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) out[c] = acc[c];
+        }
+    }
+    tot = block_sum(tot, red);
+    __syncthreads();
+    gmin = block_min(gmin, red);
+    if (threadIdx.x == 0) { a.tpart[2 * blockIdx.x] = tot; a.tpart[2 * blockIdx.x + 1] = gmin; }
+}
+
+__device__ inline int find_row(const int32_t *rows, int cnt, int64_t i) {
+    int lo = 0, hi = cnt;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (rows[mid] < i) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------------------------------------------ lb_k_rows
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    __shared__ double red[LB_BLOCK / 64];
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    const int64_t i = (int64_t)blockIdx.x * NSL + slot;
+    double quad = 0.0, gv = 0.0, tot = 0.0, gmin = INFINITY;
+    if (i < a.n) {
+        TC x[CPL], g[CPL];
+        VT<TV>::unpack(a.V[i * S + q], x);
+#pragma unroll
+        for (int c = 0; c < CPL; c++) g[c] = 0;
+        // (A V)_i
+        const int64_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+        if (e1 - e0 > a.heavyE) {
+            const int k = find_row(a.hE_rows, a.nhE, i);
+            const TC *so = (const TC *)a.segout;
+            for (int s = a.hE_seg0[k]; s < a.hE_seg0[k + 1]; s++) {
+#pragma unroll
+                for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
+            }
+        } else if (a.w) {
+            const TC *wt = (const TC *)a.w;
+            for (int64_t e = e0; e < e1; e++) {
+                const int64_t j = a.col[e];
+                TC xj[CPL];
+                VT<TV>::unpack(a.V[j * S + q], xj);
+                const TC w = wt[e];
+#pragma unroll
+                for (int c = 0; c < CPL; c++) g[c] += w * xj[c];
+            }
+        } else {
+            int64_t e = e0;
+            for (; e + 4 <= e1; e += 4) {
+                const int64_t j0 = a.col[e], j1 = a.col[e + 1], j2 = a.col[e + 2], j3 = a.col[e + 3];
+                const uint4 u0 = a.V[j0 * S + q], u1 = a.V[j1 * S + q], u2 = a.V[j2 * S + q], u3 = a.V[j3 * S + q];
+                TC x0[CPL], x1[CPL], x2[CPL], x3[CPL];
+                VT<TV>::unpack(u0, x0); VT<TV>::unpack(u1, x1); VT<TV>::unpack(u2, x2); VT<TV>::unpack(u3, x3);
+#pragma unroll
+                for (int c = 0; c < CPL; c++) g[c] += (x0[c] + x1[c]) + (x2[c] + x3[c]);
+            }
+            for (; e < e1; e++) {
+                TC xj[CPL];
+                VT<TV>::unpack(a.V[(int64_t)a.col[e] * S + q], xj);
+#pragma unroll
+                for (int c = 0; c < CPL; c++) g[c] += xj[c];
+            }
+        }
+        TC qd = 0;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) qd += x[c] * g[c];
+        qd = slot_sum<S>(qd);
+        if (q == 0) quad = (double)qd;
+        const double di = a.d[i];
+        // rank-1 correction -d_i z / L
+#pragma unroll
+        for (int c = 0; c < CPL; c++) g[c] -= (TC)(di * a.z[q * CPL + c] / a.L);
+        // pair levels: d_i (M_Q - pp_QQ d_i V_i)
+        for (int l = 0; l < a.nlev; l++) {
+            const int Q = a.Poff[l] + a.lab[l][i];
+            const double *M = a.M + (int64_t)Q * R + q * CPL;
+            const double pd = a.pdiag[Q] * di;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) g[c] += (TC)(di * (M[c] - pd * (double)x[c]));
+        }
+        // VCS: 1/2 d_i (sum_R pv_iR Y_R + Z_R(i)) - pv_own d_i^2 V_i
+        const TC *pv = (const TC *)a.pv;
+        for (int m = 0; m < a.nvcs; m++) {
+            const int Ri = a.vlab[m][i];
+            double acc[CPL];
+#pragma unroll
+            for (int c = 0; c < CPL; c++) acc[c] = 0.0;
+            double pown = 0.0;
+            for (int64_t e = a.vptr[m][i]; e < a.vptr[m][i + 1]; e++) {
+                const int Rc = a.vcol[e];
+                const double p = (double)pv[e];
+                if (Rc == Ri) pown = p;
+                if (p != 0.0) {
+                    const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+#pragma unroll
+                    for (int c = 0; c < CPL; c++) acc[c] += p * YR[c];
+                }
+            }
+            const double *ZR = a.Z + (int64_t)(a.Zoff[m] + Ri) * R + q * CPL;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) g[c] += (TC)(0.5 * di * (acc[c] + ZR[c]) - pown * di * di * (double)x[c]);
+        }
+        // triangles: T(pt) V rows through the incidence records
+        const int64_t r0 = a.iptr[i], r1 = a.iptr[i + 1];
+        if (r1 - r0 > a.heavyT) {
+            const int k = find_row(a.hT_rows, a.nhT, i);
+            const TC *so = (const TC *)a.segout;
+            for (int s = a.hT_seg0[k]; s < a.hT_seg0[k + 1]; s++) {
+#pragma unroll
+                for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
+            }
+        } else {
+            const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
+            TC *pt = (TC *)a.pt;
+            for (int64_t k = r0; k < r1; k++) {
+                const TriRec<TC> rc = rec[k];
+                TC p, tau;
+                tri_record<TV, R>(a.V, rc, x, q, a.rho, a.rt, g, p, tau);
+                if (rc.code % 3 == 0 && q == 0) {
+                    pt[rc.code / 3] = p;
+                    if (a.rho > 0.0) {
+                        tot += ((double)p * p - (double)rc.lam * rc.lam) / (2 * a.rt);
+                        gmin = fmin(gmin, (double)((TC)1 - tau));
+                    }
+                }
+            }
+        }
+        // G row and <G_i, V_i>
+        TC *G = (TC *)a.G + i * R + q * CPL;
+        TC gd = 0;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) { G[c] = g[c]; gd += g[c] * x[c]; }
+        gd = slot_sum<S>(gd);
+        if (q == 0) gv = (double)gd;
+    }
+    quad = block_sum(quad, red);
+    __syncthreads();
+    gv = block_sum(gv, red);
+    __syncthreads();
+    tot = block_sum(tot, red);
+    __syncthreads();
+    gmin = block_min(gmin, red);
+    if (threadIdx.x == 0) {
+        double *o = a.part + 4 * (int64_t)blockIdx.x;
+        o[0] = quad; o[1] = gv; o[2] = tot; o[3] = gmin;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ lb_k_finalize
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_finalize(FinArgs a) {
+    __shared__ double red[LB_BLOCK / 64];
+    double quad = 0, gv = 0, ttot = 0, gmin = INFINITY, vtot = 0, vviol = 0, ltot = 0, lviol = 0;
+    // per-thread strided partial sums, combined in a fixed order
+    for (int b = threadIdx.x; b < a.nrow_blk; b += LB_BLOCK) {
+        const double *o = a.row_part + 4 * (int64_t)b;
+        quad += o[0]; gv += o[1]; ttot += o[2]; gmin = fmin(gmin, o[3]);
+    }
+    for (int b = threadIdx.x; b < a.nseg_blk; b += LB_BLOCK) {
+        ttot += a.seg_part[2 * b]; gmin = fmin(gmin, a.seg_part[2 * b + 1]);
+    }
+    for (int b = threadIdx.x; b < a.nvcs_blk; b += LB_BLOCK) {
+        vtot += a.vcs_part[(int64_t)b * a.vcs_width]; vviol += a.vcs_part[(int64_t)b * a.vcs_width + 1];
+    }
+    for (int b = threadIdx.x; b < a.sumPpair; b += LB_BLOCK) { ltot += a.lev_tot[b]; lviol += a.lev_viol[b]; }
+    quad = block_sum(quad, red); __syncthreads();
+    gv = block_sum(gv, red); __syncthreads();
+    ttot = block_sum(ttot, red); __syncthreads();
+    gmin = block_min(gmin, red); __syncthreads();
+    vtot = block_sum(vtot, red); __syncthreads();
+    vviol = block_sum(vviol, red); __syncthreads();
+    ltot = block_sum(ltot, red); __syncthreads();
+    lviol = block_sum(lviol, red);
+    if (threadIdx.x == 0) {
+        double zz = 0.0;
+        for (int c = 0; c < a.R; c++) zz += a.z[c] * a.z[c];
+        double *p = a.parts;
+        const double lamsum = *a.lamsum;
+        p[2] = zz / a.L; p[7] = a.C0; p[11] = lamsum;
+        p[6] = quad - zz / a.L + a.C0;
+        if (a.rho > 0.0) {
+            p[1] = quad; p[3] = ltot; p[4] = ttot; p[5] = vtot; p[9] = gmin;
+            p[0] = quad - zz / a.L - ltot - ttot - vtot + a.C0;
+        } else {
+            p[1] = gv; p[3] = 0.0; p[4] = 0.0; p[5] = 0.0; p[9] = INFINITY;
+            p[0] = a.C0 + lamsum + gv;
+        }
+        p[8] = lviol; p[10] = vviol;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ small kernels
+template <typename TC>
+__global__ void lb_k_sum(const TC *x, int64_t cnt, double *out) {
+    __shared__ double red[LB_BLOCK / 64];
+    double s = 0.0;
+    for (int64_t e = threadIdx.x; e < cnt; e += LB_BLOCK) s += (double)x[e];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) *out = s;
+}
+
+template <typename TC>
+__global__ void lb_k_inc_lam(TriRec<TC> *rec, int64_t nrec, const TC *lam) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nrec; k += (int64_t)gridDim.x * blockDim.x)
+        rec[k].lam = lam[rec[k].code / 3];
+}
+
+template <typename TC>
+__global__ void lb_k_copy(TC *dst, const TC *src, int64_t cnt) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * blockDim.x) dst[e] = src[e];
+}
+
+// ================================================================================================== host side
+namespace {
+
+const char *kKernelNames[LB_NKERNELS] = {"lb_k_colsum", "lb_k_reduce(z,Y)", "lb_k_levels", "lb_k_vcs_z",
+                                         "lb_k_reduce(Z)", "lb_k_segs", "lb_k_rows", "lb_k_finalize"};
+
+struct DBuf {
+    void *p = nullptr; size_t bytes = 0;
+};
+
+}  // namespace
+
+struct lb_ctx {
+    int device = 0, vdtype = LB_F32, r = 16;
+    hipStream_t stream = nullptr;
+    std::string err;
+    size_t vbytes = 4, cbytes = 4;   // storage bytes of V elements, of TC values
+    int heavy_edges = 64, heavy_tris = 64, seg_len = 128;
+    int64_t built_heavyE = 64, built_heavyT = 64;   // thresholds the current segment tables were built with
+    // graph
+    int64_t n = 0, nnz = 0; double L = 1.0, C0 = 0.0; bool weighted = false;
+    std::vector<int64_t> h_rowptr;
+    // levels
+    int nlev = 0; std::vector<int> P; std::vector<int64_t> soff; double pw = 1.0; int sumPpair = 0;
+    // vcs
+    int nvcs = 0; std::vector<int> vP; std::vector<int64_t> vnnz, voff; double wv = 1.0; int sumPv = 0;
+    // triangles
+    int64_t T = 0;
+    std::vector<int32_t> hE_rows, hE_seg0, hT_rows, hT_seg0;
+    std::vector<Seg> segE, segT;
+    bool have_eval = false; double last_rho = 0.0;
+    // device buffers
+    DBuf rowptr, col, w, d, labs, sig, pp, rq, dsq, vlabs, vrq, vptr, vcol, vsig, pv, TI, lam, pt, iptr, rec,
+        V, G, Y, z, M, pdiag, levtot, levviol, Z, colsum_part, vcs_part, row_part, seg_part, segs, segout,
+        hErows, hEseg0, hTrows, hTseg0, parts, lamsum;
+};
+
+namespace {
+
+int fail(lb_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                                   \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess) return fail(ctx, -3, std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+int dalloc(lb_ctx *c, DBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.p && b.bytes >= bytes) return 0;
+    if (b.p) { hipFree(b.p); b.p = nullptr; b.bytes = 0; }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) return fail(c, -2, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    b.bytes = bytes;
+    return 0;
+}
+
+void dfree(DBuf &b) {
+    if (b.p) hipFree(b.p);
+    b.p = nullptr; b.bytes = 0;
+}
+
+int upload(lb_ctx *c, DBuf &b, const void *src, size_t bytes) {
+    int rc = dalloc(c, b, bytes);
+    if (rc) return rc;
+    if (bytes) HIPCHK(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// host FP64 values -> device TC values (float or double)
+int upload_tc(lb_ctx *c, DBuf &b, const double *src, int64_t cnt) {
+    if (c->cbytes == 8) return upload(c, b, src, sizeof(double) * (size_t)cnt);
+    std::vector<float> tmp((size_t)std::max<int64_t>(cnt, 1));
+    for (int64_t e = 0; e < cnt; e++) tmp[e] = (float)src[e];
+    return upload(c, b, tmp.data(), sizeof(float) * (size_t)cnt);
+}
+
+int download_tc(lb_ctx *c, const DBuf &b, double *dst, int64_t cnt) {
+    if (!cnt) return 0;
+    if (c->cbytes == 8) {
+        HIPCHK(c, hipMemcpyAsync(dst, b.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    std::vector<float> tmp(cnt);
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), b.p, sizeof(float) * cnt, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t e = 0; e < cnt; e++) dst[e] = tmp[e];
+    return 0;
+}
+
+uint16_t f2bf(float f) {  // round to nearest even
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+float bf2f(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+void build_segments(int64_t n, const int64_t *ptr, int64_t heavy, int seg_len, int kind, std::vector<int32_t> &rows,
+                    std::vector<int32_t> &seg0, std::vector<Seg> &segs, int32_t base) {
+    rows.clear(); seg0.clear(); segs.clear();
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t b = ptr[i], e = ptr[i + 1];
+        if (e - b <= heavy) continue;
+        rows.push_back((int32_t)i);
+        seg0.push_back(base + (int32_t)segs.size());
+        for (int64_t s = b; s < e; s += seg_len) segs.push_back(Seg{(int32_t)i, kind, s, std::min<int64_t>(e, s + seg_len)});
+    }
+    seg0.push_back(base + (int32_t)segs.size());
+}
+
+int rebuild_segs(lb_ctx *c) {
+    std::vector<Seg> all(c->segE);
+    all.insert(all.end(), c->segT.begin(), c->segT.end());
+    int rc;
+    if ((rc = upload(c, c->segs, all.data(), sizeof(Seg) * all.size()))) return rc;
+    if ((rc = dalloc(c, c->segout, (size_t)std::max<size_t>(all.size(), 1) * c->r * c->cbytes))) return rc;
+    const int nblk = (int)((all.size() + 3) / 4);
+    if ((rc = dalloc(c, c->seg_part, sizeof(double) * 2 * std::max(nblk, 1)))) return rc;
+    return 0;
+}
+
+template <typename TV, int R>
+struct Launch {
+    typedef typename VT<TV>::TC TC;
+    static constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+
+    static int eval(lb_ctx *c, double rho, double rt, hipEvent_t *ev) {
+        const int64_t n = c->n;
+        const int sumPall = c->sumPpair + c->sumPv;
+        int ek = 0;
+        auto mark = [&]() { if (ev) hipEventRecord(ev[ek++], c->stream); };
+        mark();
+        // --- z, Y
+        ColsumArgs ca{};
+        ca.V = (const uint4 *)c->V.p; ca.n = n; ca.d = (const double *)c->d.p;
+        ca.ls.nsets = c->nlev + c->nvcs;
+        int off = 0;
+        for (int l = 0; l < c->nlev; l++) { ca.ls.off[l] = off; ca.ls.lab[l] = (const int32_t *)c->labs.p + (int64_t)l * n; off += c->P[l]; }
+        for (int m = 0; m < c->nvcs; m++) { ca.ls.off[c->nlev + m] = off; ca.ls.lab[c->nlev + m] = (const int32_t *)c->vlabs.p + (int64_t)m * n; off += c->vP[m]; }
+        ca.ls.off[ca.ls.nsets] = off;
+        ca.sumP = sumPall;
+        const size_t ldsY = sizeof(double) * (size_t)sumPall * R;
+        ca.use_lds = ldsY <= 48 * 1024;
+        ca.part = (double *)c->colsum_part.p;
+        ca.Yglob = (double *)c->Y.p + R;  // Y buffer holds [z | Y]
+        if (!ca.use_lds && sumPall) hipLaunchKernelGGL(lb_k_fill, dim3(256), dim3(256), 0, c->stream, (double *)c->Y.p + R, (int64_t)sumPall * R, 0.0);
+        hipLaunchKernelGGL((lb_k_colsum<TV, R>), dim3(LB_RED_BLOCKS), dim3(LB_BLOCK), ca.use_lds ? ldsY : 0, c->stream, ca);
+        mark();
+        {
+            const int width = R + (ca.use_lds ? sumPall * R : 0);
+            hipLaunchKernelGGL(lb_k_reduce, dim3((width + 255) / 256), dim3(256), 0, c->stream, (const double *)c->colsum_part.p,
+                               LB_RED_BLOCKS, width, (double *)c->Y.p);
+        }
+        mark();
+        const double *zp = (const double *)c->Y.p;
+        const double *Yp = zp + R;
+        // --- pair levels
+        if (c->nlev) {
+            LevelArgs la{};
+            la.nlev = c->nlev; la.R = R;
+            int o = 0;
+            for (int l = 0; l <= c->nlev; l++) { la.Poff[l] = o; la.soff[l] = c->soff[l]; if (l < c->nlev) o += c->P[l]; }
+            la.Y = Yp; la.sig = (const double *)c->sig.p; la.pp = (double *)c->pp.p; la.rq = (const double *)c->rq.p;
+            la.dsq = (const double *)c->dsq.p; la.w = c->pw; la.rho = rho;
+            la.M = (double *)c->M.p; la.pdiag = (double *)c->pdiag.p; la.tot = (double *)c->levtot.p; la.viol = (double *)c->levviol.p;
+            hipLaunchKernelGGL(lb_k_levels, dim3(c->sumPpair), dim3(64), 0, c->stream, la);
+        }
+        mark();
+        // --- VCS multipliers and Z
+        const size_t ldsZ = sizeof(double) * (size_t)c->sumPv * R;
+        const int vcs_lds = ldsZ <= 48 * 1024;
+        const int vcs_width = 2 + (vcs_lds ? c->sumPv * R : 0);
+        if (c->nvcs) {
+            VcsArgs va{};
+            va.V = (const uint4 *)c->V.p; va.n = n; va.d = (const double *)c->d.p; va.nvcs = c->nvcs;
+            int zo = 0;
+            for (int m = 0; m < c->nvcs; m++) {
+                va.Yoff[m] = c->sumPpair + zo; va.Zoff[m] = zo; zo += c->vP[m];
+                va.lab[m] = (const int32_t *)c->vlabs.p + (int64_t)m * n;
+                va.rq[m] = (const double *)c->vrq.p + (va.Zoff[m]);
+                va.ptr[m] = (const int64_t *)c->vptr.p + (int64_t)m * (n + 1);
+            }
+            va.Zoff[c->nvcs] = zo;
+            va.col = (const int32_t *)c->vcol.p; va.sig = c->vsig.p; va.pv = c->pv.p; va.Y = Yp; va.wv = c->wv; va.rho = rho;
+            va.sumPv = c->sumPv; va.use_lds = vcs_lds; va.part = (double *)c->vcs_part.p; va.Zglob = (double *)c->Z.p;
+            if (!vcs_lds) hipLaunchKernelGGL(lb_k_fill, dim3(256), dim3(256), 0, c->stream, (double *)c->Z.p, (int64_t)c->sumPv * R, 0.0);
+            hipLaunchKernelGGL((lb_k_vcs_z<TV, R>), dim3(LB_RED_BLOCKS), dim3(LB_BLOCK), vcs_lds ? ldsZ : 0, c->stream, va);
+        }
+        mark();
+        if (c->nvcs && vcs_lds) {
+            const int width = c->sumPv * R;
+            // Z entries sit after the two scalars of each block's partial row
+            hipLaunchKernelGGL(lb_k_reduce, dim3((width + 255) / 256), dim3(256), 0, c->stream,
+                               (const double *)c->vcs_part.p + 2, LB_RED_BLOCKS, vcs_width, (double *)c->Z.p);
+        }
+        mark();
+        // --- hub segments
+        const int64_t nseg = (int64_t)(c->segE.size() + c->segT.size());
+        const int nseg_blk = (int)((nseg + 3) / 4);
+        if (nseg) {
+            SegArgs sa{};
+            sa.V = (const uint4 *)c->V.p; sa.rowptr = (const int64_t *)c->rowptr.p; sa.col = (const int32_t *)c->col.p;
+            sa.w = c->weighted ? c->w.p : nullptr; sa.rec = c->rec.p; sa.segs = (const Seg *)c->segs.p; sa.nseg = nseg;
+            sa.segout = c->segout.p; sa.pt = c->pt.p; sa.rho = rho; sa.rt = rt; sa.tpart = (double *)c->seg_part.p;
+            hipLaunchKernelGGL((lb_k_segs<TV, R>), dim3(nseg_blk), dim3(LB_BLOCK), 0, c->stream, sa);
+        }
+        mark();
+        // --- rows
+        const int64_t nrow_blk = (n + NSL - 1) / NSL;
+        {
+            RowArgs ra{};
+            ra.V = (const uint4 *)c->V.p; ra.n = n; ra.rowptr = (const int64_t *)c->rowptr.p; ra.col = (const int32_t *)c->col.p;
+            ra.w = c->weighted ? c->w.p : nullptr; ra.d = (const double *)c->d.p; ra.L = c->L; ra.z = zp;
+            ra.nlev = c->nlev;
+            int o = 0;
+            for (int l = 0; l < c->nlev; l++) { ra.Poff[l] = o; o += c->P[l]; ra.lab[l] = (const int32_t *)c->labs.p + (int64_t)l * n; }
+            ra.M = (const double *)c->M.p; ra.pdiag = (const double *)c->pdiag.p;
+            ra.nvcs = c->nvcs;
+            int zo = 0;
+            for (int m = 0; m < c->nvcs; m++) {
+                ra.Yoff[m] = c->sumPpair + zo; ra.Zoff[m] = zo; zo += c->vP[m];
+                ra.vlab[m] = (const int32_t *)c->vlabs.p + (int64_t)m * n;
+                ra.vptr[m] = (const int64_t *)c->vptr.p + (int64_t)m * (n + 1);
+            }
+            ra.vcol = (const int32_t *)c->vcol.p; ra.pv = c->pv.p; ra.Y = Yp; ra.Z = (const double *)c->Z.p;
+            ra.iptr = (const int64_t *)c->iptr.p; ra.rec = c->rec.p; ra.pt = c->pt.p; ra.rho = rho; ra.rt = rt;
+            ra.heavyE = c->built_heavyE; ra.heavyT = c->built_heavyT;
+            ra.hE_rows = (const int32_t *)c->hErows.p; ra.hE_seg0 = (const int32_t *)c->hEseg0.p; ra.nhE = (int)c->hE_rows.size();
+            ra.hT_rows = (const int32_t *)c->hTrows.p; ra.hT_seg0 = (const int32_t *)c->hTseg0.p; ra.nhT = (int)c->hT_rows.size();
+            ra.segout = c->segout.p; ra.G = c->G.p; ra.part = (double *)c->row_part.p;
+            hipLaunchKernelGGL((lb_k_rows<TV, R>), dim3((unsigned)nrow_blk), dim3(LB_BLOCK), 0, c->stream, ra);
+        }
+        mark();
+        // --- scalars
+        FinArgs fa{};
+        fa.R = R; fa.L = c->L; fa.C0 = c->C0; fa.z = zp; fa.lamsum = (const double *)c->lamsum.p;
+        fa.lev_tot = (const double *)c->levtot.p; fa.lev_viol = (const double *)c->levviol.p; fa.sumPpair = c->sumPpair;
+        fa.vcs_part = (const double *)c->vcs_part.p; fa.nvcs_blk = c->nvcs ? LB_RED_BLOCKS : 0; fa.vcs_width = vcs_width;
+        fa.row_part = (const double *)c->row_part.p; fa.nrow_blk = (int)nrow_blk;
+        fa.seg_part = (const double *)c->seg_part.p; fa.nseg_blk = nseg ? nseg_blk : 0;
+        fa.rho = rho; fa.parts = (double *)c->parts.p;
+        hipLaunchKernelGGL(lb_k_finalize, dim3(1), dim3(LB_BLOCK), 0, c->stream, fa);
+        mark();
+        HIPCHK(c, hipGetLastError());
+        return 0;
+    }
+};
+
+typedef int (*eval_fn)(lb_ctx *, double, double, hipEvent_t *);
+
+eval_fn pick_eval(int vdtype, int r) {
+#define LB_PICK(TV)                                     \
+    if (r == 8) return &Launch<TV, 8>::eval;            \
+    if (r == 16) return &Launch<TV, 16>::eval;          \
+    if (r == 32) return &Launch<TV, 32>::eval;
+    if (vdtype == LB_F32) { LB_PICK(float) }
+    if (vdtype == LB_BF16) { LB_PICK(bf16raw) }
+    if (vdtype == LB_F64) { LB_PICK(double) }
+#undef LB_PICK
+    return nullptr;
+}
+
+int ensure_eval_buffers(lb_ctx *c) {
+    const int R = c->r;
+    const int sumPall = c->sumPpair + c->sumPv;
+    int rc;
+    if ((rc = dalloc(c, c->Y, sizeof(double) * (size_t)(R + sumPall * R)))) return rc;
+    const bool ylds = sizeof(double) * (size_t)sumPall * R <= 48 * 1024;
+    if ((rc = dalloc(c, c->colsum_part, sizeof(double) * (size_t)LB_RED_BLOCKS * (R + (ylds ? sumPall * R : 0))))) return rc;
+    if ((rc = dalloc(c, c->M, sizeof(double) * (size_t)std::max(c->sumPpair, 1) * R))) return rc;
+    if ((rc = dalloc(c, c->pdiag, sizeof(double) * std::max(c->sumPpair, 1)))) return rc;
+    if ((rc = dalloc(c, c->levtot, sizeof(double) * std::max(c->sumPpair, 1)))) return rc;
+    if ((rc = dalloc(c, c->levviol, sizeof(double) * std::max(c->sumPpair, 1)))) return rc;
+    if ((rc = dalloc(c, c->Z, sizeof(double) * (size_t)std::max(c->sumPv, 1) * R))) return rc;
+    const bool zlds = sizeof(double) * (size_t)c->sumPv * R <= 48 * 1024;
+    if ((rc = dalloc(c, c->vcs_part, sizeof(double) * (size_t)LB_RED_BLOCKS * (2 + (zlds ? c->sumPv * R : 0))))) return rc;
+    const int S = (int)(R * c->vbytes / 16), NSL = LB_BLOCK / S;
+    const int64_t nrow_blk = (c->n + NSL - 1) / NSL;
+    if ((rc = dalloc(c, c->row_part, sizeof(double) * 4 * (size_t)std::max<int64_t>(nrow_blk, 1)))) return rc;
+    if ((rc = dalloc(c, c->parts, sizeof(double) * LB_NPARTS))) return rc;
+    if (!c->lamsum.p) {
+        if ((rc = dalloc(c, c->lamsum, sizeof(double)))) return rc;
+        HIPCHK(c, hipMemsetAsync(c->lamsum.p, 0, sizeof(double), c->stream));
+    }
+    return 0;
+}
+
+int refresh_lamsum(lb_ctx *c) {
+    int rc;
+    if ((rc = dalloc(c, c->lamsum, sizeof(double)))) return rc;
+    if (c->T == 0) {
+        HIPCHK(c, hipMemsetAsync(c->lamsum.p, 0, sizeof(double), c->stream));
+        return 0;
+    }
+    if (c->cbytes == 8) hipLaunchKernelGGL(lb_k_sum<double>, dim3(1), dim3(LB_BLOCK), 0, c->stream, (const double *)c->lam.p, c->T, (double *)c->lamsum.p);
+    else hipLaunchKernelGGL(lb_k_sum<float>, dim3(1), dim3(LB_BLOCK), 0, c->stream, (const float *)c->lam.p, c->T, (double *)c->lamsum.p);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int empty_triangles(lb_ctx *c) {
+    // zero-triangle pool: incidence pointer of zeros so the row kernels see no records
+    std::vector<int64_t> z(c->n + 1, 0);
+    int rc;
+    if ((rc = upload(c, c->iptr, z.data(), sizeof(int64_t) * (c->n + 1)))) return rc;
+    if ((rc = dalloc(c, c->rec, 32))) return rc;
+    if ((rc = dalloc(c, c->pt, 16))) return rc;
+    if ((rc = dalloc(c, c->lam, 16))) return rc;
+    c->hT_rows.clear(); c->hT_seg0.assign(1, (int32_t)c->segE.size()); c->segT.clear();
+    if ((rc = upload(c, c->hTrows, c->hT_rows.data(), 4))) return rc;
+    if ((rc = upload(c, c->hTseg0, c->hT_seg0.data(), 4))) return rc;
+    return 0;
+}
+
+int empty_vcs(lb_ctx *c) {
+    c->nvcs = 0; c->sumPv = 0; c->vP.clear(); c->vnnz.clear(); c->voff.clear();
+    int rc;
+    if ((rc = dalloc(c, c->vlabs, 16))) return rc;
+    if ((rc = dalloc(c, c->vptr, 16))) return rc;
+    if ((rc = dalloc(c, c->vcol, 16))) return rc;
+    if ((rc = dalloc(c, c->vsig, 16))) return rc;
+    if ((rc = dalloc(c, c->pv, 16))) return rc;
+    if ((rc = dalloc(c, c->vrq, 16))) return rc;
+    return 0;
+}
+
+int empty_levels(lb_ctx *c) {
+    c->nlev = 0; c->sumPpair = 0; c->P.clear(); c->soff.assign(1, 0);
+    int rc;
+    if ((rc = dalloc(c, c->labs, 16))) return rc;
+    if ((rc = dalloc(c, c->sig, 16))) return rc;
+    if ((rc = dalloc(c, c->pp, 16))) return rc;
+    if ((rc = dalloc(c, c->rq, 16))) return rc;
+    if ((rc = dalloc(c, c->dsq, 16))) return rc;
+    return 0;
+}
+
+}  // namespace
+
+// ================================================================================================== C ABI
+extern "C" {
+
+const char *lb_version(void) { return "lbhip 0.1 (gfx950)"; }
+
+const char *lb_kernel_name(int k) { return (k >= 0 && k < LB_NKERNELS) ? kKernelNames[k] : ""; }
+
+int lb_create(lb_ctx **out, int device, int vdtype, int r) {
+    if (!out) return -1;
+    *out = nullptr;
+    if (r != 8 && r != 16 && r != 32) return -1;
+    if (vdtype != LB_F32 && vdtype != LB_BF16 && vdtype != LB_F64) return -1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return -3;
+    if (hipSetDevice(device) != hipSuccess) return -3;
+    lb_ctx *c = new lb_ctx();
+    c->device = device; c->vdtype = vdtype; c->r = r;
+    c->vbytes = vdtype == LB_F32 ? 4 : (vdtype == LB_BF16 ? 2 : 8);
+    c->cbytes = vdtype == LB_F64 ? 8 : 4;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return -3; }
+    *out = c;
+    return 0;
+}
+
+void lb_destroy(lb_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    DBuf *bufs[] = {&c->rowptr, &c->col, &c->w, &c->d, &c->labs, &c->sig, &c->pp, &c->rq, &c->dsq, &c->vlabs, &c->vrq,
+                    &c->vptr, &c->vcol, &c->vsig, &c->pv, &c->TI, &c->lam, &c->pt, &c->iptr, &c->rec, &c->V, &c->G,
+                    &c->Y, &c->z, &c->M, &c->pdiag, &c->levtot, &c->levviol, &c->Z, &c->colsum_part, &c->vcs_part,
+                    &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
+                    &c->hTseg0, &c->parts, &c->lamsum};
+    for (DBuf *b : bufs) dfree(*b);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *lb_last_error(const lb_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int lb_set_tuning(lb_ctx *c, int heavy_edges, int heavy_tris, int seg_len) {
+    if (!c) return -1;
+    if (heavy_edges < 1 || heavy_tris < 1 || seg_len < 1) return fail(c, -1, "tuning values must be positive");
+    c->heavy_edges = heavy_edges; c->heavy_tris = heavy_tris; c->seg_len = seg_len;
+    return 0;
+}
+
+int lb_set_graph(lb_ctx *c, int64_t n, const int64_t *rowptr, const int32_t *col, const double *weights,
+                 const double *d, double L, double C0) {
+    if (!c) return -1;
+    if (n <= 0 || n >= ((int64_t)1 << 31) || !rowptr || !col || !d || !(L > 0)) return fail(c, -1, "lb_set_graph: bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (rowptr[0] != 0) return fail(c, -1, "lb_set_graph: rowptr[0] != 0");
+    for (int64_t i = 0; i < n; i++)
+        if (rowptr[i + 1] < rowptr[i]) return fail(c, -1, "lb_set_graph: rowptr not monotone");
+    const int64_t nnz = rowptr[n];
+    for (int64_t e = 0; e < nnz; e++)
+        if (col[e] < 0 || col[e] >= n) return fail(c, -1, "lb_set_graph: column index out of range");
+    c->n = n; c->nnz = nnz; c->L = L; c->C0 = C0; c->weighted = weights != nullptr;
+    c->h_rowptr.assign(rowptr, rowptr + n + 1);
+    int rc;
+    if ((rc = upload(c, c->rowptr, rowptr, sizeof(int64_t) * (n + 1)))) return rc;
+    if ((rc = upload(c, c->col, col, sizeof(int32_t) * nnz))) return rc;
+    if (weights) { if ((rc = upload_tc(c, c->w, weights, nnz))) return rc; }
+    if ((rc = upload(c, c->d, d, sizeof(double) * n))) return rc;
+    build_segments(n, rowptr, c->heavy_edges, c->seg_len, 0, c->hE_rows, c->hE_seg0, c->segE, 0);
+    c->built_heavyE = c->heavy_edges;
+    if ((rc = upload(c, c->hErows, c->hE_rows.data(), sizeof(int32_t) * std::max<size_t>(c->hE_rows.size(), 1)))) return rc;
+    if ((rc = upload(c, c->hEseg0, c->hE_seg0.data(), sizeof(int32_t) * c->hE_seg0.size()))) return rc;
+    if ((rc = dalloc(c, c->V, (size_t)n * c->r * c->vbytes))) return rc;
+    if ((rc = dalloc(c, c->G, (size_t)n * c->r * c->cbytes))) return rc;
+    // the graph defines n: reset levels, working sets and the triangle pool
+    c->T = 0; c->segT.clear();
+    if ((rc = empty_levels(c)) || (rc = empty_vcs(c)) || (rc = empty_triangles(c))) return rc;
+    if ((rc = rebuild_segs(c))) return rc;
+    if ((rc = refresh_lamsum(c))) return rc;
+    if ((rc = ensure_eval_buffers(c))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_eval = false;
+    return 0;
+}
+
+int lb_set_levels(lb_ctx *c, int nlev, const int32_t *labs, const int32_t *P, const double *sig, const double *rq,
+                  const double *dsq, double pairlvl_weight) {
+    if (!c || !c->n) return fail(c, -1, "lb_set_levels: set the graph first");
+    if (nlev < 0 || nlev > LB_MAXLEV) return fail(c, -1, "lb_set_levels: at most 8 pair levels");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (nlev == 0) { int rc = empty_levels(c); return rc ? rc : ensure_eval_buffers(c); }
+    int64_t sP = 0, sP2 = 0;
+    std::vector<int64_t> soff(1, 0);
+    for (int l = 0; l < nlev; l++) {
+        if (P[l] <= 0) return fail(c, -1, "lb_set_levels: P must be positive");
+        sP += P[l]; sP2 += (int64_t)P[l] * P[l]; soff.push_back(sP2);
+    }
+    if (sP > (1 << 20)) return fail(c, -1, "lb_set_levels: too many clusters");
+    for (int l = 0; l < nlev; l++)
+        for (int64_t i = 0; i < c->n; i++)
+            if (labs[(int64_t)l * c->n + i] < 0 || labs[(int64_t)l * c->n + i] >= P[l]) return fail(c, -1, "lb_set_levels: label out of range");
+    c->nlev = nlev; c->P.assign(P, P + nlev); c->soff = soff; c->sumPpair = (int)sP; c->pw = pairlvl_weight;
+    int rc;
+    if ((rc = upload(c, c->labs, labs, sizeof(int32_t) * (size_t)nlev * c->n))) return rc;
+    if ((rc = upload(c, c->sig, sig, sizeof(double) * sP2))) return rc;
+    if ((rc = dalloc(c, c->pp, sizeof(double) * sP2))) return rc;
+    if ((rc = upload(c, c->rq, rq, sizeof(double) * sP))) return rc;
+    if ((rc = upload(c, c->dsq, dsq, sizeof(double) * sP))) return rc;
+    c->have_eval = false;
+    return ensure_eval_buffers(c);
+}
+
+int lb_set_level_sig(lb_ctx *c, const double *sig) {
+    if (!c || !c->nlev) return fail(c, -1, "lb_set_level_sig: no pair levels");
+    HIPCHK(c, hipSetDevice(c->device));
+    return upload(c, c->sig, sig, sizeof(double) * c->soff.back());
+}
+
+int lb_set_vcs(lb_ctx *c, int nvcs, const int32_t *labs, const int32_t *P, const double *rq, const int64_t *ptr,
+               const int32_t *col, const double *sig, double wv) {
+    if (!c || !c->n) return fail(c, -1, "lb_set_vcs: set the graph first");
+    if (nvcs < 0 || nvcs > LB_MAXVCS) return fail(c, -1, "lb_set_vcs: at most 4 VCS levels");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (nvcs == 0) { int rc = empty_vcs(c); return rc ? rc : ensure_eval_buffers(c); }
+    const int64_t n = c->n;
+    std::vector<int64_t> absptr((size_t)nvcs * (n + 1));
+    std::vector<int64_t> vnnz, voff;
+    int64_t tot = 0, sP = 0;
+    for (int m = 0; m < nvcs; m++) {
+        const int64_t *pm = ptr + (int64_t)m * (n + 1);
+        if (P[m] <= 0) return fail(c, -1, "lb_set_vcs: P must be positive");
+        if (pm[0] != 0) return fail(c, -1, "lb_set_vcs: ptr must start at 0");
+        for (int64_t i = 0; i < n; i++) {
+            if (pm[i + 1] < pm[i]) return fail(c, -1, "lb_set_vcs: ptr not monotone");
+            const int32_t lb = labs[(int64_t)m * n + i];
+            if (lb < 0 || lb >= P[m]) return fail(c, -1, "lb_set_vcs: label out of range");
+        }
+        for (int64_t e = 0; e < pm[n]; e++)
+            if (col[tot + e] < 0 || col[tot + e] >= P[m]) return fail(c, -1, "lb_set_vcs: working-set cluster out of range");
+        for (int64_t i = 0; i <= n; i++) absptr[(size_t)m * (n + 1) + i] = pm[i] + tot;
+        voff.push_back(tot); vnnz.push_back(pm[n]); tot += pm[n]; sP += P[m];
+    }
+    c->nvcs = nvcs; c->vP.assign(P, P + nvcs); c->vnnz = vnnz; c->voff = voff; c->sumPv = (int)sP; c->wv = wv;
+    int rc;
+    if ((rc = upload(c, c->vlabs, labs, sizeof(int32_t) * (size_t)nvcs * n))) return rc;
+    if ((rc = upload(c, c->vptr, absptr.data(), sizeof(int64_t) * absptr.size()))) return rc;
+    if ((rc = upload(c, c->vcol, col, sizeof(int32_t) * std::max<int64_t>(tot, 1)))) return rc;
+    if ((rc = upload_tc(c, c->vsig, sig, tot))) return rc;
+    if ((rc = dalloc(c, c->pv, c->cbytes * std::max<int64_t>(tot, 1)))) return rc;
+    if ((rc = upload(c, c->vrq, rq, sizeof(double) * sP))) return rc;
+    c->have_eval = false;
+    return ensure_eval_buffers(c);
+}
+
+int lb_set_vcs_sig(lb_ctx *c, const double *sig) {
+    if (!c || !c->nvcs) return fail(c, -1, "lb_set_vcs_sig: no VCS levels");
+    HIPCHK(c, hipSetDevice(c->device));
+    int64_t tot = 0;
+    for (int64_t x : c->vnnz) tot += x;
+    return upload_tc(c, c->vsig, sig, tot);
+}
+
+int lb_set_triangles(lb_ctx *c, int64_t T, const int32_t *TI, const double *lam) {
+    if (!c || !c->n) return fail(c, -1, "lb_set_triangles: set the graph first");
+    if (T < 0 || T > ((int64_t)1 << 29)) return fail(c, -1, "lb_set_triangles: bad T");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = c->n;
+    int rc;
+    if (T == 0) {
+        c->T = 0;
+        if ((rc = empty_triangles(c)) || (rc = rebuild_segs(c)) || (rc = refresh_lamsum(c))) return rc;
+        c->have_eval = false;
+        return 0;
+    }
+    for (int64_t e = 0; e < 3 * T; e++)
+        if (TI[e] < 0 || TI[e] >= n) return fail(c, -1, "lb_set_triangles: vertex out of range");
+    // vertex -> (t, role) incidence in t order (alm_vcs.py:94-104)
+    std::vector<int64_t> cnt(n + 1, 0);
+    for (int64_t e = 0; e < 3 * T; e++) cnt[TI[e] + 1]++;
+    for (int64_t i = 0; i < n; i++) cnt[i + 1] += cnt[i];
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    const bool dbl = c->cbytes == 8;
+    const size_t rsz = dbl ? sizeof(TriRec<double>) : sizeof(TriRec<float>);
+    std::vector<unsigned char> recs((size_t)3 * T * rsz);
+    for (int64_t t = 0; t < T; t++) {
+        const int32_t a = TI[3 * t], b = TI[3 * t + 1], cc = TI[3 * t + 2];
+        for (int r = 0; r < 3; r++) {
+            const int32_t i = TI[3 * t + r];
+            const int64_t k = pos[i]++;
+            int32_t u, w;
+            if (r == 0) { u = b; w = cc; } else if (r == 1) { u = a; w = cc; } else { u = a; w = b; }
+            if (dbl) {
+                TriRec<double> x{u, w, (int32_t)(3 * t + r), 0, lam[t]};
+                std::memcpy(&recs[k * rsz], &x, rsz);
+            } else {
+                TriRec<float> x{u, w, (int32_t)(3 * t + r), (float)lam[t]};
+                std::memcpy(&recs[k * rsz], &x, rsz);
+            }
+        }
+    }
+    c->T = T;
+    if ((rc = upload(c, c->iptr, cnt.data(), sizeof(int64_t) * (n + 1)))) return rc;
+    if ((rc = upload(c, c->rec, recs.data(), recs.size()))) return rc;
+    if ((rc = upload(c, c->TI, TI, sizeof(int32_t) * 3 * T))) return rc;
+    if ((rc = upload_tc(c, c->lam, lam, T))) return rc;
+    if ((rc = dalloc(c, c->pt, c->cbytes * T))) return rc;
+    build_segments(n, cnt.data(), c->heavy_tris, c->seg_len, 1, c->hT_rows, c->hT_seg0, c->segT, (int32_t)c->segE.size());
+    c->built_heavyT = c->heavy_tris;
+    if ((rc = upload(c, c->hTrows, c->hT_rows.data(), sizeof(int32_t) * std::max<size_t>(c->hT_rows.size(), 1)))) return rc;
+    if ((rc = upload(c, c->hTseg0, c->hT_seg0.data(), sizeof(int32_t) * c->hT_seg0.size()))) return rc;
+    if ((rc = rebuild_segs(c))) return rc;
+    if ((rc = refresh_lamsum(c))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_eval = false;
+    return 0;
+}
+
+int lb_set_lam(lb_ctx *c, const double *lam) {
+    if (!c || !c->n) return fail(c, -1, "lb_set_lam: no graph");
+    if (!c->T) return 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = upload_tc(c, c->lam, lam, c->T))) return rc;
+    const int64_t nrec = 3 * c->T;
+    if (c->cbytes == 8) hipLaunchKernelGGL(lb_k_inc_lam<double>, dim3(1024), dim3(256), 0, c->stream, (TriRec<double> *)c->rec.p, nrec, (const double *)c->lam.p);
+    else hipLaunchKernelGGL(lb_k_inc_lam<float>, dim3(1024), dim3(256), 0, c->stream, (TriRec<float> *)c->rec.p, nrec, (const float *)c->lam.p);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = refresh_lamsum(c))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lb_set_V(lb_ctx *c, const void *V, int host_dtype) {
+    if (!c || !c->n || !V) return fail(c, -1, "lb_set_V: no graph or V");
+    if (host_dtype != LB_F64 && host_dtype != LB_F32) return fail(c, -1, "lb_set_V: host dtype must be LB_F64 or LB_F32");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t cnt = (size_t)c->n * c->r;
+    auto val = [&](size_t e) -> double { return host_dtype == LB_F64 ? ((const double *)V)[e] : (double)((const float *)V)[e]; };
+    if (c->vdtype == LB_F64) {
+        std::vector<double> t(cnt);
+        for (size_t e = 0; e < cnt; e++) t[e] = val(e);
+        return upload(c, c->V, t.data(), cnt * 8);
+    }
+    if (c->vdtype == LB_F32) {
+        std::vector<float> t(cnt);
+        for (size_t e = 0; e < cnt; e++) t[e] = (float)val(e);
+        return upload(c, c->V, t.data(), cnt * 4);
+    }
+    std::vector<uint16_t> t(cnt);
+    for (size_t e = 0; e < cnt; e++) t[e] = f2bf((float)val(e));
+    return upload(c, c->V, t.data(), cnt * 2);
+}
+
+int lb_get_V(lb_ctx *c, void *V, int host_dtype) {
+    if (!c || !c->n || !V) return fail(c, -1, "lb_get_V: no graph or V");
+    if (host_dtype != LB_F64 && host_dtype != LB_F32) return fail(c, -1, "lb_get_V: host dtype must be LB_F64 or LB_F32");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t cnt = (size_t)c->n * c->r;
+    std::vector<unsigned char> raw(cnt * c->vbytes);
+    HIPCHK(c, hipMemcpyAsync(raw.data(), c->V.p, raw.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t e = 0; e < cnt; e++) {
+        double v;
+        if (c->vdtype == LB_F64) v = ((const double *)raw.data())[e];
+        else if (c->vdtype == LB_F32) v = ((const float *)raw.data())[e];
+        else v = bf2f(((const uint16_t *)raw.data())[e]);
+        if (host_dtype == LB_F64) ((double *)V)[e] = v; else ((float *)V)[e] = (float)v;
+    }
+    return 0;
+}
+
+int lb_eval_fgrad(lb_ctx *c, double rho, double rt, double *parts) {
+    if (!c || !c->n) return fail(c, -1, "lb_eval_fgrad: no graph");
+    if (!(rho >= 0.0) || (rho > 0.0 && c->T && !(rt > 0.0))) return fail(c, -1, "lb_eval_fgrad: need rho >= 0 and rt > 0 when rho > 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    eval_fn f = pick_eval(c->vdtype, c->r);
+    if (!f) return fail(c, -1, "lb_eval_fgrad: unsupported rank / dtype");
+    int rc = f(c, rho, rt, nullptr);
+    if (rc) return rc;
+    c->have_eval = true; c->last_rho = rho;
+    if (parts) {
+        HIPCHK(c, hipMemcpyAsync(parts, c->parts.p, sizeof(double) * LB_NPARTS, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int lb_get_G(lb_ctx *c, void *G, int host_dtype) {
+    if (!c || !c->n || !G) return fail(c, -1, "lb_get_G: no graph or G");
+    if (!c->have_eval) return fail(c, -1, "lb_get_G: no evaluation yet");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t cnt = (size_t)c->n * c->r;
+    if (c->cbytes == 8) {
+        if (host_dtype == LB_F64) {
+            HIPCHK(c, hipMemcpyAsync(G, c->G.p, cnt * 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            return 0;
+        }
+        std::vector<double> t(cnt);
+        HIPCHK(c, hipMemcpyAsync(t.data(), c->G.p, cnt * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t e = 0; e < cnt; e++) ((float *)G)[e] = (float)t[e];
+        return 0;
+    }
+    if (host_dtype == LB_F32) {
+        HIPCHK(c, hipMemcpyAsync(G, c->G.p, cnt * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    std::vector<float> t(cnt);
+    HIPCHK(c, hipMemcpyAsync(t.data(), c->G.p, cnt * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t e = 0; e < cnt; e++) ((double *)G)[e] = t[e];
+    return 0;
+}
+
+int lb_get_multipliers(lb_ctx *c, double *pp, double *pt, double *pv) {
+    if (!c || !c->have_eval) return fail(c, -1, "lb_get_multipliers: no evaluation yet");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if (pp && c->nlev) {
+        HIPCHK(c, hipMemcpyAsync(pp, c->pp.p, sizeof(double) * c->soff.back(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (pt && c->T) { if ((rc = download_tc(c, c->pt, pt, c->T))) return rc; }
+    if (pv && c->nvcs) {
+        int64_t tot = 0;
+        for (int64_t x : c->vnnz) tot += x;
+        if ((rc = download_tc(c, c->pv, pv, tot))) return rc;
+    }
+    return 0;
+}
+
+int lb_phr_update(lb_ctx *c) {
+    if (!c || !c->have_eval) return fail(c, -1, "lb_phr_update: no evaluation yet");
+    if (!(c->last_rho > 0.0)) return fail(c, -1, "lb_phr_update: the last evaluation had rho = 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->nlev) HIPCHK(c, hipMemcpyAsync(c->sig.p, c->pp.p, sizeof(double) * c->soff.back(), hipMemcpyDeviceToDevice, c->stream));
+    if (c->nvcs) {
+        int64_t tot = 0;
+        for (int64_t x : c->vnnz) tot += x;
+        HIPCHK(c, hipMemcpyAsync(c->vsig.p, c->pv.p, c->cbytes * tot, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (c->T) {
+        HIPCHK(c, hipMemcpyAsync(c->lam.p, c->pt.p, c->cbytes * c->T, hipMemcpyDeviceToDevice, c->stream));
+        const int64_t nrec = 3 * c->T;
+        if (c->cbytes == 8) hipLaunchKernelGGL(lb_k_inc_lam<double>, dim3(1024), dim3(256), 0, c->stream, (TriRec<double> *)c->rec.p, nrec, (const double *)c->lam.p);
+        else hipLaunchKernelGGL(lb_k_inc_lam<float>, dim3(1024), dim3(256), 0, c->stream, (TriRec<float> *)c->rec.p, nrec, (const float *)c->lam.p);
+        HIPCHK(c, hipGetLastError());
+        int rc = refresh_lamsum(c);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_eval = false;
+    return 0;
+}
+
+int lb_sync(lb_ctx *c) {
+    if (!c) return -1;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lb_time_eval(lb_ctx *c, double rho, double rt, int iters, double *ms_per_eval, double *kernel_ms) {
+    if (!c || !c->n || iters <= 0) return fail(c, -1, "lb_time_eval: bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    eval_fn f = pick_eval(c->vdtype, c->r);
+    if (!f) return fail(c, -1, "lb_time_eval: unsupported rank / dtype");
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (int it = 0; it < iters; it++) {
+        int rc = f(c, rho, rt, nullptr);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    if (ms_per_eval) *ms_per_eval = ms / iters;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (kernel_ms) {
+        hipEvent_t ev[LB_NKERNELS + 1];
+        for (int k = 0; k <= LB_NKERNELS; k++) HIPCHK(c, hipEventCreate(&ev[k]));
+        for (int k = 0; k < LB_NKERNELS; k++) kernel_ms[k] = 0.0;
+        for (int it = 0; it < iters; it++) {
+            int rc = f(c, rho, rt, ev);
+            if (rc) return rc;
+            HIPCHK(c, hipEventSynchronize(ev[LB_NKERNELS]));
+            for (int k = 0; k < LB_NKERNELS; k++) {
+                float t = 0.f;
+                HIPCHK(c, hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+                kernel_ms[k] += t / iters;
+            }
+        }
+        for (int k = 0; k <= LB_NKERNELS; k++) hipEventDestroy(ev[k]);
+    }
+    c->have_eval = true; c->last_rho = rho;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,159 @@
+"""Host-side handle of one device context of liblbhip (include/lbhip.h).
+
+`Engine` uploads an `Instance` (graph, pair levels, VCS working sets, triangle pool) once, keeps V and every
+multiplier on the device, and exposes the search's device operations: the f+grad evaluation (the reference's
+`augmented` + `CV`, alm_vcs.py:1496-1533, 1299-1312), the PHR multiplier update (alm_vcs.py:1704-1711,1723) and the
+batched row-step sweeps. There is no CPU fallback: a missing library or device raises `LBError`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as LB
+from ._lib import LBError, ptr, _f64p, _i32p, _i64p
+
+DTYPES = {"f32": LB.LB_F32, "bf16": LB.LB_BF16, "f64": LB.LB_F64}
+PART_NAMES = ["f", "quad", "zz_over_L", "level", "tri", "vcs", "primal", "C0", "level_nviol", "tri_gmin",
+              "vcs_nviol", "lam_sum"]
+
+
+class Engine:
+    def __init__(self, r: int = 16, dtype: str = "f32", device: int = 0):
+        self.L = LB.load()
+        self.r = int(r)
+        self.dtype = dtype
+        self.ctx = C.c_void_p()
+        rc = self.L.lb_create(C.byref(self.ctx), int(device), DTYPES[dtype], self.r)
+        if rc != 0:
+            raise LBError(f"lb_create(device={device}, dtype={dtype}, r={r}) failed with status {rc} "
+                          "(no HIP device, or unsupported rank/dtype)")
+        self.inst = None
+
+    def close(self):
+        if self.ctx:
+            self.L.lb_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.lb_last_error(self.ctx)
+            raise LBError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+
+    # ------------------------------------------------------------------------------------------------ upload
+    def set_tuning(self, heavy_edges=64, heavy_tris=64, seg_len=128):
+        self._chk(self.L.lb_set_tuning(self.ctx, heavy_edges, heavy_tris, seg_len), "lb_set_tuning")
+
+    def set_instance(self, inst):
+        n = inst.n
+        self._keep = []
+        rowptr = np.ascontiguousarray(inst.rowptr, np.int64)
+        col = np.ascontiguousarray(inst.col, np.int32)
+        w = None if inst.w is None else np.ascontiguousarray(inst.w, np.float64)
+        d = np.ascontiguousarray(inst.d, np.float64)
+        self._chk(self.L.lb_set_graph(self.ctx, n, ptr(rowptr, _i64p), ptr(col, _i32p), ptr(w, _f64p), ptr(d, _f64p),
+                                      float(inst.L), float(inst.C0)), "lb_set_graph")
+        self.set_levels(inst)
+        self.set_vcs(inst)
+        self.set_triangles(inst.TI, inst.lam)
+        self.inst = inst
+
+    def set_levels(self, inst):
+        lv = inst.levels
+        if not lv:
+            self._chk(self.L.lb_set_levels(self.ctx, 0, None, None, None, None, None, 1.0), "lb_set_levels")
+            return
+        labs = np.ascontiguousarray(np.concatenate([l.lab for l in lv]), np.int32)
+        P = np.array([l.P for l in lv], np.int32)
+        sig = np.ascontiguousarray(np.concatenate([l.sig.ravel() for l in lv]), np.float64)
+        rq = np.ascontiguousarray(np.concatenate([l.rq for l in lv]), np.float64)
+        dsq = np.ascontiguousarray(np.concatenate([l.dsq for l in lv]), np.float64)
+        self._chk(self.L.lb_set_levels(self.ctx, len(lv), ptr(labs, _i32p), ptr(P, _i32p), ptr(sig, _f64p),
+                                       ptr(rq, _f64p), ptr(dsq, _f64p), float(inst.pairlvl_weight)), "lb_set_levels")
+
+    def set_level_sig(self, sigs):
+        sig = np.ascontiguousarray(np.concatenate([np.asarray(s, np.float64).ravel() for s in sigs]))
+        self._chk(self.L.lb_set_level_sig(self.ctx, ptr(sig, _f64p)), "lb_set_level_sig")
+
+    def set_vcs(self, inst):
+        vc = inst.vcs
+        if not vc:
+            self._chk(self.L.lb_set_vcs(self.ctx, 0, None, None, None, None, None, None, 1.0), "lb_set_vcs")
+            return
+        labs = np.ascontiguousarray(np.concatenate([v.lab for v in vc]), np.int32)
+        P = np.array([v.P for v in vc], np.int32)
+        rq = np.ascontiguousarray(np.concatenate([v.rq for v in vc]), np.float64)
+        vptr = np.ascontiguousarray(np.concatenate([v.ptr for v in vc]), np.int64)
+        col = np.ascontiguousarray(np.concatenate([v.col for v in vc]) if sum(v.nnz for v in vc) else np.zeros(1), np.int32)
+        sig = np.ascontiguousarray(np.concatenate([v.sig for v in vc]) if sum(v.nnz for v in vc) else np.zeros(1), np.float64)
+        self._chk(self.L.lb_set_vcs(self.ctx, len(vc), ptr(labs, _i32p), ptr(P, _i32p), ptr(rq, _f64p),
+                                    ptr(vptr, _i64p), ptr(col, _i32p), ptr(sig, _f64p), float(inst.wv)), "lb_set_vcs")
+
+    def set_vcs_sig(self, sigs):
+        sig = np.ascontiguousarray(np.concatenate([np.asarray(s, np.float64) for s in sigs]))
+        self._chk(self.L.lb_set_vcs_sig(self.ctx, ptr(sig, _f64p)), "lb_set_vcs_sig")
+
+    def set_triangles(self, TI, lam):
+        TI = np.ascontiguousarray(TI, np.int32).reshape(-1, 3)
+        lam = np.ascontiguousarray(lam, np.float64)
+        self._chk(self.L.lb_set_triangles(self.ctx, len(TI), ptr(TI, _i32p), ptr(lam, _f64p)), "lb_set_triangles")
+
+    def set_lam(self, lam):
+        lam = np.ascontiguousarray(lam, np.float64)
+        self._chk(self.L.lb_set_lam(self.ctx, ptr(lam, _f64p)), "lb_set_lam")
+
+    def set_V(self, V):
+        V = np.ascontiguousarray(V, np.float64)
+        assert V.shape == (self.inst.n, self.r), (V.shape, self.inst.n, self.r)
+        self._chk(self.L.lb_set_V(self.ctx, V.ctypes.data_as(C.c_void_p), LB.LB_F64), "lb_set_V")
+
+    def get_V(self) -> np.ndarray:
+        out = np.empty((self.inst.n, self.r), np.float64)
+        self._chk(self.L.lb_get_V(self.ctx, out.ctypes.data_as(C.c_void_p), LB.LB_F64), "lb_get_V")
+        return out
+
+    # ------------------------------------------------------------------------------------------------ compute
+    def eval(self, rho: float, rt: float | None = None, fetch: bool = True):
+        rt = rho if rt is None else rt
+        if not fetch:
+            self._chk(self.L.lb_eval_fgrad(self.ctx, float(rho), float(rt), None), "lb_eval_fgrad")
+            return None
+        parts = np.zeros(LB.LB_NPARTS)
+        self._chk(self.L.lb_eval_fgrad(self.ctx, float(rho), float(rt), ptr(parts, _f64p)), "lb_eval_fgrad")
+        return parts
+
+    def get_G(self) -> np.ndarray:
+        out = np.empty((self.inst.n, self.r), np.float64)
+        self._chk(self.L.lb_get_G(self.ctx, out.ctypes.data_as(C.c_void_p), LB.LB_F64), "lb_get_G")
+        return out
+
+    def get_multipliers(self):
+        inst = self.inst
+        pp = np.zeros(max(1, sum(l.P * l.P for l in inst.levels)))
+        pt = np.zeros(max(1, inst.T))
+        pv = np.zeros(max(1, sum(v.nnz for v in inst.vcs)))
+        self._chk(self.L.lb_get_multipliers(self.ctx, ptr(pp, _f64p), ptr(pt, _f64p), ptr(pv, _f64p)),
+                  "lb_get_multipliers")
+        return (pp[:sum(l.P * l.P for l in inst.levels)], pt[:inst.T], pv[:sum(v.nnz for v in inst.vcs)])
+
+    def phr_update(self):
+        self._chk(self.L.lb_phr_update(self.ctx), "lb_phr_update")
+
+    def sync(self):
+        self._chk(self.L.lb_sync(self.ctx), "lb_sync")
+
+    def time_eval(self, rho: float, rt: float | None = None, iters: int = 20, per_kernel: bool = True):
+        rt = rho if rt is None else rt
+        ms = np.zeros(1)
+        km = np.zeros(LB.LB_NKERNELS)
+        self._chk(self.L.lb_time_eval(self.ctx, float(rho), float(rt), int(iters), ptr(ms, _f64p),
+                                      ptr(km, _f64p) if per_kernel else None), "lb_time_eval")
+        names = [self.L.lb_kernel_name(k).decode() for k in range(LB.LB_NKERNELS)]
+        return float(ms[0]), dict(zip(names, km.tolist())) if per_kernel else None

@@ -1,0 +1,389 @@
+/* FP64 CPU restatement of lagbound's ALM search kernels -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This file is the oracle: the parity checker for the HIP path and the timed CPU baseline. It is never linked into
+ * or called by the product library. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ *
+ * Every function restates the loop structure of the reference's numba kernel it cites (prange -> omp parallel for,
+ * same per-row / per-entry arithmetic order, FP64 throughout). Citations are into the reference repository
+ * (src/lagbound/_core/...).
+ *
+ * Pinning (see tests/test_oracle.py): the evaluation (or_eval) is checked against a dense brute-force evaluation of
+ * the mathematical definitions on small graphs (C(y) assembled entry by entry), against central finite differences
+ * of f (gradient = 2 C(y) V in the tangent space), and against a vectorised numpy/scipy restatement at medium size.
+ * The row step (or_batch_step) is checked against its closed form (unit rows, minorant step direction) and against
+ * the numpy restatement.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+    int64_t n;
+    int32_t r;
+    const int64_t *rowptr;  /* n+1 */
+    const int32_t *col;     /* nnz */
+    const double *w;        /* nnz or NULL (unweighted) */
+    const double *d;        /* n */
+    double L, C0;
+    /* pair levels */
+    int32_t nlev;
+    const int32_t *labs;    /* nlev x n */
+    const int32_t *P;       /* nlev */
+    const double *sig;      /* concat P_l^2 */
+    const double *rq;       /* concat P_l */
+    const double *dsq;      /* concat P_l */
+    double pw;              /* pairlvl_weight */
+    /* sparse VC levels */
+    int32_t nvcs;
+    const int32_t *vlabs;   /* nvcs x n */
+    const int32_t *vP;      /* nvcs */
+    const double *vrq;      /* concat vP */
+    const int64_t *vptr;    /* nvcs x (n+1), each relative to its own level's entries */
+    const int32_t *vcol;    /* concat nnz_m */
+    const double *vsig;     /* concat nnz_m */
+    double wv;              /* vc_weight * pairlvl_weight */
+    /* triangles */
+    int64_t T;
+    const int32_t *TI;      /* T x 3 */
+    const double *lam;      /* T */
+} or_inst;
+
+static int64_t sumP(const int32_t *P, int32_t k) { int64_t s = 0; for (int32_t i = 0; i < k; i++) s += P[i]; return s; }
+static int64_t sumP2(const int32_t *P, int32_t k) { int64_t s = 0; for (int32_t i = 0; i < k; i++) s += (int64_t)P[i] * P[i]; return s; }
+
+/* vertex -> (t, role) incidence CSR, codes 3t+role in t order (alm_vcs.py:94-104) */
+void or_incidence(int64_t n, int64_t T, const int32_t *TI, int64_t *cnt, int64_t *codes) {
+    memset(cnt, 0, sizeof(int64_t) * (n + 1));
+    for (int64_t t = 0; t < T; t++)
+        for (int r = 0; r < 3; r++) cnt[TI[3 * t + r] + 1] += 1;
+    for (int64_t i = 0; i < n; i++) cnt[i + 1] += cnt[i];
+    int64_t *pos = (int64_t *)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) pos[i] = cnt[i];
+    for (int64_t t = 0; t < T; t++)
+        for (int r = 0; r < 3; r++) { int64_t i = TI[3 * t + r]; codes[pos[i]++] = 3 * t + r; }
+    free(pos);
+}
+
+/* out = A X (alm_vcs.py:528-535) */
+void or_spmm(const or_inst *I, const double *X, double *out) {
+    const int k = I->r;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < I->n; i++) {
+        double *o = out + i * k;
+        for (int c = 0; c < k; c++) o[c] = 0.0;
+        for (int64_t e = I->rowptr[i]; e < I->rowptr[i + 1]; e++) {
+            int64_t j = I->col[e]; double w = I->w ? I->w[e] : 1.0;
+            for (int c = 0; c < k; c++) o[c] += w * X[j * k + c];
+        }
+    }
+}
+
+/* sum_ij A_ij V_i.V_j (alm_vcs.py:463-474) */
+double or_quadA(const or_inst *I, const double *V) {
+    const int k = I->r; double tot = 0.0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : tot)
+    for (int64_t i = 0; i < I->n; i++) {
+        double s = 0.0;
+        for (int64_t e = I->rowptr[i]; e < I->rowptr[i + 1]; e++) {
+            int64_t j = I->col[e]; double dt = 0.0;
+            for (int c = 0; c < k; c++) dt += V[i * k + c] * V[j * k + c];
+            s += (I->w ? I->w[e] : 1.0) * dt;
+        }
+        tot += s;
+    }
+    return tot;
+}
+
+/* out[lab(i)] += d_i V_i (cluster_sums, alm_vcs.py:511-526): per-thread partial sums over contiguous row ranges,
+ * then a fixed-order reduction over the partials. */
+void or_cluster_sums(int64_t n, int k, const int32_t *lab, int32_t P, const double *d, const double *V, double *out) {
+    int nthr = 1;
+#ifdef _OPENMP
+    nthr = omp_get_max_threads();
+    if (nthr > 16) nthr = 16;
+#endif
+    double *part = (double *)calloc((size_t)nthr * P * k, sizeof(double));
+    int64_t cs = (n + nthr - 1) / nthr;
+#pragma omp parallel for num_threads(nthr)
+    for (int t = 0; t < nthr; t++) {
+        double *pt = part + (size_t)t * P * k;
+        int64_t i1 = (t + 1) * cs < n ? (t + 1) * cs : n;
+        for (int64_t i = t * cs; i < i1; i++) {
+            double di = d[i]; int64_t Q = lab[i];
+            for (int c = 0; c < k; c++) pt[Q * k + c] += di * V[i * k + c];
+        }
+    }
+    for (int64_t q = 0; q < (int64_t)P * k; q++) {
+        double s = 0.0;
+        for (int t = 0; t < nthr; t++) s += part[(size_t)t * P * k + q];
+        out[q] = s;
+    }
+    free(part);
+}
+
+/* level term of augmented() (level_aug, alm_vcs.py:476-496): h = sym(H), rr = rho (rq_Q rq_R) w,
+ * pp = max(0, sig - rr h), returns sum (pp^2 - sig^2)/(2 rr); *nviol = #(h < -1e-9). */
+double or_level_aug(int32_t P, const double *H, const double *sig, const double *rq, double w, double rho,
+                    double *pp, int64_t *nviol) {
+    double tot = 0.0; int64_t cnt = 0;
+#pragma omp parallel for reduction(+ : tot, cnt)
+    for (int32_t Q = 0; Q < P; Q++) {
+        double s = 0.0; int64_t c = 0;
+        for (int32_t R = 0; R < P; R++) {
+            double h = 0.5 * (H[(int64_t)Q * P + R] + H[(int64_t)R * P + Q]);
+            double rr = rho * ((rq[Q] * rq[R]) * w), sg = sig[(int64_t)Q * P + R];
+            double p = sg - rr * h;
+            if (p < 0.0) p = 0.0;
+            pp[(int64_t)Q * P + R] = p; s += (p * p - sg * sg) / (2 * rr);
+            if (h < -1e-9) c++;
+        }
+        tot += s; cnt += c;
+    }
+    *nviol = cnt;
+    return tot;
+}
+
+/* tri_aug (alm_vcs.py:498-509): gt = 1 - tau, pt = max(0, lam - rt gt); returns sum (pt^2 - lam^2)/(2 rt). */
+double or_tri_aug(int64_t T, const int32_t *TI, int k, const double *V, const double *lam, double rt, double *gt,
+                  double *pt, double *gmin_out) {
+    double tot = 0.0, gmin = INFINITY;
+#pragma omp parallel for reduction(+ : tot) reduction(min : gmin)
+    for (int64_t t = 0; t < T; t++) {
+        int64_t a = TI[3 * t], b = TI[3 * t + 1], c = TI[3 * t + 2]; double s = 0.0;
+        for (int q = 0; q < k; q++) s += V[a * k + q] * V[b * k + q] + V[b * k + q] * V[c * k + q] - V[a * k + q] * V[c * k + q];
+        double g = 1.0 - s; if (gt) gt[t] = g;
+        double p = lam[t] - rt * g; if (p < 0.0) p = 0.0;
+        pt[t] = p; tot += (p * p - lam[t] * lam[t]) / (2 * rt);
+        if (g < gmin) gmin = g;
+    }
+    *gmin_out = gmin;
+    return tot;
+}
+
+/* out += T(w) V via the incidence (tri_apply, alm_vcs.py:106-120) */
+void or_tri_apply(int64_t n, int k, const int32_t *TI, const double *w, const int64_t *ptr, const int64_t *codes,
+                  const double *V, double *out) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; i++) {
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+            int64_t code = codes[e], t = code / 3; int r = (int)(code % 3); double h = 0.5 * w[t];
+            if (h == 0.0) continue;
+            int64_t a = TI[3 * t], b = TI[3 * t + 1], c = TI[3 * t + 2];
+            if (r == 0) for (int q = 0; q < k; q++) out[i * k + q] += h * (V[c * k + q] - V[b * k + q]);
+            else if (r == 1) for (int q = 0; q < k; q++) out[i * k + q] -= h * (V[a * k + q] + V[c * k + q]);
+            else for (int q = 0; q < k; q++) out[i * k + q] += h * (V[a * k + q] - V[b * k + q]);
+        }
+    }
+}
+
+/* VCS augmented term over the working set (_vcs_aug_rows, vcsops.py:266-288), Y = W V of the level.
+ * pp = max(0, sig - rho r g), r = wv rq_R / d_i^2, g = d_i (V_i.Y_R - [R = R(i)] d_i). */
+double or_vcs_aug(int64_t n, int k, const double *V, const double *d, const int32_t *lab, const double *Y,
+                  const int64_t *ptr, const int32_t *col, const double *sig, const double *rq, double wv, double rho,
+                  double *pp, int64_t *nviol) {
+    double tot = 0.0; int64_t cnt = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : tot, cnt)
+    for (int64_t i = 0; i < n; i++) {
+        double di = d[i], acc = 0.0; int32_t Ri = lab[i]; int64_t cc = 0;
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+            int32_t R = col[e]; double s = 0.0;
+            for (int c = 0; c < k; c++) s += V[i * k + c] * Y[(int64_t)R * k + c];
+            double g = di * s;
+            if (R == Ri) g = di * (s - di);
+            double sg = sig[e], p = sg;
+            if (rho > 0.0 && di > 0.0) {
+                double rr = rho * (wv * rq[R] / (di * di)); p = sg - rr * g; if (p < 0.0) p = 0.0;
+                acc += (p * p - sg * sg) / (2 * rr);
+            }
+            pp[e] = p;
+            if (g < -1e-9) cc++;
+        }
+        tot += acc; cnt += cc;
+    }
+    *nviol = cnt;
+    return tot;
+}
+
+/* out += S_vc X = 1/2 d (Sig (W X) + (Sig^T (d X))[lab]) - own d^2 X (vcs_apply, vcsops.py:326-330) */
+void or_vcs_apply(int64_t n, int k, int32_t P, const int32_t *lab, const double *d, const int64_t *ptr,
+                  const int32_t *col, const double *sig, const double *X, double *out) {
+    double *YX = (double *)calloc((size_t)P * k, sizeof(double));
+    double *ZX = (double *)calloc((size_t)P * k, sizeof(double));
+    for (int64_t i = 0; i < n; i++) {
+        for (int c = 0; c < k; c++) YX[(int64_t)lab[i] * k + c] += d[i] * X[i * k + c];
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; e++)
+            for (int c = 0; c < k; c++) ZX[(int64_t)col[e] * k + c] += sig[e] * d[i] * X[i * k + c];
+    }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; i++) {
+        double di = d[i], own = 0.0; int32_t Ri = lab[i];
+        double acc[64];
+        for (int c = 0; c < k; c++) acc[c] = 0.0;
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+            int32_t R = col[e];
+            if (R == Ri) own = sig[e];
+            for (int c = 0; c < k; c++) acc[c] += sig[e] * YX[(int64_t)R * k + c];
+        }
+        for (int c = 0; c < k; c++)
+            out[i * k + c] += 0.5 * di * (acc[c] + ZX[(int64_t)Ri * k + c]) - own * di * di * X[i * k + c];
+    }
+    free(YX); free(ZX);
+}
+
+/* One f+grad evaluation.
+ *   rho > 0: f = augmented(V, rho) (alm_vcs.py:1496-1533): quadA - |d^T V|^2/L - sum_l level_aug - tri_aug - vcs_aug
+ *            + C0, effective multipliers pp (levels), pt (triangles, rt = rho * tri_weight), pv (VCS);
+ *   rho = 0: f = C0 + sum(lam) + sum_i (C V)_i . V_i  (est_dual_vcs.value, est_dual_vcs.py:245-246), multipliers as
+ *            stored.
+ *   G = C(y) V with y the effective multipliers (CV, alm_vcs.py:1299-1312; est_dual_vcs.py:227-243).
+ * parts: [0] f, [1] quadA (rho > 0) or sum_i G_i.V_i (rho = 0), [2] |z|^2/L, [3] sum of level terms, [4] triangle
+ * term, [5] VCS term, [6] primal (quadA - |z|^2/L + C0), [7] C0, [8] level #(h<-1e-9), [9] min triangle slack,
+ * [10] VCS #(g<-1e-9), [11] sum(lam).
+ * Returns 0 on success. */
+int or_eval(const or_inst *I, const double *V, double rho, double rt, double *parts, double *G, double *pp,
+            double *pt, double *pv, const int64_t *inc_ptr, const int64_t *inc_codes) {
+    const int64_t n = I->n; const int k = I->r;
+    for (int q = 0; q < 12; q++) parts[q] = 0.0;
+    double *z = (double *)calloc(k, sizeof(double));
+    for (int64_t i = 0; i < n; i++)
+        for (int c = 0; c < k; c++) z[c] += I->d[i] * V[i * k + c];
+    double zz = 0.0;
+    for (int c = 0; c < k; c++) zz += z[c] * z[c];
+    double lamsum = 0.0;
+    for (int64_t t = 0; t < I->T; t++) lamsum += I->lam[t];
+    parts[11] = lamsum; parts[7] = I->C0; parts[2] = zz / I->L;
+
+    /* effective multipliers */
+    int64_t so = 0, po = 0;
+    double levtot = 0.0; int64_t levviol = 0;
+    double **Ys = (double **)calloc(I->nlev > 0 ? I->nlev : 1, sizeof(double *));
+    for (int32_t l = 0; l < I->nlev; l++) {
+        int32_t P = I->P[l];
+        double *Y = (double *)calloc((size_t)P * k, sizeof(double));
+        or_cluster_sums(n, k, I->labs + (int64_t)l * n, P, I->d, V, Y);
+        Ys[l] = Y;
+        if (rho > 0.0) {
+            double *H = (double *)malloc(sizeof(double) * (size_t)P * P);
+            for (int32_t Q = 0; Q < P; Q++)
+                for (int32_t R = 0; R < P; R++) {
+                    double s = 0.0;
+                    for (int c = 0; c < k; c++) s += Y[(int64_t)Q * k + c] * Y[(int64_t)R * k + c];
+                    H[(int64_t)Q * P + R] = s;
+                }
+            for (int32_t Q = 0; Q < P; Q++) H[(int64_t)Q * P + Q] -= I->dsq[po + Q];
+            int64_t nv = 0;
+            levtot += or_level_aug(P, H, I->sig + so, I->rq + po, I->pw, rho, pp + so, &nv);
+            levviol += nv;
+            free(H);
+        } else {
+            memcpy(pp + so, I->sig + so, sizeof(double) * (size_t)P * P);
+        }
+        so += (int64_t)P * P; po += P;
+    }
+    parts[3] = levtot; parts[8] = (double)levviol;
+
+    double tritot = 0.0, gmin = INFINITY;
+    if (I->T) {
+        if (rho > 0.0) tritot = or_tri_aug(I->T, I->TI, k, V, I->lam, rt, NULL, pt, &gmin);
+        else memcpy(pt, I->lam, sizeof(double) * I->T);
+    }
+    parts[4] = tritot; parts[9] = gmin;
+
+    double vcstot = 0.0; int64_t vcsviol = 0; int64_t eo = 0, vpo = 0;
+    for (int32_t m = 0; m < I->nvcs; m++) {
+        int32_t P = I->vP[m]; const int32_t *lab = I->vlabs + (int64_t)m * n; const int64_t *ptr = I->vptr + (int64_t)m * (n + 1);
+        int64_t nnz = ptr[n];
+        double *Y = (double *)calloc((size_t)P * k, sizeof(double));
+        or_cluster_sums(n, k, lab, P, I->d, V, Y);
+        int64_t nv = 0;
+        if (rho > 0.0) vcstot += or_vcs_aug(n, k, V, I->d, lab, Y, ptr, I->vcol + eo, I->vsig + eo, I->vrq + vpo, I->wv, rho, pv + eo, &nv);
+        else {
+            or_vcs_aug(n, k, V, I->d, lab, Y, ptr, I->vcol + eo, I->vsig + eo, I->vrq + vpo, I->wv, 0.0, pv + eo, &nv);
+        }
+        vcsviol += nv;
+        free(Y);
+        eo += nnz; vpo += P;
+    }
+    parts[5] = vcstot; parts[10] = (double)vcsviol;
+
+    /* G = C(y) V */
+    or_spmm(I, V, G);
+#pragma omp parallel for
+    for (int64_t i = 0; i < n; i++)
+        for (int c = 0; c < k; c++) G[i * k + c] -= I->d[i] * z[c] / I->L;
+    so = 0; po = 0;
+    for (int32_t l = 0; l < I->nlev; l++) {
+        int32_t P = I->P[l]; const int32_t *lab = I->labs + (int64_t)l * n; const double *s = pp + so; double *Y = Ys[l];
+        double *SY = (double *)calloc((size_t)P * k, sizeof(double));
+        for (int32_t Q = 0; Q < P; Q++)
+            for (int32_t R = 0; R < P; R++)
+                for (int c = 0; c < k; c++) SY[(int64_t)Q * k + c] += s[(int64_t)Q * P + R] * Y[(int64_t)R * k + c];
+#pragma omp parallel for
+        for (int64_t i = 0; i < n; i++) {
+            int32_t Q = lab[i]; double di = I->d[i];
+            for (int c = 0; c < k; c++)
+                G[i * k + c] += di * SY[(int64_t)Q * k + c] - (s[(int64_t)Q * P + Q] * di * di) * V[i * k + c];
+        }
+        free(SY);
+        so += (int64_t)P * P; po += P;
+    }
+    eo = 0;
+    for (int32_t m = 0; m < I->nvcs; m++) {
+        const int64_t *ptr = I->vptr + (int64_t)m * (n + 1);
+        or_vcs_apply(n, k, I->vP[m], I->vlabs + (int64_t)m * n, I->d, ptr, I->vcol + eo, pv + eo, V, G);
+        eo += ptr[n];
+    }
+    if (I->T) {
+        /* the incidence is rebuilt only when the pool changes (alm_vcs.py:1228,1762); callers pass it in */
+        if (inc_ptr && inc_codes) {
+            or_tri_apply(n, k, I->TI, pt, inc_ptr, inc_codes, V, G);
+        } else {
+            int64_t *cnt = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+            int64_t *codes = (int64_t *)malloc(sizeof(int64_t) * 3 * I->T);
+            or_incidence(n, I->T, I->TI, cnt, codes);
+            or_tri_apply(n, k, I->TI, pt, cnt, codes, V, G);
+            free(cnt); free(codes);
+        }
+    }
+
+    if (rho > 0.0) {
+        double qa = or_quadA(I, V);
+        parts[1] = qa;
+        parts[6] = qa - zz / I->L + I->C0;
+        parts[0] = qa - zz / I->L - levtot - tritot - vcstot + I->C0;
+    } else {
+        double gv = 0.0;
+#pragma omp parallel for reduction(+ : gv)
+        for (int64_t i = 0; i < n; i++) {
+            double s = 0.0;
+            for (int c = 0; c < k; c++) s += G[i * k + c] * V[i * k + c];
+            gv += s;
+        }
+        parts[1] = gv;
+        parts[6] = or_quadA(I, V) - zz / I->L + I->C0;
+        parts[0] = I->C0 + lamsum + gv;
+    }
+    for (int32_t l = 0; l < I->nlev; l++) free(Ys[l]);
+    free(Ys); free(z);
+    return 0;
+}
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void or_set_threads(int t) {
+#ifdef _OPENMP
+    if (t > 0) omp_set_num_threads(t);
+#else
+    (void)t;
+#endif
+}

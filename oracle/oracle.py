@@ -1,0 +1,311 @@
+"""FP64 CPU oracle for the ALM search -- TEST INFRASTRUCTURE ONLY.
+
+Three independent restatements of one f+grad evaluation (alm_vcs.py:1496-1533 `augmented`, alm_vcs.py:1299-1312
+`CV`; est_dual_vcs.py:227-246 at rho = 0):
+  * ``eval_c``      -- the C restatement in lb_oracle.c (reference loop structure, OpenMP), also the CPU baseline;
+  * ``eval_numpy``  -- a vectorised numpy/scipy restatement (scipy CSR products, bincounts);
+  * ``eval_dense``  -- brute force from the mathematical definitions on a dense X = V V^T (small n only):
+                       h_QR = sum_{i in Q, j in R, i != j} d_i d_j X_ij, tau_t = X_ab + X_bc - X_ac,
+                       g_iR = d_i sum_{j in R, j != i} d_j X_ij, C(y) assembled entry by entry.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use this module; the product path never
+imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import scipy.sparse as sp
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liblb_oracle.so")
+
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class OrInst(C.Structure):
+    _fields_ = [("n", C.c_int64), ("r", C.c_int32), ("rowptr", _i64p), ("col", _i32p), ("w", _f64p), ("d", _f64p),
+                ("L", C.c_double), ("C0", C.c_double),
+                ("nlev", C.c_int32), ("labs", _i32p), ("P", _i32p), ("sig", _f64p), ("rq", _f64p), ("dsq", _f64p),
+                ("pw", C.c_double),
+                ("nvcs", C.c_int32), ("vlabs", _i32p), ("vP", _i32p), ("vrq", _f64p), ("vptr", _i64p),
+                ("vcol", _i32p), ("vsig", _f64p), ("wv", C.c_double),
+                ("T", C.c_int64), ("TI", _i32p), ("lam", _f64p)]
+
+
+def build(force: bool = False):
+    """Compile liblb_oracle.so with the committed Makefile."""
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(os.path.join(HERE, "lb_oracle.c")):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB)
+        L.or_eval.argtypes = [C.POINTER(OrInst), _f64p, C.c_double, C.c_double, _f64p, _f64p, _f64p, _f64p, _f64p,
+                              _i64p, _i64p]
+        L.or_eval.restype = C.c_int
+        L.or_num_threads.restype = C.c_int
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_incidence.argtypes = [C.c_int64, C.c_int64, _i32p, _i64p, _i64p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else C.cast(None, t)
+
+
+class _Packed:
+    """Contiguous arrays of an Instance in the oracle's C layout; keeps them alive while the struct is used."""
+
+    def __init__(self, inst, r):
+        n = inst.n
+        self.rowptr = np.ascontiguousarray(inst.rowptr, np.int64)
+        self.col = np.ascontiguousarray(inst.col, np.int32)
+        self.w = None if inst.w is None else np.ascontiguousarray(inst.w, np.float64)
+        self.d = np.ascontiguousarray(inst.d, np.float64)
+        lv = inst.levels
+        self.labs = np.ascontiguousarray(np.concatenate([l.lab for l in lv]) if lv else np.zeros(1), np.int32)
+        self.P = np.array([l.P for l in lv] or [0], np.int32)
+        self.sig = np.ascontiguousarray(np.concatenate([l.sig.ravel() for l in lv]) if lv else np.zeros(1))
+        self.rq = np.ascontiguousarray(np.concatenate([l.rq for l in lv]) if lv else np.zeros(1))
+        self.dsq = np.ascontiguousarray(np.concatenate([l.dsq for l in lv]) if lv else np.zeros(1))
+        vc = inst.vcs
+        self.vlabs = np.ascontiguousarray(np.concatenate([v.lab for v in vc]) if vc else np.zeros(1), np.int32)
+        self.vP = np.array([v.P for v in vc] or [0], np.int32)
+        self.vrq = np.ascontiguousarray(np.concatenate([v.rq for v in vc]) if vc else np.zeros(1))
+        self.vptr = np.ascontiguousarray(np.concatenate([v.ptr for v in vc]) if vc else np.zeros(1), np.int64)
+        self.vcol = np.ascontiguousarray(np.concatenate([v.col for v in vc]) if vc and sum(v.nnz for v in vc) else np.zeros(1), np.int32)
+        self.vsig = np.ascontiguousarray(np.concatenate([v.sig for v in vc]) if vc and sum(v.nnz for v in vc) else np.zeros(1))
+        self.TI = np.ascontiguousarray(inst.TI if inst.T else np.zeros((1, 3)), np.int32)
+        self.lam = np.ascontiguousarray(inst.lam if inst.T else np.zeros(1), np.float64)
+        self.s = OrInst(n, r, _p(self.rowptr, _i64p), _p(self.col, _i32p), _p(self.w, _f64p), _p(self.d, _f64p),
+                        float(inst.L), float(inst.C0),
+                        len(lv), _p(self.labs, _i32p), _p(self.P, _i32p), _p(self.sig, _f64p), _p(self.rq, _f64p),
+                        _p(self.dsq, _f64p), float(inst.pairlvl_weight),
+                        len(vc), _p(self.vlabs, _i32p), _p(self.vP, _i32p), _p(self.vrq, _f64p), _p(self.vptr, _i64p),
+                        _p(self.vcol, _i32p), _p(self.vsig, _f64p), float(inst.wv),
+                        inst.T, _p(self.TI, _i32p), _p(self.lam, _f64p))
+
+
+PART_NAMES = ["f", "quad", "zz_over_L", "level", "tri", "vcs", "primal", "C0", "level_nviol", "tri_gmin",
+              "vcs_nviol", "lam_sum"]
+
+
+def eval_c(inst, V, rho, rt=None, threads: int | None = None, incidence=None):
+    """C oracle evaluation. Returns (parts[12], G, pp (concat P_l^2), pt (T), pv (concat nnz)). `incidence` =
+    (cnt, codes) from incidence_c() skips rebuilding the triangle incidence."""
+    L_ = lib()
+    if threads:
+        L_.or_set_threads(int(threads))
+    V = np.ascontiguousarray(V, np.float64)
+    n, r = V.shape
+    assert n == inst.n
+    rt = rho if rt is None else rt
+    pk = _Packed(inst, r)
+    parts = np.zeros(12)
+    G = np.zeros((n, r))
+    pp = np.zeros(max(1, sum(l.P * l.P for l in inst.levels)))
+    pt = np.zeros(max(1, inst.T))
+    pv = np.zeros(max(1, sum(v.nnz for v in inst.vcs)))
+    ip, ic = (None, None) if incidence is None else incidence
+    rc = L_.or_eval(C.byref(pk.s), _p(V, _f64p), float(rho), float(rt), _p(parts, _f64p), _p(G, _f64p),
+                    _p(pp, _f64p), _p(pt, _f64p), _p(pv, _f64p), _p(ip, _i64p), _p(ic, _i64p))
+    assert rc == 0
+    return parts, G, pp[:sum(l.P * l.P for l in inst.levels)], pt[:inst.T], pv[:sum(v.nnz for v in inst.vcs)]
+
+
+def incidence_c(n, TI):
+    L_ = lib()
+    TI = np.ascontiguousarray(TI, np.int32)
+    cnt = np.zeros(n + 1, np.int64)
+    codes = np.zeros(max(1, 3 * len(TI)), np.int64)
+    L_.or_incidence(n, len(TI), _p(TI, _i32p), _p(cnt, _i64p), _p(codes, _i64p))
+    return cnt, codes[:3 * len(TI)]
+
+
+# ----------------------------------------------------------------------------------------------- numpy restatement
+def _A(inst):
+    data = np.ones(inst.nnz) if inst.w is None else inst.w
+    return sp.csr_matrix((data, inst.col, inst.rowptr), shape=(inst.n, inst.n))
+
+
+def _W(lab, d, P):
+    n = len(lab)
+    return sp.csr_matrix((d, (lab, np.arange(n))), shape=(P, n))
+
+
+def eval_numpy(inst, V, rho, rt=None):
+    """Vectorised restatement of augmented()/CV() (rho > 0) and est value()/CV() (rho = 0)."""
+    rt = rho if rt is None else rt
+    V = np.asarray(V, np.float64)
+    n, k = V.shape
+    d = inst.d
+    L = inst.L
+    A = _A(inst)
+    AV = A @ V
+    z = d @ V
+    zz = float(z @ z)
+    parts = np.zeros(12)
+    parts[2] = zz / L
+    parts[7] = inst.C0
+    parts[11] = float(inst.lam.sum()) if inst.T else 0.0
+    G = AV - np.outer(d, z) / L
+    pps, levtot, levv = [], 0.0, 0
+    for lv in inst.levels:
+        Y = _W(lv.lab, d, lv.P) @ V
+        if rho > 0:
+            H = Y @ Y.T
+            H[np.diag_indices(lv.P)] -= lv.dsq
+            h = 0.5 * (H + H.T)
+            rr = rho * np.outer(lv.rq, lv.rq) * inst.pairlvl_weight
+            pp = np.maximum(0.0, lv.sig - rr * h)
+            levtot += float(((pp * pp - lv.sig * lv.sig) / (2 * rr)).sum())
+            levv += int((h < -1e-9).sum())
+        else:
+            pp = lv.sig.copy()
+        pps.append(pp)
+        G += d[:, None] * (pp @ Y)[lv.lab] - (np.diag(pp)[lv.lab] * d ** 2)[:, None] * V
+    pt = inst.lam.copy() if inst.T else np.zeros(0)
+    tritot, gmin = 0.0, np.inf
+    if inst.T:
+        a, b, c = (inst.TI[:, q].astype(np.int64) for q in range(3))
+        Va, Vb, Vc = V[a], V[b], V[c]
+        tau = np.einsum("ij,ij->i", Va, Vb) + np.einsum("ij,ij->i", Vb, Vc) - np.einsum("ij,ij->i", Va, Vc)
+        g = 1.0 - tau
+        if rho > 0:
+            pt = np.maximum(0.0, inst.lam - rt * g)
+            tritot = float(((pt * pt - inst.lam * inst.lam) / (2 * rt)).sum())
+            gmin = float(g.min())
+        h = 0.5 * pt[:, None]
+        np.add.at(G, a, h * (Vc - Vb))
+        np.add.at(G, b, -h * (Va + Vc))
+        np.add.at(G, c, h * (Va - Vb))
+    pvs, vcstot, vcsv = [], 0.0, 0
+    for vl in inst.vcs:
+        Y = _W(vl.lab, d, vl.P) @ V
+        rows = np.repeat(np.arange(n), np.diff(vl.ptr))
+        R = vl.col.astype(np.int64)
+        s = np.einsum("ij,ij->i", V[rows], Y[R])
+        di = d[rows]
+        g = di * s
+        own = R == vl.lab[rows]
+        g[own] = di[own] * (s[own] - di[own])
+        if rho > 0:
+            rr = rho * (inst.wv * vl.rq[R] / (di * di))
+            pv = np.maximum(0.0, vl.sig - rr * g)
+            vcstot += float(((pv * pv - vl.sig * vl.sig) / (2 * rr)).sum())
+        else:
+            pv = vl.sig.copy()
+        vcsv += int((g < -1e-9).sum())
+        pvs.append(pv)
+        S = sp.csr_matrix((pv, vl.col, vl.ptr), shape=(n, vl.P))
+        ZX = S.T @ (d[:, None] * V)
+        ownv = np.zeros(n)
+        ownv[rows[own]] = pv[own]
+        G += 0.5 * d[:, None] * (S @ Y + ZX[vl.lab]) - (ownv * d ** 2)[:, None] * V
+    qa = float(np.einsum("ij,ij->", AV, V))
+    if rho > 0:
+        parts[1] = qa
+        parts[0] = qa - zz / L - levtot - tritot - vcstot + inst.C0
+    else:
+        parts[1] = float(np.einsum("ij,ij->", G, V))
+        parts[0] = inst.C0 + parts[11] + parts[1]
+    parts[3], parts[4], parts[5] = levtot, tritot, vcstot
+    parts[6] = qa - zz / L + inst.C0
+    parts[8], parts[9], parts[10] = levv, gmin, vcsv
+    pp_all = np.concatenate([p.ravel() for p in pps]) if pps else np.zeros(0)
+    pv_all = np.concatenate(pvs) if pvs else np.zeros(0)
+    return parts, G, pp_all, pt, pv_all
+
+
+# ----------------------------------------------------------------------------------------------- dense brute force
+def dense_C(inst, pp_list, pt, pv_list):
+    """C(y) = B + sum_l S_l + sum_m S_vc,m + T(pt), assembled entry by entry from the definitions."""
+    n = inst.n
+    d = inst.d
+    Cm = inst.dense_A() - np.outer(d, d) / inst.L
+    for lv, pp in zip(inst.levels, pp_list):
+        for i in range(n):
+            for j in range(n):
+                if i != j:
+                    Cm[i, j] += d[i] * d[j] * pp[lv.lab[i], lv.lab[j]]
+    for vl, pv in zip(inst.vcs, pv_list):
+        sig = np.zeros((n, vl.P))
+        for i in range(n):
+            for e in range(vl.ptr[i], vl.ptr[i + 1]):
+                sig[i, vl.col[e]] = pv[e]
+        for i in range(n):
+            for j in range(n):
+                if i != j:
+                    Cm[i, j] += d[i] * d[j] * 0.5 * (sig[i, vl.lab[j]] + sig[j, vl.lab[i]])
+    for t in range(inst.T):
+        a, b, c = (int(x) for x in inst.TI[t])
+        w = pt[t]
+        Cm[a, b] -= 0.5 * w; Cm[b, a] -= 0.5 * w
+        Cm[b, c] -= 0.5 * w; Cm[c, b] -= 0.5 * w
+        Cm[a, c] += 0.5 * w; Cm[c, a] += 0.5 * w
+    return Cm
+
+
+def eval_dense(inst, V, rho, rt=None):
+    """Brute-force f and G from X = V V^T (n up to a few dozen)."""
+    rt = rho if rt is None else rt
+    V = np.asarray(V, np.float64)
+    n = inst.n
+    d = inst.d
+    X = V @ V.T
+    B = inst.dense_A() - np.outer(d, d) / inst.L
+    obj = float((B * X).sum())
+    pp_list, levtot = [], 0.0
+    for lv in inst.levels:
+        P = lv.P
+        h = np.zeros((P, P))
+        for i in range(n):
+            for j in range(n):
+                if i != j:
+                    h[lv.lab[i], lv.lab[j]] += d[i] * d[j] * X[i, j]
+        if rho > 0:
+            rr = rho * np.outer(lv.rq, lv.rq) * inst.pairlvl_weight
+            pp = np.maximum(0.0, lv.sig - rr * h)
+            levtot += float(((pp ** 2 - lv.sig ** 2) / (2 * rr)).sum())
+        else:
+            pp = lv.sig.copy()
+        pp_list.append(pp)
+    pt = inst.lam.copy() if inst.T else np.zeros(0)
+    tritot = 0.0
+    if inst.T and rho > 0:
+        tau = np.array([X[a, b] + X[b, c] - X[a, c] for a, b, c in inst.TI])
+        pt = np.maximum(0.0, inst.lam - rt * (1.0 - tau))
+        tritot = float(((pt ** 2 - inst.lam ** 2) / (2 * rt)).sum())
+    pv_list, vcstot = [], 0.0
+    for vl in inst.vcs:
+        pv = np.zeros(vl.nnz)
+        for i in range(n):
+            for e in range(vl.ptr[i], vl.ptr[i + 1]):
+                R = vl.col[e]
+                g = d[i] * sum(d[j] * X[i, j] for j in range(n) if j != i and vl.lab[j] == R)
+                if rho > 0:
+                    rr = rho * inst.wv * vl.rq[R] / d[i] ** 2
+                    pv[e] = max(0.0, vl.sig[e] - rr * g)
+                    vcstot += (pv[e] ** 2 - vl.sig[e] ** 2) / (2 * rr)
+                else:
+                    pv[e] = vl.sig[e]
+        pv_list.append(pv)
+    Cm = dense_C(inst, pp_list, pt, pv_list)
+    G = Cm @ V
+    if rho > 0:
+        f = obj - levtot - tritot - vcstot + inst.C0
+    else:
+        f = inst.C0 + (float(inst.lam.sum()) if inst.T else 0.0) + float((Cm * X).sum())
+    return f, G, pp_list, pt, pv_list

@@ -1,0 +1,52 @@
+"""The C-ABI library loads and exports every entry point include/lbhip.h declares (no GPU needed); without a HIP
+device the product path raises instead of falling back to the CPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from lagbound_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "lbhip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(lb_[a-z_A-Z0-9]+)\s*\(", src)))
+
+
+def test_header_matches_binding():
+    assert _declared() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_symbol():
+    lib = ctypes.CDLL(_lib.LIBPATH)
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_version_and_kernel_names():
+    L = _lib.load()
+    assert b"gfx950" in L.lb_version()
+    names = [L.lb_kernel_name(k) for k in range(_lib.LB_NKERNELS)]
+    assert b"lb_k_rows" in names and L.lb_kernel_name(99) == b""
+
+
+def test_bad_arguments_rejected_without_device():
+    L = _lib.load()
+    ctx = ctypes.c_void_p()
+    assert L.lb_create(ctypes.byref(ctx), 0, 0, 12) == -1      # unsupported rank
+    assert L.lb_create(ctypes.byref(ctx), 0, 7, 16) == -1      # unsupported dtype
+    assert L.lb_create(None, 0, 0, 16) == -1
+
+
+def test_no_cpu_fallback():
+    """Where there is no HIP device, creating an engine raises (nothing silently runs on the CPU)."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is present")
+    from lagbound_amd.engine import Engine
+    with pytest.raises(_lib.LBError):
+        Engine(r=16, dtype="f32")

@@ -1,0 +1,123 @@
+"""Parity of the HIP f+grad evaluation (liblbhip through the C ABI) with the FP64 oracle on the same inputs.
+
+Tolerances (stated by the precision contract of the search): FP32 and BF16-stored V: f within 1e-5 relative,
+||G - G64||_F / ||G64||_F <= 1e-5, with the oracle evaluated on the stored (for BF16: dequantised) V; FP64 path:
+1e-11 relative. Effective multipliers: pp (FP64 on the device) 1e-9 relative; pt within 1e-5 * rt absolute (tau in
+FP32); pv 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import small_instance, chung_lu_instance
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f32": 1e-5, "bf16": 1e-5, "f64": 1e-11}
+
+
+def _engine(inst, V, dtype, r, tuning=None):
+    from lagbound_amd.engine import Engine
+    eng = Engine(r=r, dtype=dtype)
+    if tuning:
+        eng.set_tuning(*tuning)
+    eng.set_instance(inst)
+    eng.set_V(V)
+    return eng
+
+
+def _check(eng, inst, rho, dtype, label):
+    print(f"[parity] {label} dtype={dtype} rho={rho} n={inst.n} T={inst.T}", flush=True)
+    parts = eng.eval(rho)
+    G = eng.get_G()
+    pp, pt, pv = eng.get_multipliers()
+    Vs = eng.get_V()                       # the stored V (BF16: dequantised)
+    po, Go, ppo, pto, pvo = O.eval_c(inst, Vs, rho)
+    tol = TOL[dtype]
+    f_err = abs(parts[0] - po[0]) / max(1.0, abs(po[0]))
+    g_err = np.linalg.norm(G - Go) / max(1e-300, np.linalg.norm(Go))
+    assert f_err <= tol, (label, "f", parts[0], po[0], f_err)
+    assert g_err <= tol, (label, "G", g_err)
+    for k in (1, 2, 3, 4, 5, 6):
+        assert abs(parts[k] - po[k]) <= max(tol * max(1.0, abs(po[k])), tol * max(1.0, abs(po[0]))), (label, k, parts[k], po[k])
+    assert parts[8] == po[8] or dtype != "f64"
+    assert np.allclose(pp, ppo, rtol=1e-9 if dtype != "f64" else 1e-12, atol=1e-12)
+    if inst.T:
+        rt = rho
+        assert np.abs(pt - pto).max() <= (1e-5 * max(rt, 1e-30) + 1e-6 * np.abs(pto).max()) if dtype != "f64" else 1e-12 * max(1.0, np.abs(pto).max())
+    if len(pvo):
+        assert np.allclose(pv, pvo, rtol=1e-6 if dtype != "f64" else 1e-12, atol=1e-30)
+    return parts, G
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f64"])
+@pytest.mark.parametrize("r", [8, 16, 32])
+def test_small_parity(dtype, r):
+    inst, V = small_instance(n=60, seed=r, k=r)
+    eng = _engine(inst, V, dtype, r)
+    for rho in (0.0, 0.3):
+        _check(eng, inst, rho, dtype, f"small r={r}")
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_small_weighted_and_segments(dtype):
+    """Weighted reduced-instance form (edge multiplicities, C0 > 0) and the hub segment path forced by tiny
+    thresholds (every row with more than 2 neighbours / triangle records is split into 3-item segments)."""
+    inst, V = small_instance(n=50, seed=3, k=16, weighted=True, ntri=60)
+    eng = _engine(inst, V, dtype, 16, tuning=(2, 2, 3))
+    for rho in (0.0, 0.5):
+        _check(eng, inst, rho, dtype, "weighted+segments")
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f64"])
+def test_chung_lu_parity(dtype):
+    inst, V = chung_lu_instance(n=30000, seed=1, k=16)
+    eng = _engine(inst, V, dtype, 16)
+    for rho in (0.0, 0.02):
+        _check(eng, inst, rho, dtype, "chung-lu 30k")
+
+
+def test_empty_pools():
+    """No levels, no VCS, no triangles: G = B V and f = <B, X> (+C0)."""
+    inst, V = small_instance(n=40, seed=2, k=16, ntri=0, vcs_per_row=0)
+    inst.levels = []
+    eng = _engine(inst, V, "f32", 16)
+    for rho in (0.0, 0.1):
+        _check(eng, inst, rho, "f32", "empty pools")
+
+
+def test_phr_update_on_device():
+    """lb_phr_update (sig <- pp, lam <- pt, VCS sig <- pv) equals the oracle evaluated with the updated multipliers."""
+    inst, V = chung_lu_instance(n=8000, seed=4, k=16)
+    eng = _engine(inst, V, "f64", 16)
+    rho = 0.05
+    eng.eval(rho)
+    pp, pt, pv = eng.get_multipliers()
+    eng.phr_update()
+    o = 0
+    for lv in inst.levels:
+        lv.sig = pp[o:o + lv.P * lv.P].reshape(lv.P, lv.P).copy()
+        o += lv.P * lv.P
+    inst.lam = pt.copy()
+    inst.vcs[0].sig = pv.copy()
+    _check(eng, inst, rho, "f64", "after phr update")
+    _check(eng, inst, 0.0, "f64", "after phr update, rho=0")
+
+
+def test_repeatable():
+    inst, V = chung_lu_instance(n=20000, seed=6, k=16)
+    eng = _engine(inst, V, "f32", 16)
+    p1 = eng.eval(0.02)
+    G1 = eng.get_G()
+    p2 = eng.eval(0.02)
+    G2 = eng.get_G()
+    assert np.linalg.norm(G1 - G2) <= 1e-6 * np.linalg.norm(G1)
+    assert abs(p1[0] - p2[0]) <= 1e-9 * abs(p1[0])
+
+
+def test_metric_instance_parity():
+    """The metric instance (n ~ 1M, r = 16, nnz ~ 13M, T ~ 3.4M, pair + VCS P = 64) in FP32 against the oracle."""
+    from lagbound_amd import instances as I
+    inst, V = I.metric_instance()
+    eng = _engine(inst, V, "f32", 16)
+    _check(eng, inst, 0.02, "f32", "metric instance")
