@@ -87,6 +87,15 @@ int lb_get_multipliers(lb_ctx *ctx, double *pp, double *pt, double *pv);
  * (alm_vcs.py:1704-1711,1723). */
 int lb_phr_update(lb_ctx *ctx);
 
+/* `nsw` sweeps of batched Gauss-Seidel row steps on the resident V (sweeps + batch_step6, alm_vcs.py:1360-1436,
+ * 255-351; pair_all alm_vcs.py:378-415; kq_level alm_vcs.py:417-428; VCS pre/post vcsops.py:127-213) at penalty
+ * rho, triangle penalty rt = rho * tri_weight, damping damp and VCS curvature factor kap2 (vc_kap2). orders holds
+ * nsw permutations of 0..n-1 (int64, e.g. numpy default_rng(seed).permutation(n) per sweep, alm_vcs.py:1387);
+ * the batches are np.array_split(order, nbatch) (alm_vcs.py:1408). Every row of a batch steps from the V and
+ * aggregates of the batch start (the reference's rows read each other's fresh values within a batch). */
+int lb_sweeps(lb_ctx *ctx, int nsw, double rho, double rt, double damp, double kap2, const int64_t *orders,
+              int nbatch);
+
 /* Wait for the context's stream. */
 int lb_sync(lb_ctx *ctx);
 

@@ -647,6 +647,495 @@ __global__ void lb_k_copy(TC *dst, const TC *src, int64_t cnt) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * blockDim.x) dst[e] = src[e];
 }
 
+// ------------------------------------------------------------------------------------------------ sweeps (device)
+// Batched Gauss-Seidel row steps (sweeps, alm_vcs.py:1360-1436 with pair_refresh = 1). Per sweep: z, Y (pair and
+// VCS levels), the curvature Kq (kq_level, alm_vcs.py:417-428), the VCS state stZ / Z / cnt (vcs_state,
+// vcsops.py:308-323) and the batch -> touched-cluster mask. Per batch: fresh pair rows of the touched clusters
+// (pair_all, alm_vcs.py:378-415), the row steps of the batch from a snapshot of V and the aggregates
+// (batch_step6, alm_vcs.py:255-351, with vcs_pre, vcsops.py:127-156), then the aggregate updates z, Y
+// (_acc_rows, alm_vcs.py:430-437) and the VCS post-step (vcs_yupd / vcs_post, vcsops.py:159-213) with the new rows
+// written back.
+
+__device__ inline uint16_t lb_f2bf(float f) {   // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+template <typename TV> struct Store;
+template <> struct Store<float> {
+    __device__ static inline uint4 pack(const float *x) { return make_uint4(__float_as_uint(x[0]), __float_as_uint(x[1]), __float_as_uint(x[2]), __float_as_uint(x[3])); }
+};
+template <> struct Store<bf16raw> {
+    __device__ static inline uint4 pack(const float *x) {
+        uint4 u;
+        u.x = (uint32_t)lb_f2bf(x[0]) | ((uint32_t)lb_f2bf(x[1]) << 16);
+        u.y = (uint32_t)lb_f2bf(x[2]) | ((uint32_t)lb_f2bf(x[3]) << 16);
+        u.z = (uint32_t)lb_f2bf(x[4]) | ((uint32_t)lb_f2bf(x[5]) << 16);
+        u.w = (uint32_t)lb_f2bf(x[6]) | ((uint32_t)lb_f2bf(x[7]) << 16);
+        return u;
+    }
+};
+template <> struct Store<double> {
+    __device__ static inline uint4 pack(const double *x) {
+        const unsigned long long a = __double_as_longlong(x[0]), b = __double_as_longlong(x[1]);
+        return make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
+    }
+};
+
+struct KqArgs {
+    int nlev, R; int Poff[LB_MAXLEV + 1]; int64_t soff[LB_MAXLEV + 1];
+    const double *Y; const double *sig; const double *rq; const double *dsq; double w, rho; double *Kq;
+};
+
+// Kq_Q = 2 rho (sum_R [pp_QR > 0] r_QR |Y_R|^2 + [pp_QQ > 0] r_QQ |Y_Q|^2), pp = sig - rho r H (one wave per Q)
+__global__ __launch_bounds__(64) void lb_k_kq(KqArgs a) {
+    const int Qg = blockIdx.x;
+    int l = 0;
+    while (l + 1 < a.nlev && Qg >= a.Poff[l + 1]) l++;
+    const int P = a.Poff[l + 1] - a.Poff[l], Q = Qg - a.Poff[l], R = a.R;
+    const double *YQ = a.Y + (int64_t)Qg * R;
+    const double *sig = a.sig + a.soff[l] + (int64_t)Q * P;
+    double s = 0.0, dg = 0.0;
+    for (int Rr = threadIdx.x; Rr < P; Rr += 64) {
+        const double *YR = a.Y + (int64_t)(a.Poff[l] + Rr) * R;
+        double H = 0.0, y2 = 0.0;
+        for (int c = 0; c < R; c++) { H += YQ[c] * YR[c]; y2 += YR[c] * YR[c]; }
+        if (Rr == Q) H -= a.dsq[Qg];
+        const double r = (a.rq[Qg] * a.rq[a.Poff[l] + Rr]) * a.w;
+        const double p = sig[Rr] - a.rho * r * H;
+        if (p > 0.0) s += r * y2;
+        if (Rr == Q && p > 0.0) dg = r * y2;
+    }
+    s = wave_sum(s);
+    dg = wave_sum(dg);
+    if (threadIdx.x == 0) a.Kq[Qg] = 2 * a.rho * (s + dg);
+}
+
+struct PairRowArgs {
+    int nlev, R; int Poff[LB_MAXLEV + 1]; int64_t soff[LB_MAXLEV + 1];
+    const double *Y; const double *sig; const double *rq; const double *dsq; double w, rho;
+    const uint8_t *tmask;    // touched clusters of this batch (sumP bytes)
+    double *M; double *pdiag;
+};
+
+// pair_all for the clusters the batch touches: pp row from the current Y, M_Q = sum_R pp_QR Y_R, pdiag_Q = pp_QQ
+__global__ __launch_bounds__(64) void lb_k_pair_rows(PairRowArgs a) {
+    const int Qg = blockIdx.x;
+    if (!a.tmask[Qg]) return;
+    int l = 0;
+    while (l + 1 < a.nlev && Qg >= a.Poff[l + 1]) l++;
+    const int P = a.Poff[l + 1] - a.Poff[l], Q = Qg - a.Poff[l], R = a.R;
+    const double *YQ = a.Y + (int64_t)Qg * R;
+    const double *sig = a.sig + a.soff[l] + (int64_t)Q * P;
+    double Macc[32];
+    for (int c = 0; c < R; c++) Macc[c] = 0.0;
+    double pq = 0.0;
+    for (int Rr = threadIdx.x; Rr < P; Rr += 64) {
+        const double *YR = a.Y + (int64_t)(a.Poff[l] + Rr) * R;
+        double p = sig[Rr];
+        if (a.rho > 0.0) {
+            double h = 0.0;
+            for (int c = 0; c < R; c++) h += YQ[c] * YR[c];
+            if (Rr == Q) h -= a.dsq[Qg];
+            p = p - (a.rho * ((a.rq[Qg] * a.rq[a.Poff[l] + Rr]) * a.w)) * h;
+            if (p < 0.0) p = 0.0;
+        }
+        if (Rr == Q) pq = p;
+        if (p != 0.0)
+            for (int c = 0; c < R; c++) Macc[c] += p * YR[c];
+    }
+    pq = wave_sum(pq);
+    for (int c = 0; c < R; c++) Macc[c] = wave_sum(Macc[c]);
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < R; c++) a.M[(int64_t)Qg * R + c] = Macc[c];
+        a.pdiag[Qg] = pq;
+    }
+}
+
+// batch -> touched clusters: tmask[b * sumP + Q] = 1 (b = batch of position p of the sweep's order)
+struct TmaskArgs {
+    const int32_t *order; int64_t n; int nbatch; int nlev; int Poff[LB_MAXLEV]; const int32_t *lab[LB_MAXLEV];
+    int sumP; uint8_t *tmask;
+};
+__global__ void lb_k_tmask(TmaskArgs a) {
+    const int64_t q = a.n / a.nbatch, rem = a.n % a.nbatch;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = p < (q + 1) * rem ? p / (q + 1) : rem + (p - (q + 1) * rem) / q;
+        const int32_t i = a.order[p];
+        for (int l = 0; l < a.nlev; l++) a.tmask[b * a.sumP + a.Poff[l] + a.lab[l][i]] = 1;
+    }
+}
+
+struct VcsStateArgs {
+    const uint4 *V; int64_t n; const double *d;
+    int nvcs; int Yoff[LB_MAXVCS]; int Zoff[LB_MAXVCS];
+    const int32_t *lab[LB_MAXVCS]; const double *rq[LB_MAXVCS]; const int64_t *ptr[LB_MAXVCS];
+    const int32_t *col; const void *sig; const double *Y; double wv, rho;
+    double *stZ; double *Z; double *cnt;     // Z, cnt zeroed before the launch; FP64 atomics
+};
+
+// vcs_state: stZ = effective multipliers at (V, Y); Z_R = sum stZ d_j v_j; cnt_R = #(stZ > 0)
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_state(VcsStateArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    const TC *sig = (const TC *)a.sig;
+    for (int64_t i = (int64_t)blockIdx.x * NSL + slot; i < a.n; i += (int64_t)gridDim.x * NSL) {
+        TC x[CPL];
+        VT<TV>::unpack(a.V[i * S + q], x);
+        const double di = a.d[i];
+        for (int m = 0; m < a.nvcs; m++) {
+            const int Ri = a.lab[m][i];
+            for (int64_t e = a.ptr[m][i]; e < a.ptr[m][i + 1]; e++) {
+                const int Rc = a.col[e];
+                const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < CPL; c++) s += (double)x[c] * YR[c];
+                s = slot_sum<S>(s);
+                double p = (double)sig[e];
+                if (a.rho > 0.0 && di > 0.0) {
+                    const double g = (Rc == Ri) ? di * (s - di) : di * s;
+                    p = p - a.rho * (a.wv * a.rq[m][Rc] / (di * di)) * g;
+                    if (p < 0.0) p = 0.0;
+                }
+                if (q == 0) {
+                    a.stZ[e] = p;
+                    if (p > 0.0) atomicAdd(a.cnt + a.Zoff[m] + Rc, 1.0);
+                }
+                if (p != 0.0) {
+                    double *dst = a.Z + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+#pragma unroll
+                    for (int c = 0; c < CPL; c++) atomicAdd(dst + c, p * (di * (double)x[c]));
+                }
+            }
+        }
+    }
+}
+
+struct StepArgs {
+    const uint4 *V; const int64_t *rowptr; const int32_t *col; const void *w; const double *d; double L;
+    const double *z;
+    int nlev; int Poff[LB_MAXLEV]; const int32_t *lab[LB_MAXLEV]; const double *M; const double *pdiag; const double *Kq;
+    int nvcs; int Yoff[LB_MAXVCS]; int Zoff[LB_MAXVCS]; const int32_t *vlab[LB_MAXVCS]; const int64_t *vptr[LB_MAXVCS];
+    const double *vrq[LB_MAXVCS];
+    const int32_t *vcol; const void *vsig; const double *Y; const double *Z; const double *cnt; const double *stZ;
+    double wv, kap2;
+    const int64_t *iptr; const void *rec; double rho, rt, damp;
+    int64_t heavyE, heavyT;
+    const int32_t *hE_rows; const int32_t *hE_seg0; int nhE;
+    const int32_t *hT_rows; const int32_t *hT_seg0; int nhT;
+    const void *segout; const double *segkap;
+    const int32_t *rows; int64_t nb;            // the rows of this batch
+    uint4 *Vstage; double *DD;                  // batch-position indexed
+};
+
+// one triangle record of the row step: tau (canonical order), q, pt = lam - rt (1 - tau); if pt > 0 the record adds
+// -pt/2 q to g and rt |q|^2 to the curvature (batch_step6, alm_vcs.py:297-316)
+template <typename TV, int R>
+__device__ inline void step_record(const uint4 *V, const TriRec<typename VT<TV>::TC> &rc,
+                                   const typename VT<TV>::TC *x, int q, double rt, typename VT<TV>::TC *g, double &kap) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL;
+    TC xu[CPL], xw[CPL];
+    VT<TV>::unpack(V[(int64_t)rc.u * S + q], xu);
+    VT<TV>::unpack(V[(int64_t)rc.w * S + q], xw);
+    const int role = rc.code % 3;
+    const TC *pa = role == 0 ? x : xu;
+    const TC *pb = role == 0 ? xu : (role == 1 ? x : xw);
+    const TC *pc = role == 2 ? x : xw;
+    TC tl = 0, ql = 0;
+    TC qv[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; c++) {
+        tl += pa[c] * pb[c] + pb[c] * pc[c] - pa[c] * pc[c];
+        qv[c] = role == 0 ? xu[c] - xw[c] : (role == 1 ? xu[c] + xw[c] : xw[c] - xu[c]);
+        ql += qv[c] * qv[c];
+    }
+    const TC tau = slot_sum<S>(tl);
+    const TC q2 = slot_sum<S>(ql);
+    const TC pt = rc.lam - (TC)rt * ((TC)1 - tau);
+    if (pt > (TC)0) {
+#pragma unroll
+        for (int c = 0; c < CPL; c++) g[c] -= (TC)0.5 * pt * qv[c];
+        kap += rt * (double)q2;
+    }
+}
+
+struct StepSegArgs {
+    const uint4 *V; const int32_t *col; const void *w; const void *rec; const Seg *segs;
+    const int32_t *list; int64_t nlist;          // segment ids of this batch's hub rows
+    double rt; void *segout; double *segkap;
+};
+
+// wavefront per hub segment: partial sum of A V (edge segments) or of the triangle records' step terms
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_step_segs(StepSegArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSW = 64 / S;
+    const int64_t li = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
+    if (li >= a.nlist) return;
+    const int lane = threadIdx.x & 63, ws = lane / S, q = lane % S;
+    const int32_t sid = a.list[li];
+    const Seg s = a.segs[sid];
+    TC acc[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; c++) acc[c] = 0;
+    double kap = 0.0;
+    if (s.kind == 0) {
+        const TC *wt = (const TC *)a.w;
+        for (int64_t e = s.b + ws; e < s.e; e += NSW) {
+            TC xj[CPL];
+            VT<TV>::unpack(a.V[(int64_t)a.col[e] * S + q], xj);
+            const TC w = wt ? wt[e] : (TC)1;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) acc[c] += w * xj[c];
+        }
+    } else {
+        TC x[CPL];
+        VT<TV>::unpack(a.V[(int64_t)s.row * S + q], x);
+        const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
+        for (int64_t k = s.b + ws; k < s.e; k += NSW) step_record<TV, R>(a.V, rec[k], x, q, a.rt, acc, kap);
+    }
+#pragma unroll
+    for (int c = 0; c < CPL; c++) {
+        TC t = acc[c];
+        for (int m = S; m < 64; m <<= 1) t += __shfl_xor(t, m);
+        acc[c] = t;
+    }
+    for (int m = S; m < 64; m <<= 1) kap += __shfl_xor(kap, m);
+    if (ws == 0) {
+        TC *out = (TC *)a.segout + (int64_t)sid * R + q * CPL;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) out[c] = acc[c];
+        if (q == 0) a.segkap[sid] = kap;
+    }
+}
+
+// rank-one, pair levels, VCS pre-step terms, then the normalised minorant step; writes Vstage[pos], DD[pos]
+template <typename TV, int R>
+__device__ inline void step_finish(const StepArgs &a, const typename VT<TV>::TC *x, int64_t i, int64_t pos, int q,
+                                   typename VT<TV>::TC *g, double kap) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL;
+    const double di = a.d[i];
+#pragma unroll
+    for (int c = 0; c < CPL; c++) g[c] += (TC)(-(di / a.L) * (a.z[q * CPL + c] - di * (double)x[c]));
+    for (int l = 0; l < a.nlev; l++) {
+        const int Q = a.Poff[l] + a.lab[l][i];
+        const double *M = a.M + (int64_t)Q * R + q * CPL;
+        const double pd = a.pdiag[Q] * di;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) g[c] += (TC)(di * (M[c] - pd * (double)x[c]));
+        kap += di * di * a.Kq[Q];
+    }
+    const TC *vsig = (const TC *)a.vsig;
+    for (int m = 0; m < a.nvcs; m++) {
+        const int Ri = a.vlab[m][i];
+        double acc[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) acc[c] = 0.0;
+        double kp = 0.0, pown = 0.0, sz = 0.0;
+        for (int64_t e = a.vptr[m][i]; e < a.vptr[m][i + 1]; e++) {
+            const int Rc = a.vcol[e];
+            const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+            double s = 0.0, y2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) { s += (double)x[c] * YR[c]; y2 += YR[c] * YR[c]; }
+            s = slot_sum<S>(s);
+            double p;
+            if (a.rho > 0.0 && di > 0.0) {
+                const double g2 = (Rc == Ri) ? di * (s - di) : di * s;
+                const double rr = a.wv * a.vrq[m][Rc] / (di * di);
+                p = (double)vsig[e] - a.rho * rr * g2;
+                if (p > 0.0) {
+                    y2 = slot_sum<S>(y2);
+                    if (Rc == Ri) y2 = y2 - 2 * di * s + di * di;
+                    kp += a.rho * rr * di * di * y2;
+                } else p = 0.0;
+            } else p = (double)vsig[e];
+            if (Rc == Ri) { pown = p; sz = a.stZ[e]; }
+            if (p != 0.0) {
+#pragma unroll
+                for (int c = 0; c < CPL; c++) acc[c] += p * YR[c];
+            }
+        }
+        const double *ZR = a.Z + (int64_t)(a.Zoff[m] + Ri) * R + q * CPL;
+#pragma unroll
+        for (int c = 0; c < CPL; c++)
+            g[c] += (TC)(0.5 * di * (acc[c] - pown * di * (double)x[c]) + 0.5 * di * (ZR[c] - sz * di * (double)x[c]));
+        if (a.rho > 0.0) kp += a.kap2 * a.rho * di * di * a.wv * a.vrq[m][Ri] * (a.cnt[a.Zoff[m] + Ri] - (sz > 0.0 ? 1.0 : 0.0));
+        kap += kp;
+    }
+    TC gl = 0;
+#pragma unroll
+    for (int c = 0; c < CPL; c++) gl += g[c] * g[c];
+    const TC gn = slot_sum<S>(gl);
+    const TC w = (TC)(0.5 * kap) + (TC)a.damp * sqrt(gn);
+    TC qv[CPL], nl = 0;
+#pragma unroll
+    for (int c = 0; c < CPL; c++) { qv[c] = g[c] + w * x[c]; nl += qv[c] * qv[c]; }
+    const TC nv = slot_sum<S>(nl);
+    TC xn[CPL];
+    if (nv > (TC)0) {
+        const TC inv = (TC)1 / sqrt(nv);
+#pragma unroll
+        for (int c = 0; c < CPL; c++) xn[c] = qv[c] * inv;
+    } else {
+#pragma unroll
+        for (int c = 0; c < CPL; c++) xn[c] = x[c];
+    }
+    // round to the storage type first: DD tracks the stored rows exactly
+    const uint4 u = Store<TV>::pack(xn);
+    a.Vstage[pos * S + q] = u;
+    TC xs[CPL];
+    VT<TV>::unpack(u, xs);
+    double *dd = a.DD + pos * R + q * CPL;
+#pragma unroll
+    for (int c = 0; c < CPL; c++) dd[c] = di * ((double)xs[c] - (double)x[c]);
+}
+
+// one slot per batch row: SpMM row (or its hub segments), triangle records (or their segments), rank-one term,
+// pair levels, VCS pre-step terms, then the normalised minorant step; writes Vstage[pos] and DD[pos]
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_step(StepArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    const int64_t pos = (int64_t)blockIdx.x * NSL + slot;
+    if (pos >= a.nb) return;
+    const int64_t i = a.rows[pos];
+    TC x[CPL], g[CPL];
+    VT<TV>::unpack(a.V[i * S + q], x);
+#pragma unroll
+    for (int c = 0; c < CPL; c++) g[c] = 0;
+    double kap = 0.0;
+    const TC *so = (const TC *)a.segout;
+    const int64_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+    if (e1 - e0 > a.heavyE) {
+        const int k = find_row(a.hE_rows, a.nhE, i);
+        for (int s = a.hE_seg0[k]; s < a.hE_seg0[k + 1]; s++) {
+#pragma unroll
+            for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
+        }
+    } else {
+        const TC *wt = (const TC *)a.w;
+        for (int64_t e = e0; e < e1; e++) {
+            TC xj[CPL];
+            VT<TV>::unpack(a.V[(int64_t)a.col[e] * S + q], xj);
+            const TC w = wt ? wt[e] : (TC)1;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) g[c] += w * xj[c];
+        }
+    }
+    const int64_t r0 = a.iptr[i], r1 = a.iptr[i + 1];
+    if (r1 - r0 > a.heavyT) {
+        const int k = find_row(a.hT_rows, a.nhT, i);
+        for (int s = a.hT_seg0[k]; s < a.hT_seg0[k + 1]; s++) {
+#pragma unroll
+            for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
+            kap += a.segkap[s];
+        }
+    } else {
+        const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
+        for (int64_t k = r0; k < r1; k++) step_record<TV, R>(a.V, rec[k], x, q, a.rt, g, kap);
+    }
+    step_finish<TV, R>(a, x, i, pos, q, g, kap);
+}
+
+struct PostArgs {
+    const int32_t *rows; int64_t nb; const double *DD; int R;
+    LabelSets ls; int sumP; int use_lds;
+    double *zY;                 // [z | Y(all label sets)] updated in place (FP64 atomics)
+};
+
+// z += sum DD, Y[Q] += DD for every label set (per-block LDS partials, then one atomic per block and entry)
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_post_acc(PostArgs a) {
+    extern __shared__ double sh[];
+    const int width = a.R + a.sumP * a.R;
+    if (a.use_lds) {
+        for (int e = threadIdx.x; e < width; e += LB_BLOCK) sh[e] = 0.0;
+        __syncthreads();
+    }
+    const int c = threadIdx.x % a.R, rs = threadIdx.x / a.R, nrs = LB_BLOCK / a.R;
+    const int64_t per = (a.nb + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = b0 + per < a.nb ? b0 + per : a.nb;
+    double zc = 0.0;
+    for (int64_t p = b0 + rs; p < b1; p += nrs) {
+        const double x = a.DD[p * a.R + c];
+        zc += x;
+        const int32_t i = a.rows[p];
+        for (int s = 0; s < a.ls.nsets; s++) {
+            const int Q = a.ls.off[s] + a.ls.lab[s][i];
+            if (a.use_lds) atomicAdd(sh + a.R + (int64_t)Q * a.R + c, x);
+            else atomicAdd(a.zY + a.R + (int64_t)Q * a.R + c, x);
+        }
+    }
+    if (a.use_lds) atomicAdd(sh + c, zc); else atomicAdd(a.zY + c, zc);
+    if (a.use_lds) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < width; e += LB_BLOCK)
+            if (sh[e] != 0.0) atomicAdd(a.zY + e, sh[e]);
+    }
+}
+
+struct Post2Args {
+    uint4 *V; const uint4 *Vstage; const int32_t *rows; int64_t nb; const double *d;
+    int nvcs; int Yoff[LB_MAXVCS]; int Zoff[LB_MAXVCS]; const int32_t *lab[LB_MAXVCS]; const double *rq[LB_MAXVCS];
+    const int64_t *ptr[LB_MAXVCS]; const int32_t *col; const void *sig; const double *Y; double wv, rho;
+    double *stZ; double *Z; double *cnt;
+};
+
+// vcs_post: new effective multipliers SN of the moved rows (new v_i, updated Y), Z_R += d_i (SN v_new - stZ v_old),
+// cnt_R += [SN > 0] - [stZ > 0], stZ = SN; then the new rows are written to V.
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_post_vcs(Post2Args a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    const int64_t pos = (int64_t)blockIdx.x * NSL + slot;
+    if (pos >= a.nb) return;
+    const int64_t i = a.rows[pos];
+    const uint4 un = a.Vstage[pos * S + q];
+    TC xn[CPL], xo[CPL];
+    VT<TV>::unpack(un, xn);
+    VT<TV>::unpack(a.V[i * S + q], xo);
+    const double di = a.d[i];
+    const TC *sig = (const TC *)a.sig;
+    for (int m = 0; m < a.nvcs; m++) {
+        const int Ri = a.lab[m][i];
+        for (int64_t e = a.ptr[m][i]; e < a.ptr[m][i + 1]; e++) {
+            const int Rc = a.col[e];
+            const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) s += (double)xn[c] * YR[c];
+            s = slot_sum<S>(s);
+            double sn = (double)sig[e];
+            if (a.rho > 0.0 && di > 0.0) {
+                const double g = (Rc == Ri) ? di * (s - di) : di * s;
+                sn = sn - a.rho * (a.wv * a.rq[m][Rc] / (di * di)) * g;
+                if (sn < 0.0) sn = 0.0;
+            }
+            const double so = a.stZ[e];
+            if (sn != 0.0 || so != 0.0) {
+                double *dst = a.Z + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+#pragma unroll
+                for (int c = 0; c < CPL; c++) atomicAdd(dst + c, di * (sn * (double)xn[c] - so * (double)xo[c]));
+            }
+            if (q == 0) {
+                const double dc = (sn > 0.0 ? 1.0 : 0.0) - (so > 0.0 ? 1.0 : 0.0);
+                if (dc != 0.0) atomicAdd(a.cnt + a.Zoff[m] + Rc, dc);
+            }
+            if (q == 0) a.stZ[e] = sn;
+        }
+    }
+    a.V[i * S + q] = un;
+}
+
 // ================================================================================================== host side
 namespace {
 
@@ -678,10 +1167,13 @@ struct lb_ctx {
     std::vector<int32_t> hE_rows, hE_seg0, hT_rows, hT_seg0;
     std::vector<Seg> segE, segT;
     bool have_eval = false; double last_rho = 0.0;
+    std::vector<int32_t> hE_idx, hT_idx;   // row -> index into hE_rows / hT_rows, or -1
+    std::vector<int64_t> h_iptr;            // triangle incidence pointer (host copy)
     // device buffers
     DBuf rowptr, col, w, d, labs, sig, pp, rq, dsq, vlabs, vrq, vptr, vcol, vsig, pv, TI, lam, pt, iptr, rec,
         V, G, Y, z, M, pdiag, levtot, levviol, Z, colsum_part, vcs_part, row_part, seg_part, segs, segout,
-        hErows, hEseg0, hTrows, hTseg0, parts, lamsum;
+        hErows, hEseg0, hTrows, hTseg0, parts, lamsum,
+        order, tmask, Kq, stZ, Zs, cnts, Vstage, DDb, segkap, seglist;
 };
 
 namespace {
@@ -715,7 +1207,7 @@ void dfree(DBuf &b) {
 int upload(lb_ctx *c, DBuf &b, const void *src, size_t bytes) {
     int rc = dalloc(c, b, bytes);
     if (rc) return rc;
-    if (bytes) HIPCHK(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    if (bytes && src) HIPCHK(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -773,7 +1265,7 @@ int rebuild_segs(lb_ctx *c) {
     std::vector<Seg> all(c->segE);
     all.insert(all.end(), c->segT.begin(), c->segT.end());
     int rc;
-    if ((rc = upload(c, c->segs, all.data(), sizeof(Seg) * all.size()))) return rc;
+    if ((rc = upload(c, c->segs, all.empty() ? nullptr : all.data(), sizeof(Seg) * all.size()))) return rc;
     if ((rc = dalloc(c, c->segout, (size_t)std::max<size_t>(all.size(), 1) * c->r * c->cbytes))) return rc;
     const int nblk = (int)((all.size() + 3) / 4);
     if ((rc = dalloc(c, c->seg_part, sizeof(double) * 2 * std::max(nblk, 1)))) return rc;
@@ -785,13 +1277,10 @@ struct Launch {
     typedef typename VT<TV>::TC TC;
     static constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
 
-    static int eval(lb_ctx *c, double rho, double rt, hipEvent_t *ev) {
+    // z = d^T V and the cluster sums of every label set into c->Y = [z | Y]
+    static void zY(lb_ctx *c, hipEvent_t *ev = nullptr, int *ek = nullptr) {
         const int64_t n = c->n;
         const int sumPall = c->sumPpair + c->sumPv;
-        int ek = 0;
-        auto mark = [&]() { if (ev) hipEventRecord(ev[ek++], c->stream); };
-        mark();
-        // --- z, Y
         ColsumArgs ca{};
         ca.V = (const uint4 *)c->V.p; ca.n = n; ca.d = (const double *)c->d.p;
         ca.ls.nsets = c->nlev + c->nvcs;
@@ -803,15 +1292,22 @@ struct Launch {
         const size_t ldsY = sizeof(double) * (size_t)sumPall * R;
         ca.use_lds = ldsY <= 48 * 1024;
         ca.part = (double *)c->colsum_part.p;
-        ca.Yglob = (double *)c->Y.p + R;  // Y buffer holds [z | Y]
+        ca.Yglob = (double *)c->Y.p + R;
         if (!ca.use_lds && sumPall) hipLaunchKernelGGL(lb_k_fill, dim3(256), dim3(256), 0, c->stream, (double *)c->Y.p + R, (int64_t)sumPall * R, 0.0);
         hipLaunchKernelGGL((lb_k_colsum<TV, R>), dim3(LB_RED_BLOCKS), dim3(LB_BLOCK), ca.use_lds ? ldsY : 0, c->stream, ca);
+        if (ev) hipEventRecord(ev[(*ek)++], c->stream);
+        const int width = R + (ca.use_lds ? sumPall * R : 0);
+        hipLaunchKernelGGL(lb_k_reduce, dim3((width + 255) / 256), dim3(256), 0, c->stream, (const double *)c->colsum_part.p,
+                           LB_RED_BLOCKS, width, (double *)c->Y.p);
+    }
+
+    static int eval(lb_ctx *c, double rho, double rt, hipEvent_t *ev) {
+        const int64_t n = c->n;
+        const int sumPall = c->sumPpair + c->sumPv;
+        int ek = 0;
+        auto mark = [&]() { if (ev) hipEventRecord(ev[ek++], c->stream); };
         mark();
-        {
-            const int width = R + (ca.use_lds ? sumPall * R : 0);
-            hipLaunchKernelGGL(lb_k_reduce, dim3((width + 255) / 256), dim3(256), 0, c->stream, (const double *)c->colsum_part.p,
-                               LB_RED_BLOCKS, width, (double *)c->Y.p);
-        }
+        zY(c, ev, &ek);
         mark();
         const double *zp = (const double *)c->Y.p;
         const double *Yp = zp + R;
@@ -905,15 +1401,156 @@ struct Launch {
         HIPCHK(c, hipGetLastError());
         return 0;
     }
+
+    // one sweep of batched row steps over the uploaded order (c->order) with per-batch segment lists
+    static int sweep(lb_ctx *c, double rho, double rt, double damp, double kap2, int nbatch,
+                     const std::vector<int64_t> &bpos, const std::vector<int64_t> &lpos) {
+        const int64_t n = c->n;
+        const int sumPall = c->sumPpair + c->sumPv;
+        zY(c);
+        const double *zp = (const double *)c->Y.p;
+        const double *Yp = zp + R;
+        int Poff[LB_MAXLEV + 1];
+        {
+            int o = 0;
+            for (int l = 0; l <= c->nlev; l++) { Poff[l] = o; if (l < c->nlev) o += c->P[l]; }
+        }
+        if (c->nlev) {
+            if (rho > 0.0) {
+                KqArgs ka{};
+                ka.nlev = c->nlev; ka.R = R;
+                for (int l = 0; l <= c->nlev; l++) { ka.Poff[l] = Poff[l]; ka.soff[l] = c->soff[l]; }
+                ka.Y = Yp; ka.sig = (const double *)c->sig.p; ka.rq = (const double *)c->rq.p; ka.dsq = (const double *)c->dsq.p;
+                ka.w = c->pw; ka.rho = rho; ka.Kq = (double *)c->Kq.p;
+                hipLaunchKernelGGL(lb_k_kq, dim3(c->sumPpair), dim3(64), 0, c->stream, ka);
+            } else {
+                hipLaunchKernelGGL(lb_k_fill, dim3(64), dim3(256), 0, c->stream, (double *)c->Kq.p, (int64_t)c->sumPpair, 0.0);
+            }
+            TmaskArgs ta{};
+            ta.order = (const int32_t *)c->order.p; ta.n = n; ta.nbatch = nbatch; ta.nlev = c->nlev;
+            for (int l = 0; l < c->nlev; l++) { ta.Poff[l] = Poff[l]; ta.lab[l] = (const int32_t *)c->labs.p + (int64_t)l * n; }
+            ta.sumP = c->sumPpair; ta.tmask = (uint8_t *)c->tmask.p;
+            HIPCHK(c, hipMemsetAsync(c->tmask.p, 0, (size_t)nbatch * c->sumPpair, c->stream));
+            hipLaunchKernelGGL(lb_k_tmask, dim3(1024), dim3(256), 0, c->stream, ta);
+        }
+        VcsStateArgs vs{};
+        int Yoff[LB_MAXVCS], Zoff[LB_MAXVCS];
+        {
+            int zo = 0;
+            for (int m = 0; m < c->nvcs; m++) { Yoff[m] = c->sumPpair + zo; Zoff[m] = zo; zo += c->vP[m]; }
+        }
+        if (c->nvcs) {
+            vs.V = (const uint4 *)c->V.p; vs.n = n; vs.d = (const double *)c->d.p; vs.nvcs = c->nvcs;
+            for (int m = 0; m < c->nvcs; m++) {
+                vs.Yoff[m] = Yoff[m]; vs.Zoff[m] = Zoff[m];
+                vs.lab[m] = (const int32_t *)c->vlabs.p + (int64_t)m * n;
+                vs.rq[m] = (const double *)c->vrq.p + Zoff[m];
+                vs.ptr[m] = (const int64_t *)c->vptr.p + (int64_t)m * (n + 1);
+            }
+            vs.col = (const int32_t *)c->vcol.p; vs.sig = c->vsig.p; vs.Y = Yp; vs.wv = c->wv; vs.rho = rho;
+            vs.stZ = (double *)c->stZ.p; vs.Z = (double *)c->Zs.p; vs.cnt = (double *)c->cnts.p;
+            hipLaunchKernelGGL(lb_k_fill, dim3(64), dim3(256), 0, c->stream, (double *)c->Zs.p, (int64_t)c->sumPv * R, 0.0);
+            hipLaunchKernelGGL(lb_k_fill, dim3(64), dim3(256), 0, c->stream, (double *)c->cnts.p, (int64_t)c->sumPv, 0.0);
+            hipLaunchKernelGGL((lb_k_vcs_state<TV, R>), dim3(LB_RED_BLOCKS), dim3(LB_BLOCK), 0, c->stream, vs);
+        }
+        StepArgs sa{};
+        sa.V = (const uint4 *)c->V.p; sa.rowptr = (const int64_t *)c->rowptr.p; sa.col = (const int32_t *)c->col.p;
+        sa.w = c->weighted ? c->w.p : nullptr; sa.d = (const double *)c->d.p; sa.L = c->L; sa.z = zp;
+        sa.nlev = c->nlev;
+        for (int l = 0; l < c->nlev; l++) { sa.Poff[l] = Poff[l]; sa.lab[l] = (const int32_t *)c->labs.p + (int64_t)l * n; }
+        sa.M = (const double *)c->M.p; sa.pdiag = (const double *)c->pdiag.p; sa.Kq = (const double *)c->Kq.p;
+        sa.nvcs = c->nvcs;
+        for (int m = 0; m < c->nvcs; m++) {
+            sa.Yoff[m] = Yoff[m]; sa.Zoff[m] = Zoff[m];
+            sa.vlab[m] = (const int32_t *)c->vlabs.p + (int64_t)m * n;
+            sa.vptr[m] = (const int64_t *)c->vptr.p + (int64_t)m * (n + 1);
+            sa.vrq[m] = (const double *)c->vrq.p + Zoff[m];
+        }
+        sa.vcol = (const int32_t *)c->vcol.p; sa.vsig = c->vsig.p; sa.Y = Yp; sa.Z = (const double *)c->Zs.p;
+        sa.cnt = (const double *)c->cnts.p; sa.stZ = (const double *)c->stZ.p; sa.wv = c->wv; sa.kap2 = kap2;
+        sa.iptr = (const int64_t *)c->iptr.p; sa.rec = c->rec.p; sa.rho = rho; sa.rt = rt; sa.damp = damp;
+        sa.heavyE = c->built_heavyE; sa.heavyT = c->built_heavyT;
+        sa.hE_rows = (const int32_t *)c->hErows.p; sa.hE_seg0 = (const int32_t *)c->hEseg0.p; sa.nhE = (int)c->hE_rows.size();
+        sa.hT_rows = (const int32_t *)c->hTrows.p; sa.hT_seg0 = (const int32_t *)c->hTseg0.p; sa.nhT = (int)c->hT_rows.size();
+        sa.segout = c->segout.p; sa.segkap = (const double *)c->segkap.p;
+        sa.Vstage = (uint4 *)c->Vstage.p; sa.DD = (double *)c->DDb.p;
+        StepSegArgs ga{};
+        ga.V = (const uint4 *)c->V.p; ga.col = (const int32_t *)c->col.p; ga.w = c->weighted ? c->w.p : nullptr;
+        ga.rec = c->rec.p; ga.segs = (const Seg *)c->segs.p; ga.rt = rt; ga.segout = c->segout.p; ga.segkap = (double *)c->segkap.p;
+        PairRowArgs pa{};
+        pa.nlev = c->nlev; pa.R = R;
+        for (int l = 0; l <= c->nlev; l++) { pa.Poff[l] = Poff[l]; pa.soff[l] = c->soff[l]; }
+        pa.Y = Yp; pa.sig = (const double *)c->sig.p; pa.rq = (const double *)c->rq.p; pa.dsq = (const double *)c->dsq.p;
+        pa.w = c->pw; pa.rho = rho; pa.M = (double *)c->M.p; pa.pdiag = (double *)c->pdiag.p;
+        PostArgs qa{};
+        qa.R = R; qa.ls.nsets = c->nlev + c->nvcs;
+        {
+            int off = 0;
+            for (int l = 0; l < c->nlev; l++) { qa.ls.off[l] = off; qa.ls.lab[l] = (const int32_t *)c->labs.p + (int64_t)l * n; off += c->P[l]; }
+            for (int m = 0; m < c->nvcs; m++) { qa.ls.off[c->nlev + m] = off; qa.ls.lab[c->nlev + m] = (const int32_t *)c->vlabs.p + (int64_t)m * n; off += c->vP[m]; }
+            qa.ls.off[qa.ls.nsets] = off;
+        }
+        qa.sumP = sumPall; qa.DD = (const double *)c->DDb.p; qa.zY = (double *)c->Y.p;
+        const size_t ldsP = sizeof(double) * (size_t)(R + sumPall * R);
+        qa.use_lds = ldsP <= 48 * 1024;
+        Post2Args pb{};
+        pb.V = (uint4 *)c->V.p; pb.Vstage = (const uint4 *)c->Vstage.p; pb.d = (const double *)c->d.p; pb.nvcs = c->nvcs;
+        for (int m = 0; m < c->nvcs; m++) {
+            pb.Yoff[m] = Yoff[m]; pb.Zoff[m] = Zoff[m];
+            pb.lab[m] = (const int32_t *)c->vlabs.p + (int64_t)m * n;
+            pb.rq[m] = (const double *)c->vrq.p + Zoff[m];
+            pb.ptr[m] = (const int64_t *)c->vptr.p + (int64_t)m * (n + 1);
+        }
+        pb.col = (const int32_t *)c->vcol.p; pb.sig = c->vsig.p; pb.Y = Yp; pb.wv = c->wv; pb.rho = rho;
+        pb.stZ = (double *)c->stZ.p; pb.Z = (double *)c->Zs.p; pb.cnt = (double *)c->cnts.p;
+        const int32_t *order = (const int32_t *)c->order.p;
+        const int32_t *slist = (const int32_t *)c->seglist.p;
+        for (int b = 0; b < nbatch; b++) {
+            const int64_t p0 = bpos[b], nb = bpos[b + 1] - bpos[b];
+            if (nb == 0) continue;
+            if (c->nlev) {
+                pa.tmask = (const uint8_t *)c->tmask.p + (int64_t)b * c->sumPpair;
+                hipLaunchKernelGGL(lb_k_pair_rows, dim3(c->sumPpair), dim3(64), 0, c->stream, pa);
+            }
+            const int64_t ns = lpos[b + 1] - lpos[b];
+            if (ns) {
+                ga.list = slist + lpos[b]; ga.nlist = ns;
+                hipLaunchKernelGGL((lb_k_step_segs<TV, R>), dim3((unsigned)((ns + 3) / 4)), dim3(LB_BLOCK), 0, c->stream, ga);
+            }
+            sa.rows = order + p0; sa.nb = nb;
+            hipLaunchKernelGGL((lb_k_step<TV, R>), dim3((unsigned)((nb + NSL - 1) / NSL)), dim3(LB_BLOCK), 0, c->stream, sa);
+            qa.rows = order + p0; qa.nb = nb;
+            const unsigned pg = (unsigned)std::min<int64_t>(64, (nb + 127) / 128);
+            hipLaunchKernelGGL(lb_k_post_acc, dim3(pg), dim3(LB_BLOCK), qa.use_lds ? ldsP : 0, c->stream, qa);
+            pb.rows = order + p0; pb.nb = nb;
+            hipLaunchKernelGGL((lb_k_post_vcs<TV, R>), dim3((unsigned)((nb + NSL - 1) / NSL)), dim3(LB_BLOCK), 0, c->stream, pb);
+        }
+        HIPCHK(c, hipGetLastError());
+        return 0;
+    }
 };
 
 typedef int (*eval_fn)(lb_ctx *, double, double, hipEvent_t *);
+typedef int (*sweep_fn)(lb_ctx *, double, double, double, double, int, const std::vector<int64_t> &,
+                        const std::vector<int64_t> &);
 
 eval_fn pick_eval(int vdtype, int r) {
 #define LB_PICK(TV)                                     \
     if (r == 8) return &Launch<TV, 8>::eval;            \
     if (r == 16) return &Launch<TV, 16>::eval;          \
     if (r == 32) return &Launch<TV, 32>::eval;
+    if (vdtype == LB_F32) { LB_PICK(float) }
+    if (vdtype == LB_BF16) { LB_PICK(bf16raw) }
+    if (vdtype == LB_F64) { LB_PICK(double) }
+#undef LB_PICK
+    return nullptr;
+}
+
+sweep_fn pick_sweep(int vdtype, int r) {
+#define LB_PICK(TV)                                     \
+    if (r == 8) return &Launch<TV, 8>::sweep;           \
+    if (r == 16) return &Launch<TV, 16>::sweep;         \
+    if (r == 32) return &Launch<TV, 32>::sweep;
     if (vdtype == LB_F32) { LB_PICK(float) }
     if (vdtype == LB_BF16) { LB_PICK(bf16raw) }
     if (vdtype == LB_F64) { LB_PICK(double) }
@@ -968,8 +1605,8 @@ int empty_triangles(lb_ctx *c) {
     if ((rc = dalloc(c, c->pt, 16))) return rc;
     if ((rc = dalloc(c, c->lam, 16))) return rc;
     c->hT_rows.clear(); c->hT_seg0.assign(1, (int32_t)c->segE.size()); c->segT.clear();
-    if ((rc = upload(c, c->hTrows, c->hT_rows.data(), 4))) return rc;
-    if ((rc = upload(c, c->hTseg0, c->hT_seg0.data(), 4))) return rc;
+    if ((rc = upload(c, c->hTrows, c->hT_rows.data(), 0))) return rc;
+    if ((rc = upload(c, c->hTseg0, c->hT_seg0.data(), sizeof(int32_t)))) return rc;
     return 0;
 }
 
@@ -1030,7 +1667,8 @@ void lb_destroy(lb_ctx *c) {
                     &c->vptr, &c->vcol, &c->vsig, &c->pv, &c->TI, &c->lam, &c->pt, &c->iptr, &c->rec, &c->V, &c->G,
                     &c->Y, &c->z, &c->M, &c->pdiag, &c->levtot, &c->levviol, &c->Z, &c->colsum_part, &c->vcs_part,
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
-                    &c->hTseg0, &c->parts, &c->lamsum};
+                    &c->hTseg0, &c->parts, &c->lamsum, &c->order, &c->tmask, &c->Kq, &c->stZ, &c->Zs, &c->cnts,
+                    &c->Vstage, &c->DDb, &c->segkap, &c->seglist};
     for (DBuf *b : bufs) dfree(*b);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -1065,7 +1703,11 @@ int lb_set_graph(lb_ctx *c, int64_t n, const int64_t *rowptr, const int32_t *col
     if ((rc = upload(c, c->d, d, sizeof(double) * n))) return rc;
     build_segments(n, rowptr, c->heavy_edges, c->seg_len, 0, c->hE_rows, c->hE_seg0, c->segE, 0);
     c->built_heavyE = c->heavy_edges;
-    if ((rc = upload(c, c->hErows, c->hE_rows.data(), sizeof(int32_t) * std::max<size_t>(c->hE_rows.size(), 1)))) return rc;
+    c->hE_idx.assign(n, -1);
+    for (size_t k = 0; k < c->hE_rows.size(); k++) c->hE_idx[c->hE_rows[k]] = (int32_t)k;
+    c->hT_idx.assign(n, -1);
+    c->h_iptr.assign(n + 1, 0);
+    if ((rc = upload(c, c->hErows, c->hE_rows.data(), sizeof(int32_t) * c->hE_rows.size()))) return rc;
     if ((rc = upload(c, c->hEseg0, c->hE_seg0.data(), sizeof(int32_t) * c->hE_seg0.size()))) return rc;
     if ((rc = dalloc(c, c->V, (size_t)n * c->r * c->vbytes))) return rc;
     if ((rc = dalloc(c, c->G, (size_t)n * c->r * c->cbytes))) return rc;
@@ -1165,6 +1807,8 @@ int lb_set_triangles(lb_ctx *c, int64_t T, const int32_t *TI, const double *lam)
     int rc;
     if (T == 0) {
         c->T = 0;
+        c->hT_idx.assign(n, -1);
+        c->h_iptr.assign(n + 1, 0);
         if ((rc = empty_triangles(c)) || (rc = rebuild_segs(c)) || (rc = refresh_lamsum(c))) return rc;
         c->have_eval = false;
         return 0;
@@ -1203,7 +1847,10 @@ int lb_set_triangles(lb_ctx *c, int64_t T, const int32_t *TI, const double *lam)
     if ((rc = dalloc(c, c->pt, c->cbytes * T))) return rc;
     build_segments(n, cnt.data(), c->heavy_tris, c->seg_len, 1, c->hT_rows, c->hT_seg0, c->segT, (int32_t)c->segE.size());
     c->built_heavyT = c->heavy_tris;
-    if ((rc = upload(c, c->hTrows, c->hT_rows.data(), sizeof(int32_t) * std::max<size_t>(c->hT_rows.size(), 1)))) return rc;
+    c->hT_idx.assign(n, -1);
+    for (size_t k = 0; k < c->hT_rows.size(); k++) c->hT_idx[c->hT_rows[k]] = (int32_t)k;
+    c->h_iptr = cnt;
+    if ((rc = upload(c, c->hTrows, c->hT_rows.data(), sizeof(int32_t) * c->hT_rows.size()))) return rc;
     if ((rc = upload(c, c->hTseg0, c->hT_seg0.data(), sizeof(int32_t) * c->hT_seg0.size()))) return rc;
     if ((rc = rebuild_segs(c))) return rc;
     if ((rc = refresh_lamsum(c))) return rc;
@@ -1348,6 +1995,68 @@ int lb_phr_update(lb_ctx *c) {
         if (rc) return rc;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_eval = false;
+    return 0;
+}
+
+int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap2, const int64_t *orders, int nbatch) {
+    if (!c || !c->n) return fail(c, -1, "lb_sweeps: no graph");
+    if (nsw < 0 || nbatch < 1 || nbatch > c->n || !orders) return fail(c, -1, "lb_sweeps: bad arguments");
+    if (!(rho >= 0.0) || !(rt >= 0.0) || !(damp >= 0.0)) return fail(c, -1, "lb_sweeps: rho, rt, damp must be >= 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    sweep_fn f = pick_sweep(c->vdtype, c->r);
+    if (!f) return fail(c, -1, "lb_sweeps: unsupported rank / dtype");
+    const int64_t n = c->n;
+    const int R = c->r;
+    const int S = (int)(R * c->vbytes / 16);
+    const int64_t maxb = n / nbatch + 1;
+    int64_t nnzv = 0;
+    for (int64_t x : c->vnnz) nnzv += x;
+    const size_t nseg = c->segE.size() + c->segT.size();
+    int rc;
+    if ((rc = dalloc(c, c->order, sizeof(int32_t) * n))) return rc;
+    if ((rc = dalloc(c, c->tmask, (size_t)nbatch * std::max(c->sumPpair, 1)))) return rc;
+    if ((rc = dalloc(c, c->Kq, sizeof(double) * std::max(c->sumPpair, 1)))) return rc;
+    if ((rc = dalloc(c, c->stZ, sizeof(double) * std::max<int64_t>(nnzv, 1)))) return rc;
+    if ((rc = dalloc(c, c->Zs, sizeof(double) * (size_t)std::max(c->sumPv, 1) * R))) return rc;
+    if ((rc = dalloc(c, c->cnts, sizeof(double) * std::max(c->sumPv, 1)))) return rc;
+    if ((rc = dalloc(c, c->Vstage, (size_t)maxb * S * 16))) return rc;
+    if ((rc = dalloc(c, c->DDb, sizeof(double) * (size_t)maxb * R))) return rc;
+    if ((rc = dalloc(c, c->segkap, sizeof(double) * std::max<size_t>(nseg, 1)))) return rc;
+    std::vector<int32_t> o32(n);
+    std::vector<uint8_t> seen(n);
+    std::vector<int64_t> bpos(nbatch + 1), lpos(nbatch + 1);
+    std::vector<int32_t> list;
+    // batches = np.array_split(order, nbatch): the first n % nbatch batches hold one extra row
+    const int64_t q = n / nbatch, rem = n % nbatch;
+    for (int b = 0; b <= nbatch; b++) bpos[b] = (int64_t)b * q + std::min<int64_t>(b, rem);
+    for (int sw = 0; sw < nsw; sw++) {
+        const int64_t *ord = orders + (int64_t)sw * n;
+        std::fill(seen.begin(), seen.end(), 0);
+        for (int64_t p = 0; p < n; p++) {
+            const int64_t i = ord[p];
+            if (i < 0 || i >= n || seen[i]) return fail(c, -1, "lb_sweeps: order is not a permutation of 0..n-1");
+            seen[i] = 1;
+            o32[p] = (int32_t)i;
+        }
+        list.clear();
+        for (int b = 0; b < nbatch; b++) {
+            lpos[b] = (int64_t)list.size();
+            for (int64_t p = bpos[b]; p < bpos[b + 1]; p++) {
+                const int32_t i = o32[p];
+                const int32_t ke = c->hE_idx[i], kt = c->hT_idx[i];
+                if (ke >= 0) for (int32_t s = c->hE_seg0[ke]; s < c->hE_seg0[ke + 1]; s++) list.push_back(s);
+                if (kt >= 0) for (int32_t s = c->hT_seg0[kt]; s < c->hT_seg0[kt + 1]; s++) list.push_back(s);
+            }
+        }
+        lpos[nbatch] = (int64_t)list.size();
+        HIPCHK(c, hipMemcpyAsync(c->order.p, o32.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+        if ((rc = dalloc(c, c->seglist, sizeof(int32_t) * std::max<size_t>(list.size(), 1)))) return rc;
+        if (!list.empty()) HIPCHK(c, hipMemcpyAsync(c->seglist.p, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice, c->stream));
+        if ((rc = f(c, rho, rt, damp, kap2, nbatch, bpos, lpos))) return rc;
+        // host vectors are reused by the next sweep: wait for the uploads (and the sweep) to finish
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     c->have_eval = false;
     return 0;
 }

@@ -146,6 +146,14 @@ class Engine:
     def phr_update(self):
         self._chk(self.L.lb_phr_update(self.ctx), "lb_phr_update")
 
+    def sweeps(self, orders, nbatch: int, rho: float, rt: float | None = None, damp: float = 0.0, kap2: float = 1.0):
+        """len(orders) sweeps of batched row steps; orders: (nsw, n) permutations (int64)."""
+        rt = rho if rt is None else rt
+        orders = np.ascontiguousarray(np.atleast_2d(orders), np.int64)
+        assert orders.shape[1] == self.inst.n
+        self._chk(self.L.lb_sweeps(self.ctx, len(orders), float(rho), float(rt), float(damp), float(kap2),
+                                   ptr(orders, _i64p), int(nbatch)), "lb_sweeps")
+
     def sync(self):
         self._chk(self.L.lb_sync(self.ctx), "lb_sync")
 

@@ -387,3 +387,249 @@ void or_set_threads(int t) {
     (void)t;
 #endif
 }
+
+/* ------------------------------------------------------------------------------------------------ sweeps
+ * Batched Gauss-Seidel row steps (sweeps, alm_vcs.py:1360-1436, fast bookkeeping with pair_refresh = 1) with the
+ * per-batch snapshot semantics of the device path: every row of a batch reads V, z, Y, M, pdiag, Kq and the VCS
+ * state as they were when the batch started (the reference lets rows of one batch read each other's new values,
+ * Hogwild, alm_vcs.py:13-14); the batch's new rows, z += sum DD, Y[Q] += DD (_acc_rows, alm_vcs.py:430-437) and the
+ * VCS Y / Z / cnt / stZ updates (vcs_yupd, vcs_post, vcsops.py:159-213) are applied after the batch.
+ * orders: nsw x n row permutations; batches are np.array_split(order, nbatch) (the first n % nbatch batches hold
+ * one extra row). kap2 = vc_kap2 (default 1). */
+
+/* kq_level (alm_vcs.py:417-428): Kq_Q = 2 rho (sum_R [pp_QR > 0] r_QR y2_R + [pp_QQ > 0] r_QQ y2_Q) */
+static void or_kq_level(int32_t P, int k, const double *Y, const double *sig, const double *rq, const double *dsq,
+                        double w, double rho, double *out) {
+    double *y2 = (double *)malloc(sizeof(double) * (P > 0 ? P : 1));
+    for (int32_t R = 0; R < P; R++) { double s = 0.0; for (int c = 0; c < k; c++) s += Y[(int64_t)R * k + c] * Y[(int64_t)R * k + c]; y2[R] = s; }
+    for (int32_t Q = 0; Q < P; Q++) {
+        double s = 0.0, pq = 0.0;
+        for (int32_t R = 0; R < P; R++) {
+            double H = 0.0;
+            for (int c = 0; c < k; c++) H += Y[(int64_t)Q * k + c] * Y[(int64_t)R * k + c];
+            if (Q == R) H -= dsq[Q];
+            double r = (rq[Q] * rq[R]) * w;
+            double p = sig[(int64_t)Q * P + R] - rho * r * H;
+            if (p > 0.0) s += r * y2[R];
+            if (Q == R) pq = p;
+        }
+        double dg = pq > 0.0 ? (rq[Q] * rq[Q]) * w * y2[Q] : 0.0;
+        out[Q] = 2 * rho * (s + dg);
+    }
+    free(y2);
+}
+
+/* pair_all (alm_vcs.py:378-415): fresh pp row of cluster Q from the current Y, M_Q = sum_R pp_QR Y_R, pdiag_Q */
+static void or_pair_row(int32_t P, int k, int32_t Q, const double *Y, const double *sig, const double *rq,
+                        const double *dsq, double w, double rho, double *M, double *pdiag) {
+    double pq = 0.0;
+    for (int c = 0; c < k; c++) M[(int64_t)Q * k + c] = 0.0;
+    for (int32_t R = 0; R < P; R++) {
+        double p = sig[(int64_t)Q * P + R];
+        if (rho > 0.0) {
+            double h = 0.0;
+            for (int c = 0; c < k; c++) h += Y[(int64_t)Q * k + c] * Y[(int64_t)R * k + c];
+            if (R == Q) h -= dsq[Q];
+            p = p - (rho * ((rq[Q] * rq[R]) * w)) * h;
+            if (p < 0.0) p = 0.0;
+        }
+        if (R == Q) pq = p;
+        if (p != 0.0)
+            for (int c = 0; c < k; c++) M[(int64_t)Q * k + c] += p * Y[(int64_t)R * k + c];
+    }
+    pdiag[Q] = pq;
+}
+
+/* vcs_sigt (vcsops.py:94-108) for one row: effective multipliers at (V, Y) */
+static double or_vcs_p(double sg, double s, double di, int own, double rho, double wv, double rqR) {
+    if (rho <= 0.0 || di <= 0.0) return sg;
+    double g = own ? di * (s - di) : di * s;
+    double p = sg - rho * (wv * rqR / (di * di)) * g;
+    return p > 0.0 ? p : 0.0;
+}
+
+int or_sweeps(const or_inst *I, double *V, int nsw, double rho, double rt, double damp, const int64_t *orders,
+              int nbatch, double kap2, const int64_t *inc_ptr, const int64_t *inc_codes) {
+    const int64_t n = I->n; const int k = I->r;
+    if (nbatch < 1 || k > 64) return -1;
+    int64_t sP = 0, sPv = 0, nnzv = 0;
+    for (int32_t l = 0; l < I->nlev; l++) sP += I->P[l];
+    for (int32_t m = 0; m < I->nvcs; m++) { sPv += I->vP[m]; nnzv += I->vptr[(int64_t)m * (n + 1) + n]; }
+    double *z = (double *)calloc(k, sizeof(double));
+    double *Y = (double *)calloc((size_t)(sP ? sP : 1) * k, sizeof(double));
+    double *Yv = (double *)calloc((size_t)(sPv ? sPv : 1) * k, sizeof(double));
+    double *Kq = (double *)calloc(sP ? sP : 1, sizeof(double));
+    double *M = (double *)calloc((size_t)(sP ? sP : 1) * k, sizeof(double));
+    double *pdiag = (double *)calloc(sP ? sP : 1, sizeof(double));
+    double *Z = (double *)calloc((size_t)(sPv ? sPv : 1) * k, sizeof(double));
+    double *cnt = (double *)calloc(sPv ? sPv : 1, sizeof(double));
+    double *stZ = (double *)calloc(nnzv ? nnzv : 1, sizeof(double));
+    int64_t maxb = n / nbatch + 1;
+    double *Vn = (double *)malloc(sizeof(double) * (size_t)maxb * k);
+    double *Vo = (double *)malloc(sizeof(double) * (size_t)maxb * k);
+    double *DD = (double *)malloc(sizeof(double) * (size_t)maxb * k);
+    int64_t *cinc = NULL, *ccodes = NULL;
+    if (I->T && !(inc_ptr && inc_codes)) {
+        cinc = (int64_t *)malloc(sizeof(int64_t) * (n + 1)); ccodes = (int64_t *)malloc(sizeof(int64_t) * 3 * I->T);
+        or_incidence(n, I->T, I->TI, cinc, ccodes);
+        inc_ptr = cinc; inc_codes = ccodes;
+    }
+    for (int sw = 0; sw < nsw; sw++) {
+        const int64_t *order = orders + (int64_t)sw * n;
+        /* sweep start: z, Y (all pair levels), Kq, VCS state */
+        for (int c = 0; c < k; c++) z[c] = 0.0;
+        for (int64_t i = 0; i < n; i++) for (int c = 0; c < k; c++) z[c] += I->d[i] * V[i * k + c];
+        int64_t po = 0, so = 0;
+        for (int32_t l = 0; l < I->nlev; l++) {
+            or_cluster_sums(n, k, I->labs + (int64_t)l * n, I->P[l], I->d, V, Y + po * k);
+            if (rho > 0.0) or_kq_level(I->P[l], k, Y + po * k, I->sig + so, I->rq + po, I->dsq + po, I->pw, rho, Kq + po);
+            else for (int32_t Q = 0; Q < I->P[l]; Q++) Kq[po + Q] = 0.0;
+            po += I->P[l]; so += (int64_t)I->P[l] * I->P[l];
+        }
+        int64_t vpo = 0, eo = 0;
+        for (int32_t m = 0; m < I->nvcs; m++) {
+            const int32_t *lab = I->vlabs + (int64_t)m * n; const int64_t *ptr = I->vptr + (int64_t)m * (n + 1);
+            double *Ym = Yv + vpo * k;
+            or_cluster_sums(n, k, lab, I->vP[m], I->d, V, Ym);
+            for (int32_t R = 0; R < I->vP[m]; R++) { cnt[vpo + R] = 0.0; for (int c = 0; c < k; c++) Z[(vpo + R) * k + c] = 0.0; }
+            for (int64_t i = 0; i < n; i++) {
+                double di = I->d[i];
+                for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+                    int32_t R = I->vcol[eo + e]; double s = 0.0;
+                    for (int c = 0; c < k; c++) s += V[i * k + c] * Ym[(int64_t)R * k + c];
+                    double p = or_vcs_p(I->vsig[eo + e], s, di, R == lab[i], rho, I->wv, I->vrq[vpo + R]);
+                    stZ[eo + e] = p;
+                    if (p != 0.0) for (int c = 0; c < k; c++) Z[(vpo + R) * k + c] += p * (di * V[i * k + c]);
+                    if (p > 0.0) cnt[vpo + R] += 1.0;
+                }
+            }
+            vpo += I->vP[m]; eo += ptr[n];
+        }
+        int64_t start = 0;
+        for (int b = 0; b < nbatch; b++) {
+            int64_t bs = n / nbatch + (b < n % nbatch ? 1 : 0);
+            const int64_t *rows = order + start;
+            /* pair_all for every cluster (equal to the touched-only refresh for the rows that read it) */
+            po = 0; so = 0;
+            for (int32_t l = 0; l < I->nlev; l++) {
+                for (int32_t Q = 0; Q < I->P[l]; Q++)
+                    or_pair_row(I->P[l], k, Q, Y + po * k, I->sig + so, I->rq + po, I->dsq + po, I->pw, rho, M + po * k, pdiag + po);
+                po += I->P[l]; so += (int64_t)I->P[l] * I->P[l];
+            }
+            /* row steps from the batch snapshot (batch_step6, alm_vcs.py:255-351; vcs_pre, vcsops.py:127-156) */
+#pragma omp parallel for schedule(dynamic, 16)
+            for (int64_t idx = 0; idx < bs; idx++) {
+                int64_t i = rows[idx]; double di = I->d[i];
+                double g[64], q[64];
+                const double *vi = V + i * k;
+                for (int c = 0; c < k; c++) g[c] = -(di / I->L) * (z[c] - di * vi[c]);
+                for (int64_t e = I->rowptr[i]; e < I->rowptr[i + 1]; e++) {
+                    int64_t j = I->col[e]; double w = I->w ? I->w[e] : 1.0;
+                    for (int c = 0; c < k; c++) g[c] += w * V[j * k + c];
+                }
+                double kap = 0.0;
+                int64_t pl = 0;
+                for (int32_t l = 0; l < I->nlev; l++) {
+                    int64_t Q = pl + I->labs[(int64_t)l * n + i]; double pd = pdiag[Q] * di;
+                    for (int c = 0; c < k; c++) g[c] += di * (M[Q * k + c] - pd * vi[c]);
+                    kap += di * di * Kq[Q];
+                    pl += I->P[l];
+                }
+                int64_t vp = 0, eo2 = 0;
+                for (int32_t m = 0; m < I->nvcs; m++) {
+                    const int32_t *lab = I->vlabs + (int64_t)m * n; const int64_t *ptr = I->vptr + (int64_t)m * (n + 1);
+                    const double *Ym = Yv + vp * k; int32_t Ri = lab[i];
+                    double acc[64]; for (int c = 0; c < k; c++) acc[c] = 0.0;
+                    double kp = 0.0, pown = 0.0, sz = 0.0;
+                    for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+                        int32_t R = I->vcol[eo2 + e]; double s = 0.0;
+                        for (int c = 0; c < k; c++) s += vi[c] * Ym[(int64_t)R * k + c];
+                        double p;
+                        if (rho > 0.0 && di > 0.0) {
+                            double g2 = (R == Ri) ? di * (s - di) : di * s;
+                            double rr = I->wv * I->vrq[vp + R] / (di * di);
+                            p = I->vsig[eo2 + e] - rho * rr * g2;
+                            if (p > 0.0) {
+                                double y2 = 0.0;
+                                for (int c = 0; c < k; c++) y2 += Ym[(int64_t)R * k + c] * Ym[(int64_t)R * k + c];
+                                if (R == Ri) y2 = y2 - 2 * di * s + di * di;
+                                kp += rho * rr * di * di * y2;
+                            } else p = 0.0;
+                        } else p = I->vsig[eo2 + e];
+                        if (R == Ri) { pown = p; sz = stZ[eo2 + e]; }
+                        if (p != 0.0) for (int c = 0; c < k; c++) acc[c] += p * Ym[(int64_t)R * k + c];
+                    }
+                    for (int c = 0; c < k; c++)
+                        g[c] += 0.5 * di * (acc[c] - pown * di * vi[c]) + 0.5 * di * (Z[(vp + Ri) * k + c] - sz * di * vi[c]);
+                    if (rho > 0.0) kp += kap2 * rho * di * di * I->wv * I->vrq[vp + Ri] * (cnt[vp + Ri] - (sz > 0.0 ? 1.0 : 0.0));
+                    kap += kp;
+                    vp += I->vP[m]; eo2 += ptr[n];
+                }
+                if (I->T) {
+                    for (int64_t e = inc_ptr[i]; e < inc_ptr[i + 1]; e++) {
+                        int64_t code = inc_codes[e], t = code / 3; int r = (int)(code % 3);
+                        const double *va = V + (int64_t)I->TI[3 * t] * k, *vb = V + (int64_t)I->TI[3 * t + 1] * k, *vc = V + (int64_t)I->TI[3 * t + 2] * k;
+                        double tau = 0.0, q2 = 0.0;
+                        for (int c = 0; c < k; c++) {
+                            tau += va[c] * vb[c] + vb[c] * vc[c] - va[c] * vc[c];
+                            q[c] = r == 0 ? vb[c] - vc[c] : (r == 1 ? va[c] + vc[c] : vb[c] - va[c]);
+                            q2 += q[c] * q[c];
+                        }
+                        double pt = I->lam[t] - rt * (1.0 - tau);
+                        if (pt > 0.0) { for (int c = 0; c < k; c++) g[c] -= 0.5 * pt * q[c]; kap += rt * q2; }
+                    }
+                }
+                double gn = 0.0;
+                for (int c = 0; c < k; c++) gn += g[c] * g[c];
+                double w = 0.5 * kap + damp * sqrt(gn), nv = 0.0;
+                for (int c = 0; c < k; c++) { q[c] = g[c] + w * vi[c]; nv += q[c] * q[c]; }
+                for (int c = 0; c < k; c++) Vo[idx * k + c] = vi[c];
+                if (nv <= 0.0) {
+                    for (int c = 0; c < k; c++) { Vn[idx * k + c] = vi[c]; DD[idx * k + c] = 0.0; }
+                    continue;
+                }
+                nv = 1.0 / sqrt(nv);
+                for (int c = 0; c < k; c++) { double vn = q[c] * nv; Vn[idx * k + c] = vn; DD[idx * k + c] = di * (vn - vi[c]); }
+            }
+            /* apply the batch, then the aggregate updates in batch-row order */
+            for (int64_t idx = 0; idx < bs; idx++) {
+                int64_t i = rows[idx];
+                for (int c = 0; c < k; c++) V[i * k + c] = Vn[idx * k + c];
+            }
+            for (int64_t idx = 0; idx < bs; idx++) {
+                int64_t i = rows[idx];
+                for (int c = 0; c < k; c++) {
+                    double x = DD[idx * k + c]; z[c] += x;
+                    int64_t pl = 0;
+                    for (int32_t l = 0; l < I->nlev; l++) { Y[(pl + I->labs[(int64_t)l * n + i]) * k + c] += x; pl += I->P[l]; }
+                    int64_t vp = 0;
+                    for (int32_t m = 0; m < I->nvcs; m++) { Yv[(vp + I->vlabs[(int64_t)m * n + i]) * k + c] += x; vp += I->vP[m]; }
+                }
+            }
+            int64_t vp = 0, eo3 = 0;
+            for (int32_t m = 0; m < I->nvcs; m++) {   /* vcs_post: new effective multipliers of the moved rows */
+                const int32_t *lab = I->vlabs + (int64_t)m * n; const int64_t *ptr = I->vptr + (int64_t)m * (n + 1);
+                const double *Ym = Yv + vp * k;
+                for (int64_t idx = 0; idx < bs; idx++) {
+                    int64_t i = rows[idx]; double di = I->d[i];
+                    for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+                        int32_t R = I->vcol[eo3 + e]; double s = 0.0;
+                        for (int c = 0; c < k; c++) s += V[i * k + c] * Ym[(int64_t)R * k + c];
+                        double sn = or_vcs_p(I->vsig[eo3 + e], s, di, R == lab[i], rho, I->wv, I->vrq[vp + R]);
+                        double so2 = stZ[eo3 + e];
+                        if (sn != 0.0 || so2 != 0.0)
+                            for (int c = 0; c < k; c++) Z[(vp + R) * k + c] += di * (sn * V[i * k + c] - so2 * Vo[idx * k + c]);
+                        if (sn > 0.0) { if (!(so2 > 0.0)) cnt[vp + R] += 1.0; }
+                        else if (so2 > 0.0) cnt[vp + R] -= 1.0;
+                        stZ[eo3 + e] = sn;
+                    }
+                }
+                vp += I->vP[m]; eo3 += ptr[n];
+            }
+            start += bs;
+        }
+    }
+    free(z); free(Y); free(Yv); free(Kq); free(M); free(pdiag); free(Z); free(cnt); free(stZ); free(Vn); free(Vo); free(DD);
+    free(cinc); free(ccodes);
+    return 0;
+}

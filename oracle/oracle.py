@@ -57,6 +57,9 @@ def lib():
         L.or_num_threads.restype = C.c_int
         L.or_set_threads.argtypes = [C.c_int]
         L.or_incidence.argtypes = [C.c_int64, C.c_int64, _i32p, _i64p, _i64p]
+        L.or_sweeps.argtypes = [C.POINTER(OrInst), _f64p, C.c_int, C.c_double, C.c_double, C.c_double, _i64p, C.c_int,
+                                C.c_double, _i64p, _i64p]
+        L.or_sweeps.restype = C.c_int
         _lib = L
     return _lib
 
@@ -123,6 +126,23 @@ def eval_c(inst, V, rho, rt=None, threads: int | None = None, incidence=None):
                     _p(pp, _f64p), _p(pt, _f64p), _p(pv, _f64p), _p(ip, _i64p), _p(ic, _i64p))
     assert rc == 0
     return parts, G, pp[:sum(l.P * l.P for l in inst.levels)], pt[:inst.T], pv[:sum(v.nnz for v in inst.vcs)]
+
+
+def sweeps_c(inst, V, orders, nbatch, rho, rt=None, damp=0.0, kap2=1.0, threads=None, incidence=None):
+    """C oracle sweeps (batch-snapshot semantics). Returns the new V (the input is not modified)."""
+    L_ = lib()
+    if threads:
+        L_.or_set_threads(int(threads))
+    V = np.array(V, np.float64, order="C", copy=True)
+    n, r = V.shape
+    rt = rho if rt is None else rt
+    orders = np.ascontiguousarray(np.atleast_2d(orders), np.int64)
+    pk = _Packed(inst, r)
+    ip, ic = (None, None) if incidence is None else incidence
+    rc = L_.or_sweeps(C.byref(pk.s), _p(V, _f64p), len(orders), float(rho), float(rt), float(damp), _p(orders, _i64p),
+                      int(nbatch), float(kap2), _p(ip, _i64p), _p(ic, _i64p))
+    assert rc == 0
+    return V
 
 
 def incidence_c(n, TI):
@@ -309,3 +329,142 @@ def eval_dense(inst, V, rho, rt=None):
     else:
         f = inst.C0 + (float(inst.lam.sum()) if inst.T else 0.0) + float((Cm * X).sum())
     return f, G, pp_list, pt, pv_list
+
+
+# ----------------------------------------------------------------------------------------------- numpy sweeps
+def sweeps_numpy(inst, V, orders, nbatch, rho, rt=None, damp=0.0, kap2=1.0):
+    """Vectorised restatement of the batched row steps with per-batch snapshot semantics (sweeps, alm_vcs.py:1360-1436;
+    batch_step6 255-351; pair_all 378-415; kq_level 417-428; vcs_state / vcs_pre / vcs_post vcsops.py:127-213,
+    308-323)."""
+    rt = rho if rt is None else rt
+    V = np.array(V, np.float64, copy=True)
+    n, k = V.shape
+    d = inst.d
+    L = inst.L
+    A = _A(inst)
+    rows_t = None
+    if inst.T:
+        a_, b_, c_ = (inst.TI[:, q].astype(np.int64) for q in range(3))
+    for order in np.atleast_2d(orders):
+        z = d @ V
+        Ys = [_W(lv.lab, d, lv.P) @ V for lv in inst.levels]
+        Kqs = []
+        for lv, Y in zip(inst.levels, Ys):
+            if rho > 0:
+                H = Y @ Y.T
+                H[np.diag_indices(lv.P)] -= lv.dsq
+                r = np.outer(lv.rq, lv.rq) * inst.pairlvl_weight
+                act = (lv.sig - rho * r * H > 0) * r
+                y2 = (Y * Y).sum(1)
+                Kqs.append(2 * rho * (act @ y2 + np.diag(act) * y2))
+            else:
+                Kqs.append(np.zeros(lv.P))
+        vst = []
+        for vl in inst.vcs:
+            Y = _W(vl.lab, d, vl.P) @ V
+            rows = np.repeat(np.arange(n), np.diff(vl.ptr))
+            R = vl.col.astype(np.int64)
+            own = R == vl.lab[rows]
+            st = _vcs_eff(vl, V, Y, d, rows, R, own, rho, inst.wv)
+            Z = np.zeros_like(Y)
+            np.add.at(Z, R, (st * d[rows])[:, None] * V[rows])
+            cnt = np.bincount(R, weights=(st > 0).astype(float), minlength=vl.P)
+            vst.append(dict(Y=Y, rows=rows, R=R, own=own, st=st, Z=Z, cnt=cnt))
+        for rows_b in np.array_split(np.asarray(order, np.int64), nbatch):
+            if len(rows_b) == 0:
+                continue
+            Vs = V.copy()
+            vi = Vs[rows_b]
+            di = d[rows_b]
+            g = A[rows_b] @ Vs - (di / L)[:, None] * (z[None, :] - di[:, None] * vi)
+            kap = np.zeros(len(rows_b))
+            for lv, Y, Kq in zip(inst.levels, Ys, Kqs):
+                if rho > 0:
+                    H = Y @ Y.T
+                    H[np.diag_indices(lv.P)] -= lv.dsq
+                    pp = np.maximum(0.0, lv.sig - rho * np.outer(lv.rq, lv.rq) * inst.pairlvl_weight * H)
+                else:
+                    pp = lv.sig
+                M = pp @ Y
+                Q = lv.lab[rows_b]
+                g += di[:, None] * (M[Q] - (np.diag(pp)[Q] * di)[:, None] * vi)
+                kap += di * di * Kq[Q]
+            for vl, s_ in zip(inst.vcs, vst):
+                Y = s_["Y"]
+                sel = np.isin(s_["rows"], rows_b)
+                er = s_["rows"][sel]
+                eR = s_["R"][sel]
+                eown = s_["own"][sel]
+                pos = {int(i): t for t, i in enumerate(rows_b)}
+                ix = np.array([pos[int(i)] for i in er], np.int64)
+                s = np.einsum("ij,ij->i", Vs[er], Y[eR])
+                dd = d[er]
+                gg = np.where(eown, dd * (s - dd), dd * s)
+                if rho > 0:
+                    rr = inst.wv * vl.rq[eR] / (dd * dd)
+                    p = vl.sig[sel] - rho * rr * gg
+                    y2 = (Y[eR] ** 2).sum(1)
+                    y2 = np.where(eown, y2 - 2 * dd * s + dd * dd, y2)
+                    kp = np.where(p > 0, rho * rr * dd * dd * y2, 0.0)
+                    p = np.maximum(p, 0.0)
+                    np.add.at(kap, ix, kp)
+                else:
+                    p = vl.sig[sel]
+                acc = np.zeros((len(rows_b), k))
+                np.add.at(acc, ix, p[:, None] * Y[eR])
+                pown = np.zeros(len(rows_b))
+                sz = np.zeros(len(rows_b))
+                pown[ix[eown]] = p[eown]
+                sz[ix[eown]] = s_["st"][sel][eown]
+                Ri = vl.lab[rows_b]
+                g += 0.5 * di[:, None] * (acc - (pown * di)[:, None] * vi) + 0.5 * di[:, None] * (s_["Z"][Ri] - (sz * di)[:, None] * vi)
+                if rho > 0:
+                    kap += kap2 * rho * di * di * inst.wv * vl.rq[Ri] * (s_["cnt"][Ri] - (sz > 0))
+            if inst.T:
+                pos = np.full(n, -1, np.int64)
+                pos[rows_b] = np.arange(len(rows_b))
+                tau = (np.einsum("ij,ij->i", Vs[a_], Vs[b_]) + np.einsum("ij,ij->i", Vs[b_], Vs[c_])
+                       - np.einsum("ij,ij->i", Vs[a_], Vs[c_]))
+                pt = inst.lam - rt * (1.0 - tau)
+                act = pt > 0
+                for role, (own_, qv) in enumerate(((a_, Vs[b_] - Vs[c_]), (b_, Vs[a_] + Vs[c_]), (c_, Vs[b_] - Vs[a_]))):
+                    m = act & (pos[own_] >= 0)
+                    np.add.at(g, pos[own_[m]], -0.5 * pt[m][:, None] * qv[m])
+                    np.add.at(kap, pos[own_[m]], rt * (qv[m] ** 2).sum(1))
+            gn = np.linalg.norm(g, axis=1)
+            w = 0.5 * kap + damp * gn
+            q = g + w[:, None] * vi
+            nv = np.linalg.norm(q, axis=1)
+            ok = nv > 0
+            vn = vi.copy()
+            vn[ok] = q[ok] / nv[ok][:, None]
+            DD = di[:, None] * (vn - vi)
+            V[rows_b] = vn
+            z = z + DD.sum(0)
+            for lv, Y in zip(inst.levels, Ys):
+                np.add.at(Y, lv.lab[rows_b], DD)
+            for vl, s_ in zip(inst.vcs, vst):
+                np.add.at(s_["Y"], vl.lab[rows_b], DD)
+                sel = np.flatnonzero(np.isin(s_["rows"], rows_b))
+                er = s_["rows"][sel]
+                sn = _vcs_eff(vl, V, s_["Y"], d, er, s_["R"][sel], s_["own"][sel], rho, inst.wv, sig=vl.sig[sel])
+                so = s_["st"][sel]
+                posr = np.full(n, -1, np.int64)
+                posr[rows_b] = np.arange(len(rows_b))
+                vold = vi[posr[er]]
+                np.add.at(s_["Z"], s_["R"][sel], d[er][:, None] * (sn[:, None] * V[er] - so[:, None] * vold))
+                np.add.at(s_["cnt"], s_["R"][sel], (sn > 0).astype(float) - (so > 0).astype(float))
+                s_["st"][sel] = sn
+    return V
+
+
+def _vcs_eff(vl, V, Y, d, rows, R, own, rho, wv, sig=None):
+    sig = vl.sig if sig is None else sig
+    s = np.einsum("ij,ij->i", V[rows], Y[R])
+    dd = d[rows]
+    if rho <= 0:
+        return sig.copy()
+    g = np.where(own, dd * (s - dd), dd * s)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        p = np.maximum(0.0, sig - rho * (wv * vl.rq[R] / (dd * dd)) * g)
+    return np.where(dd > 0, p, sig)

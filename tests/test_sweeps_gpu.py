@@ -1,0 +1,83 @@
+"""Parity of lb_sweeps (batched row steps on the device) with the FP64 oracle (same orders, same batches, batch
+snapshot semantics). Tolerances: FP64 V 1e-10 max abs; FP32 V 2e-4 after the sweeps (FP32 row arithmetic, errors
+compound over steps); BF16-stored V 5e-2 (every stored row is rounded to 8 bits of mantissa while the oracle keeps
+FP64), plus unit rows to storage precision."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import small_instance, chung_lu_instance
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(inst, V, dtype, r, orders, nbatch, rho, damp, tuning=None):
+    from lagbound_amd.engine import Engine
+    eng = Engine(r=r, dtype=dtype)
+    if tuning:
+        eng.set_tuning(*tuning)
+    eng.set_instance(inst)
+    eng.set_V(V)
+    V0 = eng.get_V()
+    print(f"[sweeps] dtype={dtype} n={inst.n} nbatch={nbatch} rho={rho} nsw={len(orders)}", flush=True)
+    eng.sweeps(orders, nbatch, rho, damp=damp)
+    W = eng.get_V()
+    eng.close()
+    return V0, W
+
+
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-10), ("f32", 2e-4)])
+@pytest.mark.parametrize("rho,nbatch", [(0.0, 4), (0.3, 8), (0.3, 1)])
+def test_small_sweeps(dtype, tol, rho, nbatch):
+    inst, V = small_instance(n=60, seed=3, k=16, sig_scale=0.05)
+    rng = np.random.default_rng(4)
+    orders = np.stack([rng.permutation(inst.n) for _ in range(2)])
+    V0, W = _run(inst, V, dtype, 16, orders, nbatch, rho, 0.1)
+    Wo = O.sweeps_c(inst, V0, orders, nbatch, rho, damp=0.1)
+    assert np.abs(W - Wo).max() <= tol
+    assert np.abs(np.linalg.norm(W, axis=1) - 1).max() <= (1e-12 if dtype == "f64" else 1e-5)
+
+
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-9), ("f32", 5e-4)])
+def test_chung_lu_sweeps_with_hubs(dtype, tol):
+    """Hub rows (segment path) inside batches; weighted/unweighted edges; three sweeps."""
+    inst, V = chung_lu_instance(n=20000, seed=5, k=16)
+    rng = np.random.default_rng(6)
+    orders = np.stack([rng.permutation(inst.n) for _ in range(3)])
+    V0, W = _run(inst, V, dtype, 16, orders, 16, 0.02, 0.0)
+    Wo = O.sweeps_c(inst, V0, orders, 16, 0.02, damp=0.0)
+    assert np.abs(W - Wo).max() <= tol
+
+
+def test_segments_forced_small():
+    inst, V = small_instance(n=50, seed=6, k=8, weighted=True, ntri=80)
+    rng = np.random.default_rng(7)
+    orders = np.stack([rng.permutation(inst.n) for _ in range(2)])
+    V0, W = _run(inst, V, "f64", 8, orders, 5, 0.5, 0.2, tuning=(2, 2, 3))
+    Wo = O.sweeps_c(inst, V0, orders, 5, 0.5, damp=0.2)
+    assert np.abs(W - Wo).max() <= 1e-10
+
+
+def test_bf16_sweeps():
+    inst, V = chung_lu_instance(n=5000, seed=8, k=16)
+    rng = np.random.default_rng(9)
+    orders = rng.permutation(inst.n)[None, :]
+    V0, W = _run(inst, V, "bf16", 16, orders, 8, 0.02, 0.0)
+    Wo = O.sweeps_c(inst, V0, orders, 8, 0.02, damp=0.0)
+    assert np.abs(W - Wo).max() <= 5e-2
+    assert np.abs(np.linalg.norm(W, axis=1) - 1).max() <= 2e-2
+
+
+def test_sweeps_then_eval_consistent():
+    """After sweeps the resident V feeds the evaluation unchanged (no stale aggregates)."""
+    from lagbound_amd.engine import Engine
+    inst, V = chung_lu_instance(n=8000, seed=10, k=16)
+    eng = Engine(r=16, dtype="f64")
+    eng.set_instance(inst)
+    eng.set_V(V)
+    rng = np.random.default_rng(11)
+    eng.sweeps(rng.permutation(inst.n)[None, :], 8, 0.02)
+    parts = eng.eval(0.02)
+    po, Go, _, _, _ = O.eval_c(inst, eng.get_V(), 0.02)
+    assert abs(parts[0] - po[0]) <= 1e-10 * max(1.0, abs(po[0]))
+    eng.close()
