@@ -633,3 +633,195 @@ int or_sweeps(const or_inst *I, double *V, int nsw, double rho, double rt, doubl
     free(cinc); free(ccodes);
     return 0;
 }
+
+/* ------------------------------------------------------------------------------------------------ separation
+ * Triangle separation with the reference's semantics (separate_ref / separate1_ref, alm_vcs.py:965-984): FP64 dot
+ * products summed over q in order without contraction, 512 chunks of consecutive centres, a 2000-bin histogram of
+ * the violations vm > vtol, the first bin b0 whose tail count is <= cap, vmin = vtol + (2 - vtol) b0 / nb, and per
+ * chunk the candidates with vm >= vmin in generation order (centre, then arm pairs), at most the chunk's histogram
+ * tail count. Tie rule for the top-maxh / top-maxk cuts: descending value, ties by position (stable); the
+ * reference's unstable argsort agrees whenever the values at the cut are distinct. */
+
+typedef struct { double x; int64_t id; int64_t pos; } or_arm;
+
+static int or_cmp_desc(const void *a, const void *b) {
+    const or_arm *p = (const or_arm *)a, *q = (const or_arm *)b;
+    if (p->x > q->x) return -1;
+    if (p->x < q->x) return 1;
+    return p->pos < q->pos ? -1 : (p->pos > q->pos ? 1 : 0);
+}
+static int or_cmp_id(const void *a, const void *b) {
+    const or_arm *p = (const or_arm *)a, *q = (const or_arm *)b;
+    if (p->id < q->id) return -1;
+    if (p->id > q->id) return 1;
+    return p->pos < q->pos ? -1 : (p->pos > q->pos ? 1 : 0);
+}
+
+static double or_dot(const double *a, const double *b, int k) {
+    double s = 0.0;
+    for (int q = 0; q < k; q++) s += a[q] * b[q];
+    return s;
+}
+
+/* _centre (alm_vcs.py:554-565): arms of centre j; returns the count, arms in H (CSR order, or value-descending
+ * after the top-maxh cut) */
+static int64_t or_centre(int64_t j, const int64_t *rowptr, const int32_t *col, int k, const double *V, double thresh,
+                         int64_t maxh, or_arm *H) {
+    int64_t m = 0;
+    for (int64_t e = rowptr[j]; e < rowptr[j + 1]; e++) {
+        int64_t i = col[e];
+        double s = or_dot(V + i * k, V + j * k, k);
+        if (s > thresh) { H[m].x = s; H[m].id = i; H[m].pos = m; m++; }
+    }
+    if (m > maxh) {
+        qsort(H, (size_t)m, sizeof(or_arm), or_cmp_desc);
+        m = maxh;
+    }
+    return m;
+}
+
+static int or_isadj(const int64_t *rowptr, const int32_t *col, int64_t j, int64_t x) {
+    int64_t lo = rowptr[j], hi = rowptr[j + 1];
+    while (lo < hi) { int64_t mid = (lo + hi) / 2; if (col[mid] < x) lo = mid + 1; else hi = mid; }
+    return lo < rowptr[j + 1] && col[lo] == x;
+}
+
+/* _centre1 (alm_vcs.py:608-632): non-adjacent 2-hop arms K of centre j through the first hub2 arms of H (at most
+ * cap2 neighbours each), value > thresh, sorted by id and deduplicated, then the top-maxk by value */
+static int64_t or_centre1(int64_t j, const int64_t *rowptr, const int32_t *col, int k, const double *V, double thresh,
+                          const or_arm *H, int64_t mh, int64_t maxk, int64_t hub2, int64_t cap2, or_arm *K) {
+    int64_t nh = mh < hub2 ? mh : hub2, m = 0;
+    for (int64_t a = 0; a < nh; a++) {
+        int64_t i2 = H[a].id, lo = rowptr[i2], hi = rowptr[i2 + 1];
+        if (hi > lo + cap2) hi = lo + cap2;
+        for (int64_t e = lo; e < hi; e++) {
+            int64_t c = col[e];
+            if (c == j || or_isadj(rowptr, col, j, c)) continue;
+            double s = or_dot(V + c * k, V + j * k, k);
+            if (s > thresh) { K[m].x = s; K[m].id = c; K[m].pos = m; m++; }
+        }
+    }
+    if (m > 1) {
+        qsort(K, (size_t)m, sizeof(or_arm), or_cmp_id);
+        int64_t u = 1;
+        for (int64_t t = 1; t < m; t++) if (K[t].id != K[u - 1].id) K[u++] = K[t];
+        m = u;
+    }
+    for (int64_t t = 0; t < m; t++) K[t].pos = t;
+    if (m > maxk) {
+        qsort(K, (size_t)m, sizeof(or_arm), or_cmp_desc);
+        m = maxk;
+    }
+    return m;
+}
+
+static int or_bin(double vm, double vtol, int nb) {
+    int b = (int)((vm - vtol) / (2.0 - vtol) * nb);
+    return b < nb - 1 ? b : nb - 1;
+}
+
+/* one family over every centre of chunk ch: fam 0 = wedges (a, j, c), a, c in H; fam 1 = one-edge arm (h, j, k),
+ * h in H, k in K. mode 0: histogram (hist[nb]); mode 1: collect candidates with vm >= vmin into out/val from *m up
+ * to end. */
+static void or_sep_chunk(int fam, int mode, int64_t ch, int64_t cs, int64_t n, const int64_t *rowptr,
+                         const int32_t *col, int k, const double *V, double thresh, int64_t maxh, int64_t maxk,
+                         int64_t hub2, int64_t cap2, double vtol, int nb, int64_t *hist, double vmin, int64_t *m,
+                         int64_t end, int64_t *out, double *val, or_arm *H, or_arm *K) {
+    int64_t j1 = (ch + 1) * cs < n ? (ch + 1) * cs : n;
+    for (int64_t j = ch * cs; j < j1; j++) {
+        int64_t dj = rowptr[j + 1] - rowptr[j];
+        if (dj < (fam == 0 ? 2 : 1)) continue;
+        int64_t mh = or_centre(j, rowptr, col, k, V, thresh, maxh, H);
+        if (fam == 0) {
+            for (int64_t x = 0; x < mh; x++)
+                for (int64_t y = x + 1; y < mh; y++) {
+                    double s = or_dot(V + H[x].id * k, V + H[y].id * k, k);
+                    double vm = H[x].x + H[y].x - s - 1.0;
+                    if (mode == 0) { if (vm > vtol) hist[or_bin(vm, vtol, nb)]++; }
+                    else if (vm >= vmin && *m < end) {
+                        int64_t a = H[x].id < H[y].id ? H[x].id : H[y].id, c = H[x].id < H[y].id ? H[y].id : H[x].id;
+                        out[3 * *m] = a; out[3 * *m + 1] = j; out[3 * *m + 2] = c; val[*m] = vm; (*m)++;
+                    }
+                }
+        } else {
+            int64_t mk = or_centre1(j, rowptr, col, k, V, thresh, H, mh, maxk, hub2, cap2, K);
+            for (int64_t x = 0; x < mh; x++)
+                for (int64_t y = 0; y < mk; y++) {
+                    if (H[x].id == K[y].id) continue;
+                    double s = or_dot(V + H[x].id * k, V + K[y].id * k, k);
+                    double vm = H[x].x + K[y].x - s - 1.0;
+                    if (mode == 0) { if (vm > vtol) hist[or_bin(vm, vtol, nb)]++; }
+                    else if (vm >= vmin && *m < end) {
+                        int64_t a = H[x].id < K[y].id ? H[x].id : K[y].id, c = H[x].id < K[y].id ? K[y].id : H[x].id;
+                        out[3 * *m] = a; out[3 * *m + 1] = j; out[3 * *m + 2] = c; val[*m] = vm; (*m)++;
+                    }
+                }
+        }
+    }
+}
+
+/* separate_ref (fam 0) / separate1_ref (fam 1). out: cap x 3 int64, val: cap. Returns the number of candidates
+ * kept (val > 0), in the reference's output order; *vmin_out = vmin (or -1 if nothing is selected). */
+int64_t or_separate(int fam, int64_t n, const int64_t *rowptr, const int32_t *col, int k, const double *V,
+                    double thresh, int64_t maxh, int64_t maxk, int64_t hub2, int64_t cap2, double vtol, int64_t cap,
+                    int nchunk, int nb, int64_t *out, double *val, double *vmin_out, int64_t *hist_out) {
+    int64_t cs = (n + nchunk - 1) / nchunk;
+    int64_t maxdeg = 1;
+    for (int64_t j = 0; j < n; j++) if (rowptr[j + 1] - rowptr[j] > maxdeg) maxdeg = rowptr[j + 1] - rowptr[j];
+    /* 2-hop candidates of one centre: at most min(hub2, maxdeg) arms x min(cap2, maxdeg) neighbours */
+    int64_t kcap = (hub2 < maxdeg ? hub2 : maxdeg) * (cap2 < maxdeg ? cap2 : maxdeg);
+    if (kcap < 1) kcap = 1;
+    int64_t *hist = (int64_t *)calloc((size_t)nchunk * nb, sizeof(int64_t));
+#pragma omp parallel
+    {
+        or_arm *H = (or_arm *)malloc(sizeof(or_arm) * (size_t)maxdeg);
+        or_arm *K = (or_arm *)malloc(sizeof(or_arm) * (size_t)kcap);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t ch = 0; ch < nchunk; ch++)
+            or_sep_chunk(fam, 0, ch, cs, n, rowptr, col, k, V, thresh, maxh, maxk, hub2, cap2, vtol, nb,
+                         hist + ch * nb, 0.0, NULL, 0, NULL, NULL, H, K);
+        free(H); free(K);
+    }
+    if (hist_out) memcpy(hist_out, hist, sizeof(int64_t) * (size_t)nchunk * nb);
+    int64_t *cum = (int64_t *)calloc((size_t)nb + 1, sizeof(int64_t));
+    for (int b = nb - 1; b >= 0; b--) {
+        int64_t t = 0;
+        for (int64_t ch = 0; ch < nchunk; ch++) t += hist[ch * nb + b];
+        cum[b] = cum[b + 1] + t;
+    }
+    int b0 = nb;
+    for (int b = 0; b < nb; b++) if (cum[b] <= cap) { b0 = b; break; }
+    free(cum);
+    if (b0 >= nb) { free(hist); *vmin_out = -1.0; return 0; }
+    double vmin = vtol + (2.0 - vtol) * b0 / nb;
+    *vmin_out = vmin;
+    int64_t *counts = (int64_t *)calloc((size_t)nchunk, sizeof(int64_t));
+    int64_t *offs = (int64_t *)calloc((size_t)nchunk + 1, sizeof(int64_t));
+    for (int64_t ch = 0; ch < nchunk; ch++) {
+        for (int b = b0; b < nb; b++) counts[ch] += hist[ch * nb + b];
+        offs[ch + 1] = offs[ch] + counts[ch];
+    }
+    int64_t tot = offs[nchunk];
+    int64_t *o2 = (int64_t *)calloc((size_t)(tot > 0 ? tot : 1) * 3, sizeof(int64_t));
+    double *v2 = (double *)calloc((size_t)(tot > 0 ? tot : 1), sizeof(double));
+#pragma omp parallel
+    {
+        or_arm *H = (or_arm *)malloc(sizeof(or_arm) * (size_t)maxdeg);
+        or_arm *K = (or_arm *)malloc(sizeof(or_arm) * (size_t)kcap);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t ch = 0; ch < nchunk; ch++) {
+            int64_t m = offs[ch];
+            or_sep_chunk(fam, 1, ch, cs, n, rowptr, col, k, V, thresh, maxh, maxk, hub2, cap2, vtol, nb, NULL, vmin,
+                         &m, offs[ch] + counts[ch], o2, v2, H, K);
+        }
+        free(H); free(K);
+    }
+    int64_t kept = 0;
+    for (int64_t t = 0; t < tot; t++)
+        if (v2[t] > 0.0) {
+            out[3 * kept] = o2[3 * t]; out[3 * kept + 1] = o2[3 * t + 1]; out[3 * kept + 2] = o2[3 * t + 2];
+            val[kept] = v2[t]; kept++;
+        }
+    free(hist); free(counts); free(offs); free(o2); free(v2);
+    return kept;
+}

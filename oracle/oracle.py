@@ -60,6 +60,10 @@ def lib():
         L.or_sweeps.argtypes = [C.POINTER(OrInst), _f64p, C.c_int, C.c_double, C.c_double, C.c_double, _i64p, C.c_int,
                                 C.c_double, _i64p, _i64p]
         L.or_sweeps.restype = C.c_int
+        L.or_separate.argtypes = [C.c_int, C.c_int64, _i64p, _i32p, C.c_int, _f64p, C.c_double, C.c_int64, C.c_int64,
+                                  C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int, C.c_int, _i64p, _f64p, _f64p,
+                                  _i64p]
+        L.or_separate.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -143,6 +147,28 @@ def sweeps_c(inst, V, orders, nbatch, rho, rt=None, damp=0.0, kap2=1.0, threads=
                       int(nbatch), float(kap2), _p(ip, _i64p), _p(ic, _i64p))
     assert rc == 0
     return V
+
+
+def separate_c(inst, V, fam=0, thresh=0.3, maxh=100, maxk=60, hub2=8, cap2=200, vtol=1e-3, cap=1000,
+               nchunk=512, nb=2000, threads=None, with_hist=False):
+    """Triangle separation (fam 0: wedges, separate_ref; fam 1: one-edge arm, separate1_ref). Returns
+    (TI (m x 3 int64, (min, centre, max)), val, vmin[, hist (nchunk x nb)])."""
+    L_ = lib()
+    if threads:
+        L_.or_set_threads(int(threads))
+    V = np.ascontiguousarray(V, np.float64)
+    n, k = V.shape
+    out = np.zeros((max(1, cap), 3), np.int64)
+    val = np.zeros(max(1, cap))
+    vmin = np.zeros(1)
+    hist = np.zeros((nchunk, nb), np.int64) if with_hist else None
+    rowptr = np.ascontiguousarray(inst.rowptr, np.int64)
+    col = np.ascontiguousarray(inst.col, np.int32)
+    m = L_.or_separate(int(fam), n, _p(rowptr, _i64p), _p(col, _i32p), k, _p(V, _f64p), float(thresh), int(maxh),
+                       int(maxk), int(hub2), int(cap2), float(vtol), int(cap), int(nchunk), int(nb), _p(out, _i64p),
+                       _p(val, _f64p), _p(vmin, _f64p), _p(hist, _i64p))
+    res = (out[:m].copy(), val[:m].copy(), float(vmin[0]))
+    return res + (hist,) if with_hist else res
 
 
 def incidence_c(n, TI):
