@@ -96,6 +96,17 @@ int lb_phr_update(lb_ctx *ctx);
 int lb_sweeps(lb_ctx *ctx, int nsw, double rho, double rt, double damp, double kap2, const int64_t *orders,
               int nbatch);
 
+/* Triangle separation on the resident V (separate_ref fam 0, wedges (a, centre, c); separate1_ref fam 1, one-edge
+ * arm (h, centre, k); alm_vcs.py:965-984, kernels 554-666): arms = neighbours with X_ij > thresh (top maxh by value),
+ * violation vm = X_aj + X_cj - X_ac - 1 in FP64 summed in the reference's order, a per-chunk (nchunk chunks of
+ * centres) nb-bin histogram of vm > vtol, threshold vmin of the first bin whose tail holds at most cap candidates,
+ * and the candidates vm >= vmin in the reference's generation order. TI_out: cap x 3 int64 (min, centre, max),
+ * val_out: cap; *count = number written; *vmin = threshold (-1 if none). Bit-exact with the reference on the same V;
+ * tie rule at the top-maxh / top-maxk cuts: descending value, ties in prior order. fam 1 needs hub2 * cap2 <= 2048. */
+int lb_separate_triangles(lb_ctx *ctx, int fam, double thresh, int64_t maxh, int64_t maxk, int64_t hub2,
+                          int64_t cap2, double vtol, int64_t cap, int nchunk, int nb, int64_t *TI_out,
+                          double *val_out, int64_t *count, double *vmin);
+
 /* Wait for the context's stream. */
 int lb_sync(lb_ctx *ctx);
 

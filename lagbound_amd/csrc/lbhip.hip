@@ -17,6 +17,7 @@
 //   lb_k_finalize fixed-order FP64 reduction of every per-block partial into the parts[] scalars.
 // Global sums are FP64; per-row arithmetic is FP32 (LB_F32, LB_BF16) or FP64 (LB_F64).
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -1136,6 +1137,344 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_vcs(Post2Args a) {
     a.V[i * S + q] = un;
 }
 
+// ------------------------------------------------------------------------------------------------ triangle separation
+// Device form of separate_ref / separate1_ref (alm_vcs.py:965-984; _centre 554-565, _centre1 608-632, sep_hist
+// 567-580, sep_collect 582-597, sep1_hist/sep1_collect 634-666). Bit-exact with the reference on the same V: every dot
+// product is the FP64 sum over q in order (no contraction), the histogram bins, b0, vmin and the per-chunk caps use the
+// reference's FP64 expressions, and candidates are written in the reference's generation order (chunk, centre, arm
+// pair). Tie rule at the top-maxh / top-maxk cuts: descending value, ties in the prior order (stable).
+
+#pragma clang fp contract(off)
+template <typename TV, int R>
+__device__ inline double dot_seq(const uint4 *V, int64_t a, int64_t b) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < S; q++) {
+        TC xa[CPL], xb[CPL];
+        VT<TV>::unpack(V[a * S + q], xa);
+        VT<TV>::unpack(V[b * S + q], xb);
+#pragma unroll
+        for (int c = 0; c < CPL; c++) s = s + (double)xa[c] * (double)xb[c];
+    }
+    return s;
+}
+#pragma clang fp contract(on)
+
+struct SepParams {
+    const uint4 *V; int64_t n; const int64_t *rowptr; const int32_t *col;
+    double thresh, vtol; int64_t maxh, maxk, hub2, cap2; int nchunk, nb; int64_t cs;
+    // arms (edge space compacted): arm range of centre j = [abeg[j], abeg[j] + amh[j]), amh = min(m_j, maxh)
+    const int64_t *apos;       // exclusive scan of arm flags over edges (n_edges + 1)
+    const int32_t *arm_id; const double *arm_x;
+    // one-edge arms: K lists (maxk per centre)
+    const int32_t *kid; const double *kx; const int32_t *kcnt;
+    // outputs / state
+    unsigned long long *hist;  // nchunk x nb
+    const double *vminp;       // device scalar
+    int32_t *ccount;           // per centre candidate count (>= vmin)
+    const int64_t *cpos;       // exclusive scan of ccount over centres
+    const int64_t *chunk_cnt;  // histogram tail count per chunk (cap of the chunk)
+    const int64_t *chunk_off;  // output offset per chunk
+    int64_t *out; double *val; // slot arrays (sum of chunk_cnt)
+};
+
+__device__ inline int sep_bin(double vm, double vtol, int nb) {
+    const int b = (int)((vm - vtol) / (2.0 - vtol) * nb);
+    return b < nb - 1 ? b : nb - 1;
+}
+
+// arm values per edge: x_e = V_col[e] . V_j (reference order: s += V[i, q] * V[j, q]), flag = x_e > thresh
+template <typename TV, int R>
+__global__ void lb_k_sep_edges(SepParams p, double *xe, int32_t *flag) {
+    const int64_t nnz = p.rowptr[p.n];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = p.n;            // row of edge e: last j with rowptr[j] <= e
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (p.rowptr[mid] <= e) lo = mid; else hi = mid;
+        }
+        const double x = dot_seq<TV, R>(p.V, (int64_t)p.col[e], lo);
+        xe[e] = x;
+        flag[e] = x > p.thresh ? 1 : 0;
+    }
+}
+
+__global__ void lb_k_sep_compact(const int64_t *rowptr, int64_t n, const int32_t *col, const double *xe,
+                                 const int64_t *flag, const int64_t *apos, int32_t *arm_id, double *arm_x) {
+    const int64_t nnz = rowptr[n];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x)
+        if (flag[e]) { arm_id[apos[e]] = col[e]; arm_x[apos[e]] = xe[e]; }
+}
+
+// after the segmented sort of hub centres (value-descending, stable): copy their first maxh arms back in place
+__global__ void lb_k_sep_cutcopy(const int64_t *seg_beg, int nseg, const int32_t *sid, const double *sx,
+                                 int64_t maxh, int32_t *arm_id, double *arm_x) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)nseg * maxh; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = t / maxh, o = t % maxh, b = seg_beg[s];
+        arm_id[b + o] = sid[b + o];
+        arm_x[b + o] = sx[b + o];
+    }
+}
+
+__device__ inline int64_t sep_mh(const SepParams &p, int64_t j, int64_t &abeg) {
+    abeg = p.apos[p.rowptr[j]];
+    const int64_t m = p.apos[p.rowptr[j + 1]] - abeg;
+    return m < p.maxh ? m : p.maxh;
+}
+
+// one wavefront per centre, arm pairs in generation order; mode 0: histogram of vm > vtol; 1: count vm >= vmin;
+// 2: write the candidates (slot = chunk offset + rank inside the chunk, if below the chunk's cap)
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_sep_wedges(SepParams p, int mode) {
+    const int64_t j = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
+    if (j >= p.n) return;
+    const int lane = threadIdx.x & 63;
+    if (p.rowptr[j + 1] - p.rowptr[j] < 2) { if (mode == 1 && lane == 0) p.ccount[j] = 0; return; }
+    int64_t ab;
+    const int64_t mh = sep_mh(p, j, ab);
+    const int64_t ch = j / p.cs;
+    const double vmin = mode ? *p.vminp : 0.0;
+    int64_t cnt = 0;
+    int64_t rank0 = 0, cap_end = 0;
+    if (mode == 2) { rank0 = p.cpos[j] - p.cpos[ch * p.cs]; cap_end = p.chunk_cnt[ch]; }
+    for (int64_t x = 0; x + 1 < mh; x++) {
+        const int64_t hx = p.arm_id[ab + x];
+        const double xx = p.arm_x[ab + x];
+        for (int64_t y0 = x + 1; y0 < mh; y0 += 64) {
+            const int64_t y = y0 + lane;
+            double vm = -1.0;
+            int64_t hy = 0;
+            if (y < mh) {
+                hy = p.arm_id[ab + y];
+                const double s = dot_seq<TV, R>(p.V, hx, hy);
+                vm = xx + p.arm_x[ab + y];
+                vm = vm - s;
+                vm = vm - 1.0;
+            }
+            if (mode == 0) {
+                if (y < mh && vm > p.vtol) atomicAdd(p.hist + ch * p.nb + sep_bin(vm, p.vtol, p.nb), 1ull);
+            } else {
+                const bool ok = y < mh && vm >= vmin;
+                const unsigned long long bal = __ballot(ok);
+                if (mode == 2 && ok) {
+                    const int64_t r = rank0 + cnt + __popcll(bal & ((1ull << lane) - 1ull));
+                    if (r < cap_end) {
+                        const int64_t o = p.chunk_off[ch] + r;
+                        p.out[3 * o] = hx < hy ? hx : hy; p.out[3 * o + 1] = j; p.out[3 * o + 2] = hx < hy ? hy : hx;
+                        p.val[o] = vm;
+                    }
+                }
+                cnt += __popcll(bal);
+            }
+        }
+    }
+    if (mode == 1 && lane == 0) p.ccount[j] = (int32_t)(cnt < 0x7fffffff ? cnt : 0x7fffffff);
+}
+
+// bitonic sort of nk (<= cap, power-of-two padded) (key, id) pairs in LDS by one wavefront; order: key
+// ascending then id ascending (desc = false), or key descending then position ascending (desc = true)
+__device__ inline void wave_bitonic(double *key, int32_t *id, int32_t *pos, int np2, bool desc) {
+    const int lane = threadIdx.x & 63;
+    for (int k = 2; k <= np2; k <<= 1) {
+        for (int s = k >> 1; s > 0; s >>= 1) {
+            for (int t = lane; t < np2; t += 64) {
+                const int u = t ^ s;
+                if (u > t) {
+                    const bool up = (t & k) == 0;
+                    bool gt;   // element t ordered after element u
+                    if (desc) gt = key[t] < key[u] || (key[t] == key[u] && pos[t] > pos[u]);
+                    else gt = id[t] > id[u];
+                    if (gt == up) {
+                        double tk = key[t]; key[t] = key[u]; key[u] = tk;
+                        int32_t ti = id[t]; id[t] = id[u]; id[u] = ti;
+                        int32_t tp = pos[t]; pos[t] = pos[u]; pos[u] = tp;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+#define LB_SEP1_CAP 2048   // 2-hop candidates per centre held in LDS (hub2 * cap2 must not exceed it)
+
+// one-edge-arm lists K of every centre (_centre1): one workgroup of one wavefront per centre
+template <typename TV, int R>
+__global__ __launch_bounds__(64) void lb_k_sep_k(SepParams p, int32_t *kid, double *kx, int32_t *kcnt) {
+    __shared__ double key[LB_SEP1_CAP];
+    __shared__ int32_t id[LB_SEP1_CAP];
+    __shared__ int32_t pos[LB_SEP1_CAP];
+    __shared__ int32_t nk_s;
+    const int64_t j = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (p.rowptr[j + 1] - p.rowptr[j] < 1) { if (lane == 0) kcnt[j] = 0; return; }
+    int64_t ab;
+    const int64_t mh = sep_mh(p, j, ab);
+    const int64_t nh = mh < p.hub2 ? mh : p.hub2;
+    if (lane == 0) nk_s = 0;
+    __syncthreads();
+    const int64_t jlo = p.rowptr[j], jhi = p.rowptr[j + 1];
+    for (int64_t a = 0; a < nh; a++) {
+        const int64_t i2 = p.arm_id[ab + a];
+        const int64_t lo = p.rowptr[i2];
+        int64_t hi = p.rowptr[i2 + 1];
+        if (hi > lo + p.cap2) hi = lo + p.cap2;
+        for (int64_t e = lo + lane; e < hi; e += 64) {
+            const int32_t c = p.col[e];
+            bool keep = c != j;
+            if (keep) {       // adjacency of c to j: binary search in j's sorted row
+                int64_t l2 = jlo, h2 = jhi;
+                while (l2 < h2) { const int64_t mid = (l2 + h2) >> 1; if (p.col[mid] < c) l2 = mid + 1; else h2 = mid; }
+                keep = !(l2 < jhi && p.col[l2] == c);
+            }
+            if (keep) {
+                const int t = atomicAdd(&nk_s, 1);
+                if (t < LB_SEP1_CAP) { id[t] = c; key[t] = 0.0; pos[t] = t; }
+            }
+        }
+    }
+    __syncthreads();
+    int nk = nk_s < LB_SEP1_CAP ? nk_s : LB_SEP1_CAP;
+    int np2 = 1;
+    while (np2 < nk) np2 <<= 1;
+    for (int t = nk + lane; t < np2; t += 64) { id[t] = 0x7fffffff; key[t] = 0.0; pos[t] = t; }
+    __syncthreads();
+    wave_bitonic(key, id, pos, np2, false);           // by id (duplicates carry identical values)
+    __syncthreads();
+    // unique ids with value > thresh, in id order
+    if (lane == 0) {
+        int m = 0;
+        for (int t = 0; t < nk; t++) {
+            if (t > 0 && id[t] == id[t - 1]) continue;
+            id[m] = id[t]; m++;
+        }
+        nk_s = m;
+    }
+    __syncthreads();
+    nk = nk_s;
+    for (int t = lane; t < nk; t += 64) key[t] = dot_seq<TV, R>(p.V, (int64_t)id[t], j);
+    __syncthreads();
+    if (lane == 0) {
+        int m = 0;
+        for (int t = 0; t < nk; t++)
+            if (key[t] > p.thresh) { key[m] = key[t]; id[m] = id[t]; pos[m] = m; m++; }
+        nk_s = m;
+    }
+    __syncthreads();
+    nk = nk_s;
+    if (nk > p.maxk) {
+        np2 = 1;
+        while (np2 < nk) np2 <<= 1;
+        for (int t = nk + lane; t < np2; t += 64) { key[t] = -INFINITY; id[t] = 0x7fffffff; pos[t] = t; }
+        __syncthreads();
+        wave_bitonic(key, id, pos, np2, true);        // value descending, ties by id order (stable)
+        __syncthreads();
+        nk = (int)p.maxk;
+    }
+    for (int t = lane; t < nk; t += 64) { kid[j * p.maxk + t] = id[t]; kx[j * p.maxk + t] = key[t]; }
+    if (lane == 0) kcnt[j] = nk;
+}
+
+// one wavefront per centre: pairs (arm h, 2-hop arm k) in generation order, modes as lb_k_sep_wedges
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_sep_one_edge(SepParams p, int mode) {
+    const int64_t j = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
+    if (j >= p.n) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t mk = p.kcnt[j];
+    if (p.rowptr[j + 1] - p.rowptr[j] < 1 || mk == 0) { if (mode == 1 && lane == 0) p.ccount[j] = 0; return; }
+    int64_t ab;
+    const int64_t mh = sep_mh(p, j, ab);
+    const int64_t ch = j / p.cs;
+    const double vmin = mode ? *p.vminp : 0.0;
+    int64_t cnt = 0, rank0 = 0, cap_end = 0;
+    if (mode == 2) { rank0 = p.cpos[j] - p.cpos[ch * p.cs]; cap_end = p.chunk_cnt[ch]; }
+    for (int64_t x = 0; x < mh; x++) {
+        const int64_t hx = p.arm_id[ab + x];
+        const double xx = p.arm_x[ab + x];
+        for (int64_t y0 = 0; y0 < mk; y0 += 64) {
+            const int64_t y = y0 + lane;
+            double vm = -1.0;
+            int64_t ky = 0;
+            bool valid = false;
+            if (y < mk) {
+                ky = p.kid[j * p.maxk + y];
+                if (ky != hx) {
+                    valid = true;
+                    const double s = dot_seq<TV, R>(p.V, hx, ky);
+                    vm = xx + p.kx[j * p.maxk + y];
+                    vm = vm - s;
+                    vm = vm - 1.0;
+                }
+            }
+            if (mode == 0) {
+                if (valid && vm > p.vtol) atomicAdd(p.hist + ch * p.nb + sep_bin(vm, p.vtol, p.nb), 1ull);
+            } else {
+                const bool ok = valid && vm >= vmin;
+                const unsigned long long bal = __ballot(ok);
+                if (mode == 2 && ok) {
+                    const int64_t r = rank0 + cnt + __popcll(bal & ((1ull << lane) - 1ull));
+                    if (r < cap_end) {
+                        const int64_t o = p.chunk_off[ch] + r;
+                        p.out[3 * o] = hx < ky ? hx : ky; p.out[3 * o + 1] = j; p.out[3 * o + 2] = hx < ky ? ky : hx;
+                        p.val[o] = vm;
+                    }
+                }
+                cnt += __popcll(bal);
+            }
+        }
+    }
+    if (mode == 1 && lane == 0) p.ccount[j] = (int32_t)(cnt < 0x7fffffff ? cnt : 0x7fffffff);
+}
+
+// threshold from the histogram (single workgroup): b0 = first bin whose tail count <= cap, vmin, per-chunk caps and
+// offsets; *total = sum of the chunk caps; *vminp = -1 if no bin qualifies
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_sep_thresh(const unsigned long long *hist, int nchunk, int nb,
+                                                          double vtol, int64_t cap, double *vminp, int64_t *chunk_cnt,
+                                                          int64_t *chunk_off, int64_t *total, int *b0p) {
+    __shared__ int b0s;
+    if (threadIdx.x == 0) {
+        unsigned long long cum = 0;
+        int b0 = nb;
+        // cum[b] is non-increasing in b: scan from the top bin down, remember the lowest bin with cum <= cap
+        for (int b = nb - 1; b >= 0; b--) {
+            unsigned long long t = 0;
+            for (int c = 0; c < nchunk; c++) t += hist[(int64_t)c * nb + b];
+            cum += t;
+            if (cum <= (unsigned long long)cap) b0 = b; else break;
+        }
+        b0s = b0;
+        *b0p = b0;
+        *vminp = b0 >= nb ? -1.0 : vtol + (2.0 - vtol) * b0 / nb;
+    }
+    __syncthreads();
+    const int b0 = b0s;
+    for (int c = threadIdx.x; c < nchunk; c += LB_BLOCK) {
+        unsigned long long t = 0;
+        if (b0 < nb)
+            for (int b = b0; b < nb; b++) t += hist[(int64_t)c * nb + b];
+        chunk_cnt[c] = (int64_t)t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t o = 0;
+        for (int c = 0; c < nchunk; c++) { chunk_off[c] = o; o += chunk_cnt[c]; }
+        *total = o;
+    }
+}
+
+
+__global__ void lb_k_widen(const int32_t *in, int64_t cnt, int64_t *out) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * blockDim.x) out[e] = in[e];
+}
+
+// arm range starts of every centre: out[j] = apos[rowptr[j]] (j = 0..n)
+__global__ void lb_k_gather_pos(const int64_t *rowptr, int64_t n, const int64_t *apos, int64_t *out) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j <= n; j += (int64_t)gridDim.x * blockDim.x) out[j] = apos[rowptr[j]];
+}
+
 // ================================================================================================== host side
 namespace {
 
@@ -1528,9 +1867,146 @@ struct Launch {
         HIPCHK(c, hipGetLastError());
         return 0;
     }
+
+    // triangle separation of one family (fam 0 wedges, fam 1 one-edge arms) on the resident V; fills out (m x 3)
+    // and val (m), returns the count through *m_out (m <= cap)
+    static int separate(lb_ctx *c, int fam, double thresh, int64_t maxh, int64_t maxk, int64_t hub2, int64_t cap2,
+                        double vtol, int64_t cap, int nchunk, int nb, std::vector<int64_t> &out, std::vector<double> &val,
+                        double *vmin_out) {
+        const int64_t n = c->n, nnz = c->nnz;
+        int rc;
+        DBuf xe, flag, apos, arm_id, arm_x, mcnt, sx, sid, segb, sege, tmp, kid, kx, kcnt, hist, ccount, cpos,
+            chunk_cnt, chunk_off, dout, dval, scal;
+        auto cleanup = [&]() {
+            DBuf *bs[] = {&xe, &flag, &apos, &arm_id, &arm_x, &mcnt, &sx, &sid, &segb, &sege, &tmp, &kid, &kx, &kcnt,
+                          &hist, &ccount, &cpos, &chunk_cnt, &chunk_off, &dout, &dval, &scal};
+            for (DBuf *b : bs) dfree(*b);
+        };
+#define SEPCHK(x) do { if ((rc = (x))) { cleanup(); return rc; } } while (0)
+#define SEPHIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { cleanup(); return fail(c, -3, std::string(#x) + ": " + hipGetErrorString(e_)); } } while (0)
+        SEPCHK(dalloc(c, xe, sizeof(double) * nnz));
+        SEPCHK(dalloc(c, flag, sizeof(int64_t) * (nnz + 1)));
+        SEPCHK(dalloc(c, apos, sizeof(int64_t) * (nnz + 1)));
+        SEPCHK(dalloc(c, scal, 64));
+        SepParams p{};
+        p.V = (const uint4 *)c->V.p; p.n = n; p.rowptr = (const int64_t *)c->rowptr.p; p.col = (const int32_t *)c->col.p;
+        p.thresh = thresh; p.vtol = vtol; p.maxh = maxh; p.maxk = maxk; p.hub2 = hub2; p.cap2 = cap2;
+        p.nchunk = nchunk; p.nb = nb; p.cs = (n + nchunk - 1) / nchunk;
+        // per-edge arm values (int64 flags: the scan output is the arm position)
+        SEPHIP(hipMemsetAsync(flag.p, 0, sizeof(int64_t) * (nnz + 1), c->stream));
+        {
+            // the edge kernel writes int32 flags into the first half; widen them below
+            hipLaunchKernelGGL((lb_k_sep_edges<TV, R>), dim3(2048), dim3(256), 0, c->stream, p, (double *)xe.p, (int32_t *)apos.p);
+            hipLaunchKernelGGL(lb_k_widen, dim3(2048), dim3(256), 0, c->stream, (const int32_t *)apos.p, nnz, (int64_t *)flag.p);
+        }
+        size_t tb = 0;
+        SEPHIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (const int64_t *)flag.p, (int64_t *)apos.p, (int)(nnz + 1), c->stream));
+        SEPCHK(dalloc(c, tmp, tb));
+        SEPHIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, (const int64_t *)flag.p, (int64_t *)apos.p, (int)(nnz + 1), c->stream));
+        int64_t narms = 0;
+        SEPHIP(hipMemcpyAsync(&narms, (const int64_t *)apos.p + nnz, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+        SEPHIP(hipStreamSynchronize(c->stream));
+        SEPCHK(dalloc(c, arm_id, sizeof(int32_t) * std::max<int64_t>(narms, 1)));
+        SEPCHK(dalloc(c, arm_x, sizeof(double) * std::max<int64_t>(narms, 1)));
+        hipLaunchKernelGGL(lb_k_sep_compact, dim3(2048), dim3(256), 0, c->stream, (const int64_t *)c->rowptr.p, n,
+                           (const int32_t *)c->col.p, (const double *)xe.p, (const int64_t *)flag.p, (const int64_t *)apos.p,
+                           (int32_t *)arm_id.p, (double *)arm_x.p);
+        // hub centres with more than maxh arms: stable value-descending sort of their arm segments, keep maxh
+        SEPCHK(dalloc(c, mcnt, sizeof(int64_t) * (n + 1)));
+        hipLaunchKernelGGL(lb_k_gather_pos, dim3(1024), dim3(256), 0, c->stream, (const int64_t *)c->rowptr.p, n,
+                           (const int64_t *)apos.p, (int64_t *)mcnt.p);
+        std::vector<int64_t> ab(n + 1);
+        SEPHIP(hipMemcpyAsync(ab.data(), mcnt.p, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
+        SEPHIP(hipStreamSynchronize(c->stream));
+        std::vector<int64_t> hb, he;
+        for (int64_t j = 0; j < n; j++)
+            if (ab[j + 1] - ab[j] > maxh) { hb.push_back(ab[j]); he.push_back(ab[j + 1]); }
+        if (!hb.empty()) {
+            SEPCHK(upload(c, segb, hb.data(), sizeof(int64_t) * hb.size()));
+            SEPCHK(upload(c, sege, he.data(), sizeof(int64_t) * he.size()));
+            SEPCHK(dalloc(c, sx, sizeof(double) * narms));
+            SEPCHK(dalloc(c, sid, sizeof(int32_t) * narms));
+            size_t tb2 = 0;
+            SEPHIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(nullptr, tb2, (const double *)arm_x.p, (double *)sx.p,
+                   (const int32_t *)arm_id.p, (int32_t *)sid.p, (int)narms, (int)hb.size(), (const int64_t *)segb.p,
+                   (const int64_t *)sege.p, 0, 64, c->stream));
+            SEPCHK(dalloc(c, tmp, tb2));
+            SEPHIP(hipcub::DeviceSegmentedRadixSort::SortPairsDescending(tmp.p, tb2, (const double *)arm_x.p, (double *)sx.p,
+                   (const int32_t *)arm_id.p, (int32_t *)sid.p, (int)narms, (int)hb.size(), (const int64_t *)segb.p,
+                   (const int64_t *)sege.p, 0, 64, c->stream));
+            hipLaunchKernelGGL(lb_k_sep_cutcopy, dim3(256), dim3(256), 0, c->stream, (const int64_t *)segb.p, (int)hb.size(),
+                               (const int32_t *)sid.p, (const double *)sx.p, maxh, (int32_t *)arm_id.p, (double *)arm_x.p);
+        }
+        p.apos = (const int64_t *)apos.p; p.arm_id = (const int32_t *)arm_id.p; p.arm_x = (const double *)arm_x.p;
+        if (fam == 1) {
+            SEPCHK(dalloc(c, kid, sizeof(int32_t) * (size_t)n * maxk));
+            SEPCHK(dalloc(c, kx, sizeof(double) * (size_t)n * maxk));
+            SEPCHK(dalloc(c, kcnt, sizeof(int32_t) * n));
+            hipLaunchKernelGGL((lb_k_sep_k<TV, R>), dim3((unsigned)n), dim3(64), 0, c->stream, p, (int32_t *)kid.p,
+                               (double *)kx.p, (int32_t *)kcnt.p);
+            p.kid = (const int32_t *)kid.p; p.kx = (const double *)kx.p; p.kcnt = (const int32_t *)kcnt.p;
+        }
+        SEPCHK(dalloc(c, hist, sizeof(unsigned long long) * (size_t)nchunk * nb));
+        SEPHIP(hipMemsetAsync(hist.p, 0, sizeof(unsigned long long) * (size_t)nchunk * nb, c->stream));
+        p.hist = (unsigned long long *)hist.p;
+        const unsigned g = (unsigned)((n + 3) / 4);
+        if (fam == 0) hipLaunchKernelGGL((lb_k_sep_wedges<TV, R>), dim3(g), dim3(LB_BLOCK), 0, c->stream, p, 0);
+        else hipLaunchKernelGGL((lb_k_sep_one_edge<TV, R>), dim3(g), dim3(LB_BLOCK), 0, c->stream, p, 0);
+        SEPCHK(dalloc(c, chunk_cnt, sizeof(int64_t) * nchunk));
+        SEPCHK(dalloc(c, chunk_off, sizeof(int64_t) * nchunk));
+        double *dvmin = (double *)scal.p;
+        int64_t *dtotal = (int64_t *)((char *)scal.p + 8);
+        int *db0 = (int *)((char *)scal.p + 16);
+        hipLaunchKernelGGL(lb_k_sep_thresh, dim3(1), dim3(LB_BLOCK), 0, c->stream, (const unsigned long long *)hist.p,
+                           nchunk, nb, vtol, cap, dvmin, (int64_t *)chunk_cnt.p, (int64_t *)chunk_off.p, dtotal, db0);
+        double vmin = -1.0;
+        int64_t total = 0;
+        SEPHIP(hipMemcpyAsync(&vmin, dvmin, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        SEPHIP(hipMemcpyAsync(&total, dtotal, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+        SEPHIP(hipStreamSynchronize(c->stream));
+        *vmin_out = vmin;
+        out.clear(); val.clear();
+        if (vmin < 0.0 || total == 0) { cleanup(); return 0; }
+        p.vminp = dvmin; p.chunk_cnt = (const int64_t *)chunk_cnt.p; p.chunk_off = (const int64_t *)chunk_off.p;
+        SEPCHK(dalloc(c, ccount, sizeof(int64_t) * (n + 1)));
+        SEPCHK(dalloc(c, cpos, sizeof(int64_t) * (n + 1)));
+        SEPHIP(hipMemsetAsync(ccount.p, 0, sizeof(int64_t) * (n + 1), c->stream));
+        SEPCHK(dalloc(c, mcnt, sizeof(int64_t) * (n + 1)));
+        p.ccount = (int32_t *)mcnt.p;
+        SEPHIP(hipMemsetAsync(mcnt.p, 0, sizeof(int64_t) * (n + 1), c->stream));
+        if (fam == 0) hipLaunchKernelGGL((lb_k_sep_wedges<TV, R>), dim3(g), dim3(LB_BLOCK), 0, c->stream, p, 1);
+        else hipLaunchKernelGGL((lb_k_sep_one_edge<TV, R>), dim3(g), dim3(LB_BLOCK), 0, c->stream, p, 1);
+        hipLaunchKernelGGL(lb_k_widen, dim3(1024), dim3(256), 0, c->stream, (const int32_t *)mcnt.p, n, (int64_t *)ccount.p);
+        size_t tb3 = 0;
+        SEPHIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, (const int64_t *)ccount.p, (int64_t *)cpos.p, (int)(n + 1), c->stream));
+        SEPCHK(dalloc(c, tmp, tb3));
+        SEPHIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb3, (const int64_t *)ccount.p, (int64_t *)cpos.p, (int)(n + 1), c->stream));
+        p.cpos = (const int64_t *)cpos.p;
+        SEPCHK(dalloc(c, dout, sizeof(int64_t) * 3 * total));
+        SEPCHK(dalloc(c, dval, sizeof(double) * total));
+        SEPHIP(hipMemsetAsync(dout.p, 0, sizeof(int64_t) * 3 * total, c->stream));
+        SEPHIP(hipMemsetAsync(dval.p, 0, sizeof(double) * total, c->stream));
+        p.out = (int64_t *)dout.p; p.val = (double *)dval.p;
+        if (fam == 0) hipLaunchKernelGGL((lb_k_sep_wedges<TV, R>), dim3(g), dim3(LB_BLOCK), 0, c->stream, p, 2);
+        else hipLaunchKernelGGL((lb_k_sep_one_edge<TV, R>), dim3(g), dim3(LB_BLOCK), 0, c->stream, p, 2);
+        SEPHIP(hipGetLastError());
+        std::vector<int64_t> o3(3 * total);
+        std::vector<double> v1(total);
+        SEPHIP(hipMemcpyAsync(o3.data(), dout.p, sizeof(int64_t) * 3 * total, hipMemcpyDeviceToHost, c->stream));
+        SEPHIP(hipMemcpyAsync(v1.data(), dval.p, sizeof(double) * total, hipMemcpyDeviceToHost, c->stream));
+        SEPHIP(hipStreamSynchronize(c->stream));
+        for (int64_t t = 0; t < total; t++)
+            if (v1[t] > 0.0) { out.push_back(o3[3 * t]); out.push_back(o3[3 * t + 1]); out.push_back(o3[3 * t + 2]); val.push_back(v1[t]); }
+        cleanup();
+#undef SEPCHK
+#undef SEPHIP
+        return 0;
+    }
 };
 
 typedef int (*eval_fn)(lb_ctx *, double, double, hipEvent_t *);
+typedef int (*sep_fn)(lb_ctx *, int, double, int64_t, int64_t, int64_t, int64_t, double, int64_t, int, int,
+                      std::vector<int64_t> &, std::vector<double> &, double *);
 typedef int (*sweep_fn)(lb_ctx *, double, double, double, double, int, const std::vector<int64_t> &,
                         const std::vector<int64_t> &);
 
@@ -1551,6 +2027,18 @@ sweep_fn pick_sweep(int vdtype, int r) {
     if (r == 8) return &Launch<TV, 8>::sweep;           \
     if (r == 16) return &Launch<TV, 16>::sweep;         \
     if (r == 32) return &Launch<TV, 32>::sweep;
+    if (vdtype == LB_F32) { LB_PICK(float) }
+    if (vdtype == LB_BF16) { LB_PICK(bf16raw) }
+    if (vdtype == LB_F64) { LB_PICK(double) }
+#undef LB_PICK
+    return nullptr;
+}
+
+sep_fn pick_sep(int vdtype, int r) {
+#define LB_PICK(TV)                                     \
+    if (r == 8) return &Launch<TV, 8>::separate;        \
+    if (r == 16) return &Launch<TV, 16>::separate;      \
+    if (r == 32) return &Launch<TV, 32>::separate;
     if (vdtype == LB_F32) { LB_PICK(float) }
     if (vdtype == LB_BF16) { LB_PICK(bf16raw) }
     if (vdtype == LB_F64) { LB_PICK(double) }
@@ -2058,6 +2546,30 @@ int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     c->have_eval = false;
+    return 0;
+}
+
+int lb_separate_triangles(lb_ctx *c, int fam, double thresh, int64_t maxh, int64_t maxk, int64_t hub2, int64_t cap2,
+                          double vtol, int64_t cap, int nchunk, int nb, int64_t *TI_out, double *val_out,
+                          int64_t *count, double *vmin) {
+    if (!c || !c->n) return fail(c, -1, "lb_separate_triangles: no graph");
+    if ((fam != 0 && fam != 1) || maxh < 1 || maxk < 1 || hub2 < 0 || cap2 < 0 || cap < 0 || nchunk < 1 || nb < 1 ||
+        !(vtol > 0.0) || !(vtol < 2.0) || !TI_out || !val_out || !count || !vmin)
+        return fail(c, -1, "lb_separate_triangles: bad arguments");
+    if (fam == 1 && hub2 * cap2 > LB_SEP1_CAP) return fail(c, -1, "lb_separate_triangles: hub2 * cap2 exceeds 2048");
+    if (fam == 1 && maxk > LB_SEP1_CAP) return fail(c, -1, "lb_separate_triangles: maxk exceeds 2048");
+    HIPCHK(c, hipSetDevice(c->device));
+    sep_fn f = pick_sep(c->vdtype, c->r);
+    if (!f) return fail(c, -1, "lb_separate_triangles: unsupported rank / dtype");
+    std::vector<int64_t> out;
+    std::vector<double> val;
+    int rc = f(c, fam, thresh, maxh, maxk, hub2, cap2, vtol, cap, nchunk, nb, out, val, vmin);
+    if (rc) return rc;
+    const int64_t m = (int64_t)val.size();
+    if (m > cap) return fail(c, 1, "lb_separate_triangles: more candidates than cap");
+    std::memcpy(TI_out, out.data(), sizeof(int64_t) * 3 * m);
+    std::memcpy(val_out, val.data(), sizeof(double) * m);
+    *count = m;
     return 0;
 }
 

@@ -3,7 +3,7 @@
 Tolerances (stated by the precision contract of the search): FP32 and BF16-stored V: f within 1e-5 relative,
 ||G - G64||_F / ||G64||_F <= 1e-5, with the oracle evaluated on the stored (for BF16: dequantised) V; FP64 path:
 1e-11 relative. Effective multipliers: pp (FP64 on the device) 1e-9 relative; pt within 1e-5 * rt absolute (tau in
-FP32); pv 1e-6 relative.
+FP32); pv 1e-6 relative to |pv| + |sig| (FP32-stored multipliers).
 """
 import numpy as np
 import pytest
@@ -46,7 +46,11 @@ def _check(eng, inst, rho, dtype, label):
         rt = rho
         assert np.abs(pt - pto).max() <= (1e-5 * max(rt, 1e-30) + 1e-6 * np.abs(pto).max()) if dtype != "f64" else 1e-12 * max(1.0, np.abs(pto).max())
     if len(pvo):
-        assert np.allclose(pv, pvo, rtol=1e-6 if dtype != "f64" else 1e-12, atol=1e-30)
+        # FP32-stored multipliers: pv = sig - rho r g loses relative accuracy when it nearly cancels, so the bound is
+        # relative to the inputs |sig| (and |pv|)
+        sig = np.concatenate([v.sig for v in inst.vcs])
+        rel = 1e-6 if dtype != "f64" else 1e-12
+        assert np.all(np.abs(pv - pvo) <= rel * (np.abs(pvo) + np.abs(sig)) + 1e-300)
     return parts, G
 
 
