@@ -67,6 +67,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-CPU oracle baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the BF16 / FP64 / plain-FP64 comparison runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--tuning", default="", help="heavy_edges,heavy_tris,seg_len,row_grid")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}). "
@@ -83,6 +84,9 @@ def main():
     rho = a.rho
 
     eng = Engine(r=r, dtype="f32")
+    tune = [int(x) for x in a.tuning.split(",")] if a.tuning else None
+    if tune:
+        eng.set_tuning(*tune)
     eng.set_instance(inst)
     eng.set_V(V)
     eng.eval(rho)
@@ -128,27 +132,21 @@ def main():
                          "hbm_frac": round(b2 / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             log(f"{dt}: {s2 * 1e3:.4f} ms/eval")
             e2.close()
+        # the plain FP64 GPU form runs in a process of its own (torch brings its own HIP runtime)
+        import subprocess
         try:
-            import torch
-            from lagbound_amd.baseline_fp64 import PlainFP64
-            pl = PlainFP64(inst, device="cuda")
-            Vt = torch.as_tensor(V, dtype=torch.float64, device="cuda")
-            for _ in range(3):
-                pl.eval(Vt, rho)
-            torch.cuda.synchronize()
-            k = 20
-            t1 = time.perf_counter()
-            for _ in range(k):
-                f, G = pl.eval(Vt, rho)
-            torch.cuda.synchronize()
-            s3 = (time.perf_counter() - t1) / k
-            extra["plain_fp64_gpu"] = {"evals_per_s": round(1.0 / s3, 2), "ms_per_eval": round(s3 * 1e3, 4),
-                                       "form": "torch FP64 CSR SpMM + gathers/index_add"}
-            out["speedup_vs_plain_fp64_gpu"] = round(value * s3, 2)
-            log(f"plain fp64 gpu: {s3 * 1e3:.3f} ms/eval")
-            del pl, Vt, f, G
+            res = subprocess.run([sys.executable, "-m", "lagbound_amd._fp64_baseline_main", "--n", str(a.n),
+                                  "--rho", str(rho), "--steps", "10"], capture_output=True, text=True, timeout=600,
+                                 cwd=ROOT)
+            line = [l for l in res.stdout.splitlines() if l.startswith("{")]
+            if res.returncode != 0 or not line:
+                raise RuntimeError(res.stderr[-300:])
+            pf = json.loads(line[-1])
+            extra["plain_fp64_gpu"] = pf
+            out["speedup_vs_plain_fp64_gpu"] = round(value * pf["ms_per_eval"] * 1e-3, 2)
+            log(f"plain fp64 gpu: {pf['ms_per_eval']:.3f} ms/eval")
         except Exception as exc:  # the comparison leg is reported, never silently replaced
-            extra["plain_fp64_gpu"] = {"error": repr(exc)[:200]}
+            extra["plain_fp64_gpu"] = {"error": repr(exc)[:300]}
         out["variants"] = extra
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -15,7 +15,7 @@ LB_NKERNELS = 8
 # every symbol include/lbhip.h declares
 EXPORTS = ["lb_create", "lb_destroy", "lb_last_error", "lb_set_graph", "lb_set_levels", "lb_set_level_sig",
            "lb_set_vcs", "lb_set_vcs_sig", "lb_set_triangles", "lb_set_lam", "lb_set_V", "lb_get_V",
-           "lb_eval_fgrad", "lb_get_G", "lb_get_multipliers", "lb_phr_update", "lb_sweeps", "lb_sync", "lb_time_eval",
+           "lb_eval_fgrad", "lb_get_G", "lb_get_multipliers", "lb_phr_update", "lb_sweeps", "lb_separate_triangles", "lb_sync", "lb_time_eval",
            "lb_kernel_name", "lb_set_tuning", "lb_version"]
 
 _i64p = C.POINTER(C.c_int64)
@@ -61,7 +61,9 @@ def load():
         "lb_sync": ([ctx], C.c_int),
         "lb_time_eval": ([ctx, C.c_double, C.c_double, C.c_int, _f64p, _f64p], C.c_int),
         "lb_kernel_name": ([C.c_int], C.c_char_p),
-        "lb_set_tuning": ([ctx, C.c_int, C.c_int, C.c_int], C.c_int),
+        "lb_set_tuning": ([ctx, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+        "lb_separate_triangles": ([ctx, C.c_int, C.c_double, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_double,
+                                   C.c_int64, C.c_int, C.c_int, _i64p, _f64p, _i64p, _f64p], C.c_int),
         "lb_version": ([], C.c_char_p),
     }
     for name, (args, res) in sig.items():

@@ -162,6 +162,7 @@ struct RowArgs {
     const void *segout;
     void *G;
     double *part;              // per block: quad, gv, tri tot, tri gmin
+    int stage, sumPpair, sumPv;  // stage [z | M | pdiag | Y_vcs | Z] in LDS
 };
 
 struct FinArgs {
@@ -224,13 +225,25 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_colsum(ColsumArgs a) {
     for (int e = threadIdx.x; e < width; e += LB_BLOCK) out[R + e] = shY[e];
 }
 
-// fixed-order sum of per-block partials: out[e] = sum_b part[b * width + e]
-__global__ void lb_k_reduce(const double *part, int nblk, int width, double *out) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= width) return;
-    double s = 0.0;
-    for (int b = 0; b < nblk; b++) s += part[(int64_t)b * width + e];
-    out[e] = s;
+// fixed-order sum of per-block partials: out[e] = sum_b part[b * stride + e]. 64 columns per workgroup (one per
+// lane, coalesced), the 16 wavefronts split the blocks, then a fixed-order combine in LDS.
+__global__ __launch_bounds__(1024) void lb_k_reduce(const double *part, int nblk, int width, int stride, double *out) {
+    __shared__ double sh[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int e = blockIdx.x * 64 + lane;
+    double s0 = 0.0, s1 = 0.0;
+    if (e < width) {
+        int b = w;
+        for (; b + 16 < nblk; b += 32) { s0 += part[(int64_t)b * stride + e]; s1 += part[(int64_t)(b + 16) * stride + e]; }
+        for (; b < nblk; b += 16) s0 += part[(int64_t)b * stride + e];
+    }
+    sh[w][lane] = s0 + s1;
+    __syncthreads();
+    if (w == 0 && e < width) {
+        double t = 0.0;
+        for (int k = 0; k < 16; k++) t += sh[k][lane];
+        out[e] = t;
+    }
 }
 
 __global__ void lb_k_fill(double *p, int64_t cnt, double v) {
@@ -279,16 +292,25 @@ __global__ __launch_bounds__(64) void lb_k_levels(LevelArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------ lb_k_vcs_z
+// LDS path (use_lds): the VCS cluster sums Y are staged in LDS (FP64) and Z is accumulated in one LDS copy per
+// wavefront in the row arithmetic type (FP32, FP64 for LB_F64), combined in FP64 at the end of the workgroup.
 template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
     typedef typename VT<TV>::TC TC;
-    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
-    extern __shared__ double shZ[];
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S, NW = LB_BLOCK / 64;
+    extern __shared__ double shY[];
     __shared__ double red[LB_BLOCK / 64];
-    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    const int slot = threadIdx.x / S, q = threadIdx.x % S, wv = threadIdx.x >> 6;
     const int width = a.use_lds ? a.sumPv * R : 0;
-    for (int e = threadIdx.x; e < width; e += LB_BLOCK) shZ[e] = 0.0;
-    __syncthreads();
+    TC *shZ = (TC *)(shY + width);
+    const double *Yb = a.Y + (int64_t)a.Yoff[0] * R;   // the VCS cluster sums (all VCS levels, consecutive)
+    if (a.use_lds) {
+        for (int e = threadIdx.x; e < width; e += LB_BLOCK) shY[e] = Yb[e];
+        for (int e = threadIdx.x; e < NW * width; e += LB_BLOCK) shZ[e] = (TC)0;
+        __syncthreads();
+        Yb = shY;
+    }
+    TC *myZ = shZ + wv * width;
     const TC *sig = (const TC *)a.sig;
     TC *pv = (TC *)a.pv;
     double tot = 0.0, viol = 0.0;
@@ -301,7 +323,7 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
             const int64_t e0 = a.ptr[m][i], e1 = a.ptr[m][i + 1];
             for (int64_t e = e0; e < e1; e++) {
                 const int Rc = a.col[e];
-                const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+                const double *YR = Yb + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
                 double s = 0.0;
 #pragma unroll
                 for (int c = 0; c < CPL; c++) s += (double)x[c] * YR[c];
@@ -321,10 +343,15 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
                 }
                 const TC pc = (TC)p;  // the stored multiplier is what G uses
                 if (pc != (TC)0) {
-                    double *dst = a.use_lds ? (shZ + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL)
-                                            : (a.Zglob + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL);
+                    if (a.use_lds) {
+                        TC *dst = myZ + (a.Zoff[m] + Rc) * R + q * CPL;
 #pragma unroll
-                    for (int c = 0; c < CPL; c++) atomicAdd(dst + c, (double)pc * (di * (double)x[c]));
+                        for (int c = 0; c < CPL; c++) atomicAdd(dst + c, (TC)((double)pc * (di * (double)x[c])));
+                    } else {
+                        double *dst = a.Zglob + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+#pragma unroll
+                        for (int c = 0; c < CPL; c++) atomicAdd(dst + c, (double)pc * (di * (double)x[c]));
+                    }
                 }
             }
         }
@@ -335,23 +362,24 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
     const int pw = 2 + width;
     double *out = a.part + (int64_t)blockIdx.x * pw;
     if (threadIdx.x == 0) { out[0] = tot; out[1] = viol; }
-    for (int e = threadIdx.x; e < width; e += LB_BLOCK) out[2 + e] = shZ[e];
+    for (int e = threadIdx.x; e < width; e += LB_BLOCK) {
+        double z = 0.0;
+        for (int k = 0; k < NW; k++) z += (double)shZ[k * width + e];
+        out[2 + e] = z;
+    }
 }
 
 // ------------------------------------------------------------------------------------------------ triangle record
-// One incidence record of vertex i (x = its V chunk): tau of the triangle in the canonical (a, b, c) order (the
-// same arithmetic for all three owners, as batch_step6 does, alm_vcs.py:297-312), pt, and the row contribution of
-// T(pt) V (tri_apply, alm_vcs.py:106-120). Returns pt and tau.
+// One incidence record of vertex i (x = its V chunk, xu / xw = the chunks of the two other vertices): tau of the
+// triangle in the canonical (a, b, c) order (the same arithmetic for all three owners, as batch_step6 does,
+// alm_vcs.py:297-312), pt, and the row contribution of T(pt) V (tri_apply, alm_vcs.py:106-120).
 template <typename TV, int R>
-__device__ inline void tri_record(const uint4 *V, const TriRec<typename VT<TV>::TC> &rc, const typename VT<TV>::TC *x,
-                                  int q, double rho, double rt, typename VT<TV>::TC *g, typename VT<TV>::TC &pt_out,
-                                  typename VT<TV>::TC &tau_out) {
+__device__ inline void tri_record_x(int code, typename VT<TV>::TC lam, const typename VT<TV>::TC *x,
+                                    const typename VT<TV>::TC *xu, const typename VT<TV>::TC *xw, double rho, double rt,
+                                    typename VT<TV>::TC *g, typename VT<TV>::TC &pt_out, typename VT<TV>::TC &tau_out) {
     typedef typename VT<TV>::TC TC;
     constexpr int CPL = VT<TV>::CPL, S = R / CPL;
-    TC xu[CPL], xw[CPL];
-    VT<TV>::unpack(V[(int64_t)rc.u * S + q], xu);
-    VT<TV>::unpack(V[(int64_t)rc.w * S + q], xw);
-    const int role = rc.code % 3;
+    const int role = code % 3;
     const TC *pa = role == 0 ? x : xu;
     const TC *pb = role == 0 ? xu : (role == 1 ? x : xw);
     const TC *pc = role == 2 ? x : xw;
@@ -359,9 +387,9 @@ __device__ inline void tri_record(const uint4 *V, const TriRec<typename VT<TV>::
 #pragma unroll
     for (int c = 0; c < CPL; c++) tl += pa[c] * pb[c] + pb[c] * pc[c] - pa[c] * pc[c];
     const TC tau = slot_sum<S>(tl);
-    TC p = rc.lam;
+    TC p = lam;
     if (rho > 0.0) {
-        p = rc.lam - (TC)rt * ((TC)1 - tau);
+        p = lam - (TC)rt * ((TC)1 - tau);
         if (p < (TC)0) p = (TC)0;
     }
     if (p != (TC)0) {
@@ -379,6 +407,18 @@ __device__ inline void tri_record(const uint4 *V, const TriRec<typename VT<TV>::
     }
     pt_out = p;
     tau_out = tau;
+}
+
+template <typename TV, int R>
+__device__ inline void tri_record(const uint4 *V, const TriRec<typename VT<TV>::TC> &rc, const typename VT<TV>::TC *x,
+                                  int q, double rho, double rt, typename VT<TV>::TC *g, typename VT<TV>::TC &pt_out,
+                                  typename VT<TV>::TC &tau_out) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL;
+    TC xu[CPL], xw[CPL];
+    VT<TV>::unpack(V[(int64_t)rc.u * S + q], xu);
+    VT<TV>::unpack(V[(int64_t)rc.w * S + q], xw);
+    tri_record_x<TV, R>(rc.code, rc.lam, x, xu, xw, rho, rt, g, pt_out, tau_out);
 }
 
 // ------------------------------------------------------------------------------------------------ lb_k_segs
@@ -451,15 +491,32 @@ __device__ inline int find_row(const int32_t *rows, int cnt, int64_t i) {
 }
 
 // ------------------------------------------------------------------------------------------------ lb_k_rows
+// Grid-stride over vertices, one slot of S lanes per vertex (16 bytes of the V row per lane). The small per-cluster
+// arrays are staged in LDS. Neighbour gathers are issued 8 at a time and triangle records 4 at a time (all row loads
+// of a group in flight before any is used), so each slot keeps several independent gathers outstanding.
 template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
     typedef typename VT<TV>::TC TC;
     constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    extern __shared__ double sm[];
     __shared__ double red[LB_BLOCK / 64];
+    const double *zs = a.z, *Ms = a.M, *pds = a.pdiag, *Ys = a.Y + (int64_t)a.sumPpair * R, *Zs = a.Z;
+    if (a.stage) {
+        double *p0 = sm, *p1 = p0 + R, *p2 = p1 + a.sumPpair * R, *p3 = p2 + a.sumPpair, *p4 = p3 + a.sumPv * R;
+        for (int e = threadIdx.x; e < R; e += LB_BLOCK) p0[e] = a.z[e];
+        for (int e = threadIdx.x; e < a.sumPpair * R; e += LB_BLOCK) p1[e] = a.M[e];
+        for (int e = threadIdx.x; e < a.sumPpair; e += LB_BLOCK) p2[e] = a.pdiag[e];
+        for (int e = threadIdx.x; e < a.sumPv * R; e += LB_BLOCK) { p3[e] = Ys[e]; p4[e] = a.Z[e]; }
+        __syncthreads();
+        zs = p0; Ms = p1; pds = p2; Ys = p3; Zs = p4;
+    }
     const int slot = threadIdx.x / S, q = threadIdx.x % S;
-    const int64_t i = (int64_t)blockIdx.x * NSL + slot;
+    const TC *wt = (const TC *)a.w;
+    const TC *pv = (const TC *)a.pv;
+    const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
+    TC *pt = (TC *)a.pt;
     double quad = 0.0, gv = 0.0, tot = 0.0, gmin = INFINITY;
-    if (i < a.n) {
+    for (int64_t i = (int64_t)blockIdx.x * NSL + slot; i < a.n; i += (int64_t)gridDim.x * NSL) {
         TC x[CPL], g[CPL];
         VT<TV>::unpack(a.V[i * S + q], x);
 #pragma unroll
@@ -473,52 +530,47 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
 #pragma unroll
                 for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
             }
-        } else if (a.w) {
-            const TC *wt = (const TC *)a.w;
-            for (int64_t e = e0; e < e1; e++) {
-                const int64_t j = a.col[e];
-                TC xj[CPL];
-                VT<TV>::unpack(a.V[j * S + q], xj);
-                const TC w = wt[e];
-#pragma unroll
-                for (int c = 0; c < CPL; c++) g[c] += w * xj[c];
-            }
         } else {
-            int64_t e = e0;
-            for (; e + 4 <= e1; e += 4) {
-                const int64_t j0 = a.col[e], j1 = a.col[e + 1], j2 = a.col[e + 2], j3 = a.col[e + 3];
-                const uint4 u0 = a.V[j0 * S + q], u1 = a.V[j1 * S + q], u2 = a.V[j2 * S + q], u3 = a.V[j3 * S + q];
-                TC x0[CPL], x1[CPL], x2[CPL], x3[CPL];
-                VT<TV>::unpack(u0, x0); VT<TV>::unpack(u1, x1); VT<TV>::unpack(u2, x2); VT<TV>::unpack(u3, x3);
+            for (int64_t e = e0; e < e1; e += 8) {
+                const int64_t nE = e1 - e;
+                int64_t jj[8];
 #pragma unroll
-                for (int c = 0; c < CPL; c++) g[c] += (x0[c] + x1[c]) + (x2[c] + x3[c]);
-            }
-            for (; e < e1; e++) {
-                TC xj[CPL];
-                VT<TV>::unpack(a.V[(int64_t)a.col[e] * S + q], xj);
+                for (int t = 0; t < 8; t++) {   // clamped index: always in bounds; lanes past the row reuse row i
+                    const int32_t cj = a.col[t < nE ? e + t : e1 - 1];
+                    jj[t] = t < nE ? (int64_t)cj : i;
+                }
+                uint4 u[8];
 #pragma unroll
-                for (int c = 0; c < CPL; c++) g[c] += xj[c];
+                for (int t = 0; t < 8; t++) u[t] = a.V[jj[t] * S + q];
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    TC xj[CPL];
+                    VT<TV>::unpack(u[t], xj);
+                    TC w = (TC)0;
+                    if (t < nE) w = wt ? wt[e + t] : (TC)1;
+#pragma unroll
+                    for (int c = 0; c < CPL; c++) g[c] += w * xj[c];
+                }
             }
         }
         TC qd = 0;
 #pragma unroll
         for (int c = 0; c < CPL; c++) qd += x[c] * g[c];
         qd = slot_sum<S>(qd);
-        if (q == 0) quad = (double)qd;
+        if (q == 0) quad += (double)qd;
         const double di = a.d[i];
         // rank-1 correction -d_i z / L
 #pragma unroll
-        for (int c = 0; c < CPL; c++) g[c] -= (TC)(di * a.z[q * CPL + c] / a.L);
+        for (int c = 0; c < CPL; c++) g[c] -= (TC)(di * zs[q * CPL + c] / a.L);
         // pair levels: d_i (M_Q - pp_QQ d_i V_i)
         for (int l = 0; l < a.nlev; l++) {
             const int Q = a.Poff[l] + a.lab[l][i];
-            const double *M = a.M + (int64_t)Q * R + q * CPL;
-            const double pd = a.pdiag[Q] * di;
+            const double *M = Ms + (int64_t)Q * R + q * CPL;
+            const double pd = pds[Q] * di;
 #pragma unroll
             for (int c = 0; c < CPL; c++) g[c] += (TC)(di * (M[c] - pd * (double)x[c]));
         }
         // VCS: 1/2 d_i (sum_R pv_iR Y_R + Z_R(i)) - pv_own d_i^2 V_i
-        const TC *pv = (const TC *)a.pv;
         for (int m = 0; m < a.nvcs; m++) {
             const int Ri = a.vlab[m][i];
             double acc[CPL];
@@ -529,13 +581,11 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
                 const int Rc = a.vcol[e];
                 const double p = (double)pv[e];
                 if (Rc == Ri) pown = p;
-                if (p != 0.0) {
-                    const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+                const double *YR = Ys + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
 #pragma unroll
-                    for (int c = 0; c < CPL; c++) acc[c] += p * YR[c];
-                }
+                for (int c = 0; c < CPL; c++) acc[c] += p * YR[c];
             }
-            const double *ZR = a.Z + (int64_t)(a.Zoff[m] + Ri) * R + q * CPL;
+            const double *ZR = Zs + (int64_t)(a.Zoff[m] + Ri) * R + q * CPL;
 #pragma unroll
             for (int c = 0; c < CPL; c++) g[c] += (TC)(0.5 * di * (acc[c] + ZR[c]) - pown * di * di * (double)x[c]);
         }
@@ -549,17 +599,28 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
                 for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
             }
         } else {
-            const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
-            TC *pt = (TC *)a.pt;
-            for (int64_t k = r0; k < r1; k++) {
-                const TriRec<TC> rc = rec[k];
-                TC p, tau;
-                tri_record<TV, R>(a.V, rc, x, q, a.rho, a.rt, g, p, tau);
-                if (rc.code % 3 == 0 && q == 0) {
-                    pt[rc.code / 3] = p;
-                    if (a.rho > 0.0) {
-                        tot += ((double)p * p - (double)rc.lam * rc.lam) / (2 * a.rt);
-                        gmin = fmin(gmin, (double)((TC)1 - tau));
+            for (int64_t k = r0; k < r1; k += 4) {
+                const int64_t nK = r1 - k;
+                TriRec<TC> rc[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) rc[t] = rec[t < nK ? k + t : r1 - 1];
+                uint4 uu[4], uw[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) { uu[t] = a.V[(int64_t)rc[t].u * S + q]; uw[t] = a.V[(int64_t)rc[t].w * S + q]; }
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    if (t < nK) {
+                        TC xu[CPL], xw[CPL], p, tau;
+                        VT<TV>::unpack(uu[t], xu);
+                        VT<TV>::unpack(uw[t], xw);
+                        tri_record_x<TV, R>(rc[t].code, rc[t].lam, x, xu, xw, a.rho, a.rt, g, p, tau);
+                        if (rc[t].code % 3 == 0 && q == 0) {
+                            pt[rc[t].code / 3] = p;
+                            if (a.rho > 0.0) {
+                                tot += ((double)p * p - (double)rc[t].lam * rc[t].lam) / (2 * a.rt);
+                                gmin = fmin(gmin, (double)((TC)1 - tau));
+                            }
+                        }
                     }
                 }
             }
@@ -570,7 +631,7 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
 #pragma unroll
         for (int c = 0; c < CPL; c++) { G[c] = g[c]; gd += g[c] * x[c]; }
         gd = slot_sum<S>(gd);
-        if (q == 0) gv = (double)gd;
+        if (q == 0) gv += (double)gd;
     }
     quad = block_sum(quad, red);
     __syncthreads();
@@ -586,21 +647,21 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------ lb_k_finalize
-__global__ __launch_bounds__(LB_BLOCK) void lb_k_finalize(FinArgs a) {
-    __shared__ double red[LB_BLOCK / 64];
+__global__ __launch_bounds__(1024) void lb_k_finalize(FinArgs a) {
+    __shared__ double red[16];
     double quad = 0, gv = 0, ttot = 0, gmin = INFINITY, vtot = 0, vviol = 0, ltot = 0, lviol = 0;
     // per-thread strided partial sums, combined in a fixed order
-    for (int b = threadIdx.x; b < a.nrow_blk; b += LB_BLOCK) {
+    for (int b = threadIdx.x; b < a.nrow_blk; b += 1024) {
         const double *o = a.row_part + 4 * (int64_t)b;
         quad += o[0]; gv += o[1]; ttot += o[2]; gmin = fmin(gmin, o[3]);
     }
-    for (int b = threadIdx.x; b < a.nseg_blk; b += LB_BLOCK) {
+    for (int b = threadIdx.x; b < a.nseg_blk; b += 1024) {
         ttot += a.seg_part[2 * b]; gmin = fmin(gmin, a.seg_part[2 * b + 1]);
     }
-    for (int b = threadIdx.x; b < a.nvcs_blk; b += LB_BLOCK) {
+    for (int b = threadIdx.x; b < a.nvcs_blk; b += 1024) {
         vtot += a.vcs_part[(int64_t)b * a.vcs_width]; vviol += a.vcs_part[(int64_t)b * a.vcs_width + 1];
     }
-    for (int b = threadIdx.x; b < a.sumPpair; b += LB_BLOCK) { ltot += a.lev_tot[b]; lviol += a.lev_viol[b]; }
+    for (int b = threadIdx.x; b < a.sumPpair; b += 1024) { ltot += a.lev_tot[b]; lviol += a.lev_viol[b]; }
     quad = block_sum(quad, red); __syncthreads();
     gv = block_sum(gv, red); __syncthreads();
     ttot = block_sum(ttot, red); __syncthreads();
@@ -628,13 +689,14 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_finalize(FinArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------ small kernels
+// per-block partial sums (fixed order) of x, reduced by lb_k_reduce
 template <typename TC>
-__global__ void lb_k_sum(const TC *x, int64_t cnt, double *out) {
+__global__ void lb_k_sum(const TC *x, int64_t cnt, double *part) {
     __shared__ double red[LB_BLOCK / 64];
     double s = 0.0;
-    for (int64_t e = threadIdx.x; e < cnt; e += LB_BLOCK) s += (double)x[e];
+    for (int64_t e = (int64_t)blockIdx.x * LB_BLOCK + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * LB_BLOCK) s += (double)x[e];
     s = block_sum(s, red);
-    if (threadIdx.x == 0) *out = s;
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 template <typename TC>
@@ -1493,6 +1555,7 @@ struct lb_ctx {
     std::string err;
     size_t vbytes = 4, cbytes = 4;   // storage bytes of V elements, of TC values
     int heavy_edges = 64, heavy_tris = 64, seg_len = 128;
+    int64_t row_grid = 4096;                         // workgroups of the grid-stride row kernel
     int64_t built_heavyE = 64, built_heavyT = 64;   // thresholds the current segment tables were built with
     // graph
     int64_t n = 0, nnz = 0; double L = 1.0, C0 = 0.0; bool weighted = false;
@@ -1636,8 +1699,8 @@ struct Launch {
         hipLaunchKernelGGL((lb_k_colsum<TV, R>), dim3(LB_RED_BLOCKS), dim3(LB_BLOCK), ca.use_lds ? ldsY : 0, c->stream, ca);
         if (ev) hipEventRecord(ev[(*ek)++], c->stream);
         const int width = R + (ca.use_lds ? sumPall * R : 0);
-        hipLaunchKernelGGL(lb_k_reduce, dim3((width + 255) / 256), dim3(256), 0, c->stream, (const double *)c->colsum_part.p,
-                           LB_RED_BLOCKS, width, (double *)c->Y.p);
+        hipLaunchKernelGGL(lb_k_reduce, dim3((width + 63) / 64), dim3(1024), 0, c->stream, (const double *)c->colsum_part.p,
+                           LB_RED_BLOCKS, width, width, (double *)c->Y.p);
     }
 
     static int eval(lb_ctx *c, double rho, double rt, hipEvent_t *ev) {
@@ -1663,7 +1726,7 @@ struct Launch {
         }
         mark();
         // --- VCS multipliers and Z
-        const size_t ldsZ = sizeof(double) * (size_t)c->sumPv * R;
+        const size_t ldsZ = (sizeof(double) + (LB_BLOCK / 64) * sizeof(TC)) * (size_t)c->sumPv * R;
         const int vcs_lds = ldsZ <= 48 * 1024;
         const int vcs_width = 2 + (vcs_lds ? c->sumPv * R : 0);
         if (c->nvcs) {
@@ -1686,8 +1749,8 @@ struct Launch {
         if (c->nvcs && vcs_lds) {
             const int width = c->sumPv * R;
             // Z entries sit after the two scalars of each block's partial row
-            hipLaunchKernelGGL(lb_k_reduce, dim3((width + 255) / 256), dim3(256), 0, c->stream,
-                               (const double *)c->vcs_part.p + 2, LB_RED_BLOCKS, vcs_width, (double *)c->Z.p);
+            hipLaunchKernelGGL(lb_k_reduce, dim3((width + 63) / 64), dim3(1024), 0, c->stream,
+                               (const double *)c->vcs_part.p + 2, LB_RED_BLOCKS, width, vcs_width, (double *)c->Z.p);
         }
         mark();
         // --- hub segments
@@ -1702,7 +1765,7 @@ struct Launch {
         }
         mark();
         // --- rows
-        const int64_t nrow_blk = (n + NSL - 1) / NSL;
+        const int64_t nrow_blk = std::min<int64_t>((n + NSL - 1) / NSL, c->row_grid);
         {
             RowArgs ra{};
             ra.V = (const uint4 *)c->V.p; ra.n = n; ra.rowptr = (const int64_t *)c->rowptr.p; ra.col = (const int32_t *)c->col.p;
@@ -1724,7 +1787,10 @@ struct Launch {
             ra.hE_rows = (const int32_t *)c->hErows.p; ra.hE_seg0 = (const int32_t *)c->hEseg0.p; ra.nhE = (int)c->hE_rows.size();
             ra.hT_rows = (const int32_t *)c->hTrows.p; ra.hT_seg0 = (const int32_t *)c->hTseg0.p; ra.nhT = (int)c->hT_rows.size();
             ra.segout = c->segout.p; ra.G = c->G.p; ra.part = (double *)c->row_part.p;
-            hipLaunchKernelGGL((lb_k_rows<TV, R>), dim3((unsigned)nrow_blk), dim3(LB_BLOCK), 0, c->stream, ra);
+            ra.sumPpair = c->sumPpair; ra.sumPv = c->sumPv;
+            const size_t stage_bytes = sizeof(double) * ((size_t)R + (size_t)c->sumPpair * R + c->sumPpair + 2 * (size_t)c->sumPv * R);
+            ra.stage = stage_bytes <= 48 * 1024;
+            hipLaunchKernelGGL((lb_k_rows<TV, R>), dim3((unsigned)nrow_blk), dim3(LB_BLOCK), ra.stage ? stage_bytes : 0, c->stream, ra);
         }
         mark();
         // --- scalars
@@ -1735,7 +1801,7 @@ struct Launch {
         fa.row_part = (const double *)c->row_part.p; fa.nrow_blk = (int)nrow_blk;
         fa.seg_part = (const double *)c->seg_part.p; fa.nseg_blk = nseg ? nseg_blk : 0;
         fa.rho = rho; fa.parts = (double *)c->parts.p;
-        hipLaunchKernelGGL(lb_k_finalize, dim3(1), dim3(LB_BLOCK), 0, c->stream, fa);
+        hipLaunchKernelGGL(lb_k_finalize, dim3(1), dim3(1024), 0, c->stream, fa);
         mark();
         HIPCHK(c, hipGetLastError());
         return 0;
@@ -2058,7 +2124,7 @@ int ensure_eval_buffers(lb_ctx *c) {
     if ((rc = dalloc(c, c->levtot, sizeof(double) * std::max(c->sumPpair, 1)))) return rc;
     if ((rc = dalloc(c, c->levviol, sizeof(double) * std::max(c->sumPpair, 1)))) return rc;
     if ((rc = dalloc(c, c->Z, sizeof(double) * (size_t)std::max(c->sumPv, 1) * R))) return rc;
-    const bool zlds = sizeof(double) * (size_t)c->sumPv * R <= 48 * 1024;
+    const bool zlds = (sizeof(double) + (LB_BLOCK / 64) * c->cbytes) * (size_t)c->sumPv * R <= 48 * 1024;
     if ((rc = dalloc(c, c->vcs_part, sizeof(double) * (size_t)LB_RED_BLOCKS * (2 + (zlds ? c->sumPv * R : 0))))) return rc;
     const int S = (int)(R * c->vbytes / 16), NSL = LB_BLOCK / S;
     const int64_t nrow_blk = (c->n + NSL - 1) / NSL;
@@ -2078,8 +2144,10 @@ int refresh_lamsum(lb_ctx *c) {
         HIPCHK(c, hipMemsetAsync(c->lamsum.p, 0, sizeof(double), c->stream));
         return 0;
     }
-    if (c->cbytes == 8) hipLaunchKernelGGL(lb_k_sum<double>, dim3(1), dim3(LB_BLOCK), 0, c->stream, (const double *)c->lam.p, c->T, (double *)c->lamsum.p);
-    else hipLaunchKernelGGL(lb_k_sum<float>, dim3(1), dim3(LB_BLOCK), 0, c->stream, (const float *)c->lam.p, c->T, (double *)c->lamsum.p);
+    if ((rc = dalloc(c, c->colsum_part, sizeof(double) * LB_RED_BLOCKS))) return rc;
+    if (c->cbytes == 8) hipLaunchKernelGGL(lb_k_sum<double>, dim3(LB_RED_BLOCKS), dim3(LB_BLOCK), 0, c->stream, (const double *)c->lam.p, c->T, (double *)c->colsum_part.p);
+    else hipLaunchKernelGGL(lb_k_sum<float>, dim3(LB_RED_BLOCKS), dim3(LB_BLOCK), 0, c->stream, (const float *)c->lam.p, c->T, (double *)c->colsum_part.p);
+    hipLaunchKernelGGL(lb_k_reduce, dim3(1), dim3(1024), 0, c->stream, (const double *)c->colsum_part.p, LB_RED_BLOCKS, 1, 1, (double *)c->lamsum.p);
     HIPCHK(c, hipGetLastError());
     return 0;
 }
@@ -2164,10 +2232,11 @@ void lb_destroy(lb_ctx *c) {
 
 const char *lb_last_error(const lb_ctx *c) { return c ? c->err.c_str() : "no context"; }
 
-int lb_set_tuning(lb_ctx *c, int heavy_edges, int heavy_tris, int seg_len) {
+int lb_set_tuning(lb_ctx *c, int heavy_edges, int heavy_tris, int seg_len, int row_grid) {
     if (!c) return -1;
-    if (heavy_edges < 1 || heavy_tris < 1 || seg_len < 1) return fail(c, -1, "tuning values must be positive");
+    if (heavy_edges < 1 || heavy_tris < 1 || seg_len < 1 || row_grid < 0) return fail(c, -1, "tuning values must be positive");
     c->heavy_edges = heavy_edges; c->heavy_tris = heavy_tris; c->seg_len = seg_len;
+    if (row_grid > 0) c->row_grid = row_grid;
     return 0;
 }
 

@@ -48,8 +48,8 @@ class Engine:
             raise LBError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
 
     # ------------------------------------------------------------------------------------------------ upload
-    def set_tuning(self, heavy_edges=64, heavy_tris=64, seg_len=128):
-        self._chk(self.L.lb_set_tuning(self.ctx, heavy_edges, heavy_tris, seg_len), "lb_set_tuning")
+    def set_tuning(self, heavy_edges=64, heavy_tris=64, seg_len=128, row_grid=0):
+        self._chk(self.L.lb_set_tuning(self.ctx, heavy_edges, heavy_tris, seg_len, row_grid), "lb_set_tuning")
 
     def set_instance(self, inst):
         n = inst.n
@@ -153,6 +153,21 @@ class Engine:
         assert orders.shape[1] == self.inst.n
         self._chk(self.L.lb_sweeps(self.ctx, len(orders), float(rho), float(rt), float(damp), float(kap2),
                                    ptr(orders, _i64p), int(nbatch)), "lb_sweeps")
+
+    def separate_triangles(self, fam=0, thresh=0.3, maxh=100, maxk=60, hub2=8, cap2=200, vtol=1e-3, cap=1000,
+                           nchunk=512, nb=2000):
+        """Triangle separation on the resident V (fam 0 wedges, fam 1 one-edge arms). Returns (TI int64 m x 3, val,
+        vmin)."""
+        out = np.zeros((max(1, cap), 3), np.int64)
+        val = np.zeros(max(1, cap))
+        cnt = np.zeros(1, np.int64)
+        vmin = np.zeros(1)
+        self._chk(self.L.lb_separate_triangles(self.ctx, int(fam), float(thresh), int(maxh), int(maxk), int(hub2),
+                                               int(cap2), float(vtol), int(cap), int(nchunk), int(nb), ptr(out, _i64p),
+                                               ptr(val, _f64p), ptr(cnt, _i64p), ptr(vmin, _f64p)),
+                  "lb_separate_triangles")
+        m = int(cnt[0])
+        return out[:m].copy(), val[:m].copy(), float(vmin[0])
 
     def sync(self):
         self._chk(self.L.lb_sync(self.ctx), "lb_sync")
