@@ -59,6 +59,13 @@ int lb_set_level_sig(lb_ctx *ctx, const double *sig);
 int lb_set_vcs(lb_ctx *ctx, int nvcs, const int32_t *labs, const int32_t *P, const double *rq, const int64_t *ptr,
                const int32_t *col, const double *sig, double wv);
 int lb_set_vcs_sig(lb_ctx *ctx, const double *sig);
+/* VCS entry ages (consecutive outers with multiplier 0 and slack, uint8 per entry, vcsops.py:300-305): set / read
+ * together with the multipliers. lb_phr_update advances the ages from the last evaluation's slack flags. */
+int lb_set_vcs_age(lb_ctx *ctx, const uint8_t *age);
+int lb_get_vcs_state(lb_ctx *ctx, double *sig, uint8_t *age);
+/* Change the pair weight (r_QR = rq_Q rq_R w) and the VCS weight wv = vc_weight * w everywhere they enter
+ * (set_pair_weight, alm_vcs.py:1270-1275). */
+int lb_set_weights(lb_ctx *ctx, double pairlvl_weight, double wv);
 
 /* Triangle pool: TI (T x 3 int32, (a, centre, c)), multipliers lam (FP64). Builds the vertex -> (t, role)
  * incidence in t order (alm_vcs.py:94-104) and the incidence-ordered records (inc_abc / inc_lam,
@@ -83,8 +90,8 @@ int lb_get_G(lb_ctx *ctx, void *G, int host_dtype);
  * any pointer may be NULL. */
 int lb_get_multipliers(lb_ctx *ctx, double *pp, double *pt, double *pv);
 
-/* PHR multiplier update on the device: sig <- pp, lam <- pt, VCS sig <- pv of the last rho > 0 evaluation
- * (alm_vcs.py:1704-1711,1723). */
+/* PHR multiplier update on the device: sig <- pp, lam <- pt, VCS sig <- pv of the last rho > 0 evaluation, and the
+ * VCS ages from its slack flags (alm_vcs.py:1704-1711,1723; vcsops.age_update). */
 int lb_phr_update(lb_ctx *ctx);
 
 /* `nsw` sweeps of batched Gauss-Seidel row steps on the resident V (sweeps + batch_step6, alm_vcs.py:1360-1436,
@@ -106,6 +113,13 @@ int lb_sweeps(lb_ctx *ctx, int nsw, double rho, double rt, double damp, double k
 int lb_separate_triangles(lb_ctx *ctx, int fam, double thresh, int64_t maxh, int64_t maxk, int64_t hub2,
                           int64_t cap2, double vtol, int64_t cap, int nchunk, int nb, int64_t *TI_out,
                           double *val_out, int64_t *count, double *vmin);
+
+/* VCS separation of level m on the resident V (vcsops.separate, vcsops.py:395-436): the nadd best violated
+ * vertex-cluster entries outside the working set (mean-X violation s / vol < -tol; score mode 0 pen = wv rq_R s^2,
+ * 1 mean, 2 mass), score descending with ties in (i, R) order; out_* hold nadd entries, *count the number written;
+ * stats[4] = (#violated, violation mass, #violated in the working set, mass in the working set). */
+int lb_separate_vcs(lb_ctx *ctx, int m, double tol, int mode, int64_t nadd, int64_t *out_i, int32_t *out_R,
+                    double *out_score, int64_t *count, double *stats);
 
 /* Wait for the context's stream. */
 int lb_sync(lb_ctx *ctx);

@@ -15,7 +15,8 @@ LB_NKERNELS = 8
 # every symbol include/lbhip.h declares
 EXPORTS = ["lb_create", "lb_destroy", "lb_last_error", "lb_set_graph", "lb_set_levels", "lb_set_level_sig",
            "lb_set_vcs", "lb_set_vcs_sig", "lb_set_triangles", "lb_set_lam", "lb_set_V", "lb_get_V",
-           "lb_eval_fgrad", "lb_get_G", "lb_get_multipliers", "lb_phr_update", "lb_sweeps", "lb_separate_triangles", "lb_sync", "lb_time_eval",
+           "lb_eval_fgrad", "lb_get_G", "lb_get_multipliers", "lb_phr_update", "lb_sweeps", "lb_separate_triangles", "lb_separate_vcs", "lb_set_vcs_age", "lb_get_vcs_state",
+           "lb_set_weights", "lb_sync", "lb_time_eval",
            "lb_kernel_name", "lb_set_tuning", "lb_version"]
 
 _i64p = C.POINTER(C.c_int64)
@@ -58,6 +59,10 @@ def load():
         "lb_get_multipliers": ([ctx, _f64p, _f64p, _f64p], C.c_int),
         "lb_phr_update": ([ctx], C.c_int),
         "lb_sweeps": ([ctx, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _i64p, C.c_int], C.c_int),
+        "lb_separate_vcs": ([ctx, C.c_int, C.c_double, C.c_int, C.c_int64, _i64p, _i32p, _f64p, _i64p, _f64p], C.c_int),
+        "lb_set_vcs_age": ([ctx, C.c_void_p], C.c_int),
+        "lb_get_vcs_state": ([ctx, _f64p, C.c_void_p], C.c_int),
+        "lb_set_weights": ([ctx, C.c_double, C.c_double], C.c_int),
         "lb_sync": ([ctx], C.c_int),
         "lb_time_eval": ([ctx, C.c_double, C.c_double, C.c_int, _f64p, _f64p], C.c_int),
         "lb_kernel_name": ([C.c_int], C.c_char_p),

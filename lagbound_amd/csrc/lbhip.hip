@@ -136,7 +136,7 @@ struct VcsArgs {
     int nvcs; int Yoff[LB_MAXVCS]; int Zoff[LB_MAXVCS + 1];
     const int32_t *lab[LB_MAXVCS]; const double *rq[LB_MAXVCS];
     const int64_t *ptr[LB_MAXVCS];   // absolute entry offsets
-    const int32_t *col; const void *sig; void *pv;
+    const int32_t *col; const void *sig; void *pv; uint8_t *slack;
     const double *Y; double wv, rho; int sumPv; int use_lds;
     double *part;              // gridDim x (2 + sumPv R) when use_lds, else gridDim x 2
     double *Zglob;
@@ -340,6 +340,7 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
                 if (q == 0) {
                     pv[e] = (TC)p;
                     if (g < -1e-9) viol += 1.0;
+                    a.slack[e] = (p == 0.0 && g > 0.0) ? 1 : 0;   // inactive with slack (vcsops.py:285)
                 }
                 const TC pc = (TC)p;  // the stored multiplier is what G uses
                 if (pc != (TC)0) {
@@ -697,6 +698,14 @@ __global__ void lb_k_sum(const TC *x, int64_t cnt, double *part) {
     for (int64_t e = (int64_t)blockIdx.x * LB_BLOCK + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * LB_BLOCK) s += (double)x[e];
     s = block_sum(s, red);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// age_update (vcsops.py:300-305): consecutive outers an entry stayed inactive with slack
+__global__ void lb_k_age(uint8_t *age, const uint8_t *slack, int64_t cnt) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * blockDim.x) {
+        if (slack[e]) { if (age[e] < 255) age[e] = age[e] + 1; }
+        else age[e] = 0;
+    }
 }
 
 template <typename TC>
@@ -1537,6 +1546,93 @@ __global__ void lb_k_gather_pos(const int64_t *rowptr, int64_t n, const int64_t 
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j <= n; j += (int64_t)gridDim.x * blockDim.x) out[j] = apos[rowptr[j]];
 }
 
+
+// ------------------------------------------------------------------------------------------------ VCS separation
+// vcsops.separate (vcsops.py:395-436; _sep_rows 334-366): per vertex i and cluster R, s = V_i . Y_R (FP64), own
+// cluster s -= d_i and vol -= d_i, violated if s / vol < -tol; entries outside the working set are candidates with
+// score pen = wv rq_R s^2 | mean = -s / vol | mass = -d_i s (> 0). One thread per vertex; Y staged in LDS.
+struct VcsSepArgs {
+    const uint4 *V; int64_t n; const double *d; const int32_t *lab; const int64_t *ptr; const int32_t *col;
+    const double *Y; const double *vol; const double *rq; int P; double wv, tol; int mode, use_lds;
+    const int64_t *base;      // write pass: output offset per vertex
+    int64_t *cnt;             // count pass: candidates per vertex
+    int32_t *oi; int32_t *oR; double *os;
+    double *stats;            // per block: #viol, mass, #viol in ws, mass in ws
+};
+
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_sep(VcsSepArgs a, int write) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL;
+    extern __shared__ double shY[];
+    __shared__ double red[LB_BLOCK / 64];
+    const double *Y = a.Y;
+    if (a.use_lds) {
+        for (int e = threadIdx.x; e < a.P * R; e += LB_BLOCK) shY[e] = a.Y[e];
+        __syncthreads();
+        Y = shY;
+    }
+    double nv = 0.0, mv = 0.0, nw = 0.0, mw = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * LB_BLOCK + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * LB_BLOCK) {
+        double x[R];
+#pragma unroll
+        for (int qq = 0; qq < S; qq++) {
+            TC t[CPL];
+            VT<TV>::unpack(a.V[i * S + qq], t);
+#pragma unroll
+            for (int c = 0; c < CPL; c++) x[qq * CPL + c] = (double)t[c];
+        }
+        const double di = a.d[i];
+        const int Ri = a.lab[i];
+        int64_t q = a.ptr[i];
+        const int64_t qe = a.ptr[i + 1];
+        int64_t cnt = 0;
+        int64_t o = write ? a.base[i] : 0;
+        for (int Rc = 0; Rc < a.P; Rc++) {
+            const double *YR = Y + (int64_t)Rc * R;
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < R; c++) s += x[c] * YR[c];
+            double vR = a.vol[Rc];
+            if (Rc == Ri) { s = s - di; vR = vR - di; }
+            if (vR <= 0.0) continue;
+            const double xb = s / vR;
+            if (xb >= -a.tol) continue;
+            while (q < qe && a.col[q] < Rc) q++;
+            const bool inws = q < qe && a.col[q] == Rc;
+            nv += 1.0; mv += -di * s;
+            if (inws) { nw += 1.0; mw += -di * s; continue; }
+            const double sc = a.mode == 0 ? a.wv * a.rq[Rc] * s * s : (a.mode == 1 ? -xb : -di * s);
+            if (sc <= 0.0) continue;
+            if (write) { a.oi[o] = (int32_t)i; a.oR[o] = Rc; a.os[o] = sc; o++; }
+            cnt++;
+        }
+        if (!write) a.cnt[i] = cnt;
+    }
+    if (!write) {
+        nv = block_sum(nv, red); __syncthreads();
+        mv = block_sum(mv, red); __syncthreads();
+        nw = block_sum(nw, red); __syncthreads();
+        mw = block_sum(mw, red);
+        if (threadIdx.x == 0) {
+            double *st = a.stats + 4 * (int64_t)blockIdx.x;
+            st[0] = nv; st[1] = mv; st[2] = nw; st[3] = mw;
+        }
+    }
+}
+
+__global__ void lb_k_iota64(int64_t *x, int64_t cnt) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * blockDim.x) x[e] = e;
+}
+
+__global__ void lb_k_take(const int64_t *idx, int64_t cnt, const int32_t *oi, const int32_t *oR, const double *os,
+                          int32_t *ti, int32_t *tR, double *ts) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = idx[e];
+        ti[e] = oi[k]; tR[e] = oR[k]; ts[e] = os[k];
+    }
+}
+
 // ================================================================================================== host side
 namespace {
 
@@ -1575,7 +1671,8 @@ struct lb_ctx {
     DBuf rowptr, col, w, d, labs, sig, pp, rq, dsq, vlabs, vrq, vptr, vcol, vsig, pv, TI, lam, pt, iptr, rec,
         V, G, Y, z, M, pdiag, levtot, levviol, Z, colsum_part, vcs_part, row_part, seg_part, segs, segout,
         hErows, hEseg0, hTrows, hTseg0, parts, lamsum,
-        order, tmask, Kq, stZ, Zs, cnts, Vstage, DDb, segkap, seglist;
+        order, tmask, Kq, stZ, Zs, cnts, Vstage, DDb, segkap, seglist, vslack, vage, vvol;
+    std::vector<double> h_d;                // vertex volumes (host copy)
 };
 
 namespace {
@@ -1740,7 +1837,8 @@ struct Launch {
                 va.ptr[m] = (const int64_t *)c->vptr.p + (int64_t)m * (n + 1);
             }
             va.Zoff[c->nvcs] = zo;
-            va.col = (const int32_t *)c->vcol.p; va.sig = c->vsig.p; va.pv = c->pv.p; va.Y = Yp; va.wv = c->wv; va.rho = rho;
+            va.col = (const int32_t *)c->vcol.p; va.sig = c->vsig.p; va.pv = c->pv.p; va.slack = (uint8_t *)c->vslack.p;
+            va.Y = Yp; va.wv = c->wv; va.rho = rho;
             va.sumPv = c->sumPv; va.use_lds = vcs_lds; va.part = (double *)c->vcs_part.p; va.Zglob = (double *)c->Z.p;
             if (!vcs_lds) hipLaunchKernelGGL(lb_k_fill, dim3(256), dim3(256), 0, c->stream, (double *)c->Z.p, (int64_t)c->sumPv * R, 0.0);
             hipLaunchKernelGGL((lb_k_vcs_z<TV, R>), dim3(LB_RED_BLOCKS), dim3(LB_BLOCK), vcs_lds ? ldsZ : 0, c->stream, va);
@@ -2068,11 +2166,92 @@ struct Launch {
 #undef SEPHIP
         return 0;
     }
+
+    // VCS separation of level m: the nadd best candidates (score descending, ties in (i, R) order) and statistics
+    static int vcs_separate(lb_ctx *c, int m, double tol, int mode, int64_t nadd, std::vector<int64_t> &oi,
+                            std::vector<int32_t> &oR, std::vector<double> &os, double *stats) {
+        const int64_t n = c->n;
+        int rc;
+        zY(c);
+        DBuf cnt, base, ci, cR, cs, st, tmp, idx, idx2, sk, ti, tR, ts;
+        auto cleanup = [&]() { DBuf *bs[] = {&cnt, &base, &ci, &cR, &cs, &st, &tmp, &idx, &idx2, &sk, &ti, &tR, &ts}; for (DBuf *b : bs) dfree(*b); };
+#define VCHK(x) do { if ((rc = (x))) { cleanup(); return rc; } } while (0)
+#define VHIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { cleanup(); return fail(c, -3, std::string(#x) + ": " + hipGetErrorString(e_)); } } while (0)
+        int zo = 0;
+        for (int k = 0; k < m; k++) zo += c->vP[k];
+        const int P = c->vP[m];
+        VcsSepArgs a{};
+        a.V = (const uint4 *)c->V.p; a.n = n; a.d = (const double *)c->d.p; a.lab = (const int32_t *)c->vlabs.p + (int64_t)m * n;
+        a.ptr = (const int64_t *)c->vptr.p + (int64_t)m * (n + 1); a.col = (const int32_t *)c->vcol.p;
+        a.Y = (const double *)c->Y.p + R + (int64_t)(c->sumPpair + zo) * R; a.vol = (const double *)c->vvol.p + zo;
+        a.rq = (const double *)c->vrq.p + zo; a.P = P; a.wv = c->wv; a.tol = tol; a.mode = mode;
+        const size_t lds = sizeof(double) * (size_t)P * R;
+        a.use_lds = lds <= 48 * 1024;
+        const unsigned grid = (unsigned)std::min<int64_t>((n + LB_BLOCK - 1) / LB_BLOCK, 4096);
+        VCHK(dalloc(c, cnt, sizeof(int64_t) * (n + 1)));
+        VCHK(dalloc(c, base, sizeof(int64_t) * (n + 1)));
+        VCHK(dalloc(c, st, sizeof(double) * 4 * grid));
+        VHIP(hipMemsetAsync(cnt.p, 0, sizeof(int64_t) * (n + 1), c->stream));
+        a.cnt = (int64_t *)cnt.p; a.stats = (double *)st.p;
+        hipLaunchKernelGGL((lb_k_vcs_sep<TV, R>), dim3(grid), dim3(LB_BLOCK), a.use_lds ? lds : 0, c->stream, a, 0);
+        size_t tb = 0;
+        VHIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (const int64_t *)cnt.p, (int64_t *)base.p, (int)(n + 1), c->stream));
+        VCHK(dalloc(c, tmp, tb));
+        VHIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, (const int64_t *)cnt.p, (int64_t *)base.p, (int)(n + 1), c->stream));
+        int64_t total = 0;
+        std::vector<double> sth(4 * grid);
+        VHIP(hipMemcpyAsync(&total, (const int64_t *)base.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+        VHIP(hipMemcpyAsync(sth.data(), st.p, sizeof(double) * 4 * grid, hipMemcpyDeviceToHost, c->stream));
+        VHIP(hipStreamSynchronize(c->stream));
+        for (int k = 0; k < 4; k++) { double t = 0.0; for (unsigned b = 0; b < grid; b++) t += sth[4 * b + k]; stats[k] = t; }
+        oi.clear(); oR.clear(); os.clear();
+        if (total == 0 || nadd <= 0) { cleanup(); return 0; }
+        VCHK(dalloc(c, ci, sizeof(int32_t) * total));
+        VCHK(dalloc(c, cR, sizeof(int32_t) * total));
+        VCHK(dalloc(c, cs, sizeof(double) * total));
+        a.base = (const int64_t *)base.p; a.oi = (int32_t *)ci.p; a.oR = (int32_t *)cR.p; a.os = (double *)cs.p;
+        hipLaunchKernelGGL((lb_k_vcs_sep<TV, R>), dim3(grid), dim3(LB_BLOCK), a.use_lds ? lds : 0, c->stream, a, 1);
+        const int64_t keep = std::min<int64_t>(nadd, total);
+        const int32_t *pi = (const int32_t *)ci.p, *pR = (const int32_t *)cR.p;
+        const double *ps = (const double *)cs.p;
+        if (total > nadd) {   // stable descending sort of the scores: ties stay in (i, R) order
+            VCHK(dalloc(c, idx, sizeof(int64_t) * total));
+            VCHK(dalloc(c, idx2, sizeof(int64_t) * total));
+            VCHK(dalloc(c, sk, sizeof(double) * total));
+            hipLaunchKernelGGL(lb_k_iota64, dim3(1024), dim3(256), 0, c->stream, (int64_t *)idx.p, total);
+            size_t tb2 = 0;
+            VHIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb2, ps, (double *)sk.p, (const int64_t *)idx.p,
+                                                              (int64_t *)idx2.p, (int)total, 0, 64, c->stream));
+            VCHK(dalloc(c, tmp, tb2));
+            VHIP(hipcub::DeviceRadixSort::SortPairsDescending(tmp.p, tb2, ps, (double *)sk.p, (const int64_t *)idx.p,
+                                                              (int64_t *)idx2.p, (int)total, 0, 64, c->stream));
+            VCHK(dalloc(c, ti, sizeof(int32_t) * keep));
+            VCHK(dalloc(c, tR, sizeof(int32_t) * keep));
+            VCHK(dalloc(c, ts, sizeof(double) * keep));
+            hipLaunchKernelGGL(lb_k_take, dim3(1024), dim3(256), 0, c->stream, (const int64_t *)idx2.p, keep, pi, pR, ps,
+                               (int32_t *)ti.p, (int32_t *)tR.p, (double *)ts.p);
+            pi = (const int32_t *)ti.p; pR = (const int32_t *)tR.p; ps = (const double *)ts.p;
+        }
+        std::vector<int32_t> hi(keep);
+        oR.resize(keep); os.resize(keep);
+        VHIP(hipMemcpyAsync(hi.data(), pi, sizeof(int32_t) * keep, hipMemcpyDeviceToHost, c->stream));
+        VHIP(hipMemcpyAsync(oR.data(), pR, sizeof(int32_t) * keep, hipMemcpyDeviceToHost, c->stream));
+        VHIP(hipMemcpyAsync(os.data(), ps, sizeof(double) * keep, hipMemcpyDeviceToHost, c->stream));
+        VHIP(hipGetLastError());
+        VHIP(hipStreamSynchronize(c->stream));
+        oi.assign(hi.begin(), hi.end());
+        cleanup();
+#undef VCHK
+#undef VHIP
+        return 0;
+    }
 };
 
 typedef int (*eval_fn)(lb_ctx *, double, double, hipEvent_t *);
 typedef int (*sep_fn)(lb_ctx *, int, double, int64_t, int64_t, int64_t, int64_t, double, int64_t, int, int,
                       std::vector<int64_t> &, std::vector<double> &, double *);
+typedef int (*vsep_fn)(lb_ctx *, int, double, int, int64_t, std::vector<int64_t> &, std::vector<int32_t> &,
+                       std::vector<double> &, double *);
 typedef int (*sweep_fn)(lb_ctx *, double, double, double, double, int, const std::vector<int64_t> &,
                         const std::vector<int64_t> &);
 
@@ -2105,6 +2284,18 @@ sep_fn pick_sep(int vdtype, int r) {
     if (r == 8) return &Launch<TV, 8>::separate;        \
     if (r == 16) return &Launch<TV, 16>::separate;      \
     if (r == 32) return &Launch<TV, 32>::separate;
+    if (vdtype == LB_F32) { LB_PICK(float) }
+    if (vdtype == LB_BF16) { LB_PICK(bf16raw) }
+    if (vdtype == LB_F64) { LB_PICK(double) }
+#undef LB_PICK
+    return nullptr;
+}
+
+vsep_fn pick_vsep(int vdtype, int r) {
+#define LB_PICK(TV)                                     \
+    if (r == 8) return &Launch<TV, 8>::vcs_separate;    \
+    if (r == 16) return &Launch<TV, 16>::vcs_separate;  \
+    if (r == 32) return &Launch<TV, 32>::vcs_separate;
     if (vdtype == LB_F32) { LB_PICK(float) }
     if (vdtype == LB_BF16) { LB_PICK(bf16raw) }
     if (vdtype == LB_F64) { LB_PICK(double) }
@@ -2224,7 +2415,7 @@ void lb_destroy(lb_ctx *c) {
                     &c->Y, &c->z, &c->M, &c->pdiag, &c->levtot, &c->levviol, &c->Z, &c->colsum_part, &c->vcs_part,
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
                     &c->hTseg0, &c->parts, &c->lamsum, &c->order, &c->tmask, &c->Kq, &c->stZ, &c->Zs, &c->cnts,
-                    &c->Vstage, &c->DDb, &c->segkap, &c->seglist};
+                    &c->Vstage, &c->DDb, &c->segkap, &c->seglist, &c->vslack, &c->vage, &c->vvol};
     for (DBuf *b : bufs) dfree(*b);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -2253,6 +2444,7 @@ int lb_set_graph(lb_ctx *c, int64_t n, const int64_t *rowptr, const int32_t *col
         if (col[e] < 0 || col[e] >= n) return fail(c, -1, "lb_set_graph: column index out of range");
     c->n = n; c->nnz = nnz; c->L = L; c->C0 = C0; c->weighted = weights != nullptr;
     c->h_rowptr.assign(rowptr, rowptr + n + 1);
+    c->h_d.assign(d, d + n);
     int rc;
     if ((rc = upload(c, c->rowptr, rowptr, sizeof(int64_t) * (n + 1)))) return rc;
     if ((rc = upload(c, c->col, col, sizeof(int32_t) * nnz))) return rc;
@@ -2344,8 +2536,53 @@ int lb_set_vcs(lb_ctx *c, int nvcs, const int32_t *labs, const int32_t *P, const
     if ((rc = upload_tc(c, c->vsig, sig, tot))) return rc;
     if ((rc = dalloc(c, c->pv, c->cbytes * std::max<int64_t>(tot, 1)))) return rc;
     if ((rc = upload(c, c->vrq, rq, sizeof(double) * sP))) return rc;
+    if ((rc = dalloc(c, c->vslack, std::max<int64_t>(tot, 1)))) return rc;
+    if ((rc = dalloc(c, c->vage, std::max<int64_t>(tot, 1)))) return rc;
+    HIPCHK(c, hipMemsetAsync(c->vage.p, 0, std::max<int64_t>(tot, 1), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->vslack.p, 0, std::max<int64_t>(tot, 1), c->stream));
+    // cluster volumes max(sum_{i in R} d_i, 1) (vcsops.py:37)
+    std::vector<double> vol(sP, 0.0);
+    {
+        int64_t o = 0;
+        for (int m = 0; m < nvcs; m++) {
+            for (int64_t i = 0; i < n; i++) vol[o + labs[(int64_t)m * n + i]] += c->h_d[i];
+            o += P[m];
+        }
+        for (auto &x : vol) x = std::max(x, 1.0);
+    }
+    if ((rc = upload(c, c->vvol, vol.data(), sizeof(double) * sP))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     c->have_eval = false;
     return ensure_eval_buffers(c);
+}
+
+int lb_set_vcs_age(lb_ctx *c, const uint8_t *age) {
+    if (!c || !c->nvcs || !age) return fail(c, -1, "lb_set_vcs_age: no VCS levels");
+    HIPCHK(c, hipSetDevice(c->device));
+    int64_t tot = 0;
+    for (int64_t x : c->vnnz) tot += x;
+    return upload(c, c->vage, age, tot);
+}
+
+int lb_get_vcs_state(lb_ctx *c, double *sig, uint8_t *age) {
+    if (!c || !c->nvcs) return fail(c, -1, "lb_get_vcs_state: no VCS levels");
+    HIPCHK(c, hipSetDevice(c->device));
+    int64_t tot = 0;
+    for (int64_t x : c->vnnz) tot += x;
+    int rc;
+    if (sig && (rc = download_tc(c, c->vsig, sig, tot))) return rc;
+    if (age && tot) {
+        HIPCHK(c, hipMemcpyAsync(age, c->vage.p, tot, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int lb_set_weights(lb_ctx *c, double pairlvl_weight, double wv) {
+    if (!c) return -1;
+    if (!(pairlvl_weight > 0.0) || !(wv > 0.0)) return fail(c, -1, "lb_set_weights: weights must be positive");
+    c->pw = pairlvl_weight; c->wv = wv;
+    return 0;
 }
 
 int lb_set_vcs_sig(lb_ctx *c, const double *sig) {
@@ -2541,6 +2778,7 @@ int lb_phr_update(lb_ctx *c) {
         int64_t tot = 0;
         for (int64_t x : c->vnnz) tot += x;
         HIPCHK(c, hipMemcpyAsync(c->vsig.p, c->pv.p, c->cbytes * tot, hipMemcpyDeviceToDevice, c->stream));
+        if (tot) hipLaunchKernelGGL(lb_k_age, dim3(512), dim3(256), 0, c->stream, (uint8_t *)c->vage.p, (const uint8_t *)c->vslack.p, tot);
     }
     if (c->T) {
         HIPCHK(c, hipMemcpyAsync(c->lam.p, c->pt.p, c->cbytes * c->T, hipMemcpyDeviceToDevice, c->stream));
@@ -2639,6 +2877,27 @@ int lb_separate_triangles(lb_ctx *c, int fam, double thresh, int64_t maxh, int64
     std::memcpy(TI_out, out.data(), sizeof(int64_t) * 3 * m);
     std::memcpy(val_out, val.data(), sizeof(double) * m);
     *count = m;
+    return 0;
+}
+
+int lb_separate_vcs(lb_ctx *c, int m, double tol, int mode, int64_t nadd, int64_t *out_i, int32_t *out_R,
+                    double *out_score, int64_t *count, double *stats) {
+    if (!c || !c->n || m < 0 || m >= c->nvcs) return fail(c, -1, "lb_separate_vcs: no such VCS level");
+    if (mode < 0 || mode > 2 || nadd < 0 || !out_i || !out_R || !out_score || !count || !stats)
+        return fail(c, -1, "lb_separate_vcs: bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    vsep_fn f = pick_vsep(c->vdtype, c->r);
+    if (!f) return fail(c, -1, "lb_separate_vcs: unsupported rank / dtype");
+    std::vector<int64_t> oi;
+    std::vector<int32_t> oR;
+    std::vector<double> os;
+    int rc = f(c, m, tol, mode, nadd, oi, oR, os, stats);
+    if (rc) return rc;
+    const int64_t k = (int64_t)oi.size();
+    std::memcpy(out_i, oi.data(), sizeof(int64_t) * k);
+    std::memcpy(out_R, oR.data(), sizeof(int32_t) * k);
+    std::memcpy(out_score, os.data(), sizeof(double) * k);
+    *count = k;
     return 0;
 }
 

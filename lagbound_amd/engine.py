@@ -169,6 +169,35 @@ class Engine:
         m = int(cnt[0])
         return out[:m].copy(), val[:m].copy(), float(vmin[0])
 
+    def separate_vcs(self, m=0, nadd=1000, tol=1e-4, mode="pen"):
+        """VCS separation of level m. Returns (i int64, R int32, score, stats dict)."""
+        oi = np.zeros(max(1, nadd), np.int64)
+        oR = np.zeros(max(1, nadd), np.int32)
+        os_ = np.zeros(max(1, nadd))
+        cnt = np.zeros(1, np.int64)
+        st = np.zeros(4)
+        md = {"pen": 0, "mean": 1, "mass": 2}[mode]
+        self._chk(self.L.lb_separate_vcs(self.ctx, int(m), float(tol), md, int(nadd), ptr(oi, _i64p), ptr(oR, _i32p),
+                                         ptr(os_, _f64p), ptr(cnt, _i64p), ptr(st, _f64p)), "lb_separate_vcs")
+        k = int(cnt[0])
+        return oi[:k].copy(), oR[:k].copy(), os_[:k].copy(), dict(nviol=st[0], mass=st[1], nviol_ws=st[2], mass_ws=st[3])
+
+    def get_vcs_state(self):
+        """(sig, age) of every VCS entry (concatenated over the levels)."""
+        tot = sum(v.nnz for v in self.inst.vcs)
+        sig = np.zeros(max(1, tot))
+        age = np.zeros(max(1, tot), np.uint8)
+        self._chk(self.L.lb_get_vcs_state(self.ctx, ptr(sig, _f64p), age.ctypes.data_as(C.c_void_p)), "lb_get_vcs_state")
+        return sig[:tot], age[:tot]
+
+    def set_vcs_age(self, age):
+        age = np.ascontiguousarray(age, np.uint8)
+        self._chk(self.L.lb_set_vcs_age(self.ctx, age.ctypes.data_as(C.c_void_p)), "lb_set_vcs_age")
+
+    def set_weights(self, pairlvl_weight, vc_weight=1.0):
+        self._chk(self.L.lb_set_weights(self.ctx, float(pairlvl_weight), float(vc_weight) * float(pairlvl_weight)),
+                  "lb_set_weights")
+
     def sync(self):
         self._chk(self.L.lb_sync(self.ctx), "lb_sync")
 

@@ -1,0 +1,81 @@
+"""Drop-in for lagbound's solver stage call (pipeline.Workspace.solve, pipeline.py:105-136).
+
+`solve(ws, ...)` takes the reference's Workspace (anything with `.data`, `.root`, `.runs` and `.run`-style
+subprocess conventions) and runs the device solver `python -m lagbound_amd.alm` with exactly the production flag
+set of the reference (main phase or proximal phase), then returns (last checkpoint path, pair weight) parsed from
+the `-> pairlvl_weight X` log line -- the same return value as Workspace.solve.
+"""
+from __future__ import annotations
+
+import os
+import re
+import subprocess
+import sys
+
+TAG = "g"
+
+
+class PipelineError(RuntimeError):
+    """The solver stage failed; the message ends with the tail of its log."""
+
+
+def solver_args(name, pairs, vc, lb, time_limit, outer, nbatch, runs_rel="runs", init=None, resume=None,
+                pair_weight=None, proximal=False, max_clusters=6000, ckpt_every=None, tag=TAG + "__rx"):
+    """The argument list of the production launch (pipeline.py:116-128)."""
+    args = ["--tag", tag, "--pair_labels"] + list(pairs) + [
+        "--max_clusters", max_clusters, "--k", 32, "--nbatch", nbatch, "--inner", 1, "--est_every", 0,
+        "--tri_max", 16000000, "--mom", 0, "--lb", lb, "--shift_p", 0, "--time_limit", time_limit, "--seed", 0,
+        "--vcs_labels", vc, "--vcs_max", 150000000, "--vcs_add", 10000000, "--outer", outer,
+        "--save", os.path.join(runs_rel, name + ".npz")]
+    if proximal:
+        args += ["--resume", resume, "--tri_add", 0, "--tri1_add", 0, "--tri_every", 1000000,
+                 "--pairlvl_weight", pair_weight, "--phase2_at_it", 1, "--phase2_w", pair_weight,
+                 "--phase2_inner", 4, "--phase2_tri_every", 1000000, "--phase2_est_every", 0,
+                 "--ckpt_every", ckpt_every or 20]
+    else:
+        args += ["--tri_add", 600000, "--tri1_add", 300000, "--tri_every", 9, "--init_partition", init,
+                 "--pw_rule", "default", "--ckpt_every", ckpt_every or 50]
+    return [str(x) for x in args]
+
+
+def solve(ws, name, pairs, vc, lb, time_limit, outer, nbatch, init=None, resume=None, pair_weight=None,
+          proximal=False, max_clusters=6000, ckpt_every=None, extra_args=(), k=None):
+    """Workspace.solve with the device solver. `extra_args` (e.g. ["--dtype", "bf16"]) are appended; `k` overrides
+    the rank (the device supports 8, 16, 32)."""
+    args = solver_args(name, pairs, vc, lb, time_limit, outer, nbatch, init=init, resume=resume,
+                       pair_weight=pair_weight, proximal=proximal, max_clusters=max_clusters, ckpt_every=ckpt_every)
+    if k is not None:
+        args[args.index("--k") + 1] = str(k)
+    args += [str(x) for x in extra_args]
+    env = dict(os.environ, DATA_DIR=ws.data + os.sep)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env["PYTHONPATH"] = root + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+    log_path = os.path.join(ws.runs, name + ".log")
+    with open(log_path, "w") as fh:
+        proc = subprocess.run([sys.executable, "-m", "lagbound_amd.alm"] + args, cwd=ws.root, env=env, stdout=fh,
+                              stderr=subprocess.STDOUT)
+    if proc.returncode != 0:
+        with open(log_path) as fh:
+            tail = "".join(fh.readlines()[-15:])
+        raise PipelineError(f"lagbound_amd.alm failed (exit {proc.returncode}); log {log_path}:\n{tail}")
+    weight = pair_weight
+    with open(log_path) as fh:
+        for line in fh:
+            hit = re.search(r"-> pairlvl_weight ([0-9.eE+-]+)", line)
+            if hit:
+                weight = float(hit.group(1))
+    return os.path.join(ws.runs, name + "_last.npz"), weight
+
+
+class Workspace:
+    """Minimal working directory with the reference's layout (data/<tag>/..., runs/) for running the solver stage
+    on an instance without the rest of the pipeline."""
+
+    def __init__(self, root):
+        self.root = os.path.abspath(root)
+        self.data = os.path.join(self.root, "data")
+        self.runs = os.path.join(self.root, "runs")
+        os.makedirs(self.runs, exist_ok=True)
+
+    def solve(self, *args, **kw):
+        return solve(self, *args, **kw)

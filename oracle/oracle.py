@@ -494,3 +494,59 @@ def _vcs_eff(vl, V, Y, d, rows, R, own, rho, wv, sig=None):
     with np.errstate(divide="ignore", invalid="ignore"):
         p = np.maximum(0.0, sig - rho * (wv * vl.rq[R] / (dd * dd)) * g)
     return np.where(dd > 0, p, sig)
+
+
+# ----------------------------------------------------------------------------------------------- VCS separation
+def vcs_separate_numpy(vl, V, d, wv, nadd, tol=1e-4, mode="pen"):
+    """vcsops.separate (vcsops.py:395-436) with the full candidate set: for every row i and cluster R not in the
+    working set, s = V_i . Y_R (minus d_i and vol minus d_i for the own cluster), violated if s / vol < -tol, score
+    pen = wv rq_R s^2 | mean = -s/vol | mass = -d_i s; returns the nadd best (score desc, ties by (i, R)) and the
+    statistics (#violated, violation mass, both over all entries and over the working set)."""
+    n, k = V.shape
+    P = vl.P
+    Y = _W(vl.lab, d, P) @ V
+    vol = np.maximum(np.bincount(vl.lab, weights=d, minlength=P), 1.0)
+    G = V @ Y.T
+    S = G.copy()
+    VR = np.broadcast_to(vol, (n, P)).copy()
+    S[np.arange(n), vl.lab] -= d
+    VR[np.arange(n), vl.lab] -= d
+    valid = VR > 0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        xb = np.where(valid, S / np.where(valid, VR, 1.0), 0.0)
+    viol = valid & (xb < -tol)
+    inws = np.zeros((n, P), bool)
+    rows = np.repeat(np.arange(n), np.diff(vl.ptr))
+    inws[rows, vl.col] = True
+    mass = -d[:, None] * S
+    stats = dict(nviol=float(viol.sum()), mass=float(mass[viol].sum()), nviol_ws=float((viol & inws).sum()),
+                 mass_ws=float(mass[viol & inws].sum()))
+    cand = viol & ~inws
+    if mode == "pen":
+        sc = wv * vl.rq[None, :] * S * S
+    elif mode == "mean":
+        sc = -xb
+    else:
+        sc = mass
+    cand &= sc > 0
+    ii, RR = np.nonzero(cand)
+    ss = sc[ii, RR]
+    o = np.lexsort((RR, ii, -ss))[:nadd]
+    return ii[o].astype(np.int64), RR[o].astype(np.int32), ss[o], stats
+
+
+def vcs_update_ws(vl, keep, ni, nR):
+    """update_ws / _merge (vcsops.py:439-464): drop entries with keep False, add (ni, nR) with sig 0 and age 0.
+    Returns (ptr, col, sig, age, row_old_index_or_-1)."""
+    n = len(vl.lab)
+    rows_old = np.repeat(np.arange(n), np.diff(vl.ptr))
+    ko = np.flatnonzero(keep)
+    r = np.r_[rows_old[ko], np.asarray(ni, np.int64)]
+    c = np.r_[vl.col[ko].astype(np.int64), np.asarray(nR, np.int64)]
+    src = np.r_[ko, np.full(len(ni), -1)]
+    o = np.lexsort((c, r))
+    r, c, src = r[o], c[o], src[o]
+    ptr = np.zeros(n + 1, np.int64)
+    np.add.at(ptr, r + 1, 1)
+    ptr = np.cumsum(ptr)
+    return ptr, c.astype(np.int32), src

@@ -1,0 +1,57 @@
+"""The device solver accepts the reference pipeline's production flag sets and rejects what it does not run."""
+import numpy as np
+import pytest
+
+from lagbound_amd import alm, pipeline
+from lagbound_amd import instances as I
+
+
+def test_production_flags_parse():
+    main = pipeline.solver_args("main", ["a.npy", "b.npy"], "c.npy", 0.5, 100.0, 3000, 512, init="p.npy")
+    prox = pipeline.solver_args("proximal", ["a.npy"], "c.npy", 0.5, 100.0, 3000, 512, resume="r.npz",
+                                pair_weight=0.01, proximal=True)
+    for args in (main, prox):
+        args = list(args)
+        args[args.index("--k") + 1] = "16"
+        a = alm.parse(args)
+        assert a.nbatch == 512 and a.est_every == 0 and a.vcs_add == 10000000
+    a = alm.parse([x if x != "32" else "16" for x in main])
+    assert a.tri_add == 600000 and a.tri1_add == 300000 and a.tri_every == 9 and a.pw_rule == "default"
+
+
+@pytest.mark.parametrize("bad", [["--mom", "0.5"], ["--est_every", "10"], ["--ne_split", "1"], ["--k", "12"],
+                                 ["--nbr_cut", "1"]])
+def test_unsupported_flags_rejected(bad):
+    with pytest.raises(SystemExit):
+        alm.parse(["--tag", "g", "--save", "x.npz", "--est_every", "0", "--k", "16"] + bad)
+
+
+def test_load_instance_reduced(tmp_path):
+    """Reduced-instance files (weights, vol, meta) load as the reference's A, d, L, C0 (alm_vcs.py:1103-1117)."""
+    import json
+    D = tmp_path / "g__rx"
+    D.mkdir()
+    n = 5
+    keys = np.array([0 * n + 1, 1 * n + 2, 2 * n + 3, 3 * n + 4, 0 * n + 4], np.int64)
+    w = np.array([1.0, 2.0, 1.0, 3.0, 1.0])
+    vol = np.array([3, 4, 4, 5, 6], np.int64)
+    json.dump({"n": n, "m": 5}, open(D / "graph.json", "w"))
+    np.save(D / "keys.npy", keys)
+    np.save(D / "weights.npy", w)
+    np.save(D / "vol.npy", vol)
+    json.dump({"L_orig": int(vol.sum()), "leaf_edges": 2, "inside_edges": 2}, open(D / "meta.json", "w"))
+    inst = alm.load_instance(str(D))
+    assert inst.n == 5 and inst.L == 22 and inst.C0 == 4.0 and np.array_equal(inst.d, vol)
+    A = inst.dense_A()
+    assert A[1, 2] == 2.0 and A[2, 1] == 2.0 and A[3, 4] == 3.0 and A.sum() == 2 * w.sum()
+
+
+def test_init_partition_colouring():
+    n_, keys, lab = I.sbm(400, 8, 0.2, 0.01, 1)
+    inst = I.build_instance(n_, keys)
+    V = alm.init_partition_V(inst, lab, 4, np.random.default_rng(0), lambda *s: None)
+    cols = np.argmax(V, axis=1)
+    # every cluster sits on one colour, and the noise is small
+    for q in range(8):
+        assert len(set(cols[lab == q].tolist())) == 1
+    assert np.abs(V - np.eye(4)[cols]).max() < 0.01

@@ -53,8 +53,11 @@ def test_small_uncapped(fam):
 
 
 def test_nothing_selected():
+    """No arm passes the threshold: every tail count is 0 <= cap, so the first bin qualifies (vmin = vtol, as the
+    reference's cum <= cap rule gives) and no candidate is returned."""
     inst, V = small_instance(n=40, seed=5, k=8)
     eng = _engine(inst, V, "f32", 8)
     TI, val, vmin = eng.separate_triangles(0, thresh=0.99999, cap=100)
-    assert len(TI) == 0 and vmin == -1.0
+    want = O.separate_c(inst, eng.get_V(), 0, thresh=0.99999, cap=100)
+    assert len(TI) == 0 and len(want[0]) == 0 and vmin == want[2] == 1e-3
     eng.close()
