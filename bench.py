@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the BF16 / FP64 / plain-FP64 comparison runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tuning", default="", help="heavy_edges,heavy_tris,seg_len,row_grid")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a plain copy kernel of known size (for scaling hardware byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}). "
@@ -166,6 +168,16 @@ def main():
                                          f"reference loop structure), median {med:.3f}s"}
         out["speedup_vs_cpu"] = round(value * med, 1)
         log(f"cpu oracle ({cores} threads): {med:.3f} s/eval")
+    if a.calibrate:
+        import ctypes
+        lp = ctypes.CDLL(os.path.join(ROOT, "lagbound_amd", "liblbprobe.so"))
+        lp.lbp_copy.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        cms = ctypes.c_double()
+        nbytes = 256 << 20
+        rc = lp.lbp_copy(nbytes, 5, ctypes.byref(cms))
+        out["copy_probe"] = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": round(cms.value, 5),
+                             "GBps": round(2 * nbytes / (cms.value * 1e-3) / 1e9, 1), "rc": rc}
+        log(f"copy probe: {2 * nbytes / (cms.value * 1e-3) / 1e9:.1f} GB/s (read+write)")
     eng.close()
     print(json.dumps(out), flush=True)
 

@@ -7,6 +7,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc", "lbhip.hip")
 OUT = os.path.join(HERE, "liblbhip.so")
+PROBE_SRC = os.path.join(HERE, "csrc", "probe.hip")
+PROBE_OUT = os.path.join(HERE, "liblbprobe.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-munsafe-fp-atomics",
@@ -15,16 +17,22 @@ HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
                "-Wno-unused-value", "-Wno-unused-result"]
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every HIP source of the package into liblbhip.so (always recompiles)."""
+def _compile(src, out, verbose):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc] + HIPCC_FLAGS + ["-I", INCLUDE, "-o", OUT + ".tmp", SRC]
+    cmd = [hipcc] + HIPCC_FLAGS + ["-I", INCLUDE, "-o", out + ".tmp", src]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + res.stderr[-4000:])
-    os.replace(OUT + ".tmp", OUT)
+    os.replace(out + ".tmp", out)
     if verbose:
         print(" ".join(cmd))
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every HIP source of the package (liblbhip.so, and the bandwidth probe liblbprobe.so used by the
+    benchmark's traffic calibration); always recompiles."""
+    _compile(SRC, OUT, verbose)
+    _compile(PROBE_SRC, PROBE_OUT, verbose)
     return OUT
 
 

@@ -32,6 +32,7 @@
 #define LB_MAXLEV 8
 #define LB_MAXVCS 4
 #define LB_RED_BLOCKS 512
+#define LB_VCS_BLOCKS 2048
 #define LB_BLOCK 256
 
 typedef uint16_t bf16raw;
@@ -68,9 +69,23 @@ template <> struct alignas(8) TriRec<double> { int32_t u, w, code, pad; double l
 
 struct Seg { int32_t row, kind; int64_t b, e; };
 
+// cross-lane exchange inside a quad (lanes 4k..4k+3) by DPP quad_perm: xor 1 = [1,0,3,2], xor 2 = [2,3,0,1]
+template <int M> __device__ inline float dpp_xor(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), M == 1 ? 0xB1 : 0x4E, 0xF, 0xF, false));
+}
+template <int M> __device__ inline double dpp_xor(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(u & 0xffffffffu), M == 1 ? 0xB1 : 0x4E, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), M == 1 ? 0xB1 : 0x4E, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// sum over the S consecutive lanes of a slot (S a power of two <= 64): DPP inside quads, shuffles beyond
 template <int S, typename T> __device__ inline T slot_sum(T v) {
+    if (S >= 2) v += dpp_xor<1>(v);
+    if (S >= 4) v += dpp_xor<2>(v);
 #pragma unroll
-    for (int m = 1; m < S; m <<= 1) v += __shfl_xor(v, m);
+    for (int m = 4; m < S; m <<= 1) v += __shfl_xor(v, m);
     return v;
 }
 
@@ -535,10 +550,14 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
             for (int64_t e = e0; e < e1; e += 8) {
                 const int64_t nE = e1 - e;
                 int64_t jj[8];
+                TC ww[8];
 #pragma unroll
                 for (int t = 0; t < 8; t++) {   // clamped index: always in bounds; lanes past the row reuse row i
-                    const int32_t cj = a.col[t < nE ? e + t : e1 - 1];
+                    const int64_t ec = t < nE ? e + t : e1 - 1;
+                    const int32_t cj = a.col[ec];
                     jj[t] = t < nE ? (int64_t)cj : i;
+                    ww[t] = wt ? wt[ec] : (TC)1;
+                    if (t >= nE) ww[t] = (TC)0;
                 }
                 uint4 u[8];
 #pragma unroll
@@ -547,10 +566,8 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
                 for (int t = 0; t < 8; t++) {
                     TC xj[CPL];
                     VT<TV>::unpack(u[t], xj);
-                    TC w = (TC)0;
-                    if (t < nE) w = wt ? wt[e + t] : (TC)1;
 #pragma unroll
-                    for (int c = 0; c < CPL; c++) g[c] += w * xj[c];
+                    for (int c = 0; c < CPL; c++) g[c] += ww[t] * xj[c];
                 }
             }
         }
@@ -610,17 +627,18 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
                 for (int t = 0; t < 4; t++) { uu[t] = a.V[(int64_t)rc[t].u * S + q]; uw[t] = a.V[(int64_t)rc[t].w * S + q]; }
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
-                    if (t < nK) {
-                        TC xu[CPL], xw[CPL], p, tau;
-                        VT<TV>::unpack(uu[t], xu);
-                        VT<TV>::unpack(uw[t], xw);
-                        tri_record_x<TV, R>(rc[t].code, rc[t].lam, x, xu, xw, a.rho, a.rt, g, p, tau);
-                        if (rc[t].code % 3 == 0 && q == 0) {
-                            pt[rc[t].code / 3] = p;
-                            if (a.rho > 0.0) {
-                                tot += ((double)p * p - (double)rc[t].lam * rc[t].lam) / (2 * a.rt);
-                                gmin = fmin(gmin, (double)((TC)1 - tau));
-                            }
+                    // records past the row's end are processed with lam = 0 and rho = 0: they add nothing
+                    const bool valid = t < nK;
+                    TC xu[CPL], xw[CPL], p, tau;
+                    VT<TV>::unpack(uu[t], xu);
+                    VT<TV>::unpack(uw[t], xw);
+                    tri_record_x<TV, R>(rc[t].code, valid ? rc[t].lam : (TC)0, x, xu, xw, valid ? a.rho : 0.0, a.rt, g,
+                                        p, tau);
+                    if (valid && rc[t].code % 3 == 0 && q == 0) {
+                        pt[rc[t].code / 3] = p;
+                        if (a.rho > 0.0) {
+                            tot += ((double)p * p - (double)rc[t].lam * rc[t].lam) / (2 * a.rt);
+                            gmin = fmin(gmin, (double)((TC)1 - tau));
                         }
                     }
                 }
@@ -1841,14 +1859,14 @@ struct Launch {
             va.Y = Yp; va.wv = c->wv; va.rho = rho;
             va.sumPv = c->sumPv; va.use_lds = vcs_lds; va.part = (double *)c->vcs_part.p; va.Zglob = (double *)c->Z.p;
             if (!vcs_lds) hipLaunchKernelGGL(lb_k_fill, dim3(256), dim3(256), 0, c->stream, (double *)c->Z.p, (int64_t)c->sumPv * R, 0.0);
-            hipLaunchKernelGGL((lb_k_vcs_z<TV, R>), dim3(LB_RED_BLOCKS), dim3(LB_BLOCK), vcs_lds ? ldsZ : 0, c->stream, va);
+            hipLaunchKernelGGL((lb_k_vcs_z<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), vcs_lds ? ldsZ : 0, c->stream, va);
         }
         mark();
         if (c->nvcs && vcs_lds) {
             const int width = c->sumPv * R;
             // Z entries sit after the two scalars of each block's partial row
             hipLaunchKernelGGL(lb_k_reduce, dim3((width + 63) / 64), dim3(1024), 0, c->stream,
-                               (const double *)c->vcs_part.p + 2, LB_RED_BLOCKS, width, vcs_width, (double *)c->Z.p);
+                               (const double *)c->vcs_part.p + 2, LB_VCS_BLOCKS, width, vcs_width, (double *)c->Z.p);
         }
         mark();
         // --- hub segments
@@ -1895,7 +1913,7 @@ struct Launch {
         FinArgs fa{};
         fa.R = R; fa.L = c->L; fa.C0 = c->C0; fa.z = zp; fa.lamsum = (const double *)c->lamsum.p;
         fa.lev_tot = (const double *)c->levtot.p; fa.lev_viol = (const double *)c->levviol.p; fa.sumPpair = c->sumPpair;
-        fa.vcs_part = (const double *)c->vcs_part.p; fa.nvcs_blk = c->nvcs ? LB_RED_BLOCKS : 0; fa.vcs_width = vcs_width;
+        fa.vcs_part = (const double *)c->vcs_part.p; fa.nvcs_blk = c->nvcs ? LB_VCS_BLOCKS : 0; fa.vcs_width = vcs_width;
         fa.row_part = (const double *)c->row_part.p; fa.nrow_blk = (int)nrow_blk;
         fa.seg_part = (const double *)c->seg_part.p; fa.nseg_blk = nseg ? nseg_blk : 0;
         fa.rho = rho; fa.parts = (double *)c->parts.p;
@@ -2316,7 +2334,7 @@ int ensure_eval_buffers(lb_ctx *c) {
     if ((rc = dalloc(c, c->levviol, sizeof(double) * std::max(c->sumPpair, 1)))) return rc;
     if ((rc = dalloc(c, c->Z, sizeof(double) * (size_t)std::max(c->sumPv, 1) * R))) return rc;
     const bool zlds = (sizeof(double) + (LB_BLOCK / 64) * c->cbytes) * (size_t)c->sumPv * R <= 48 * 1024;
-    if ((rc = dalloc(c, c->vcs_part, sizeof(double) * (size_t)LB_RED_BLOCKS * (2 + (zlds ? c->sumPv * R : 0))))) return rc;
+    if ((rc = dalloc(c, c->vcs_part, sizeof(double) * (size_t)LB_VCS_BLOCKS * (2 + (zlds ? c->sumPv * R : 0))))) return rc;
     const int S = (int)(R * c->vbytes / 16), NSL = LB_BLOCK / S;
     const int64_t nrow_blk = (c->n + NSL - 1) / NSL;
     if ((rc = dalloc(c, c->row_part, sizeof(double) * 4 * (size_t)std::max<int64_t>(nrow_blk, 1)))) return rc;
