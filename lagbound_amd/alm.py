@@ -28,6 +28,7 @@ import time
 import numpy as np
 
 from . import instances as I
+from .engine import Engine
 
 
 def parse(argv=None):
@@ -101,7 +102,6 @@ def parse(argv=None):
 
 class Solver:
     def __init__(self, a):
-        from .engine import Engine
         self.a = a
         self.t0 = time.time()
         D = os.environ.get("DATA_DIR", "data/")
@@ -266,8 +266,8 @@ class Solver:
             if room == 0:
                 ni, nR = ni[:0], nR[:0]
             ptr, col, src = _merge_ws(vl, keep, ni, nR)
-            sig = np.where(src >= 0, vl.sig[np.maximum(src, 0)], 0.0)
-            age = np.where(src >= 0, self.ages[m][np.maximum(src, 0)], 0).astype(np.uint8)
+            sig = _take_kept(vl.sig, src, np.float64)
+            age = _take_kept(self.ages[m], src, np.uint8)
             vl.ptr, vl.col, vl.sig = ptr, col, sig
             self.ages[m] = age
             changed = True
@@ -388,20 +388,21 @@ class Solver:
             parts = self.eng.eval(rho, rho * a.tri_weight)
             Lc, prim, nviol = parts[0], parts[6], int(parts[8])
             gmin = parts[9]
+            tviol = float(max(0.0, -gmin)) if inst.T and math.isfinite(gmin) else 0.0
+            sep_now = a.tri and it % a.tri_every == 0 and not self.stop
+            need_host = sep_now or (a.ckpt_every > 0 and it % a.ckpt_every == 0)
+            last = self.stop or it == a.outer or time.time() - self.t0 > a.time_limit
+            if need_host or last:
+                # the evaluation's PHR values are the multipliers after the update below
+                pp, pt, _ = self.fetch_multipliers()
+                self.sync_level_sig(pp)
+                inst.lam = pt.copy() if inst.T else np.zeros(0)
             self.eng.phr_update()                       # sig <- pp, lam <- pt, VCS sig <- pv, ages
             msg = ""
             if inst.vcs and not self.stop and it < a.outer and (it % a.vcs_every == 0 or it == a.vcs_first):
                 ts = time.time()
                 msg += self.vcs_separation()
                 tvcs += time.time() - ts
-            tviol = float(max(0.0, -gmin)) if inst.T and math.isfinite(gmin) else 0.0
-            sep_now = a.tri and it % a.tri_every == 0 and not self.stop
-            need_host = sep_now or (a.ckpt_every > 0 and it % a.ckpt_every == 0)
-            last = self.stop or it == a.outer or time.time() - self.t0 > a.time_limit
-            if need_host or last:
-                pp, pt, _ = self.fetch_multipliers()
-                self.sync_level_sig(pp)
-                inst.lam = pt.copy() if inst.T else np.zeros(0)
             if sep_now:
                 ts = time.time()
                 msg += self.tri_separation(inst.lam)
@@ -420,6 +421,14 @@ class Solver:
             if last:
                 break
         self.eng.close()
+
+
+def _take_kept(a, src, dtype):
+    """out[j] = a[src[j]] where src[j] >= 0 (a kept entry), 0 for a new entry; `a` may be empty."""
+    out = np.zeros(len(src), dtype)
+    k = src >= 0
+    out[k] = a[src[k]]
+    return out
 
 
 def _merge_ws(vl, keep, ni, nR):
@@ -484,7 +493,7 @@ def init_partition_V(inst, lab, k, rng, log):
     V[np.arange(n), col[lab]] = 1.0
     V += 1e-3 * rng.standard_normal(V.shape)
     coo = Cg.tocoo()
-    log(f"init partition: {P} clusters -> {k} colours, {int((col[coo.row] == col[coo.col]).sum())} clashing cluster "
+    log(f"init partition: {P} clusters -> {k} colours, {int(((col[coo.row] == col[coo.col]) & (coo.row < coo.col)).sum())} clashing cluster "
         f"adjacencies")
     return V
 

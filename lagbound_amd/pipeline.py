@@ -58,13 +58,17 @@ def solve(ws, name, pairs, vc, lb, time_limit, outer, nbatch, init=None, resume=
         with open(log_path) as fh:
             tail = "".join(fh.readlines()[-15:])
         raise PipelineError(f"lagbound_amd.alm failed (exit {proc.returncode}); log {log_path}:\n{tail}")
-    weight = pair_weight
     with open(log_path) as fh:
-        for line in fh:
-            hit = re.search(r"-> pairlvl_weight ([0-9.eE+-]+)", line)
-            if hit:
-                weight = float(hit.group(1))
+        weight = parse_weight(fh.read(), pair_weight)
     return os.path.join(ws.runs, name + "_last.npz"), weight
+
+
+def parse_weight(log_text, default=None):
+    """The last `-> pairlvl_weight X` value of a solver log (pipeline.py:131-135), or `default`."""
+    weight = default
+    for hit in re.finditer(r"-> pairlvl_weight ([0-9.eE+-]+)", log_text):
+        weight = float(hit.group(1))
+    return weight
 
 
 class Workspace:

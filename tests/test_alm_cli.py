@@ -55,3 +55,24 @@ def test_init_partition_colouring():
     for q in range(8):
         assert len(set(cols[lab == q].tolist())) == 1
     assert np.abs(V - np.eye(4)[cols]).max() < 0.01
+
+
+def test_merge_ws_from_empty_working_set():
+    """First VCS separation starts from an empty working set: kept values are gathered only for kept entries."""
+    from lagbound_amd.alm import _merge_ws, _take_kept
+    from lagbound_amd import instances as I
+    lab = np.array([0, 1, 1, 0, 2])
+    vl = I.VCSLevelData(lab=lab, P=3, rq=np.ones(3), ptr=np.zeros(6, np.int64), col=np.zeros(0, np.int32),
+                        sig=np.zeros(0))
+    ptr, col, src = _merge_ws(vl, np.zeros(0, bool), np.array([3, 0, 3]), np.array([2, 1, 0]))
+    assert ptr.tolist() == [0, 1, 1, 1, 3, 3]
+    assert col.tolist() == [1, 0, 2]
+    assert (src == -1).all()
+    assert _take_kept(vl.sig, src, np.float64).tolist() == [0.0, 0.0, 0.0]
+    # one kept entry among new ones
+    vl2 = I.VCSLevelData(lab=lab, P=3, rq=np.ones(3), ptr=np.array([0, 0, 1, 1, 2, 2]), col=np.array([2, 1], np.int32),
+                         sig=np.array([0.5, 0.25]))
+    ptr, col, src = _merge_ws(vl2, np.array([False, True]), np.array([1]), np.array([0]))
+    assert ptr.tolist() == [0, 0, 1, 1, 2, 2]
+    assert col.tolist() == [0, 1]
+    assert _take_kept(vl2.sig, src, np.float64).tolist() == [0.0, 0.25]
