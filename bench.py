@@ -168,6 +168,14 @@ def main():
                                          f"reference loop structure), median {med:.3f}s"}
         out["speedup_vs_cpu"] = round(value * med, 1)
         log(f"cpu oracle ({cores} threads): {med:.3f} s/eval")
+        t1s = []
+        for _ in range(2):                                  # the 1-thread figure (two evaluations)
+            t1 = time.perf_counter()
+            O.eval_c(inst, V, rho, threads=1, incidence=inc)
+            t1s.append(time.perf_counter() - t1)
+        out["cpu_baseline"]["single_thread"] = {"value": round(1.0 / min(t1s), 4), "unit": "evals/s", "cores": 1,
+                                                "sample": "2 full FP64 evaluations, best"}
+        log(f"cpu oracle (1 thread): {min(t1s):.3f} s/eval")
     if a.calibrate:
         import ctypes
         lp = ctypes.CDLL(os.path.join(ROOT, "lagbound_amd", "liblbprobe.so"))

@@ -1,9 +1,11 @@
-"""Drop-in for lagbound's solver stage call (pipeline.Workspace.solve, pipeline.py:105-136).
+"""Drop-in for lagbound's solver and estimate stage calls (pipeline.Workspace.solve, pipeline.py:105-136;
+Workspace.estimate, pipeline.py:156-170).
 
 `solve(ws, ...)` takes the reference's Workspace (anything with `.data`, `.root`, `.runs` and `.run`-style
 subprocess conventions) and runs the device solver `python -m lagbound_amd.alm` with exactly the production flag
 set of the reference (main phase or proximal phase), then returns (last checkpoint path, pair weight) parsed from
-the `-> pairlvl_weight X` log line -- the same return value as Workspace.solve.
+the `-> pairlvl_weight X` log line -- the same return value as Workspace.solve. `estimate(ws, ...)` runs
+`python -m lagbound_amd.estimate` with the reference's estimate arguments and returns its result JSON.
 """
 from __future__ import annotations
 
@@ -38,6 +40,42 @@ def solver_args(name, pairs, vc, lb, time_limit, outer, nbatch, runs_rel="runs",
     return [str(x) for x in args]
 
 
+def _run_module(ws, module, args, log_name):
+    """Run `python -m module args` in the workspace root with DATA_DIR set (Workspace.run, pipeline.py:58-75);
+    raise PipelineError with the log's tail on failure. Returns the log path."""
+    env = dict(os.environ, DATA_DIR=ws.data + os.sep)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env["PYTHONPATH"] = root + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+    log_path = os.path.join(ws.runs, log_name)
+    with open(log_path, "w") as fh:
+        proc = subprocess.run([sys.executable, "-m", module] + [str(x) for x in args], cwd=ws.root, env=env,
+                              stdout=fh, stderr=subprocess.STDOUT)
+    if proc.returncode != 0:
+        with open(log_path) as fh:
+            tail = "".join(fh.readlines()[-15:])
+        raise PipelineError(f"{module} failed (exit {proc.returncode}); log {log_path}:\n{tail}")
+    return log_path
+
+
+def estimate(ws, name, ckpt, lb, init_v=None, save_w=None, polish_time=900, eig_time=900, extra_args=(),
+             tag=TAG + "__rx"):
+    """Workspace.estimate with the device operator (pipeline.py:156-170): the reference's argument list, run as
+    `python -m lagbound_amd.estimate`; returns the result JSON as a dict with its path under "json"."""
+    out = os.path.join(ws.runs, name + ".json")
+    args = ["--tag", tag, "--ckpt", ckpt, "--lb", lb, "--polish", "rgd", "--polish_time", polish_time,
+            "--scale", "vol", "--eig_time", eig_time, "--out", out]
+    if init_v:
+        args += ["--initV", init_v]
+    if save_w:
+        args += ["--saveW", save_w]
+    _run_module(ws, "lagbound_amd.estimate", args + list(extra_args), name + ".log")
+    import json
+    with open(out) as fh:
+        res = json.load(fh)
+    res["json"] = out
+    return res
+
+
 def solve(ws, name, pairs, vc, lb, time_limit, outer, nbatch, init=None, resume=None, pair_weight=None,
           proximal=False, max_clusters=6000, ckpt_every=None, extra_args=(), k=None):
     """Workspace.solve with the device solver. `extra_args` (e.g. ["--dtype", "bf16"]) are appended; `k` overrides
@@ -47,17 +85,7 @@ def solve(ws, name, pairs, vc, lb, time_limit, outer, nbatch, init=None, resume=
     if k is not None:
         args[args.index("--k") + 1] = str(k)
     args += [str(x) for x in extra_args]
-    env = dict(os.environ, DATA_DIR=ws.data + os.sep)
-    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    env["PYTHONPATH"] = root + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
-    log_path = os.path.join(ws.runs, name + ".log")
-    with open(log_path, "w") as fh:
-        proc = subprocess.run([sys.executable, "-m", "lagbound_amd.alm"] + args, cwd=ws.root, env=env, stdout=fh,
-                              stderr=subprocess.STDOUT)
-    if proc.returncode != 0:
-        with open(log_path) as fh:
-            tail = "".join(fh.readlines()[-15:])
-        raise PipelineError(f"lagbound_amd.alm failed (exit {proc.returncode}); log {log_path}:\n{tail}")
+    log_path = _run_module(ws, "lagbound_amd.alm", args, name + ".log")
     with open(log_path) as fh:
         weight = parse_weight(fh.read(), pair_weight)
     return os.path.join(ws.runs, name + "_last.npz"), weight
@@ -83,3 +111,6 @@ class Workspace:
 
     def solve(self, *args, **kw):
         return solve(self, *args, **kw)
+
+    def estimate(self, *args, **kw):
+        return estimate(self, *args, **kw)

@@ -131,3 +131,21 @@ class OracleEngine:
     def separate_vcs(self, m=0, nadd=1000, tol=1e-4, mode="pen"):
         vl = self.inst.vcs[m]
         return O.vcs_separate_numpy(vl, self.V, self.inst.d, self.inst.wv, nadd, tol, mode)
+
+
+class OracleLagrangian:
+    """C(y) X at the instance's multipliers with the C oracle (evaluation at rho = 0, G = C(y) X); the CPU stand-in
+    for lagbound_amd.bound.DeviceLagrangian."""
+
+    def __init__(self, inst, r=16):
+        self.inst = inst
+        self.r = r
+        self.const = float(inst.C0) + (float(np.sum(inst.lam)) if inst.T else 0.0)
+        self.inc = O.incidence_c(inst.n, inst.TI) if inst.T else None
+
+    def CX(self, X):
+        X = np.ascontiguousarray(X, np.float64)
+        return O.eval_c(self.inst, X, 0.0, threads=2, incidence=self.inc)[1]
+
+    def close(self):
+        pass

@@ -41,3 +41,38 @@ def test_main_then_proximal_host(tmp_path, monkeypatch, capsys):
     ck2 = _run(monkeypatch, ws, "proximal", pairs, vc, 3, resume=ck, pair_weight=w, proximal=True)
     z2 = _check_ckpt(ck2, n, 8)
     assert len(z2["TI"]) <= len(z["TI"])
+
+
+def test_solve_estimate_certify_host(tmp_path, monkeypatch, capsys):
+    """Solver -> estimate stage (reference arguments, result JSON) -> dense certificate, on the CPU operators. The
+    certified bound must be at least the planted partition's modularity and within the rounding price plus the
+    margin of the estimate."""
+    from lagbound_amd import bound as B
+    from lagbound_amd import estimate as E
+    from oracle_engine import OracleLagrangian
+    from test_bound import _modularity
+    monkeypatch.setattr(alm, "Engine", OracleEngine)
+    monkeypatch.setattr(B, "DeviceLagrangian", lambda inst, r, dtype="f64", device=0: OracleLagrangian(inst, r))
+    ws = pipeline.Workspace(str(tmp_path))
+    D, n, keys = _write(str(tmp_path), n=240, blocks=4)
+    pairs = [os.path.join(D, "planted.npy"), os.path.join(D, "coarse.npy")]
+    vc = os.path.join(D, "planted.npy")
+    ck = _run(monkeypatch, ws, "main", pairs, vc, 10, init=os.path.join(D, "planted.npy"),
+              extra_args=["--tri_every", "3", "--vcs_every", "2", "--tri_add", "400", "--tri1_add", "200"])
+    out = os.path.join(ws.runs, "est.json")
+    assert E.main(["--tag", "g__rx", "--ckpt", ck, "--lb", "0.3", "--polish", "rgd", "--polish_time", "60",
+                   "--scale", "vol", "--eig_time", "60", "--out", out, "--rgd_iters", "40", "--certify", "1",
+                   "--saveW", os.path.join(ws.runs, "W.npz")]) == 0
+    res = json.load(open(out))
+    for key in ("dv0", "dv", "ub", "s_vol", "escape_rounds", "gap", "certificate"):
+        assert key in res, key
+    assert res["ub"] >= res["dv"] and res["dv"] >= res["dv0"] - 1e-12
+    cert = res["certificate"]
+    assert cert["passed"]
+    lab = np.load(os.path.join(D, "planted.npy"))
+    inst = alm.load_instance(D)
+    assert cert["UB"] >= _modularity(inst, lab)
+    sums = res["s_vol"]["sum_s"]
+    assert abs(cert["UB"] - res["ub"]) <= 1e-7 * sums / inst.L + cert["price"] + 1e-12
+    W = np.load(os.path.join(ws.runs, "W.npz"))["V"]
+    assert W.shape[0] == n and W.dtype == np.float32

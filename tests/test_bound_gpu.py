@@ -1,0 +1,73 @@
+"""Candidate bound on the device: C(y) X products against the oracle, the estimate against the oracle-operator
+estimate, |U_fp32 - U_fp64|, and the full chain solver -> estimate -> dense certificate through the drop-in stages."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from lagbound_amd import bound as B
+from lagbound_amd import instances as I
+from lagbound_amd import pipeline
+from alm_case import _write
+from oracle_engine import OracleLagrangian
+from test_bound import _modularity, _with_multipliers
+
+pytestmark = pytest.mark.gpu
+
+
+def _inst(n=600, seed=1):
+    n_, keys, lab = I.sbm(n, 6, 0.06, 0.005, seed)
+    return _with_multipliers(I.build_instance(n_, keys), 9, scale=0.01, ntri=200), lab
+
+
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-12), ("f32", 2e-6)])
+def test_device_CX_matches_oracle(dtype, tol):
+    inst, _ = _inst()
+    X = np.random.default_rng(0).standard_normal((inst.n, 20))
+    print(f"[bound] CX dtype={dtype} n={inst.n} width=20", flush=True)
+    lag = B.DeviceLagrangian(inst, r=8, dtype=dtype)
+    got = lag.CX(X)
+    lag.close()
+    ref = OracleLagrangian(inst, 8).CX(X)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err <= tol, err
+
+
+def test_estimate_device_matches_oracle_and_fp32():
+    inst, lab = _inst()
+    V = np.random.default_rng(2).standard_normal((inst.n, 8))
+    kw = dict(rgd_iters=40, eig_maxiter=120, log=lambda *a: None)
+    print("[bound] estimate f64 device vs oracle", flush=True)
+    ref = B.estimate(inst, V, lag=OracleLagrangian(inst, 8), **kw)
+    e64 = B.estimate(inst, V, dtype="f64", **kw)
+    assert abs(e64["dv"] - ref["dv"]) <= 1e-9 * abs(ref["dv"])
+    assert abs(e64["ub"] - ref["ub"]) <= 1e-7 * abs(ref["ub"])
+    print("[bound] estimate f32 device", flush=True)
+    e32 = B.estimate(inst, V, dtype="f32", **kw)
+    du = abs(e32["ub"] - e64["ub"])
+    print(f"[bound] U_f64 {e64['ub']:.9f} U_f32 {e32['ub']:.9f} |dU| {du:.3e}", flush=True)
+    assert du <= 1e-4 * abs(e64["ub"])
+    cert = B.certify_estimate(inst, e64)
+    assert cert["passed"] and cert["UB"] >= _modularity(inst, lab)
+
+
+def test_solve_estimate_certify_end_to_end(tmp_path):
+    ws = pipeline.Workspace(str(tmp_path))
+    D, n, keys = _write(str(tmp_path), n=2000, blocks=10)
+    pairs = [os.path.join(D, "planted.npy"), os.path.join(D, "coarse.npy")]
+    vc = os.path.join(D, "planted.npy")
+    print("[bound] solve n=2000", flush=True)
+    ck, w = ws.solve("main", pairs, vc, 0.5, 120.0, 30, 8, init=os.path.join(D, "planted.npy"), k=16,
+                     extra_args=["--tri_every", "5", "--vcs_every", "4", "--tri_add", "20000", "--tri1_add", "10000"])
+    print("[bound] estimate", flush=True)
+    res = ws.estimate("est", ck, 0.5, polish_time=120, eig_time=120, extra_args=["--certify", "1"])
+    cert = res["certificate"]
+    assert cert["passed"], cert
+    from lagbound_amd.alm import load_instance
+    inst = load_instance(D)
+    q = _modularity(inst, np.load(os.path.join(D, "planted.npy")))
+    print(f"[bound] Q(planted) {q:.6f} dv {res['dv']:.6f} ub {res['ub']:.6f} certified {cert['UB']:.9f} "
+          f"price {cert['price']:.2e}", flush=True)
+    assert cert["UB"] >= q
+    assert abs(cert["UB"] - res["ub"]) <= 1e-7 * res["s_vol"]["sum_s"] / inst.L + cert["price"] + 1e-12
