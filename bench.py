@@ -149,6 +149,28 @@ def main():
             log(f"plain fp64 gpu: {pf['ms_per_eval']:.3f} ms/eval")
         except Exception as exc:  # the comparison leg is reported, never silently replaced
             extra["plain_fp64_gpu"] = {"error": repr(exc)[:300]}
+        # exact reduction (device) on a 300k-vertex power-law graph with planted twins and leaves; the reference's
+        # reduction is a serial host loop, timed here as the single-threaded C oracle on the same graph
+        from lagbound_amd import reduce as RD
+        nr, kr = I.chung_lu(300000, 2.1, 13, 11)
+        nr, kr = I.add_twins_and_leaves(nr, kr, 5000, 20000, 12)
+        RD.reduce_graph(nr, kr)                                        # warm-up (module load, allocations)
+        tr = []
+        for _ in range(5):
+            t1 = time.perf_counter()
+            resr = RD.reduce_graph(nr, kr)
+            tr.append(time.perf_counter() - t1)
+        red = {"n": nr, "m": len(kr), "groups": resr["N"], "ms_gpu_call": round(1e3 * float(np.median(tr)), 3),
+               "note": "host keys in, host results out (PCIe and context set-up included)"}
+        if not a.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            t1 = time.perf_counter()
+            ro = O.reduce_c(nr, kr)
+            red["ms_cpu_oracle_1thread"] = round(1e3 * (time.perf_counter() - t1), 3)
+            red["bit_exact_vs_oracle"] = all(np.array_equal(np.asarray(ro[k]), np.asarray(resr[k])) for k in ro)
+        extra["reduction_300k"] = red
+        log(f"reduction n={nr} m={len(kr)}: {red['ms_gpu_call']:.2f} ms (device call)")
         out["variants"] = extra
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -143,6 +143,16 @@ int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int
 /* Library version string. */
 const char *lb_version(void);
 
+/* Exact optimum-preserving reduction (reduce_exact.py:12-62): merge leaves into their neighbour and adjacent true
+ * twins (N[u] = N[v], d^2 < 2m, d > 1). keys: m sorted unique u*n + v (u < v < n). rhash: n hash weights of the
+ * candidate filter (the reference's are numpy default_rng(11).integers(1, 2**62, n)) or NULL for the library's own;
+ * the result does not depend on them. Outputs (caller-allocated): group_of (n), vol (n; first N used), keys_out and
+ * w_out (m; first nkeys used: sorted unique group-pair keys a*N + b, a < b, and their edge counts), stats =
+ * {N, nkeys, leaf_unions, twin_unions, inside_edges}. Bit-exact with the reference. Uses the context's device and
+ * stream only (not its graph). */
+int lb_reduce_exact(lb_ctx *ctx, int64_t n, int64_t m, const int64_t *keys, const uint64_t *rhash, int64_t *group_of,
+                    int64_t *vol, int64_t *keys_out, double *w_out, int64_t *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2967,3 +2967,276 @@ int lb_time_eval(lb_ctx *c, double rho, double rt, int iters, double *ms_per_eva
 }
 
 }  // extern "C"
+
+// ================================================================================================ exact reduction
+// reduce_exact.py:12-62 on the device. The groups are the connected components of the eligible edges (leaf edges,
+// and adjacent true twins with d^2 < 2m, d > 1), each numbered by the rank of its smallest vertex: the reference's
+// union-find links par[max root] = min root, so its roots are the component minima, and np.unique(root) numbers them
+// in increasing order. The same components come out of a lock-free union-find that only ever hooks a larger root
+// under a smaller one (atomic compare-and-swap), so the result does not depend on the order of the unions. The twin
+// test compares the sorted adjacency rows exactly; the hash only filters candidates (a true twin pair always passes
+// it), so the hash weights do not change the result.
+namespace {
+
+__global__ void lb_k_red_deg(const int64_t *keys, int64_t m, int64_t n, unsigned long long *deg, int64_t *adjkey) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t u = keys[e] / n, v = keys[e] % n;
+        atomicAdd(&deg[u], 1ull);
+        atomicAdd(&deg[v], 1ull);
+        adjkey[2 * e] = u * n + v;                  // both directions; a radix sort orders them by (row, column)
+        adjkey[2 * e + 1] = v * n + u;
+    }
+}
+
+__global__ void lb_k_red_hash(const int64_t *ptr, const int64_t *adjkey, int64_t n, const uint64_t *rh, uint64_t *h) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t s = rh[x];
+        for (int64_t k = ptr[x]; k < ptr[x + 1]; k++) s += rh[adjkey[k] % n];
+        h[x] = s;
+    }
+}
+
+__global__ void lb_k_red_splitmix(uint64_t *rh, int64_t n) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = (uint64_t)x + 0x9E3779B97F4A7C15ull;   // splitmix64 (Steele, Lea, Flood 2014)
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        rh[x] = (z ^ (z >> 31)) | 1ull;
+    }
+}
+
+// flag[e]: 1 = leaf edge, 2 = verified twin pair, 0 = neither; cnt[0] += leaves, cnt[1] += twins
+__global__ void lb_k_red_edges(const int64_t *keys, int64_t m, int64_t n, const unsigned long long *deg,
+                               const int64_t *ptr, const int64_t *adjkey, const uint64_t *h, int8_t *flag,
+                               unsigned long long *cnt) {
+    unsigned long long nl = 0, nt = 0;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t u = keys[e] / n, v = keys[e] % n;
+        const int64_t du = (int64_t)deg[u], dv = (int64_t)deg[v];
+        int8_t f = 0;
+        if (du == 1 || dv == 1) {
+            f = 1;
+            nl++;
+        } else if (h[u] == h[v] && du == dv && (double)du * (double)du < 2.0 * (double)m && du > 1) {
+            // N[u] == N[v]: the sorted rows agree once v is dropped from row u and u from row v
+            int64_t a = ptr[u], ae = ptr[u + 1], b = ptr[v], be = ptr[v + 1];
+            bool same = true;
+            while (true) {
+                if (a < ae && adjkey[a] % n == v) a++;
+                if (b < be && adjkey[b] % n == u) b++;
+                if (a == ae || b == be) { same = (a == ae && b == be); break; }
+                if (adjkey[a] % n != adjkey[b] % n) { same = false; break; }
+                a++;
+                b++;
+            }
+            if (same) { f = 2; nt++; }
+        }
+        flag[e] = f;
+    }
+    if (nl) atomicAdd(&cnt[0], nl);
+    if (nt) atomicAdd(&cnt[1], nt);
+}
+
+__device__ __forceinline__ int64_t red_load(const int64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// root of x: parents strictly decrease along a path, so the walk ends within `limit` steps unless the forest is
+// broken, which is reported through bad[0]
+__device__ __forceinline__ int64_t red_find(const int64_t *par, int64_t x, int64_t limit, unsigned int *bad) {
+    for (int64_t s = 0; s <= limit; s++) {
+        const int64_t p = red_load(par + x);
+        if (p == x) return x;
+        x = p;
+    }
+    atomicOr(bad, 1u);
+    return x;
+}
+
+__global__ void lb_k_red_union(const int64_t *keys, int64_t m, int64_t n, const int8_t *flag, int64_t *par,
+                               unsigned int *bad) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+        if (!flag[e]) continue;
+        const int64_t a = keys[e] / n, b = keys[e] % n;
+        // each failed compare-and-swap means another thread hooked that root: the loop is lock-free; the bound
+        // only guards against a broken forest
+        for (int64_t tries = 0; tries <= n; tries++) {
+            const int64_t ra = red_find(par, a, n, bad), rb = red_find(par, b, n, bad);
+            if (ra == rb || *bad) break;
+            const int64_t lo = ra < rb ? ra : rb, hi = ra < rb ? rb : ra;
+            unsigned long long *slot = (unsigned long long *)(par + hi);
+            if (atomicCAS(slot, (unsigned long long)hi, (unsigned long long)lo) == (unsigned long long)hi) break;
+            if (tries == n) atomicOr(bad, 1u);
+        }
+    }
+}
+
+__global__ void lb_k_red_roots(const int64_t *par, int64_t n, int64_t *root, int64_t *isroot, unsigned int *bad) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = red_find(par, x, n, bad);
+        root[x] = r;
+        isroot[x] = (r == x) ? 1 : 0;
+    }
+}
+
+__global__ void lb_k_red_groups(const int64_t *root, const int64_t *rank, const unsigned long long *deg, int64_t n,
+                                int64_t *grp, unsigned long long *vol) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = rank[root[x]];
+        grp[x] = g;
+        if (deg[x]) atomicAdd(&vol[g], deg[x]);
+    }
+}
+
+// group-pair key of each cross edge; inside edges get INT64_MAX (sorted to the end) and are counted
+__global__ void lb_k_red_pairs(const int64_t *keys, int64_t m, int64_t n, const int64_t *grp, int64_t N,
+                               int64_t *pk, unsigned long long *cnt) {
+    unsigned long long ins = 0;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t gu = grp[keys[e] / n], gv = grp[keys[e] % n];
+        if (gu == gv) { pk[e] = INT64_MAX; ins++; }
+        else pk[e] = (gu < gv ? gu : gv) * N + (gu < gv ? gv : gu);
+    }
+    if (ins) atomicAdd(&cnt[2], ins);
+}
+
+__global__ void lb_k_red_counts(const int *rl, int64_t k, double *w) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < k; t += (int64_t)gridDim.x * blockDim.x)
+        w[t] = (double)rl[t];
+}
+
+int bits_for(uint64_t x) {
+    int b = 1;
+    while (b < 64 && (x >> b)) b++;
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lb_reduce_exact(lb_ctx *c, int64_t n, int64_t m, const int64_t *keys, const uint64_t *rhash, int64_t *group_of,
+                    int64_t *vol, int64_t *keys_out, double *w_out, int64_t *stats) {
+    if (!c) return -1;
+    if (n < 1 || m < 0 || (m && !keys) || !group_of || !vol || !stats || (m && (!keys_out || !w_out)))
+        return fail(c, -1, "lb_reduce_exact: bad arguments");
+    if ((double)n * (double)n >= 9.2e18 || 2 * m > (int64_t)INT32_MAX)
+        return fail(c, -1, "lb_reduce_exact: graph too large for 64-bit pair keys / 32-bit sort counts");
+    for (int64_t e = 0; e < m; e++) {                    // keys must be sorted, unique, u < v, in range
+        const int64_t u = keys[e] / n, v = keys[e] % n;
+        if (keys[e] < 0 || u >= n || u >= v || (e && keys[e] <= keys[e - 1]))
+            return fail(c, -1, "lb_reduce_exact: keys must be sorted unique u*n+v with u < v < n");
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    DBuf dkeys, deg, adjkey, adjkey2, ptr, rh, h, flag, par, root, isroot, rank, grp, dvol, pk, pk2, uk, rl, nrun, dw,
+        cnt, tmp;
+    auto cleanup = [&]() {
+        DBuf *bs[] = {&dkeys, &deg, &adjkey, &adjkey2, &ptr, &rh, &h, &flag, &par, &root, &isroot, &rank, &grp, &dvol,
+                      &pk, &pk2, &uk, &rl, &nrun, &dw, &cnt, &tmp};
+        for (DBuf *b : bs) dfree(*b);
+    };
+    int rc;
+#define RCHK(x) do { if ((rc = (x))) { cleanup(); return rc; } } while (0)
+#define RHIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { cleanup(); return fail(c, -3, std::string(#x) + ": " + hipGetErrorString(e_)); } } while (0)
+    hipStream_t s = c->stream;
+    const int G = 2048, B = 256;
+    const size_t M2 = (size_t)std::max<int64_t>(2 * m, 1);
+    RCHK(upload(c, dkeys, keys, sizeof(int64_t) * (size_t)m));
+    RCHK(dalloc(c, deg, sizeof(uint64_t) * (size_t)(n + 1)));
+    RCHK(dalloc(c, adjkey, sizeof(int64_t) * M2));
+    RCHK(dalloc(c, adjkey2, sizeof(int64_t) * M2));
+    RCHK(dalloc(c, ptr, sizeof(int64_t) * (size_t)(n + 1)));
+    RCHK(dalloc(c, cnt, 64));
+    RHIP(hipMemsetAsync(deg.p, 0, sizeof(uint64_t) * (size_t)(n + 1), s));
+    RHIP(hipMemsetAsync(cnt.p, 0, 64, s));
+    if (m) hipLaunchKernelGGL(lb_k_red_deg, dim3(G), dim3(B), 0, s, (const int64_t *)dkeys.p, m, n, (unsigned long long *)deg.p, (int64_t *)adjkey.p);
+    RHIP(hipGetLastError());
+    size_t tb = 0, tb2 = 0;
+    const int endbit = bits_for((uint64_t)n * (uint64_t)n);
+    // adjacency sorted by (row, column): one radix sort of the 2m directed keys
+    RHIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, (const int64_t *)adjkey.p, (int64_t *)adjkey2.p, (int)(2 * m), 0, endbit, s));
+    RHIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, (const unsigned long long *)deg.p, (unsigned long long *)ptr.p, (int)(n + 1), s));
+    RCHK(dalloc(c, tmp, std::max(tb, tb2)));
+    if (m) RHIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, (const int64_t *)adjkey.p, (int64_t *)adjkey2.p, (int)(2 * m), 0, endbit, s));
+    RHIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb2, (const unsigned long long *)deg.p, (unsigned long long *)ptr.p, (int)(n + 1), s));
+    // hash filter and the per-edge rules
+    RCHK(dalloc(c, rh, sizeof(uint64_t) * (size_t)n));
+    RCHK(dalloc(c, h, sizeof(uint64_t) * (size_t)n));
+    if (rhash) RHIP(hipMemcpyAsync(rh.p, rhash, sizeof(uint64_t) * (size_t)n, hipMemcpyHostToDevice, s));
+    else hipLaunchKernelGGL(lb_k_red_splitmix, dim3(G), dim3(B), 0, s, (uint64_t *)rh.p, n);
+    hipLaunchKernelGGL(lb_k_red_hash, dim3(G), dim3(B), 0, s, (const int64_t *)ptr.p, (const int64_t *)adjkey2.p, n, (const uint64_t *)rh.p, (uint64_t *)h.p);
+    RCHK(dalloc(c, flag, (size_t)std::max<int64_t>(m, 1)));
+    if (m) hipLaunchKernelGGL(lb_k_red_edges, dim3(G), dim3(B), 0, s, (const int64_t *)dkeys.p, m, n, (const unsigned long long *)deg.p,
+                              (const int64_t *)ptr.p, (const int64_t *)adjkey2.p, (const uint64_t *)h.p, (int8_t *)flag.p, (unsigned long long *)cnt.p);
+    RHIP(hipGetLastError());
+    // components: parents start as the identity
+    RCHK(dalloc(c, par, sizeof(int64_t) * (size_t)n));
+    RCHK(dalloc(c, root, sizeof(int64_t) * (size_t)n));
+    RCHK(dalloc(c, isroot, sizeof(int64_t) * (size_t)(n + 1)));
+    RCHK(dalloc(c, rank, sizeof(int64_t) * (size_t)(n + 1)));
+    hipLaunchKernelGGL(lb_k_iota64, dim3(G), dim3(B), 0, s, (int64_t *)par.p, n);
+    unsigned int *bad = (unsigned int *)((char *)cnt.p + 32);
+    if (m) hipLaunchKernelGGL(lb_k_red_union, dim3(G), dim3(B), 0, s, (const int64_t *)dkeys.p, m, n, (const int8_t *)flag.p, (int64_t *)par.p, bad);
+    RHIP(hipMemsetAsync(isroot.p, 0, sizeof(int64_t) * (size_t)(n + 1), s));
+    hipLaunchKernelGGL(lb_k_red_roots, dim3(G), dim3(B), 0, s, (const int64_t *)par.p, n, (int64_t *)root.p, (int64_t *)isroot.p, bad);
+    RHIP(hipGetLastError());
+    size_t tb3 = 0;
+    RHIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, (const int64_t *)isroot.p, (int64_t *)rank.p, (int)(n + 1), s));
+    RCHK(dalloc(c, tmp, tb3));
+    RHIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb3, (const int64_t *)isroot.p, (int64_t *)rank.p, (int)(n + 1), s));
+    int64_t N = 0;
+    unsigned int hbad = 0;
+    RHIP(hipMemcpyAsync(&N, (const int64_t *)rank.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    RHIP(hipMemcpyAsync(&hbad, bad, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    RHIP(hipStreamSynchronize(s));
+    if (hbad || N < 1 || N > n) { cleanup(); return fail(c, 1, "lb_reduce_exact: union-find did not converge"); }
+    RCHK(dalloc(c, grp, sizeof(int64_t) * (size_t)n));
+    RCHK(dalloc(c, dvol, sizeof(uint64_t) * (size_t)N));
+    RHIP(hipMemsetAsync(dvol.p, 0, sizeof(uint64_t) * (size_t)N, s));
+    hipLaunchKernelGGL(lb_k_red_groups, dim3(G), dim3(B), 0, s, (const int64_t *)root.p, (const int64_t *)rank.p, (const unsigned long long *)deg.p, n,
+                       (int64_t *)grp.p, (unsigned long long *)dvol.p);
+    RHIP(hipGetLastError());
+    // cross-group edges -> sorted unique group-pair keys with counts
+    int64_t nk = 0;
+    unsigned long long hc[3] = {0, 0, 0};
+    if (m) {
+        RCHK(dalloc(c, pk, sizeof(int64_t) * (size_t)m));
+        RCHK(dalloc(c, pk2, sizeof(int64_t) * (size_t)m));
+        RCHK(dalloc(c, uk, sizeof(int64_t) * (size_t)m));
+        RCHK(dalloc(c, rl, sizeof(int) * (size_t)m));
+        RCHK(dalloc(c, nrun, 16));
+        RCHK(dalloc(c, dw, sizeof(double) * (size_t)m));
+        hipLaunchKernelGGL(lb_k_red_pairs, dim3(G), dim3(B), 0, s, (const int64_t *)dkeys.p, m, n, (const int64_t *)grp.p, N, (int64_t *)pk.p,
+                           (unsigned long long *)cnt.p);
+        RHIP(hipGetLastError());
+        size_t t4 = 0, t5 = 0;
+        RHIP(hipcub::DeviceRadixSort::SortKeys(nullptr, t4, (const int64_t *)pk.p, (int64_t *)pk2.p, (int)m, 0, 64, s));
+        RHIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, t5, (const int64_t *)pk2.p, (int64_t *)uk.p, (int *)rl.p, (int *)nrun.p, (int)m, s));
+        RCHK(dalloc(c, tmp, std::max(t4, t5)));
+        RHIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, t4, (const int64_t *)pk.p, (int64_t *)pk2.p, (int)m, 0, 64, s));
+        RHIP(hipMemcpyAsync(hc, cnt.p, sizeof(hc), hipMemcpyDeviceToHost, s));
+        RHIP(hipStreamSynchronize(s));
+        const int64_t ncross = m - (int64_t)hc[2];
+        if (ncross > 0) {
+            RHIP(hipcub::DeviceRunLengthEncode::Encode(tmp.p, t5, (const int64_t *)pk2.p, (int64_t *)uk.p, (int *)rl.p, (int *)nrun.p, (int)ncross, s));
+            int hn = 0;
+            RHIP(hipMemcpyAsync(&hn, nrun.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            RHIP(hipStreamSynchronize(s));
+            nk = hn;
+            hipLaunchKernelGGL(lb_k_red_counts, dim3(G), dim3(B), 0, s, (const int *)rl.p, nk, (double *)dw.p);
+            RHIP(hipGetLastError());
+            RHIP(hipMemcpyAsync(keys_out, uk.p, sizeof(int64_t) * (size_t)nk, hipMemcpyDeviceToHost, s));
+            RHIP(hipMemcpyAsync(w_out, dw.p, sizeof(double) * (size_t)nk, hipMemcpyDeviceToHost, s));
+        }
+    }
+    RHIP(hipMemcpyAsync(group_of, grp.p, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToHost, s));
+    RHIP(hipMemcpyAsync(vol, dvol.p, sizeof(int64_t) * (size_t)N, hipMemcpyDeviceToHost, s));
+    RHIP(hipStreamSynchronize(s));
+    stats[0] = N; stats[1] = nk; stats[2] = (int64_t)hc[0]; stats[3] = (int64_t)hc[1]; stats[4] = (int64_t)hc[2];
+    cleanup();
+#undef RCHK
+#undef RHIP
+    return 0;
+}
+
+}  // extern "C"

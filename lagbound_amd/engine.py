@@ -200,6 +200,26 @@ class Engine:
         self._chk(self.L.lb_set_weights(self.ctx, float(pairlvl_weight), float(vc_weight) * float(pairlvl_weight)),
                   "lb_set_weights")
 
+    def reduce_exact(self, n, keys, rhash=None):
+        """Exact reduction of the graph with sorted unique keys u*n+v (u < v) on this context's device
+        (reduce_exact.py:12-62). Returns dict(group_of, N, vol, keys, weights, leaf_unions, twin_unions,
+        inside_edges)."""
+        keys = np.ascontiguousarray(keys, np.int64)
+        m = len(keys)
+        g = np.zeros(max(1, n), np.int64)
+        vol = np.zeros(max(1, n), np.int64)
+        kk = np.zeros(max(1, m), np.int64)
+        w = np.zeros(max(1, m))
+        st = np.zeros(5, np.int64)
+        rh = None if rhash is None else np.ascontiguousarray(rhash, np.uint64)
+        self._chk(self.L.lb_reduce_exact(self.ctx, int(n), m, ptr(keys, _i64p),
+                                         None if rh is None else rh.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         ptr(g, _i64p), ptr(vol, _i64p), ptr(kk, _i64p), ptr(w, _f64p),
+                                         ptr(st, _i64p)), "lb_reduce_exact")
+        N, nk = int(st[0]), int(st[1])
+        return dict(group_of=g[:n].copy(), N=N, vol=vol[:N].copy(), keys=kk[:nk].copy(), weights=w[:nk].copy(),
+                    leaf_unions=int(st[2]), twin_unions=int(st[3]), inside_edges=int(st[4]))
+
     def sync(self):
         self._chk(self.L.lb_sync(self.ctx), "lb_sync")
 

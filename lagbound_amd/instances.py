@@ -385,3 +385,37 @@ def metric_instance(n: int = 1_000_000, gamma: float = 2.1, kbar: float = 13.0, 
     inst.lam = lam
     V = init_V(n_, 16, np.random.default_rng(s_v))
     return inst, V
+
+
+def add_twins_and_leaves(n: int, keys: np.ndarray, ntwin: int, nleaf: int, seed: int, clique: int = 3):
+    """Plant reduction structure into a graph (keys u*n+v, u < v): `ntwin` groups of `clique` new vertices that are
+    true twins of a random low-degree vertex x (adjacent to x, to each other and to every neighbour of x), and
+    `nleaf` new degree-1 vertices attached to random vertices. Returns (n', sorted unique keys)."""
+    rng = np.random.default_rng(seed)
+    u, v = keys // n, keys % n
+    nb = [[] for _ in range(n)]
+    for a, b in zip(u.tolist(), v.tolist()):
+        nb[a].append(b)
+        nb[b].append(a)
+    deg = np.array([len(x) for x in nb])
+    low = np.flatnonzero((deg >= 1) & (deg <= 4))
+    new = []
+    nxt = n
+    for x in rng.choice(low, size=min(ntwin, len(low)), replace=False).tolist() if len(low) else []:
+        grp = [x] + list(range(nxt, nxt + clique - 1))
+        nxt += clique - 1
+        for i, a in enumerate(grp):
+            for b in grp[i + 1:]:
+                new.append((a, b))
+        for y in nb[x]:
+            for a in grp[1:]:
+                new.append((y, a))
+    hosts = rng.integers(0, nxt, size=nleaf)
+    for h in hosts.tolist():
+        new.append((h, nxt))
+        nxt += 1
+    n2 = nxt
+    a = np.r_[u, [p for p, _ in new]].astype(np.int64)
+    b = np.r_[v, [q for _, q in new]].astype(np.int64)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    return n2, np.unique(lo * n2 + hi)

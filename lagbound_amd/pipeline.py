@@ -1,5 +1,5 @@
-"""Drop-in for lagbound's solver and estimate stage calls (pipeline.Workspace.solve, pipeline.py:105-136;
-Workspace.estimate, pipeline.py:156-170).
+"""Drop-in for lagbound's reduction, solver and estimate stage calls (pipeline.Workspace.reduce, pipeline.py:96-103;
+Workspace.solve, pipeline.py:105-136; Workspace.estimate, pipeline.py:156-170).
 
 `solve(ws, ...)` takes the reference's Workspace (anything with `.data`, `.root`, `.runs` and `.run`-style
 subprocess conventions) and runs the device solver `python -m lagbound_amd.alm` with exactly the production flag
@@ -55,6 +55,13 @@ def _run_module(ws, module, args, log_name):
             tail = "".join(fh.readlines()[-15:])
         raise PipelineError(f"{module} failed (exit {proc.returncode}); log {log_path}:\n{tail}")
     return log_path
+
+
+def reduce(ws, label_files, tag=TAG):
+    """Workspace.reduce with the device reduction (pipeline.py:96-99): `python -m lagbound_amd.reduce <tag> labels`
+    in the workspace root; returns the reduced tag."""
+    _run_module(ws, "lagbound_amd.reduce", [tag] + list(label_files), "reduce.log")
+    return tag + "__rx"
 
 
 def estimate(ws, name, ckpt, lb, init_v=None, save_w=None, polish_time=900, eig_time=900, extra_args=(),
@@ -114,3 +121,10 @@ class Workspace:
 
     def estimate(self, *args, **kw):
         return estimate(self, *args, **kw)
+
+    def reduce(self, label_files):
+        return reduce(self, label_files)
+
+    def reduced_labels(self, label_file, tag=TAG):
+        """Path of a label file restricted to the reduced instance (pipeline.py:101-103)."""
+        return os.path.join(self.data, tag + "__rx", os.path.basename(label_file))

@@ -825,3 +825,94 @@ int64_t or_separate(int fam, int64_t n, const int64_t *rowptr, const int32_t *co
     free(hist); free(counts); free(offs); free(o2); free(v2);
     return kept;
 }
+
+/* ------------------------------------------------------------------------------------------------ exact reduction
+ * reduce_exact.py:12-62. keys: m sorted keys u*n + v (u < v); rhash: the n hash weights of the twin candidate filter
+ * (reduce_exact.py:31, numpy default_rng(11).integers(1, 2**62, n)). Union-find with par[max] = min (27-29), leaf
+ * rule over the edges in key order (31-33), candidate twins (h[u] == h[v], d[u] == d[v], d^2 < 2m, d > 1) checked
+ * exactly on the closed neighbourhoods in key order (34-37), groups = rank of the root among the sorted roots (38),
+ * volumes, inside edges and the sorted unique group-pair keys with their counts (39-42).
+ * Outputs: group_of (n), vol (n, first N used), kk / w (m, first nkeys used), stats = {N, nkeys, leaf_unions,
+ * twin_unions, inside_edges}. Returns 0. */
+static int64_t or_find(int64_t *par, int64_t x) {
+    int64_t r = x;
+    while (par[r] != r) r = par[r];
+    while (par[x] != r) { int64_t nx = par[x]; par[x] = r; x = nx; }
+    return r;
+}
+
+static void or_union(int64_t *par, int64_t a, int64_t b) {
+    int64_t ra = or_find(par, a), rb = or_find(par, b);
+    if (ra != rb) par[ra > rb ? ra : rb] = ra < rb ? ra : rb;
+}
+
+static int or_cmp_i64(const void *a, const void *b) {
+    int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+int or_reduce(int64_t n, int64_t m, const int64_t *keys, const uint64_t *rhash, int64_t *group_of, int64_t *vol,
+              int64_t *kk, double *w, int64_t *stats) {
+    int64_t *d = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    int64_t *ptr = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    int64_t *adj = (int64_t *)malloc(sizeof(int64_t) * (size_t)(2 * m + 1));
+    int64_t *fill = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    int64_t *par = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    uint64_t *h = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(n + 1));
+    for (int64_t e = 0; e < m; e++) { d[keys[e] / n]++; d[keys[e] % n]++; }
+    for (int64_t i = 0; i < n; i++) ptr[i + 1] = ptr[i] + d[i];
+    for (int64_t e = 0; e < m; e++) {
+        int64_t u = keys[e] / n, v = keys[e] % n;
+        adj[ptr[u] + fill[u]++] = v;
+        adj[ptr[v] + fill[v]++] = u;
+    }
+    for (int64_t i = 0; i < n; i++) qsort(adj + ptr[i], (size_t)d[i], sizeof(int64_t), or_cmp_i64);
+    for (int64_t i = 0; i < n; i++) par[i] = i;
+    int64_t nL = 0, nT = 0;
+    for (int64_t e = 0; e < m; e++) {                       /* rule L */
+        int64_t u = keys[e] / n, v = keys[e] % n;
+        if (d[u] == 1 || d[v] == 1) { or_union(par, u, v); nL++; }
+    }
+    for (int64_t i = 0; i < n; i++) h[i] = rhash[i];
+    for (int64_t e = 0; e < m; e++) {
+        int64_t u = keys[e] / n, v = keys[e] % n;
+        h[u] += rhash[v]; h[v] += rhash[u];
+    }
+    for (int64_t e = 0; e < m; e++) {                       /* rule T */
+        int64_t u = keys[e] / n, v = keys[e] % n;
+        if (!(h[u] == h[v] && d[u] == d[v] && (double)d[u] * (double)d[u] < 2.0 * (double)m && d[u] > 1)) continue;
+        /* N[u] == N[v] for adjacent u, v: the sorted rows agree once v is dropped from row u and u from row v */
+        int64_t a = ptr[u], ae = ptr[u + 1], b = ptr[v], be = ptr[v + 1], same = 1;
+        while (1) {
+            if (a < ae && adj[a] == v) a++;
+            if (b < be && adj[b] == u) b++;
+            if (a == ae || b == be) { same = (a == ae && b == be); break; }
+            if (adj[a] != adj[b]) { same = 0; break; }
+            a++; b++;
+        }
+        if (same) { or_union(par, u, v); nT++; }
+    }
+    int64_t *root = fill;                                   /* reuse */
+    for (int64_t x = 0; x < n; x++) root[x] = or_find(par, x);
+    int64_t *rank = ptr;                                    /* reuse: rank of each root id */
+    int64_t N = 0;
+    for (int64_t x = 0; x < n; x++) rank[x] = (root[x] == x) ? N++ : -1;
+    for (int64_t x = 0; x < n; x++) group_of[x] = rank[root[x]];
+    for (int64_t g = 0; g < N; g++) vol[g] = 0;
+    for (int64_t x = 0; x < n; x++) vol[group_of[x]] += d[x];
+    int64_t C0e = 0, nc = 0;
+    for (int64_t e = 0; e < m; e++) {
+        int64_t gu = group_of[keys[e] / n], gv = group_of[keys[e] % n];
+        if (gu == gv) { C0e++; continue; }
+        adj[nc++] = (gu < gv ? gu : gv) * N + (gu < gv ? gv : gu);
+    }
+    qsort(adj, (size_t)nc, sizeof(int64_t), or_cmp_i64);
+    int64_t nk = 0;
+    for (int64_t t = 0; t < nc; t++) {
+        if (nk > 0 && kk[nk - 1] == adj[t]) { w[nk - 1] += 1.0; continue; }
+        kk[nk] = adj[t]; w[nk] = 1.0; nk++;
+    }
+    stats[0] = N; stats[1] = nk; stats[2] = nL; stats[3] = nT; stats[4] = C0e;
+    free(d); free(ptr); free(adj); free(fill); free(par); free(h);
+    return 0;
+}

@@ -64,6 +64,8 @@ def lib():
                                   C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int, C.c_int, _i64p, _f64p, _f64p,
                                   _i64p]
         L.or_separate.restype = C.c_int64
+        L.or_reduce.argtypes = [C.c_int64, C.c_int64, _i64p, C.POINTER(C.c_uint64), _i64p, _i64p, _i64p, _f64p, _i64p]
+        L.or_reduce.restype = C.c_int
         _lib = L
     return _lib
 
@@ -550,3 +552,63 @@ def vcs_update_ws(vl, keep, ni, nR):
     np.add.at(ptr, r + 1, 1)
     ptr = np.cumsum(ptr)
     return ptr, c.astype(np.int32), src
+
+
+# ----------------------------------------------------------------------------------------------- exact reduction
+def reduce_hash(n):
+    """The twin-filter hash weights of reduce_exact.py:31 (numpy PCG64, seed 11)."""
+    rng = np.random.default_rng(11)
+    return rng.integers(1, 2 ** 62, size=n, dtype=np.int64).astype(np.uint64)
+
+
+def reduce_c(n, keys, rhash=None):
+    """C restatement of reduce_exact.py:12-62. Returns dict(group_of, N, vol, keys, weights, leaf_unions,
+    twin_unions, inside_edges)."""
+    L_ = lib()
+    keys = np.ascontiguousarray(keys, np.int64)
+    m = len(keys)
+    rh = np.ascontiguousarray(reduce_hash(n) if rhash is None else rhash, np.uint64)
+    g = np.zeros(max(1, n), np.int64)
+    vol = np.zeros(max(1, n), np.int64)
+    kk = np.zeros(max(1, m), np.int64)
+    w = np.zeros(max(1, m))
+    st = np.zeros(5, np.int64)
+    rc = L_.or_reduce(n, m, _p(keys, _i64p), rh.ctypes.data_as(C.POINTER(C.c_uint64)), _p(g, _i64p), _p(vol, _i64p),
+                      _p(kk, _i64p), _p(w, _f64p), _p(st, _i64p))
+    assert rc == 0
+    N, nk = int(st[0]), int(st[1])
+    return dict(group_of=g[:n], N=N, vol=vol[:N], keys=kk[:nk], weights=w[:nk], leaf_unions=int(st[2]),
+                twin_unions=int(st[3]), inside_edges=int(st[4]))
+
+
+def reduce_numpy(n, keys):
+    """Independent restatement of the reduction's result: the groups are the connected components of the eligible
+    edges (leaf edges; adjacent true twins with d^2 < 2m, d > 1, tested on the closed neighbourhoods as sets,
+    without any hash), numbered by their smallest vertex in increasing order (scipy connected_components)."""
+    import scipy.sparse.csgraph as cg
+    keys = np.asarray(keys, np.int64)
+    m = len(keys)
+    u, v = keys // n, keys % n
+    d = np.bincount(np.r_[u, v], minlength=n)
+    nb = [set() for _ in range(n)]
+    for a, b in zip(u.tolist(), v.tolist()):
+        nb[a].add(b)
+        nb[b].add(a)
+    leaf = (d[u] == 1) | (d[v] == 1)
+    twin = np.array([d[a] == d[b] and d[a] > 1 and float(d[a]) ** 2 < 2 * m and (nb[a] | {a}) == (nb[b] | {b})
+                     for a, b in zip(u.tolist(), v.tolist())], bool) if m else np.zeros(0, bool)
+    el = leaf | twin
+    G = sp.csr_matrix((np.ones(int(el.sum())), (u[el], v[el])), shape=(n, n))
+    nc, lab = cg.connected_components(G, directed=False)
+    first = np.full(nc, n)
+    np.minimum.at(first, lab, np.arange(n))
+    order = np.argsort(first)
+    rank = np.empty(nc, np.int64)
+    rank[order] = np.arange(nc)
+    grp = rank[lab]
+    vol = np.bincount(grp, weights=d, minlength=nc).astype(np.int64)
+    gu, gv = grp[u], grp[v]
+    inside = gu == gv
+    kk, w = np.unique(np.minimum(gu, gv)[~inside] * nc + np.maximum(gu, gv)[~inside], return_counts=True)
+    return dict(group_of=grp, N=nc, vol=vol, keys=kk.astype(np.int64), weights=w.astype(np.float64),
+                leaf_unions=int(leaf.sum()), twin_unions=int(twin.sum()), inside_edges=int(inside.sum()))
