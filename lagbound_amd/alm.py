@@ -40,7 +40,7 @@ def parse(argv=None):
     p.add_argument("--init_V", default="")
     p.add_argument("--k", type=int, default=32)
     p.add_argument("--seed", type=int, default=0)
-    p.add_argument("--save", required=True)
+    p.add_argument("--save", default="")
     p.add_argument("--save_last", type=int, default=1)
     p.add_argument("--outer", type=int, default=10 ** 9)
     p.add_argument("--inner", type=int, default=3)
@@ -101,24 +101,31 @@ def parse(argv=None):
 
 
 class Solver:
-    def __init__(self, a):
+    """The solver stage. From the command line it reads DATA_DIR/<tag> and the label files named by the flags. In
+    process, `inst` (graph only), `labels` / `vcs_labels` (label arrays) and `init_labels` replace the files;
+    `log` replaces the timestamped print; `signals` False leaves SIGTERM / SIGINT alone (worker threads)."""
+
+    def __init__(self, a, inst=None, labels=None, vcs_labels=None, init_labels=None, log=None, signals=True):
         self.a = a
         self.t0 = time.time()
-        D = os.environ.get("DATA_DIR", "data/")
-        if not D.endswith("/"):
-            D += "/"
-        self.inst = load_instance(D + a.tag)
-        inst = self.inst
+        if log is not None:
+            self.log = log
+        if inst is None:
+            D = os.environ.get("DATA_DIR", "data/")
+            if not D.endswith("/"):
+                D += "/"
+            inst = load_instance(D + a.tag)
+        self.inst = inst
         n = inst.n
         self.log(f"loaded {a.tag}: n {n} nnz {inst.nnz} L {inst.L:g} C0/L {inst.C0 / inst.L:.6f}")
         z0 = np.load(a.resume) if a.resume else None
         # pair levels (alm_vcs.py:1125-1138, 1254-1259)
-        if z0 is not None and not a.pair_labels:
+        if z0 is not None and not a.pair_labels and labels is None:
             labs = [z0[f"lab{li}"] for li in range(int(z0["nlev"]))]
         else:
             labs = []
-            for f in a.pair_labels:
-                lab = np.unique(np.load(f), return_inverse=True)[1]
+            for f in (labels if labels is not None else a.pair_labels):
+                lab = np.unique(np.load(f) if isinstance(f, str) else np.asarray(f), return_inverse=True)[1]
                 if a.max_clusters and lab.max() + 1 > a.max_clusters:
                     vol_ = np.bincount(lab, weights=inst.d)
                     keep_ = np.argsort(-vol_)[:a.max_clusters - 1]
@@ -132,8 +139,9 @@ class Solver:
         for lv in inst.levels:
             self.log("pair level clusters", lv.P)
         # sparse VC levels (alm_vcs.py:1155-1167): empty working sets, or the checkpoint's
-        vfiles = list(a.vcs_labels)
-        inst.vcs = [I.vcs_level(np.load(f), inst.d, np.zeros(n + 1, np.int64), np.zeros(0, np.int32)) for f in vfiles]
+        vfiles = list(vcs_labels) if vcs_labels is not None else list(a.vcs_labels)
+        inst.vcs = [I.vcs_level(np.load(f) if isinstance(f, str) else np.asarray(f), inst.d, np.zeros(n + 1, np.int64),
+                                np.zeros(0, np.int32)) for f in vfiles]
         self.ages = [np.zeros(0, np.uint8) for _ in inst.vcs]
         if z0 is not None and "nvcs" in z0.files:
             if not inst.vcs:
@@ -160,8 +168,9 @@ class Solver:
                 if f"sig{li}" in z0.files and z0[f"sig{li}"].shape == lv.sig.shape:
                     lv.sig = z0[f"sig{li}"].astype(np.float64)
             z0.close()
-        elif a.init_partition:
-            V = init_partition_V(inst, np.load(a.init_partition), a.k, self.rng, self.log)
+        elif init_labels is not None or a.init_partition:
+            part = np.asarray(init_labels) if init_labels is not None else np.load(a.init_partition)
+            V = init_partition_V(inst, part, a.k, self.rng, self.log)
         elif a.init_V:
             V = np.load(a.init_V).astype(np.float64)
         else:
@@ -178,8 +187,9 @@ class Solver:
         if (a.pw_rule) and inst.levels:
             self.pw_rule(V)
         self.stop = False
-        signal.signal(signal.SIGTERM, self._stop)
-        signal.signal(signal.SIGINT, self._stop)
+        if signals:
+            signal.signal(signal.SIGTERM, self._stop)
+            signal.signal(signal.SIGINT, self._stop)
 
     # ------------------------------------------------------------------------------------------------ helpers
     def log(self, *s):
@@ -414,13 +424,24 @@ class Solver:
             self.log(f"it {it} L {Lc / L:.6f} prim {prim / L:.6f} viol {nviol} tviol {tviol:.3f} ntri {inst.T}{msg} "
                      f"[sweep {tsweep:.0f}s sep {tsep:.0f}s vcs {tvcs:.0f}s]")
             hist.append(rec)
-            if last or (a.ckpt_every > 0 and it % a.ckpt_every == 0):
+            if a.save and (last or (a.ckpt_every > 0 and it % a.ckpt_every == 0)):
                 self.save(a.save.replace(".npz", "_last.npz"))
-            if it % 20 == 0 or last:
+            if a.save and (it % 20 == 0 or last):
                 json.dump(hist, open(a.save.replace(".npz", ".history.json"), "w"))
             if last:
                 break
+        # final state on the host: V and the multipliers after the last PHR update (levels and lam were fetched
+        # above at the last outer; the VCS multipliers are read here)
+        self.V = self.eng.get_V()
+        if inst.vcs and sum(v.nnz for v in inst.vcs):
+            sig_all, _ = self.eng.get_vcs_state()
+            o = 0
+            for vl in inst.vcs:
+                vl.sig = sig_all[o:o + vl.nnz].copy()
+                o += vl.nnz
+        self.history = hist
         self.eng.close()
+        return hist
 
 
 def _take_kept(a, src, dtype):
@@ -500,6 +521,8 @@ def init_partition_V(inst, lab, k, rng, log):
 
 def main(argv=None):
     a = parse(argv)
+    if not a.save:
+        raise SystemExit("--save is required")
     s = Solver(a)
     s.run()
     return 0

@@ -28,6 +28,11 @@ from fractions import Fraction
 
 import numpy as np
 
+# LOBPCG runs in chunks of a few iterations and the chunk loop checks convergence itself: scipy's per-chunk
+# "not reaching the requested tolerance" notices are expected (filters set once; catch_warnings is not thread-safe)
+for _msg in (".*not reaching the requested tolerance.*", ".*Exited at iteration.*", ".*Exited postprocessing.*"):
+    warnings.filterwarnings("ignore", message=_msg, category=UserWarning)
+
 
 # ----------------------------------------------------------------------------------------------- exact arithmetic
 def exact_sum(a) -> Fraction:
@@ -182,9 +187,7 @@ def pencil_lmin(lag, g, s, b, maxiter=400, chunk=20, tol=1e-9, atol=1e-5, eig_ti
     th_prev = np.inf
     trace = []
     while True:
-        with warnings.catch_warnings():
-            warnings.simplefilter("ignore")
-            ev, Xe = sla.lobpcg(Aop, Xe, B=Bop, M=Pop, largest=False, maxiter=chunk, tol=tol)
+        ev, Xe = sla.lobpcg(Aop, Xe, B=Bop, M=Pop, largest=False, maxiter=chunk, tol=tol)
         it_tot += chunk
         Xe = Xe[:, np.argsort(ev)]
         MX = Mop(Xe)

@@ -8,6 +8,7 @@ batched row-step sweeps. There is no CPU fallback: a missing library or device r
 from __future__ import annotations
 
 import ctypes as C
+import time
 
 import numpy as np
 
@@ -19,9 +20,31 @@ PART_NAMES = ["f", "quad", "zz_over_L", "level", "tri", "vcs", "primal", "C0", "
               "vcs_nviol", "lam_sum"]
 
 
+class _TimedLib:
+    """The library with the wall time of every entry-point call added to `owner.lib_s` (most entry points end with a
+    stream synchronisation, so this is the context's device time plus launch overhead)."""
+
+    def __init__(self, lib, owner):
+        self._lib = lib
+        self._owner = owner
+
+    def __getattr__(self, name):
+        fn = getattr(self._lib, name)
+        owner = self._owner
+
+        def call(*args):
+            t = time.perf_counter()
+            try:
+                return fn(*args)
+            finally:
+                owner.lib_s += time.perf_counter() - t
+        return call
+
+
 class Engine:
     def __init__(self, r: int = 16, dtype: str = "f32", device: int = 0):
-        self.L = LB.load()
+        self.lib_s = 0.0
+        self.L = _TimedLib(LB.load(), self)
         self.r = int(r)
         self.dtype = dtype
         self.ctx = C.c_void_p()

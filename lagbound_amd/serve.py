@@ -1,0 +1,259 @@
+"""Continuous-batching bound service on one GPU: requests (graphs) arrive over time, are admitted as worker slots
+and device memory free up, and each runs the whole bound pipeline with its V and multipliers resident on the device:
+instance build -> seeded label-propagation levels -> device ALM search (pair level, VCS level, triangle and VCS
+separation, PHR updates; lagbound_amd.alm.Solver in process) -> candidate bound (device C(y)X products, FP64
+assembly; lagbound_amd.bound.estimate) -> dense FP64 certificate when the graph is small enough.
+
+Concurrency: one host thread per slot, each with its own device contexts (own HIP stream), so the work of requests
+that are in flight at the same time overlaps on the GPU; the library's calls release the GIL (ctypes). Admission
+is by projected HBM footprint: a request waits until the sum of the footprints in flight plus its own fits the
+budget. Every computation of a request depends only on its own inputs and seeds, so its result is the same alone
+and in the mix (up to the summation order of floating-point atomics inside the device reductions).
+
+No reference counterpart: the reference computes one bound per process (api.upper_bound, src/lagbound/api.py).
+"""
+from __future__ import annotations
+
+import math
+import queue
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import alm
+from . import bound as B
+from . import instances as I
+
+
+@dataclass
+class Request:
+    rid: int
+    n: int
+    keys: np.ndarray                 # sorted unique u*n+v, u < v
+    arrival: float = 0.0             # seconds after the stream starts
+    seed: int = 0
+
+
+@dataclass
+class Params:
+    k: int = 16
+    dtype: str = "f32"
+    outer: int = 24
+    nbatch: int = 0                  # 0: the reference's rule (nbatch_rule)
+    tri_every: int = 4
+    tri_add: int = 0                 # 0: 2 n
+    vcs_every: int = 3
+    P: int = 64                      # label-propagation start clusters
+    rgd_iters: int = 120
+    eig_maxiter: int = 160
+    escape: int = 0
+    cert_max_n: int = 4000           # dense FP64 certificate up to this many vertices
+    cert_margin: float = 1e-7
+
+
+@dataclass
+class Result:
+    rid: int
+    n: int
+    m: int
+    arrival: float
+    start: float
+    finish: float
+    lb: float                        # modularity of the level partition (a feasible point)
+    dv: float
+    ub: float
+    certified: bool
+    UB: float | None
+    device_s: float
+    log: list = field(default_factory=list)
+
+    @property
+    def latency(self):
+        return self.finish - self.arrival
+
+
+def nbatch_rule(n):
+    """Solver batches as the reference's upper_bound sets them (api.py:247-250): 512 from 32,768 rows, else
+    min(n, max(8, n / 64))."""
+    return 512 if n >= 512 * 64 else min(n, max(8, n // 64))
+
+
+def footprint_bytes(n, m, k, p: Params):
+    """Projected device memory of one request: the solver context (V, G and sweep snapshots in FP32, CSR, the
+    triangle pool at its cap, VCS working set and separation scratch) plus the FP64 estimate context."""
+    tri_cap = (p.tri_add or 2 * n) * max(1, p.outer // max(1, p.tri_every))
+    solver = 6 * n * k * 4 + 2 * m * 12 + tri_cap * (16 * 3 + 16) + 8 * n * 24 + 64 * n
+    est = 3 * n * k * 8 + 2 * m * 12 + tri_cap * (24 * 3 + 16) + 8 * n * 24
+    return int(1.25 * (solver + est)) + (32 << 20)
+
+
+def modularity(inst, lab):
+    u = np.repeat(np.arange(inst.n), np.diff(inst.rowptr))
+    same = lab[u] == lab[inst.col]
+    w = np.ones(inst.nnz) if inst.w is None else inst.w
+    vol = np.bincount(lab, weights=inst.d)
+    return float((inst.C0 + w[same].sum() - (vol ** 2).sum() / inst.L) / inst.L)
+
+
+def solve_request(req: Request, p: Params, device=0):
+    """The bound pipeline of one request; returns (Result fields without timing, seconds spent inside the device
+    library's calls)."""
+    logs = []
+    log = lambda *s: logs.append(" ".join(str(x) for x in s))
+    inst = I.build_instance(req.n, req.keys)
+    lab = I.label_propagation(inst.rowptr, inst.col, inst.n, req.seed, P=p.P)
+    nb = p.nbatch or nbatch_rule(inst.n)
+    argv = ["--tag", f"req{req.rid}", "--k", str(p.k), "--nbatch", str(nb), "--inner", "1", "--est_every", "0",
+            "--mom", "0", "--outer", str(p.outer), "--time_limit", "1e9", "--seed", str(req.seed),
+            "--tri_add", str(p.tri_add or 2 * inst.n), "--tri1_add", "0", "--tri_every", str(p.tri_every),
+            "--tri_max", "16000000", "--vcs_every", str(p.vcs_every), "--vcs_add", str(4 * inst.n),
+            "--vcs_max", str(16 * inst.n), "--pw_rule", "default", "--dtype", p.dtype, "--device", str(device)]
+    a = alm.parse(argv)
+    s = alm.Solver(a, inst=inst, labels=[lab], vcs_labels=[lab], init_labels=lab, log=log, signals=False)
+    s.run()
+    lag = B.DeviceLagrangian(inst, r=p.k, dtype="f64", device=device)
+    try:
+        est = B.estimate(inst, s.V, lag=lag, rgd_iters=p.rgd_iters, eig_maxiter=p.eig_maxiter, escape=p.escape,
+                         log=log)
+    finally:
+        lag.close()
+    t_dev = s.eng.lib_s + lag.eng.lib_s
+    cert = None
+    if inst.n <= p.cert_max_n:
+        cert = B.certify_estimate(inst, est, margin=p.cert_margin)
+    return dict(n=inst.n, m=inst.nnz // 2, lb=modularity(inst, lab.astype(np.int64)), dv=est["dv"], ub=est["ub"],
+                certified=bool(cert and cert["passed"]), UB=(cert["UB"] if cert and cert["passed"] else None),
+                log=logs), t_dev
+
+
+class Server:
+    """Continuous batching: `slots` worker threads, HBM-footprint admission against `budget_bytes`."""
+
+    def __init__(self, p: Params, slots=4, budget_bytes=None, device=0):
+        self.p = p
+        self.slots = slots
+        self.device = device
+        self.budget = int(budget_bytes) if budget_bytes is not None else 64 << 30   # of the 288 GB of HBM3E
+        self.used = 0
+        self.cv = threading.Condition()
+        self.results = {}
+        self.errors = {}
+
+    def _admit(self, fp):
+        with self.cv:
+            while self.used + fp > self.budget and self.used > 0:
+                self.cv.wait()
+            self.used += fp
+
+    def _release(self, fp):
+        with self.cv:
+            self.used -= fp
+            self.cv.notify_all()
+
+    def run(self, requests, realtime=True):
+        """Serve `requests` (sorted by arrival). Returns {rid: Result}."""
+        q = queue.Queue()
+        t0 = time.perf_counter()
+
+        def worker():
+            while True:
+                req = q.get()
+                if req is None:
+                    return
+                fp = footprint_bytes(req.n, len(req.keys), self.p.k, self.p)
+                self._admit(fp)
+                start = time.perf_counter() - t0
+                try:
+                    out, tdev = solve_request(req, self.p, self.device)
+                    self.results[req.rid] = Result(rid=req.rid, arrival=req.arrival, start=start,
+                                                   finish=time.perf_counter() - t0, device_s=tdev, **out)
+                except Exception as exc:                      # reported per request, never swallowed
+                    self.errors[req.rid] = repr(exc)
+                finally:
+                    self._release(fp)
+
+        ths = [threading.Thread(target=worker, daemon=True) for _ in range(self.slots)]
+        for t in ths:
+            t.start()
+        for req in requests:
+            if realtime:
+                dt = req.arrival - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+            q.put(req)
+        for _ in ths:
+            q.put(None)
+        for t in ths:
+            t.join()
+        if self.errors:
+            raise RuntimeError(f"{len(self.errors)} request(s) failed: {self.errors}")
+        return self.results
+
+
+def poisson_stream(count, rate, n_lo, n_hi, kbar_lo, kbar_hi, seed):
+    """Poisson arrivals at `rate` req/s of Chung-Lu graphs (gamma 2.1) with n log-uniform in [n_lo, n_hi] and mean
+    degree uniform in [kbar_lo, kbar_hi]."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    reqs = []
+    for i in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi)))))
+        kb = float(rng.uniform(kbar_lo, kbar_hi))
+        n2, keys = I.chung_lu(n, 2.1, kb, seed * 100003 + i)
+        reqs.append(Request(rid=i, n=n2, keys=keys, arrival=float(t[i]), seed=i))
+    return reqs
+
+
+def summarize(results, wall):
+    lat = np.array([r.latency for r in results.values()])
+    svc = np.array([r.finish - r.start for r in results.values()])
+    wait = np.array([r.start - r.arrival for r in results.values()])
+    gaps = np.array([r.ub - r.lb for r in results.values()])
+    cert = [r for r in results.values() if r.certified]
+    return {"requests": len(results), "wall_s": round(wall, 3), "solves_per_s": round(len(results) / wall, 4),
+            "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+            "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+            "service_p50_s": round(float(np.percentile(svc, 50)), 4),
+            "queue_wait_p50_s": round(float(np.percentile(wait, 50)), 4),
+            "device_call_share": round(float(sum(r.device_s for r in results.values()) / max(svc.sum(), 1e-9)), 4),
+            "gap_ub_minus_lb": {"p10": round(float(np.percentile(gaps, 10)), 5),
+                                "p50": round(float(np.percentile(gaps, 50)), 5),
+                                "p90": round(float(np.percentile(gaps, 90)), 5)},
+            "certified": len(cert)}
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(prog="lagbound_amd.serve", description="Serve a Poisson stream of bound requests "
+                                 "on one GPU and print the latency summary as one JSON line.")
+    ap.add_argument("--count", type=int, default=24)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--n_lo", type=int, default=1000)
+    ap.add_argument("--n_hi", type=int, default=200000)
+    ap.add_argument("--kbar_lo", type=float, default=6.0)
+    ap.add_argument("--kbar_hi", type=float, default=30.0)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--budget_gb", type=float, default=64.0)
+    ap.add_argument("--outer", type=int, default=24)
+    ap.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args(argv)
+    t = time.perf_counter()
+    reqs = poisson_stream(a.count, a.rate, a.n_lo, a.n_hi, a.kbar_lo, a.kbar_hi, a.seed)
+    gen = time.perf_counter() - t
+    p = Params(outer=a.outer)
+    t = time.perf_counter()
+    res = Server(p, slots=a.slots, budget_bytes=a.budget_gb * 2 ** 30).run(reqs)
+    wall = time.perf_counter() - t
+    out = summarize(res, wall)
+    out.update(stream=dict(count=a.count, rate=a.rate, n=[a.n_lo, a.n_hi], kbar=[a.kbar_lo, a.kbar_hi], seed=a.seed,
+                           slots=a.slots, budget_gb=a.budget_gb, outer=a.outer, generate_s=round(gen, 2)))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())
