@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the BF16 / FP64 / plain-FP64 comparison runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tuning", default="", help="heavy_edges,heavy_tris,seg_len,row_grid")
+    ap.add_argument("--option", action="append", default=[], help="name=value kernel switch (lb_set_option)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain copy kernel of known size (for scaling hardware byte counters)")
     a = ap.parse_args()
@@ -89,6 +90,9 @@ def main():
     tune = [int(x) for x in a.tuning.split(",")] if a.tuning else None
     if tune:
         eng.set_tuning(*tune)
+    for opt in a.option:
+        k_, v_ = opt.split("=")
+        eng.set_option(k_, int(v_))
     eng.set_instance(inst)
     eng.set_V(V)
     eng.eval(rho)

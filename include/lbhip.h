@@ -139,6 +139,9 @@ const char *lb_kernel_name(int k);
  * 128, 128, 256, 2048 (measured best on the 1M-row power-law instance). No reference
  * counterpart (the reference's parallel shapes are numba prange and chunk sizes, alm_vcs.py:1420-1423). */
 int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int row_grid);
+/* Named switches of the kernels (no reference counterpart): "row_order" (default 1) processes the rows of the f+grad
+ * row kernel in descending order of their work instead of index order (same results up to summation order). */
+int lb_set_option(lb_ctx *ctx, const char *name, int64_t value);
 
 /* Library version string. */
 const char *lb_version(void);

@@ -305,47 +305,55 @@ def estimate(inst, V, lag=None, polish="rgd", rgd_iters=400, polish_time=600.0, 
 # ----------------------------------------------------------------------------------------------- certificate
 def dense_C(inst):
     """C(y) as a dense symmetric FP64 matrix from the instance's multipliers (upper triangle mirrored, so it is
-    exactly symmetric), and F >= |fl(C) - C| entrywise: each entry is a sum of h terms of at most 3 roundings each,
-    so |error| <= gamma(h + 3) sum |terms|."""
+    exactly symmetric), and F >= |fl(C) - C| entrywise: each entry is a sum of at most h terms of at most 3
+    roundings each, so |error| <= gamma(h + 3) sum |terms|, with h the largest number of terms of any entry."""
     n = inst.n
     d = inst.d
-    A = inst.dense_A()
+    Cm = inst.dense_A()
+    absC = np.abs(Cm)
     dd = np.outer(d, d)
-    Cm = A - dd / inst.L
-    absC = np.abs(A) + dd / inst.L
-    cnt = np.full((n, n), 2.0)
+    dd /= inst.L
+    Cm -= dd
+    absC += dd
+    dd = np.outer(d, d)
+    h = 2
     for lv in inst.levels:
-        S = dd * lv.sig[np.ix_(lv.lab, lv.lab)]
+        S = lv.sig[np.ix_(lv.lab, lv.lab)]
+        S *= dd
         np.fill_diagonal(S, 0.0)
         Cm += S
         absC += np.abs(S)
-        cnt += 1
+        h += 1
     for vl in inst.vcs:
         sig = np.zeros((n, vl.P))
         rows = np.repeat(np.arange(n), np.diff(vl.ptr))
         sig[rows, vl.col] = vl.sig
-        S = 0.5 * dd * (sig[:, vl.lab] + sig[:, vl.lab].T)
+        S = sig[:, vl.lab]
+        S += S.T.copy()
+        S *= 0.5
+        S *= dd
         np.fill_diagonal(S, 0.0)
         Cm += S
         absC += np.abs(S)
-        cnt += 2
+        h += 2
     if inst.T:
         a, b, c = (inst.TI[:, q].astype(np.int64) for q in range(3))
-        h = 0.5 * inst.lam
-        tc = np.zeros((n, n))
-        for (i, j, sgn) in ((a, b, -1.0), (b, c, -1.0), (a, c, 1.0)):
-            for (p, q) in ((i, j), (j, i)):
-                np.add.at(Cm, (p, q), sgn * h)
-                np.add.at(absC, (p, q), h)
-                np.add.at(tc, (p, q), 1.0)
-        cnt += tc
+        hl = 0.5 * inst.lam
+        ii = np.concatenate([a, b, b, c, a, c])
+        jj = np.concatenate([b, a, c, b, c, a])
+        vals = np.concatenate([-hl, -hl, -hl, -hl, hl, hl])
+        key = ii * n + jj
+        Cm += np.bincount(key, weights=vals, minlength=n * n).reshape(n, n)
+        absC += np.bincount(key, weights=np.abs(vals), minlength=n * n).reshape(n, n)
+        h += int(np.bincount(key).max())
     iu = np.triu_indices(n, 1)
+    diag = np.diag(Cm).copy()
     Cs = np.zeros((n, n))
     Cs[iu] = Cm[iu]
-    Cs = Cs + Cs.T
-    Cs[np.diag_indices(n)] = np.diag(Cm)
-    F = gamma(cnt.max() + 3) * absC
-    return Cs, F
+    Cs += Cs.T.copy()
+    Cs[np.diag_indices(n)] = diag
+    absC *= gamma(h + 3)
+    return Cs, absC
 
 
 def check_multipliers(inst):

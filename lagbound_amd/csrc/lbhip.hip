@@ -177,7 +177,8 @@ struct RowArgs {
     const void *segout;
     void *G;
     double *part;              // per block: quad, gv, tri tot, tri gmin
-    int stage, sumPpair, sumPv;  // stage [z | M | pdiag | Y_vcs | Z] in LDS
+    int stage, sumPpair, sumPv;  // stage [z | M | pdiag | Y_vcs | Z] in LDS (as the compute type)
+    const int32_t *order;        // processing order of the rows (similar work next to each other), or null
 };
 
 struct FinArgs {
@@ -516,23 +517,32 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
     constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
     extern __shared__ double sm[];
     __shared__ double red[LB_BLOCK / 64];
-    const double *zs = a.z, *Ms = a.M, *pds = a.pdiag, *Ys = a.Y + (int64_t)a.sumPpair * R, *Zs = a.Z;
+    const double *Yg = a.Y + (int64_t)a.sumPpair * R;
+    // staged copies in the compute type (FP32 for FP32 / BF16 V): half the LDS of FP64 staging, so more resident
+    // workgroups per CU; the global arrays stay FP64 and are read directly when the staging does not fit
+    TC *s0 = reinterpret_cast<TC *>(sm), *s1 = s0 + R, *s2 = s1 + a.sumPpair * R, *s3 = s2 + a.sumPpair,
+       *s4 = s3 + a.sumPv * R;
     if (a.stage) {
-        double *p0 = sm, *p1 = p0 + R, *p2 = p1 + a.sumPpair * R, *p3 = p2 + a.sumPpair, *p4 = p3 + a.sumPv * R;
-        for (int e = threadIdx.x; e < R; e += LB_BLOCK) p0[e] = a.z[e];
-        for (int e = threadIdx.x; e < a.sumPpair * R; e += LB_BLOCK) p1[e] = a.M[e];
-        for (int e = threadIdx.x; e < a.sumPpair; e += LB_BLOCK) p2[e] = a.pdiag[e];
-        for (int e = threadIdx.x; e < a.sumPv * R; e += LB_BLOCK) { p3[e] = Ys[e]; p4[e] = a.Z[e]; }
+        for (int e = threadIdx.x; e < R; e += LB_BLOCK) s0[e] = (TC)a.z[e];
+        for (int e = threadIdx.x; e < a.sumPpair * R; e += LB_BLOCK) s1[e] = (TC)a.M[e];
+        for (int e = threadIdx.x; e < a.sumPpair; e += LB_BLOCK) s2[e] = (TC)a.pdiag[e];
+        for (int e = threadIdx.x; e < a.sumPv * R; e += LB_BLOCK) { s3[e] = (TC)Yg[e]; s4[e] = (TC)a.Z[e]; }
         __syncthreads();
-        zs = p0; Ms = p1; pds = p2; Ys = p3; Zs = p4;
     }
+    const bool stg = a.stage != 0;
+    auto ldz = [&](int64_t k) -> double { return stg ? (double)s0[k] : a.z[k]; };
+    auto ldM = [&](int64_t k) -> double { return stg ? (double)s1[k] : a.M[k]; };
+    auto ldpd = [&](int64_t k) -> double { return stg ? (double)s2[k] : a.pdiag[k]; };
+    auto ldY = [&](int64_t k) -> double { return stg ? (double)s3[k] : Yg[k]; };
+    auto ldZ = [&](int64_t k) -> double { return stg ? (double)s4[k] : a.Z[k]; };
     const int slot = threadIdx.x / S, q = threadIdx.x % S;
     const TC *wt = (const TC *)a.w;
     const TC *pv = (const TC *)a.pv;
     const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
     TC *pt = (TC *)a.pt;
     double quad = 0.0, gv = 0.0, tot = 0.0, gmin = INFINITY;
-    for (int64_t i = (int64_t)blockIdx.x * NSL + slot; i < a.n; i += (int64_t)gridDim.x * NSL) {
+    for (int64_t ii = (int64_t)blockIdx.x * NSL + slot; ii < a.n; ii += (int64_t)gridDim.x * NSL) {
+        const int64_t i = a.order ? (int64_t)a.order[ii] : ii;
         TC x[CPL], g[CPL];
         VT<TV>::unpack(a.V[i * S + q], x);
 #pragma unroll
@@ -579,14 +589,14 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
         const double di = a.d[i];
         // rank-1 correction -d_i z / L
 #pragma unroll
-        for (int c = 0; c < CPL; c++) g[c] -= (TC)(di * zs[q * CPL + c] / a.L);
+        for (int c = 0; c < CPL; c++) g[c] -= (TC)(di * ldz(q * CPL + c) / a.L);
         // pair levels: d_i (M_Q - pp_QQ d_i V_i)
         for (int l = 0; l < a.nlev; l++) {
             const int Q = a.Poff[l] + a.lab[l][i];
-            const double *M = Ms + (int64_t)Q * R + q * CPL;
-            const double pd = pds[Q] * di;
+            const int64_t mo = (int64_t)Q * R + q * CPL;
+            const double pd = ldpd(Q) * di;
 #pragma unroll
-            for (int c = 0; c < CPL; c++) g[c] += (TC)(di * (M[c] - pd * (double)x[c]));
+            for (int c = 0; c < CPL; c++) g[c] += (TC)(di * (ldM(mo + c) - pd * (double)x[c]));
         }
         // VCS: 1/2 d_i (sum_R pv_iR Y_R + Z_R(i)) - pv_own d_i^2 V_i
         for (int m = 0; m < a.nvcs; m++) {
@@ -599,13 +609,13 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
                 const int Rc = a.vcol[e];
                 const double p = (double)pv[e];
                 if (Rc == Ri) pown = p;
-                const double *YR = Ys + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+                const int64_t yo = (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
 #pragma unroll
-                for (int c = 0; c < CPL; c++) acc[c] += p * YR[c];
+                for (int c = 0; c < CPL; c++) acc[c] += p * ldY(yo + c);
             }
-            const double *ZR = Zs + (int64_t)(a.Zoff[m] + Ri) * R + q * CPL;
+            const int64_t zo = (int64_t)(a.Zoff[m] + Ri) * R + q * CPL;
 #pragma unroll
-            for (int c = 0; c < CPL; c++) g[c] += (TC)(0.5 * di * (acc[c] + ZR[c]) - pown * di * di * (double)x[c]);
+            for (int c = 0; c < CPL; c++) g[c] += (TC)(0.5 * di * (acc[c] + ldZ(zo + c)) - pown * di * di * (double)x[c]);
         }
         // triangles: T(pt) V rows through the incidence records
         const int64_t r0 = a.iptr[i], r1 = a.iptr[i + 1];
@@ -1691,6 +1701,8 @@ struct lb_ctx {
         hErows, hEseg0, hTrows, hTseg0, parts, lamsum,
         order, tmask, Kq, stZ, Zs, cnts, Vstage, DDb, segkap, seglist, vslack, vage, vvol;
     std::vector<double> h_d;                // vertex volumes (host copy)
+    DBuf rorder;                            // row processing order of lb_k_rows (descending work)
+    bool use_order = true;
 };
 
 namespace {
@@ -1778,10 +1790,34 @@ void build_segments(int64_t n, const int64_t *ptr, int64_t heavy, int seg_len, i
     seg0.push_back(base + (int32_t)segs.size());
 }
 
+// Row processing order of lb_k_rows: descending per-row work (light edges + 2 x light triangle records, or the
+// number of hub segments summed), stable in the row index (counting sort). Rows of similar work then share a
+// wavefront, so its slots finish together.
+int build_row_order(lb_ctx *c) {
+    const int64_t n = c->n;
+    if (!n) return 0;
+    const int64_t W = 2048;
+    std::vector<int32_t> w((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t de = c->h_rowptr[i + 1] - c->h_rowptr[i];
+        const int64_t dt = c->h_iptr.size() == (size_t)n + 1 ? c->h_iptr[i + 1] - c->h_iptr[i] : 0;
+        const int64_t we = de > c->built_heavyE ? (de + c->seg_len - 1) / c->seg_len : de;
+        const int64_t wt = dt > c->built_heavyT ? (dt + c->seg_len - 1) / c->seg_len : dt;
+        w[i] = (int32_t)std::min<int64_t>(W - 1, we + 2 * wt);
+    }
+    std::vector<int64_t> cnt(W + 1, 0);
+    for (int64_t i = 0; i < n; i++) cnt[W - 1 - w[i] + 1]++;
+    for (int64_t b = 0; b < W; b++) cnt[b + 1] += cnt[b];
+    std::vector<int32_t> ord((size_t)n);
+    for (int64_t i = 0; i < n; i++) ord[cnt[W - 1 - w[i]]++] = (int32_t)i;
+    return upload(c, c->rorder, ord.data(), sizeof(int32_t) * (size_t)n);
+}
+
 int rebuild_segs(lb_ctx *c) {
     std::vector<Seg> all(c->segE);
     all.insert(all.end(), c->segT.begin(), c->segT.end());
     int rc;
+    if ((rc = build_row_order(c))) return rc;
     if ((rc = upload(c, c->segs, all.empty() ? nullptr : all.data(), sizeof(Seg) * all.size()))) return rc;
     if ((rc = dalloc(c, c->segout, (size_t)std::max<size_t>(all.size(), 1) * c->r * c->cbytes))) return rc;
     const int nblk = (int)((all.size() + 3) / 4);
@@ -1904,8 +1940,9 @@ struct Launch {
             ra.hT_rows = (const int32_t *)c->hTrows.p; ra.hT_seg0 = (const int32_t *)c->hTseg0.p; ra.nhT = (int)c->hT_rows.size();
             ra.segout = c->segout.p; ra.G = c->G.p; ra.part = (double *)c->row_part.p;
             ra.sumPpair = c->sumPpair; ra.sumPv = c->sumPv;
-            const size_t stage_bytes = sizeof(double) * ((size_t)R + (size_t)c->sumPpair * R + c->sumPpair + 2 * (size_t)c->sumPv * R);
+            const size_t stage_bytes = sizeof(TC) * ((size_t)R + (size_t)c->sumPpair * R + c->sumPpair + 2 * (size_t)c->sumPv * R);
             ra.stage = stage_bytes <= 48 * 1024;
+            ra.order = c->use_order ? (const int32_t *)c->rorder.p : nullptr;
             hipLaunchKernelGGL((lb_k_rows<TV, R>), dim3((unsigned)nrow_blk), dim3(LB_BLOCK), ra.stage ? stage_bytes : 0, c->stream, ra);
         }
         mark();
@@ -2433,13 +2470,19 @@ void lb_destroy(lb_ctx *c) {
                     &c->Y, &c->z, &c->M, &c->pdiag, &c->levtot, &c->levviol, &c->Z, &c->colsum_part, &c->vcs_part,
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
                     &c->hTseg0, &c->parts, &c->lamsum, &c->order, &c->tmask, &c->Kq, &c->stZ, &c->Zs, &c->cnts,
-                    &c->Vstage, &c->DDb, &c->segkap, &c->seglist, &c->vslack, &c->vage, &c->vvol};
+                    &c->Vstage, &c->DDb, &c->segkap, &c->seglist, &c->vslack, &c->vage, &c->vvol, &c->rorder};
     for (DBuf *b : bufs) dfree(*b);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
 
 const char *lb_last_error(const lb_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
+    if (!c || !name) return fail(c, -1, "lb_set_option: bad arguments");
+    if (std::strcmp(name, "row_order") == 0) { c->use_order = value != 0; return 0; }
+    return fail(c, -1, std::string("lb_set_option: unknown option ") + name);
+}
 
 int lb_set_tuning(lb_ctx *c, int heavy_edges, int heavy_tris, int seg_len, int row_grid) {
     if (!c) return -1;

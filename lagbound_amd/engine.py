@@ -76,6 +76,9 @@ class Engine:
         the current grid."""
         self._chk(self.L.lb_set_tuning(self.ctx, heavy_edges, heavy_tris, seg_len, row_grid), "lb_set_tuning")
 
+    def set_option(self, name: str, value: int):
+        self._chk(self.L.lb_set_option(self.ctx, name.encode(), int(value)), "lb_set_option")
+
     def set_instance(self, inst):
         n = inst.n
         self._keep = []

@@ -419,3 +419,43 @@ def add_twins_and_leaves(n: int, keys: np.ndarray, ntwin: int, nleaf: int, seed:
     b = np.r_[v, [q for _, q in new]].astype(np.int64)
     lo, hi = np.minimum(a, b), np.maximum(a, b)
     return n2, np.unique(lo * n2 + hi)
+
+
+def local_moving(rowptr: np.ndarray, col: np.ndarray, w, d: np.ndarray, L: float, gamma: float, seed: int,
+                 passes: int = 20) -> np.ndarray:
+    """Seeded greedy local moving at resolution gamma (the node-moving phase of Louvain/Leiden): start from
+    singletons, visit the vertices in a random order and move each to the neighbouring cluster with the largest gain
+    k_{i,C} - gamma d_i vol_C / L (its own cluster counted without it) while some move gains; repeat up to `passes`
+    times. High gamma gives many small dense clusters (the reference's levels are Leiden runs at gamma 16 and 64,
+    api.py:208-210). A host loop: meant for graphs of up to a few tens of thousands of vertices."""
+    n = len(d)
+    rng = np.random.default_rng(seed)
+    lab = np.arange(n)
+    vol = np.asarray(d, np.float64).copy()
+    ww = np.ones(len(col)) if w is None else np.asarray(w, np.float64)
+    for _ in range(passes):
+        moved = 0
+        for i in rng.permutation(n).tolist():
+            a, b = rowptr[i], rowptr[i + 1]
+            if a == b:
+                continue
+            nb = col[a:b]
+            li = lab[i]
+            vol[li] -= d[i]
+            k = {}
+            for j, x in zip(nb.tolist(), ww[a:b].tolist()):
+                if j != i:
+                    c = lab[j]
+                    k[c] = k.get(c, 0.0) + x
+            best, bgain = li, k.get(li, 0.0) - gamma * d[i] * vol[li] / L
+            for c, kc in k.items():
+                g = kc - gamma * d[i] * vol[c] / L
+                if g > bgain + 1e-12:
+                    best, bgain = c, g
+            lab[i] = best
+            vol[best] += d[i]
+            if best != li:
+                moved += 1
+        if moved == 0:
+            break
+    return np.unique(lab, return_inverse=True)[1].astype(np.int32)

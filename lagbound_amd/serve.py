@@ -47,10 +47,12 @@ class Params:
     vcs_every: int = 3
     P: int = 64                      # label-propagation start clusters
     rgd_iters: int = 120
-    eig_maxiter: int = 160
+    eig_maxiter: int = 60
+    eig_extra: int = 8
     escape: int = 0
     cert_max_n: int = 4000           # dense FP64 certificate up to this many vertices
     cert_margin: float = 1e-7
+    blas_threads: int = 4            # host BLAS threads while serving (slots x threads ~ the host cores)
 
 
 @dataclass
@@ -115,8 +117,8 @@ def solve_request(req: Request, p: Params, device=0):
     s.run()
     lag = B.DeviceLagrangian(inst, r=p.k, dtype="f64", device=device)
     try:
-        est = B.estimate(inst, s.V, lag=lag, rgd_iters=p.rgd_iters, eig_maxiter=p.eig_maxiter, escape=p.escape,
-                         log=log)
+        est = B.estimate(inst, s.V, lag=lag, rgd_iters=p.rgd_iters, eig_maxiter=p.eig_maxiter, eig_extra=p.eig_extra,
+                         escape=p.escape, log=log)
     finally:
         lag.close()
     t_dev = s.eng.lib_s + lag.eng.lib_s
@@ -131,7 +133,8 @@ def solve_request(req: Request, p: Params, device=0):
 class Server:
     """Continuous batching: `slots` worker threads, HBM-footprint admission against `budget_bytes`."""
 
-    def __init__(self, p: Params, slots=4, budget_bytes=None, device=0):
+    def __init__(self, p: Params, slots=4, budget_bytes=None, device=0, progress=None):
+        self.progress = progress                 # callable(Result) after each completed request
         self.p = p
         self.slots = slots
         self.device = device
@@ -154,6 +157,11 @@ class Server:
 
     def run(self, requests, realtime=True):
         """Serve `requests` (sorted by arrival). Returns {rid: Result}."""
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=self.p.blas_threads):
+            return self._run(requests, realtime)
+
+    def _run(self, requests, realtime):
         q = queue.Queue()
         t0 = time.perf_counter()
 
@@ -167,8 +175,11 @@ class Server:
                 start = time.perf_counter() - t0
                 try:
                     out, tdev = solve_request(req, self.p, self.device)
-                    self.results[req.rid] = Result(rid=req.rid, arrival=req.arrival, start=start,
-                                                   finish=time.perf_counter() - t0, device_s=tdev, **out)
+                    res = Result(rid=req.rid, arrival=req.arrival, start=start, finish=time.perf_counter() - t0,
+                                 device_s=tdev, **out)
+                    self.results[req.rid] = res
+                    if self.progress:
+                        self.progress(res)
                 except Exception as exc:                      # reported per request, never swallowed
                     self.errors[req.rid] = repr(exc)
                 finally:
@@ -227,6 +238,7 @@ def summarize(results, wall):
 def main(argv=None):
     import argparse
     import json
+    import sys
     ap = argparse.ArgumentParser(prog="lagbound_amd.serve", description="Serve a Poisson stream of bound requests "
                                  "on one GPU and print the latency summary as one JSON line.")
     ap.add_argument("--count", type=int, default=24)
@@ -245,7 +257,11 @@ def main(argv=None):
     gen = time.perf_counter() - t
     p = Params(outer=a.outer)
     t = time.perf_counter()
-    res = Server(p, slots=a.slots, budget_bytes=a.budget_gb * 2 ** 30).run(reqs)
+    def progress(r):
+        print(f"[serve] rid {r.rid} n {r.n} m {r.m} latency {r.latency:.3f}s service {r.finish - r.start:.3f}s "
+              f"lb {r.lb:.5f} ub {r.ub:.5f} certified {r.certified}", file=sys.stderr, flush=True)
+
+    res = Server(p, slots=a.slots, budget_bytes=a.budget_gb * 2 ** 30, progress=progress).run(reqs)
     wall = time.perf_counter() - t
     out = summarize(res, wall)
     out.update(stream=dict(count=a.count, rate=a.rate, n=[a.n_lo, a.n_hi], kbar=[a.kbar_lo, a.kbar_hi], seed=a.seed,
