@@ -156,6 +156,15 @@ const char *lb_version(void);
 int lb_reduce_exact(lb_ctx *ctx, int64_t n, int64_t m, const int64_t *keys, const uint64_t *rhash, int64_t *group_of,
                     int64_t *vol, int64_t *keys_out, double *w_out, int64_t *stats);
 
+/* Dense FP64 certificate step on the device (the dense analogue of the reference's LDL certificate,
+ * certify_lifted_vc.py / verify_cert.py psd_ldl, verify_cert.py:92-104): assembles M = Diag(dg) - C(y) from the
+ * context's graph, levels, working sets and triangle pool (FP64 context only, so the multipliers are the exact
+ * values), factors it with rocSOLVER dpotrf and returns out[0] = info (0: factored; k > 0: leading minor k not
+ * positive), out[1] = sum of R_ij^2 of the factor, out[2] = sum over the off-diagonal entries of (sum of the
+ * magnitudes of the terms of C_ij)^2, from which the caller prices the rounding: lambda_min(M) >= -(gamma(n+1)
+ * out[1] + ||F||_F). Needs 16 n^2 bytes of free device memory. */
+int lb_certify_dense(lb_ctx *ctx, const double *dg, double *out);
+
 #ifdef __cplusplus
 }
 #endif

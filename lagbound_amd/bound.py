@@ -209,13 +209,14 @@ def pencil_lmin(lag, g, s, b, maxiter=400, chunk=20, tol=1e-9, atol=1e-5, eig_ti
 def estimate(inst, V, lag=None, polish="rgd", rgd_iters=400, polish_time=600.0, ptol=2e-5, scale="vol",
              eig_extra=16, eig_maxiter=400, eig_tol=1e-9, eig_chunk=20, eig_atol=1e-5, eig_time=1200.0, escape=2,
              escape_min=0.002, escape_p=4, escape_ratio=0.3, seed=0, no_w_init=False, device=0, dtype="f64",
-             log=print):
+             eig_block=None, log=print):
     """Candidate bound at the instance's multipliers (levels' sig, lam, VCS sig): polish, then per scale s the
     pencil eigenvalue, ub = dv + max(0, -(theta_1 - r_1)) sum(s) / L, then up to `escape` rank-escape rounds
     (append the isolated bottom eigenvectors to W, re-polish, re-estimate; the best round is kept) -- est_dual_vcs.py
     :316-389. Returns the reference's result record (dv0, dv, s_<scale> records, escape_rounds, ub, ...) plus the
     best round's W, g and s (`W`, `g`, `s` keys, for certification). `lag` (anything with .inst/.const/.CX) defaults
-    to the device operator. ub is an estimate, not a proof."""
+    to the device operator. `eig_block` overrides the LOBPCG block width k + eig_extra (a narrower block costs
+    less host work per iteration). ub is an estimate, not a proof."""
     own = lag is None
     if own:
         lag = DeviceLagrangian(inst, r=V.shape[1], dtype=dtype, device=device)
@@ -236,7 +237,7 @@ def estimate(inst, V, lag=None, polish="rgd", rgd_iters=400, polish_time=600.0, 
         recs, vecs = {}, {}
         for sname in scales:
             s = SCALES[sname](inst.d, L, n)
-            b = W.shape[1] + eig_extra
+            b = W.shape[1] + eig_extra if eig_block is None else int(eig_block)
             X0 = None
             if not no_w_init:
                 Uw, Sw, _ = np.linalg.svd(W, full_matrices=False)
@@ -306,7 +307,9 @@ def estimate(inst, V, lag=None, polish="rgd", rgd_iters=400, polish_time=600.0, 
 def dense_C(inst):
     """C(y) as a dense symmetric FP64 matrix from the instance's multipliers (upper triangle mirrored, so it is
     exactly symmetric), and F >= |fl(C) - C| entrywise: each entry is a sum of at most h terms of at most 3
-    roundings each, so |error| <= gamma(h + 3) sum |terms|, with h the largest number of terms of any entry."""
+    roundings each, so |error| <= gamma(h + 3) sum |terms|, with h the largest number of terms of any entry; the
+    computed sum of magnitudes is itself within a factor (1 + gamma(h + 3)) of the exact one, hence
+    F = gamma(2h + 8) fl(sum |terms|)."""
     n = inst.n
     d = inst.d
     Cm = inst.dense_A()
@@ -352,7 +355,7 @@ def dense_C(inst):
     Cs[iu] = Cm[iu]
     Cs += Cs.T.copy()
     Cs[np.diag_indices(n)] = diag
-    absC *= gamma(h + 3)
+    absC *= gamma(2 * h + 8)                        # covers the rounding of absC itself as well
     return Cs, absC
 
 
@@ -396,9 +399,57 @@ def certify_dense(inst, g, t=None):
     return dict(passed=True, UB=UB, eps=eps, price=n * eps / inst.L)
 
 
-def certify_estimate(inst, est, margin=1e-7):
-    """certify_dense at the estimate's best round (g, s, lmin): t = (max(0, -lmin) + margin) s."""
+def max_terms(inst):
+    """Largest number of terms of any off-diagonal entry of C(y): A, the rank-1 part, one per pair level, two per
+    VCS level, and the most triangles sharing one vertex pair."""
+    h = 2 + len(inst.levels) + 2 * len(inst.vcs)
+    if inst.T:
+        n = inst.n
+        a, b, c = (inst.TI[:, q].astype(np.int64) for q in range(3))
+        ii = np.concatenate([np.minimum(a, b), np.minimum(b, c), np.minimum(a, c)])
+        jj = np.concatenate([np.maximum(a, b), np.maximum(b, c), np.maximum(a, c)])
+        h += int(np.unique(ii * n + jj, return_counts=True)[1].max())
+    return h
+
+
+def certify_dense_device(inst, g, t=None, device=0, eng=None):
+    """certify_dense with the assembly and the Cholesky factorisation on the device (lb_certify_dense, rocSOLVER
+    dpotrf in FP64); the rounding price and the exact rational total are computed here as in certify_dense."""
+    from .engine import Engine
+    check_multipliers(inst)
+    n = inst.n
+    t = np.zeros(n) if t is None else np.asarray(t, np.float64)
+    dg = np.ascontiguousarray(g + t)                # the exact diagonal used below is fl(g + t)
+    own = eng is None
+    if own:
+        eng = Engine(r=8, dtype="f64", device=device)
+        eng.set_instance(inst)
+    try:
+        out = eng.certify_dense(dg)
+    finally:
+        if own:
+            eng.close()
+    info, sr, sab = int(out[0]), float(out[1]), float(out[2])
+    if info != 0:
+        return dict(passed=False, reason=f"Cholesky failed at leading minor {info}: M is not positive definite")
+    gh = gamma(2 * max_terms(inst) + 8)
+    diagC = inst.d ** 2 / inst.L
+    Fd = gh * diagC + gamma(1) * (np.abs(dg) + diagC)
+    nF = math.sqrt(gh * gh * sab * (1 + gamma(2 * n * n + 8)) + float(np.sum(Fd * Fd)) * (1 + gamma(n + 8)))
+    nF *= 1 + gamma(8)
+    nR = sr * (1 + gamma(n * n + 8))
+    eps = (gamma(n + 1) * nR + nF) * 1.01
+    total = Fraction(inst.C0) + (exact_sum(inst.lam) if inst.T else 0) + exact_sum(dg) + Fraction(n) * Fraction(eps)
+    UB = frac_up(total / Fraction(inst.L))
+    return dict(passed=True, UB=UB, eps=eps, price=n * eps / inst.L, device=True)
+
+
+def certify_estimate(inst, est, margin=1e-7, device=None):
+    """certify_dense at the estimate's best round (g, s, lmin): t = (max(0, -lmin) + margin) s; on the device when
+    `device` is given."""
     t = (max(0.0, -est["lmin"]) + margin) * est["s"]
+    if device is not None:
+        return certify_dense_device(inst, est["g"], t, device=device)
     return certify_dense(inst, est["g"], t)
 
 

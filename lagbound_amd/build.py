@@ -17,9 +17,9 @@ HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
                "-Wno-unused-value", "-Wno-unused-result"]
 
 
-def _compile(src, out, verbose):
+def _compile(src, out, verbose, libs=()):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc] + HIPCC_FLAGS + ["-I", INCLUDE, "-o", out + ".tmp", src]
+    cmd = [hipcc] + HIPCC_FLAGS + ["-I", INCLUDE, "-o", out + ".tmp", src] + list(libs)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + res.stderr[-4000:])
@@ -31,7 +31,7 @@ def _compile(src, out, verbose):
 def build(verbose: bool = False) -> str:
     """Compile every HIP source of the package (liblbhip.so, and the bandwidth probe liblbprobe.so used by the
     benchmark's traffic calibration); always recompiles."""
-    _compile(SRC, OUT, verbose)
+    _compile(SRC, OUT, verbose, libs=["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     _compile(PROBE_SRC, PROBE_OUT, verbose)
     return OUT
 

@@ -337,15 +337,27 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
         for (int m = 0; m < a.nvcs; m++) {
             const int Ri = a.lab[m][i];
             const int64_t e0 = a.ptr[m][i], e1 = a.ptr[m][i + 1];
-            for (int64_t e = e0; e < e1; e++) {
-                const int Rc = a.col[e];
+            for (int64_t eb = e0; eb < e1; eb += 4) {
+              // the entries' columns and multipliers are loaded 4 at a time before any is used
+              int Rq[4];
+              TC sq[4];
+#pragma unroll
+              for (int t = 0; t < 4; t++) {
+                  const int64_t ec = eb + t < e1 ? eb + t : e1 - 1;
+                  Rq[t] = a.col[ec];
+                  sq[t] = sig[ec];
+              }
+              const int nb = (int)(e1 - eb < 4 ? e1 - eb : 4);
+              for (int t = 0; t < nb; t++) {
+                const int64_t e = eb + t;
+                const int Rc = Rq[t];
                 const double *YR = Yb + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
                 double s = 0.0;
 #pragma unroll
                 for (int c = 0; c < CPL; c++) s += (double)x[c] * YR[c];
                 s = slot_sum<S>(s);
                 const double g = (Rc == Ri) ? di * (s - di) : di * s;
-                const double sg = (double)sig[e];
+                const double sg = (double)sq[t];
                 double p = sg;
                 if (a.rho > 0.0 && di > 0.0) {
                     const double rr = a.rho * (a.wv * a.rq[m][Rc] / (di * di));
@@ -370,6 +382,7 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
                         for (int c = 0; c < CPL; c++) atomicAdd(dst + c, (double)pc * (di * (double)x[c]));
                     }
                 }
+              }
             }
         }
     }
@@ -605,13 +618,23 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
 #pragma unroll
             for (int c = 0; c < CPL; c++) acc[c] = 0.0;
             double pown = 0.0;
-            for (int64_t e = a.vptr[m][i]; e < a.vptr[m][i + 1]; e++) {
-                const int Rc = a.vcol[e];
-                const double p = (double)pv[e];
-                if (Rc == Ri) pown = p;
-                const int64_t yo = (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+            const int64_t v0 = a.vptr[m][i], v1 = a.vptr[m][i + 1];
+            for (int64_t eb = v0; eb < v1; eb += 4) {
+                int Rq[4];
+                double pq[4];
 #pragma unroll
-                for (int c = 0; c < CPL; c++) acc[c] += p * ldY(yo + c);
+                for (int t = 0; t < 4; t++) {           // 4 entries in flight; entries past the row add 0
+                    const int64_t ec = eb + t < v1 ? eb + t : v1 - 1;
+                    Rq[t] = a.vcol[ec];
+                    pq[t] = eb + t < v1 ? (double)pv[ec] : 0.0;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    if (Rq[t] == Ri && eb + t < v1) pown = pq[t];
+                    const int64_t yo = (int64_t)(a.Zoff[m] + Rq[t]) * R + q * CPL;
+#pragma unroll
+                    for (int c = 0; c < CPL; c++) acc[c] += pq[t] * ldY(yo + c);
+                }
             }
             const int64_t zo = (int64_t)(a.Zoff[m] + Ri) * R + q * CPL;
 #pragma unroll
@@ -3279,6 +3302,185 @@ int lb_reduce_exact(lb_ctx *c, int64_t n, int64_t m, const int64_t *keys, const 
     cleanup();
 #undef RCHK
 #undef RHIP
+    return 0;
+}
+
+}  // extern "C"
+
+// ================================================================================================ dense certificate
+// M = Diag(dg) - C(y) assembled in device memory (column-major, lower triangle), with the entrywise magnitudes
+// |terms| of C summed into a second matrix for the rounding bound, then factored by rocSOLVER (dpotrf). The caller
+// turns (info, sum R_ij^2, sum of the magnitudes squared) into a proved shift: R^T R = M + E with
+// |E| <= gamma(n+1) |R^T||R| for any conventional Cholesky (Higham, Accuracy and Stability of Numerical Algorithms,
+// Thm 10.3), so lambda_min(M) >= -(gamma(n+1) ||R||_F^2 + ||F||_F).
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+namespace {
+
+struct CertArgs {
+    int64_t n; const double *d; double L; const double *dg;
+    int nlev; int Poff[LB_MAXLEV]; int64_t soff[LB_MAXLEV]; const int32_t *lab[LB_MAXLEV]; const double *sig;
+    int nvcs; const int32_t *vlab[LB_MAXVCS]; const int64_t *vptr[LB_MAXVCS]; const int32_t *vcol; const double *vsig;
+    double *M; double *Ab;
+};
+
+__device__ __forceinline__ double vcs_lookup(const int64_t *ptr, const int32_t *col, const double *sig, int64_t row,
+                                             int32_t R) {
+    int64_t lo = ptr[row], hi = ptr[row + 1];
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (col[mid] < R) lo = mid + 1; else hi = mid;
+    }
+    return (lo < ptr[row + 1] && col[lo] == R) ? sig[lo] : 0.0;
+}
+
+// lower triangle (i >= j) of M = Diag(dg) - (-d d^T / L + levels + VCS); the A and triangle terms are scattered next
+__global__ void lb_k_cert_fill(CertArgs a) {
+    const int64_t n = a.n, tot = n * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx % n, j = idx / n;
+        if (i < j) continue;
+        const double di = a.d[i], dj = a.d[j];
+        if (i == j) {
+            a.M[idx] = a.dg[i] + di * di / a.L;
+            a.Ab[idx] = 0.0;
+            continue;
+        }
+        const double dd = di * dj;
+        double c = -dd / a.L, ab = dd / a.L;
+        for (int l = 0; l < a.nlev; l++) {
+            const double s = dd * a.sig[a.soff[l] + (int64_t)a.lab[l][i] * (a.Poff[l + 1] - a.Poff[l]) + a.lab[l][j]];
+            c += s;
+            ab += fabs(s);
+        }
+        for (int m = 0; m < a.nvcs; m++) {
+            const double s = 0.5 * dd * (vcs_lookup(a.vptr[m], a.vcol, a.vsig, i, a.vlab[m][j]) +
+                                         vcs_lookup(a.vptr[m], a.vcol, a.vsig, j, a.vlab[m][i]));
+            c += s;
+            ab += fabs(s);
+        }
+        a.M[idx] = -c;
+        a.Ab[idx] = ab;
+    }
+}
+
+__global__ void lb_k_cert_edges(const int64_t *rowptr, const int32_t *col, const double *w, int64_t n, double *M,
+                                double *Ab) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        for (int64_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+            const int64_t j = col[e];
+            if (j >= i) continue;                     // each lower entry (i > j) has one writer: row i
+            const double x = w ? w[e] : 1.0;
+            M[j * n + i] -= x;
+            Ab[j * n + i] += fabs(x);
+        }
+    }
+}
+
+__global__ void lb_k_cert_tris(const int32_t *TI, const double *lam, int64_t T, int64_t n, double *M, double *Ab) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t a = TI[3 * t], b = TI[3 * t + 1], c = TI[3 * t + 2];
+        const double h = 0.5 * lam[t];
+        // C_ab -= h, C_bc -= h, C_ac += h  ->  M = -C
+        const int64_t p[3][2] = {{a, b}, {b, c}, {a, c}};
+        const double sgn[3] = {1.0, 1.0, -1.0};
+        for (int q = 0; q < 3; q++) {
+            const int64_t hi = p[q][0] > p[q][1] ? p[q][0] : p[q][1], lo = p[q][0] > p[q][1] ? p[q][1] : p[q][0];
+            atomicAdd(M + lo * n + hi, sgn[q] * h);
+            atomicAdd(Ab + lo * n + hi, h);
+        }
+    }
+}
+
+// per-block partial sums of squares over the strictly lower (diag = 0) or lower-with-diagonal (diag = 1) triangle
+__global__ __launch_bounds__(256) void lb_k_cert_sumsq(const double *X, int64_t n, int diag, double *part) {
+    __shared__ double red[4];
+    double s = 0.0;
+    const int64_t tot = n * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx % n, j = idx / n;
+        if (i > j || (diag && i == j)) s += X[idx] * X[idx];
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lb_certify_dense(lb_ctx *c, const double *dg, double *out) {
+    if (!c) return -1;
+    if (!c->n || !dg || !out) return fail(c, -1, "lb_certify_dense: bad arguments");
+    if (c->vdtype != LB_F64) return fail(c, -1, "lb_certify_dense: needs an FP64 context (multipliers stored exactly)");
+    const int64_t n = c->n;
+    HIPCHK(c, hipSetDevice(c->device));
+    size_t fr = 0, tt = 0;
+    HIPCHK(c, hipMemGetInfo(&fr, &tt));
+    const size_t need = 2 * sizeof(double) * (size_t)n * (size_t)n + (64u << 20);
+    if (need > fr) return fail(c, -2, "lb_certify_dense: " + std::to_string(need >> 20) + " MiB needed, " +
+                                      std::to_string(fr >> 20) + " MiB free");
+    DBuf M, Ab, ddg, part, dinfo;
+    auto cleanup = [&]() { dfree(M); dfree(Ab); dfree(ddg); dfree(part); dfree(dinfo); };
+    int rc;
+#define CCHK(x) do { if ((rc = (x))) { cleanup(); return rc; } } while (0)
+#define CHIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { cleanup(); return fail(c, -3, std::string(#x) + ": " + hipGetErrorString(e_)); } } while (0)
+    hipStream_t s = c->stream;
+    const int G = 4096, B = 256;
+    CCHK(dalloc(c, M, sizeof(double) * (size_t)n * n));
+    CCHK(dalloc(c, Ab, sizeof(double) * (size_t)n * n));
+    CCHK(upload(c, ddg, dg, sizeof(double) * (size_t)n));
+    CCHK(dalloc(c, part, sizeof(double) * G));
+    CCHK(dalloc(c, dinfo, sizeof(int)));
+    CertArgs ca{};
+    ca.n = n; ca.d = (const double *)c->d.p; ca.L = c->L; ca.dg = (const double *)ddg.p;
+    ca.nlev = c->nlev;
+    int o = 0;
+    for (int l = 0; l < c->nlev; l++) { ca.Poff[l] = o; o += c->P[l]; ca.soff[l] = c->soff[l]; ca.lab[l] = (const int32_t *)c->labs.p + (int64_t)l * n; }
+    ca.Poff[c->nlev] = o;
+    ca.sig = (const double *)c->sig.p;
+    ca.nvcs = c->nvcs;
+    for (int m = 0; m < c->nvcs; m++) {
+        ca.vlab[m] = (const int32_t *)c->vlabs.p + (int64_t)m * n;
+        ca.vptr[m] = (const int64_t *)c->vptr.p + (int64_t)m * (n + 1);
+    }
+    ca.vcol = (const int32_t *)c->vcol.p; ca.vsig = (const double *)c->vsig.p;
+    ca.M = (double *)M.p; ca.Ab = (double *)Ab.p;
+    hipLaunchKernelGGL(lb_k_cert_fill, dim3(G), dim3(B), 0, s, ca);
+    hipLaunchKernelGGL(lb_k_cert_edges, dim3(G), dim3(B), 0, s, (const int64_t *)c->rowptr.p, (const int32_t *)c->col.p,
+                       c->weighted ? (const double *)c->w.p : nullptr, n, (double *)M.p, (double *)Ab.p);
+    if (c->T) hipLaunchKernelGGL(lb_k_cert_tris, dim3(G), dim3(B), 0, s, (const int32_t *)c->TI.p, (const double *)c->lam.p, c->T, n,
+                                 (double *)M.p, (double *)Ab.p);
+    CHIP(hipGetLastError());
+    std::vector<double> hp(G);
+    hipLaunchKernelGGL(lb_k_cert_sumsq, dim3(G), dim3(B), 0, s, (const double *)Ab.p, n, 0, (double *)part.p);
+    CHIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * G, hipMemcpyDeviceToHost, s));
+    CHIP(hipStreamSynchronize(s));
+    double sab = 0.0;
+    for (double x : hp) sab += x;
+    // factor
+    rocblas_handle hb = nullptr;
+    if (rocblas_create_handle(&hb) != rocblas_status_success) { cleanup(); return fail(c, -3, "rocblas_create_handle failed"); }
+    rocblas_set_stream(hb, s);
+    const rocblas_status st = rocsolver_dpotrf(hb, rocblas_fill_lower, (rocblas_int)n, (double *)M.p, (rocblas_int)n, (rocblas_int *)dinfo.p);
+    int info = -1;
+    hipError_t e1 = hipMemcpyAsync(&info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, s);
+    hipError_t e2 = hipStreamSynchronize(s);
+    rocblas_destroy_handle(hb);
+    if (st != rocblas_status_success) { cleanup(); return fail(c, -3, "rocsolver_dpotrf failed with status " + std::to_string((int)st)); }
+    CHIP(e1);
+    CHIP(e2);
+    double sr = 0.0;
+    if (info == 0) {
+        hipLaunchKernelGGL(lb_k_cert_sumsq, dim3(G), dim3(B), 0, s, (const double *)M.p, n, 1, (double *)part.p);
+        CHIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * G, hipMemcpyDeviceToHost, s));
+        CHIP(hipStreamSynchronize(s));
+        for (double x : hp) sr += x;
+    }
+    out[0] = (double)info; out[1] = sr; out[2] = 2.0 * sab;
+    cleanup();
+#undef CCHK
+#undef CHIP
     return 0;
 }
 

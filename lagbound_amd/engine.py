@@ -226,6 +226,14 @@ class Engine:
         self._chk(self.L.lb_set_weights(self.ctx, float(pairlvl_weight), float(vc_weight) * float(pairlvl_weight)),
                   "lb_set_weights")
 
+    def certify_dense(self, dg):
+        """lb_certify_dense at diagonal dg: returns (info, sum R_ij^2, sum of squared off-diagonal magnitudes)."""
+        dg = np.ascontiguousarray(dg, np.float64)
+        assert len(dg) == self.inst.n
+        out = np.zeros(3)
+        self._chk(self.L.lb_certify_dense(self.ctx, ptr(dg, _f64p), ptr(out, _f64p)), "lb_certify_dense")
+        return out
+
     def reduce_exact(self, n, keys, rhash=None):
         """Exact reduction of the graph with sorted unique keys u*n+v (u < v) on this context's device
         (reduce_exact.py:12-62). Returns dict(group_of, N, vol, keys, weights, leaf_unions, twin_unions,

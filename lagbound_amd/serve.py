@@ -2,7 +2,8 @@
 and device memory free up, and each runs the whole bound pipeline with its V and multipliers resident on the device:
 instance build -> seeded label-propagation levels -> device ALM search (pair level, VCS level, triangle and VCS
 separation, PHR updates; lagbound_amd.alm.Solver in process) -> candidate bound (device C(y)X products, FP64
-assembly; lagbound_amd.bound.estimate) -> dense FP64 certificate when the graph is small enough.
+assembly; lagbound_amd.bound.estimate) -> dense FP64 certificate (assembly and Cholesky on the device) when the
+graph fits it.
 
 Concurrency: one host thread per slot, each with its own device contexts (own HIP stream), so the work of requests
 that are in flight at the same time overlaps on the GPU; the library's calls release the GIL (ctypes). Admission
@@ -48,10 +49,12 @@ class Params:
     P: int = 64                      # label-propagation start clusters
     rgd_iters: int = 120
     eig_maxiter: int = 60
-    eig_extra: int = 8
+    eig_block: int = 8               # LOBPCG block width (the smallest eigenvalue only is needed)
+    lpa_iters: int = 3
     escape: int = 0
-    cert_max_n: int = 4000           # dense FP64 certificate up to this many vertices
+    cert_max_n: int = 40000          # dense FP64 certificate (device, rocSOLVER) up to this many vertices
     cert_margin: float = 1e-7
+    cert_on_device: bool = True
     blas_threads: int = 4            # host BLAS threads while serving (slots x threads ~ the host cores)
 
 
@@ -88,7 +91,8 @@ def footprint_bytes(n, m, k, p: Params):
     tri_cap = (p.tri_add or 2 * n) * max(1, p.outer // max(1, p.tri_every))
     solver = 6 * n * k * 4 + 2 * m * 12 + tri_cap * (16 * 3 + 16) + 8 * n * 24 + 64 * n
     est = 3 * n * k * 8 + 2 * m * 12 + tri_cap * (24 * 3 + 16) + 8 * n * 24
-    return int(1.25 * (solver + est)) + (32 << 20)
+    cert = 16 * n * n if n <= p.cert_max_n else 0          # dense M and its magnitudes (FP64)
+    return int(1.25 * (solver + est)) + cert + (32 << 20)
 
 
 def modularity(inst, lab):
@@ -105,7 +109,7 @@ def solve_request(req: Request, p: Params, device=0):
     logs = []
     log = lambda *s: logs.append(" ".join(str(x) for x in s))
     inst = I.build_instance(req.n, req.keys)
-    lab = I.label_propagation(inst.rowptr, inst.col, inst.n, req.seed, P=p.P)
+    lab = I.label_propagation(inst.rowptr, inst.col, inst.n, req.seed, P=p.P, iters=p.lpa_iters)
     nb = p.nbatch or nbatch_rule(inst.n)
     argv = ["--tag", f"req{req.rid}", "--k", str(p.k), "--nbatch", str(nb), "--inner", "1", "--est_every", "0",
             "--mom", "0", "--outer", str(p.outer), "--time_limit", "1e9", "--seed", str(req.seed),
@@ -117,14 +121,14 @@ def solve_request(req: Request, p: Params, device=0):
     s.run()
     lag = B.DeviceLagrangian(inst, r=p.k, dtype="f64", device=device)
     try:
-        est = B.estimate(inst, s.V, lag=lag, rgd_iters=p.rgd_iters, eig_maxiter=p.eig_maxiter, eig_extra=p.eig_extra,
+        est = B.estimate(inst, s.V, lag=lag, rgd_iters=p.rgd_iters, eig_maxiter=p.eig_maxiter, eig_block=p.eig_block,
                          escape=p.escape, log=log)
     finally:
         lag.close()
     t_dev = s.eng.lib_s + lag.eng.lib_s
     cert = None
     if inst.n <= p.cert_max_n:
-        cert = B.certify_estimate(inst, est, margin=p.cert_margin)
+        cert = B.certify_estimate(inst, est, margin=p.cert_margin, device=device if p.cert_on_device else None)
     return dict(n=inst.n, m=inst.nnz // 2, lb=modularity(inst, lab.astype(np.int64)), dv=est["dv"], ub=est["ub"],
                 certified=bool(cert and cert["passed"]), UB=(cert["UB"] if cert and cert["passed"] else None),
                 log=logs), t_dev

@@ -71,3 +71,26 @@ def test_solve_estimate_certify_end_to_end(tmp_path):
           f"price {cert['price']:.2e}", flush=True)
     assert cert["UB"] >= q
     assert abs(cert["UB"] - res["ub"]) <= 1e-7 * res["s_vol"]["sum_s"] / inst.L + cert["price"] + 1e-12
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_device_certificate_matches_host(weighted):
+    n_, keys, lab = I.sbm(500, 5, 0.08, 0.01, 4)
+    w = np.random.default_rng(1).integers(1, 4, len(keys)).astype(np.float64) if weighted else None
+    inst = _with_multipliers(I.build_instance(n_, keys, weights=w), 3, scale=0.01, ntri=300)
+    Cm, _ = B.dense_C(inst)
+    W = np.random.default_rng(2).standard_normal((n_, 8))
+    W /= np.linalg.norm(W, axis=1, keepdims=True)
+    g = np.einsum("ij,ij->i", Cm @ W, W)
+    s = np.maximum(inst.d, 1.0)
+    import scipy.linalg as sl
+    mu = float(sl.eigh(np.diag(g) - Cm, np.diag(s), eigvals_only=True)[0])
+    print(f"[bound] device certificate weighted={weighted} n={n_} mu={mu:.6e}", flush=True)
+    t = (max(0.0, -mu) + 1e-7) * s
+    host = B.certify_dense(inst, g, t)
+    dev = B.certify_dense_device(inst, g, t)
+    assert host["passed"] and dev["passed"]
+    assert abs(dev["UB"] - host["UB"]) <= 1e-9 * abs(host["UB"])
+    assert dev["price"] < 1e-9
+    bad = B.certify_dense_device(inst, g, (max(0.0, -mu) - 1e-5) * s)
+    assert not bad["passed"]

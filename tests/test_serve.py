@@ -22,7 +22,8 @@ class _Lag(OracleLagrangian):
 
 
 def _params():
-    return serve.Params(k=8, outer=6, tri_every=3, vcs_every=2, rgd_iters=20, eig_maxiter=40, cert_max_n=400)
+    return serve.Params(k=8, outer=6, tri_every=3, vcs_every=2, rgd_iters=20, eig_maxiter=40, cert_max_n=400,
+                        cert_on_device=False)
 
 
 def test_footprint_and_rule():
