@@ -165,6 +165,18 @@ int lb_reduce_exact(lb_ctx *ctx, int64_t n, int64_t m, const int64_t *keys, cons
  * out[1] + ||F||_F). Needs 16 n^2 bytes of free device memory. */
 int lb_certify_dense(lb_ctx *ctx, const double *dg, double *out);
 
+/* Batch mode (no reference counterpart: the reference solves one graph per process): the graph set by lb_set_graph
+ * is the block-diagonal union of G graphs with contiguous row ranges row_off (G+1, from 0 to n), each with its own
+ * L_g and C0_g. Edges and triangles must stay inside one graph (checked). lb_batch_set_levels gives each graph one
+ * pair level: lab (n, cluster ids local to the row's graph), P (G), sig (concatenated P_g x P_g), rq and dsq
+ * (concatenated P_g). lb_batch_eval evaluates every graph in one launch of each kernel and writes parts (G x
+ * LB_NPARTS, the layout of lb_eval_fgrad per graph) and G = C(y)V for all rows (lb_get_G). VCS levels, sweeps and
+ * separation act on single-graph contexts only. */
+int lb_batch_set(lb_ctx *ctx, int G, const int64_t *row_off, const double *L, const double *C0);
+int lb_batch_set_levels(lb_ctx *ctx, const int32_t *lab, const int32_t *P, const double *sig, const double *rq,
+                        const double *dsq, double pairlvl_weight);
+int lb_batch_eval(lb_ctx *ctx, double rho, double rt, double *parts);
+
 #ifdef __cplusplus
 }
 #endif

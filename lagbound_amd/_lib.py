@@ -17,7 +17,8 @@ EXPORTS = ["lb_create", "lb_destroy", "lb_last_error", "lb_set_graph", "lb_set_l
            "lb_set_vcs", "lb_set_vcs_sig", "lb_set_triangles", "lb_set_lam", "lb_set_V", "lb_get_V",
            "lb_eval_fgrad", "lb_get_G", "lb_get_multipliers", "lb_phr_update", "lb_sweeps", "lb_separate_triangles", "lb_separate_vcs", "lb_set_vcs_age", "lb_get_vcs_state",
            "lb_set_weights", "lb_sync", "lb_time_eval",
-           "lb_kernel_name", "lb_set_tuning", "lb_version", "lb_reduce_exact", "lb_set_option", "lb_certify_dense"]
+           "lb_kernel_name", "lb_set_tuning", "lb_version", "lb_reduce_exact", "lb_set_option", "lb_certify_dense", "lb_batch_set",
+           "lb_batch_set_levels", "lb_batch_eval"]
 
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
@@ -72,6 +73,9 @@ def load():
         "lb_version": ([], C.c_char_p),
         "lb_set_option": ([ctx, C.c_char_p, C.c_int64], C.c_int),
         "lb_certify_dense": ([ctx, _f64p, _f64p], C.c_int),
+        "lb_batch_set": ([ctx, C.c_int, _i64p, _f64p, _f64p], C.c_int),
+        "lb_batch_set_levels": ([ctx, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double], C.c_int),
+        "lb_batch_eval": ([ctx, C.c_double, C.c_double, _f64p], C.c_int),
         "lb_reduce_exact": ([ctx, C.c_int64, C.c_int64, _i64p, C.POINTER(C.c_uint64), _i64p, _i64p, _i64p, _f64p,
                              _i64p], C.c_int),
     }

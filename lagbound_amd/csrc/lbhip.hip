@@ -1726,6 +1726,11 @@ struct lb_ctx {
     std::vector<double> h_d;                // vertex volumes (host copy)
     DBuf rorder;                            // row processing order of lb_k_rows (descending work)
     bool use_order = true;
+    // batch mode: the graph is the block-diagonal union of bG graphs with contiguous row ranges
+    int bG = 0; std::vector<int64_t> hboff; std::vector<int> bP; int bsumP = 0; bool blev = false;
+    DBuf b_off, b_gid, b_L, b_C0, b_Poff, b_soff, b_cg, b_lab, b_sig, b_pp, b_rq, b_dsq, b_zY, b_M, b_pdiag, b_ltot,
+        b_lviol, b_rowv, b_parts, b_lamsum;
+    double bw = 1.0;
 };
 
 namespace {
@@ -2493,7 +2498,10 @@ void lb_destroy(lb_ctx *c) {
                     &c->Y, &c->z, &c->M, &c->pdiag, &c->levtot, &c->levviol, &c->Z, &c->colsum_part, &c->vcs_part,
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
                     &c->hTseg0, &c->parts, &c->lamsum, &c->order, &c->tmask, &c->Kq, &c->stZ, &c->Zs, &c->cnts,
-                    &c->Vstage, &c->DDb, &c->segkap, &c->seglist, &c->vslack, &c->vage, &c->vvol, &c->rorder};
+                    &c->Vstage, &c->DDb, &c->segkap, &c->seglist, &c->vslack, &c->vage, &c->vvol, &c->rorder,
+                    &c->b_off, &c->b_gid, &c->b_L, &c->b_C0, &c->b_Poff, &c->b_soff, &c->b_cg, &c->b_lab, &c->b_sig,
+                    &c->b_pp, &c->b_rq, &c->b_dsq, &c->b_zY, &c->b_M, &c->b_pdiag, &c->b_ltot, &c->b_lviol, &c->b_rowv,
+                    &c->b_parts, &c->b_lamsum};
     for (DBuf *b : bufs) dfree(*b);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -2527,6 +2535,7 @@ int lb_set_graph(lb_ctx *c, int64_t n, const int64_t *rowptr, const int32_t *col
     for (int64_t e = 0; e < nnz; e++)
         if (col[e] < 0 || col[e] >= n) return fail(c, -1, "lb_set_graph: column index out of range");
     c->n = n; c->nnz = nnz; c->L = L; c->C0 = C0; c->weighted = weights != nullptr;
+    c->bG = 0; c->blev = false; c->bsumP = 0;
     c->h_rowptr.assign(rowptr, rowptr + n + 1);
     c->h_d.assign(d, d + n);
     int rc;
@@ -2693,6 +2702,14 @@ int lb_set_triangles(lb_ctx *c, int64_t T, const int32_t *TI, const double *lam)
     }
     for (int64_t e = 0; e < 3 * T; e++)
         if (TI[e] < 0 || TI[e] >= n) return fail(c, -1, "lb_set_triangles: vertex out of range");
+    if (c->bG) {                                      // batch mode: a triangle lies inside one graph
+        auto graph_of = [&](int64_t v) { return std::upper_bound(c->hboff.begin(), c->hboff.end(), v) - c->hboff.begin(); };
+        for (int64_t t = 0; t < T; t++) {
+            const auto g0 = graph_of(TI[3 * t]);
+            if (graph_of(TI[3 * t + 1]) != g0 || graph_of(TI[3 * t + 2]) != g0)
+                return fail(c, -1, "lb_set_triangles: a triangle joins two graphs of the batch");
+        }
+    }
     // vertex -> (t, role) incidence in t order (alm_vcs.py:94-104)
     std::vector<int64_t> cnt(n + 1, 0);
     for (int64_t e = 0; e < 3 * T; e++) cnt[TI[e] + 1]++;
@@ -2793,6 +2810,7 @@ int lb_get_V(lb_ctx *c, void *V, int host_dtype) {
 
 int lb_eval_fgrad(lb_ctx *c, double rho, double rt, double *parts) {
     if (!c || !c->n) return fail(c, -1, "lb_eval_fgrad: no graph");
+    if (c->bG) return fail(c, -1, "lb_eval_fgrad: the context holds a batch (use lb_batch_eval)");
     if (!(rho >= 0.0) || (rho > 0.0 && c->T && !(rt > 0.0))) return fail(c, -1, "lb_eval_fgrad: need rho >= 0 and rt > 0 when rho > 0");
     HIPCHK(c, hipSetDevice(c->device));
     eval_fn f = pick_eval(c->vdtype, c->r);
@@ -3481,6 +3499,378 @@ int lb_certify_dense(lb_ctx *c, const double *dg, double *out) {
     cleanup();
 #undef CCHK
 #undef CHIP
+    return 0;
+}
+
+}  // extern "C"
+
+// ================================================================================================ batch mode
+// Many independent graphs in one context: the graph uploaded by lb_set_graph is the block-diagonal union of bG
+// graphs with contiguous row ranges; each graph g has its own L_g, C0_g, rank-1 term z_g = d_g^T V_g and one pair
+// level of P_g clusters (the triangle pool holds triangles inside the graphs). One launch of each kernel below covers
+// every graph: per-graph column sums (one workgroup per graph), per-graph cluster rows, one wavefront per row over
+// all rows of all graphs, and per-graph fixed-order finals. The arithmetic per row and per cluster pair is that of
+// the single-graph evaluation (augmented / CV, alm_vcs.py:1496-1533, 1299-1312).
+namespace {
+
+struct BArgs {
+    const uint4 *V; int64_t n; const int64_t *rowptr; const int32_t *col; const void *w; const double *d;
+    const int32_t *gid; const int64_t *boff; int G; const double *Lg; const double *C0g;
+    int lev, ylds; const int32_t *lab; const int32_t *Poff; const int64_t *soff; const int32_t *cg; int sumP;
+    const double *sig; double *pp; const double *rq; const double *dsq; double wpair;
+    double *zY; double *M; double *pdiag; double *ltot; double *lviol;
+    const int64_t *iptr; const void *rec; void *pt; double rho, rt; int64_t T; const int32_t *TI; const void *lam;
+    void *Gout; double *rowv; double *parts; double *lamsum;
+};
+
+// z_g and the cluster sums of graph g = blockIdx.x (Y in LDS when it fits, else global atomics)
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_bcolsum(BArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    extern __shared__ double shY[];
+    __shared__ double zpart[LB_BLOCK * CPL];
+    const int g = blockIdx.x;
+    const int64_t r0 = a.boff[g], r1 = a.boff[g + 1];
+    const int P = a.lev ? a.Poff[g + 1] - a.Poff[g] : 0;
+    const bool lds = a.lev && a.ylds;              // the launch sized the LDS for the largest P of the batch
+    double *Yg = a.zY + (int64_t)a.G * R + (a.lev ? (int64_t)a.Poff[g] * R : 0);
+    if (lds) {
+        for (int e = threadIdx.x; e < P * R; e += LB_BLOCK) shY[e] = 0.0;
+        __syncthreads();
+    }
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    double zc[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; c++) zc[c] = 0.0;
+    for (int64_t i = r0 + slot; i < r1; i += NSL) {
+        TC x[CPL];
+        VT<TV>::unpack(a.V[i * S + q], x);
+        const double di = a.d[i];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) zc[c] += di * (double)x[c];
+        if (a.lev) {
+            const int Q = a.lab[i];
+            double *dst = lds ? shY + Q * R + q * CPL : Yg + (int64_t)Q * R + q * CPL;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) atomicAdd(dst + c, di * (double)x[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CPL; c++) zpart[threadIdx.x * CPL + c] = zc[c];
+    __syncthreads();
+    // z: column k = q*CPL + c is held by threads with threadIdx.x % S == q; summed over the slots in a fixed order
+    for (int k = threadIdx.x; k < R; k += LB_BLOCK) {
+        const int qq = k / CPL, cc = k % CPL;
+        double z = 0.0;
+        for (int sl = 0; sl < NSL; sl++) z += zpart[(sl * S + qq) * CPL + cc];
+        a.zY[(int64_t)g * R + k] = z;
+    }
+    if (lds)
+        for (int e = threadIdx.x; e < P * R; e += LB_BLOCK) Yg[e] = shY[e];
+}
+
+// one wavefront per cluster row Q (global id blockIdx.x) of its graph's pair level (lb_k_levels per graph)
+__global__ __launch_bounds__(64) void lb_k_blevels(BArgs a, int R) {
+    const int Qg = blockIdx.x, g = a.cg[Qg];
+    const int P = a.Poff[g + 1] - a.Poff[g], Q = Qg - a.Poff[g];
+    const double *Yb = a.zY + (int64_t)a.G * R + (int64_t)a.Poff[g] * R;
+    const double *YQ = Yb + (int64_t)Q * R;
+    const double *sig = a.sig + a.soff[g] + (int64_t)Q * P;
+    double *pp = a.pp + a.soff[g] + (int64_t)Q * P;
+    const double rqQ = a.rq[Qg];
+    double Macc[32];
+    for (int c = 0; c < R; c++) Macc[c] = 0.0;
+    double tot = 0.0, viol = 0.0, pq = 0.0;
+    for (int Rr = threadIdx.x; Rr < P; Rr += 64) {
+        const double *YR = Yb + (int64_t)Rr * R;
+        double p = sig[Rr];
+        if (a.rho > 0.0) {
+            double h = 0.0;
+            for (int c = 0; c < R; c++) h += YQ[c] * YR[c];
+            if (Rr == Q) h -= a.dsq[Qg];
+            const double rr = a.rho * ((rqQ * a.rq[a.Poff[g] + Rr]) * a.wpair);
+            const double sg = p;
+            p = sg - rr * h;
+            if (p < 0.0) p = 0.0;
+            tot += (p * p - sg * sg) / (2 * rr);
+            if (h < -1e-9) viol += 1.0;
+        }
+        pp[Rr] = p;
+        if (Rr == Q) pq = p;
+        if (p != 0.0)
+            for (int c = 0; c < R; c++) Macc[c] += p * YR[c];
+    }
+    tot = wave_sum(tot); viol = wave_sum(viol); pq = wave_sum(pq);
+    for (int c = 0; c < R; c++) Macc[c] = wave_sum(Macc[c]);
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < R; c++) a.M[(int64_t)Qg * R + c] = Macc[c];
+        a.pdiag[Qg] = pq; a.ltot[Qg] = tot; a.lviol[Qg] = viol;
+    }
+}
+
+// one wavefront per row: NG = 64 / S groups of S lanes take the row's edges and triangle records NG at a time,
+// then an xor-shuffle sums the groups; rowv[4 i ..] = (V_i.(AV)_i, G_i.V_i, triangle PHR term, min 1 - tau)
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_brows(BArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NG = 64 / S;
+    const int lane = threadIdx.x & 63, grp = lane / S, q = lane % S;
+    const int64_t w0 = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
+    const int64_t nw = (int64_t)gridDim.x * (LB_BLOCK / 64);
+    const TC *wt = (const TC *)a.w;
+    const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
+    TC *pt = (TC *)a.pt;
+    for (int64_t i = w0; i < a.n; i += nw) {
+        TC x[CPL], ga[CPL], gt[CPL];
+        VT<TV>::unpack(a.V[i * S + q], x);
+#pragma unroll
+        for (int c = 0; c < CPL; c++) { ga[c] = 0; gt[c] = 0; }
+        const int64_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+        for (int64_t e = e0 + grp; e < e1; e += NG) {
+            TC xj[CPL];
+            VT<TV>::unpack(a.V[(int64_t)a.col[e] * S + q], xj);
+            const TC we = wt ? wt[e] : (TC)1;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) ga[c] += we * xj[c];
+        }
+        double ttot = 0.0, tmin = INFINITY;
+        const int64_t k0 = a.iptr[i], k1 = a.iptr[i + 1];
+        for (int64_t k = k0 + grp; k < k1; k += NG) {
+            const TriRec<TC> rc = rec[k];
+            TC p, tau;
+            tri_record<TV, R>(a.V, rc, x, q, a.rho, a.rt, gt, p, tau);
+            if (rc.code % 3 == 0 && q == 0) {
+                pt[rc.code / 3] = p;
+                if (a.rho > 0.0) {
+                    ttot += ((double)p * p - (double)rc.lam * rc.lam) / (2 * a.rt);
+                    tmin = fmin(tmin, (double)((TC)1 - tau));
+                }
+            }
+        }
+        for (int m = S; m < 64; m <<= 1) {
+#pragma unroll
+            for (int c = 0; c < CPL; c++) { ga[c] += __shfl_xor(ga[c], m); gt[c] += __shfl_xor(gt[c], m); }
+            ttot += __shfl_xor(ttot, m);
+            tmin = fmin(tmin, __shfl_xor(tmin, m));
+        }
+        // every group now holds the sums; group 0 finishes the row
+        const int g = a.gid[i];
+        const double di = a.d[i], Lg = a.Lg[g];
+        TC qd = 0, g2[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) qd += x[c] * ga[c];
+        qd = slot_sum<S>(qd);
+#pragma unroll
+        for (int c = 0; c < CPL; c++) g2[c] = ga[c] + gt[c] - (TC)(di * a.zY[(int64_t)g * R + q * CPL + c] / Lg);
+        if (a.lev) {
+            const int Qg = a.Poff[g] + a.lab[i];
+            const double pd = a.pdiag[Qg] * di;
+            const double *M = a.M + (int64_t)Qg * R + q * CPL;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) g2[c] += (TC)(di * (M[c] - pd * (double)x[c]));
+        }
+        TC gd = 0;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) gd += g2[c] * x[c];
+        gd = slot_sum<S>(gd);
+        if (grp == 0) {
+            TC *Go = (TC *)a.Gout + i * R + q * CPL;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) Go[c] = g2[c];
+            if (q == 0) {
+                double *o = a.rowv + 4 * i;
+                o[0] = (double)qd; o[1] = (double)gd; o[2] = ttot; o[3] = tmin;
+            }
+        }
+    }
+}
+
+template <typename TC>
+__global__ void lb_k_blamsum(BArgs a) {
+    const TC *lam = (const TC *)a.lam;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < a.T; t += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(a.lamsum + a.gid[a.TI[3 * t]], (double)lam[t]);
+}
+
+// per-graph scalars (graph = blockIdx.x), fixed-order sums over its rows and clusters
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_bfinal(BArgs a, int R) {
+    __shared__ double red[LB_BLOCK / 64];
+    const int g = blockIdx.x;
+    double quad = 0, gv = 0, tt = 0, tmin = INFINITY, lt = 0, lv = 0;
+    for (int64_t i = a.boff[g] + threadIdx.x; i < a.boff[g + 1]; i += LB_BLOCK) {
+        const double *o = a.rowv + 4 * i;
+        quad += o[0]; gv += o[1]; tt += o[2]; tmin = fmin(tmin, o[3]);
+    }
+    if (a.lev)
+        for (int Q = a.Poff[g] + threadIdx.x; Q < a.Poff[g + 1]; Q += LB_BLOCK) { lt += a.ltot[Q]; lv += a.lviol[Q]; }
+    quad = block_sum(quad, red); __syncthreads();
+    gv = block_sum(gv, red); __syncthreads();
+    tt = block_sum(tt, red); __syncthreads();
+    tmin = block_min(tmin, red); __syncthreads();
+    lt = block_sum(lt, red); __syncthreads();
+    lv = block_sum(lv, red);
+    if (threadIdx.x == 0) {
+        double zz = 0.0;
+        for (int c = 0; c < R; c++) zz += a.zY[(int64_t)g * R + c] * a.zY[(int64_t)g * R + c];
+        const double L = a.Lg[g], C0 = a.C0g[g], ls = a.lamsum[g];
+        double *p = a.parts + (int64_t)g * LB_NPARTS;
+        p[2] = zz / L; p[7] = C0; p[11] = ls;
+        p[6] = quad - zz / L + C0;
+        if (a.rho > 0.0) {
+            p[1] = quad; p[3] = lt; p[4] = tt; p[5] = 0.0; p[9] = tmin;
+            p[0] = quad - zz / L - lt - tt + C0;
+        } else {
+            p[1] = gv; p[3] = 0.0; p[4] = 0.0; p[5] = 0.0; p[9] = INFINITY;
+            p[0] = C0 + ls + gv;
+        }
+        p[8] = lv; p[10] = 0.0;
+    }
+}
+
+template <typename TV, int R>
+struct BLaunch {
+    typedef typename VT<TV>::TC TC;
+    static int eval(lb_ctx *c, double rho, double rt, const BArgs &a0) {
+        BArgs a = a0;
+        a.rho = rho; a.rt = rt;
+        hipStream_t s = c->stream;
+        const int64_t ylen = (int64_t)c->bG * R + (int64_t)c->bsumP * R;
+        hipLaunchKernelGGL(lb_k_fill, dim3(256), dim3(256), 0, s, a.zY, ylen, 0.0);
+        hipLaunchKernelGGL(lb_k_fill, dim3(64), dim3(256), 0, s, a.lamsum, (int64_t)c->bG, 0.0);
+        int maxP = 0;
+        for (int P : c->bP) maxP = std::max(maxP, P);
+        const size_t lds = (c->blev && (size_t)maxP * R * 8 <= 32 * 1024) ? (size_t)maxP * R * 8 : 0;
+        a.ylds = lds > 0 ? 1 : 0;
+        hipLaunchKernelGGL((lb_k_bcolsum<TV, R>), dim3(c->bG), dim3(LB_BLOCK), lds, s, a);
+        if (c->blev && c->bsumP) hipLaunchKernelGGL(lb_k_blevels, dim3(c->bsumP), dim3(64), 0, s, a, R);
+        if (c->T) {
+            if (c->cbytes == 8) hipLaunchKernelGGL(lb_k_blamsum<double>, dim3(1024), dim3(256), 0, s, a);
+            else hipLaunchKernelGGL(lb_k_blamsum<float>, dim3(1024), dim3(256), 0, s, a);
+        }
+        const int64_t nblk = std::min<int64_t>((c->n + 3) / 4, 8192);
+        hipLaunchKernelGGL((lb_k_brows<TV, R>), dim3((unsigned)nblk), dim3(LB_BLOCK), 0, s, a);
+        hipLaunchKernelGGL(lb_k_bfinal, dim3(c->bG), dim3(LB_BLOCK), 0, s, a, R);
+        HIPCHK(c, hipGetLastError());
+        return 0;
+    }
+};
+
+typedef int (*beval_fn)(lb_ctx *, double, double, const BArgs &);
+beval_fn pick_beval(int vdtype, int r) {
+#define LB_PICK(TV)                                     \
+    if (r == 8) return &BLaunch<TV, 8>::eval;           \
+    if (r == 16) return &BLaunch<TV, 16>::eval;         \
+    if (r == 32) return &BLaunch<TV, 32>::eval;
+    if (vdtype == LB_F32) { LB_PICK(float) }
+    if (vdtype == LB_BF16) { LB_PICK(bf16raw) }
+    if (vdtype == LB_F64) { LB_PICK(double) }
+#undef LB_PICK
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lb_batch_set(lb_ctx *c, int G, const int64_t *row_off, const double *L, const double *C0) {
+    if (!c || !c->n) return fail(c, -1, "lb_batch_set: set the graph (the union of the batch) first");
+    if (G < 1 || !row_off || !L || !C0) return fail(c, -1, "lb_batch_set: bad arguments");
+    if (row_off[0] != 0 || row_off[G] != c->n) return fail(c, -1, "lb_batch_set: row offsets must run from 0 to n");
+    for (int g = 0; g < G; g++) {
+        if (row_off[g + 1] <= row_off[g]) return fail(c, -1, "lb_batch_set: every graph needs at least one row");
+        if (!(L[g] > 0)) return fail(c, -1, "lb_batch_set: L must be positive");
+    }
+    if (c->nvcs) return fail(c, -1, "lb_batch_set: VCS levels are not supported in batch mode");
+    std::vector<int32_t> gid((size_t)c->n);
+    for (int g = 0; g < G; g++)
+        for (int64_t i = row_off[g]; i < row_off[g + 1]; i++) gid[i] = g;
+    // edges must stay inside their graph (the union is block-diagonal)
+    std::vector<int32_t> col((size_t)c->nnz);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpy(col.data(), c->col.p, sizeof(int32_t) * (size_t)c->nnz, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < c->n; i++)
+        for (int64_t e = c->h_rowptr[i]; e < c->h_rowptr[i + 1]; e++)
+            if (gid[col[e]] != gid[i]) return fail(c, -1, "lb_batch_set: an edge joins two graphs");
+    int rc;
+    c->bG = G;
+    c->hboff.assign(row_off, row_off + G + 1);
+    c->blev = false; c->bP.assign(G, 0); c->bsumP = 0;
+    if ((rc = upload(c, c->b_off, row_off, sizeof(int64_t) * (G + 1)))) return rc;
+    if ((rc = upload(c, c->b_gid, gid.data(), sizeof(int32_t) * gid.size()))) return rc;
+    if ((rc = upload(c, c->b_L, L, sizeof(double) * G))) return rc;
+    if ((rc = upload(c, c->b_C0, C0, sizeof(double) * G))) return rc;
+    if ((rc = dalloc(c, c->b_rowv, sizeof(double) * 4 * (size_t)c->n))) return rc;
+    if ((rc = dalloc(c, c->b_parts, sizeof(double) * LB_NPARTS * (size_t)G))) return rc;
+    if ((rc = dalloc(c, c->b_lamsum, sizeof(double) * (size_t)G))) return rc;
+    if ((rc = dalloc(c, c->b_zY, sizeof(double) * (size_t)G * c->r))) return rc;
+    c->have_eval = false;
+    return 0;
+}
+
+int lb_batch_set_levels(lb_ctx *c, const int32_t *lab, const int32_t *P, const double *sig, const double *rq,
+                        const double *dsq, double pairlvl_weight) {
+    if (!c || !c->bG) return fail(c, -1, "lb_batch_set_levels: call lb_batch_set first");
+    if (!lab || !P || !sig || !rq || !dsq) return fail(c, -1, "lb_batch_set_levels: bad arguments");
+    const int G = c->bG;
+    std::vector<int32_t> Poff(G + 1, 0), cg;
+    std::vector<int64_t> soff(G + 1, 0);
+    for (int g = 0; g < G; g++) {
+        if (P[g] < 1) return fail(c, -1, "lb_batch_set_levels: every graph needs at least one cluster");
+        Poff[g + 1] = Poff[g] + P[g];
+        soff[g + 1] = soff[g] + (int64_t)P[g] * P[g];
+        for (int q = 0; q < P[g]; q++) cg.push_back(g);
+        for (int64_t i = c->hboff[g]; i < c->hboff[g + 1]; i++)
+            if (lab[i] < 0 || lab[i] >= P[g]) return fail(c, -1, "lb_batch_set_levels: label out of range");
+    }
+    int rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int sumP = Poff[G];
+    c->bP.assign(P, P + G); c->bsumP = sumP; c->blev = true; c->bw = pairlvl_weight;
+    if ((rc = upload(c, c->b_lab, lab, sizeof(int32_t) * (size_t)c->n))) return rc;
+    if ((rc = upload(c, c->b_Poff, Poff.data(), sizeof(int32_t) * (G + 1)))) return rc;
+    if ((rc = upload(c, c->b_soff, soff.data(), sizeof(int64_t) * (G + 1)))) return rc;
+    if ((rc = upload(c, c->b_cg, cg.data(), sizeof(int32_t) * cg.size()))) return rc;
+    if ((rc = upload(c, c->b_sig, sig, sizeof(double) * soff[G]))) return rc;
+    if ((rc = dalloc(c, c->b_pp, sizeof(double) * soff[G]))) return rc;
+    if ((rc = upload(c, c->b_rq, rq, sizeof(double) * sumP))) return rc;
+    if ((rc = upload(c, c->b_dsq, dsq, sizeof(double) * sumP))) return rc;
+    if ((rc = dalloc(c, c->b_zY, sizeof(double) * ((size_t)G + sumP) * c->r))) return rc;
+    if ((rc = dalloc(c, c->b_M, sizeof(double) * (size_t)sumP * c->r))) return rc;
+    if ((rc = dalloc(c, c->b_pdiag, sizeof(double) * sumP))) return rc;
+    if ((rc = dalloc(c, c->b_ltot, sizeof(double) * sumP))) return rc;
+    if ((rc = dalloc(c, c->b_lviol, sizeof(double) * sumP))) return rc;
+    c->have_eval = false;
+    return 0;
+}
+
+int lb_batch_eval(lb_ctx *c, double rho, double rt, double *parts) {
+    if (!c || !c->bG) return fail(c, -1, "lb_batch_eval: not in batch mode (lb_batch_set)");
+    if (!(rho >= 0.0) || (rho > 0.0 && c->T && !(rt > 0.0))) return fail(c, -1, "lb_batch_eval: need rho >= 0 and rt > 0 when rho > 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    beval_fn f = pick_beval(c->vdtype, c->r);
+    if (!f) return fail(c, -1, "lb_batch_eval: unsupported rank / dtype");
+    BArgs a{};
+    a.V = (const uint4 *)c->V.p; a.n = c->n; a.rowptr = (const int64_t *)c->rowptr.p; a.col = (const int32_t *)c->col.p;
+    a.w = c->weighted ? c->w.p : nullptr; a.d = (const double *)c->d.p;
+    a.gid = (const int32_t *)c->b_gid.p; a.boff = (const int64_t *)c->b_off.p; a.G = c->bG;
+    a.Lg = (const double *)c->b_L.p; a.C0g = (const double *)c->b_C0.p;
+    a.lev = c->blev ? 1 : 0; a.lab = (const int32_t *)c->b_lab.p; a.Poff = (const int32_t *)c->b_Poff.p;
+    a.soff = (const int64_t *)c->b_soff.p; a.cg = (const int32_t *)c->b_cg.p; a.sumP = c->bsumP;
+    a.sig = (const double *)c->b_sig.p; a.pp = (double *)c->b_pp.p; a.rq = (const double *)c->b_rq.p;
+    a.dsq = (const double *)c->b_dsq.p; a.wpair = c->bw;
+    a.zY = (double *)c->b_zY.p; a.M = (double *)c->b_M.p; a.pdiag = (double *)c->b_pdiag.p;
+    a.ltot = (double *)c->b_ltot.p; a.lviol = (double *)c->b_lviol.p;
+    a.iptr = (const int64_t *)c->iptr.p; a.rec = c->rec.p; a.pt = c->pt.p; a.T = c->T; a.TI = (const int32_t *)c->TI.p;
+    a.lam = c->lam.p;
+    a.Gout = c->G.p; a.rowv = (double *)c->b_rowv.p; a.parts = (double *)c->b_parts.p; a.lamsum = (double *)c->b_lamsum.p;
+    int rc = f(c, rho, rt, a);
+    if (rc) return rc;
+    if (parts) {
+        HIPCHK(c, hipMemcpyAsync(parts, c->b_parts.p, sizeof(double) * LB_NPARTS * (size_t)c->bG, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    c->have_eval = true; c->last_rho = rho;
     return 0;
 }
 

@@ -226,6 +226,34 @@ class Engine:
         self._chk(self.L.lb_set_weights(self.ctx, float(pairlvl_weight), float(vc_weight) * float(pairlvl_weight)),
                   "lb_set_weights")
 
+    # ------------------------------------------------------------------------------------------------ batch mode
+    def set_batch(self, row_off, L, C0):
+        row_off = np.ascontiguousarray(row_off, np.int64)
+        L = np.ascontiguousarray(L, np.float64)
+        C0 = np.ascontiguousarray(C0, np.float64)
+        self._chk(self.L.lb_batch_set(self.ctx, len(row_off) - 1, ptr(row_off, _i64p), ptr(L, _f64p), ptr(C0, _f64p)),
+                  "lb_batch_set")
+        self.batch_G = len(row_off) - 1
+
+    def set_batch_levels(self, lab, P, sig, rq, dsq, pairlvl_weight=1.0):
+        lab = np.ascontiguousarray(lab, np.int32)
+        P = np.ascontiguousarray(P, np.int32)
+        sig = np.ascontiguousarray(sig, np.float64)
+        rq = np.ascontiguousarray(rq, np.float64)
+        dsq = np.ascontiguousarray(dsq, np.float64)
+        self._chk(self.L.lb_batch_set_levels(self.ctx, ptr(lab, _i32p), ptr(P, _i32p), ptr(sig, _f64p), ptr(rq, _f64p),
+                                             ptr(dsq, _f64p), float(pairlvl_weight)), "lb_batch_set_levels")
+
+    def batch_eval(self, rho, rt=None, fetch=True):
+        """Evaluate every graph of the batch; returns parts (G x LB_NPARTS) or None (asynchronous)."""
+        rt = rho if rt is None else rt
+        if not fetch:
+            self._chk(self.L.lb_batch_eval(self.ctx, float(rho), float(rt), None), "lb_batch_eval")
+            return None
+        parts = np.zeros((self.batch_G, LB.LB_NPARTS))
+        self._chk(self.L.lb_batch_eval(self.ctx, float(rho), float(rt), ptr(parts, _f64p)), "lb_batch_eval")
+        return parts
+
     def certify_dense(self, dg):
         """lb_certify_dense at diagonal dg: returns (info, sum R_ij^2, sum of squared off-diagonal magnitudes)."""
         dg = np.ascontiguousarray(dg, np.float64)
