@@ -102,3 +102,93 @@ class BatchEngine:
 
     def close(self):
         self.eng.close()
+
+
+def c4_batch(count=512, n_lo=200, n_hi=20000, r=8, tri_per_n=10, seed=0, log=None):
+    """The batched-uneven workload: `count` SBM graphs, n log-uniform in [n_lo, n_hi], block count uniform in
+    {4..40}, p_in uniform in [0.02, 0.2], p_out = p_in / 25; a pair level on the planted blocks with sig = sym
+    |N(0,1)| / L; 10 n wedge triangles per graph with lam half 0, half Exp(1) / 4."""
+    rng = np.random.default_rng(seed)
+    insts = []
+    for g in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))
+        B = int(rng.integers(4, 41))
+        pin = float(rng.uniform(0.02, 0.2))
+        n2, keys, lab = I.sbm(n, min(B, n), pin, pin / 25, seed * 100003 + g)
+        inst = I.build_instance(n2, keys)
+        lv = I.pair_level(lab, inst.d)
+        S = np.abs(rng.standard_normal((lv.P, lv.P))) / inst.L
+        lv.sig = 0.5 * (S + S.T)
+        inst.levels = [lv]
+        inst.TI = I.wedge_triangles(inst.rowptr, inst.col, inst.d, tri_per_n * n2, seed * 100003 + g + 1)
+        inst.lam = np.where(rng.random(len(inst.TI)) < 0.5, 0.0, rng.exponential(0.25, len(inst.TI)))
+        insts.append(inst)
+        if log and (g + 1) % 64 == 0:
+            log(f"generated {g + 1}/{count} graphs")
+    return insts
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import sys
+    import time
+    ap = argparse.ArgumentParser(prog="lagbound_amd.batch", description="Time the batched evaluation of many "
+                                 "uneven graphs against one context per graph; prints one JSON line.")
+    ap.add_argument("--count", type=int, default=512)
+    ap.add_argument("--n_lo", type=int, default=200)
+    ap.add_argument("--n_hi", type=int, default=20000)
+    ap.add_argument("--r", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--rho", type=float, default=0.02)
+    ap.add_argument("--single", type=int, default=1, help="also time one context per graph")
+    a = ap.parse_args(argv)
+    log = lambda *s: print("[batch]", *s, file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    insts = c4_batch(a.count, a.n_lo, a.n_hi, a.r, log=log)
+    p = Packed(insts)
+    log(f"{p.G} graphs, n {p.n}, nnz {len(p.col)}, T {len(p.TI)}, built in {time.perf_counter() - t:.1f}s")
+    rng = np.random.default_rng(1)
+    V = rng.standard_normal((p.n, a.r))
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    be = BatchEngine(p, r=a.r, dtype="f32")
+    be.set_V(V)
+    for _ in range(3):
+        be.eval(a.rho, fetch=False)
+    be.sync()
+    t = time.perf_counter()
+    for _ in range(a.steps):
+        be.eval(a.rho, fetch=False)
+    be.sync()
+    ms_batch = 1e3 * (time.perf_counter() - t) / a.steps
+    be.close()
+    out = {"graphs": p.G, "n": p.n, "nnz": int(len(p.col)), "T": int(len(p.TI)), "r": a.r, "rho": a.rho,
+           "ms_per_batch_eval": round(ms_batch, 4), "graph_evals_per_s": round(p.G * 1e3 / ms_batch, 1),
+           "alg_bytes": int(p.n * a.r * 8 + len(p.col) * 4 + p.n * 24 + len(p.TI) * 16)}
+    out["hbm_GBps"] = round(out["alg_bytes"] / (ms_batch * 1e-3) / 1e9, 1)
+    log(f"batch: {ms_batch:.3f} ms per evaluation of all {p.G} graphs")
+    if a.single:
+        from .engine import Engine
+        tot = 0.0
+        for g, x in enumerate(insts):
+            e = Engine(r=a.r, dtype="f32")
+            e.set_instance(x)
+            e.set_V(V[p.rows(g)])
+            e.eval(a.rho, fetch=False)
+            e.sync()
+            t = time.perf_counter()
+            for _ in range(5):
+                e.eval(a.rho, fetch=False)
+            e.sync()
+            tot += (time.perf_counter() - t) / 5
+            e.close()
+        out["ms_one_context_per_graph"] = round(1e3 * tot, 4)
+        out["batch_speedup"] = round(1e3 * tot / ms_batch, 2)
+        log(f"one context per graph: {1e3 * tot:.3f} ms for all graphs")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())

@@ -2897,6 +2897,7 @@ int lb_phr_update(lb_ctx *c) {
 }
 
 int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap2, const int64_t *orders, int nbatch) {
+    if (c && c->bG) return fail(c, -1, "lb_sweeps: not available on a batch context");
     if (!c || !c->n) return fail(c, -1, "lb_sweeps: no graph");
     if (nsw < 0 || nbatch < 1 || nbatch > c->n || !orders) return fail(c, -1, "lb_sweeps: bad arguments");
     if (!(rho >= 0.0) || !(rt >= 0.0) || !(damp >= 0.0)) return fail(c, -1, "lb_sweeps: rho, rt, damp must be >= 0");
@@ -2961,6 +2962,7 @@ int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap
 int lb_separate_triangles(lb_ctx *c, int fam, double thresh, int64_t maxh, int64_t maxk, int64_t hub2, int64_t cap2,
                           double vtol, int64_t cap, int nchunk, int nb, int64_t *TI_out, double *val_out,
                           int64_t *count, double *vmin) {
+    if (c && c->bG) return fail(c, -1, "lb_separate_triangles: not available on a batch context");
     if (!c || !c->n) return fail(c, -1, "lb_separate_triangles: no graph");
     if ((fam != 0 && fam != 1) || maxh < 1 || maxk < 1 || hub2 < 0 || cap2 < 0 || cap < 0 || nchunk < 1 || nb < 1 ||
         !(vtol > 0.0) || !(vtol < 2.0) || !TI_out || !val_out || !count || !vmin)
@@ -2984,6 +2986,7 @@ int lb_separate_triangles(lb_ctx *c, int fam, double thresh, int64_t maxh, int64
 
 int lb_separate_vcs(lb_ctx *c, int m, double tol, int mode, int64_t nadd, int64_t *out_i, int32_t *out_R,
                     double *out_score, int64_t *count, double *stats) {
+    if (c && c->bG) return fail(c, -1, "lb_separate_vcs: not available on a batch context");
     if (!c || !c->n || m < 0 || m >= c->nvcs) return fail(c, -1, "lb_separate_vcs: no such VCS level");
     if (mode < 0 || mode > 2 || nadd < 0 || !out_i || !out_R || !out_score || !count || !stats)
         return fail(c, -1, "lb_separate_vcs: bad arguments");
@@ -3011,6 +3014,7 @@ int lb_sync(lb_ctx *c) {
 }
 
 int lb_time_eval(lb_ctx *c, double rho, double rt, int iters, double *ms_per_eval, double *kernel_ms) {
+    if (c && c->bG) return fail(c, -1, "lb_time_eval: not available on a batch context");
     if (!c || !c->n || iters <= 0) return fail(c, -1, "lb_time_eval: bad arguments");
     HIPCHK(c, hipSetDevice(c->device));
     eval_fn f = pick_eval(c->vdtype, c->r);
