@@ -31,7 +31,7 @@ def _compile(src, out, verbose, libs=()):
 def build(verbose: bool = False) -> str:
     """Compile every HIP source of the package (liblbhip.so, and the bandwidth probe liblbprobe.so used by the
     benchmark's traffic calibration); always recompiles."""
-    _compile(SRC, OUT, verbose, libs=["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+    _compile(SRC, OUT, verbose, libs=["-ldl"])
     _compile(PROBE_SRC, PROBE_OUT, verbose)
     return OUT
 

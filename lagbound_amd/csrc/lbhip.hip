@@ -3335,9 +3335,40 @@ int lb_reduce_exact(lb_ctx *c, int64_t n, int64_t m, const int64_t *keys, const 
 // turns (info, sum R_ij^2, sum of the magnitudes squared) into a proved shift: R^T R = M + E with
 // |E| <= gamma(n+1) |R^T||R| for any conventional Cholesky (Higham, Accuracy and Stability of Numerical Algorithms,
 // Thm 10.3), so lambda_min(M) >= -(gamma(n+1) ||R||_F^2 + ||F||_F).
+#include <dlfcn.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 namespace {
+
+// rocBLAS / rocSOLVER are opened on first use only (dlopen), so processes that never certify do not load them
+struct SolverApi {
+    bool ok = false; std::string err;
+    rocblas_status (*create)(rocblas_handle *) = nullptr;
+    rocblas_status (*destroy)(rocblas_handle) = nullptr;
+    rocblas_status (*set_stream)(rocblas_handle, hipStream_t) = nullptr;
+    rocblas_status (*potrf)(rocblas_handle, rocblas_fill, rocblas_int, double *, rocblas_int, rocblas_int *) = nullptr;
+};
+
+SolverApi load_solver_api() {
+    SolverApi a;
+    void *hb = dlopen("librocblas.so.5", RTLD_NOW | RTLD_LOCAL);
+    if (!hb) hb = dlopen("/opt/rocm/lib/librocblas.so.5", RTLD_NOW | RTLD_LOCAL);
+    void *hs = dlopen("librocsolver.so.0", RTLD_NOW | RTLD_LOCAL);
+    if (!hs) hs = dlopen("/opt/rocm/lib/librocsolver.so.0", RTLD_NOW | RTLD_LOCAL);
+    if (!hb || !hs) { a.err = std::string("cannot open rocBLAS / rocSOLVER: ") + dlerror(); return a; }
+    a.create = (rocblas_status(*)(rocblas_handle *))dlsym(hb, "rocblas_create_handle");
+    a.destroy = (rocblas_status(*)(rocblas_handle))dlsym(hb, "rocblas_destroy_handle");
+    a.set_stream = (rocblas_status(*)(rocblas_handle, hipStream_t))dlsym(hb, "rocblas_set_stream");
+    a.potrf = (rocblas_status(*)(rocblas_handle, rocblas_fill, rocblas_int, double *, rocblas_int, rocblas_int *))dlsym(hs, "rocsolver_dpotrf");
+    a.ok = a.create && a.destroy && a.set_stream && a.potrf;
+    if (!a.ok) a.err = "rocBLAS / rocSOLVER entry points not found";
+    return a;
+}
+
+const SolverApi &solver_api() {
+    static SolverApi a = load_solver_api();
+    return a;
+}
 
 struct CertArgs {
     int64_t n; const double *d; double L; const double *dg;
@@ -3481,14 +3512,16 @@ int lb_certify_dense(lb_ctx *c, const double *dg, double *out) {
     double sab = 0.0;
     for (double x : hp) sab += x;
     // factor
+    const SolverApi &api = solver_api();
+    if (!api.ok) { cleanup(); return fail(c, -3, "lb_certify_dense: " + api.err); }
     rocblas_handle hb = nullptr;
-    if (rocblas_create_handle(&hb) != rocblas_status_success) { cleanup(); return fail(c, -3, "rocblas_create_handle failed"); }
-    rocblas_set_stream(hb, s);
-    const rocblas_status st = rocsolver_dpotrf(hb, rocblas_fill_lower, (rocblas_int)n, (double *)M.p, (rocblas_int)n, (rocblas_int *)dinfo.p);
+    if (api.create(&hb) != rocblas_status_success) { cleanup(); return fail(c, -3, "rocblas_create_handle failed"); }
+    api.set_stream(hb, s);
+    const rocblas_status st = api.potrf(hb, rocblas_fill_lower, (rocblas_int)n, (double *)M.p, (rocblas_int)n, (rocblas_int *)dinfo.p);
     int info = -1;
     hipError_t e1 = hipMemcpyAsync(&info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, s);
     hipError_t e2 = hipStreamSynchronize(s);
-    rocblas_destroy_handle(hb);
+    api.destroy(hb);
     if (st != rocblas_status_success) { cleanup(); return fail(c, -3, "rocsolver_dpotrf failed with status " + std::to_string((int)st)); }
     CHIP(e1);
     CHIP(e2);

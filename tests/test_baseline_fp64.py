@@ -2,7 +2,6 @@
 CPU here; the bench runs it on the GPU)."""
 import numpy as np
 import pytest
-import torch
 
 import oracle as O
 from conftest import small_instance, chung_lu_instance
@@ -10,7 +9,8 @@ from conftest import small_instance, chung_lu_instance
 
 @pytest.mark.parametrize("rho", [0.0, 0.2])
 def test_plain_fp64_matches_oracle(rho):
-    from lagbound_amd.baseline_fp64 import PlainFP64
+    import torch                     # imported here only: a test process that drives the device library keeps
+    from lagbound_amd.baseline_fp64 import PlainFP64   # torch's bundled HIP runtime out (bench runs it apart)
     for inst, V in (small_instance(n=40, seed=1, k=16), chung_lu_instance(n=3000, seed=2, k=16)):
         pl = PlainFP64(inst, device="cpu")
         f, G = pl.eval(torch.as_tensor(V), rho)
