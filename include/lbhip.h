@@ -177,6 +177,13 @@ int lb_batch_set_levels(lb_ctx *ctx, const int32_t *lab, const int32_t *P, const
                         const double *dsq, double pairlvl_weight);
 int lb_batch_eval(lb_ctx *ctx, double rho, double rt, double *parts);
 
+/* Captured evaluation (no reference counterpart): record the evaluation chain (lb_eval_fgrad, or lb_batch_eval on
+ * a batch context) at fixed rho, rt into a HIP graph on the context's stream, then replay it `count` times with one
+ * launch each (parts as in the eager call, may be NULL). Any lb_set_*, lb_sweeps, lb_phr_update or separation call
+ * ends the capture; lb_set_V does not (replays read the current V). */
+int lb_capture_eval(lb_ctx *ctx, double rho, double rt);
+int lb_replay_eval(lb_ctx *ctx, int count, double *parts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1731,6 +1731,7 @@ struct lb_ctx {
     DBuf b_off, b_gid, b_L, b_C0, b_Poff, b_soff, b_cg, b_lab, b_sig, b_pp, b_rq, b_dsq, b_zY, b_M, b_pdiag, b_ltot,
         b_lviol, b_rowv, b_parts, b_lamsum;
     double bw = 1.0;
+    void *gexec = nullptr; double g_rho = 0.0;   // captured evaluation (hipGraphExec_t)
 };
 
 namespace {
@@ -1738,6 +1739,14 @@ namespace {
 int fail(lb_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
     return code;
+}
+
+// any change of the context's buffers or of the scalars passed to the kernels ends a captured evaluation
+void drop_capture(lb_ctx *c) {
+    if (c && c->gexec) {
+        hipGraphExecDestroy((hipGraphExec_t)c->gexec);
+        c->gexec = nullptr;
+    }
 }
 
 #define HIPCHK(ctx, call)                                                                                   \
@@ -2503,6 +2512,7 @@ void lb_destroy(lb_ctx *c) {
                     &c->b_pp, &c->b_rq, &c->b_dsq, &c->b_zY, &c->b_M, &c->b_pdiag, &c->b_ltot, &c->b_lviol, &c->b_rowv,
                     &c->b_parts, &c->b_lamsum};
     for (DBuf *b : bufs) dfree(*b);
+    if (c->gexec) hipGraphExecDestroy((hipGraphExec_t)c->gexec);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2510,12 +2520,14 @@ void lb_destroy(lb_ctx *c) {
 const char *lb_last_error(const lb_ctx *c) { return c ? c->err.c_str() : "no context"; }
 
 int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
+    drop_capture(c);
     if (!c || !name) return fail(c, -1, "lb_set_option: bad arguments");
     if (std::strcmp(name, "row_order") == 0) { c->use_order = value != 0; return 0; }
     return fail(c, -1, std::string("lb_set_option: unknown option ") + name);
 }
 
 int lb_set_tuning(lb_ctx *c, int heavy_edges, int heavy_tris, int seg_len, int row_grid) {
+    drop_capture(c);
     if (!c) return -1;
     if (heavy_edges < 1 || heavy_tris < 1 || seg_len < 1 || row_grid < 0) return fail(c, -1, "tuning values must be positive");
     c->heavy_edges = heavy_edges; c->heavy_tris = heavy_tris; c->seg_len = seg_len;
@@ -2525,6 +2537,7 @@ int lb_set_tuning(lb_ctx *c, int heavy_edges, int heavy_tris, int seg_len, int r
 
 int lb_set_graph(lb_ctx *c, int64_t n, const int64_t *rowptr, const int32_t *col, const double *weights,
                  const double *d, double L, double C0) {
+    drop_capture(c);
     if (!c) return -1;
     if (n <= 0 || n >= ((int64_t)1 << 31) || !rowptr || !col || !d || !(L > 0)) return fail(c, -1, "lb_set_graph: bad arguments");
     HIPCHK(c, hipSetDevice(c->device));
@@ -2566,6 +2579,7 @@ int lb_set_graph(lb_ctx *c, int64_t n, const int64_t *rowptr, const int32_t *col
 
 int lb_set_levels(lb_ctx *c, int nlev, const int32_t *labs, const int32_t *P, const double *sig, const double *rq,
                   const double *dsq, double pairlvl_weight) {
+    drop_capture(c);
     if (!c || !c->n) return fail(c, -1, "lb_set_levels: set the graph first");
     if (nlev < 0 || nlev > LB_MAXLEV) return fail(c, -1, "lb_set_levels: at most 8 pair levels");
     HIPCHK(c, hipSetDevice(c->device));
@@ -2592,6 +2606,7 @@ int lb_set_levels(lb_ctx *c, int nlev, const int32_t *labs, const int32_t *P, co
 }
 
 int lb_set_level_sig(lb_ctx *c, const double *sig) {
+    drop_capture(c);
     if (!c || !c->nlev) return fail(c, -1, "lb_set_level_sig: no pair levels");
     HIPCHK(c, hipSetDevice(c->device));
     return upload(c, c->sig, sig, sizeof(double) * c->soff.back());
@@ -2599,6 +2614,7 @@ int lb_set_level_sig(lb_ctx *c, const double *sig) {
 
 int lb_set_vcs(lb_ctx *c, int nvcs, const int32_t *labs, const int32_t *P, const double *rq, const int64_t *ptr,
                const int32_t *col, const double *sig, double wv) {
+    drop_capture(c);
     if (!c || !c->n) return fail(c, -1, "lb_set_vcs: set the graph first");
     if (nvcs < 0 || nvcs > LB_MAXVCS) return fail(c, -1, "lb_set_vcs: at most 4 VCS levels");
     HIPCHK(c, hipSetDevice(c->device));
@@ -2650,6 +2666,7 @@ int lb_set_vcs(lb_ctx *c, int nvcs, const int32_t *labs, const int32_t *P, const
 }
 
 int lb_set_vcs_age(lb_ctx *c, const uint8_t *age) {
+    drop_capture(c);
     if (!c || !c->nvcs || !age) return fail(c, -1, "lb_set_vcs_age: no VCS levels");
     HIPCHK(c, hipSetDevice(c->device));
     int64_t tot = 0;
@@ -2672,6 +2689,7 @@ int lb_get_vcs_state(lb_ctx *c, double *sig, uint8_t *age) {
 }
 
 int lb_set_weights(lb_ctx *c, double pairlvl_weight, double wv) {
+    drop_capture(c);
     if (!c) return -1;
     if (!(pairlvl_weight > 0.0) || !(wv > 0.0)) return fail(c, -1, "lb_set_weights: weights must be positive");
     c->pw = pairlvl_weight; c->wv = wv;
@@ -2679,6 +2697,7 @@ int lb_set_weights(lb_ctx *c, double pairlvl_weight, double wv) {
 }
 
 int lb_set_vcs_sig(lb_ctx *c, const double *sig) {
+    drop_capture(c);
     if (!c || !c->nvcs) return fail(c, -1, "lb_set_vcs_sig: no VCS levels");
     HIPCHK(c, hipSetDevice(c->device));
     int64_t tot = 0;
@@ -2687,6 +2706,7 @@ int lb_set_vcs_sig(lb_ctx *c, const double *sig) {
 }
 
 int lb_set_triangles(lb_ctx *c, int64_t T, const int32_t *TI, const double *lam) {
+    drop_capture(c);
     if (!c || !c->n) return fail(c, -1, "lb_set_triangles: set the graph first");
     if (T < 0 || T > ((int64_t)1 << 29)) return fail(c, -1, "lb_set_triangles: bad T");
     HIPCHK(c, hipSetDevice(c->device));
@@ -2755,6 +2775,7 @@ int lb_set_triangles(lb_ctx *c, int64_t T, const int32_t *TI, const double *lam)
 }
 
 int lb_set_lam(lb_ctx *c, const double *lam) {
+    drop_capture(c);
     if (!c || !c->n) return fail(c, -1, "lb_set_lam: no graph");
     if (!c->T) return 0;
     HIPCHK(c, hipSetDevice(c->device));
@@ -2872,6 +2893,7 @@ int lb_get_multipliers(lb_ctx *c, double *pp, double *pt, double *pv) {
 }
 
 int lb_phr_update(lb_ctx *c) {
+    drop_capture(c);
     if (!c || !c->have_eval) return fail(c, -1, "lb_phr_update: no evaluation yet");
     if (!(c->last_rho > 0.0)) return fail(c, -1, "lb_phr_update: the last evaluation had rho = 0");
     HIPCHK(c, hipSetDevice(c->device));
@@ -2897,6 +2919,7 @@ int lb_phr_update(lb_ctx *c) {
 }
 
 int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap2, const int64_t *orders, int nbatch) {
+    drop_capture(c);
     if (c && c->bG) return fail(c, -1, "lb_sweeps: not available on a batch context");
     if (!c || !c->n) return fail(c, -1, "lb_sweeps: no graph");
     if (nsw < 0 || nbatch < 1 || nbatch > c->n || !orders) return fail(c, -1, "lb_sweeps: bad arguments");
@@ -2962,6 +2985,7 @@ int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap
 int lb_separate_triangles(lb_ctx *c, int fam, double thresh, int64_t maxh, int64_t maxk, int64_t hub2, int64_t cap2,
                           double vtol, int64_t cap, int nchunk, int nb, int64_t *TI_out, double *val_out,
                           int64_t *count, double *vmin) {
+    drop_capture(c);
     if (c && c->bG) return fail(c, -1, "lb_separate_triangles: not available on a batch context");
     if (!c || !c->n) return fail(c, -1, "lb_separate_triangles: no graph");
     if ((fam != 0 && fam != 1) || maxh < 1 || maxk < 1 || hub2 < 0 || cap2 < 0 || cap < 0 || nchunk < 1 || nb < 1 ||
@@ -2986,6 +3010,7 @@ int lb_separate_triangles(lb_ctx *c, int fam, double thresh, int64_t maxh, int64
 
 int lb_separate_vcs(lb_ctx *c, int m, double tol, int mode, int64_t nadd, int64_t *out_i, int32_t *out_R,
                     double *out_score, int64_t *count, double *stats) {
+    drop_capture(c);
     if (c && c->bG) return fail(c, -1, "lb_separate_vcs: not available on a batch context");
     if (!c || !c->n || m < 0 || m >= c->nvcs) return fail(c, -1, "lb_separate_vcs: no such VCS level");
     if (mode < 0 || mode > 2 || nadd < 0 || !out_i || !out_R || !out_score || !count || !stats)
@@ -3811,6 +3836,7 @@ beval_fn pick_beval(int vdtype, int r) {
 extern "C" {
 
 int lb_batch_set(lb_ctx *c, int G, const int64_t *row_off, const double *L, const double *C0) {
+    drop_capture(c);
     if (!c || !c->n) return fail(c, -1, "lb_batch_set: set the graph (the union of the batch) first");
     if (G < 1 || !row_off || !L || !C0) return fail(c, -1, "lb_batch_set: bad arguments");
     if (row_off[0] != 0 || row_off[G] != c->n) return fail(c, -1, "lb_batch_set: row offsets must run from 0 to n");
@@ -3847,6 +3873,7 @@ int lb_batch_set(lb_ctx *c, int G, const int64_t *row_off, const double *L, cons
 
 int lb_batch_set_levels(lb_ctx *c, const int32_t *lab, const int32_t *P, const double *sig, const double *rq,
                         const double *dsq, double pairlvl_weight) {
+    drop_capture(c);
     if (!c || !c->bG) return fail(c, -1, "lb_batch_set_levels: call lb_batch_set first");
     if (!lab || !P || !sig || !rq || !dsq) return fail(c, -1, "lb_batch_set_levels: bad arguments");
     const int G = c->bG;
@@ -3908,6 +3935,73 @@ int lb_batch_eval(lb_ctx *c, double rho, double rt, double *parts) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     c->have_eval = true; c->last_rho = rho;
+    return 0;
+}
+
+}  // extern "C"
+
+// ================================================================================================ captured evaluation
+// The evaluation chain (single graph or batch) recorded once into a HIP graph on the context's stream and replayed:
+// no host work and one launch per replay. The chain clears its accumulators with fill kernels (no memset nodes).
+extern "C" {
+
+int lb_capture_eval(lb_ctx *c, double rho, double rt) {
+    if (!c || !c->n) return fail(c, -1, "lb_capture_eval: no graph");
+    if (!(rho >= 0.0) || (rho > 0.0 && c->T && !(rt > 0.0))) return fail(c, -1, "lb_capture_eval: need rho >= 0 and rt > 0 when rho > 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->gexec) { hipGraphExecDestroy((hipGraphExec_t)c->gexec); c->gexec = nullptr; }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    if (c->bG) {
+        beval_fn f = pick_beval(c->vdtype, c->r);
+        if (!f) rc = fail(c, -1, "lb_capture_eval: unsupported rank / dtype");
+        else {
+            BArgs a{};
+            a.V = (const uint4 *)c->V.p; a.n = c->n; a.rowptr = (const int64_t *)c->rowptr.p; a.col = (const int32_t *)c->col.p;
+            a.w = c->weighted ? c->w.p : nullptr; a.d = (const double *)c->d.p;
+            a.gid = (const int32_t *)c->b_gid.p; a.boff = (const int64_t *)c->b_off.p; a.G = c->bG;
+            a.Lg = (const double *)c->b_L.p; a.C0g = (const double *)c->b_C0.p;
+            a.lev = c->blev ? 1 : 0; a.lab = (const int32_t *)c->b_lab.p; a.Poff = (const int32_t *)c->b_Poff.p;
+            a.soff = (const int64_t *)c->b_soff.p; a.cg = (const int32_t *)c->b_cg.p; a.sumP = c->bsumP;
+            a.sig = (const double *)c->b_sig.p; a.pp = (double *)c->b_pp.p; a.rq = (const double *)c->b_rq.p;
+            a.dsq = (const double *)c->b_dsq.p; a.wpair = c->bw;
+            a.zY = (double *)c->b_zY.p; a.M = (double *)c->b_M.p; a.pdiag = (double *)c->b_pdiag.p;
+            a.ltot = (double *)c->b_ltot.p; a.lviol = (double *)c->b_lviol.p;
+            a.iptr = (const int64_t *)c->iptr.p; a.rec = c->rec.p; a.pt = c->pt.p; a.T = c->T; a.TI = (const int32_t *)c->TI.p;
+            a.lam = c->lam.p;
+            a.Gout = c->G.p; a.rowv = (double *)c->b_rowv.p; a.parts = (double *)c->b_parts.p; a.lamsum = (double *)c->b_lamsum.p;
+            rc = f(c, rho, rt, a);
+        }
+    } else {
+        eval_fn f = pick_eval(c->vdtype, c->r);
+        if (!f) rc = fail(c, -1, "lb_capture_eval: unsupported rank / dtype");
+        else rc = f(c, rho, rt, nullptr);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(c, -3, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(c, -3, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    c->gexec = ge; c->g_rho = rho;
+    return 0;
+}
+
+int lb_replay_eval(lb_ctx *c, int count, double *parts) {
+    if (!c || !c->gexec) return fail(c, -1, "lb_replay_eval: nothing captured (lb_capture_eval)");
+    if (count < 1) return fail(c, -1, "lb_replay_eval: count must be positive");
+    HIPCHK(c, hipSetDevice(c->device));
+    for (int k = 0; k < count; k++) HIPCHK(c, hipGraphLaunch((hipGraphExec_t)c->gexec, c->stream));
+    if (parts) {
+        const DBuf &src = c->bG ? c->b_parts : c->parts;
+        const size_t cnt = (size_t)LB_NPARTS * (c->bG ? c->bG : 1);
+        HIPCHK(c, hipMemcpyAsync(parts, src.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    c->have_eval = true; c->last_rho = c->g_rho;
     return 0;
 }
 

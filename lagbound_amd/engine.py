@@ -80,6 +80,7 @@ class Engine:
         self._chk(self.L.lb_set_option(self.ctx, name.encode(), int(value)), "lb_set_option")
 
     def set_instance(self, inst):
+        self.batch_G = 0
         n = inst.n
         self._keep = []
         rowptr = np.ascontiguousarray(inst.rowptr, np.int64)
@@ -252,6 +253,20 @@ class Engine:
             return None
         parts = np.zeros((self.batch_G, LB.LB_NPARTS))
         self._chk(self.L.lb_batch_eval(self.ctx, float(rho), float(rt), ptr(parts, _f64p)), "lb_batch_eval")
+        return parts
+
+    def capture_eval(self, rho, rt=None):
+        """Record the evaluation chain at (rho, rt) into a HIP graph (single graph or batch)."""
+        rt = rho if rt is None else rt
+        self._chk(self.L.lb_capture_eval(self.ctx, float(rho), float(rt)), "lb_capture_eval")
+
+    def replay_eval(self, count=1, fetch=True):
+        if not fetch:
+            self._chk(self.L.lb_replay_eval(self.ctx, int(count), None), "lb_replay_eval")
+            return None
+        G = getattr(self, "batch_G", 0)
+        parts = np.zeros((G, LB.LB_NPARTS) if G else LB.LB_NPARTS)
+        self._chk(self.L.lb_replay_eval(self.ctx, int(count), ptr(parts, _f64p)), "lb_replay_eval")
         return parts
 
     def certify_dense(self, dg):
