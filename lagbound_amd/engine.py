@@ -44,6 +44,7 @@ class _TimedLib:
 class Engine:
     def __init__(self, r: int = 16, dtype: str = "f32", device: int = 0):
         self.lib_s = 0.0
+        self.n_pp = self.n_t = self.n_pv = 0          # sizes of what was uploaded (host buffers of the getters)
         self.L = _TimedLib(LB.load(), self)
         self.r = int(r)
         self.dtype = dtype
@@ -96,6 +97,7 @@ class Engine:
 
     def set_levels(self, inst):
         lv = inst.levels
+        self.n_pp = sum(l.P * l.P for l in lv)
         if not lv:
             self._chk(self.L.lb_set_levels(self.ctx, 0, None, None, None, None, None, 1.0), "lb_set_levels")
             return
@@ -113,6 +115,7 @@ class Engine:
 
     def set_vcs(self, inst):
         vc = inst.vcs
+        self.n_pv = sum(v.nnz for v in vc)
         if not vc:
             self._chk(self.L.lb_set_vcs(self.ctx, 0, None, None, None, None, None, None, 1.0), "lb_set_vcs")
             return
@@ -133,6 +136,7 @@ class Engine:
         TI = np.ascontiguousarray(TI, np.int32).reshape(-1, 3)
         lam = np.ascontiguousarray(lam, np.float64)
         self._chk(self.L.lb_set_triangles(self.ctx, len(TI), ptr(TI, _i32p), ptr(lam, _f64p)), "lb_set_triangles")
+        self.n_t = len(TI)
 
     def set_lam(self, lam):
         lam = np.ascontiguousarray(lam, np.float64)
@@ -164,13 +168,14 @@ class Engine:
         return out
 
     def get_multipliers(self):
-        inst = self.inst
-        pp = np.zeros(max(1, sum(l.P * l.P for l in inst.levels)))
-        pt = np.zeros(max(1, inst.T))
-        pv = np.zeros(max(1, sum(v.nnz for v in inst.vcs)))
+        """(pp, pt, pv) of the last evaluation. The buffers are sized by what this engine uploaded (n_pp, n_t, n_pv),
+        which is what the library writes: the C call takes no sizes."""
+        pp = np.zeros(max(1, self.n_pp))
+        pt = np.zeros(max(1, self.n_t))
+        pv = np.zeros(max(1, self.n_pv))
         self._chk(self.L.lb_get_multipliers(self.ctx, ptr(pp, _f64p), ptr(pt, _f64p), ptr(pv, _f64p)),
                   "lb_get_multipliers")
-        return (pp[:sum(l.P * l.P for l in inst.levels)], pt[:inst.T], pv[:sum(v.nnz for v in inst.vcs)])
+        return pp[:self.n_pp], pt[:self.n_t], pv[:self.n_pv]
 
     def phr_update(self):
         self._chk(self.L.lb_phr_update(self.ctx), "lb_phr_update")
@@ -213,7 +218,7 @@ class Engine:
 
     def get_vcs_state(self):
         """(sig, age) of every VCS entry (concatenated over the levels)."""
-        tot = sum(v.nnz for v in self.inst.vcs)
+        tot = self.n_pv
         sig = np.zeros(max(1, tot))
         age = np.zeros(max(1, tot), np.uint8)
         self._chk(self.L.lb_get_vcs_state(self.ctx, ptr(sig, _f64p), age.ctypes.data_as(C.c_void_p)), "lb_get_vcs_state")
