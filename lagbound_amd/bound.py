@@ -444,13 +444,26 @@ def certify_dense_device(inst, g, t=None, device=0, eng=None):
     return dict(passed=True, UB=UB, eps=eps, price=n * eps / inst.L, device=True)
 
 
-def certify_estimate(inst, est, margin=1e-7, device=None):
-    """certify_dense at the estimate's best round (g, s, lmin): t = (max(0, -lmin) + margin) s; on the device when
-    `device` is given."""
-    t = (max(0.0, -est["lmin"]) + margin) * est["s"]
-    if device is not None:
-        return certify_dense_device(inst, est["g"], t, device=device)
-    return certify_dense(inst, est["g"], t)
+def certify_estimate(inst, est, margin=1e-7, device=None, tries=8):
+    """certify_dense at the estimate's best round (g, s, lmin) with t = (factor max(0, -lmin) + margin) s, starting at
+    factor 1 and doubling factor and margin while the factorisation fails (the eigensolver may have stopped above
+    the smallest eigenvalue; the reference likewise starts its certificate at a multiple of the estimated shift,
+    api.py:216-218). On the device when `device` is given. The result records the factor used."""
+    shift = max(0.0, -est["lmin"])
+    factor, mg = 1.0, margin
+    res = None
+    for _ in range(tries):
+        t = (factor * shift + mg) * est["s"]
+        if device is not None:
+            res = certify_dense_device(inst, est["g"], t, device=device)
+        else:
+            res = certify_dense(inst, est["g"], t)
+        if res["passed"]:
+            res["factor"] = factor
+            return res
+        factor *= 2.0
+        mg = max(mg * 10.0, 1e-6 * max(shift, 1e-3))
+    return res
 
 
 # ----------------------------------------------------------------------------------------------- checkpoints
