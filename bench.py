@@ -8,9 +8,12 @@ Inputs are resident in HBM before the timed region; the timed region is K back-t
 device synchronisation.
 
 Also reported on the same instance: the HIP path with BF16-stored V and with FP64 V, the plain FP64 GPU form
-(torch FP64 CSR SpMM + gathers/index_add), and the FP64 C/OpenMP oracle on the host cores (cpu_baseline).
+(torch FP64 CSR SpMM + gathers/index_add), and the FP64 C/OpenMP oracle on the host cores (cpu_baseline). The
+metric's second half, batched bound latency, comes from a Poisson stream of whole bound requests (`serving`:
+p50 / p99 latency, solves/s, certified count) and from the batched evaluation of 512 uneven graphs in one launch
+per kernel (`batch`).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-extra]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-extra] [--no-serve] [--no-batch]
 """
 from __future__ import annotations
 
@@ -69,6 +72,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tuning", default="", help="heavy_edges,heavy_tris,seg_len,row_grid")
     ap.add_argument("--option", action="append", default=[], help="name=value kernel switch (lb_set_option)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving stream (batched bound latency)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched evaluation")
+    ap.add_argument("--serve-count", type=int, default=16)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain copy kernel of known size (for scaling hardware byte counters)")
     a = ap.parse_args()
@@ -202,6 +208,28 @@ def main():
         out["cpu_baseline"]["single_thread"] = {"value": round(1.0 / min(t1s), 4), "unit": "evals/s", "cores": 1,
                                                 "sample": "2 full FP64 evaluations, best"}
         log(f"cpu oracle (1 thread): {min(t1s):.3f} s/eval")
+    # batched bound latency: a Poisson stream of whole bound requests (lagbound_amd.serve) and the batched
+    # evaluation of 512 uneven graphs (lagbound_amd.batch), each in a process of its own
+    import subprocess
+
+    def _module_json(args, timeout):
+        res = subprocess.run([sys.executable, "-m"] + args, capture_output=True, text=True, timeout=timeout, cwd=ROOT)
+        line = [l for l in res.stdout.splitlines() if l.startswith("{")]
+        if res.returncode != 0 or not line:
+            return {"error": (res.stderr or "")[-400:]}
+        return json.loads(line[-1])
+
+    if not a.no_serve:
+        out["serving"] = _module_json(["lagbound_amd.serve", "--count", str(a.serve_count), "--rate", "200"], 900)
+        sv = out["serving"]
+        if "latency_p50_s" in sv:
+            log(f"serving: p50 {sv['latency_p50_s']} s, p99 {sv['latency_p99_s']} s, {sv['solves_per_s']} solves/s")
+    if not a.no_batch:
+        out["batch"] = _module_json(["lagbound_amd.batch", "--count", "512"], 900)
+        bt = out["batch"]
+        if "ms_per_batch_eval" in bt:
+            log(f"batch: {bt['ms_per_batch_eval']} ms per evaluation of {bt['graphs']} graphs "
+                f"({bt['graph_evals_per_s']} graph evaluations/s)")
     if a.calibrate:
         import ctypes
         lp = ctypes.CDLL(os.path.join(ROOT, "lagbound_amd", "liblbprobe.so"))

@@ -3689,24 +3689,56 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_brows(BArgs a) {
 #pragma unroll
         for (int c = 0; c < CPL; c++) { ga[c] = 0; gt[c] = 0; }
         const int64_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
-        for (int64_t e = e0 + grp; e < e1; e += NG) {
-            TC xj[CPL];
-            VT<TV>::unpack(a.V[(int64_t)a.col[e] * S + q], xj);
-            const TC we = wt ? wt[e] : (TC)1;
+        // edges: each group takes 8 of its edges at a time (8 gathers in flight per lane); indices past the row are
+        // clamped to the row's last edge and weighted 0
+        for (int64_t eb = e0 + grp; eb < e1; eb += 8 * NG) {
+            int64_t jj[8];
+            TC ww[8];
 #pragma unroll
-            for (int c = 0; c < CPL; c++) ga[c] += we * xj[c];
+            for (int t = 0; t < 8; t++) {
+                const int64_t e = eb + (int64_t)t * NG;
+                const int64_t ec = e < e1 ? e : e1 - 1;
+                jj[t] = a.col[ec];
+                ww[t] = e < e1 ? (wt ? wt[ec] : (TC)1) : (TC)0;
+            }
+            uint4 u[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) u[t] = a.V[jj[t] * S + q];
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                TC xj[CPL];
+                VT<TV>::unpack(u[t], xj);
+#pragma unroll
+                for (int c = 0; c < CPL; c++) ga[c] += ww[t] * xj[c];
+            }
         }
         double ttot = 0.0, tmin = INFINITY;
         const int64_t k0 = a.iptr[i], k1 = a.iptr[i + 1];
-        for (int64_t k = k0 + grp; k < k1; k += NG) {
-            const TriRec<TC> rc = rec[k];
-            TC p, tau;
-            tri_record<TV, R>(a.V, rc, x, q, a.rho, a.rt, gt, p, tau);
-            if (rc.code % 3 == 0 && q == 0) {
-                pt[rc.code / 3] = p;
-                if (a.rho > 0.0) {
-                    ttot += ((double)p * p - (double)rc.lam * rc.lam) / (2 * a.rt);
-                    tmin = fmin(tmin, (double)((TC)1 - tau));
+        // triangle records: 4 per group at a time; records past the row are processed with lam = 0 and rho = 0
+        for (int64_t kb = k0 + grp; kb < k1; kb += 4 * NG) {
+            TriRec<TC> rc[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int64_t k = kb + (int64_t)t * NG;
+                rc[t] = rec[k < k1 ? k : k1 - 1];
+            }
+            uint4 uu[4], uw[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) { uu[t] = a.V[(int64_t)rc[t].u * S + q]; uw[t] = a.V[(int64_t)rc[t].w * S + q]; }
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const bool valid = kb + (int64_t)t * NG < k1;
+                TC xu[CPL], xw[CPL], p, tau;
+                VT<TV>::unpack(uu[t], xu);
+                VT<TV>::unpack(uw[t], xw);
+                tri_record_x<TV, R>(rc[t].code, valid ? rc[t].lam : (TC)0, x, xu, xw, valid ? a.rho : 0.0, a.rt, gt,
+                                    p, tau);
+                if (valid && rc[t].code % 3 == 0 && q == 0) {
+                    pt[rc[t].code / 3] = p;
+                    if (a.rho > 0.0) {
+                        ttot += ((double)p * p - (double)rc[t].lam * rc[t].lam) / (2 * a.rt);
+                        tmin = fmin(tmin, (double)((TC)1 - tau));
+                    }
                 }
             }
         }
