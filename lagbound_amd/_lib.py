@@ -6,7 +6,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "liblbhip.so")
+LIBPATH = os.environ.get("LBHIP_LIB") or os.path.join(HERE, "liblbhip.so")   # LBHIP_LIB: a build variant to load
 
 LB_F32, LB_BF16, LB_F64 = 0, 1, 2
 LB_NPARTS = 12

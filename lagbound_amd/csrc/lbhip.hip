@@ -29,6 +29,20 @@
 
 #include "../../include/lbhip.h"
 
+// gathers kept in flight per lane: row kernel edges / triangle records, batch row kernel edges / records
+#ifndef LB_EB
+#define LB_EB 8
+#endif
+#ifndef LB_RB
+#define LB_RB 4
+#endif
+#ifndef LB_BEB
+#define LB_BEB 8
+#endif
+#ifndef LB_BRB
+#define LB_BRB 4
+#endif
+
 #define LB_MAXLEV 8
 #define LB_MAXVCS 4
 #define LB_RED_BLOCKS 512
@@ -570,23 +584,23 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
                 for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
             }
         } else {
-            for (int64_t e = e0; e < e1; e += 8) {
+            for (int64_t e = e0; e < e1; e += LB_EB) {
                 const int64_t nE = e1 - e;
-                int64_t jj[8];
-                TC ww[8];
+                int32_t jj[LB_EB];
+                TC ww[LB_EB];
 #pragma unroll
-                for (int t = 0; t < 8; t++) {   // clamped index: always in bounds; lanes past the row reuse row i
+                for (int t = 0; t < LB_EB; t++) {   // clamped index: always in bounds; lanes past the row reuse row i
                     const int64_t ec = t < nE ? e + t : e1 - 1;
                     const int32_t cj = a.col[ec];
-                    jj[t] = t < nE ? (int64_t)cj : i;
+                    jj[t] = t < nE ? cj : (int32_t)i;
                     ww[t] = wt ? wt[ec] : (TC)1;
                     if (t >= nE) ww[t] = (TC)0;
                 }
-                uint4 u[8];
+                uint4 u[LB_EB];
 #pragma unroll
-                for (int t = 0; t < 8; t++) u[t] = a.V[jj[t] * S + q];
+                for (int t = 0; t < LB_EB; t++) u[t] = a.V[(int64_t)jj[t] * S + q];
 #pragma unroll
-                for (int t = 0; t < 8; t++) {
+                for (int t = 0; t < LB_EB; t++) {
                     TC xj[CPL];
                     VT<TV>::unpack(u[t], xj);
 #pragma unroll
@@ -650,16 +664,16 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
                 for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
             }
         } else {
-            for (int64_t k = r0; k < r1; k += 4) {
+            for (int64_t k = r0; k < r1; k += LB_RB) {
                 const int64_t nK = r1 - k;
-                TriRec<TC> rc[4];
+                TriRec<TC> rc[LB_RB];
 #pragma unroll
-                for (int t = 0; t < 4; t++) rc[t] = rec[t < nK ? k + t : r1 - 1];
-                uint4 uu[4], uw[4];
+                for (int t = 0; t < LB_RB; t++) rc[t] = rec[t < nK ? k + t : r1 - 1];
+                uint4 uu[LB_RB], uw[LB_RB];
 #pragma unroll
-                for (int t = 0; t < 4; t++) { uu[t] = a.V[(int64_t)rc[t].u * S + q]; uw[t] = a.V[(int64_t)rc[t].w * S + q]; }
+                for (int t = 0; t < LB_RB; t++) { uu[t] = a.V[(int64_t)rc[t].u * S + q]; uw[t] = a.V[(int64_t)rc[t].w * S + q]; }
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
+                for (int t = 0; t < LB_RB; t++) {
                     // records past the row's end are processed with lam = 0 and rho = 0: they add nothing
                     const bool valid = t < nK;
                     TC xu[CPL], xw[CPL], p, tau;
@@ -3691,21 +3705,21 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_brows(BArgs a) {
         const int64_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
         // edges: each group takes 8 of its edges at a time (8 gathers in flight per lane); indices past the row are
         // clamped to the row's last edge and weighted 0
-        for (int64_t eb = e0 + grp; eb < e1; eb += 8 * NG) {
-            int64_t jj[8];
-            TC ww[8];
+        for (int64_t eb = e0 + grp; eb < e1; eb += LB_BEB * NG) {
+            int64_t jj[LB_BEB];
+            TC ww[LB_BEB];
 #pragma unroll
-            for (int t = 0; t < 8; t++) {
+            for (int t = 0; t < LB_BEB; t++) {
                 const int64_t e = eb + (int64_t)t * NG;
                 const int64_t ec = e < e1 ? e : e1 - 1;
                 jj[t] = a.col[ec];
                 ww[t] = e < e1 ? (wt ? wt[ec] : (TC)1) : (TC)0;
             }
-            uint4 u[8];
+            uint4 u[LB_BEB];
 #pragma unroll
-            for (int t = 0; t < 8; t++) u[t] = a.V[jj[t] * S + q];
+            for (int t = 0; t < LB_BEB; t++) u[t] = a.V[jj[t] * S + q];
 #pragma unroll
-            for (int t = 0; t < 8; t++) {
+            for (int t = 0; t < LB_BEB; t++) {
                 TC xj[CPL];
                 VT<TV>::unpack(u[t], xj);
 #pragma unroll
@@ -3715,18 +3729,18 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_brows(BArgs a) {
         double ttot = 0.0, tmin = INFINITY;
         const int64_t k0 = a.iptr[i], k1 = a.iptr[i + 1];
         // triangle records: 4 per group at a time; records past the row are processed with lam = 0 and rho = 0
-        for (int64_t kb = k0 + grp; kb < k1; kb += 4 * NG) {
-            TriRec<TC> rc[4];
+        for (int64_t kb = k0 + grp; kb < k1; kb += LB_BRB * NG) {
+            TriRec<TC> rc[LB_BRB];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
+            for (int t = 0; t < LB_BRB; t++) {
                 const int64_t k = kb + (int64_t)t * NG;
                 rc[t] = rec[k < k1 ? k : k1 - 1];
             }
-            uint4 uu[4], uw[4];
+            uint4 uu[LB_BRB], uw[LB_BRB];
 #pragma unroll
-            for (int t = 0; t < 4; t++) { uu[t] = a.V[(int64_t)rc[t].u * S + q]; uw[t] = a.V[(int64_t)rc[t].w * S + q]; }
+            for (int t = 0; t < LB_BRB; t++) { uu[t] = a.V[(int64_t)rc[t].u * S + q]; uw[t] = a.V[(int64_t)rc[t].w * S + q]; }
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
+            for (int t = 0; t < LB_BRB; t++) {
                 const bool valid = kb + (int64_t)t * NG < k1;
                 TC xu[CPL], xw[CPL], p, tau;
                 VT<TV>::unpack(uu[t], xu);
