@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tuning", default="", help="heavy_edges,heavy_tris,seg_len,row_grid")
     ap.add_argument("--option", action="append", default=[], help="name=value kernel switch (lb_set_option)")
+    ap.add_argument("--no-autotune", action="store_true", help="use the library's default launch shapes")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream (batched bound latency)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched evaluation")
     ap.add_argument("--serve-count", type=int, default=16)
@@ -94,8 +95,17 @@ def main():
 
     eng = Engine(r=r, dtype="f32")
     tune = [int(x) for x in a.tuning.split(",")] if a.tuning else None
+    tuned = None
     if tune:
         eng.set_tuning(*tune)
+    elif not a.no_autotune:
+        # launch shapes picked by the autotuner on this instance (hub threshold, segment length, row grid)
+        from lagbound_amd import tune as TU
+        t1 = time.time()
+        tuned, _ = TU.autotune(inst, V, rho, r=r, grid={"heavy": [64, 128, 256], "seg_len": [256],
+                                                        "row_grid": [2048, 4096]}, iters=10)
+        TU.apply(eng, tuned)
+        log(f"autotuned {tuned} in {time.time() - t1:.1f}s")
     for opt in a.option:
         k_, v_ = opt.split("=")
         eng.set_option(k_, int(v_))
@@ -124,7 +134,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "f+grad eval, Chung-Lu gamma=2.1 mean-degree~13 (metric instance M)", "n": inst.n,
                    "nnz": inst.nnz, "r": r, "T": inst.T, "pair_P": inst.levels[0].P, "vcs_P": inst.vcs[0].P,
-                   "vcs_ws": inst.vcs[0].nnz, "rho": rho, "V": "fp32, fp64 global sums"},
+                   "vcs_ws": inst.vcs[0].nnz, "rho": rho, "V": "fp32, fp64 global sums",
+                   "launch_shapes": tuned if tuned else ({"tuning": a.tuning} if a.tuning else "library defaults")},
         "roofline": {"bound": "hbm", "kernel": "one f+grad evaluation (lb_k_colsum .. lb_k_finalize chain)",
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "alg_bytes": B, "ms_per_eval_events": round(ms_ev, 5),

@@ -31,7 +31,7 @@
 
 // gathers kept in flight per lane: row kernel edges / triangle records, batch row kernel edges / records
 #ifndef LB_EB
-#define LB_EB 8
+#define LB_EB 16
 #endif
 #ifndef LB_RB
 #define LB_RB 4
@@ -1743,7 +1743,7 @@ struct lb_ctx {
     // batch mode: the graph is the block-diagonal union of bG graphs with contiguous row ranges
     int bG = 0; std::vector<int64_t> hboff; std::vector<int> bP; int bsumP = 0; bool blev = false;
     DBuf b_off, b_gid, b_L, b_C0, b_Poff, b_soff, b_cg, b_lab, b_sig, b_pp, b_rq, b_dsq, b_zY, b_M, b_pdiag, b_ltot,
-        b_lviol, b_rowv, b_parts, b_lamsum;
+        b_lviol, b_rowv, b_parts, b_lamsum, b_tptr, b_tidx;
     double bw = 1.0;
     void *gexec = nullptr; double g_rho = 0.0;   // captured evaluation (hipGraphExec_t)
 };
@@ -2524,7 +2524,7 @@ void lb_destroy(lb_ctx *c) {
                     &c->Vstage, &c->DDb, &c->segkap, &c->seglist, &c->vslack, &c->vage, &c->vvol, &c->rorder,
                     &c->b_off, &c->b_gid, &c->b_L, &c->b_C0, &c->b_Poff, &c->b_soff, &c->b_cg, &c->b_lab, &c->b_sig,
                     &c->b_pp, &c->b_rq, &c->b_dsq, &c->b_zY, &c->b_M, &c->b_pdiag, &c->b_ltot, &c->b_lviol, &c->b_rowv,
-                    &c->b_parts, &c->b_lamsum};
+                    &c->b_parts, &c->b_lamsum, &c->b_tptr, &c->b_tidx};
     for (DBuf *b : bufs) dfree(*b);
     if (c->gexec) hipGraphExecDestroy((hipGraphExec_t)c->gexec);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -2737,12 +2737,23 @@ int lb_set_triangles(lb_ctx *c, int64_t T, const int32_t *TI, const double *lam)
     for (int64_t e = 0; e < 3 * T; e++)
         if (TI[e] < 0 || TI[e] >= n) return fail(c, -1, "lb_set_triangles: vertex out of range");
     if (c->bG) {                                      // batch mode: a triangle lies inside one graph
-        auto graph_of = [&](int64_t v) { return std::upper_bound(c->hboff.begin(), c->hboff.end(), v) - c->hboff.begin(); };
+        auto graph_of = [&](int64_t v) { return std::upper_bound(c->hboff.begin(), c->hboff.end(), v) - c->hboff.begin() - 1; };
+        std::vector<int64_t> tptr(c->bG + 1, 0);
+        std::vector<int32_t> tg((size_t)T);
         for (int64_t t = 0; t < T; t++) {
             const auto g0 = graph_of(TI[3 * t]);
             if (graph_of(TI[3 * t + 1]) != g0 || graph_of(TI[3 * t + 2]) != g0)
                 return fail(c, -1, "lb_set_triangles: a triangle joins two graphs of the batch");
+            tg[t] = (int32_t)g0;
+            tptr[g0 + 1]++;
         }
+        for (int g = 0; g < c->bG; g++) tptr[g + 1] += tptr[g];
+        std::vector<int64_t> fillp(tptr.begin(), tptr.end() - 1);
+        std::vector<int32_t> tidx((size_t)T);
+        for (int64_t t = 0; t < T; t++) tidx[fillp[tg[t]]++] = (int32_t)t;
+        int rc2;
+        if ((rc2 = upload(c, c->b_tptr, tptr.data(), sizeof(int64_t) * tptr.size()))) return rc2;
+        if ((rc2 = upload(c, c->b_tidx, tidx.data(), sizeof(int32_t) * tidx.size()))) return rc2;
     }
     // vertex -> (t, role) incidence in t order (alm_vcs.py:94-104)
     std::vector<int64_t> cnt(n + 1, 0);
@@ -3597,6 +3608,7 @@ struct BArgs {
     double *zY; double *M; double *pdiag; double *ltot; double *lviol;
     const int64_t *iptr; const void *rec; void *pt; double rho, rt; int64_t T; const int32_t *TI; const void *lam;
     void *Gout; double *rowv; double *parts; double *lamsum;
+    const int64_t *tptr; const int32_t *tidx;   // triangles of each graph (CSR over graphs)
 };
 
 // z_g and the cluster sums of graph g = blockIdx.x (Y in LDS when it fits, else global atomics)
@@ -3794,11 +3806,16 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_brows(BArgs a) {
     }
 }
 
+// sum of lam over each graph's triangles (graph = blockIdx.x; its triangle ids in tptr/tidx), in a fixed order
 template <typename TC>
-__global__ void lb_k_blamsum(BArgs a) {
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_blamsum(BArgs a) {
+    __shared__ double red[LB_BLOCK / 64];
     const TC *lam = (const TC *)a.lam;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < a.T; t += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(a.lamsum + a.gid[a.TI[3 * t]], (double)lam[t]);
+    const int g = blockIdx.x;
+    double s = 0.0;
+    for (int64_t k = a.tptr[g] + threadIdx.x; k < a.tptr[g + 1]; k += LB_BLOCK) s += (double)lam[a.tidx[k]];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) a.lamsum[g] = s;
 }
 
 // per-graph scalars (graph = blockIdx.x), fixed-order sums over its rows and clusters
@@ -3853,8 +3870,8 @@ struct BLaunch {
         hipLaunchKernelGGL((lb_k_bcolsum<TV, R>), dim3(c->bG), dim3(LB_BLOCK), lds, s, a);
         if (c->blev && c->bsumP) hipLaunchKernelGGL(lb_k_blevels, dim3(c->bsumP), dim3(64), 0, s, a, R);
         if (c->T) {
-            if (c->cbytes == 8) hipLaunchKernelGGL(lb_k_blamsum<double>, dim3(1024), dim3(256), 0, s, a);
-            else hipLaunchKernelGGL(lb_k_blamsum<float>, dim3(1024), dim3(256), 0, s, a);
+            if (c->cbytes == 8) hipLaunchKernelGGL(lb_k_blamsum<double>, dim3(c->bG), dim3(LB_BLOCK), 0, s, a);
+            else hipLaunchKernelGGL(lb_k_blamsum<float>, dim3(c->bG), dim3(LB_BLOCK), 0, s, a);
         }
         const int64_t nblk = std::min<int64_t>((c->n + 3) / 4, 8192);
         hipLaunchKernelGGL((lb_k_brows<TV, R>), dim3((unsigned)nblk), dim3(LB_BLOCK), 0, s, a);
@@ -3891,6 +3908,7 @@ int lb_batch_set(lb_ctx *c, int G, const int64_t *row_off, const double *L, cons
         if (!(L[g] > 0)) return fail(c, -1, "lb_batch_set: L must be positive");
     }
     if (c->nvcs) return fail(c, -1, "lb_batch_set: VCS levels are not supported in batch mode");
+    if (c->T) return fail(c, -1, "lb_batch_set: set the triangles after lb_batch_set (they are indexed per graph)");
     std::vector<int32_t> gid((size_t)c->n);
     for (int g = 0; g < G; g++)
         for (int64_t i = row_off[g]; i < row_off[g + 1]; i++) gid[i] = g;
@@ -3960,6 +3978,7 @@ int lb_batch_eval(lb_ctx *c, double rho, double rt, double *parts) {
     HIPCHK(c, hipSetDevice(c->device));
     beval_fn f = pick_beval(c->vdtype, c->r);
     if (!f) return fail(c, -1, "lb_batch_eval: unsupported rank / dtype");
+    if (c->T && !c->b_tptr.p) return fail(c, -1, "lb_batch_eval: triangles were set before lb_batch_set");
     BArgs a{};
     a.V = (const uint4 *)c->V.p; a.n = c->n; a.rowptr = (const int64_t *)c->rowptr.p; a.col = (const int32_t *)c->col.p;
     a.w = c->weighted ? c->w.p : nullptr; a.d = (const double *)c->d.p;
@@ -3974,6 +3993,7 @@ int lb_batch_eval(lb_ctx *c, double rho, double rt, double *parts) {
     a.iptr = (const int64_t *)c->iptr.p; a.rec = c->rec.p; a.pt = c->pt.p; a.T = c->T; a.TI = (const int32_t *)c->TI.p;
     a.lam = c->lam.p;
     a.Gout = c->G.p; a.rowv = (double *)c->b_rowv.p; a.parts = (double *)c->b_parts.p; a.lamsum = (double *)c->b_lamsum.p;
+            a.tptr = (const int64_t *)c->b_tptr.p; a.tidx = (const int32_t *)c->b_tidx.p;
     int rc = f(c, rho, rt, a);
     if (rc) return rc;
     if (parts) {
@@ -4002,6 +4022,7 @@ int lb_capture_eval(lb_ctx *c, double rho, double rt) {
     if (c->bG) {
         beval_fn f = pick_beval(c->vdtype, c->r);
         if (!f) rc = fail(c, -1, "lb_capture_eval: unsupported rank / dtype");
+        else if (c->T && !c->b_tptr.p) rc = fail(c, -1, "lb_capture_eval: triangles were set before lb_batch_set");
         else {
             BArgs a{};
             a.V = (const uint4 *)c->V.p; a.n = c->n; a.rowptr = (const int64_t *)c->rowptr.p; a.col = (const int32_t *)c->col.p;
@@ -4017,6 +4038,7 @@ int lb_capture_eval(lb_ctx *c, double rho, double rt) {
             a.iptr = (const int64_t *)c->iptr.p; a.rec = c->rec.p; a.pt = c->pt.p; a.T = c->T; a.TI = (const int32_t *)c->TI.p;
             a.lam = c->lam.p;
             a.Gout = c->G.p; a.rowv = (double *)c->b_rowv.p; a.parts = (double *)c->b_parts.p; a.lamsum = (double *)c->b_lamsum.p;
+            a.tptr = (const int64_t *)c->b_tptr.p; a.tidx = (const int32_t *)c->b_tidx.p;
             rc = f(c, rho, rt, a);
         }
     } else {
