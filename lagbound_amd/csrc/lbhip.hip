@@ -1746,6 +1746,7 @@ struct lb_ctx {
         b_lviol, b_rowv, b_parts, b_lamsum, b_tptr, b_tidx;
     double bw = 1.0;
     void *gexec = nullptr; double g_rho = 0.0;   // captured evaluation (hipGraphExec_t)
+    bool b_lamsum_ok = false;                    // batch: per-graph lam sums up to date
 };
 
 namespace {
@@ -2721,6 +2722,7 @@ int lb_set_vcs_sig(lb_ctx *c, const double *sig) {
 
 int lb_set_triangles(lb_ctx *c, int64_t T, const int32_t *TI, const double *lam) {
     drop_capture(c);
+    if (c) c->b_lamsum_ok = false;
     if (!c || !c->n) return fail(c, -1, "lb_set_triangles: set the graph first");
     if (T < 0 || T > ((int64_t)1 << 29)) return fail(c, -1, "lb_set_triangles: bad T");
     HIPCHK(c, hipSetDevice(c->device));
@@ -2801,6 +2803,7 @@ int lb_set_triangles(lb_ctx *c, int64_t T, const int32_t *TI, const double *lam)
 
 int lb_set_lam(lb_ctx *c, const double *lam) {
     drop_capture(c);
+    if (c) c->b_lamsum_ok = false;
     if (!c || !c->n) return fail(c, -1, "lb_set_lam: no graph");
     if (!c->T) return 0;
     HIPCHK(c, hipSetDevice(c->device));
@@ -2919,6 +2922,7 @@ int lb_get_multipliers(lb_ctx *c, double *pp, double *pt, double *pv) {
 
 int lb_phr_update(lb_ctx *c) {
     drop_capture(c);
+    if (c) c->b_lamsum_ok = false;
     if (!c || !c->have_eval) return fail(c, -1, "lb_phr_update: no evaluation yet");
     if (!(c->last_rho > 0.0)) return fail(c, -1, "lb_phr_update: the last evaluation had rho = 0");
     HIPCHK(c, hipSetDevice(c->device));
@@ -3862,17 +3866,22 @@ struct BLaunch {
         hipStream_t s = c->stream;
         const int64_t ylen = (int64_t)c->bG * R + (int64_t)c->bsumP * R;
         hipLaunchKernelGGL(lb_k_fill, dim3(256), dim3(256), 0, s, a.zY, ylen, 0.0);
-        hipLaunchKernelGGL(lb_k_fill, dim3(64), dim3(256), 0, s, a.lamsum, (int64_t)c->bG, 0.0);
+        // per-graph sums of lam: recomputed only after lam changed (lb_set_triangles / lb_set_lam / lb_phr_update)
+        if (!c->b_lamsum_ok) {
+            if (c->T) {
+                if (c->cbytes == 8) hipLaunchKernelGGL(lb_k_blamsum<double>, dim3(c->bG), dim3(LB_BLOCK), 0, s, a);
+                else hipLaunchKernelGGL(lb_k_blamsum<float>, dim3(c->bG), dim3(LB_BLOCK), 0, s, a);
+            } else {
+                hipLaunchKernelGGL(lb_k_fill, dim3(64), dim3(256), 0, s, a.lamsum, (int64_t)c->bG, 0.0);
+            }
+            c->b_lamsum_ok = true;
+        }
         int maxP = 0;
         for (int P : c->bP) maxP = std::max(maxP, P);
         const size_t lds = (c->blev && (size_t)maxP * R * 8 <= 32 * 1024) ? (size_t)maxP * R * 8 : 0;
         a.ylds = lds > 0 ? 1 : 0;
         hipLaunchKernelGGL((lb_k_bcolsum<TV, R>), dim3(c->bG), dim3(LB_BLOCK), lds, s, a);
         if (c->blev && c->bsumP) hipLaunchKernelGGL(lb_k_blevels, dim3(c->bsumP), dim3(64), 0, s, a, R);
-        if (c->T) {
-            if (c->cbytes == 8) hipLaunchKernelGGL(lb_k_blamsum<double>, dim3(c->bG), dim3(LB_BLOCK), 0, s, a);
-            else hipLaunchKernelGGL(lb_k_blamsum<float>, dim3(c->bG), dim3(LB_BLOCK), 0, s, a);
-        }
         const int64_t nblk = std::min<int64_t>((c->n + 3) / 4, 8192);
         hipLaunchKernelGGL((lb_k_brows<TV, R>), dim3((unsigned)nblk), dim3(LB_BLOCK), 0, s, a);
         hipLaunchKernelGGL(lb_k_bfinal, dim3(c->bG), dim3(LB_BLOCK), 0, s, a, R);
@@ -3900,6 +3909,7 @@ extern "C" {
 
 int lb_batch_set(lb_ctx *c, int G, const int64_t *row_off, const double *L, const double *C0) {
     drop_capture(c);
+    if (c) c->b_lamsum_ok = false;
     if (!c || !c->n) return fail(c, -1, "lb_batch_set: set the graph (the union of the batch) first");
     if (G < 1 || !row_off || !L || !C0) return fail(c, -1, "lb_batch_set: bad arguments");
     if (row_off[0] != 0 || row_off[G] != c->n) return fail(c, -1, "lb_batch_set: row offsets must run from 0 to n");
