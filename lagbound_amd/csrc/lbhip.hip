@@ -169,6 +169,7 @@ struct VcsArgs {
     const double *Y; double wv, rho; int sumPv; int use_lds;
     double *part;              // gridDim x (2 + sumPv R) when use_lds, else gridDim x 2
     double *Zglob;
+    int zacc;                  // 1: lb_k_vcs_z also accumulates Z; 0: lb_k_vcs_acc does it in a second pass
 };
 
 struct SegArgs {
@@ -336,7 +337,8 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
     const double *Yb = a.Y + (int64_t)a.Yoff[0] * R;   // the VCS cluster sums (all VCS levels, consecutive)
     if (a.use_lds) {
         for (int e = threadIdx.x; e < width; e += LB_BLOCK) shY[e] = Yb[e];
-        for (int e = threadIdx.x; e < NW * width; e += LB_BLOCK) shZ[e] = (TC)0;
+        if (a.zacc)
+            for (int e = threadIdx.x; e < NW * width; e += LB_BLOCK) shZ[e] = (TC)0;
         __syncthreads();
         Yb = shY;
     }
@@ -385,7 +387,7 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
                     a.slack[e] = (p == 0.0 && g > 0.0) ? 1 : 0;   // inactive with slack (vcsops.py:285)
                 }
                 const TC pc = (TC)p;  // the stored multiplier is what G uses
-                if (pc != (TC)0) {
+                if (a.zacc && pc != (TC)0) {
                     if (a.use_lds) {
                         TC *dst = myZ + (a.Zoff[m] + Rc) * R + q * CPL;
 #pragma unroll
@@ -406,11 +408,59 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
     const int pw = 2 + width;
     double *out = a.part + (int64_t)blockIdx.x * pw;
     if (threadIdx.x == 0) { out[0] = tot; out[1] = viol; }
-    for (int e = threadIdx.x; e < width; e += LB_BLOCK) {
-        double z = 0.0;
-        for (int k = 0; k < NW; k++) z += (double)shZ[k * width + e];
-        out[2 + e] = z;
+    if (a.zacc)
+        for (int e = threadIdx.x; e < width; e += LB_BLOCK) {
+            double z = 0.0;
+            for (int k = 0; k < NW; k++) z += (double)shZ[k * width + e];
+            out[2 + e] = z;
+        }
+}
+
+// Z_R = sum over the working-set entries (i, R) of pv_iR d_i V_i, after lb_k_vcs_z wrote pv: a pass with nothing
+// but the accumulation (one FP64 copy of Z per workgroup in LDS, the entries' columns and multipliers loaded four
+// at a time ahead of the adds), written to the same per-workgroup partial rows as lb_k_vcs_z.
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_acc(VcsArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    extern __shared__ double shZ[];
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    const int width = a.sumPv * R;
+    for (int e = threadIdx.x; e < width; e += LB_BLOCK) shZ[e] = 0.0;
+    __syncthreads();
+    const TC *pv = (const TC *)a.pv;
+    for (int64_t i = (int64_t)blockIdx.x * NSL + slot; i < a.n; i += (int64_t)gridDim.x * NSL) {
+        TC x[CPL];
+        VT<TV>::unpack(a.V[i * S + q], x);
+        const double di = a.d[i];
+        double v[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) v[c] = di * (double)x[c];
+        for (int m = 0; m < a.nvcs; m++) {
+            const int64_t e0 = a.ptr[m][i], e1 = a.ptr[m][i + 1];
+            for (int64_t eb = e0; eb < e1; eb += 4) {
+                int Rq[4];
+                double pq[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int64_t ec = eb + t < e1 ? eb + t : e1 - 1;
+                    Rq[t] = a.col[ec];
+                    pq[t] = eb + t < e1 ? (double)pv[ec] : 0.0;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    if (pq[t] != 0.0) {
+                        double *dst = shZ + (int64_t)(a.Zoff[m] + Rq[t]) * R + q * CPL;
+#pragma unroll
+                        for (int c = 0; c < CPL; c++) atomicAdd(dst + c, pq[t] * v[c]);
+                    }
+                }
+            }
+        }
     }
+    __syncthreads();
+    double *out = a.part + (int64_t)blockIdx.x * (2 + width);
+    for (int e = threadIdx.x; e < width; e += LB_BLOCK) out[2 + e] = shZ[e];
 }
 
 // ------------------------------------------------------------------------------------------------ triangle record
@@ -1946,8 +1996,12 @@ struct Launch {
             va.col = (const int32_t *)c->vcol.p; va.sig = c->vsig.p; va.pv = c->pv.p; va.slack = (uint8_t *)c->vslack.p;
             va.Y = Yp; va.wv = c->wv; va.rho = rho;
             va.sumPv = c->sumPv; va.use_lds = vcs_lds; va.part = (double *)c->vcs_part.p; va.Zglob = (double *)c->Z.p;
+            // with the LDS path, the multipliers and the Z sums are two passes (the second has only the adds)
+            va.zacc = vcs_lds ? 0 : 1;
+            const size_t ldsY = sizeof(double) * (size_t)c->sumPv * R;
             if (!vcs_lds) hipLaunchKernelGGL(lb_k_fill, dim3(256), dim3(256), 0, c->stream, (double *)c->Z.p, (int64_t)c->sumPv * R, 0.0);
-            hipLaunchKernelGGL((lb_k_vcs_z<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), vcs_lds ? ldsZ : 0, c->stream, va);
+            hipLaunchKernelGGL((lb_k_vcs_z<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), vcs_lds ? ldsY : 0, c->stream, va);
+            if (vcs_lds) hipLaunchKernelGGL((lb_k_vcs_acc<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), ldsY, c->stream, va);
         }
         mark();
         if (c->nvcs && vcs_lds) {
