@@ -60,6 +60,38 @@ def time_engine(eng, rho, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def large_graph(tuned, rho, steps):
+    """The HBM-scale evaluation: Chung-Lu n = 2.4M, gamma 2.2, mean degree 50 (~120M CSR entries, int64 row
+    pointers), 8M wedge triangles, pair + VCS levels over 64 LPA clusters; BF16-stored V (FP32 accumulation) and
+    FP32 V, with the metric instance's launch shapes. Reports evals/s and compulsory GB/s against the HBM peak."""
+    from lagbound_amd import instances as I
+    from lagbound_amd import tune as TU
+    from lagbound_amd.engine import Engine
+    t0 = time.time()
+    log("building the n=2.4M graph (gamma 2.2, mean degree 50, 8M triangles)")
+    inst, V = I.metric_instance(2_400_000, 2.2, 50.0, T=8_000_000)
+    out = {"n": inst.n, "nnz": inst.nnz, "T": inst.T, "pair_P": inst.levels[0].P, "vcs_ws": inst.vcs[0].nnz,
+           "r": V.shape[1], "rho": rho, "built_s": round(time.time() - t0, 1)}
+    log(f"n={inst.n} nnz={inst.nnz} T={inst.T} built in {out['built_s']}s")
+    for dt, sv in (("bf16", 2), ("f32", 4)):
+        e = Engine(r=V.shape[1], dtype=dt)
+        if tuned:
+            TU.apply(e, tuned)
+        e.set_instance(inst)
+        e.set_V(V)
+        e.eval(rho)
+        sec = time_engine(e, rho, max(20, steps // 4), 5)
+        ms, _ = e.time_eval(rho, iters=20, per_kernel=False)
+        b = alg_bytes(inst, V.shape[1], sv, 4, 4)
+        gbs = b / (ms * 1e-3) / 1e9
+        out[dt] = {"evals_per_s": round(1.0 / sec, 2), "ms_per_eval": round(sec * 1e3, 4),
+                   "ms_per_eval_events": round(ms, 4), "alg_bytes": b, "achieved_GBs": round(gbs, 1),
+                   "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"n=2.4M {dt}: {sec * 1e3:.3f} ms/eval, {gbs:.0f} GB/s compulsory")
+        e.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +108,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream (batched bound latency)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched evaluation")
     ap.add_argument("--serve-count", type=int, default=16)
+    ap.add_argument("--no-large", action="store_true", help="skip the n=2.4M BF16 evaluation")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain copy kernel of known size (for scaling hardware byte counters)")
     a = ap.parse_args()
@@ -192,6 +225,8 @@ def main():
             red["bit_exact_vs_oracle"] = all(np.array_equal(np.asarray(ro[k]), np.asarray(resr[k])) for k in ro)
         extra["reduction_300k"] = red
         log(f"reduction n={nr} m={len(kr)}: {red['ms_gpu_call']:.2f} ms (device call)")
+        if not a.no_large:
+            extra["chung_lu_2400k"] = large_graph(tuned, rho, a.steps)
         out["variants"] = extra
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

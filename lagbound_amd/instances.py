@@ -94,17 +94,21 @@ def csr_from_keys(n: int, keys: np.ndarray, weights: np.ndarray | None = None):
     keys = np.asarray(keys, np.int64)
     u = keys // n
     v = keys % n
-    r = np.r_[u, v]
-    c = np.r_[v, u]
-    order = np.lexsort((c, r))
-    r = r[order]
+    # keys sorted (u, v): entries (v, u) first, then (u, v); a stable sort by row alone then leaves every row's
+    # columns ascending (below-diagonal ones from the first half, above-diagonal ones from the second)
+    korder = np.argsort(keys, kind="stable")
+    u = u[korder]
+    v = v[korder]
+    r = np.r_[v, u]
+    c = np.r_[u, v]
+    order = np.argsort(r, kind="stable")
     c = c[order]
     rowptr = np.zeros(n + 1, np.int64)
-    np.add.at(rowptr, r + 1, 1)
-    rowptr = np.cumsum(rowptr)
+    rowptr[1:] = np.cumsum(np.bincount(r, minlength=n))
     w = None
     if weights is not None:
-        ww = np.r_[weights, weights].astype(np.float64)[order]
+        wk = np.asarray(weights, np.float64)[korder]
+        ww = np.r_[wk, wk][order]
         if not np.all(ww == 1.0):
             w = ww
     return rowptr, c.astype(np.int32), w
@@ -196,6 +200,28 @@ def _finish_graph(n_all: int, a: np.ndarray, b: np.ndarray):
     return n, new_id[lo] * n + new_id[hi]
 
 
+def _search_sorted_queries(cw: np.ndarray, q: np.ndarray) -> np.ndarray:
+    """np.searchsorted(cw, q, side="right") for queries q >= 0 and a nondecreasing cw, exactly, through a bucket
+    table: tab[k] = searchsorted(cw, k / B) is a lower bound of the answer for every q in [k / B, (k + 1) / B)
+    (B a power of two, so q * B and k / B are exact), and a few vectorised steps of `while cw[i] <= q: i += 1` finish
+    it. Avoids one cache-missing binary search per query."""
+    m = len(cw)
+    B = 1 << max(10, min(26, int(np.ceil(np.log2(4 * max(m, 1))))))
+    tab = np.searchsorted(cw, np.arange(B, dtype=np.float64) / B, side="right")
+    out = tab[np.minimum((q * B).astype(np.int64), B - 1)]
+    act = np.flatnonzero(out < m)
+    for _ in range(64):
+        if len(act) == 0:
+            break
+        step = cw[out[act]] <= q[act]
+        act = act[step]
+        out[act] += 1
+        act = act[out[act] < m]
+    if len(act):
+        out[act] = np.searchsorted(cw, q[act], side="right")
+    return out
+
+
 def chung_lu(n: int, gamma: float, kbar: float, seed: int, permute: bool = True):
     """Chung-Lu power-law graph: expected degrees w_i = kbar (g-2)/(g-1) (n/(i+1))^(1/(g-1)) (minimum ~1 for
     g = 2.1), m = n kbar / 2 endpoint pairs drawn independently proportional to w, self-loops and duplicates dropped,
@@ -209,8 +235,8 @@ def chung_lu(n: int, gamma: float, kbar: float, seed: int, permute: bool = True)
     cw = np.cumsum(w)
     cw /= cw[-1]
     m = int(round(n * kbar / 2.0))
-    a = np.searchsorted(cw, rng.random(m), side="right")
-    b = np.searchsorted(cw, rng.random(m), side="right")
+    a = _search_sorted_queries(cw, rng.random(m))
+    b = _search_sorted_queries(cw, rng.random(m))
     a = np.minimum(a, n - 1)
     b = np.minimum(b, n - 1)
     return _finish_graph(n, a, b)
@@ -266,7 +292,7 @@ def sbm(n: int, blocks: int, p_in: float, p_out: float, seed: int):
 
 
 def label_propagation(rowptr: np.ndarray, col: np.ndarray, n: int, seed: int, P: int | None = None,
-                      iters: int = 8) -> np.ndarray:
+                      iters: int = 8, dense: bool | None = None) -> np.ndarray:
     """Seeded synchronous label propagation: every vertex takes the most frequent label among its neighbours, ties
     broken by a seeded random key per label; isolated vertices keep their label. Starts from singleton labels
     (P None) or from uniform random labels in [0, P) (a balanced partition of a graph without planted structure)."""
@@ -276,7 +302,23 @@ def label_propagation(rowptr: np.ndarray, col: np.ndarray, n: int, seed: int, P:
     key = rng.permutation(nk).astype(np.int64)
     rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr))
     colv = col.astype(np.int64)
+    dense = P is not None and (n * nk <= 400_000_000 if dense is None else dense)
+    if dense:
+        deg = np.diff(rowptr)
+        tie = (nk - 1 - key).astype(np.int64)          # larger for a smaller random key
     for _ in range(iters):
+        if dense:
+            # per-row label counts as an n x P table; score = count * P + tie picks the max count, then the label of
+            # the smallest random key, as the sort below does; rows without neighbours keep their label
+            cnt = np.bincount(rows * nk + lab[colv], minlength=n * nk).reshape(n, nk)
+            score = cnt * nk + tie[None, :]
+            best = np.argmax(score, axis=1)
+            new = np.where(deg > 0, best, lab)
+            del cnt, score
+            if np.array_equal(new, lab):
+                break
+            lab = new
+            continue
         kk = rows * nk + lab[colv]
         uq, cnt = np.unique(kk, return_counts=True)
         r_ = uq // nk
@@ -293,7 +335,8 @@ def label_propagation(rowptr: np.ndarray, col: np.ndarray, n: int, seed: int, P:
     return np.unique(lab, return_inverse=True)[1].astype(np.int32)
 
 
-def wedge_triangles(rowptr: np.ndarray, col: np.ndarray, d: np.ndarray, T: int, seed: int) -> np.ndarray:
+def wedge_triangles(rowptr: np.ndarray, col: np.ndarray, d: np.ndarray, T: int, seed: int,
+                    packed: bool | None = None) -> np.ndarray:
     """T distinct wedge triangles (a, j, c): centre j drawn proportional to degree (>= 2), a != c two distinct
     neighbours of j; canonical (min(a, c), j, max(a, c)); duplicates dropped and redrawn."""
     rng = np.random.default_rng(seed)
@@ -305,7 +348,9 @@ def wedge_triangles(rowptr: np.ndarray, col: np.ndarray, d: np.ndarray, T: int, 
     p = deg[cand].astype(np.float64)
     cp = np.cumsum(p)
     cp /= cp[-1]
-    got = np.zeros(0, np.int64)
+    # one int64 key (lo n + j) n + hi per triangle while n^3 fits in int64; beyond (n >= 2^21) unique rows
+    packed = n ** 3 < 2 ** 63 if packed is None else (packed and n ** 3 < 2 ** 63)
+    got = np.zeros(0, np.int64) if packed else np.zeros((0, 3), np.int64)
     rounds = 0
     while len(got) < T and rounds < 50:
         need = int((T - len(got)) * 1.2) + 64
@@ -318,9 +363,15 @@ def wedge_triangles(rowptr: np.ndarray, col: np.ndarray, d: np.ndarray, T: int, 
         c = col[rowptr[j] + y].astype(np.int64)
         lo = np.minimum(a, c)
         hi = np.maximum(a, c)
-        kk = (lo * n + j) * n + hi
-        got = np.unique(np.r_[got, kk])
+        if packed:
+            got = np.unique(np.r_[got, (lo * n + j) * n + hi])
+        else:
+            got = _unique_rows(np.r_[got, np.c_[lo, j, hi]])
         rounds += 1
+    if not packed:
+        if len(got) > T:
+            got = got[np.sort(rng.permutation(len(got))[:T])]
+        return got.astype(np.int32)
     if len(got) > T:
         got = rng.permutation(got)[:T]
     got = np.sort(got)
@@ -328,6 +379,15 @@ def wedge_triangles(rowptr: np.ndarray, col: np.ndarray, d: np.ndarray, T: int, 
     j = (got // n) % n
     c = got % n
     return np.c_[a, j, c].astype(np.int32)
+
+
+def _unique_rows(M: np.ndarray) -> np.ndarray:
+    """Distinct rows of an (m, 3) integer array in lexicographic order."""
+    if len(M) == 0:
+        return M
+    M = M[np.lexsort((M[:, 2], M[:, 1], M[:, 0]))]
+    keep = np.r_[True, np.any(M[1:] != M[:-1], axis=1)]
+    return M[keep]
 
 
 def random_ws(lab: np.ndarray, P: int, per_row: int, seed: int):

@@ -97,6 +97,8 @@ class _Packed:
         self.vcol = np.ascontiguousarray(np.concatenate([v.col for v in vc]) if vc and sum(v.nnz for v in vc) else np.zeros(1), np.int32)
         self.vsig = np.ascontiguousarray(np.concatenate([v.sig for v in vc]) if vc and sum(v.nnz for v in vc) else np.zeros(1))
         self.TI = np.ascontiguousarray(inst.TI if inst.T else np.zeros((1, 3)), np.int32)
+        if inst.T and (self.TI.min() < 0 or self.TI.max() >= n):
+            raise ValueError("triangle vertex out of range")
         self.lam = np.ascontiguousarray(inst.lam if inst.T else np.zeros(1), np.float64)
         self.s = OrInst(n, r, _p(self.rowptr, _i64p), _p(self.col, _i32p), _p(self.w, _f64p), _p(self.d, _f64p),
                         float(inst.L), float(inst.C0),

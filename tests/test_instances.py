@@ -72,3 +72,50 @@ def test_wedge_triangles_are_wedges():
         assert min(a, j) * n + max(a, j) in E and min(c, j) * n + max(c, j) in E
     kk = (TI[:, 0].astype(np.int64) * n + TI[:, 1]) * n + TI[:, 2]
     assert len(np.unique(kk)) == len(kk)
+
+
+def test_wedge_triangles_row_path():
+    """The unpacked path (taken for n >= 2^21, where (lo n + j) n + hi overflows int64) yields distinct, sorted,
+    canonical wedges of the graph, as many as asked."""
+    n, keys = I.chung_lu(4000, 2.1, 10.0, 2)
+    inst = I.build_instance(n, keys)
+    TI = I.wedge_triangles(inst.rowptr, inst.col, inst.d, 5000, 3, packed=False)
+    assert len(TI) == 5000 and np.all(TI[:, 0] < TI[:, 2])
+    E = set(map(int, keys))
+    for a, j, c in TI:
+        assert min(a, j) * n + max(a, j) in E and min(c, j) * n + max(c, j) in E
+    kk = (TI[:, 0].astype(np.int64) * n + TI[:, 1]) * n + TI[:, 2]
+    assert np.all(np.diff(kk) > 0)
+
+
+def test_search_sorted_queries_exact():
+    rng = np.random.default_rng(0)
+    for n in (1, 2, 7, 1000, 100000):
+        for w in (rng.random(n), (n / (np.arange(n) + 1.0)) ** 0.9,
+                  np.where(rng.random(n) < 0.5, 0.0, 1.0) + (np.arange(n) == n - 1)):
+            cw = np.cumsum(w)
+            cw /= cw[-1]
+            q = np.r_[rng.random(20000), 0.0, np.nextafter(1.0, 0.0), cw[:5], 1.0]
+            assert np.array_equal(I._search_sorted_queries(cw, q), np.searchsorted(cw, q, side="right"))
+
+
+def test_label_propagation_dense_equals_sorted():
+    n, keys = I.chung_lu(20000, 2.1, 12.0, 4)
+    inst = I.build_instance(n, keys)
+    for P, it, seed in ((64, 2, 100), (7, 6, 3)):
+        a = I.label_propagation(inst.rowptr, inst.col, n, seed, P, it, dense=True)
+        b = I.label_propagation(inst.rowptr, inst.col, n, seed, P, it, dense=False)
+        assert np.array_equal(a, b)
+
+
+def test_csr_from_unsorted_weighted_keys():
+    n, keys = I.chung_lu(3000, 2.1, 8.0, 5)
+    w = np.arange(1, len(keys) + 1, dtype=np.float64)
+    perm = np.random.default_rng(1).permutation(len(keys))
+    r0, c0, w0 = I.csr_from_keys(n, keys, w)
+    r1, c1, w1 = I.csr_from_keys(n, keys[perm], w[perm])
+    assert np.array_equal(r0, r1) and np.array_equal(c0, c1) and np.array_equal(w0, w1)
+    rows = np.repeat(np.arange(n), np.diff(r0))
+    lo, hi = np.minimum(rows, c0), np.maximum(rows, c0)
+    kk = lo.astype(np.int64) * n + hi
+    assert np.array_equal(w0, w[np.searchsorted(keys, kk)])
