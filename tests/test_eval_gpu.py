@@ -2,7 +2,8 @@
 
 Tolerances (stated by the precision contract of the search): FP32 and BF16-stored V: f within 1e-5 relative,
 ||G - G64||_F / ||G64||_F <= 1e-5, with the oracle evaluated on the stored (for BF16: dequantised) V; FP64 path:
-1e-11 relative. Effective multipliers: pp (FP64 on the device) 1e-9 relative; pt within 1e-5 * rt absolute (tau in
+1e-11 relative. The penalty parts (and f) also get their FP64 rounding floor 4u sum (p^2 + s^2) / (2 c) (see
+_penalty_floor), which only matters where rho c is tiny against the multipliers. Effective multipliers: pp (FP64 on the device) 1e-9 relative; pt within 1e-5 * rt absolute (tau in
 FP32); pv 1e-6 relative to |pv| + |sig| (FP32-stored multipliers).
 """
 import numpy as np
@@ -14,6 +15,38 @@ from conftest import small_instance, chung_lu_instance
 pytestmark = pytest.mark.gpu
 
 TOL = {"f32": 1e-5, "bf16": 1e-5, "f64": 1e-11}
+U64 = 2.0 ** -53
+
+
+def _penalty_floor(inst, rho, pp, pt, pv):
+    """Absolute rounding floor of the penalty parts, each a sum of (p^2 - s^2) / (2 c) over its entries (level:
+    c = rho rq_Q rq_R w, alm_vcs.py:476-496; triangles: c = rho, 498-509; VCS: c = rho wv rq_R / d_i^2,
+    vcsops.py:266-288). With c tiny the difference of squares cancels, and any change of rounding (summation order,
+    FMA contraction) moves a term by a few ulps of (p^2 + s^2) / (2 c): the oracle built with and without FMA
+    contraction differs by 4e-11 relative in f on the 300k-vertex instance. Returns {3: level, 4: tri, 5: vcs}."""
+    if rho <= 0.0:
+        return {3: 0.0, 4: 0.0, 5: 0.0}
+    fl = {3: 0.0, 4: 0.0, 5: 0.0}
+    o = 0
+    for lv in inst.levels:
+        P = lv.P
+        rr = rho * np.outer(lv.rq, lv.rq) * inst.pairlvl_weight
+        p_ = pp[o:o + P * P].reshape(P, P)
+        ok = rr > 0
+        fl[3] += float(np.sum((p_[ok] ** 2 + lv.sig[ok] ** 2) / (2 * rr[ok])))
+        o += P * P
+    if inst.T:
+        fl[4] = float(np.sum((pt ** 2 + inst.lam ** 2) / (2 * rho)))
+    o = 0
+    for v in inst.vcs:
+        rows = np.repeat(np.arange(inst.n), np.diff(v.ptr))
+        d = inst.d[rows]
+        c = rho * inst.wv * v.rq[v.col] / np.where(d > 0, d * d, 1.0)
+        ok = (d > 0) & (c > 0)
+        p_ = pv[o:o + v.nnz]
+        fl[5] += float(np.sum((p_[ok] ** 2 + v.sig[ok] ** 2) / (2 * c[ok])))
+        o += v.nnz
+    return {k: 4 * U64 * x for k, x in fl.items()}
 
 
 def _engine(inst, V, dtype, r, tuning=None):
@@ -34,12 +67,14 @@ def _check(eng, inst, rho, dtype, label):
     Vs = eng.get_V()                       # the stored V (BF16: dequantised)
     po, Go, ppo, pto, pvo = O.eval_c(inst, Vs, rho)
     tol = TOL[dtype]
-    f_err = abs(parts[0] - po[0]) / max(1.0, abs(po[0]))
+    floor = _penalty_floor(inst, rho, ppo, pto, pvo)
+    f_err = abs(parts[0] - po[0])
     g_err = np.linalg.norm(G - Go) / max(1e-300, np.linalg.norm(Go))
-    assert f_err <= tol, (label, "f", parts[0], po[0], f_err)
+    assert f_err <= tol * max(1.0, abs(po[0])) + sum(floor.values()), (label, "f", parts[0], po[0], f_err, floor)
     assert g_err <= tol, (label, "G", g_err)
     for k in (1, 2, 3, 4, 5, 6):
-        assert abs(parts[k] - po[k]) <= max(tol * max(1.0, abs(po[k])), tol * max(1.0, abs(po[0]))), (label, k, parts[k], po[k])
+        assert abs(parts[k] - po[k]) <= max(tol * max(1.0, abs(po[k])), tol * max(1.0, abs(po[0]))) + floor.get(k, 0.0), \
+            (label, k, parts[k], po[k], floor.get(k, 0.0))
     assert parts[8] == po[8] or dtype != "f64"
     assert np.allclose(pp, ppo, rtol=1e-9 if dtype != "f64" else 1e-12, atol=1e-12)
     if inst.T:
