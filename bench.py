@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched evaluation")
     ap.add_argument("--serve-count", type=int, default=16)
     ap.add_argument("--no-large", action="store_true", help="skip the n=2.4M BF16 evaluation")
+    ap.add_argument("--only-large", action="store_true",
+                    help="run only the n=2.4M evaluation (with --tuning if given) and print its JSON object")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain copy kernel of known size (for scaling hardware byte counters)")
     a = ap.parse_args()
@@ -116,6 +118,14 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if a.only_large:
+        tl = None
+        if a.tuning:
+            h, ht, sl, rg = [int(x) for x in a.tuning.split(",")]
+            tl = dict(heavy_edges=h, heavy_tris=ht, seg_len=sl, row_grid=rg, row_order=1)
+        print(json.dumps({"workload": "f+grad eval, Chung-Lu n=2.4M gamma=2.2 mean-degree~50",
+                          "launch_shapes": tl or "library defaults", **large_graph(tl, a.rho, a.steps)}), flush=True)
+        return
     import numpy as np
     from lagbound_amd import instances as I
     from lagbound_amd.engine import Engine
