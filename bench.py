@@ -145,7 +145,7 @@ def main():
         # launch shapes picked by the autotuner on this instance (hub threshold, segment length, row grid)
         from lagbound_amd import tune as TU
         t1 = time.time()
-        tuned, _ = TU.autotune(inst, V, rho, r=r, grid={"heavy": [64, 128, 256], "seg_len": [256],
+        tuned, _ = TU.autotune(inst, V, rho, r=r, grid={"heavy": [64, 128, 256, 512], "seg_len": [256, 512, 1024],
                                                         "row_grid": [2048, 4096]}, iters=10)
         TU.apply(eng, tuned)
         log(f"autotuned {tuned} in {time.time() - t1:.1f}s")

@@ -77,7 +77,7 @@ class BatchEngine:
 
     def __init__(self, packed: Packed, r=8, dtype="f32", device=0):
         self.p = packed
-        self.eng = Engine(r=r, dtype=dtype, device=device)
+        self.eng = Engine(r=r, dtype=dtype, device=device, order="given")
         e = self.eng
         union = I.Instance(n=packed.n, rowptr=packed.rowptr, col=packed.col, w=packed.w, d=packed.d,
                            L=float(packed.L.sum()), C0=0.0)

@@ -14,6 +14,7 @@ import numpy as np
 
 from . import _lib as LB
 from ._lib import LBError, ptr, _f64p, _i32p, _i64p
+from .vertex_order import Relabeled
 
 DTYPES = {"f32": LB.LB_F32, "bf16": LB.LB_BF16, "f64": LB.LB_F64}
 PART_NAMES = ["f", "quad", "zz_over_L", "level", "tri", "vcs", "primal", "C0", "level_nviol", "tri_gmin",
@@ -41,7 +42,9 @@ class _TimedLib:
         return call
 
 
-class Engine:
+class DeviceEngine:
+    """One device context, holding the instance in the numbering it was given."""
+
     def __init__(self, r: int = 16, dtype: str = "f32", device: int = 0):
         self.lib_s = 0.0
         self.n_pp = self.n_t = self.n_pv = 0          # sizes of what was uploaded (host buffers of the getters)
@@ -313,3 +316,16 @@ class Engine:
                                       ptr(km, _f64p) if per_kernel else None), "lb_time_eval")
         names = [self.L.lb_kernel_name(k).decode() for k in range(LB.LB_NKERNELS)]
         return float(ms[0]), dict(zip(names, km.tolist())) if per_kernel else None
+
+
+DEFAULT_ORDER = "degree"
+
+
+class Engine(Relabeled):
+    """The device handle used by the search: a `DeviceEngine` holding the instance with its vertices numbered by
+    descending degree (lagbound_amd.vertex_order), every argument and result in the caller's numbering.
+    order="given" uploads the instance as given (batch mode packs graphs by their given numbering)."""
+
+    def __init__(self, r: int = 16, dtype: str = "f32", device: int = 0, order: str | None = None):
+        make = lambda: DeviceEngine(r=r, dtype=dtype, device=device)  # noqa: E731
+        super().__init__(make(), make, order or DEFAULT_ORDER)

@@ -8,11 +8,12 @@ from __future__ import annotations
 import itertools
 
 from .engine import Engine
+from .vertex_order import VertexOrder
 
 DEFAULT_GRID = {
-    "heavy": [64, 128, 256],
-    "seg_len": [128, 256, 512],
-    "row_grid": [1024, 2048, 4096],
+    "heavy": [64, 128, 256, 512],
+    "seg_len": [256, 512, 1024],
+    "row_grid": [2048, 4096],
     "row_order": [1],
 }
 
@@ -30,12 +31,17 @@ def apply(eng: Engine, cfg):
     eng.set_option("row_order", cfg["row_order"])
 
 
-def autotune(inst, V, rho=0.02, r=None, dtype="f32", iters=20, grid=None, device=0, log=None):
-    """Time every candidate on (inst, V); returns (best config, [(config, ms per evaluation)])."""
+def autotune(inst, V, rho=0.02, r=None, dtype="f32", iters=20, grid=None, device=0, log=None, order="degree"):
+    """Time every candidate on (inst, V) in the vertex order the engines use (relabelled once here); returns
+    (best config, [(config, ms per evaluation)])."""
     r = r or V.shape[1]
+    if order == "degree":
+        vo = VertexOrder(inst.rowptr)
+        inst, _ = vo.instance(inst)
+        V = vo.rows_in(V)
     results = []
     for cfg in candidates(grid):
-        e = Engine(r=r, dtype=dtype, device=device)
+        e = Engine(r=r, dtype=dtype, device=device, order="given")
         try:
             apply(e, cfg)
             e.set_instance(inst)

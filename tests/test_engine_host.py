@@ -1,4 +1,4 @@
-"""Host bookkeeping of Engine without a device (the library replaced by a stub that accepts every call): the
+"""Host bookkeeping of the device context handle (DeviceEngine) without a device (the library replaced by a stub that accepts every call): the
 getters size their output buffers by what the engine uploaded. Regression test: a batch context uploads its
 triangles with set_triangles after set_instance(union without triangles); get_multipliers used to size pt by the
 instance's triangle count (0) while the library writes one value per uploaded triangle, overrunning the host
@@ -15,7 +15,7 @@ class _StubLib:
 
 
 def _engine(r=8):
-    e = object.__new__(E.Engine)
+    e = object.__new__(E.DeviceEngine)
     e.lib_s = 0.0
     e.n_pp = e.n_t = e.n_pv = 0
     e.r = r
