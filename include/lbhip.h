@@ -136,7 +136,8 @@ const char *lb_kernel_name(int k);
 /* Tuning knobs of the row kernels: rows with more than `heavy_edges` neighbours or `heavy_tris` triangle records
  * are split into segments of `seg_len` items processed by whole wavefronts (takes effect at the next lb_set_graph /
  * lb_set_triangles); `row_grid` workgroups run the grid-stride row kernel (0 keeps the current value). Defaults
- * 128, 128, 256, 2048 (measured best on the 1M-row power-law instance). No reference
+ * 128, 128, 1024, 8192: within 2% of the best measured on degree-numbered power-law graphs of 1M rows (mean degree
+ * 13) and 2.4M rows (mean degree 50). No reference
  * counterpart (the reference's parallel shapes are numba prange and chunk sizes, alm_vcs.py:1420-1423). */
 int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int row_grid);
 /* Named switches of the kernels (no reference counterpart): "row_order" (default 1) processes the rows of the f+grad

@@ -1765,8 +1765,8 @@ struct lb_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     size_t vbytes = 4, cbytes = 4;   // storage bytes of V elements, of TC values
-    int heavy_edges = 128, heavy_tris = 128, seg_len = 256;   // rows above these counts are split into segments
-    int64_t row_grid = 2048;                         // workgroups of the grid-stride row kernel
+    int heavy_edges = 128, heavy_tris = 128, seg_len = 1024;  // rows above these counts are split into segments
+    int64_t row_grid = 8192;                         // workgroups of the grid-stride row kernel
     int64_t built_heavyE = 64, built_heavyT = 64;   // thresholds the current segment tables were built with
     // graph
     int64_t n = 0, nnz = 0; double L = 1.0, C0 = 0.0; bool weighted = false;

@@ -75,7 +75,7 @@ class DeviceEngine:
             raise LBError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
 
     # ------------------------------------------------------------------------------------------------ upload
-    def set_tuning(self, heavy_edges=128, heavy_tris=128, seg_len=256, row_grid=0):
+    def set_tuning(self, heavy_edges=128, heavy_tris=128, seg_len=1024, row_grid=0):
         """Hub thresholds and segment length (take effect at the next set_graph / set_triangles); row_grid 0 keeps
         the current grid."""
         self._chk(self.L.lb_set_tuning(self.ctx, heavy_edges, heavy_tris, seg_len, row_grid), "lb_set_tuning")
