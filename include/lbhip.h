@@ -77,6 +77,12 @@ int lb_set_lam(lb_ctx *ctx, const double *lam);
  * rounds to nearest even and lb_get_V returns the stored (dequantised) values. */
 int lb_set_V(lb_ctx *ctx, const void *V, int host_dtype);
 int lb_get_V(lb_ctx *ctx, void *V, int host_dtype);
+/* Row order of the V / G transfers: device row k is the caller's row order[k] (a permutation of 0..n-1), applied by
+ * lb_set_V, lb_get_V and lb_get_G in their host copy loops; NULL restores the identity, and lb_set_graph clears
+ * it. For a graph uploaded with its vertices renumbered (new id k = old vertex order[k]), the caller then keeps V
+ * and G in the old numbering. Refused on batch contexts. No reference counterpart (the reference keeps the given
+ * numbering). */
+int lb_set_row_order(lb_ctx *ctx, const int64_t *order);
 
 /* One Lagrangian f+grad evaluation at the resident V and multipliers (the metric's unit of work):
  *   rho > 0: f = augmented(V, rho) (alm_vcs.py:1496-1533) with rt = rho * tri_weight;

@@ -1770,6 +1770,7 @@ struct lb_ctx {
     int64_t built_heavyE = 64, built_heavyT = 64;   // thresholds the current segment tables were built with
     // graph
     int64_t n = 0, nnz = 0; double L = 1.0, C0 = 0.0; bool weighted = false;
+    std::vector<int64_t> row_order;   // empty, or: device row k holds the caller's row row_order[k] (V, G transfers)
     std::vector<int64_t> h_rowptr;
     // levels
     int nlev = 0; std::vector<int> P; std::vector<int64_t> soff; double pw = 1.0; int sumPpair = 0;
@@ -2618,6 +2619,7 @@ int lb_set_graph(lb_ctx *c, int64_t n, const int64_t *rowptr, const int32_t *col
         if (col[e] < 0 || col[e] >= n) return fail(c, -1, "lb_set_graph: column index out of range");
     c->n = n; c->nnz = nnz; c->L = L; c->C0 = C0; c->weighted = weights != nullptr;
     c->bG = 0; c->blev = false; c->bsumP = 0;
+    c->row_order.clear();
     c->h_rowptr.assign(rowptr, rowptr + n + 1);
     c->h_d.assign(d, d + n);
     int rc;
@@ -2877,7 +2879,13 @@ int lb_set_V(lb_ctx *c, const void *V, int host_dtype) {
     if (host_dtype != LB_F64 && host_dtype != LB_F32) return fail(c, -1, "lb_set_V: host dtype must be LB_F64 or LB_F32");
     HIPCHK(c, hipSetDevice(c->device));
     const size_t cnt = (size_t)c->n * c->r;
-    auto val = [&](size_t e) -> double { return host_dtype == LB_F64 ? ((const double *)V)[e] : (double)((const float *)V)[e]; };
+    const int64_t *ord = c->row_order.empty() ? nullptr : c->row_order.data();
+    const size_t rr = (size_t)c->r;
+    // element e of the device array comes from the caller's row ord[e / r] (or row e / r)
+    auto val = [&](size_t e) -> double {
+        const size_t h = ord ? (size_t)ord[e / rr] * rr + e % rr : e;
+        return host_dtype == LB_F64 ? ((const double *)V)[h] : (double)((const float *)V)[h];
+    };
     if (c->vdtype == LB_F64) {
         std::vector<double> t(cnt);
         for (size_t e = 0; e < cnt; e++) t[e] = val(e);
@@ -2901,12 +2909,15 @@ int lb_get_V(lb_ctx *c, void *V, int host_dtype) {
     std::vector<unsigned char> raw(cnt * c->vbytes);
     HIPCHK(c, hipMemcpyAsync(raw.data(), c->V.p, raw.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t *ord = c->row_order.empty() ? nullptr : c->row_order.data();
+    const size_t rr = (size_t)c->r;
     for (size_t e = 0; e < cnt; e++) {
         double v;
         if (c->vdtype == LB_F64) v = ((const double *)raw.data())[e];
         else if (c->vdtype == LB_F32) v = ((const float *)raw.data())[e];
         else v = bf2f(((const uint16_t *)raw.data())[e]);
-        if (host_dtype == LB_F64) ((double *)V)[e] = v; else ((float *)V)[e] = (float)v;
+        const size_t h = ord ? (size_t)ord[e / rr] * rr + e % rr : e;
+        if (host_dtype == LB_F64) ((double *)V)[h] = v; else ((float *)V)[h] = (float)v;
     }
     return 0;
 }
@@ -2931,29 +2942,38 @@ int lb_eval_fgrad(lb_ctx *c, double rho, double rt, double *parts) {
 int lb_get_G(lb_ctx *c, void *G, int host_dtype) {
     if (!c || !c->n || !G) return fail(c, -1, "lb_get_G: no graph or G");
     if (!c->have_eval) return fail(c, -1, "lb_get_G: no evaluation yet");
+    if (host_dtype != LB_F64 && host_dtype != LB_F32) return fail(c, -1, "lb_get_G: host dtype must be LB_F64 or LB_F32");
     HIPCHK(c, hipSetDevice(c->device));
     const size_t cnt = (size_t)c->n * c->r;
-    if (c->cbytes == 8) {
-        if (host_dtype == LB_F64) {
-            HIPCHK(c, hipMemcpyAsync(G, c->G.p, cnt * 8, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            return 0;
-        }
-        std::vector<double> t(cnt);
-        HIPCHK(c, hipMemcpyAsync(t.data(), c->G.p, cnt * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (size_t e = 0; e < cnt; e++) ((float *)G)[e] = (float)t[e];
-        return 0;
-    }
-    if (host_dtype == LB_F32) {
-        HIPCHK(c, hipMemcpyAsync(G, c->G.p, cnt * 4, hipMemcpyDeviceToHost, c->stream));
+    const size_t hb = host_dtype == LB_F64 ? 8 : 4;
+    if (c->row_order.empty() && hb == c->cbytes) {     // same layout and type: one copy
+        HIPCHK(c, hipMemcpyAsync(G, c->G.p, cnt * hb, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return 0;
     }
-    std::vector<float> t(cnt);
-    HIPCHK(c, hipMemcpyAsync(t.data(), c->G.p, cnt * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<unsigned char> raw(cnt * c->cbytes);
+    HIPCHK(c, hipMemcpyAsync(raw.data(), c->G.p, raw.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (size_t e = 0; e < cnt; e++) ((double *)G)[e] = t[e];
+    const int64_t *ord = c->row_order.empty() ? nullptr : c->row_order.data();
+    const size_t rr = (size_t)c->r;
+    for (size_t e = 0; e < cnt; e++) {
+        const double v = c->cbytes == 8 ? ((const double *)raw.data())[e] : (double)((const float *)raw.data())[e];
+        const size_t h = ord ? (size_t)ord[e / rr] * rr + e % rr : e;
+        if (host_dtype == LB_F64) ((double *)G)[h] = v; else ((float *)G)[h] = (float)v;
+    }
+    return 0;
+}
+
+int lb_set_row_order(lb_ctx *c, const int64_t *order) {
+    if (!c || !c->n) return fail(c, -1, "lb_set_row_order: set the graph first");
+    if (c->bG) return fail(c, -1, "lb_set_row_order: not available on a batch context");
+    if (!order) { c->row_order.clear(); return 0; }
+    std::vector<unsigned char> seen((size_t)c->n, 0);
+    for (int64_t k = 0; k < c->n; k++) {
+        if (order[k] < 0 || order[k] >= c->n || seen[(size_t)order[k]]) return fail(c, -1, "lb_set_row_order: not a permutation of the rows");
+        seen[(size_t)order[k]] = 1;
+    }
+    c->row_order.assign(order, order + c->n);
     return 0;
 }
 

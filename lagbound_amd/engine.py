@@ -150,6 +150,14 @@ class DeviceEngine:
         assert V.shape == (self.inst.n, self.r), (V.shape, self.inst.n, self.r)
         self._chk(self.L.lb_set_V(self.ctx, V.ctypes.data_as(C.c_void_p), LB.LB_F64), "lb_set_V")
 
+    def set_row_order(self, order):
+        """V / G transfers in the caller's numbering: device row k <-> caller row order[k] (None: identity)."""
+        if order is None:
+            self._chk(self.L.lb_set_row_order(self.ctx, None), "lb_set_row_order")
+            return
+        order = np.ascontiguousarray(order, np.int64)
+        self._chk(self.L.lb_set_row_order(self.ctx, ptr(order, _i64p)), "lb_set_row_order")
+
     def get_V(self) -> np.ndarray:
         out = np.empty((self.inst.n, self.r), np.float64)
         self._chk(self.L.lb_get_V(self.ctx, out.ctypes.data_as(C.c_void_p), LB.LB_F64), "lb_get_V")

@@ -55,7 +55,7 @@ class Params:
     cert_max_n: int = 40000          # dense FP64 certificate (device, rocSOLVER) up to this many vertices
     cert_margin: float = 1e-7
     cert_on_device: bool = True
-    blas_threads: int = 4            # host BLAS threads while serving (slots x threads ~ the host cores)
+    blas_threads: int = 2            # host BLAS threads per slot while serving (slots x threads ~ the host cores)
 
 
 @dataclass
@@ -137,7 +137,7 @@ def solve_request(req: Request, p: Params, device=0):
 class Server:
     """Continuous batching: `slots` worker threads, HBM-footprint admission against `budget_bytes`."""
 
-    def __init__(self, p: Params, slots=4, budget_bytes=None, device=0, progress=None):
+    def __init__(self, p: Params, slots=8, budget_bytes=None, device=0, progress=None):
         self.progress = progress                 # callable(Result) after each completed request
         self.p = p
         self.slots = slots
@@ -251,7 +251,8 @@ def main(argv=None):
     ap.add_argument("--n_hi", type=int, default=200000)
     ap.add_argument("--kbar_lo", type=float, default=6.0)
     ap.add_argument("--kbar_hi", type=float, default=30.0)
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--blas_threads", type=int, default=None, help="host BLAS threads per slot (default: Params)")
     ap.add_argument("--budget_gb", type=float, default=64.0)
     ap.add_argument("--outer", type=int, default=24)
     ap.add_argument("--seed", type=int, default=1)
@@ -260,6 +261,8 @@ def main(argv=None):
     reqs = poisson_stream(a.count, a.rate, a.n_lo, a.n_hi, a.kbar_lo, a.kbar_hi, a.seed)
     gen = time.perf_counter() - t
     p = Params(outer=a.outer)
+    if a.blas_threads:
+        p.blas_threads = a.blas_threads
     t = time.perf_counter()
     def progress(r):
         print(f"[serve] rid {r.rid} n {r.n} m {r.m} latency {r.latency:.3f}s service {r.finish - r.start:.3f}s "
@@ -269,7 +272,8 @@ def main(argv=None):
     wall = time.perf_counter() - t
     out = summarize(res, wall)
     out.update(stream=dict(count=a.count, rate=a.rate, n=[a.n_lo, a.n_hi], kbar=[a.kbar_lo, a.kbar_hi], seed=a.seed,
-                           slots=a.slots, budget_gb=a.budget_gb, outer=a.outer, generate_s=round(gen, 2)))
+                           slots=a.slots, blas_threads=p.blas_threads, budget_gb=a.budget_gb, outer=a.outer,
+                           generate_s=round(gen, 2)))
     print(json.dumps(out), flush=True)
     return 0
 

@@ -10,7 +10,8 @@ vertex `order[k]` (stable sort by descending degree). `Relabeled` wraps an engin
 context, or any object with the same methods) that holds the relabelled instance, and translates every argument and
 result back to the caller's numbering, so callers never see the device order:
 
-- V, G, sweep orders, diagonals: rows permuted;
+- V, G, sweep orders, diagonals: rows permuted (V and G by the backend's own copy loops when it has
+  `set_row_order`, as the device context does);
 - pair and VCS labels: permuted with their rows; VCS working-set entries (sig, age, effective multipliers) through
   an entry map;
 - triangles: vertex ids mapped, pool order unchanged (so lam and pt keep their indices);
@@ -125,6 +126,7 @@ class Relabeled:
         self.vmap = None
         self.inst = None
         self._sep = None
+        self._native_rows = False     # the backend permutes V / G rows in its own copies (set_row_order)
 
     def __getattr__(self, name):                      # everything without a translation goes to the backend
         b = self.__dict__.get("b")
@@ -153,6 +155,10 @@ class Relabeled:
         else:
             self.vo, self.vmap, J = None, None, inst
         self.b.set_instance(J)
+        self._native_rows = False
+        if self.vo is not None and hasattr(self.b, "set_row_order"):
+            self.b.set_row_order(self.vo.order)
+            self._native_rows = True
         self.inst = inst
 
     def set_levels(self, inst):
@@ -206,15 +212,15 @@ class Relabeled:
     def set_V(self, V):
         V = np.asarray(V, np.float64)
         assert V.shape == (self.inst.n, self.b.r), (V.shape, self.inst.n, self.b.r)
-        return self.b.set_V(V if self.vo is None else self.vo.rows_in(V))
+        return self.b.set_V(V if self.vo is None or self._native_rows else self.vo.rows_in(V))
 
     def get_V(self):
         V = self.b.get_V()
-        return V if self.vo is None else self.vo.rows_out(V)
+        return V if self.vo is None or self._native_rows else self.vo.rows_out(V)
 
     def get_G(self):
         G = self.b.get_G()
-        return G if self.vo is None else self.vo.rows_out(G)
+        return G if self.vo is None or self._native_rows else self.vo.rows_out(G)
 
     def get_multipliers(self):
         pp, pt, pv = self.b.get_multipliers()
