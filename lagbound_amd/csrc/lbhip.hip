@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/lbhip.h"
@@ -1771,6 +1772,7 @@ struct lb_ctx {
     // graph
     int64_t n = 0, nnz = 0; double L = 1.0, C0 = 0.0; bool weighted = false;
     std::vector<int64_t> row_order;   // empty, or: device row k holds the caller's row row_order[k] (V, G transfers)
+    void *hstage = nullptr; size_t hstage_bytes = 0;   // pinned host staging of the V / G transfers
     std::vector<int64_t> h_rowptr;
     // levels
     int nlev = 0; std::vector<int> P; std::vector<int64_t> soff; double pw = 1.0; int sumPpair = 0;
@@ -1878,6 +1880,54 @@ float bf2f(uint16_t h) {
     float f;
     std::memcpy(&f, &u, 4);
     return f;
+}
+
+// Pinned host staging buffer of at least `bytes`, kept by the context (the V / G copies go through it at DMA speed).
+int host_stage(lb_ctx *c, size_t bytes, void **out) {
+    if (c->hstage_bytes < bytes) {
+        if (c->hstage) hipHostFree(c->hstage);
+        c->hstage = nullptr; c->hstage_bytes = 0;
+        HIPCHK(c, hipHostMalloc(&c->hstage, bytes, hipHostMallocDefault));
+        c->hstage_bytes = bytes;
+    }
+    *out = c->hstage;
+    return 0;
+}
+
+// f(a, b) over row ranges [a, b) of [0, n), on up to 8 host threads for large n.
+template <class F>
+void par_rows(int64_t n, F f) {
+    const int nt = (int)std::min<int64_t>(8, std::max<int64_t>(1, n / 32768));
+    if (nt <= 1) { f((int64_t)0, n); return; }
+    const int64_t cs = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++) {
+        const int64_t a = t * cs, b = std::min<int64_t>(n, a + cs);
+        if (a < b) th.emplace_back(f, a, b);
+    }
+    for (auto &x : th) x.join();
+}
+
+// Host rows -> device storage rows: device row k <- host row ord[k] (or k), converted to `dt`.
+template <typename HT>
+void rows_to_storage(int64_t a, int64_t b, int r, const HT *src, int dt, void *dst, const int64_t *ord) {
+    for (int64_t k = a; k < b; k++) {
+        const HT *s = src + (ord ? ord[k] : k) * r;
+        if (dt == LB_F64) { double *d = (double *)dst + k * r; for (int j = 0; j < r; j++) d[j] = (double)s[j]; }
+        else if (dt == LB_F32) { float *d = (float *)dst + k * r; for (int j = 0; j < r; j++) d[j] = (float)s[j]; }
+        else { uint16_t *d = (uint16_t *)dst + k * r; for (int j = 0; j < r; j++) d[j] = f2bf((float)s[j]); }
+    }
+}
+
+// Device storage rows (type `st`: LB_F64, LB_F32 or LB_BF16) -> host rows: host row ord[k] (or k) <- device row k.
+template <typename HT>
+void storage_to_rows(int64_t a, int64_t b, int r, const void *src, int st, HT *dst, const int64_t *ord) {
+    for (int64_t k = a; k < b; k++) {
+        HT *d = dst + (ord ? ord[k] : k) * r;
+        if (st == LB_F64) { const double *s = (const double *)src + k * r; for (int j = 0; j < r; j++) d[j] = (HT)s[j]; }
+        else if (st == LB_F32) { const float *s = (const float *)src + k * r; for (int j = 0; j < r; j++) d[j] = (HT)s[j]; }
+        else { const uint16_t *s = (const uint16_t *)src + k * r; for (int j = 0; j < r; j++) d[j] = (HT)bf2f(s[j]); }
+    }
 }
 
 void build_segments(int64_t n, const int64_t *ptr, int64_t heavy, int seg_len, int kind, std::vector<int32_t> &rows,
@@ -2582,6 +2632,8 @@ void lb_destroy(lb_ctx *c) {
                     &c->b_pp, &c->b_rq, &c->b_dsq, &c->b_zY, &c->b_M, &c->b_pdiag, &c->b_ltot, &c->b_lviol, &c->b_rowv,
                     &c->b_parts, &c->b_lamsum, &c->b_tptr, &c->b_tidx};
     for (DBuf *b : bufs) dfree(*b);
+    if (c->hstage) hipHostFree(c->hstage);
+    c->hstage = nullptr; c->hstage_bytes = 0;
     if (c->gexec) hipGraphExecDestroy((hipGraphExec_t)c->gexec);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -2878,47 +2930,37 @@ int lb_set_V(lb_ctx *c, const void *V, int host_dtype) {
     if (!c || !c->n || !V) return fail(c, -1, "lb_set_V: no graph or V");
     if (host_dtype != LB_F64 && host_dtype != LB_F32) return fail(c, -1, "lb_set_V: host dtype must be LB_F64 or LB_F32");
     HIPCHK(c, hipSetDevice(c->device));
-    const size_t cnt = (size_t)c->n * c->r;
+    const int64_t n = c->n;
+    const int r = c->r, dt = c->vdtype;
+    const size_t bytes = (size_t)n * r * c->vbytes;
     const int64_t *ord = c->row_order.empty() ? nullptr : c->row_order.data();
-    const size_t rr = (size_t)c->r;
-    // element e of the device array comes from the caller's row ord[e / r] (or row e / r)
-    auto val = [&](size_t e) -> double {
-        const size_t h = ord ? (size_t)ord[e / rr] * rr + e % rr : e;
-        return host_dtype == LB_F64 ? ((const double *)V)[h] : (double)((const float *)V)[h];
-    };
-    if (c->vdtype == LB_F64) {
-        std::vector<double> t(cnt);
-        for (size_t e = 0; e < cnt; e++) t[e] = val(e);
-        return upload(c, c->V, t.data(), cnt * 8);
-    }
-    if (c->vdtype == LB_F32) {
-        std::vector<float> t(cnt);
-        for (size_t e = 0; e < cnt; e++) t[e] = (float)val(e);
-        return upload(c, c->V, t.data(), cnt * 4);
-    }
-    std::vector<uint16_t> t(cnt);
-    for (size_t e = 0; e < cnt; e++) t[e] = f2bf((float)val(e));
-    return upload(c, c->V, t.data(), cnt * 2);
+    void *st = nullptr;
+    int rc;
+    if ((rc = host_stage(c, bytes, &st))) return rc;
+    par_rows(n, [&](int64_t a, int64_t b) {
+        if (host_dtype == LB_F64) rows_to_storage(a, b, r, (const double *)V, dt, st, ord);
+        else rows_to_storage(a, b, r, (const float *)V, dt, st, ord);
+    });
+    return upload(c, c->V, st, bytes);
 }
 
 int lb_get_V(lb_ctx *c, void *V, int host_dtype) {
     if (!c || !c->n || !V) return fail(c, -1, "lb_get_V: no graph or V");
     if (host_dtype != LB_F64 && host_dtype != LB_F32) return fail(c, -1, "lb_get_V: host dtype must be LB_F64 or LB_F32");
     HIPCHK(c, hipSetDevice(c->device));
-    const size_t cnt = (size_t)c->n * c->r;
-    std::vector<unsigned char> raw(cnt * c->vbytes);
-    HIPCHK(c, hipMemcpyAsync(raw.data(), c->V.p, raw.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t n = c->n;
+    const int r = c->r, dt = c->vdtype;
+    const size_t bytes = (size_t)n * r * c->vbytes;
     const int64_t *ord = c->row_order.empty() ? nullptr : c->row_order.data();
-    const size_t rr = (size_t)c->r;
-    for (size_t e = 0; e < cnt; e++) {
-        double v;
-        if (c->vdtype == LB_F64) v = ((const double *)raw.data())[e];
-        else if (c->vdtype == LB_F32) v = ((const float *)raw.data())[e];
-        else v = bf2f(((const uint16_t *)raw.data())[e]);
-        const size_t h = ord ? (size_t)ord[e / rr] * rr + e % rr : e;
-        if (host_dtype == LB_F64) ((double *)V)[h] = v; else ((float *)V)[h] = (float)v;
-    }
+    void *st = nullptr;
+    int rc;
+    if ((rc = host_stage(c, bytes, &st))) return rc;
+    HIPCHK(c, hipMemcpyAsync(st, c->V.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    par_rows(n, [&](int64_t a, int64_t b) {
+        if (host_dtype == LB_F64) storage_to_rows(a, b, r, st, dt, (double *)V, ord);
+        else storage_to_rows(a, b, r, st, dt, (float *)V, ord);
+    });
     return 0;
 }
 
@@ -2944,23 +2986,19 @@ int lb_get_G(lb_ctx *c, void *G, int host_dtype) {
     if (!c->have_eval) return fail(c, -1, "lb_get_G: no evaluation yet");
     if (host_dtype != LB_F64 && host_dtype != LB_F32) return fail(c, -1, "lb_get_G: host dtype must be LB_F64 or LB_F32");
     HIPCHK(c, hipSetDevice(c->device));
-    const size_t cnt = (size_t)c->n * c->r;
-    const size_t hb = host_dtype == LB_F64 ? 8 : 4;
-    if (c->row_order.empty() && hb == c->cbytes) {     // same layout and type: one copy
-        HIPCHK(c, hipMemcpyAsync(G, c->G.p, cnt * hb, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        return 0;
-    }
-    std::vector<unsigned char> raw(cnt * c->cbytes);
-    HIPCHK(c, hipMemcpyAsync(raw.data(), c->G.p, raw.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t n = c->n;
+    const int r = c->r, gt = c->cbytes == 8 ? LB_F64 : LB_F32;
+    const size_t bytes = (size_t)n * r * c->cbytes;
     const int64_t *ord = c->row_order.empty() ? nullptr : c->row_order.data();
-    const size_t rr = (size_t)c->r;
-    for (size_t e = 0; e < cnt; e++) {
-        const double v = c->cbytes == 8 ? ((const double *)raw.data())[e] : (double)((const float *)raw.data())[e];
-        const size_t h = ord ? (size_t)ord[e / rr] * rr + e % rr : e;
-        if (host_dtype == LB_F64) ((double *)G)[h] = v; else ((float *)G)[h] = (float)v;
-    }
+    void *st = nullptr;
+    int rc;
+    if ((rc = host_stage(c, bytes, &st))) return rc;
+    HIPCHK(c, hipMemcpyAsync(st, c->G.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    par_rows(n, [&](int64_t a, int64_t b) {
+        if (host_dtype == LB_F64) storage_to_rows(a, b, r, st, gt, (double *)G, ord);
+        else storage_to_rows(a, b, r, st, gt, (float *)G, ord);
+    });
     return 0;
 }
 
