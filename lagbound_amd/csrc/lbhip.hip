@@ -171,6 +171,8 @@ struct VcsArgs {
     double *part;              // gridDim x (2 + sumPv R) when use_lds, else gridDim x 2
     double *Zglob;
     int zacc;                  // 1: lb_k_vcs_z also accumulates Z; 0: lb_k_vcs_acc does it in a second pass
+    const int32_t *vrow;       // entry -> row (entry-parallel passes)
+    int64_t eoff[LB_MAXVCS + 1];   // entry range of each level: [eoff[m], eoff[m + 1])
 };
 
 struct SegArgs {
@@ -417,9 +419,63 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
         }
 }
 
-// Z_R = sum over the working-set entries (i, R) of pv_iR d_i V_i, after lb_k_vcs_z wrote pv: a pass with nothing
-// but the accumulation (one FP64 copy of Z per workgroup in LDS, the entries' columns and multipliers loaded four
-// at a time ahead of the adds), written to the same per-workgroup partial rows as lb_k_vcs_z.
+// Entry-parallel form of the multipliers pass (used when the VCS cluster sums fit in LDS): one slot per working-set
+// entry (i, R) -- its row comes from vrow, so a slot's loads are the entry, then the row's V chunk, d_i and label,
+// with no serial walk over a row's entries. Same arithmetic as lb_k_vcs_z; Z is left to lb_k_vcs_acc.
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_pv(VcsArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    extern __shared__ double shY[];
+    __shared__ double red[LB_BLOCK / 64];
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    const int width = a.sumPv * R;
+    const double *Yg = a.Y + (int64_t)a.Yoff[0] * R;   // the VCS cluster sums (all VCS levels, consecutive)
+    for (int e = threadIdx.x; e < width; e += LB_BLOCK) shY[e] = Yg[e];
+    __syncthreads();
+    const TC *sig = (const TC *)a.sig;
+    TC *pv = (TC *)a.pv;
+    const int64_t ntot = a.eoff[a.nvcs];
+    double tot = 0.0, viol = 0.0;
+    for (int64_t e = (int64_t)blockIdx.x * NSL + slot; e < ntot; e += (int64_t)gridDim.x * NSL) {
+        int m = 0;
+        while (m + 1 < a.nvcs && e >= a.eoff[m + 1]) m++;
+        const int64_t i = a.vrow[e];
+        const int Rc = a.col[e];
+        const double sg = (double)sig[e];
+        TC x[CPL];
+        VT<TV>::unpack(a.V[i * S + q], x);
+        const double di = a.d[i];
+        const int Ri = a.lab[m][i];
+        const double *YR = shY + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) s += (double)x[c] * YR[c];
+        s = slot_sum<S>(s);
+        const double g = (Rc == Ri) ? di * (s - di) : di * s;
+        double p = sg;
+        if (a.rho > 0.0 && di > 0.0) {
+            const double rr = a.rho * (a.wv * a.rq[m][Rc] / (di * di));
+            p = sg - rr * g;
+            if (p < 0.0) p = 0.0;
+            if (q == 0) tot += (p * p - sg * sg) / (2 * rr);
+        }
+        if (q == 0) {
+            pv[e] = (TC)p;
+            if (g < -1e-9) viol += 1.0;
+            a.slack[e] = (p == 0.0 && g > 0.0) ? 1 : 0;   // inactive with slack (vcsops.py:285)
+        }
+    }
+    tot = block_sum(tot, red);
+    viol = block_sum(viol, red);
+    if (threadIdx.x == 0) {
+        double *out = a.part + (int64_t)blockIdx.x * (2 + width);
+        out[0] = tot; out[1] = viol;
+    }
+}
+
+// Z_R = sum over the working-set entries (i, R) of pv_iR d_i V_i, after lb_k_vcs_pv wrote pv: one slot per entry,
+// one FP64 copy of Z per workgroup in LDS, written to the same per-workgroup partial rows as the multipliers pass.
 template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_acc(VcsArgs a) {
     typedef typename VT<TV>::TC TC;
@@ -430,34 +486,20 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_acc(VcsArgs a) {
     for (int e = threadIdx.x; e < width; e += LB_BLOCK) shZ[e] = 0.0;
     __syncthreads();
     const TC *pv = (const TC *)a.pv;
-    for (int64_t i = (int64_t)blockIdx.x * NSL + slot; i < a.n; i += (int64_t)gridDim.x * NSL) {
+    const int64_t ntot = a.eoff[a.nvcs];
+    for (int64_t e = (int64_t)blockIdx.x * NSL + slot; e < ntot; e += (int64_t)gridDim.x * NSL) {
+        const double p = (double)pv[e];
+        if (p == 0.0) continue;
+        int m = 0;
+        while (m + 1 < a.nvcs && e >= a.eoff[m + 1]) m++;
+        const int64_t i = a.vrow[e];
+        const int Rc = a.col[e];
         TC x[CPL];
         VT<TV>::unpack(a.V[i * S + q], x);
         const double di = a.d[i];
-        double v[CPL];
+        double *dst = shZ + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
 #pragma unroll
-        for (int c = 0; c < CPL; c++) v[c] = di * (double)x[c];
-        for (int m = 0; m < a.nvcs; m++) {
-            const int64_t e0 = a.ptr[m][i], e1 = a.ptr[m][i + 1];
-            for (int64_t eb = e0; eb < e1; eb += 4) {
-                int Rq[4];
-                double pq[4];
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const int64_t ec = eb + t < e1 ? eb + t : e1 - 1;
-                    Rq[t] = a.col[ec];
-                    pq[t] = eb + t < e1 ? (double)pv[ec] : 0.0;
-                }
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    if (pq[t] != 0.0) {
-                        double *dst = shZ + (int64_t)(a.Zoff[m] + Rq[t]) * R + q * CPL;
-#pragma unroll
-                        for (int c = 0; c < CPL; c++) atomicAdd(dst + c, pq[t] * v[c]);
-                    }
-                }
-            }
-        }
+        for (int c = 0; c < CPL; c++) atomicAdd(dst + c, p * (di * (double)x[c]));
     }
     __syncthreads();
     double *out = a.part + (int64_t)blockIdx.x * (2 + width);
@@ -1752,7 +1794,7 @@ __global__ void lb_k_take(const int64_t *idx, int64_t cnt, const int32_t *oi, co
 // ================================================================================================== host side
 namespace {
 
-const char *kKernelNames[LB_NKERNELS] = {"lb_k_colsum", "lb_k_reduce(z,Y)", "lb_k_levels", "lb_k_vcs_z",
+const char *kKernelNames[LB_NKERNELS] = {"lb_k_colsum", "lb_k_reduce(z,Y)", "lb_k_levels", "lb_k_vcs",
                                          "lb_k_reduce(Z)", "lb_k_segs", "lb_k_rows", "lb_k_finalize"};
 
 struct DBuf {
@@ -1786,6 +1828,7 @@ struct lb_ctx {
     std::vector<int32_t> hE_idx, hT_idx;   // row -> index into hE_rows / hT_rows, or -1
     std::vector<int64_t> h_iptr;            // triangle incidence pointer (host copy)
     // device buffers
+    DBuf vrow;                        // VCS working-set entry -> its row (entry-parallel VCS passes)
     DBuf rowptr, col, w, d, labs, sig, pp, rq, dsq, vlabs, vrq, vptr, vcol, vsig, pv, TI, lam, pt, iptr, rec,
         V, G, Y, z, M, pdiag, levtot, levviol, Z, colsum_part, vcs_part, row_part, seg_part, segs, segout,
         hErows, hEseg0, hTrows, hTseg0, parts, lamsum,
@@ -2046,13 +2089,20 @@ struct Launch {
             va.Zoff[c->nvcs] = zo;
             va.col = (const int32_t *)c->vcol.p; va.sig = c->vsig.p; va.pv = c->pv.p; va.slack = (uint8_t *)c->vslack.p;
             va.Y = Yp; va.wv = c->wv; va.rho = rho;
+            va.vrow = (const int32_t *)c->vrow.p;
+            va.eoff[0] = 0;
+            for (int m = 0; m < c->nvcs; m++) va.eoff[m + 1] = va.eoff[m] + c->vnnz[m];
             va.sumPv = c->sumPv; va.use_lds = vcs_lds; va.part = (double *)c->vcs_part.p; va.Zglob = (double *)c->Z.p;
             // with the LDS path, the multipliers and the Z sums are two passes (the second has only the adds)
             va.zacc = vcs_lds ? 0 : 1;
             const size_t ldsY = sizeof(double) * (size_t)c->sumPv * R;
             if (!vcs_lds) hipLaunchKernelGGL(lb_k_fill, dim3(256), dim3(256), 0, c->stream, (double *)c->Z.p, (int64_t)c->sumPv * R, 0.0);
-            hipLaunchKernelGGL((lb_k_vcs_z<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), vcs_lds ? ldsY : 0, c->stream, va);
-            if (vcs_lds) hipLaunchKernelGGL((lb_k_vcs_acc<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), ldsY, c->stream, va);
+            if (vcs_lds) {
+                hipLaunchKernelGGL((lb_k_vcs_pv<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), ldsY, c->stream, va);
+                hipLaunchKernelGGL((lb_k_vcs_acc<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), ldsY, c->stream, va);
+            } else {
+                hipLaunchKernelGGL((lb_k_vcs_z<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), 0, c->stream, va);
+            }
         }
         mark();
         if (c->nvcs && vcs_lds) {
@@ -2575,6 +2625,7 @@ int empty_vcs(lb_ctx *c) {
     if ((rc = dalloc(c, c->vlabs, 16))) return rc;
     if ((rc = dalloc(c, c->vptr, 16))) return rc;
     if ((rc = dalloc(c, c->vcol, 16))) return rc;
+    if ((rc = dalloc(c, c->vrow, 16))) return rc;
     if ((rc = dalloc(c, c->vsig, 16))) return rc;
     if ((rc = dalloc(c, c->pv, 16))) return rc;
     if ((rc = dalloc(c, c->vrq, 16))) return rc;
@@ -2623,7 +2674,7 @@ void lb_destroy(lb_ctx *c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     DBuf *bufs[] = {&c->rowptr, &c->col, &c->w, &c->d, &c->labs, &c->sig, &c->pp, &c->rq, &c->dsq, &c->vlabs, &c->vrq,
-                    &c->vptr, &c->vcol, &c->vsig, &c->pv, &c->TI, &c->lam, &c->pt, &c->iptr, &c->rec, &c->V, &c->G,
+                    &c->vptr, &c->vcol, &c->vsig, &c->pv, &c->vrow, &c->TI, &c->lam, &c->pt, &c->iptr, &c->rec, &c->V, &c->G,
                     &c->Y, &c->z, &c->M, &c->pdiag, &c->levtot, &c->levviol, &c->Z, &c->colsum_part, &c->vcs_part,
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
                     &c->hTseg0, &c->parts, &c->lamsum, &c->order, &c->tmask, &c->Kq, &c->stZ, &c->Zs, &c->cnts,
@@ -2765,6 +2816,15 @@ int lb_set_vcs(lb_ctx *c, int nvcs, const int32_t *labs, const int32_t *P, const
     if ((rc = upload(c, c->vlabs, labs, sizeof(int32_t) * (size_t)nvcs * n))) return rc;
     if ((rc = upload(c, c->vptr, absptr.data(), sizeof(int64_t) * absptr.size()))) return rc;
     if ((rc = upload(c, c->vcol, col, sizeof(int32_t) * std::max<int64_t>(tot, 1)))) return rc;
+    {
+        std::vector<int32_t> vrow((size_t)std::max<int64_t>(tot, 1), 0);
+        for (int m = 0; m < nvcs; m++) {
+            const int64_t *pm = absptr.data() + (size_t)m * (n + 1);
+            for (int64_t i = 0; i < n; i++)
+                for (int64_t e = pm[i]; e < pm[i + 1]; e++) vrow[(size_t)e] = (int32_t)i;
+        }
+        if ((rc = upload(c, c->vrow, vrow.data(), sizeof(int32_t) * vrow.size()))) return rc;
+    }
     if ((rc = upload_tc(c, c->vsig, sig, tot))) return rc;
     if ((rc = dalloc(c, c->pv, c->cbytes * std::max<int64_t>(tot, 1)))) return rc;
     if ((rc = upload(c, c->vrq, rq, sizeof(double) * sP))) return rc;
