@@ -420,8 +420,10 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
 }
 
 // Entry-parallel form of the multipliers pass (used when the VCS cluster sums fit in LDS): one slot per working-set
-// entry (i, R) -- its row comes from vrow, so a slot's loads are the entry, then the row's V chunk, d_i and label,
-// with no serial walk over a row's entries. Same arithmetic as lb_k_vcs_z; Z is left to lb_k_vcs_acc.
+// entry (i, R) -- its row comes from vrow -- and LB_VU entries per slot in flight: every entry's loads are issued
+// before any is used (the pass is bound by dependent load latency, not by bandwidth or ALU). Same arithmetic as
+// lb_k_vcs_z; Z is left to lb_k_vcs_acc.
+#define LB_VU 4
 template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_pv(VcsArgs a) {
     typedef typename VT<TV>::TC TC;
@@ -435,35 +437,58 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_pv(VcsArgs a) {
     __syncthreads();
     const TC *sig = (const TC *)a.sig;
     TC *pv = (TC *)a.pv;
-    const int64_t ntot = a.eoff[a.nvcs];
+    const int64_t ntot = a.eoff[a.nvcs], stride = (int64_t)gridDim.x * NSL;
     double tot = 0.0, viol = 0.0;
-    for (int64_t e = (int64_t)blockIdx.x * NSL + slot; e < ntot; e += (int64_t)gridDim.x * NSL) {
-        int m = 0;
-        while (m + 1 < a.nvcs && e >= a.eoff[m + 1]) m++;
-        const int64_t i = a.vrow[e];
-        const int Rc = a.col[e];
-        const double sg = (double)sig[e];
-        TC x[CPL];
-        VT<TV>::unpack(a.V[i * S + q], x);
-        const double di = a.d[i];
-        const int Ri = a.lab[m][i];
-        const double *YR = shY + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
-        double s = 0.0;
+    for (int64_t e0 = (int64_t)blockIdx.x * NSL + slot; e0 < ntot; e0 += LB_VU * stride) {
+        int64_t ee[LB_VU], ii[LB_VU];
+        int Rq[LB_VU];
+        TC sq[LB_VU];
 #pragma unroll
-        for (int c = 0; c < CPL; c++) s += (double)x[c] * YR[c];
-        s = slot_sum<S>(s);
-        const double g = (Rc == Ri) ? di * (s - di) : di * s;
-        double p = sg;
-        if (a.rho > 0.0 && di > 0.0) {
-            const double rr = a.rho * (a.wv * a.rq[m][Rc] / (di * di));
-            p = sg - rr * g;
-            if (p < 0.0) p = 0.0;
-            if (q == 0) tot += (p * p - sg * sg) / (2 * rr);
+        for (int t = 0; t < LB_VU; t++) {
+            const int64_t e = e0 + t * stride;
+            ee[t] = e < ntot ? e : ntot - 1;           // past the end: a valid entry, loaded and not used
+            ii[t] = a.vrow[ee[t]];
+            Rq[t] = a.col[ee[t]];
+            sq[t] = sig[ee[t]];
         }
-        if (q == 0) {
-            pv[e] = (TC)p;
-            if (g < -1e-9) viol += 1.0;
-            a.slack[e] = (p == 0.0 && g > 0.0) ? 1 : 0;   // inactive with slack (vcsops.py:285)
+        TC xq[LB_VU][CPL];
+        double dq[LB_VU];
+        int mq[LB_VU], Rl[LB_VU];
+        double rqq[LB_VU];
+#pragma unroll
+        for (int t = 0; t < LB_VU; t++) {
+            int m = 0;
+            while (m + 1 < a.nvcs && ee[t] >= a.eoff[m + 1]) m++;
+            mq[t] = m;
+            VT<TV>::unpack(a.V[ii[t] * S + q], xq[t]);
+            dq[t] = a.d[ii[t]];
+            Rl[t] = a.lab[m][ii[t]];
+            rqq[t] = a.rq[m][Rq[t]];
+        }
+#pragma unroll
+        for (int t = 0; t < LB_VU; t++) {
+            if (e0 + t * stride >= ntot) break;          // slot-uniform
+            const int Rc = Rq[t];
+            const double di = dq[t];
+            const double *YR = shY + (int64_t)(a.Zoff[mq[t]] + Rc) * R + q * CPL;
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) s += (double)xq[t][c] * YR[c];
+            s = slot_sum<S>(s);
+            const double g = (Rc == Rl[t]) ? di * (s - di) : di * s;
+            const double sg = (double)sq[t];
+            double p = sg;
+            if (a.rho > 0.0 && di > 0.0) {
+                const double rr = a.rho * (a.wv * rqq[t] / (di * di));
+                p = sg - rr * g;
+                if (p < 0.0) p = 0.0;
+                if (q == 0) tot += (p * p - sg * sg) / (2 * rr);
+            }
+            if (q == 0) {
+                pv[ee[t]] = (TC)p;
+                if (g < -1e-9) viol += 1.0;
+                a.slack[ee[t]] = (p == 0.0 && g > 0.0) ? 1 : 0;   // inactive with slack (vcsops.py:285)
+            }
         }
     }
     tot = block_sum(tot, red);
@@ -475,7 +500,8 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_pv(VcsArgs a) {
 }
 
 // Z_R = sum over the working-set entries (i, R) of pv_iR d_i V_i, after lb_k_vcs_pv wrote pv: one slot per entry,
-// one FP64 copy of Z per workgroup in LDS, written to the same per-workgroup partial rows as the multipliers pass.
+// LB_VU entries in flight per slot, one FP64 copy of Z per workgroup in LDS, written to the same per-workgroup
+// partial rows as the multipliers pass.
 template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_acc(VcsArgs a) {
     typedef typename VT<TV>::TC TC;
@@ -486,20 +512,35 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_acc(VcsArgs a) {
     for (int e = threadIdx.x; e < width; e += LB_BLOCK) shZ[e] = 0.0;
     __syncthreads();
     const TC *pv = (const TC *)a.pv;
-    const int64_t ntot = a.eoff[a.nvcs];
-    for (int64_t e = (int64_t)blockIdx.x * NSL + slot; e < ntot; e += (int64_t)gridDim.x * NSL) {
-        const double p = (double)pv[e];
-        if (p == 0.0) continue;
-        int m = 0;
-        while (m + 1 < a.nvcs && e >= a.eoff[m + 1]) m++;
-        const int64_t i = a.vrow[e];
-        const int Rc = a.col[e];
-        TC x[CPL];
-        VT<TV>::unpack(a.V[i * S + q], x);
-        const double di = a.d[i];
-        double *dst = shZ + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+    const int64_t ntot = a.eoff[a.nvcs], stride = (int64_t)gridDim.x * NSL;
+    for (int64_t e0 = (int64_t)blockIdx.x * NSL + slot; e0 < ntot; e0 += LB_VU * stride) {
+        int64_t ee[LB_VU], ii[LB_VU];
+        int Rq[LB_VU];
+        double pq[LB_VU];
 #pragma unroll
-        for (int c = 0; c < CPL; c++) atomicAdd(dst + c, p * (di * (double)x[c]));
+        for (int t = 0; t < LB_VU; t++) {
+            const int64_t e = e0 + t * stride;
+            ee[t] = e < ntot ? e : ntot - 1;
+            ii[t] = a.vrow[ee[t]];
+            Rq[t] = a.col[ee[t]];
+            pq[t] = e < ntot ? (double)pv[ee[t]] : 0.0;
+        }
+        TC xq[LB_VU][CPL];
+        double dq[LB_VU];
+#pragma unroll
+        for (int t = 0; t < LB_VU; t++) {
+            VT<TV>::unpack(a.V[ii[t] * S + q], xq[t]);
+            dq[t] = a.d[ii[t]];
+        }
+#pragma unroll
+        for (int t = 0; t < LB_VU; t++) {
+            if (pq[t] == 0.0) continue;
+            int m = 0;
+            while (m + 1 < a.nvcs && ee[t] >= a.eoff[m + 1]) m++;
+            double *dst = shZ + (int64_t)(a.Zoff[m] + Rq[t]) * R + q * CPL;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) atomicAdd(dst + c, pq[t] * (dq[t] * (double)xq[t][c]));
+        }
     }
     __syncthreads();
     double *out = a.part + (int64_t)blockIdx.x * (2 + width);
