@@ -424,6 +424,9 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
 // before any is used (the pass is bound by dependent load latency, not by bandwidth or ALU). Same arithmetic as
 // lb_k_vcs_z; Z is left to lb_k_vcs_acc.
 #define LB_VU 4
+// LDS row stride (doubles) of the per-cluster rows in the VCS passes: two doubles of padding shift consecutive
+// cluster rows by 16 B, so slots working on different clusters spread over the banks
+#define LB_VPAD 2
 template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_pv(VcsArgs a) {
     typedef typename VT<TV>::TC TC;
@@ -433,7 +436,8 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_pv(VcsArgs a) {
     const int slot = threadIdx.x / S, q = threadIdx.x % S;
     const int width = a.sumPv * R;
     const double *Yg = a.Y + (int64_t)a.Yoff[0] * R;   // the VCS cluster sums (all VCS levels, consecutive)
-    for (int e = threadIdx.x; e < width; e += LB_BLOCK) shY[e] = Yg[e];
+    constexpr int RS = R + LB_VPAD;
+    for (int e = threadIdx.x; e < width; e += LB_BLOCK) shY[(e / R) * RS + e % R] = Yg[e];
     __syncthreads();
     const TC *sig = (const TC *)a.sig;
     TC *pv = (TC *)a.pv;
@@ -470,7 +474,7 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_pv(VcsArgs a) {
             if (e0 + t * stride >= ntot) break;          // slot-uniform
             const int Rc = Rq[t];
             const double di = dq[t];
-            const double *YR = shY + (int64_t)(a.Zoff[mq[t]] + Rc) * R + q * CPL;
+            const double *YR = shY + (int64_t)(a.Zoff[mq[t]] + Rc) * RS + q * CPL;
             double s = 0.0;
 #pragma unroll
             for (int c = 0; c < CPL; c++) s += (double)xq[t][c] * YR[c];
@@ -509,7 +513,8 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_acc(VcsArgs a) {
     extern __shared__ double shZ[];
     const int slot = threadIdx.x / S, q = threadIdx.x % S;
     const int width = a.sumPv * R;
-    for (int e = threadIdx.x; e < width; e += LB_BLOCK) shZ[e] = 0.0;
+    constexpr int RS = R + LB_VPAD;
+    for (int e = threadIdx.x; e < a.sumPv * RS; e += LB_BLOCK) shZ[e] = 0.0;
     __syncthreads();
     const TC *pv = (const TC *)a.pv;
     const int64_t ntot = a.eoff[a.nvcs], stride = (int64_t)gridDim.x * NSL;
@@ -537,14 +542,14 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_acc(VcsArgs a) {
             if (pq[t] == 0.0) continue;
             int m = 0;
             while (m + 1 < a.nvcs && ee[t] >= a.eoff[m + 1]) m++;
-            double *dst = shZ + (int64_t)(a.Zoff[m] + Rq[t]) * R + q * CPL;
+            double *dst = shZ + (int64_t)(a.Zoff[m] + Rq[t]) * RS + q * CPL;
 #pragma unroll
             for (int c = 0; c < CPL; c++) atomicAdd(dst + c, pq[t] * (dq[t] * (double)xq[t][c]));
         }
     }
     __syncthreads();
     double *out = a.part + (int64_t)blockIdx.x * (2 + width);
-    for (int e = threadIdx.x; e < width; e += LB_BLOCK) out[2 + e] = shZ[e];
+    for (int e = threadIdx.x; e < width; e += LB_BLOCK) out[2 + e] = shZ[(e / R) * RS + e % R];
 }
 
 // ------------------------------------------------------------------------------------------------ triangle record
@@ -2139,8 +2144,9 @@ struct Launch {
             const size_t ldsY = sizeof(double) * (size_t)c->sumPv * R;
             if (!vcs_lds) hipLaunchKernelGGL(lb_k_fill, dim3(256), dim3(256), 0, c->stream, (double *)c->Z.p, (int64_t)c->sumPv * R, 0.0);
             if (vcs_lds) {
-                hipLaunchKernelGGL((lb_k_vcs_pv<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), ldsY, c->stream, va);
-                hipLaunchKernelGGL((lb_k_vcs_acc<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), ldsY, c->stream, va);
+                const size_t ldsYp = sizeof(double) * (size_t)c->sumPv * (R + LB_VPAD);
+                hipLaunchKernelGGL((lb_k_vcs_pv<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), ldsYp, c->stream, va);
+                hipLaunchKernelGGL((lb_k_vcs_acc<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), ldsYp, c->stream, va);
             } else {
                 hipLaunchKernelGGL((lb_k_vcs_z<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), 0, c->stream, va);
             }
