@@ -461,11 +461,11 @@ def _merge_ws(vl, keep, ni, nR):
     r = np.r_[rows_old[ko], np.asarray(ni, np.int64)]
     c = np.r_[vl.col[ko].astype(np.int64), np.asarray(nR, np.int64)]
     src = np.r_[ko, np.full(len(ni), -1)]
-    o = np.lexsort((c, r))
-    r, c, src = r[o], c[o], src[o]
+    o = np.argsort(r * np.int64(vl.P) + c, kind="stable")     # by (i, R): one integer key
+    c, src = c[o], src[o]
     ptr = np.zeros(n + 1, np.int64)
-    np.add.at(ptr, r + 1, 1)
-    return np.cumsum(ptr), c.astype(np.int32), src
+    np.cumsum(np.bincount(r, minlength=n), out=ptr[1:])
+    return ptr, c.astype(np.int32), src
 
 
 def load_instance(D):

@@ -252,6 +252,8 @@ class Relabeled:
         if self.vo is None or len(i) == 0:
             return i, R, sc, st
         i = self.vo.ids_out(i)
+        if not np.any(sc[1:] == sc[:-1]):            # no tied scores: the order (score descending) is final
+            return i, R, sc, st
         o = np.lexsort((R, i, -sc))
         return i[o], R[o], sc[o], st
 
