@@ -150,6 +150,21 @@ int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int
  * row kernel in descending order of their work instead of index order (same results up to summation order). */
 int lb_set_option(lb_ctx *ctx, const char *name, int64_t value);
 
+/* Device row work of the candidate bound's Riemannian gradient polish (polish_rgd, est_dual_vcs.py:276-308), with
+ * the step rule kept by the caller. The context holds the instance and its multipliers; the iterate W (n x r, unit
+ * rows, caller row order) lives on the device in FP64, and each evaluation reads its storage-type copy at rho = 0.
+ *   lb_polish_begin(W): out[0] = sum_i g_i (g_i = (C W)_i . W_i), out[1] = sum Rr . R with Rr = C W - g W and
+ *     R = Rr / max(|(C W)_i|, 1e-12);
+ *   lb_polish_trial(alpha): candidate Wn = normalise(W + alpha R), evaluated; *gsum = sum_i g_i of Wn;
+ *   lb_polish_accept: W <- Wn (the last candidate), new R; out4 = (sum Rr . R, sy, ss, yy), the Barzilai-Borwein
+ *     sums sum S.(Y nr), S.(S nr), Y.(Y nr) with S = Wn - W, Y = Rn - R, nr the new row norms of C Wn;
+ *   lb_polish_get: W and g of the current iterate (either pointer may be NULL).
+ * A trial overwrites the context's V. No reference counterpart beyond the polish itself (its loop is numpy). */
+int lb_polish_begin(lb_ctx *ctx, const double *W, double *out);
+int lb_polish_trial(lb_ctx *ctx, double alpha, double *gsum);
+int lb_polish_accept(lb_ctx *ctx, double *out4);
+int lb_polish_get(lb_ctx *ctx, double *W, double *g);
+
 /* Library version string. */
 const char *lb_version(void);
 

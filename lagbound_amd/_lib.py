@@ -14,7 +14,8 @@ LB_NKERNELS = 8
 
 # every symbol include/lbhip.h declares
 EXPORTS = ["lb_create", "lb_destroy", "lb_last_error", "lb_set_graph", "lb_set_levels", "lb_set_level_sig",
-           "lb_set_vcs", "lb_set_vcs_sig", "lb_set_triangles", "lb_set_lam", "lb_set_V", "lb_get_V", "lb_set_row_order",
+           "lb_set_vcs", "lb_set_vcs_sig", "lb_set_triangles", "lb_set_lam", "lb_set_V", "lb_get_V", "lb_set_row_order", "lb_polish_begin", "lb_polish_trial",
+           "lb_polish_accept", "lb_polish_get",
            "lb_eval_fgrad", "lb_get_G", "lb_get_multipliers", "lb_phr_update", "lb_sweeps", "lb_separate_triangles", "lb_separate_vcs", "lb_set_vcs_age", "lb_get_vcs_state",
            "lb_set_weights", "lb_sync", "lb_time_eval",
            "lb_kernel_name", "lb_set_tuning", "lb_version", "lb_reduce_exact", "lb_set_option", "lb_certify_dense", "lb_batch_set",
@@ -58,6 +59,10 @@ def load():
         "lb_eval_fgrad": ([ctx, C.c_double, C.c_double, _f64p], C.c_int),
         "lb_get_G": ([ctx, _vp, C.c_int], C.c_int),
         "lb_set_row_order": ([ctx, _i64p], C.c_int),
+        "lb_polish_begin": ([ctx, _f64p, _f64p], C.c_int),
+        "lb_polish_trial": ([ctx, C.c_double, _f64p], C.c_int),
+        "lb_polish_accept": ([ctx, _f64p], C.c_int),
+        "lb_polish_get": ([ctx, _f64p, _f64p], C.c_int),
         "lb_get_multipliers": ([ctx, _f64p, _f64p, _f64p], C.c_int),
         "lb_phr_update": ([ctx], C.c_int),
         "lb_sweeps": ([ctx, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _i64p, C.c_int], C.c_int),

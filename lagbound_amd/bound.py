@@ -153,6 +153,51 @@ def polish_rgd(lag, V, iters=400, polish_time=600.0, ptol=2e-5, log=print):
     return V, f, g
 
 
+def polish_rgd_device(lag, V, iters=400, polish_time=600.0, ptol=2e-5, log=print):
+    """polish_rgd with the row work on the device (lb_polish_*: step and normalise, g and its sum, new direction and
+    the Barzilai-Borwein sums on an FP64 copy of the iterate); the step rule, the backtracking and the
+    Zhang-Hager reference value are the same as polish_rgd's, on the scalars read back."""
+    eng = lag.eng
+    L = lag.inst.L
+    cst = lag.const
+    tp = time.time()
+    gs, grs = eng.polish_begin(V)
+    f = (cst + gs) / L
+    Cref, Q, alpha = f, 1.0, 2.0
+    last_chk = f
+    it = 0
+    while it < iters and time.time() - tp < polish_time:
+        it += 1
+        gr = grs / L
+        for bt in range(12):
+            fn = (cst + eng.polish_trial(alpha)) / L
+            if fn >= Cref + 1e-4 * alpha * gr:
+                break
+            alpha *= 0.5
+        grs, sy, ss, yy = eng.polish_accept()
+        if sy < 0:
+            alpha = (ss / (-sy)) if it % 2 else (-sy / yy)
+        else:
+            alpha = min(2 * alpha, 50.0)
+        alpha = float(np.clip(alpha, 0.05, 50.0))
+        f = fn
+        Qn = 0.85 * Q + 1
+        Cref = (0.85 * Q * Cref + f) / Qn
+        Q = Qn
+        if it % 10 == 0:
+            log(f"rgd it {it} dv {f:.6f} alpha {alpha:.3g} bt {bt}")
+            if f - last_chk < ptol * 2:
+                break
+            last_chk = f
+    V, g = eng.polish_get()
+    return V, f, g
+
+
+def _device_polish_ok(lag, W):
+    eng = getattr(lag, "eng", None)
+    return eng is not None and hasattr(eng, "polish_begin") and W.shape[1] == getattr(lag, "r", -1)
+
+
 SCALES = {"ones": lambda d, L, n: np.ones(n), "vol": lambda d, L, n: np.maximum(d, 1.0),
           "sqrtvol": lambda d, L, n: np.sqrt(np.maximum(d, 1.0)) * math.sqrt(L / n)}
 
@@ -209,14 +254,15 @@ def pencil_lmin(lag, g, s, b, maxiter=400, chunk=20, tol=1e-9, atol=1e-5, eig_ti
 def estimate(inst, V, lag=None, polish="rgd", rgd_iters=400, polish_time=600.0, ptol=2e-5, scale="vol",
              eig_extra=16, eig_maxiter=400, eig_tol=1e-9, eig_chunk=20, eig_atol=1e-5, eig_time=1200.0, escape=2,
              escape_min=0.002, escape_p=4, escape_ratio=0.3, seed=0, no_w_init=False, device=0, dtype="f64",
-             eig_block=None, log=print):
+             eig_block=None, device_polish=True, log=print):
     """Candidate bound at the instance's multipliers (levels' sig, lam, VCS sig): polish, then per scale s the
     pencil eigenvalue, ub = dv + max(0, -(theta_1 - r_1)) sum(s) / L, then up to `escape` rank-escape rounds
     (append the isolated bottom eigenvectors to W, re-polish, re-estimate; the best round is kept) -- est_dual_vcs.py
     :316-389. Returns the reference's result record (dv0, dv, s_<scale> records, escape_rounds, ub, ...) plus the
     best round's W, g and s (`W`, `g`, `s` keys, for certification). `lag` (anything with .inst/.const/.CX) defaults
     to the device operator. `eig_block` overrides the LOBPCG block width k + eig_extra (a narrower block costs
-    less host work per iteration). ub is an estimate, not a proof."""
+    less host work per iteration). With the device operator and W as wide as its rank, the polish runs its row work
+    on the device (polish_rgd_device; device_polish=False keeps it on the host). ub is an estimate, not a proof."""
     own = lag is None
     if own:
         lag = DeviceLagrangian(inst, r=V.shape[1], dtype=dtype, device=device)
@@ -228,7 +274,8 @@ def estimate(inst, V, lag=None, polish="rgd", rgd_iters=400, polish_time=600.0, 
     def do_polish(W):
         for meth in polish.split("+"):
             if meth == "rgd":
-                W = polish_rgd(lag, W, iters=rgd_iters, polish_time=polish_time, ptol=ptol, log=log)[0]
+                pol = polish_rgd_device if device_polish and _device_polish_ok(lag, W) else polish_rgd
+                W = pol(lag, W, iters=rgd_iters, polish_time=polish_time, ptol=ptol, log=log)[0]
             elif meth not in ("none", ""):
                 raise ValueError(f"polish method {meth!r} is not supported (rgd | none)")
         return W

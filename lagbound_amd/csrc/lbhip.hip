@@ -1849,6 +1849,98 @@ struct DBuf {
 
 }  // namespace
 
+// ------------------------------------------------------------------------------------------------ device RGD polish
+// Row work of the estimate's Riemannian gradient polish (polish_rgd, est_dual_vcs.py:276-308) on an FP64 master copy
+// of the iterate: the step-and-normalise, the row values g_i = (C W)_i . W_i with their sum, and the new direction
+// R = (C W - g W) / max(|C W|, 1e-12) with the Barzilai-Borwein sums. Two FP64 columns per lane (S = R / 2 lanes per
+// row); products and sums written without contraction so each rounds as the host form does.
+struct PolArgs {
+    int64_t n; double alpha; int init;
+    const double *V;           // current iterate (FP64 master, n x r)
+    double *Rd;                // current direction (n x r), replaced by the new one in lb_k_pol_post
+    double *Vn;                // candidate iterate (FP64)
+    void *Vst;                 // the context's V storage (the evaluation reads it)
+    const void *G;             // C(y) Vn from the evaluation (TC)
+    double *g;                 // row values of the candidate
+    double *part;              // per-block partial sums
+};
+
+template <typename TV> __device__ inline void pol_store(void *Vst, int64_t e, double x);
+template <> __device__ inline void pol_store<float>(void *Vst, int64_t e, double x) { ((float *)Vst)[e] = (float)x; }
+template <> __device__ inline void pol_store<bf16raw>(void *Vst, int64_t e, double x) { ((uint16_t *)Vst)[e] = lb_f2bf((float)x); }
+template <> __device__ inline void pol_store<double>(void *Vst, int64_t e, double x) { ((double *)Vst)[e] = x; }
+
+// Vn = (V + alpha R) / |V + alpha R| row by row, and its storage copy
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_pol_step(PolArgs a) {
+    constexpr int S = R / 2, NSL = LB_BLOCK / S;
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    for (int64_t i = (int64_t)blockIdx.x * NSL + slot; i < a.n; i += (int64_t)gridDim.x * NSL) {
+        const int64_t e = i * R + 2 * q;
+        const double x0 = __dadd_rn(a.V[e], __dmul_rn(a.alpha, a.Rd[e]));
+        const double x1 = __dadd_rn(a.V[e + 1], __dmul_rn(a.alpha, a.Rd[e + 1]));
+        const double nrm = sqrt(slot_sum<S>(__dadd_rn(__dmul_rn(x0, x0), __dmul_rn(x1, x1))));
+        const double y0 = x0 / nrm, y1 = x1 / nrm;
+        a.Vn[e] = y0; a.Vn[e + 1] = y1;
+        pol_store<TV>(a.Vst, e, y0);
+        pol_store<TV>(a.Vst, e + 1, y1);
+    }
+}
+
+// g_i = G_i . Vn_i (FP64) and the per-block sum of the g_i
+template <typename TC, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_pol_gsum(PolArgs a) {
+    constexpr int S = R / 2, NSL = LB_BLOCK / S;
+    __shared__ double red[LB_BLOCK / 64];
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    const TC *G = (const TC *)a.G;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * NSL + slot; i < a.n; i += (int64_t)gridDim.x * NSL) {
+        const int64_t e = i * R + 2 * q;
+        const double gi = slot_sum<S>(__dadd_rn(__dmul_rn((double)G[e], a.Vn[e]), __dmul_rn((double)G[e + 1], a.Vn[e + 1])));
+        if (q == 0) { a.g[i] = gi; acc += gi; }
+    }
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) a.part[blockIdx.x] = acc;
+}
+
+// New direction from G at Vn: nr = max(|G_i|, 1e-12), Rr = G - g Vn, Rn = Rr / nr; per-block sums of
+// Rr . Rn and (unless init) of S.(Y nr), S.(S nr), Y.(Y nr) with S = Vn - V, Y = Rn - R; Rd <- Rn.
+template <typename TC, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_pol_post(PolArgs a) {
+    constexpr int S = R / 2, NSL = LB_BLOCK / S;
+    __shared__ double red[LB_BLOCK / 64];
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    const TC *G = (const TC *)a.G;
+    double gr = 0.0, sy = 0.0, ss = 0.0, yy = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * NSL + slot; i < a.n; i += (int64_t)gridDim.x * NSL) {
+        const int64_t e = i * R + 2 * q;
+        const double c0 = (double)G[e], c1 = (double)G[e + 1];
+        const double nr = fmax(sqrt(slot_sum<S>(__dadd_rn(__dmul_rn(c0, c0), __dmul_rn(c1, c1)))), 1e-12);
+        const double gi = a.g[i];
+        const double v0 = a.Vn[e], v1 = a.Vn[e + 1];
+        const double r0 = __dsub_rn(c0, __dmul_rn(gi, v0)), r1 = __dsub_rn(c1, __dmul_rn(gi, v1));
+        const double n0 = r0 / nr, n1 = r1 / nr;
+        gr += __dadd_rn(__dmul_rn(r0, n0), __dmul_rn(r1, n1));
+        if (!a.init) {
+            const double s0 = __dsub_rn(v0, a.V[e]), s1 = __dsub_rn(v1, a.V[e + 1]);
+            const double y0 = __dsub_rn(n0, a.Rd[e]), y1 = __dsub_rn(n1, a.Rd[e + 1]);
+            sy += __dadd_rn(__dmul_rn(s0, __dmul_rn(y0, nr)), __dmul_rn(s1, __dmul_rn(y1, nr)));
+            ss += __dadd_rn(__dmul_rn(s0, __dmul_rn(s0, nr)), __dmul_rn(s1, __dmul_rn(s1, nr)));
+            yy += __dadd_rn(__dmul_rn(y0, __dmul_rn(y0, nr)), __dmul_rn(y1, __dmul_rn(y1, nr)));
+        }
+        a.Rd[e] = n0; a.Rd[e + 1] = n1;
+    }
+    gr = block_sum(gr, red);
+    sy = block_sum(sy, red);
+    ss = block_sum(ss, red);
+    yy = block_sum(yy, red);
+    if (threadIdx.x == 0) {
+        double *o = a.part + (int64_t)blockIdx.x * 4;
+        o[0] = gr; o[1] = sy; o[2] = ss; o[3] = yy;
+    }
+}
+
 struct lb_ctx {
     int device = 0, vdtype = LB_F32, r = 16;
     hipStream_t stream = nullptr;
@@ -1875,6 +1967,8 @@ struct lb_ctx {
     std::vector<int64_t> h_iptr;            // triangle incidence pointer (host copy)
     // device buffers
     DBuf vrow;                        // VCS working-set entry -> its row (entry-parallel VCS passes)
+    DBuf pol_V, pol_Vn, pol_R, pol_g, pol_part, pol_sum;   // device RGD polish state (n x r FP64, n, partials)
+    bool pol_ok = false;
     DBuf rowptr, col, w, d, labs, sig, pp, rq, dsq, vlabs, vrq, vptr, vcol, vsig, pv, TI, lam, pt, iptr, rec,
         V, G, Y, z, M, pdiag, levtot, levviol, Z, colsum_part, vcs_part, row_part, seg_part, segs, segout,
         hErows, hEseg0, hTrows, hTseg0, parts, lamsum,
@@ -2721,7 +2815,8 @@ void lb_destroy(lb_ctx *c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     DBuf *bufs[] = {&c->rowptr, &c->col, &c->w, &c->d, &c->labs, &c->sig, &c->pp, &c->rq, &c->dsq, &c->vlabs, &c->vrq,
-                    &c->vptr, &c->vcol, &c->vsig, &c->pv, &c->vrow, &c->TI, &c->lam, &c->pt, &c->iptr, &c->rec, &c->V, &c->G,
+                    &c->vptr, &c->vcol, &c->vsig, &c->pv, &c->vrow, &c->pol_V, &c->pol_Vn, &c->pol_R, &c->pol_g,
+                    &c->pol_part, &c->pol_sum, &c->TI, &c->lam, &c->pt, &c->iptr, &c->rec, &c->V, &c->G,
                     &c->Y, &c->z, &c->M, &c->pdiag, &c->levtot, &c->levviol, &c->Z, &c->colsum_part, &c->vcs_part,
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
                     &c->hTseg0, &c->parts, &c->lamsum, &c->order, &c->tmask, &c->Kq, &c->stZ, &c->Zs, &c->cnts,
@@ -4299,6 +4394,133 @@ int lb_replay_eval(lb_ctx *c, int count, double *parts) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     c->have_eval = true; c->last_rho = c->g_rho;
+    return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------ device RGD polish
+#define LB_POL_BLOCKS 1024
+
+template <typename TV, int R>
+int pol_launch(lb_ctx *c, int which, double alpha) {
+    typedef typename VT<TV>::TC TC;
+    PolArgs a{};
+    a.n = c->n; a.alpha = alpha; a.init = which == 3;
+    a.V = (const double *)c->pol_V.p; a.Rd = (double *)c->pol_R.p;
+    a.Vn = which == 3 ? (double *)c->pol_V.p : (double *)c->pol_Vn.p;
+    a.Vst = c->V.p; a.G = c->G.p; a.g = (double *)c->pol_g.p; a.part = (double *)c->pol_part.p;
+    if (which == 0) hipLaunchKernelGGL((lb_k_pol_step<TV, R>), dim3(LB_POL_BLOCKS), dim3(LB_BLOCK), 0, c->stream, a);
+    else if (which == 1) hipLaunchKernelGGL((lb_k_pol_gsum<TC, R>), dim3(LB_POL_BLOCKS), dim3(LB_BLOCK), 0, c->stream, a);
+    else hipLaunchKernelGGL((lb_k_pol_post<TC, R>), dim3(LB_POL_BLOCKS), dim3(LB_BLOCK), 0, c->stream, a);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+// which: 0 step, 1 g and its sum, 2 post, 3 post at the current iterate (init)
+int pol_kernels(lb_ctx *c, int which, double alpha) {
+    const int v = c->vdtype, r = c->r;
+    if (v == LB_F32) return r == 8 ? pol_launch<float, 8>(c, which, alpha) : r == 16 ? pol_launch<float, 16>(c, which, alpha) : pol_launch<float, 32>(c, which, alpha);
+    if (v == LB_BF16) return r == 8 ? pol_launch<bf16raw, 8>(c, which, alpha) : r == 16 ? pol_launch<bf16raw, 16>(c, which, alpha) : pol_launch<bf16raw, 32>(c, which, alpha);
+    return r == 8 ? pol_launch<double, 8>(c, which, alpha) : r == 16 ? pol_launch<double, 16>(c, which, alpha) : pol_launch<double, 32>(c, which, alpha);
+}
+
+// evaluation at rho = 0 of the storage copy, then g and sum(g) -> *gsum
+int pol_value(lb_ctx *c, double *gsum) {
+    eval_fn f = pick_eval(c->vdtype, c->r);
+    if (!f) return fail(c, -1, "polish: unsupported rank / dtype");
+    int rc;
+    if ((rc = f(c, 0.0, 0.0, nullptr))) return rc;
+    c->have_eval = true; c->last_rho = 0.0;
+    if ((rc = pol_kernels(c, 1, 0.0))) return rc;
+    hipLaunchKernelGGL(lb_k_reduce, dim3(1), dim3(1024), 0, c->stream, (const double *)c->pol_part.p, LB_POL_BLOCKS, 1, 1,
+                       (double *)c->pol_sum.p);
+    HIPCHK(c, hipMemcpyAsync(gsum, c->pol_sum.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// post pass (init or accept): direction update and its four sums -> out4
+int pol_post(lb_ctx *c, int init, double *out4) {
+    int rc;
+    if ((rc = pol_kernels(c, init ? 3 : 2, 0.0))) return rc;
+    hipLaunchKernelGGL(lb_k_reduce, dim3(1), dim3(1024), 0, c->stream, (const double *)c->pol_part.p, LB_POL_BLOCKS, 4, 4,
+                       (double *)c->pol_sum.p);
+    HIPCHK(c, hipMemcpyAsync(out4, c->pol_sum.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" {
+
+int lb_polish_begin(lb_ctx *c, const double *V, double *out) {
+    drop_capture(c);
+    if (!c || !c->n || !V || !out) return fail(c, -1, "lb_polish_begin: no graph, V or output");
+    if (c->bG) return fail(c, -1, "lb_polish_begin: not available on a batch context");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = c->n;
+    const int r = c->r;
+    const size_t bytes = sizeof(double) * (size_t)n * r;
+    int rc;
+    if ((rc = dalloc(c, c->pol_V, bytes)) || (rc = dalloc(c, c->pol_Vn, bytes)) || (rc = dalloc(c, c->pol_R, bytes)) ||
+        (rc = dalloc(c, c->pol_g, sizeof(double) * (size_t)n)) ||
+        (rc = dalloc(c, c->pol_part, sizeof(double) * 4 * LB_POL_BLOCKS)) || (rc = dalloc(c, c->pol_sum, sizeof(double) * 4)))
+        return rc;
+    // the iterate in device row order (FP64), then its storage copy through lb_set_V's path
+    const int64_t *ord = c->row_order.empty() ? nullptr : c->row_order.data();
+    void *st = nullptr;
+    if ((rc = host_stage(c, bytes, &st))) return rc;
+    par_rows(n, [&](int64_t a, int64_t b) { rows_to_storage(a, b, r, V, LB_F64, st, ord); });
+    HIPCHK(c, hipMemcpyAsync(c->pol_V.p, st, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = lb_set_V(c, V, LB_F64))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->pol_Vn.p, c->pol_V.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+    double gs = 0.0, s4[4];
+    if ((rc = pol_value(c, &gs))) return rc;
+    if ((rc = pol_post(c, 1, s4))) return rc;
+    out[0] = gs; out[1] = s4[0];
+    c->pol_ok = true;
+    return 0;
+}
+
+int lb_polish_trial(lb_ctx *c, double alpha, double *gsum) {
+    if (!c || !c->pol_ok || !gsum) return fail(c, -1, "lb_polish_trial: no polish state (lb_polish_begin)");
+    drop_capture(c);
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = pol_kernels(c, 0, alpha))) return rc;
+    return pol_value(c, gsum);
+}
+
+int lb_polish_accept(lb_ctx *c, double *out4) {
+    if (!c || !c->pol_ok || !out4) return fail(c, -1, "lb_polish_accept: no polish state (lb_polish_begin)");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = pol_post(c, 0, out4))) return rc;
+    std::swap(c->pol_V, c->pol_Vn);                 // the candidate becomes the iterate
+    return 0;
+}
+
+int lb_polish_get(lb_ctx *c, double *V, double *g) {
+    if (!c || !c->pol_ok) return fail(c, -1, "lb_polish_get: no polish state (lb_polish_begin)");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = c->n;
+    const int r = c->r;
+    const int64_t *ord = c->row_order.empty() ? nullptr : c->row_order.data();
+    int rc;
+    void *st = nullptr;
+    if ((rc = host_stage(c, sizeof(double) * (size_t)n * r, &st))) return rc;
+    if (V) {
+        HIPCHK(c, hipMemcpyAsync(st, c->pol_V.p, sizeof(double) * (size_t)n * r, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        par_rows(n, [&](int64_t a, int64_t b) { storage_to_rows(a, b, r, st, LB_F64, V, ord); });
+    }
+    if (g) {
+        HIPCHK(c, hipMemcpyAsync(st, c->pol_g.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const double *gs = (const double *)st;
+        for (int64_t k = 0; k < n; k++) g[ord ? ord[k] : k] = gs[k];
+    }
     return 0;
 }
 

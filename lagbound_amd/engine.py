@@ -158,6 +158,34 @@ class DeviceEngine:
         order = np.ascontiguousarray(order, np.int64)
         self._chk(self.L.lb_set_row_order(self.ctx, ptr(order, _i64p)), "lb_set_row_order")
 
+    # ------------------------------------------------------------------------------------------------ RGD polish
+    def polish_begin(self, V):
+        """Start a device polish at V (caller order, n x r): returns (sum_i g_i, sum Rr.R) of the start."""
+        V = np.ascontiguousarray(V, np.float64)
+        assert V.shape == (self.inst.n, self.r), (V.shape, self.inst.n, self.r)
+        out = np.zeros(2)
+        self._chk(self.L.lb_polish_begin(self.ctx, ptr(V, _f64p), ptr(out, _f64p)), "lb_polish_begin")
+        return float(out[0]), float(out[1])
+
+    def polish_trial(self, alpha: float) -> float:
+        """Evaluate the candidate normalise(V + alpha R): returns its sum_i g_i."""
+        out = np.zeros(1)
+        self._chk(self.L.lb_polish_trial(self.ctx, float(alpha), ptr(out, _f64p)), "lb_polish_trial")
+        return float(out[0])
+
+    def polish_accept(self):
+        """Take the last candidate: returns (sum Rr.R, sy, ss, yy) of the new direction."""
+        out = np.zeros(4)
+        self._chk(self.L.lb_polish_accept(self.ctx, ptr(out, _f64p)), "lb_polish_accept")
+        return tuple(float(x) for x in out)
+
+    def polish_get(self):
+        """(V, g) of the current iterate, caller order."""
+        V = np.empty((self.inst.n, self.r), np.float64)
+        g = np.empty(self.inst.n, np.float64)
+        self._chk(self.L.lb_polish_get(self.ctx, ptr(V, _f64p), ptr(g, _f64p)), "lb_polish_get")
+        return V, g
+
     def get_V(self) -> np.ndarray:
         out = np.empty((self.inst.n, self.r), np.float64)
         self._chk(self.L.lb_get_V(self.ctx, out.ctypes.data_as(C.c_void_p), LB.LB_F64), "lb_get_V")

@@ -257,6 +257,17 @@ class Relabeled:
         o = np.lexsort((R, i, -sc))
         return i[o], R[o], sc[o], st
 
+    def _native(self, what):
+        if self.vo is not None and not self._native_rows:
+            raise ValueError(f"{what} needs a backend that keeps the row order itself (set_row_order)")
+        return self.b
+
+    def polish_begin(self, V):
+        return self._native("polish_begin").polish_begin(V)
+
+    def polish_get(self):
+        return self._native("polish_get").polish_get()
+
     def certify_dense(self, dg):
         return self.b.certify_dense(dg if self.vo is None else self.vo.rows_in(np.asarray(dg, np.float64)))
 

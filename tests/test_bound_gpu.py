@@ -94,3 +94,24 @@ def test_device_certificate_matches_host(weighted):
     assert dev["price"] < 1e-9
     bad = B.certify_dense_device(inst, g, (max(0.0, -mu) - 1e-5) * s)
     assert not bad["passed"]
+
+
+@pytest.mark.parametrize("dtype,ftol,vtol", [("f64", 1e-10, 1e-7), ("f32", 1e-6, 1e-3)])
+def test_device_polish_matches_host_polish(dtype, ftol, vtol):
+    """polish_rgd_device (row work on the device, FP64 iterate) against polish_rgd (row work in numpy) on the same
+    device operator: the same step and backtracking decisions, the same iterate up to summation order."""
+    inst, _ = _inst()
+    V = np.random.default_rng(3).standard_normal((inst.n, 8))
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    print(f"[bound] device polish dtype={dtype} n={inst.n}", flush=True)
+    lag = B.DeviceLagrangian(inst, r=8, dtype=dtype)
+    try:
+        Vh, fh, gh = B.polish_rgd(lag, V, iters=40, log=lambda *a: None)
+        Vd, fd, gd = B.polish_rgd_device(lag, V, iters=40, log=lambda *a: None)
+    finally:
+        lag.close()
+    print(f"[bound] f host {fh:.12f} device {fd:.12f}", flush=True)
+    assert abs(fd - fh) <= ftol * abs(fh)
+    assert np.abs(Vd - Vh).max() <= vtol
+    assert np.abs(np.linalg.norm(Vd, axis=1) - 1).max() <= 1e-12
+    assert np.abs(gd - gh).max() <= vtol * max(1.0, np.abs(gh).max())
