@@ -1112,13 +1112,11 @@ struct StepArgs {
 // one triangle record of the row step: tau (canonical order), q, pt = lam - rt (1 - tau); if pt > 0 the record adds
 // -pt/2 q to g and rt |q|^2 to the curvature (batch_step6, alm_vcs.py:297-316)
 template <typename TV, int R>
-__device__ inline void step_record(const uint4 *V, const TriRec<typename VT<TV>::TC> &rc,
-                                   const typename VT<TV>::TC *x, int q, double rt, typename VT<TV>::TC *g, double &kap) {
+__device__ inline void step_record_x(const TriRec<typename VT<TV>::TC> &rc, const typename VT<TV>::TC *x,
+                                     const typename VT<TV>::TC *xu, const typename VT<TV>::TC *xw, double rt,
+                                     typename VT<TV>::TC *g, double &kap) {
     typedef typename VT<TV>::TC TC;
     constexpr int CPL = VT<TV>::CPL, S = R / CPL;
-    TC xu[CPL], xw[CPL];
-    VT<TV>::unpack(V[(int64_t)rc.u * S + q], xu);
-    VT<TV>::unpack(V[(int64_t)rc.w * S + q], xw);
     const int role = rc.code % 3;
     const TC *pa = role == 0 ? x : xu;
     const TC *pb = role == 0 ? xu : (role == 1 ? x : xw);
@@ -1139,6 +1137,16 @@ __device__ inline void step_record(const uint4 *V, const TriRec<typename VT<TV>:
         for (int c = 0; c < CPL; c++) g[c] -= (TC)0.5 * pt * qv[c];
         kap += rt * (double)q2;
     }
+}
+template <typename TV, int R>
+__device__ inline void step_record(const uint4 *V, const TriRec<typename VT<TV>::TC> &rc,
+                                   const typename VT<TV>::TC *x, int q, double rt, typename VT<TV>::TC *g, double &kap) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL;
+    TC xu[CPL], xw[CPL];
+    VT<TV>::unpack(V[(int64_t)rc.u * S + q], xu);
+    VT<TV>::unpack(V[(int64_t)rc.w * S + q], xw);
+    step_record_x<TV, R>(rc, x, xu, xw, rt, g, kap);
 }
 
 struct StepSegArgs {
@@ -1298,13 +1306,29 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step(StepArgs a) {
             for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
         }
     } else {
+        // LB_EB gathers in flight; lanes past the row read row i with weight 0 (adds an exact zero, same order)
         const TC *wt = (const TC *)a.w;
-        for (int64_t e = e0; e < e1; e++) {
-            TC xj[CPL];
-            VT<TV>::unpack(a.V[(int64_t)a.col[e] * S + q], xj);
-            const TC w = wt ? wt[e] : (TC)1;
+        for (int64_t e = e0; e < e1; e += LB_EB) {
+            const int64_t nE = e1 - e;
+            int32_t jj[LB_EB];
+            TC ww[LB_EB];
 #pragma unroll
-            for (int c = 0; c < CPL; c++) g[c] += w * xj[c];
+            for (int t = 0; t < LB_EB; t++) {
+                const int64_t ec = t < nE ? e + t : e1 - 1;
+                const int32_t cj = a.col[ec];
+                jj[t] = t < nE ? cj : (int32_t)i;
+                ww[t] = t < nE ? (wt ? wt[ec] : (TC)1) : (TC)0;
+            }
+            uint4 u[LB_EB];
+#pragma unroll
+            for (int t = 0; t < LB_EB; t++) u[t] = a.V[(int64_t)jj[t] * S + q];
+#pragma unroll
+            for (int t = 0; t < LB_EB; t++) {
+                TC xj[CPL];
+                VT<TV>::unpack(u[t], xj);
+#pragma unroll
+                for (int c = 0; c < CPL; c++) g[c] += ww[t] * xj[c];
+            }
         }
     }
     const int64_t r0 = a.iptr[i], r1 = a.iptr[i + 1];
@@ -1316,8 +1340,25 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step(StepArgs a) {
             kap += a.segkap[s];
         }
     } else {
+        // LB_RB records in flight (their two rows gathered together), applied in record order
         const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
-        for (int64_t k = r0; k < r1; k++) step_record<TV, R>(a.V, rec[k], x, q, a.rt, g, kap);
+        for (int64_t k = r0; k < r1; k += LB_RB) {
+            const int64_t nK = r1 - k;
+            TriRec<TC> rc[LB_RB];
+#pragma unroll
+            for (int t = 0; t < LB_RB; t++) rc[t] = rec[t < nK ? k + t : r1 - 1];
+            uint4 uu[LB_RB], uw[LB_RB];
+#pragma unroll
+            for (int t = 0; t < LB_RB; t++) { uu[t] = a.V[(int64_t)rc[t].u * S + q]; uw[t] = a.V[(int64_t)rc[t].w * S + q]; }
+#pragma unroll
+            for (int t = 0; t < LB_RB; t++) {
+                if (t >= nK) break;                          // slot-uniform
+                TC xu[CPL], xw[CPL];
+                VT<TV>::unpack(uu[t], xu);
+                VT<TV>::unpack(uw[t], xw);
+                step_record_x<TV, R>(rc[t], x, xu, xw, a.rt, g, kap);
+            }
+        }
     }
     step_finish<TV, R>(a, x, i, pos, q, g, kap);
 }
@@ -1383,20 +1424,34 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_vcs(Post2Args a) {
     const TC *sig = (const TC *)a.sig;
     for (int m = 0; m < a.nvcs; m++) {
         const int Ri = a.lab[m][i];
-        for (int64_t e = a.ptr[m][i]; e < a.ptr[m][i + 1]; e++) {
-            const int Rc = a.col[e];
+        const int64_t ea = a.ptr[m][i], eb = a.ptr[m][i + 1];
+        for (int64_t e4 = ea; e4 < eb; e4 += 4) {
+          // the entries' columns, multipliers and snapshot values are loaded 4 at a time before any is used
+          int Rq[4];
+          double sq[4], oq[4];
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+              const int64_t ec = e4 + t < eb ? e4 + t : eb - 1;
+              Rq[t] = a.col[ec];
+              sq[t] = (double)sig[ec];
+              oq[t] = a.stZ[ec];
+          }
+          const int nq = (int)(eb - e4 < 4 ? eb - e4 : 4);
+          for (int t = 0; t < nq; t++) {
+            const int64_t e = e4 + t;
+            const int Rc = Rq[t];
             const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
             double s = 0.0;
 #pragma unroll
             for (int c = 0; c < CPL; c++) s += (double)xn[c] * YR[c];
             s = slot_sum<S>(s);
-            double sn = (double)sig[e];
+            double sn = sq[t];
             if (a.rho > 0.0 && di > 0.0) {
                 const double g = (Rc == Ri) ? di * (s - di) : di * s;
                 sn = sn - a.rho * (a.wv * a.rq[m][Rc] / (di * di)) * g;
                 if (sn < 0.0) sn = 0.0;
             }
-            const double so = a.stZ[e];
+            const double so = oq[t];
             if (sn != 0.0 || so != 0.0) {
                 double *dst = a.Z + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
 #pragma unroll
@@ -1407,6 +1462,7 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_vcs(Post2Args a) {
                 if (dc != 0.0) atomicAdd(a.cnt + a.Zoff[m] + Rc, dc);
             }
             if (q == 0) a.stZ[e] = sn;
+          }
         }
     }
     a.V[i * S + q] = un;
