@@ -165,7 +165,17 @@ class Server:
         with threadpool_limits(limits=self.p.blas_threads):
             return self._run(requests, realtime)
 
+    def _prepare_device(self):
+        """Before any worker starts: one tiny device certificate, so rocBLAS / rocSOLVER are opened (dlopen, kernel
+        registration, first handle) by this thread and never by a worker while others launch kernels."""
+        if not (self.p.cert_on_device and self.p.cert_max_n > 0):
+            return
+        keys = np.array([0 * 3 + 1, 0 * 3 + 2, 1 * 3 + 2], np.int64)          # a triangle
+        tiny = I.build_instance(3, keys)
+        B.certify_dense_device(tiny, np.zeros(3), np.ones(3), device=self.device)
+
     def _run(self, requests, realtime):
+        self._prepare_device()
         q = queue.Queue()
         t0 = time.perf_counter()
 
