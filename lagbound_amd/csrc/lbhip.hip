@@ -1997,6 +1997,16 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_pol_post(PolArgs a) {
     }
 }
 
+// Hub-segment tables of the sweeps. A sweep batch holds a few hundred rows and runs as long as its longest row, so
+// the sweeps split rows at a lower threshold into shorter segments than the evaluation (which is tuned for
+// throughput over all rows).
+struct SweepSegs {
+    int64_t heavyE = 32, heavyT = 32; int seg_len = 128;
+    std::vector<int32_t> hE_rows, hE_seg0, hT_rows, hT_seg0, hE_idx, hT_idx;
+    std::vector<Seg> segE, segT;
+    DBuf hErows, hEseg0, hTrows, hTseg0, segs, segout, segkap;
+};
+
 struct lb_ctx {
     int device = 0, vdtype = LB_F32, r = 16;
     hipStream_t stream = nullptr;
@@ -2020,6 +2030,7 @@ struct lb_ctx {
     std::vector<Seg> segE, segT;
     bool have_eval = false; double last_rho = 0.0;
     std::vector<int32_t> hE_idx, hT_idx;   // row -> index into hE_rows / hT_rows, or -1
+    SweepSegs sw;                          // the sweeps' own hub-segment tables
     std::vector<int64_t> h_iptr;            // triangle incidence pointer (host copy)
     // device buffers
     DBuf vrow;                        // VCS working-set entry -> its row (entry-parallel VCS passes)
@@ -2205,11 +2216,42 @@ int build_row_order(lb_ctx *c) {
     return upload(c, c->rorder, ord.data(), sizeof(int32_t) * (size_t)n);
 }
 
+// the sweeps' tables from the host row pointers and the triangle incidence counts (h_rowptr, h_iptr)
+int build_sweep_segs(lb_ctx *c) {
+    SweepSegs &w = c->sw;
+    const int64_t n = c->n;
+    int rc;
+    build_segments(n, c->h_rowptr.data(), w.heavyE, w.seg_len, 0, w.hE_rows, w.hE_seg0, w.segE, 0);
+    if (c->h_iptr.size() == (size_t)n + 1)
+        build_segments(n, c->h_iptr.data(), w.heavyT, w.seg_len, 1, w.hT_rows, w.hT_seg0, w.segT, (int32_t)w.segE.size());
+    else { w.hT_rows.clear(); w.hT_seg0.assign(1, (int32_t)w.segE.size()); w.segT.clear(); }
+    w.hE_idx.assign(n, -1);
+    for (size_t k = 0; k < w.hE_rows.size(); k++) w.hE_idx[w.hE_rows[k]] = (int32_t)k;
+    w.hT_idx.assign(n, -1);
+    for (size_t k = 0; k < w.hT_rows.size(); k++) w.hT_idx[w.hT_rows[k]] = (int32_t)k;
+    std::vector<Seg> all(w.segE);
+    all.insert(all.end(), w.segT.begin(), w.segT.end());
+    const size_t ns = std::max<size_t>(all.size(), 1);
+    if ((rc = dalloc(c, w.hErows, sizeof(int32_t) * std::max<size_t>(w.hE_rows.size(), 1))) ||
+        (rc = dalloc(c, w.hTrows, sizeof(int32_t) * std::max<size_t>(w.hT_rows.size(), 1))))
+        return rc;
+    if (!w.hE_rows.empty() && (rc = upload(c, w.hErows, w.hE_rows.data(), sizeof(int32_t) * w.hE_rows.size()))) return rc;
+    if (!w.hT_rows.empty() && (rc = upload(c, w.hTrows, w.hT_rows.data(), sizeof(int32_t) * w.hT_rows.size()))) return rc;
+    if ((rc = upload(c, w.hEseg0, w.hE_seg0.data(), sizeof(int32_t) * w.hE_seg0.size()))) return rc;
+    if ((rc = upload(c, w.hTseg0, w.hT_seg0.data(), sizeof(int32_t) * w.hT_seg0.size()))) return rc;
+    if ((rc = dalloc(c, w.segs, sizeof(Seg) * ns))) return rc;
+    if (!all.empty() && (rc = upload(c, w.segs, all.data(), sizeof(Seg) * all.size()))) return rc;
+    if ((rc = dalloc(c, w.segout, ns * c->r * c->cbytes))) return rc;
+    if ((rc = dalloc(c, w.segkap, sizeof(double) * ns))) return rc;
+    return 0;
+}
+
 int rebuild_segs(lb_ctx *c) {
     std::vector<Seg> all(c->segE);
     all.insert(all.end(), c->segT.begin(), c->segT.end());
     int rc;
     if ((rc = build_row_order(c))) return rc;
+    if ((rc = build_sweep_segs(c))) return rc;
     if ((rc = upload(c, c->segs, all.empty() ? nullptr : all.data(), sizeof(Seg) * all.size()))) return rc;
     if ((rc = dalloc(c, c->segout, (size_t)std::max<size_t>(all.size(), 1) * c->r * c->cbytes))) return rc;
     const int nblk = (int)((all.size() + 3) / 4);
@@ -2431,14 +2473,15 @@ struct Launch {
         sa.vcol = (const int32_t *)c->vcol.p; sa.vsig = c->vsig.p; sa.Y = Yp; sa.Z = (const double *)c->Zs.p;
         sa.cnt = (const double *)c->cnts.p; sa.stZ = (const double *)c->stZ.p; sa.wv = c->wv; sa.kap2 = kap2;
         sa.iptr = (const int64_t *)c->iptr.p; sa.rec = c->rec.p; sa.rho = rho; sa.rt = rt; sa.damp = damp;
-        sa.heavyE = c->built_heavyE; sa.heavyT = c->built_heavyT;
-        sa.hE_rows = (const int32_t *)c->hErows.p; sa.hE_seg0 = (const int32_t *)c->hEseg0.p; sa.nhE = (int)c->hE_rows.size();
-        sa.hT_rows = (const int32_t *)c->hTrows.p; sa.hT_seg0 = (const int32_t *)c->hTseg0.p; sa.nhT = (int)c->hT_rows.size();
-        sa.segout = c->segout.p; sa.segkap = (const double *)c->segkap.p;
+        const SweepSegs &w = c->sw;
+        sa.heavyE = w.heavyE; sa.heavyT = w.heavyT;
+        sa.hE_rows = (const int32_t *)w.hErows.p; sa.hE_seg0 = (const int32_t *)w.hEseg0.p; sa.nhE = (int)w.hE_rows.size();
+        sa.hT_rows = (const int32_t *)w.hTrows.p; sa.hT_seg0 = (const int32_t *)w.hTseg0.p; sa.nhT = (int)w.hT_rows.size();
+        sa.segout = w.segout.p; sa.segkap = (const double *)w.segkap.p;
         sa.Vstage = (uint4 *)c->Vstage.p; sa.DD = (double *)c->DDb.p;
         StepSegArgs ga{};
         ga.V = (const uint4 *)c->V.p; ga.col = (const int32_t *)c->col.p; ga.w = c->weighted ? c->w.p : nullptr;
-        ga.rec = c->rec.p; ga.segs = (const Seg *)c->segs.p; ga.rt = rt; ga.segout = c->segout.p; ga.segkap = (double *)c->segkap.p;
+        ga.rec = c->rec.p; ga.segs = (const Seg *)w.segs.p; ga.rt = rt; ga.segout = w.segout.p; ga.segkap = (double *)w.segkap.p;
         PairRowArgs pa{};
         pa.nlev = c->nlev; pa.R = R;
         for (int l = 0; l <= c->nlev; l++) { pa.Poff[l] = Poff[l]; pa.soff[l] = c->soff[l]; }
@@ -2872,7 +2915,8 @@ void lb_destroy(lb_ctx *c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     DBuf *bufs[] = {&c->rowptr, &c->col, &c->w, &c->d, &c->labs, &c->sig, &c->pp, &c->rq, &c->dsq, &c->vlabs, &c->vrq,
                     &c->vptr, &c->vcol, &c->vsig, &c->pv, &c->vrow, &c->pol_V, &c->pol_Vn, &c->pol_R, &c->pol_g,
-                    &c->pol_part, &c->pol_sum, &c->TI, &c->lam, &c->pt, &c->iptr, &c->rec, &c->V, &c->G,
+                    &c->pol_part, &c->pol_sum, &c->sw.hErows, &c->sw.hEseg0, &c->sw.hTrows, &c->sw.hTseg0,
+                    &c->sw.segs, &c->sw.segout, &c->sw.segkap, &c->TI, &c->lam, &c->pt, &c->iptr, &c->rec, &c->V, &c->G,
                     &c->Y, &c->z, &c->M, &c->pdiag, &c->levtot, &c->levviol, &c->Z, &c->colsum_part, &c->vcs_part,
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
                     &c->hTseg0, &c->parts, &c->lamsum, &c->order, &c->tmask, &c->Kq, &c->stZ, &c->Zs, &c->cnts,
@@ -2894,6 +2938,11 @@ int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
     drop_capture(c);
     if (!c || !name) return fail(c, -1, "lb_set_option: bad arguments");
     if (std::strcmp(name, "row_order") == 0) { c->use_order = value != 0; return 0; }
+    if (std::strcmp(name, "sweep_heavy") == 0 || std::strcmp(name, "sweep_seg") == 0) {
+        if (value < 1 || value > ((int64_t)1 << 30)) return fail(c, -1, "lb_set_option: sweep segment values must be positive");
+        if (name[6] == 'h') { c->sw.heavyE = value; c->sw.heavyT = value; } else c->sw.seg_len = (int)value;
+        return c->n ? build_sweep_segs(c) : 0;
+    }
     return fail(c, -1, std::string("lb_set_option: unknown option ") + name);
 }
 
@@ -3332,7 +3381,6 @@ int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap
     const int64_t maxb = n / nbatch + 1;
     int64_t nnzv = 0;
     for (int64_t x : c->vnnz) nnzv += x;
-    const size_t nseg = c->segE.size() + c->segT.size();
     int rc;
     if ((rc = dalloc(c, c->order, sizeof(int32_t) * n))) return rc;
     if ((rc = dalloc(c, c->tmask, (size_t)nbatch * std::max(c->sumPpair, 1)))) return rc;
@@ -3342,7 +3390,6 @@ int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap
     if ((rc = dalloc(c, c->cnts, sizeof(double) * std::max(c->sumPv, 1)))) return rc;
     if ((rc = dalloc(c, c->Vstage, (size_t)maxb * S * 16))) return rc;
     if ((rc = dalloc(c, c->DDb, sizeof(double) * (size_t)maxb * R))) return rc;
-    if ((rc = dalloc(c, c->segkap, sizeof(double) * std::max<size_t>(nseg, 1)))) return rc;
     std::vector<int32_t> o32(n);
     std::vector<uint8_t> seen(n);
     std::vector<int64_t> bpos(nbatch + 1), lpos(nbatch + 1);
@@ -3364,9 +3411,10 @@ int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap
             lpos[b] = (int64_t)list.size();
             for (int64_t p = bpos[b]; p < bpos[b + 1]; p++) {
                 const int32_t i = o32[p];
-                const int32_t ke = c->hE_idx[i], kt = c->hT_idx[i];
-                if (ke >= 0) for (int32_t s = c->hE_seg0[ke]; s < c->hE_seg0[ke + 1]; s++) list.push_back(s);
-                if (kt >= 0) for (int32_t s = c->hT_seg0[kt]; s < c->hT_seg0[kt + 1]; s++) list.push_back(s);
+                const SweepSegs &w = c->sw;
+                const int32_t ke = w.hE_idx[i], kt = w.hT_idx[i];
+                if (ke >= 0) for (int32_t s = w.hE_seg0[ke]; s < w.hE_seg0[ke + 1]; s++) list.push_back(s);
+                if (kt >= 0) for (int32_t s = w.hT_seg0[kt]; s < w.hT_seg0[kt + 1]; s++) list.push_back(s);
             }
         }
         lpos[nbatch] = (int64_t)list.size();
