@@ -1404,6 +1404,7 @@ struct Post2Args {
     int nvcs; int Yoff[LB_MAXVCS]; int Zoff[LB_MAXVCS]; const int32_t *lab[LB_MAXVCS]; const double *rq[LB_MAXVCS];
     const int64_t *ptr[LB_MAXVCS]; const int32_t *col; const void *sig; const double *Y; double wv, rho;
     double *stZ; double *Z; double *cnt;
+    int sumPv; int use_lds;     // Z / cnt increments gathered per workgroup in LDS first (sumPv (R + 1) doubles)
 };
 
 // vcs_post: new effective multipliers SN of the moved rows (new v_i, updated Y), Z_R += d_i (SN v_new - stZ v_old),
@@ -1412,9 +1413,17 @@ template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_vcs(Post2Args a) {
     typedef typename VT<TV>::TC TC;
     constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    extern __shared__ double shZc[];
     const int slot = threadIdx.x / S, q = threadIdx.x % S;
     const int64_t pos = (int64_t)blockIdx.x * NSL + slot;
-    if (pos >= a.nb) return;
+    const int wz = a.sumPv * R;
+    double *Zacc = a.use_lds ? shZc : a.Z;
+    double *Cacc = a.use_lds ? shZc + wz : a.cnt;
+    if (a.use_lds) {
+        for (int e = threadIdx.x; e < wz + a.sumPv; e += LB_BLOCK) shZc[e] = 0.0;
+        __syncthreads();
+    }
+    if (pos < a.nb) {
     const int64_t i = a.rows[pos];
     const uint4 un = a.Vstage[pos * S + q];
     TC xn[CPL], xo[CPL];
@@ -1453,19 +1462,27 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_vcs(Post2Args a) {
             }
             const double so = oq[t];
             if (sn != 0.0 || so != 0.0) {
-                double *dst = a.Z + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+                double *dst = Zacc + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
 #pragma unroll
                 for (int c = 0; c < CPL; c++) atomicAdd(dst + c, di * (sn * (double)xn[c] - so * (double)xo[c]));
             }
             if (q == 0) {
                 const double dc = (sn > 0.0 ? 1.0 : 0.0) - (so > 0.0 ? 1.0 : 0.0);
-                if (dc != 0.0) atomicAdd(a.cnt + a.Zoff[m] + Rc, dc);
+                if (dc != 0.0) atomicAdd(Cacc + a.Zoff[m] + Rc, dc);
             }
             if (q == 0) a.stZ[e] = sn;
           }
         }
     }
     a.V[i * S + q] = un;
+    }
+    if (a.use_lds) {                                     // one global add per workgroup and nonzero entry
+        __syncthreads();
+        for (int e = threadIdx.x; e < wz; e += LB_BLOCK)
+            if (shZc[e] != 0.0) atomicAdd(a.Z + e, shZc[e]);
+        for (int e = threadIdx.x; e < a.sumPv; e += LB_BLOCK)
+            if (shZc[wz + e] != 0.0) atomicAdd(a.cnt + e, shZc[wz + e]);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------ triangle separation
@@ -1760,19 +1777,31 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_sep_one_edge(SepParams p, int m
     if (mode == 1 && lane == 0) p.ccount[j] = (int32_t)(cnt < 0x7fffffff ? cnt : 0x7fffffff);
 }
 
+#define LB_THRESH_MAXNB 4096
 // threshold from the histogram (single workgroup): b0 = first bin whose tail count <= cap, vmin, per-chunk caps and
 // offsets; *total = sum of the chunk caps; *vminp = -1 if no bin qualifies
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_sep_thresh(const unsigned long long *hist, int nchunk, int nb,
                                                           double vtol, int64_t cap, double *vminp, int64_t *chunk_cnt,
                                                           int64_t *chunk_off, int64_t *total, int *b0p) {
     __shared__ int b0s;
+    extern __shared__ unsigned long long btot[];    // per-bin totals over the chunks (nb entries) when they fit
+    const bool par = nb <= LB_THRESH_MAXNB;
+    if (par) {
+        for (int b = threadIdx.x; b < nb; b += LB_BLOCK) {
+            unsigned long long t = 0;
+            for (int c = 0; c < nchunk; c++) t += hist[(int64_t)c * nb + b];
+            btot[b] = t;
+        }
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
         unsigned long long cum = 0;
         int b0 = nb;
         // cum[b] is non-increasing in b: scan from the top bin down, remember the lowest bin with cum <= cap
         for (int b = nb - 1; b >= 0; b--) {
             unsigned long long t = 0;
-            for (int c = 0; c < nchunk; c++) t += hist[(int64_t)c * nb + b];
+            if (par) t = btot[b];
+            else for (int c = 0; c < nchunk; c++) t += hist[(int64_t)c * nb + b];
             cum += t;
             if (cum <= (unsigned long long)cap) b0 = b; else break;
         }
@@ -2508,6 +2537,9 @@ struct Launch {
         }
         pb.col = (const int32_t *)c->vcol.p; pb.sig = c->vsig.p; pb.Y = Yp; pb.wv = c->wv; pb.rho = rho;
         pb.stZ = (double *)c->stZ.p; pb.Z = (double *)c->Zs.p; pb.cnt = (double *)c->cnts.p;
+        pb.sumPv = c->sumPv;
+        const size_t ldsV = sizeof(double) * (size_t)c->sumPv * (R + 1);
+        pb.use_lds = c->sumPv > 0 && ldsV <= 48 * 1024;
         const int32_t *order = (const int32_t *)c->order.p;
         const int32_t *slist = (const int32_t *)c->seglist.p;
         for (int b = 0; b < nbatch; b++) {
@@ -2528,7 +2560,8 @@ struct Launch {
             const unsigned pg = (unsigned)std::min<int64_t>(64, (nb + 127) / 128);
             hipLaunchKernelGGL(lb_k_post_acc, dim3(pg), dim3(LB_BLOCK), qa.use_lds ? ldsP : 0, c->stream, qa);
             pb.rows = order + p0; pb.nb = nb;
-            hipLaunchKernelGGL((lb_k_post_vcs<TV, R>), dim3((unsigned)((nb + NSL - 1) / NSL)), dim3(LB_BLOCK), 0, c->stream, pb);
+            hipLaunchKernelGGL((lb_k_post_vcs<TV, R>), dim3((unsigned)((nb + NSL - 1) / NSL)), dim3(LB_BLOCK),
+                               pb.use_lds ? ldsV : 0, c->stream, pb);
         }
         HIPCHK(c, hipGetLastError());
         return 0;
@@ -2623,7 +2656,8 @@ struct Launch {
         double *dvmin = (double *)scal.p;
         int64_t *dtotal = (int64_t *)((char *)scal.p + 8);
         int *db0 = (int *)((char *)scal.p + 16);
-        hipLaunchKernelGGL(lb_k_sep_thresh, dim3(1), dim3(LB_BLOCK), 0, c->stream, (const unsigned long long *)hist.p,
+        const size_t lds_t = nb <= LB_THRESH_MAXNB ? sizeof(unsigned long long) * (size_t)nb : 0;
+        hipLaunchKernelGGL(lb_k_sep_thresh, dim3(1), dim3(LB_BLOCK), lds_t, c->stream, (const unsigned long long *)hist.p,
                            nchunk, nb, vtol, cap, dvmin, (int64_t *)chunk_cnt.p, (int64_t *)chunk_off.p, dtotal, db0);
         double vmin = -1.0;
         int64_t total = 0;
