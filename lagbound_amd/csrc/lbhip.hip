@@ -1049,46 +1049,76 @@ struct VcsStateArgs {
     int nvcs; int Yoff[LB_MAXVCS]; int Zoff[LB_MAXVCS];
     const int32_t *lab[LB_MAXVCS]; const double *rq[LB_MAXVCS]; const int64_t *ptr[LB_MAXVCS];
     const int32_t *col; const void *sig; const double *Y; double wv, rho;
-    double *stZ; double *Z; double *cnt;     // Z, cnt zeroed before the launch; FP64 atomics
+    double *stZ; double *Z; double *cnt;     // without LDS: Z, cnt zeroed before the launch; FP64 atomics
+    int sumPv; int use_lds;     // with LDS: per-workgroup sums of [Z | cnt] written to part (reduced afterwards)
+    double *part;
 };
 
-// vcs_state: stZ = effective multipliers at (V, Y); Z_R = sum stZ d_j v_j; cnt_R = #(stZ > 0)
+// vcs_state: stZ = effective multipliers at (V, Y); Z_R = sum stZ d_j v_j; cnt_R = #(stZ > 0). With LDS, each
+// workgroup sums its rows' contributions in LDS and writes one partial row [Z | cnt] (reduced in a fixed order
+// afterwards); the entries of a row are loaded four at a time before any is used.
 template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_state(VcsStateArgs a) {
     typedef typename VT<TV>::TC TC;
     constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    extern __shared__ double shS[];
     const int slot = threadIdx.x / S, q = threadIdx.x % S;
     const TC *sig = (const TC *)a.sig;
+    const int wz = a.sumPv * R, wt = wz + a.sumPv;
+    double *Zacc = a.use_lds ? shS : a.Z;
+    double *Cacc = a.use_lds ? shS + wz : a.cnt;
+    if (a.use_lds) {
+        for (int e = threadIdx.x; e < wt; e += LB_BLOCK) shS[e] = 0.0;
+        __syncthreads();
+    }
     for (int64_t i = (int64_t)blockIdx.x * NSL + slot; i < a.n; i += (int64_t)gridDim.x * NSL) {
         TC x[CPL];
         VT<TV>::unpack(a.V[i * S + q], x);
         const double di = a.d[i];
         for (int m = 0; m < a.nvcs; m++) {
             const int Ri = a.lab[m][i];
-            for (int64_t e = a.ptr[m][i]; e < a.ptr[m][i + 1]; e++) {
-                const int Rc = a.col[e];
-                const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
-                double s = 0.0;
+            const int64_t ea = a.ptr[m][i], eb = a.ptr[m][i + 1];
+            for (int64_t e4 = ea; e4 < eb; e4 += 4) {
+                int Rq[4];
+                double sq[4];
 #pragma unroll
-                for (int c = 0; c < CPL; c++) s += (double)x[c] * YR[c];
-                s = slot_sum<S>(s);
-                double p = (double)sig[e];
-                if (a.rho > 0.0 && di > 0.0) {
-                    const double g = (Rc == Ri) ? di * (s - di) : di * s;
-                    p = p - a.rho * (a.wv * a.rq[m][Rc] / (di * di)) * g;
-                    if (p < 0.0) p = 0.0;
+                for (int t = 0; t < 4; t++) {
+                    const int64_t ec = e4 + t < eb ? e4 + t : eb - 1;
+                    Rq[t] = a.col[ec];
+                    sq[t] = (double)sig[ec];
                 }
-                if (q == 0) {
-                    a.stZ[e] = p;
-                    if (p > 0.0) atomicAdd(a.cnt + a.Zoff[m] + Rc, 1.0);
-                }
-                if (p != 0.0) {
-                    double *dst = a.Z + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+                const int nq = (int)(eb - e4 < 4 ? eb - e4 : 4);
+                for (int t = 0; t < nq; t++) {
+                    const int64_t e = e4 + t;
+                    const int Rc = Rq[t];
+                    const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+                    double s = 0.0;
 #pragma unroll
-                    for (int c = 0; c < CPL; c++) atomicAdd(dst + c, p * (di * (double)x[c]));
+                    for (int c = 0; c < CPL; c++) s += (double)x[c] * YR[c];
+                    s = slot_sum<S>(s);
+                    double p = sq[t];
+                    if (a.rho > 0.0 && di > 0.0) {
+                        const double g = (Rc == Ri) ? di * (s - di) : di * s;
+                        p = p - a.rho * (a.wv * a.rq[m][Rc] / (di * di)) * g;
+                        if (p < 0.0) p = 0.0;
+                    }
+                    if (q == 0) {
+                        a.stZ[e] = p;
+                        if (p > 0.0) atomicAdd(Cacc + a.Zoff[m] + Rc, 1.0);
+                    }
+                    if (p != 0.0) {
+                        double *dst = Zacc + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+#pragma unroll
+                        for (int c = 0; c < CPL; c++) atomicAdd(dst + c, p * (di * (double)x[c]));
+                    }
                 }
             }
         }
+    }
+    if (a.use_lds) {
+        __syncthreads();
+        double *out = a.part + (int64_t)blockIdx.x * wt;
+        for (int e = threadIdx.x; e < wt; e += LB_BLOCK) out[e] = shS[e];
     }
 }
 
@@ -2063,6 +2093,7 @@ struct lb_ctx {
     std::vector<int64_t> h_iptr;            // triangle incidence pointer (host copy)
     // device buffers
     DBuf vrow;                        // VCS working-set entry -> its row (entry-parallel VCS passes)
+    DBuf vst_part;                    // per-workgroup [Z | cnt] partials of the sweeps' VCS state pass
     DBuf pol_V, pol_Vn, pol_R, pol_g, pol_part, pol_sum;   // device RGD polish state (n x r FP64, n, partials)
     bool pol_ok = false;
     DBuf rowptr, col, w, d, labs, sig, pp, rq, dsq, vlabs, vrq, vptr, vcol, vsig, pv, TI, lam, pt, iptr, rec,
@@ -2482,9 +2513,20 @@ struct Launch {
             }
             vs.col = (const int32_t *)c->vcol.p; vs.sig = c->vsig.p; vs.Y = Yp; vs.wv = c->wv; vs.rho = rho;
             vs.stZ = (double *)c->stZ.p; vs.Z = (double *)c->Zs.p; vs.cnt = (double *)c->cnts.p;
-            hipLaunchKernelGGL(lb_k_fill, dim3(64), dim3(256), 0, c->stream, (double *)c->Zs.p, (int64_t)c->sumPv * R, 0.0);
-            hipLaunchKernelGGL(lb_k_fill, dim3(64), dim3(256), 0, c->stream, (double *)c->cnts.p, (int64_t)c->sumPv, 0.0);
-            hipLaunchKernelGGL((lb_k_vcs_state<TV, R>), dim3(LB_RED_BLOCKS), dim3(LB_BLOCK), 0, c->stream, vs);
+            const int wz = c->sumPv * R, wt = wz + c->sumPv;
+            const size_t ldsS = sizeof(double) * (size_t)wt;
+            vs.sumPv = c->sumPv; vs.use_lds = ldsS <= 48 * 1024; vs.part = (double *)c->vst_part.p;
+            if (vs.use_lds) {
+                hipLaunchKernelGGL((lb_k_vcs_state<TV, R>), dim3(LB_RED_BLOCKS), dim3(LB_BLOCK), ldsS, c->stream, vs);
+                hipLaunchKernelGGL(lb_k_reduce, dim3((wz + 63) / 64), dim3(1024), 0, c->stream, (const double *)c->vst_part.p,
+                                   LB_RED_BLOCKS, wz, wt, (double *)c->Zs.p);
+                hipLaunchKernelGGL(lb_k_reduce, dim3((c->sumPv + 63) / 64), dim3(1024), 0, c->stream,
+                                   (const double *)c->vst_part.p + wz, LB_RED_BLOCKS, c->sumPv, wt, (double *)c->cnts.p);
+            } else {
+                hipLaunchKernelGGL(lb_k_fill, dim3(64), dim3(256), 0, c->stream, (double *)c->Zs.p, (int64_t)c->sumPv * R, 0.0);
+                hipLaunchKernelGGL(lb_k_fill, dim3(64), dim3(256), 0, c->stream, (double *)c->cnts.p, (int64_t)c->sumPv, 0.0);
+                hipLaunchKernelGGL((lb_k_vcs_state<TV, R>), dim3(LB_RED_BLOCKS), dim3(LB_BLOCK), 0, c->stream, vs);
+            }
         }
         StepArgs sa{};
         sa.V = (const uint4 *)c->V.p; sa.rowptr = (const int64_t *)c->rowptr.p; sa.col = (const int32_t *)c->col.p;
@@ -2949,7 +2991,7 @@ void lb_destroy(lb_ctx *c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     DBuf *bufs[] = {&c->rowptr, &c->col, &c->w, &c->d, &c->labs, &c->sig, &c->pp, &c->rq, &c->dsq, &c->vlabs, &c->vrq,
                     &c->vptr, &c->vcol, &c->vsig, &c->pv, &c->vrow, &c->pol_V, &c->pol_Vn, &c->pol_R, &c->pol_g,
-                    &c->pol_part, &c->pol_sum, &c->sw.hErows, &c->sw.hEseg0, &c->sw.hTrows, &c->sw.hTseg0,
+                    &c->pol_part, &c->pol_sum, &c->vst_part, &c->sw.hErows, &c->sw.hEseg0, &c->sw.hTrows, &c->sw.hTseg0,
                     &c->sw.segs, &c->sw.segout, &c->sw.segkap, &c->TI, &c->lam, &c->pt, &c->iptr, &c->rec, &c->V, &c->G,
                     &c->Y, &c->z, &c->M, &c->pdiag, &c->levtot, &c->levviol, &c->Z, &c->colsum_part, &c->vcs_part,
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
@@ -3422,6 +3464,7 @@ int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap
     if ((rc = dalloc(c, c->stZ, sizeof(double) * std::max<int64_t>(nnzv, 1)))) return rc;
     if ((rc = dalloc(c, c->Zs, sizeof(double) * (size_t)std::max(c->sumPv, 1) * R))) return rc;
     if ((rc = dalloc(c, c->cnts, sizeof(double) * std::max(c->sumPv, 1)))) return rc;
+    if ((rc = dalloc(c, c->vst_part, sizeof(double) * (size_t)LB_RED_BLOCKS * std::max(c->sumPv * (R + 1), 1)))) return rc;
     if ((rc = dalloc(c, c->Vstage, (size_t)maxb * S * 16))) return rc;
     if ((rc = dalloc(c, c->DDb, sizeof(double) * (size_t)maxb * R))) return rc;
     std::vector<int32_t> o32(n);
