@@ -1253,9 +1253,31 @@ __device__ inline void step_finish(const StepArgs &a, const typename VT<TV>::TC 
 #pragma unroll
         for (int c = 0; c < CPL; c++) acc[c] = 0.0;
         double kp = 0.0, pown = 0.0, sz = 0.0;
-        for (int64_t e = a.vptr[m][i]; e < a.vptr[m][i + 1]; e++) {
-            const int Rc = a.vcol[e];
-            const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+        const int64_t ea = a.vptr[m][i], eb = a.vptr[m][i + 1];
+        for (int64_t e4 = ea; e4 < eb; e4 += 4) {
+          // four entries at a time: their columns and multipliers, then their cluster rows, before any is used
+          int Rq[4];
+          double sq[4], zq[4];
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+              const int64_t ec = e4 + t < eb ? e4 + t : eb - 1;
+              Rq[t] = a.vcol[ec];
+              sq[t] = (double)vsig[ec];
+              zq[t] = a.stZ[ec];
+          }
+          double yq[4][CPL];
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+              const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rq[t]) * R + q * CPL;
+#pragma unroll
+              for (int c = 0; c < CPL; c++) yq[t][c] = YR[c];
+          }
+          const int nq = (int)(eb - e4 < 4 ? eb - e4 : 4);
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            if (t >= nq) break;                              // slot-uniform
+            const int Rc = Rq[t];
+            const double *YR = yq[t];
             double s = 0.0, y2 = 0.0;
 #pragma unroll
             for (int c = 0; c < CPL; c++) { s += (double)x[c] * YR[c]; y2 += YR[c] * YR[c]; }
@@ -1264,18 +1286,19 @@ __device__ inline void step_finish(const StepArgs &a, const typename VT<TV>::TC 
             if (a.rho > 0.0 && di > 0.0) {
                 const double g2 = (Rc == Ri) ? di * (s - di) : di * s;
                 const double rr = a.wv * a.vrq[m][Rc] / (di * di);
-                p = (double)vsig[e] - a.rho * rr * g2;
+                p = sq[t] - a.rho * rr * g2;
                 if (p > 0.0) {
                     y2 = slot_sum<S>(y2);
                     if (Rc == Ri) y2 = y2 - 2 * di * s + di * di;
                     kp += a.rho * rr * di * di * y2;
                 } else p = 0.0;
-            } else p = (double)vsig[e];
-            if (Rc == Ri) { pown = p; sz = a.stZ[e]; }
+            } else p = sq[t];
+            if (Rc == Ri) { pown = p; sz = zq[t]; }
             if (p != 0.0) {
 #pragma unroll
                 for (int c = 0; c < CPL; c++) acc[c] += p * YR[c];
             }
+          }
         }
         const double *ZR = a.Z + (int64_t)(a.Zoff[m] + Ri) * R + q * CPL;
 #pragma unroll
@@ -1475,11 +1498,20 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_vcs(Post2Args a) {
               sq[t] = (double)sig[ec];
               oq[t] = a.stZ[ec];
           }
+          double yq[4][CPL];
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+              const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rq[t]) * R + q * CPL;
+#pragma unroll
+              for (int c = 0; c < CPL; c++) yq[t][c] = YR[c];
+          }
           const int nq = (int)(eb - e4 < 4 ? eb - e4 : 4);
-          for (int t = 0; t < nq; t++) {
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            if (t >= nq) break;                              // slot-uniform
             const int64_t e = e4 + t;
             const int Rc = Rq[t];
-            const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+            const double *YR = yq[t];
             double s = 0.0;
 #pragma unroll
             for (int c = 0; c < CPL; c++) s += (double)xn[c] * YR[c];
