@@ -178,3 +178,24 @@ def test_chung_lu_2400k_separation(chung_lu_2400k):
     eng = _engine(inst, V, "bf16", 16)
     _separations(eng, inst, 8_000_000, "chung-lu 2.4M")
     eng.close()
+
+
+@pytest.mark.timeout(900)
+def test_chung_lu_300k_fp32_vs_fp64_search():
+    """The same bound request on the 300k-vertex Chung-Lu graph searched in FP32 and in FP64 (label-propagation
+    levels, triangle and VCS separation, then the FP64-assembled candidate bound): abs(U_fp32 - U_fp64) is printed;
+    the searches may take different paths, so the bar is loose (1e-2 of U_fp64)."""
+    from lagbound_amd import instances as I
+    from lagbound_amd import serve
+    n, keys = I.chung_lu_target(300_000, 2.1, 13.0, 0)
+    out = {}
+    for dtype in ("f32", "f64"):
+        print(f"[config] chung-lu 300k solve + estimate, {dtype} search", flush=True)
+        p = serve.Params(dtype=dtype, outer=8, cert_max_n=0)
+        res, _ = serve.solve_request(serve.Request(rid=0, n=n, keys=keys, seed=3), p)
+        out[dtype] = res
+        print(f"[config] {dtype}: dv {res['dv']:.9f} ub {res['ub']:.9f} lb {res['lb']:.6f}", flush=True)
+    du = abs(out["f32"]["ub"] - out["f64"]["ub"])
+    print(f"[config] abs(U_fp32 - U_fp64) = {du:.3e} (relative {du / abs(out['f64']['ub']):.3e})", flush=True)
+    assert np.isfinite(out["f64"]["ub"]) and out["f64"]["ub"] >= out["f64"]["lb"]
+    assert du <= 1e-2 * abs(out["f64"]["ub"])
