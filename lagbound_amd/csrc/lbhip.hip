@@ -31,6 +31,9 @@
 #include "../../include/lbhip.h"
 
 // gathers kept in flight per lane: row kernel edges / triangle records, batch row kernel edges / records
+// LDS row stride padding (doubles) of per-cluster rows accumulated or read by many slots at once (colsum, the VCS
+// passes): two doubles shift consecutive cluster rows by 16 B, so slots on different clusters spread over the banks
+#define LB_VPAD 2
 #ifndef LB_EB
 #define LB_EB 16
 #endif
@@ -47,7 +50,9 @@
 #define LB_MAXLEV 8
 #define LB_MAXVCS 4
 #define LB_RED_BLOCKS 512
+#ifndef LB_VCS_BLOCKS
 #define LB_VCS_BLOCKS 2048
+#endif
 #define LB_BLOCK 256
 
 typedef uint16_t bf16raw;
@@ -217,7 +222,8 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_colsum(ColsumArgs a) {
     __shared__ double zred[LB_BLOCK / 64][R];
     const int slot = threadIdx.x / S, q = threadIdx.x % S;
     const int width = a.use_lds ? a.sumP * R : 0;
-    for (int e = threadIdx.x; e < width; e += LB_BLOCK) shY[e] = 0.0;
+    constexpr int RS = R + LB_VPAD;                  // padded LDS row stride
+    for (int e = threadIdx.x; e < a.sumP * RS && a.use_lds; e += LB_BLOCK) shY[e] = 0.0;
     __syncthreads();
     double zacc[CPL];
 #pragma unroll
@@ -231,7 +237,7 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_colsum(ColsumArgs a) {
         for (int c = 0; c < CPL; c++) { v[c] = di * (double)x[c]; zacc[c] += v[c]; }
         for (int s = 0; s < a.ls.nsets; s++) {
             const int Q = a.ls.off[s] + a.ls.lab[s][i];
-            double *dst = a.use_lds ? (shY + (int64_t)Q * R + q * CPL) : (a.Yglob + (int64_t)Q * R + q * CPL);
+            double *dst = a.use_lds ? (shY + (int64_t)Q * RS + q * CPL) : (a.Yglob + (int64_t)Q * R + q * CPL);
 #pragma unroll
             for (int c = 0; c < CPL; c++) atomicAdd(dst + c, v[c]);
         }
@@ -256,7 +262,7 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_colsum(ColsumArgs a) {
         for (int k = 0; k < LB_BLOCK / 64; k++) s += zred[k][threadIdx.x];
         out[threadIdx.x] = s;
     }
-    for (int e = threadIdx.x; e < width; e += LB_BLOCK) out[R + e] = shY[e];
+    for (int e = threadIdx.x; e < width; e += LB_BLOCK) out[R + e] = shY[(e / R) * RS + e % R];
 }
 
 // fixed-order sum of per-block partials: out[e] = sum_b part[b * stride + e]. 64 columns per workgroup (one per
@@ -424,9 +430,6 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_z(VcsArgs a) {
 // before any is used (the pass is bound by dependent load latency, not by bandwidth or ALU). Same arithmetic as
 // lb_k_vcs_z; Z is left to lb_k_vcs_acc.
 #define LB_VU 4
-// LDS row stride (doubles) of the per-cluster rows in the VCS passes: two doubles of padding shift consecutive
-// cluster rows by 16 B, so slots working on different clusters spread over the banks
-#define LB_VPAD 2
 template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_pv(VcsArgs a) {
     typedef typename VT<TV>::TC TC;
@@ -853,7 +856,6 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
 
 // ------------------------------------------------------------------------------------------------ lb_k_finalize
 __global__ __launch_bounds__(1024) void lb_k_finalize(FinArgs a) {
-    __shared__ double red[16];
     double quad = 0, gv = 0, ttot = 0, gmin = INFINITY, vtot = 0, vviol = 0, ltot = 0, lviol = 0;
     // per-thread strided partial sums, combined in a fixed order
     for (int b = threadIdx.x; b < a.nrow_blk; b += 1024) {
@@ -867,14 +869,26 @@ __global__ __launch_bounds__(1024) void lb_k_finalize(FinArgs a) {
         vtot += a.vcs_part[(int64_t)b * a.vcs_width]; vviol += a.vcs_part[(int64_t)b * a.vcs_width + 1];
     }
     for (int b = threadIdx.x; b < a.sumPpair; b += 1024) { ltot += a.lev_tot[b]; lviol += a.lev_viol[b]; }
-    quad = block_sum(quad, red); __syncthreads();
-    gv = block_sum(gv, red); __syncthreads();
-    ttot = block_sum(ttot, red); __syncthreads();
-    gmin = block_min(gmin, red); __syncthreads();
-    vtot = block_sum(vtot, red); __syncthreads();
-    vviol = block_sum(vviol, red); __syncthreads();
-    ltot = block_sum(ltot, red); __syncthreads();
-    lviol = block_sum(lviol, red);
+    // the eight block reductions in one pass: each value keeps block_sum's (block_min's) wave butterfly and its
+    // fixed order over the waves
+    __shared__ double red8[16][8];
+    {
+        double v8[8] = {quad, gv, ttot, gmin, vtot, vviol, ltot, lviol};
+#pragma unroll
+        for (int j = 0; j < 8; j++) v8[j] = j == 3 ? wave_min(v8[j]) : wave_sum(v8[j]);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0)
+#pragma unroll
+            for (int j = 0; j < 8; j++) red8[w][j] = v8[j];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t8[8] = {0, 0, 0, INFINITY, 0, 0, 0, 0};
+            for (int k = 0; k < (int)(blockDim.x >> 6); k++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) t8[j] = j == 3 ? fmin(t8[j], red8[k][j]) : t8[j] + red8[k][j];
+            quad = t8[0]; gv = t8[1]; ttot = t8[2]; gmin = t8[3]; vtot = t8[4]; vviol = t8[5]; ltot = t8[6]; lviol = t8[7];
+        }
+    }
     if (threadIdx.x == 0) {
         double zz = 0.0;
         for (int c = 0; c < a.R; c++) zz += a.z[c] * a.z[c];
@@ -2368,7 +2382,7 @@ struct Launch {
         for (int m = 0; m < c->nvcs; m++) { ca.ls.off[c->nlev + m] = off; ca.ls.lab[c->nlev + m] = (const int32_t *)c->vlabs.p + (int64_t)m * n; off += c->vP[m]; }
         ca.ls.off[ca.ls.nsets] = off;
         ca.sumP = sumPall;
-        const size_t ldsY = sizeof(double) * (size_t)sumPall * R;
+        const size_t ldsY = sizeof(double) * (size_t)sumPall * (R + LB_VPAD);
         ca.use_lds = ldsY <= 48 * 1024;
         ca.part = (double *)c->colsum_part.p;
         ca.Yglob = (double *)c->Y.p + R;
@@ -2918,7 +2932,7 @@ int ensure_eval_buffers(lb_ctx *c) {
     const int sumPall = c->sumPpair + c->sumPv;
     int rc;
     if ((rc = dalloc(c, c->Y, sizeof(double) * (size_t)(R + sumPall * R)))) return rc;
-    const bool ylds = sizeof(double) * (size_t)sumPall * R <= 48 * 1024;
+    const bool ylds = sizeof(double) * (size_t)sumPall * (R + LB_VPAD) <= 48 * 1024;   // as lb_k_colsum's launch
     if ((rc = dalloc(c, c->colsum_part, sizeof(double) * (size_t)LB_RED_BLOCKS * (R + (ylds ? sumPall * R : 0))))) return rc;
     if ((rc = dalloc(c, c->M, sizeof(double) * (size_t)std::max(c->sumPpair, 1) * R))) return rc;
     if ((rc = dalloc(c, c->pdiag, sizeof(double) * std::max(c->sumPpair, 1)))) return rc;
