@@ -51,7 +51,7 @@
 #define LB_MAXVCS 4
 #define LB_RED_BLOCKS 512
 #ifndef LB_VCS_BLOCKS
-#define LB_VCS_BLOCKS 2048
+#define LB_VCS_BLOCKS 1024
 #endif
 #define LB_BLOCK 256
 
