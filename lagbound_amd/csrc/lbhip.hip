@@ -622,29 +622,60 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_segs(SegArgs a) {
 #pragma unroll
         for (int c = 0; c < CPL; c++) acc[c] = 0;
         if (s.kind == 0) {
+            // a slot's items are e = b + ws + k NSW; LB_EB of them gathered together, accumulated in item order
+            // (items past the segment read the hub row with weight 0: an exact zero)
             const TC *wt = (const TC *)a.w;
-            for (int64_t e = s.b + ws; e < s.e; e += NSW) {
-                const int64_t j = a.col[e];
-                TC xj[CPL];
-                VT<TV>::unpack(a.V[j * S + q], xj);
-                const TC w = wt ? wt[e] : (TC)1;
+            for (int64_t e0 = s.b + ws; e0 < s.e; e0 += (int64_t)LB_EB * NSW) {
+                int64_t jj[LB_EB];
+                TC ww[LB_EB];
 #pragma unroll
-                for (int c = 0; c < CPL; c++) acc[c] += w * xj[c];
+                for (int t = 0; t < LB_EB; t++) {
+                    const int64_t e = e0 + (int64_t)t * NSW;
+                    const bool in = e < s.e;
+                    jj[t] = in ? (int64_t)a.col[e] : (int64_t)s.row;
+                    ww[t] = in ? (wt ? wt[e] : (TC)1) : (TC)0;
+                }
+                uint4 u[LB_EB];
+#pragma unroll
+                for (int t = 0; t < LB_EB; t++) u[t] = a.V[jj[t] * S + q];
+#pragma unroll
+                for (int t = 0; t < LB_EB; t++) {
+                    TC xj[CPL];
+                    VT<TV>::unpack(u[t], xj);
+#pragma unroll
+                    for (int c = 0; c < CPL; c++) acc[c] += ww[t] * xj[c];
+                }
             }
         } else {
             TC x[CPL];
             VT<TV>::unpack(a.V[(int64_t)s.row * S + q], x);
             const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
             TC *pt = (TC *)a.pt;
-            for (int64_t k = s.b + ws; k < s.e; k += NSW) {
-                const TriRec<TC> rc = rec[k];
-                TC p, tau;
-                tri_record<TV, R>(a.V, rc, x, q, a.rho, a.rt, acc, p, tau);
-                if (rc.code % 3 == 0 && q == 0) {
-                    pt[rc.code / 3] = p;
-                    if (a.rho > 0.0) {
-                        tot += ((double)p * p - (double)rc.lam * rc.lam) / (2 * a.rt);
-                        gmin = fmin(gmin, (double)((TC)1 - tau));
+            // LB_RB records of the slot (strided by NSW) with their two rows gathered together, in record order
+            for (int64_t k0 = s.b + ws; k0 < s.e; k0 += (int64_t)LB_RB * NSW) {
+                TriRec<TC> rq[LB_RB];
+#pragma unroll
+                for (int t = 0; t < LB_RB; t++) {
+                    const int64_t k = k0 + (int64_t)t * NSW;
+                    rq[t] = rec[k < s.e ? k : k0];
+                }
+                uint4 uu[LB_RB], uw[LB_RB];
+#pragma unroll
+                for (int t = 0; t < LB_RB; t++) { uu[t] = a.V[(int64_t)rq[t].u * S + q]; uw[t] = a.V[(int64_t)rq[t].w * S + q]; }
+#pragma unroll
+                for (int t = 0; t < LB_RB; t++) {
+                    if (k0 + (int64_t)t * NSW >= s.e) break;     // slot-uniform
+                    const TriRec<TC> &rc = rq[t];
+                    TC xu[CPL], xw[CPL], p, tau;
+                    VT<TV>::unpack(uu[t], xu);
+                    VT<TV>::unpack(uw[t], xw);
+                    tri_record_x<TV, R>(rc.code, rc.lam, x, xu, xw, a.rho, a.rt, acc, p, tau);
+                    if (rc.code % 3 == 0 && q == 0) {
+                        pt[rc.code / 3] = p;
+                        if (a.rho > 0.0) {
+                            tot += ((double)p * p - (double)rc.lam * rc.lam) / (2 * a.rt);
+                            gmin = fmin(gmin, (double)((TC)1 - tau));
+                        }
                     }
                 }
             }
