@@ -2171,6 +2171,8 @@ struct lb_ctx {
     // device buffers
     DBuf vrow;                        // VCS working-set entry -> its row (entry-parallel VCS passes)
     DBuf vst_part;                    // per-workgroup [Z | cnt] partials of the sweeps' VCS state pass
+    DBuf b_light, b_heavy;            // batch rows by work: one slot per row / one wavefront per row
+    int64_t b_nlight = 0, b_nheavy = 0; bool b_lists_ok = false;
     DBuf pol_V, pol_Vn, pol_R, pol_g, pol_part, pol_sum;   // device RGD polish state (n x r FP64, n, partials)
     bool pol_ok = false;
     DBuf rowptr, col, w, d, labs, sig, pp, rq, dsq, vlabs, vrq, vptr, vcol, vsig, pv, TI, lam, pt, iptr, rec,
@@ -3068,7 +3070,7 @@ void lb_destroy(lb_ctx *c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     DBuf *bufs[] = {&c->rowptr, &c->col, &c->w, &c->d, &c->labs, &c->sig, &c->pp, &c->rq, &c->dsq, &c->vlabs, &c->vrq,
                     &c->vptr, &c->vcol, &c->vsig, &c->pv, &c->vrow, &c->pol_V, &c->pol_Vn, &c->pol_R, &c->pol_g,
-                    &c->pol_part, &c->pol_sum, &c->vst_part, &c->sw.hErows, &c->sw.hEseg0, &c->sw.hTrows, &c->sw.hTseg0,
+                    &c->pol_part, &c->pol_sum, &c->vst_part, &c->b_light, &c->b_heavy, &c->sw.hErows, &c->sw.hEseg0, &c->sw.hTrows, &c->sw.hTseg0,
                     &c->sw.segs, &c->sw.segout, &c->sw.segkap, &c->TI, &c->lam, &c->pt, &c->iptr, &c->rec, &c->V, &c->G,
                     &c->Y, &c->z, &c->M, &c->pdiag, &c->levtot, &c->levviol, &c->Z, &c->colsum_part, &c->vcs_part,
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
@@ -3123,6 +3125,7 @@ int lb_set_graph(lb_ctx *c, int64_t n, const int64_t *rowptr, const int32_t *col
     c->n = n; c->nnz = nnz; c->L = L; c->C0 = C0; c->weighted = weights != nullptr;
     c->bG = 0; c->blev = false; c->bsumP = 0;
     c->row_order.clear();
+    c->b_lists_ok = false;
     c->h_rowptr.assign(rowptr, rowptr + n + 1);
     c->h_d.assign(d, d + n);
     int rc;
@@ -3290,7 +3293,7 @@ int lb_set_vcs_sig(lb_ctx *c, const double *sig) {
 
 int lb_set_triangles(lb_ctx *c, int64_t T, const int32_t *TI, const double *lam) {
     drop_capture(c);
-    if (c) c->b_lamsum_ok = false;
+    if (c) { c->b_lamsum_ok = false; c->b_lists_ok = false; }
     if (!c || !c->n) return fail(c, -1, "lb_set_triangles: set the graph first");
     if (T < 0 || T > ((int64_t)1 << 29)) return fail(c, -1, "lb_set_triangles: bad T");
     HIPCHK(c, hipSetDevice(c->device));
@@ -4185,7 +4188,12 @@ struct BArgs {
     const int64_t *iptr; const void *rec; void *pt; double rho, rt; int64_t T; const int32_t *TI; const void *lam;
     void *Gout; double *rowv; double *parts; double *lamsum;
     const int64_t *tptr; const int32_t *tidx;   // triangles of each graph (CSR over graphs)
+    const int32_t *light; int64_t nlight;       // rows with few edges and records: one slot per row
+    const int32_t *heavy; int64_t nheavy;       // the others: one wavefront per row
 };
+
+// rows with at most this many edges and triangle records run one slot per row in a batch
+#define LB_BLIGHT 128
 
 // z_g and the cluster sums of graph g = blockIdx.x (Y in LDS when it fits, else global atomics)
 template <typename TV, int R>
@@ -4273,6 +4281,44 @@ __global__ __launch_bounds__(64) void lb_k_blevels(BArgs a, int R) {
     }
 }
 
+// the end of a batch row: G_i = (A V)_i + triangle terms - d_i z_g / L_g + level terms, rowv[4 i ..] =
+// (V_i.(AV)_i, G_i.V_i, triangle PHR term, min 1 - tau); written by the lanes with `writer` set
+template <typename TV, int R>
+__device__ inline void brow_finish(const BArgs &a, int64_t i, int q, bool writer, const typename VT<TV>::TC *x,
+                                   const typename VT<TV>::TC *ga, const typename VT<TV>::TC *gt, double ttot,
+                                   double tmin) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL;
+    const int g = a.gid[i];
+    const double di = a.d[i], Lg = a.Lg[g];
+    TC qd = 0, g2[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; c++) qd += x[c] * ga[c];
+    qd = slot_sum<S>(qd);
+#pragma unroll
+    for (int c = 0; c < CPL; c++) g2[c] = ga[c] + gt[c] - (TC)(di * a.zY[(int64_t)g * R + q * CPL + c] / Lg);
+    if (a.lev) {
+        const int Qg = a.Poff[g] + a.lab[i];
+        const double pd = a.pdiag[Qg] * di;
+        const double *M = a.M + (int64_t)Qg * R + q * CPL;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) g2[c] += (TC)(di * (M[c] - pd * (double)x[c]));
+    }
+    TC gd = 0;
+#pragma unroll
+    for (int c = 0; c < CPL; c++) gd += g2[c] * x[c];
+    gd = slot_sum<S>(gd);
+    if (writer) {
+        TC *Go = (TC *)a.Gout + i * R + q * CPL;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) Go[c] = g2[c];
+        if (q == 0) {
+            double *o = a.rowv + 4 * i;
+            o[0] = (double)qd; o[1] = (double)gd; o[2] = ttot; o[3] = tmin;
+        }
+    }
+}
+
 // one wavefront per row: NG = 64 / S groups of S lanes take the row's edges and triangle records NG at a time,
 // then an xor-shuffle sums the groups; rowv[4 i ..] = (V_i.(AV)_i, G_i.V_i, triangle PHR term, min 1 - tau)
 template <typename TV, int R>
@@ -4285,7 +4331,8 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_brows(BArgs a) {
     const TC *wt = (const TC *)a.w;
     const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
     TC *pt = (TC *)a.pt;
-    for (int64_t i = w0; i < a.n; i += nw) {
+    for (int64_t w = w0; w < a.nheavy; w += nw) {
+        const int64_t i = a.heavy[w];
         TC x[CPL], ga[CPL], gt[CPL];
         VT<TV>::unpack(a.V[i * S + q], x);
 #pragma unroll
@@ -4350,35 +4397,75 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_brows(BArgs a) {
             ttot += __shfl_xor(ttot, m);
             tmin = fmin(tmin, __shfl_xor(tmin, m));
         }
-        // every group now holds the sums; group 0 finishes the row
-        const int g = a.gid[i];
-        const double di = a.d[i], Lg = a.Lg[g];
-        TC qd = 0, g2[CPL];
+        // every group now holds the sums; group 0 writes the row
+        brow_finish<TV, R>(a, i, q, grp == 0, x, ga, gt, ttot, tmin);
+    }
+}
+
+// one slot of S lanes per light row (LB_BLIGHT): its edges LB_EB and its triangle records LB_RB at a time, in order
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_brows_slot(BArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
+    const int slot = threadIdx.x / S, q = threadIdx.x % S;
+    const TC *wt = (const TC *)a.w;
+    const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
+    TC *pt = (TC *)a.pt;
+    for (int64_t p = (int64_t)blockIdx.x * NSL + slot; p < a.nlight; p += (int64_t)gridDim.x * NSL) {
+        const int64_t i = a.light[p];
+        TC x[CPL], ga[CPL], gt[CPL];
+        VT<TV>::unpack(a.V[i * S + q], x);
 #pragma unroll
-        for (int c = 0; c < CPL; c++) qd += x[c] * ga[c];
-        qd = slot_sum<S>(qd);
+        for (int c = 0; c < CPL; c++) { ga[c] = 0; gt[c] = 0; }
+        const int64_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+        for (int64_t e = e0; e < e1; e += LB_EB) {
+            const int64_t nE = e1 - e;
+            int64_t jj[LB_EB];
+            TC ww[LB_EB];
 #pragma unroll
-        for (int c = 0; c < CPL; c++) g2[c] = ga[c] + gt[c] - (TC)(di * a.zY[(int64_t)g * R + q * CPL + c] / Lg);
-        if (a.lev) {
-            const int Qg = a.Poff[g] + a.lab[i];
-            const double pd = a.pdiag[Qg] * di;
-            const double *M = a.M + (int64_t)Qg * R + q * CPL;
+            for (int t = 0; t < LB_EB; t++) {
+                const int64_t ec = t < nE ? e + t : e1 - 1;
+                jj[t] = t < nE ? (int64_t)a.col[ec] : i;
+                ww[t] = t < nE ? (wt ? wt[ec] : (TC)1) : (TC)0;
+            }
+            uint4 u[LB_EB];
 #pragma unroll
-            for (int c = 0; c < CPL; c++) g2[c] += (TC)(di * (M[c] - pd * (double)x[c]));
-        }
-        TC gd = 0;
+            for (int t = 0; t < LB_EB; t++) u[t] = a.V[jj[t] * S + q];
 #pragma unroll
-        for (int c = 0; c < CPL; c++) gd += g2[c] * x[c];
-        gd = slot_sum<S>(gd);
-        if (grp == 0) {
-            TC *Go = (TC *)a.Gout + i * R + q * CPL;
+            for (int t = 0; t < LB_EB; t++) {
+                TC xj[CPL];
+                VT<TV>::unpack(u[t], xj);
 #pragma unroll
-            for (int c = 0; c < CPL; c++) Go[c] = g2[c];
-            if (q == 0) {
-                double *o = a.rowv + 4 * i;
-                o[0] = (double)qd; o[1] = (double)gd; o[2] = ttot; o[3] = tmin;
+                for (int c = 0; c < CPL; c++) ga[c] += ww[t] * xj[c];
             }
         }
+        double ttot = 0.0, tmin = INFINITY;
+        const int64_t k0 = a.iptr[i], k1 = a.iptr[i + 1];
+        for (int64_t k = k0; k < k1; k += LB_RB) {
+            const int64_t nK = k1 - k;
+            TriRec<TC> rc[LB_RB];
+#pragma unroll
+            for (int t = 0; t < LB_RB; t++) rc[t] = rec[t < nK ? k + t : k1 - 1];
+            uint4 uu[LB_RB], uw[LB_RB];
+#pragma unroll
+            for (int t = 0; t < LB_RB; t++) { uu[t] = a.V[(int64_t)rc[t].u * S + q]; uw[t] = a.V[(int64_t)rc[t].w * S + q]; }
+#pragma unroll
+            for (int t = 0; t < LB_RB; t++) {
+                if (t >= nK) break;                            // slot-uniform
+                TC xu[CPL], xw[CPL], p, tau;
+                VT<TV>::unpack(uu[t], xu);
+                VT<TV>::unpack(uw[t], xw);
+                tri_record_x<TV, R>(rc[t].code, rc[t].lam, x, xu, xw, a.rho, a.rt, gt, p, tau);
+                if (rc[t].code % 3 == 0 && q == 0) {
+                    pt[rc[t].code / 3] = p;
+                    if (a.rho > 0.0) {
+                        ttot += ((double)p * p - (double)rc[t].lam * rc[t].lam) / (2 * a.rt);
+                        tmin = fmin(tmin, (double)((TC)1 - tau));
+                    }
+                }
+            }
+        }
+        brow_finish<TV, R>(a, i, q, true, x, ga, gt, ttot, tmin);
     }
 }
 
@@ -4454,8 +4541,15 @@ struct BLaunch {
         a.ylds = lds > 0 ? 1 : 0;
         hipLaunchKernelGGL((lb_k_bcolsum<TV, R>), dim3(c->bG), dim3(LB_BLOCK), lds, s, a);
         if (c->blev && c->bsumP) hipLaunchKernelGGL(lb_k_blevels, dim3(c->bsumP), dim3(64), 0, s, a, R);
-        const int64_t nblk = std::min<int64_t>((c->n + 3) / 4, 8192);
-        hipLaunchKernelGGL((lb_k_brows<TV, R>), dim3((unsigned)nblk), dim3(LB_BLOCK), 0, s, a);
+        constexpr int NSLb = LB_BLOCK / (R / VT<TV>::CPL);   // slots per workgroup of the light-row kernel
+        if (a.nlight) {
+            const int64_t lblk = std::min<int64_t>((a.nlight + NSLb - 1) / NSLb, 8192);
+            hipLaunchKernelGGL((lb_k_brows_slot<TV, R>), dim3((unsigned)lblk), dim3(LB_BLOCK), 0, s, a);
+        }
+        if (a.nheavy) {
+            const int64_t nblk = std::min<int64_t>((a.nheavy + 3) / 4, 8192);
+            hipLaunchKernelGGL((lb_k_brows<TV, R>), dim3((unsigned)nblk), dim3(LB_BLOCK), 0, s, a);
+        }
         hipLaunchKernelGGL(lb_k_bfinal, dim3(c->bG), dim3(LB_BLOCK), 0, s, a, R);
         HIPCHK(c, hipGetLastError());
         return 0;
@@ -4463,6 +4557,28 @@ struct BLaunch {
 };
 
 typedef int (*beval_fn)(lb_ctx *, double, double, const BArgs &);
+
+// light / heavy row lists of a batch context (rebuilt after lb_batch_set or lb_set_triangles)
+int ensure_batch_lists(lb_ctx *c) {
+    if (c->b_lists_ok) return 0;
+    const int64_t n = c->n;
+    const bool has_t = c->h_iptr.size() == (size_t)n + 1;
+    std::vector<int32_t> L, H;
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t de = c->h_rowptr[i + 1] - c->h_rowptr[i];
+        const int64_t dt = has_t ? c->h_iptr[i + 1] - c->h_iptr[i] : 0;
+        (de <= LB_BLIGHT && dt <= LB_BLIGHT ? L : H).push_back((int32_t)i);
+    }
+    int rc;
+    if ((rc = dalloc(c, c->b_light, sizeof(int32_t) * std::max<size_t>(L.size(), 1))) ||
+        (rc = dalloc(c, c->b_heavy, sizeof(int32_t) * std::max<size_t>(H.size(), 1))))
+        return rc;
+    if (!L.empty() && (rc = upload(c, c->b_light, L.data(), sizeof(int32_t) * L.size()))) return rc;
+    if (!H.empty() && (rc = upload(c, c->b_heavy, H.data(), sizeof(int32_t) * H.size()))) return rc;
+    c->b_nlight = (int64_t)L.size(); c->b_nheavy = (int64_t)H.size();
+    c->b_lists_ok = true;
+    return 0;
+}
 beval_fn pick_beval(int vdtype, int r) {
 #define LB_PICK(TV)                                     \
     if (r == 8) return &BLaunch<TV, 8>::eval;           \
@@ -4481,7 +4597,7 @@ extern "C" {
 
 int lb_batch_set(lb_ctx *c, int G, const int64_t *row_off, const double *L, const double *C0) {
     drop_capture(c);
-    if (c) c->b_lamsum_ok = false;
+    if (c) { c->b_lamsum_ok = false; c->b_lists_ok = false; }
     if (!c || !c->n) return fail(c, -1, "lb_batch_set: set the graph (the union of the batch) first");
     if (G < 1 || !row_off || !L || !C0) return fail(c, -1, "lb_batch_set: bad arguments");
     if (row_off[0] != 0 || row_off[G] != c->n) return fail(c, -1, "lb_batch_set: row offsets must run from 0 to n");
@@ -4561,6 +4677,8 @@ int lb_batch_eval(lb_ctx *c, double rho, double rt, double *parts) {
     beval_fn f = pick_beval(c->vdtype, c->r);
     if (!f) return fail(c, -1, "lb_batch_eval: unsupported rank / dtype");
     if (c->T && !c->b_tptr.p) return fail(c, -1, "lb_batch_eval: triangles were set before lb_batch_set");
+    int rcl;
+    if ((rcl = ensure_batch_lists(c))) return rcl;
     BArgs a{};
     a.V = (const uint4 *)c->V.p; a.n = c->n; a.rowptr = (const int64_t *)c->rowptr.p; a.col = (const int32_t *)c->col.p;
     a.w = c->weighted ? c->w.p : nullptr; a.d = (const double *)c->d.p;
@@ -4576,6 +4694,8 @@ int lb_batch_eval(lb_ctx *c, double rho, double rt, double *parts) {
     a.lam = c->lam.p;
     a.Gout = c->G.p; a.rowv = (double *)c->b_rowv.p; a.parts = (double *)c->b_parts.p; a.lamsum = (double *)c->b_lamsum.p;
             a.tptr = (const int64_t *)c->b_tptr.p; a.tidx = (const int32_t *)c->b_tidx.p;
+    a.light = (const int32_t *)c->b_light.p; a.nlight = c->b_nlight;
+    a.heavy = (const int32_t *)c->b_heavy.p; a.nheavy = c->b_nheavy;
     int rc = f(c, rho, rt, a);
     if (rc) return rc;
     if (parts) {
@@ -4598,6 +4718,10 @@ int lb_capture_eval(lb_ctx *c, double rho, double rt) {
     if (!(rho >= 0.0) || (rho > 0.0 && c->T && !(rt > 0.0))) return fail(c, -1, "lb_capture_eval: need rho >= 0 and rt > 0 when rho > 0");
     HIPCHK(c, hipSetDevice(c->device));
     if (c->gexec) { hipGraphExecDestroy((hipGraphExec_t)c->gexec); c->gexec = nullptr; }
+    if (c->bG) {                                       // the batch row lists are uploaded before the capture
+        int rcl;
+        if ((rcl = ensure_batch_lists(c))) return rcl;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
@@ -4621,6 +4745,8 @@ int lb_capture_eval(lb_ctx *c, double rho, double rt) {
             a.lam = c->lam.p;
             a.Gout = c->G.p; a.rowv = (double *)c->b_rowv.p; a.parts = (double *)c->b_parts.p; a.lamsum = (double *)c->b_lamsum.p;
             a.tptr = (const int64_t *)c->b_tptr.p; a.tidx = (const int32_t *)c->b_tidx.p;
+            a.light = (const int32_t *)c->b_light.p; a.nlight = c->b_nlight;
+            a.heavy = (const int32_t *)c->b_heavy.p; a.nheavy = c->b_nheavy;
             rc = f(c, rho, rt, a);
         }
     } else {
