@@ -4193,7 +4193,9 @@ struct BArgs {
 };
 
 // rows with at most this many edges and triangle records run one slot per row in a batch
+#ifndef LB_BLIGHT
 #define LB_BLIGHT 128
+#endif
 
 // z_g and the cluster sums of graph g = blockIdx.x (Y in LDS when it fits, else global atomics)
 template <typename TV, int R>

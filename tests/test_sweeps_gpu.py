@@ -11,11 +11,13 @@ from conftest import small_instance, chung_lu_instance
 pytestmark = pytest.mark.gpu
 
 
-def _run(inst, V, dtype, r, orders, nbatch, rho, damp, tuning=None):
+def _run(inst, V, dtype, r, orders, nbatch, rho, damp, tuning=None, options=None):
     from lagbound_amd.engine import Engine
     eng = Engine(r=r, dtype=dtype)
     if tuning:
         eng.set_tuning(*tuning)
+    for k, v in (options or {}).items():
+        eng.set_option(k, v)
     eng.set_instance(inst)
     eng.set_V(V)
     V0 = eng.get_V()
@@ -50,10 +52,12 @@ def test_chung_lu_sweeps_with_hubs(dtype, tol):
 
 
 def test_segments_forced_small():
+    """Tiny hub thresholds for the evaluation's tables and for the sweeps' own (rows above 2 edges / records split
+    into 3-item segments), so the segment paths of both run on a small weighted instance."""
     inst, V = small_instance(n=50, seed=6, k=8, weighted=True, ntri=80)
     rng = np.random.default_rng(7)
     orders = np.stack([rng.permutation(inst.n) for _ in range(2)])
-    V0, W = _run(inst, V, "f64", 8, orders, 5, 0.5, 0.2, tuning=(2, 2, 3))
+    V0, W = _run(inst, V, "f64", 8, orders, 5, 0.5, 0.2, tuning=(2, 2, 3), options={"sweep_heavy": 2, "sweep_seg": 3})
     Wo = O.sweeps_c(inst, V0, orders, 5, 0.5, damp=0.2)
     assert np.abs(W - Wo).max() <= 1e-10
 
