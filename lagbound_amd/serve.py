@@ -5,17 +5,22 @@ separation, PHR updates; lagbound_amd.alm.Solver in process) -> candidate bound 
 assembly; lagbound_amd.bound.estimate) -> dense FP64 certificate (assembly and Cholesky on the device) when the
 graph fits it.
 
-Concurrency: one host thread per slot, each with its own device contexts (own HIP stream), so the work of requests
-that are in flight at the same time overlaps on the GPU; the library's calls release the GIL (ctypes). Admission
-is by projected HBM footprint: a request waits until the sum of the footprints in flight plus its own fits the
-budget. Every computation of a request depends only on its own inputs and seeds, so its result is the same alone
-and in the mix (up to the summation order of floating-point atomics inside the device reductions).
+Concurrency: one worker per slot, each with its own device contexts (own HIP stream), so the work of requests that
+are in flight at the same time overlaps on the GPU. By default a worker is a process of its own (spawned and warmed
+up before the stream starts; one request at a time, so the device library is called from one thread per process and
+the host parts of concurrent requests -- LOBPCG, merges, label propagation -- do not share one interpreter lock).
+`workers="thread"` keeps the slots as threads of this process (the library's calls release the GIL). Admission
+is by projected HBM footprint, in this process: a request waits until the sum of the footprints in flight plus its
+own fits the budget. Every computation of a request depends only on its own inputs and seeds, so its result is the
+same alone and in the mix (up to the summation order of floating-point atomics inside the device reductions).
 
 No reference counterpart: the reference computes one bound per process (api.upper_bound, src/lagbound/api.py).
 """
 from __future__ import annotations
 
 import math
+import multiprocessing as mp
+import os
 import queue
 import threading
 import time
@@ -56,6 +61,7 @@ class Params:
     cert_margin: float = 1e-7
     cert_on_device: bool = True
     blas_threads: int = 2            # host BLAS threads per slot while serving (slots x threads ~ the host cores)
+    hw_queues: int = 1               # process slots: GPU_MAX_HW_QUEUES of each worker (0: the runtime's default)
 
 
 @dataclass
@@ -134,13 +140,97 @@ def solve_request(req: Request, p: Params, device=0):
                 log=logs), t_dev
 
 
-class Server:
-    """Continuous batching: `slots` worker threads, HBM-footprint admission against `budget_bytes`."""
+def prepare_device(p: Params, device=0):
+    """One tiny device certificate, so rocBLAS / rocSOLVER are opened (dlopen, kernel registration, first handle)
+    before any request runs and never while other slots launch kernels."""
+    if not (p.cert_on_device and p.cert_max_n > 0):
+        return
+    keys = np.array([0 * 3 + 1, 0 * 3 + 2, 1 * 3 + 2], np.int64)          # a triangle
+    tiny = I.build_instance(3, keys)
+    B.certify_dense_device(tiny, np.zeros(3), np.ones(3), device=device)
 
-    def __init__(self, p: Params, slots=8, budget_bytes=None, device=0, progress=None):
+
+def _worker_process(conn, p: Params, device: int):
+    """A slot's process: limit host BLAS threads, open the device and the solver libraries, report ready, then solve
+    one request per message until None arrives. Replies ("ok", (out, device seconds)) or ("error", text)."""
+    try:
+        if p.hw_queues > 0:                 # read by the HIP runtime when it initialises, which has not happened yet
+            os.environ["GPU_MAX_HW_QUEUES"] = str(int(p.hw_queues))
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(limits=p.blas_threads)
+        prepare_device(p, device)
+        conn.send(("ready", None))
+    except Exception as exc:
+        conn.send(("error", repr(exc)))
+        return
+    while True:
+        req = conn.recv()
+        if req is None:
+            return
+        try:
+            conn.send(("ok", solve_request(req, p, device)))
+        except Exception as exc:
+            conn.send(("error", repr(exc)))
+
+
+class _ProcessSlots:
+    """`count` spawned worker processes, started and warmed up together; `call(k, req)` runs a request on slot k."""
+
+    def __init__(self, count, p: Params, device: int, start_timeout=600.0):
+        ctx = mp.get_context("spawn")
+        self.conns, self.procs = [], []
+        for _ in range(count):
+            a, b = ctx.Pipe()
+            pr = ctx.Process(target=_worker_process, args=(b, p, device), daemon=True)
+            pr.start()
+            b.close()
+            self.conns.append(a)
+            self.procs.append(pr)
+        try:
+            for k, c in enumerate(self.conns):
+                if not c.poll(start_timeout):
+                    raise RuntimeError(f"serving slot {k} did not start within {start_timeout:.0f} s")
+                st, msg = c.recv()
+                if st != "ready":
+                    raise RuntimeError(f"serving slot {k} failed to start: {msg}")
+        except BaseException:
+            self.close()
+            raise
+
+    def call(self, k, req):
+        self.conns[k].send(req)
+        st, payload = self.conns[k].recv()
+        if st != "ok":
+            raise RuntimeError(payload)
+        return payload
+
+    def close(self):
+        for c in self.conns:
+            try:
+                c.send(None)
+            except (OSError, BrokenPipeError):
+                pass
+        for pr in self.procs:
+            pr.join(timeout=30)
+            if pr.is_alive():
+                pr.terminate()
+                pr.join(timeout=10)
+        for c in self.conns:
+            c.close()
+        self.conns, self.procs = [], []
+
+
+class Server:
+    """Continuous batching: `slots` workers (processes by default, or threads), HBM-footprint admission against
+    `budget_bytes`."""
+
+    def __init__(self, p: Params, slots=8, budget_bytes=None, device=0, progress=None, workers="process"):
+        if workers not in ("process", "thread"):
+            raise ValueError(f"workers must be 'process' or 'thread', not {workers!r}")
         self.progress = progress                 # callable(Result) after each completed request
         self.p = p
         self.slots = slots
+        self.workers = workers
         self.device = device
         self.budget = int(budget_bytes) if budget_bytes is not None else 64 << 30   # of the 288 GB of HBM3E
         self.used = 0
@@ -160,26 +250,27 @@ class Server:
             self.cv.notify_all()
 
     def run(self, requests, realtime=True):
-        """Serve `requests` (sorted by arrival). Returns {rid: Result}."""
-        from threadpoolctl import threadpool_limits
-        with threadpool_limits(limits=self.p.blas_threads):
-            return self._run(requests, realtime)
+        """Serve `requests` (sorted by arrival). Returns {rid: Result}. Process slots are started and warmed up
+        before the stream's clock starts and stopped when it ends."""
+        self.start_s = 0.0
+        if self.workers == "thread":
+            from threadpoolctl import threadpool_limits
+            with threadpool_limits(limits=self.p.blas_threads):
+                prepare_device(self.p, self.device)
+                return self._run(requests, realtime, lambda k, req: solve_request(req, self.p, self.device))
+        t = time.perf_counter()
+        slots = _ProcessSlots(self.slots, self.p, self.device)
+        self.start_s = time.perf_counter() - t
+        try:
+            return self._run(requests, realtime, slots.call)
+        finally:
+            slots.close()
 
-    def _prepare_device(self):
-        """Before any worker starts: one tiny device certificate, so rocBLAS / rocSOLVER are opened (dlopen, kernel
-        registration, first handle) by this thread and never by a worker while others launch kernels."""
-        if not (self.p.cert_on_device and self.p.cert_max_n > 0):
-            return
-        keys = np.array([0 * 3 + 1, 0 * 3 + 2, 1 * 3 + 2], np.int64)          # a triangle
-        tiny = I.build_instance(3, keys)
-        B.certify_dense_device(tiny, np.zeros(3), np.ones(3), device=self.device)
-
-    def _run(self, requests, realtime):
-        self._prepare_device()
+    def _run(self, requests, realtime, solve):
         q = queue.Queue()
         t0 = time.perf_counter()
 
-        def worker():
+        def worker(k):
             while True:
                 req = q.get()
                 if req is None:
@@ -188,7 +279,7 @@ class Server:
                 self._admit(fp)
                 start = time.perf_counter() - t0
                 try:
-                    out, tdev = solve_request(req, self.p, self.device)
+                    out, tdev = solve(k, req)
                     res = Result(rid=req.rid, arrival=req.arrival, start=start, finish=time.perf_counter() - t0,
                                  device_s=tdev, **out)
                     self.results[req.rid] = res
@@ -199,7 +290,7 @@ class Server:
                 finally:
                     self._release(fp)
 
-        ths = [threading.Thread(target=worker, daemon=True) for _ in range(self.slots)]
+        ths = [threading.Thread(target=worker, args=(k,), daemon=True) for k in range(self.slots)]
         for t in ths:
             t.start()
         for req in requests:
@@ -262,7 +353,10 @@ def main(argv=None):
     ap.add_argument("--kbar_lo", type=float, default=6.0)
     ap.add_argument("--kbar_hi", type=float, default=30.0)
     ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--workers", choices=["process", "thread"], default="process",
+                    help="one process per slot (default) or one thread per slot")
     ap.add_argument("--blas_threads", type=int, default=None, help="host BLAS threads per slot (default: Params)")
+    ap.add_argument("--hw_queues", type=int, default=None, help="process slots: hardware queues per worker")
     ap.add_argument("--budget_gb", type=float, default=64.0)
     ap.add_argument("--outer", type=int, default=24)
     ap.add_argument("--seed", type=int, default=1)
@@ -273,17 +367,21 @@ def main(argv=None):
     p = Params(outer=a.outer)
     if a.blas_threads:
         p.blas_threads = a.blas_threads
-    t = time.perf_counter()
+    if a.hw_queues is not None:
+        p.hw_queues = a.hw_queues
+
     def progress(r):
         print(f"[serve] rid {r.rid} n {r.n} m {r.m} latency {r.latency:.3f}s service {r.finish - r.start:.3f}s "
               f"lb {r.lb:.5f} ub {r.ub:.5f} certified {r.certified}", file=sys.stderr, flush=True)
 
-    res = Server(p, slots=a.slots, budget_bytes=a.budget_gb * 2 ** 30, progress=progress).run(reqs)
-    wall = time.perf_counter() - t
+    srv = Server(p, slots=a.slots, budget_bytes=a.budget_gb * 2 ** 30, progress=progress, workers=a.workers)
+    res = srv.run(reqs)
+    wall = max(r.finish for r in res.values())
     out = summarize(res, wall)
     out.update(stream=dict(count=a.count, rate=a.rate, n=[a.n_lo, a.n_hi], kbar=[a.kbar_lo, a.kbar_hi], seed=a.seed,
-                           slots=a.slots, blas_threads=p.blas_threads, budget_gb=a.budget_gb, outer=a.outer,
-                           generate_s=round(gen, 2)))
+                           slots=a.slots, workers=a.workers, blas_threads=p.blas_threads, budget_gb=a.budget_gb,
+                           outer=a.outer, hw_queues=p.hw_queues, generate_s=round(gen, 2),
+                           slots_start_s=round(srv.start_s, 2)))
     print(json.dumps(out), flush=True)
     return 0
 

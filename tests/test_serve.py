@@ -1,5 +1,6 @@
-"""Serving loop host logic on the CPU (device engine replaced by the FP64 oracle): admission by footprint, every
-request answered, the same results alone and in the mix, latency summary fields."""
+"""Serving loop host logic on the CPU (device engine replaced by the FP64 oracle, thread slots): admission by
+footprint, every request answered, the same results alone and in the mix, latency summary fields; process slots
+start, report a request that fails in a worker, and stop."""
 import numpy as np
 import pytest
 
@@ -36,12 +37,12 @@ def test_alone_equals_mixed_and_admission(cpu_engine):
     OracleEngine.lib_s = 0.0
     reqs = serve.poisson_stream(5, 200.0, 150, 400, 6, 12, seed=3)
     p = _params()
-    mixed = serve.Server(p, slots=3).run(reqs, realtime=False)
+    mixed = serve.Server(p, slots=3, workers="thread").run(reqs, realtime=False)
     # a budget below any single footprint serialises the requests, but every one is still served
-    tight = serve.Server(p, slots=3, budget_bytes=1).run(reqs, realtime=False)
+    tight = serve.Server(p, slots=3, budget_bytes=1, workers="thread").run(reqs, realtime=False)
     assert sorted(mixed) == sorted(tight) == [r.rid for r in reqs]
     for r in reqs:
-        alone = serve.Server(p, slots=1).run([r], realtime=False)[r.rid]
+        alone = serve.Server(p, slots=1, workers="thread").run([r], realtime=False)[r.rid]
         for res in (mixed[r.rid], tight[r.rid]):
             assert res.ub == pytest.approx(alone.ub, rel=1e-12)
             assert res.dv == pytest.approx(alone.dv, rel=1e-12)
@@ -50,3 +51,14 @@ def test_alone_equals_mixed_and_admission(cpu_engine):
     summ = serve.summarize(mixed, wall=1.0)
     for key in ("latency_p50_s", "latency_p99_s", "solves_per_s", "gap_ub_minus_lb", "certified"):
         assert key in summ
+
+
+def test_process_slots_report_worker_errors():
+    """Process slots on a host without a device: both workers start (no certificate warm-up), each request fails
+    inside its worker (no device context can be created) and the failure reaches the caller."""
+    reqs = serve.poisson_stream(2, 200.0, 100, 200, 6, 8, seed=4)
+    p = serve.Params(k=8, outer=2, cert_max_n=0)
+    with pytest.raises(RuntimeError, match="2 request"):
+        serve.Server(p, slots=2).run(reqs, realtime=False)
+    with pytest.raises(ValueError):
+        serve.Server(p, workers="fiber")
