@@ -147,7 +147,11 @@ const char *lb_kernel_name(int k);
  * counterpart (the reference's parallel shapes are numba prange and chunk sizes, alm_vcs.py:1420-1423). */
 int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int row_grid);
 /* Named switches of the kernels (no reference counterpart): "row_order" (default 1) processes the rows of the f+grad
- * row kernel in descending order of their work instead of index order (same results up to summation order). */
+ * row kernel in descending order of their work instead of index order (same results up to summation order);
+ * "overlap" (default 1) runs the evaluation's hub-segment kernel on a second stream of the context, concurrently
+ * with the cluster-sum and VCS kernels, joined before the row kernel (identical results; lb_time_eval's per-kernel
+ * figures always come from the one-stream order); "sweep_heavy" / "sweep_seg" (defaults 32 / 128) are the hub
+ * threshold and segment length of the sweeps' own segment tables. */
 int lb_set_option(lb_ctx *ctx, const char *name, int64_t value);
 
 /* Device row work of the candidate bound's Riemannian gradient polish (polish_rgd, est_dual_vcs.py:276-308), with

@@ -291,17 +291,20 @@ __global__ void lb_k_fill(double *p, int64_t cnt, double v) {
 }
 
 // ------------------------------------------------------------------------------------------------ lb_k_levels
-// One wavefront per cluster row Q of a pair level (global row id = blockIdx.x).
+// One wavefront per cluster row Q of a pair level (global row id = blockIdx.x); the rank is a compile-time constant
+// so the per-lane M accumulator stays in registers.
+template <int R>
 __global__ __launch_bounds__(64) void lb_k_levels(LevelArgs a) {
     const int Qg = blockIdx.x;
     int l = 0;
     while (l + 1 < a.nlev && Qg >= a.Poff[l + 1]) l++;
-    const int P = a.Poff[l + 1] - a.Poff[l], Q = Qg - a.Poff[l], R = a.R;
+    const int P = a.Poff[l + 1] - a.Poff[l], Q = Qg - a.Poff[l];
     const double *YQ = a.Y + (int64_t)Qg * R;
     const double *sig = a.sig + a.soff[l] + (int64_t)Q * P;
     double *pp = a.pp + a.soff[l] + (int64_t)Q * P;
     const double rqQ = a.rq[Qg];
-    double Macc[32];
+    double Macc[R];
+#pragma unroll
     for (int c = 0; c < R; c++) Macc[c] = 0.0;
     double tot = 0.0, viol = 0.0, pq = 0.0;
     for (int Rr = threadIdx.x; Rr < P; Rr += 64) {
@@ -309,6 +312,7 @@ __global__ __launch_bounds__(64) void lb_k_levels(LevelArgs a) {
         double p = sig[Rr];
         if (a.rho > 0.0) {
             double h = 0.0;
+#pragma unroll
             for (int c = 0; c < R; c++) h += YQ[c] * YR[c];
             if (Rr == Q) h -= a.dsq[Qg];
             const double rr = a.rho * ((rqQ * a.rq[a.Poff[l] + Rr]) * a.w);
@@ -320,12 +324,16 @@ __global__ __launch_bounds__(64) void lb_k_levels(LevelArgs a) {
         }
         pp[Rr] = p;
         if (Rr == Q) pq = p;
-        if (p != 0.0)
+        if (p != 0.0) {
+#pragma unroll
             for (int c = 0; c < R; c++) Macc[c] += p * YR[c];
+        }
     }
     tot = wave_sum(tot); viol = wave_sum(viol); pq = wave_sum(pq);
+#pragma unroll
     for (int c = 0; c < R; c++) Macc[c] = wave_sum(Macc[c]);
     if (threadIdx.x == 0) {
+#pragma unroll
         for (int c = 0; c < R; c++) a.M[(int64_t)Qg * R + c] = Macc[c];
         a.pdiag[Qg] = pq; a.tot[Qg] = tot; a.viol[Qg] = viol;
     }
@@ -613,10 +621,12 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_segs(SegArgs a) {
     typedef typename VT<TV>::TC TC;
     constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSW = 64 / S;
     __shared__ double red[LB_BLOCK / 64];
-    const int64_t sg = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63, ws = lane / S, q = lane % S;
     double tot = 0.0, gmin = INFINITY;
-    if (sg < a.nseg) {
+    // one wavefront per segment; a grid smaller than nseg / 4 blocks strides over the segments (a fixed assignment,
+    // so the per-block partials do not depend on timing)
+    for (int64_t sg = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6); sg < a.nseg;
+         sg += (int64_t)gridDim.x * (LB_BLOCK / 64)) {
         const Seg s = a.segs[sg];
         TC acc[CPL];
 #pragma unroll
@@ -2146,6 +2156,12 @@ struct SweepSegs {
 struct lb_ctx {
     int device = 0, vdtype = LB_F32, r = 16;
     hipStream_t stream = nullptr;
+    // the evaluation's hub-segment kernel runs on `side` while the cluster-sum / VCS kernels run on `stream`
+    // (fork / join through the two events); option "overlap" 0 keeps the whole chain on `stream`
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    bool overlap = true;
+    int64_t seg_grid = 0;              // workgroups of the evaluation's hub-segment kernel (0: one per 4 segments)
     std::string err;
     size_t vbytes = 4, cbytes = 4;   // storage bytes of V elements, of TC values
     int heavy_edges = 128, heavy_tris = 128, seg_len = 1024;  // rows above these counts are split into segments
@@ -2432,6 +2448,24 @@ struct Launch {
         const int sumPall = c->sumPpair + c->sumPv;
         int ek = 0;
         auto mark = [&]() { if (ev) hipEventRecord(ev[ek++], c->stream); };
+        const int64_t nseg = (int64_t)(c->segE.size() + c->segT.size());
+        const int nseg_blk = (int)(c->seg_grid > 0 ? std::min<int64_t>((nseg + 3) / 4, c->seg_grid) : (nseg + 3) / 4);
+        auto launch_segs = [&](hipStream_t st) {
+            SegArgs sa{};
+            sa.V = (const uint4 *)c->V.p; sa.rowptr = (const int64_t *)c->rowptr.p; sa.col = (const int32_t *)c->col.p;
+            sa.w = c->weighted ? c->w.p : nullptr; sa.rec = c->rec.p; sa.segs = (const Seg *)c->segs.p; sa.nseg = nseg;
+            sa.segout = c->segout.p; sa.pt = c->pt.p; sa.rho = rho; sa.rt = rt; sa.tpart = (double *)c->seg_part.p;
+            hipLaunchKernelGGL((lb_k_segs<TV, R>), dim3(nseg_blk), dim3(LB_BLOCK), 0, st, sa);
+        };
+        // the hub segments read only V, the CSR and the triangle records, none of which the kernels before lb_k_rows
+        // write: they run on the side stream from the start (per-kernel timing keeps one stream)
+        const bool fork = nseg && c->overlap && !ev;
+        if (fork) {
+            HIPCHK(c, hipEventRecord(c->fork_ev, c->stream));
+            HIPCHK(c, hipStreamWaitEvent(c->side, c->fork_ev, 0));
+            launch_segs(c->side);
+            HIPCHK(c, hipEventRecord(c->join_ev, c->side));
+        }
         mark();
         zY(c, ev, &ek);
         mark();
@@ -2446,7 +2480,7 @@ struct Launch {
             la.Y = Yp; la.sig = (const double *)c->sig.p; la.pp = (double *)c->pp.p; la.rq = (const double *)c->rq.p;
             la.dsq = (const double *)c->dsq.p; la.w = c->pw; la.rho = rho;
             la.M = (double *)c->M.p; la.pdiag = (double *)c->pdiag.p; la.tot = (double *)c->levtot.p; la.viol = (double *)c->levviol.p;
-            hipLaunchKernelGGL(lb_k_levels, dim3(c->sumPpair), dim3(64), 0, c->stream, la);
+            hipLaunchKernelGGL(lb_k_levels<R>, dim3(c->sumPpair), dim3(64), 0, c->stream, la);
         }
         mark();
         // --- VCS multipliers and Z
@@ -2490,16 +2524,9 @@ struct Launch {
                                (const double *)c->vcs_part.p + 2, LB_VCS_BLOCKS, width, vcs_width, (double *)c->Z.p);
         }
         mark();
-        // --- hub segments
-        const int64_t nseg = (int64_t)(c->segE.size() + c->segT.size());
-        const int nseg_blk = (int)((nseg + 3) / 4);
-        if (nseg) {
-            SegArgs sa{};
-            sa.V = (const uint4 *)c->V.p; sa.rowptr = (const int64_t *)c->rowptr.p; sa.col = (const int32_t *)c->col.p;
-            sa.w = c->weighted ? c->w.p : nullptr; sa.rec = c->rec.p; sa.segs = (const Seg *)c->segs.p; sa.nseg = nseg;
-            sa.segout = c->segout.p; sa.pt = c->pt.p; sa.rho = rho; sa.rt = rt; sa.tpart = (double *)c->seg_part.p;
-            hipLaunchKernelGGL((lb_k_segs<TV, R>), dim3(nseg_blk), dim3(LB_BLOCK), 0, c->stream, sa);
-        }
+        // --- hub segments (joined here when they ran on the side stream)
+        if (fork) HIPCHK(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
+        else if (nseg) launch_segs(c->stream);
         mark();
         // --- rows
         const int64_t nrow_blk = std::min<int64_t>((n + NSL - 1) / NSL, c->row_grid);
@@ -3060,6 +3087,12 @@ int lb_create(lb_ctx **out, int device, int vdtype, int r) {
     c->vbytes = vdtype == LB_F32 ? 4 : (vdtype == LB_BF16 ? 2 : 8);
     c->cbytes = vdtype == LB_F64 ? 8 : 4;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return -3; }
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
+        lb_destroy(c);
+        return -3;
+    }
     *out = c;
     return 0;
 }
@@ -3068,6 +3101,7 @@ void lb_destroy(lb_ctx *c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->side) hipStreamSynchronize(c->side);
     DBuf *bufs[] = {&c->rowptr, &c->col, &c->w, &c->d, &c->labs, &c->sig, &c->pp, &c->rq, &c->dsq, &c->vlabs, &c->vrq,
                     &c->vptr, &c->vcol, &c->vsig, &c->pv, &c->vrow, &c->pol_V, &c->pol_Vn, &c->pol_R, &c->pol_g,
                     &c->pol_part, &c->pol_sum, &c->vst_part, &c->b_light, &c->b_heavy, &c->sw.hErows, &c->sw.hEseg0, &c->sw.hTrows, &c->sw.hTseg0,
@@ -3083,6 +3117,9 @@ void lb_destroy(lb_ctx *c) {
     if (c->hstage) hipHostFree(c->hstage);
     c->hstage = nullptr; c->hstage_bytes = 0;
     if (c->gexec) hipGraphExecDestroy((hipGraphExec_t)c->gexec);
+    if (c->fork_ev) hipEventDestroy(c->fork_ev);
+    if (c->join_ev) hipEventDestroy(c->join_ev);
+    if (c->side) hipStreamDestroy(c->side);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -3093,6 +3130,12 @@ int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
     drop_capture(c);
     if (!c || !name) return fail(c, -1, "lb_set_option: bad arguments");
     if (std::strcmp(name, "row_order") == 0) { c->use_order = value != 0; return 0; }
+    if (std::strcmp(name, "overlap") == 0) { c->overlap = value != 0; return 0; }
+    if (std::strcmp(name, "seg_grid") == 0) {
+        if (value < 0 || value > ((int64_t)1 << 30)) return fail(c, -1, "lb_set_option: seg_grid must be >= 0");
+        c->seg_grid = value;
+        return 0;
+    }
     if (std::strcmp(name, "sweep_heavy") == 0 || std::strcmp(name, "sweep_seg") == 0) {
         if (value < 1 || value > ((int64_t)1 << 30)) return fail(c, -1, "lb_set_option: sweep segment values must be positive");
         if (name[6] == 'h') { c->sw.heavyE = value; c->sw.heavyT = value; } else c->sw.seg_len = (int)value;

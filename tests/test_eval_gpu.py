@@ -154,6 +154,39 @@ def test_repeatable():
     assert abs(p1[0] - p2[0]) <= 1e-9 * abs(p1[0])
 
 
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_overlap_matches_one_stream(dtype):
+    """The hub segments on the context's second stream (option "overlap", the default) give the same results as the
+    whole chain on one stream, with and without a strided segment grid, and match the oracle. Same up to the
+    run-to-run rounding of the FP64 LDS atomics in the cluster sums (as test_repeatable): 1e-12 relative in FP64,
+    1e-9 (f parts) / 1e-6 (G) in FP32."""
+    inst, V = chung_lu_instance(n=20000, seed=7, k=16)
+    eng = _engine(inst, V, dtype, 16, tuning=(8, 8, 16, 0))
+    out = {}
+    for ov, grid in ((1, 0), (0, 0), (1, 3), (0, 3)):
+        eng.set_option("overlap", ov)
+        eng.set_option("seg_grid", grid)
+        for rho in (0.0, 0.02):
+            p = eng.eval(rho)
+            out[(ov, grid, rho)] = (p, eng.get_G(), eng.get_multipliers()[1])
+    tf, tg = (1e-12, 1e-12) if dtype == "f64" else (1e-9, 1e-6)
+
+    def same(a, b):
+        fin = np.isfinite(a[0])
+        assert np.array_equal(fin, np.isfinite(b[0]))
+        assert np.allclose(a[0][fin], b[0][fin], rtol=tf, atol=tf * abs(a[0][0]))
+        assert np.linalg.norm(a[1] - b[1]) <= tg * np.linalg.norm(a[1])
+        assert np.allclose(a[2], b[2], rtol=tg, atol=tg * max(1.0, np.abs(a[2]).max(initial=0.0)))
+
+    for rho in (0.0, 0.02):
+        for key in ((0, 0), (1, 3), (0, 3)):
+            same(out[(1, 0, rho)], out[key + (rho,)])
+    eng.set_option("overlap", 1)
+    eng.set_option("seg_grid", 0)
+    for rho in (0.0, 0.02):
+        _check(eng, inst, rho, dtype, "overlap")
+
+
 def test_metric_instance_parity():
     """The metric instance (n ~ 1M, r = 16, nnz ~ 13M, T ~ 3.4M, pair + VCS P = 64) in FP32 against the oracle."""
     from lagbound_amd import instances as I
