@@ -27,6 +27,7 @@
 #include <string>
 #include <thread>
 #include <vector>
+#include <array>
 
 #include "../../include/lbhip.h"
 
@@ -2174,6 +2175,9 @@ struct lb_ctx {
     std::vector<int64_t> h_rowptr;
     // levels
     int nlev = 0; std::vector<int> P; std::vector<int64_t> soff; double pw = 1.0; int sumPpair = 0;
+    // host copies of the pair-level and VCS-level labels; vdup[m] = a pair level with the same labels and cluster
+    // count as VCS level m (its cluster sums are then copied, not accumulated twice), or -1
+    std::vector<int32_t> h_labs, h_vlabs; std::vector<int> vdup;
     // vcs
     int nvcs = 0; std::vector<int> vP; std::vector<int64_t> vnnz, voff; double wv = 1.0; int sumPv = 0;
     // triangles
@@ -2425,11 +2429,23 @@ struct Launch {
         const int sumPall = c->sumPpair + c->sumPv;
         ColsumArgs ca{};
         ca.V = (const uint4 *)c->V.p; ca.n = n; ca.d = (const double *)c->d.p;
-        ca.ls.nsets = c->nlev + c->nvcs;
-        int off = 0;
-        for (int l = 0; l < c->nlev; l++) { ca.ls.off[l] = off; ca.ls.lab[l] = (const int32_t *)c->labs.p + (int64_t)l * n; off += c->P[l]; }
-        for (int m = 0; m < c->nvcs; m++) { ca.ls.off[c->nlev + m] = off; ca.ls.lab[c->nlev + m] = (const int32_t *)c->vlabs.p + (int64_t)m * n; off += c->vP[m]; }
-        ca.ls.off[ca.ls.nsets] = off;
+        // a VCS level with the labels of a pair level is not accumulated again: its rows of Y stay zero in the
+        // partials and are copied from the pair level's after the reduction
+        int off = 0, ns = 0;
+        std::vector<int> poff(c->nlev);
+        for (int l = 0; l < c->nlev; l++) {
+            poff[l] = off;
+            ca.ls.off[ns] = off; ca.ls.lab[ns++] = (const int32_t *)c->labs.p + (int64_t)l * n; off += c->P[l];
+        }
+        std::vector<std::array<int, 3>> dups;
+        for (int m = 0; m < c->nvcs; m++) {
+            const int dl = m < (int)c->vdup.size() ? c->vdup[m] : -1;
+            if (dl >= 0) dups.push_back({poff[dl], off, c->vP[m]});
+            else { ca.ls.off[ns] = off; ca.ls.lab[ns++] = (const int32_t *)c->vlabs.p + (int64_t)m * n; }
+            off += c->vP[m];
+        }
+        ca.ls.nsets = ns;
+        ca.ls.off[ns] = off;
         ca.sumP = sumPall;
         const size_t ldsY = sizeof(double) * (size_t)sumPall * (R + LB_VPAD);
         ca.use_lds = ldsY <= 48 * 1024;
@@ -2441,6 +2457,12 @@ struct Launch {
         const int width = R + (ca.use_lds ? sumPall * R : 0);
         hipLaunchKernelGGL(lb_k_reduce, dim3((width + 63) / 64), dim3(1024), 0, c->stream, (const double *)c->colsum_part.p,
                            LB_RED_BLOCKS, width, width, (double *)c->Y.p);
+        double *Yb = (double *)c->Y.p + R;
+        for (const auto &dp : dups) {                       // (pair-level offset, VCS-level offset, clusters)
+            const int64_t cnt = (int64_t)dp[2] * R;
+            hipLaunchKernelGGL((lb_k_copy<double>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream,
+                               Yb + (int64_t)dp[1] * R, (const double *)(Yb + (int64_t)dp[0] * R), cnt);
+        }
     }
 
     static int eval(lb_ctx *c, double rho, double rt, hipEvent_t *ev) {
@@ -3041,8 +3063,22 @@ int empty_triangles(lb_ctx *c) {
     return 0;
 }
 
+void find_label_dups(lb_ctx *c) {
+    c->vdup.assign(c->nvcs, -1);
+    const size_t n = (size_t)c->n;
+    if (c->h_labs.size() != (size_t)c->nlev * n || c->h_vlabs.size() != (size_t)c->nvcs * n) return;
+    for (int m = 0; m < c->nvcs; m++)
+        for (int l = 0; l < c->nlev; l++)
+            if (c->P[l] == c->vP[m] &&
+                std::memcmp(c->h_labs.data() + l * n, c->h_vlabs.data() + m * n, n * sizeof(int32_t)) == 0) {
+                c->vdup[m] = l;
+                break;
+            }
+}
+
 int empty_vcs(lb_ctx *c) {
     c->nvcs = 0; c->sumPv = 0; c->vP.clear(); c->vnnz.clear(); c->voff.clear();
+    c->h_vlabs.clear(); c->vdup.clear();
     int rc;
     if ((rc = dalloc(c, c->vlabs, 16))) return rc;
     if ((rc = dalloc(c, c->vptr, 16))) return rc;
@@ -3056,6 +3092,8 @@ int empty_vcs(lb_ctx *c) {
 
 int empty_levels(lb_ctx *c) {
     c->nlev = 0; c->sumPpair = 0; c->P.clear(); c->soff.assign(1, 0);
+    c->h_labs.clear();
+    find_label_dups(c);
     int rc;
     if ((rc = dalloc(c, c->labs, 16))) return rc;
     if ((rc = dalloc(c, c->sig, 16))) return rc;
@@ -3215,6 +3253,8 @@ int lb_set_levels(lb_ctx *c, int nlev, const int32_t *labs, const int32_t *P, co
         for (int64_t i = 0; i < c->n; i++)
             if (labs[(int64_t)l * c->n + i] < 0 || labs[(int64_t)l * c->n + i] >= P[l]) return fail(c, -1, "lb_set_levels: label out of range");
     c->nlev = nlev; c->P.assign(P, P + nlev); c->soff = soff; c->sumPpair = (int)sP; c->pw = pairlvl_weight;
+    c->h_labs.assign(labs, labs + (size_t)nlev * c->n);
+    find_label_dups(c);
     int rc;
     if ((rc = upload(c, c->labs, labs, sizeof(int32_t) * (size_t)nlev * c->n))) return rc;
     if ((rc = upload(c, c->sig, sig, sizeof(double) * sP2))) return rc;
@@ -3258,6 +3298,8 @@ int lb_set_vcs(lb_ctx *c, int nvcs, const int32_t *labs, const int32_t *P, const
         voff.push_back(tot); vnnz.push_back(pm[n]); tot += pm[n]; sP += P[m];
     }
     c->nvcs = nvcs; c->vP.assign(P, P + nvcs); c->vnnz = vnnz; c->voff = voff; c->sumPv = (int)sP; c->wv = wv;
+    c->h_vlabs.assign(labs, labs + (size_t)nvcs * n);
+    find_label_dups(c);
     int rc;
     if ((rc = upload(c, c->vlabs, labs, sizeof(int32_t) * (size_t)nvcs * n))) return rc;
     if ((rc = upload(c, c->vptr, absptr.data(), sizeof(int64_t) * absptr.size()))) return rc;
@@ -4288,7 +4330,8 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_bcolsum(BArgs a) {
 }
 
 // one wavefront per cluster row Q (global id blockIdx.x) of its graph's pair level (lb_k_levels per graph)
-__global__ __launch_bounds__(64) void lb_k_blevels(BArgs a, int R) {
+template <int R>
+__global__ __launch_bounds__(64) void lb_k_blevels(BArgs a) {
     const int Qg = blockIdx.x, g = a.cg[Qg];
     const int P = a.Poff[g + 1] - a.Poff[g], Q = Qg - a.Poff[g];
     const double *Yb = a.zY + (int64_t)a.G * R + (int64_t)a.Poff[g] * R;
@@ -4296,7 +4339,8 @@ __global__ __launch_bounds__(64) void lb_k_blevels(BArgs a, int R) {
     const double *sig = a.sig + a.soff[g] + (int64_t)Q * P;
     double *pp = a.pp + a.soff[g] + (int64_t)Q * P;
     const double rqQ = a.rq[Qg];
-    double Macc[32];
+    double Macc[R];
+#pragma unroll
     for (int c = 0; c < R; c++) Macc[c] = 0.0;
     double tot = 0.0, viol = 0.0, pq = 0.0;
     for (int Rr = threadIdx.x; Rr < P; Rr += 64) {
@@ -4304,6 +4348,7 @@ __global__ __launch_bounds__(64) void lb_k_blevels(BArgs a, int R) {
         double p = sig[Rr];
         if (a.rho > 0.0) {
             double h = 0.0;
+#pragma unroll
             for (int c = 0; c < R; c++) h += YQ[c] * YR[c];
             if (Rr == Q) h -= a.dsq[Qg];
             const double rr = a.rho * ((rqQ * a.rq[a.Poff[g] + Rr]) * a.wpair);
@@ -4315,12 +4360,16 @@ __global__ __launch_bounds__(64) void lb_k_blevels(BArgs a, int R) {
         }
         pp[Rr] = p;
         if (Rr == Q) pq = p;
-        if (p != 0.0)
+        if (p != 0.0) {
+#pragma unroll
             for (int c = 0; c < R; c++) Macc[c] += p * YR[c];
+        }
     }
     tot = wave_sum(tot); viol = wave_sum(viol); pq = wave_sum(pq);
+#pragma unroll
     for (int c = 0; c < R; c++) Macc[c] = wave_sum(Macc[c]);
     if (threadIdx.x == 0) {
+#pragma unroll
         for (int c = 0; c < R; c++) a.M[(int64_t)Qg * R + c] = Macc[c];
         a.pdiag[Qg] = pq; a.ltot[Qg] = tot; a.lviol[Qg] = viol;
     }
@@ -4585,7 +4634,7 @@ struct BLaunch {
         const size_t lds = (c->blev && (size_t)maxP * R * 8 <= 32 * 1024) ? (size_t)maxP * R * 8 : 0;
         a.ylds = lds > 0 ? 1 : 0;
         hipLaunchKernelGGL((lb_k_bcolsum<TV, R>), dim3(c->bG), dim3(LB_BLOCK), lds, s, a);
-        if (c->blev && c->bsumP) hipLaunchKernelGGL(lb_k_blevels, dim3(c->bsumP), dim3(64), 0, s, a, R);
+        if (c->blev && c->bsumP) hipLaunchKernelGGL(lb_k_blevels<R>, dim3(c->bsumP), dim3(64), 0, s, a);
         constexpr int NSLb = LB_BLOCK / (R / VT<TV>::CPL);   // slots per workgroup of the light-row kernel
         if (a.nlight) {
             const int64_t lblk = std::min<int64_t>((a.nlight + NSLb - 1) / NSLb, 8192);

@@ -187,6 +187,29 @@ def test_overlap_matches_one_stream(dtype):
         _check(eng, inst, rho, dtype, "overlap")
 
 
+@pytest.mark.parametrize("which", ["distinct", "second_level", "two_vcs"])
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_vcs_label_sets(dtype, which):
+    """The cluster sums of a VCS level whose labels equal a pair level's are copied from that level, not
+    accumulated again. Against the oracle: VCS labels different from every pair level ("distinct"), equal to the
+    second pair level ("second_level"), and two VCS levels, one a copy of the first pair level and one distinct."""
+    from lagbound_amd import instances as I
+    inst, V = small_instance(n=60, seed=11, k=16, ntri=40, blocks=4)
+    rng = np.random.default_rng(5)
+    lab1 = np.asarray(inst.levels[0].lab)
+    other = rng.integers(0, 4, inst.n)
+    other[:4] = [0, 1, 2, 3]
+    sets = {"distinct": [other], "second_level": [np.asarray(inst.levels[1].lab)], "two_vcs": [lab1, other]}[which]
+    inst.vcs = []
+    for k, lab in enumerate(sets):
+        P = int(lab.max()) + 1
+        ptr, col = I.random_ws(lab, P, 2, 3 + k)
+        inst.vcs.append(I.vcs_level(lab, inst.d, ptr, col, rng.exponential(0.1, len(col))))
+    eng = _engine(inst, V, dtype, 16)
+    for rho in (0.0, 0.3):
+        _check(eng, inst, rho, dtype, f"vcs labels {which}")
+
+
 def test_metric_instance_parity():
     """The metric instance (n ~ 1M, r = 16, nnz ~ 13M, T ~ 3.4M, pair + VCS P = 64) in FP32 against the oracle."""
     from lagbound_amd import instances as I
