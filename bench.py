@@ -165,7 +165,6 @@ def main():
     B = alg_bytes(inst, r, 4, 4, 4)
     ach = B / (ms_ev * 1e-3) / 1e9
     dom = max(kms, key=kms.get)
-    rows_bytes = None
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "eval_metric_f32_traffic.json")
     if os.path.exists(tpath):
@@ -184,9 +183,10 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "one f+grad evaluation (lb_k_colsum .. lb_k_finalize chain)",
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "alg_bytes": B, "ms_per_eval_events": round(ms_ev, 5),
-                     "dominant_kernel": dom, "kernel_ms": {k: round(v, 5) for k, v in kms.items()}},
+                     "dominant_kernel": dom, "kernel_ms": {k: round(v, 5) for k, v in kms.items()},
+                     "kernel_ms_note": "per-kernel HIP events with the chain on one stream; the timed evaluation runs "
+                                       "lb_k_segs on a second stream beside lb_k_colsum .. lb_k_reduce(Z)"},
     }
-    del rows_bytes
     if not a.no_extra:
         extra = {}
         for dt, sv, sc in (("bf16", 2, 4), ("f64", 8, 8)):
