@@ -161,12 +161,21 @@ def main(argv=None):
         be.eval(a.rho, fetch=False)
     be.sync()
     ms_batch = 1e3 * (time.perf_counter() - t) / a.steps
+    # the same evaluation captured once as a HIP graph and replayed (lb_capture_eval / lb_replay_eval)
+    be.eng.capture_eval(a.rho)
+    be.eng.replay_eval(3, fetch=False)
+    be.sync()
+    t = time.perf_counter()
+    be.eng.replay_eval(a.steps, fetch=False)
+    be.sync()
+    ms_replay = 1e3 * (time.perf_counter() - t) / a.steps
     be.close()
     out = {"graphs": p.G, "n": p.n, "nnz": int(len(p.col)), "T": int(len(p.TI)), "r": a.r, "rho": a.rho,
            "ms_per_batch_eval": round(ms_batch, 4), "graph_evals_per_s": round(p.G * 1e3 / ms_batch, 1),
+           "ms_per_batch_eval_captured": round(ms_replay, 4),
            "alg_bytes": int(p.n * a.r * 8 + len(p.col) * 4 + p.n * 24 + len(p.TI) * 16)}
     out["hbm_GBps"] = round(out["alg_bytes"] / (ms_batch * 1e-3) / 1e9, 1)
-    log(f"batch: {ms_batch:.3f} ms per evaluation of all {p.G} graphs")
+    log(f"batch: {ms_batch:.3f} ms per evaluation of all {p.G} graphs; captured replays {ms_replay:.3f} ms")
     if a.single:
         from .engine import Engine
         tot = 0.0
