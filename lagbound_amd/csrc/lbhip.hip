@@ -35,6 +35,15 @@
 // LDS row stride padding (doubles) of per-cluster rows accumulated or read by many slots at once (colsum, the VCS
 // passes): two doubles shift consecutive cluster rows by 16 B, so slots on different clusters spread over the banks
 #define LB_VPAD 2
+#ifndef LB_SEB
+#define LB_SEB 8           // sweep hub segments: edge gathers per slot in flight
+#endif
+#ifndef LB_SRB
+#define LB_SRB 4           // sweep hub segments: triangle records per slot in flight
+#endif
+#ifndef LB_POST_ROWS
+#define LB_POST_ROWS 32    // sweep aggregate increments (lb_k_post_acc): batch rows per workgroup
+#endif
 #ifndef LB_EB
 #define LB_EB 16
 #endif
@@ -1053,15 +1062,17 @@ struct PairRowArgs {
 };
 
 // pair_all for the clusters the batch touches: pp row from the current Y, M_Q = sum_R pp_QR Y_R, pdiag_Q = pp_QQ
+template <int R>
 __global__ __launch_bounds__(64) void lb_k_pair_rows(PairRowArgs a) {
     const int Qg = blockIdx.x;
     if (!a.tmask[Qg]) return;
     int l = 0;
     while (l + 1 < a.nlev && Qg >= a.Poff[l + 1]) l++;
-    const int P = a.Poff[l + 1] - a.Poff[l], Q = Qg - a.Poff[l], R = a.R;
+    const int P = a.Poff[l + 1] - a.Poff[l], Q = Qg - a.Poff[l];
     const double *YQ = a.Y + (int64_t)Qg * R;
     const double *sig = a.sig + a.soff[l] + (int64_t)Q * P;
-    double Macc[32];
+    double Macc[R];
+#pragma unroll
     for (int c = 0; c < R; c++) Macc[c] = 0.0;
     double pq = 0.0;
     for (int Rr = threadIdx.x; Rr < P; Rr += 64) {
@@ -1069,18 +1080,23 @@ __global__ __launch_bounds__(64) void lb_k_pair_rows(PairRowArgs a) {
         double p = sig[Rr];
         if (a.rho > 0.0) {
             double h = 0.0;
+#pragma unroll
             for (int c = 0; c < R; c++) h += YQ[c] * YR[c];
             if (Rr == Q) h -= a.dsq[Qg];
             p = p - (a.rho * ((a.rq[Qg] * a.rq[a.Poff[l] + Rr]) * a.w)) * h;
             if (p < 0.0) p = 0.0;
         }
         if (Rr == Q) pq = p;
-        if (p != 0.0)
+        if (p != 0.0) {
+#pragma unroll
             for (int c = 0; c < R; c++) Macc[c] += p * YR[c];
+        }
     }
     pq = wave_sum(pq);
+#pragma unroll
     for (int c = 0; c < R; c++) Macc[c] = wave_sum(Macc[c]);
     if (threadIdx.x == 0) {
+#pragma unroll
         for (int c = 0; c < R; c++) a.M[(int64_t)Qg * R + c] = Macc[c];
         a.pdiag[Qg] = pq;
     }
@@ -1256,19 +1272,53 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step_segs(StepSegArgs a) {
     for (int c = 0; c < CPL; c++) acc[c] = 0;
     double kap = 0.0;
     if (s.kind == 0) {
+        // LB_SEB edges per slot in flight (items past the segment read the hub row with weight 0: an exact zero)
         const TC *wt = (const TC *)a.w;
-        for (int64_t e = s.b + ws; e < s.e; e += NSW) {
-            TC xj[CPL];
-            VT<TV>::unpack(a.V[(int64_t)a.col[e] * S + q], xj);
-            const TC w = wt ? wt[e] : (TC)1;
+        for (int64_t e0 = s.b + ws; e0 < s.e; e0 += (int64_t)LB_SEB * NSW) {
+            int64_t jj[LB_SEB];
+            TC ww[LB_SEB];
 #pragma unroll
-            for (int c = 0; c < CPL; c++) acc[c] += w * xj[c];
+            for (int t = 0; t < LB_SEB; t++) {
+                const int64_t e = e0 + (int64_t)t * NSW;
+                const bool in = e < s.e;
+                jj[t] = in ? (int64_t)a.col[e] : (int64_t)s.row;
+                ww[t] = in ? (wt ? wt[e] : (TC)1) : (TC)0;
+            }
+            uint4 u[LB_SEB];
+#pragma unroll
+            for (int t = 0; t < LB_SEB; t++) u[t] = a.V[jj[t] * S + q];
+#pragma unroll
+            for (int t = 0; t < LB_SEB; t++) {
+                TC xj[CPL];
+                VT<TV>::unpack(u[t], xj);
+#pragma unroll
+                for (int c = 0; c < CPL; c++) acc[c] += ww[t] * xj[c];
+            }
         }
     } else {
         TC x[CPL];
         VT<TV>::unpack(a.V[(int64_t)s.row * S + q], x);
         const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
-        for (int64_t k = s.b + ws; k < s.e; k += NSW) step_record<TV, R>(a.V, rec[k], x, q, a.rt, acc, kap);
+        // LB_SRB records per slot in flight (their two rows gathered together), applied in record order
+        for (int64_t k0 = s.b + ws; k0 < s.e; k0 += (int64_t)LB_SRB * NSW) {
+            TriRec<TC> rq[LB_SRB];
+#pragma unroll
+            for (int t = 0; t < LB_SRB; t++) {
+                const int64_t k = k0 + (int64_t)t * NSW;
+                rq[t] = rec[k < s.e ? k : k0];
+            }
+            uint4 uu[LB_SRB], uw[LB_SRB];
+#pragma unroll
+            for (int t = 0; t < LB_SRB; t++) { uu[t] = a.V[(int64_t)rq[t].u * S + q]; uw[t] = a.V[(int64_t)rq[t].w * S + q]; }
+#pragma unroll
+            for (int t = 0; t < LB_SRB; t++) {
+                if (k0 + (int64_t)t * NSW >= s.e) break;      // slot-uniform
+                TC xu[CPL], xw[CPL];
+                VT<TV>::unpack(uu[t], xu);
+                VT<TV>::unpack(uw[t], xw);
+                step_record_x<TV, R>(rq[t], x, xu, xw, a.rt, acc, kap);
+            }
+        }
     }
 #pragma unroll
     for (int c = 0; c < CPL; c++) {
@@ -1288,7 +1338,7 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step_segs(StepSegArgs a) {
 // rank-one, pair levels, VCS pre-step terms, then the normalised minorant step; writes Vstage[pos], DD[pos]
 template <typename TV, int R>
 __device__ inline void step_finish(const StepArgs &a, const typename VT<TV>::TC *x, int64_t i, int64_t pos, int q,
-                                   typename VT<TV>::TC *g, double kap) {
+                                   typename VT<TV>::TC *g, double kap, bool writer = true) {
     typedef typename VT<TV>::TC TC;
     constexpr int CPL = VT<TV>::CPL, S = R / CPL;
     const double di = a.d[i];
@@ -1383,6 +1433,104 @@ __device__ inline void step_finish(const StepArgs &a, const typename VT<TV>::TC 
     }
     // round to the storage type first: DD tracks the stored rows exactly
     const uint4 u = Store<TV>::pack(xn);
+    if (!writer) return;
+    a.Vstage[pos * S + q] = u;
+    TC xs[CPL];
+    VT<TV>::unpack(u, xs);
+    double *dd = a.DD + pos * R + q * CPL;
+#pragma unroll
+    for (int c = 0; c < CPL; c++) dd[c] = di * ((double)xs[c] - (double)x[c]);
+}
+
+template <typename TV, int R>
+__device__ inline void step_finish_wave(const StepArgs &a, const typename VT<TV>::TC *x, int64_t i, int64_t pos, int q,
+                                        int ws, typename VT<TV>::TC *g, double kap) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL;
+    const double di = a.d[i];
+#pragma unroll
+    for (int c = 0; c < CPL; c++) g[c] += (TC)(-(di / a.L) * (a.z[q * CPL + c] - di * (double)x[c]));
+    for (int l = 0; l < a.nlev; l++) {
+        const int Q = a.Poff[l] + a.lab[l][i];
+        const double *M = a.M + (int64_t)Q * R + q * CPL;
+        const double pd = a.pdiag[Q] * di;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) g[c] += (TC)(di * (M[c] - pd * (double)x[c]));
+        kap += di * di * a.Kq[Q];
+    }
+    const TC *vsig = (const TC *)a.vsig;
+    for (int m = 0; m < a.nvcs; m++) {
+        const int Ri = a.vlab[m][i];
+        double acc[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) acc[c] = 0.0;
+        double kp = 0.0, pown = 0.0, sz = 0.0;
+        const int64_t ea = a.vptr[m][i], eb = a.vptr[m][i + 1];
+        for (int64_t e = ea + ws; e < eb; e += 64 / S) {       // the row's entries dealt to the slots
+            const int Rc = a.vcol[e];
+            const double sqe = (double)vsig[e], zqe = a.stZ[e];
+            const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+            double yv[CPL];
+#pragma unroll
+            for (int c = 0; c < CPL; c++) yv[c] = YR[c];
+            double s = 0.0, y2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) { s += (double)x[c] * yv[c]; y2 += yv[c] * yv[c]; }
+            s = slot_sum<S>(s);
+            double p;
+            if (a.rho > 0.0 && di > 0.0) {
+                const double g2 = (Rc == Ri) ? di * (s - di) : di * s;
+                const double rr = a.wv * a.vrq[m][Rc] / (di * di);
+                p = sqe - a.rho * rr * g2;
+                if (p > 0.0) {
+                    y2 = slot_sum<S>(y2);
+                    if (Rc == Ri) y2 = y2 - 2 * di * s + di * di;
+                    kp += a.rho * rr * di * di * y2;
+                } else p = 0.0;
+            } else p = sqe;
+            if (Rc == Ri) { pown = p; sz = zqe; }
+            if (p != 0.0) {
+#pragma unroll
+                for (int c = 0; c < CPL; c++) acc[c] += p * yv[c];
+            }
+        }
+        // sum over the slots (the own-cluster entry sits in one slot; the others hold zeros there)
+#pragma unroll
+        for (int c = 0; c < CPL; c++)
+            for (int mm = S; mm < 64; mm <<= 1) acc[c] += __shfl_xor(acc[c], mm);
+        for (int mm = S; mm < 64; mm <<= 1) {
+            kp += __shfl_xor(kp, mm);
+            pown += __shfl_xor(pown, mm);
+            sz += __shfl_xor(sz, mm);
+        }
+        const double *ZR = a.Z + (int64_t)(a.Zoff[m] + Ri) * R + q * CPL;
+#pragma unroll
+        for (int c = 0; c < CPL; c++)
+            g[c] += (TC)(0.5 * di * (acc[c] - pown * di * (double)x[c]) + 0.5 * di * (ZR[c] - sz * di * (double)x[c]));
+        if (a.rho > 0.0) kp += a.kap2 * a.rho * di * di * a.wv * a.vrq[m][Ri] * (a.cnt[a.Zoff[m] + Ri] - (sz > 0.0 ? 1.0 : 0.0));
+        kap += kp;
+    }
+    TC gl = 0;
+#pragma unroll
+    for (int c = 0; c < CPL; c++) gl += g[c] * g[c];
+    const TC gn = slot_sum<S>(gl);
+    const TC w = (TC)(0.5 * kap) + (TC)a.damp * sqrt(gn);
+    TC qv[CPL], nl = 0;
+#pragma unroll
+    for (int c = 0; c < CPL; c++) { qv[c] = g[c] + w * x[c]; nl += qv[c] * qv[c]; }
+    const TC nv = slot_sum<S>(nl);
+    TC xn[CPL];
+    if (nv > (TC)0) {
+        const TC inv = (TC)1 / sqrt(nv);
+#pragma unroll
+        for (int c = 0; c < CPL; c++) xn[c] = qv[c] * inv;
+    } else {
+#pragma unroll
+        for (int c = 0; c < CPL; c++) xn[c] = x[c];
+    }
+    // round to the storage type first: DD tracks the stored rows exactly
+    const uint4 u = Store<TV>::pack(xn);
+    if (ws != 0) return;
     a.Vstage[pos * S + q] = u;
     TC xs[CPL];
     VT<TV>::unpack(u, xs);
@@ -1470,6 +1618,102 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step(StepArgs a) {
         }
     }
     step_finish<TV, R>(a, x, i, pos, q, g, kap);
+}
+
+// One wavefront per batch row (a batch holds a few hundred rows, far fewer than the GPU's wavefront slots): the
+// row's edges and triangle records are dealt to its 64 / S slots, so all their gathers are in flight at once, and the
+// slot partials are summed by a butterfly over the slots. Every slot then holds the same g and curvature and runs
+// the rest of the step (the slot-0 lanes write). Hub rows take their segment sums as in lb_k_step.
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_step_wave(StepArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSW = 64 / S;
+    const int lane = threadIdx.x & 63, ws = lane / S, q = lane % S;
+    const int64_t pos = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
+    if (pos >= a.nb) return;                           // wavefront-uniform
+    const int64_t i = a.rows[pos];
+    TC x[CPL], g[CPL];
+    VT<TV>::unpack(a.V[i * S + q], x);
+#pragma unroll
+    for (int c = 0; c < CPL; c++) g[c] = 0;
+    double kap = 0.0;
+    const TC *so = (const TC *)a.segout;
+    const int64_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+    const bool hubE = e1 - e0 > a.heavyE;
+    if (!hubE) {
+        // two edges per slot in flight; items past the row read row i with weight 0 (an exact zero)
+        const TC *wt = (const TC *)a.w;
+        for (int64_t eb = e0 + ws; eb < e1; eb += 2 * NSW) {
+            int32_t jj[2];
+            TC ww[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int64_t e = eb + (int64_t)t * NSW;
+                const bool in = e < e1;
+                jj[t] = in ? a.col[e] : (int32_t)i;
+                ww[t] = in ? (wt ? wt[e] : (TC)1) : (TC)0;
+            }
+            uint4 u[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) u[t] = a.V[(int64_t)jj[t] * S + q];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                TC xj[CPL];
+                VT<TV>::unpack(u[t], xj);
+#pragma unroll
+                for (int c = 0; c < CPL; c++) g[c] += ww[t] * xj[c];
+            }
+        }
+    }
+    const int64_t r0 = a.iptr[i], r1 = a.iptr[i + 1];
+    const bool hubT = r1 - r0 > a.heavyT;
+    if (!hubT) {
+        // two records per slot in flight (their four rows gathered together)
+        const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
+        for (int64_t kb = r0 + ws; kb < r1; kb += 2 * NSW) {
+            TriRec<TC> rc[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int64_t k = kb + (int64_t)t * NSW;
+                rc[t] = rec[k < r1 ? k : kb];
+            }
+            uint4 uu[2], uw[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) { uu[t] = a.V[(int64_t)rc[t].u * S + q]; uw[t] = a.V[(int64_t)rc[t].w * S + q]; }
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                if (kb + (int64_t)t * NSW >= r1) break;          // slot-uniform
+                TC xu[CPL], xw[CPL];
+                VT<TV>::unpack(uu[t], xu);
+                VT<TV>::unpack(uw[t], xw);
+                step_record_x<TV, R>(rc[t], x, xu, xw, a.rt, g, kap);
+            }
+        }
+    }
+    // sum over the slots (lanes with equal q): every slot ends with the row's totals
+#pragma unroll
+    for (int c = 0; c < CPL; c++) {
+        TC t = g[c];
+        for (int m = S; m < 64; m <<= 1) t += __shfl_xor(t, m);
+        g[c] = t;
+    }
+    for (int m = S; m < 64; m <<= 1) kap += __shfl_xor(kap, m);
+    if (hubE) {
+        const int k = find_row(a.hE_rows, a.nhE, i);
+        for (int s = a.hE_seg0[k]; s < a.hE_seg0[k + 1]; s++) {
+#pragma unroll
+            for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
+        }
+    }
+    if (hubT) {
+        const int k = find_row(a.hT_rows, a.nhT, i);
+        for (int s = a.hT_seg0[k]; s < a.hT_seg0[k + 1]; s++) {
+#pragma unroll
+            for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
+            kap += a.segkap[s];
+        }
+    }
+    step_finish_wave<TV, R>(a, x, i, pos, q, ws, g, kap);
 }
 
 struct PostArgs {
@@ -1593,6 +1837,71 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_vcs(Post2Args a) {
         }
     }
     a.V[i * S + q] = un;
+    }
+    if (a.use_lds) {                                     // one global add per workgroup and nonzero entry
+        __syncthreads();
+        for (int e = threadIdx.x; e < wz; e += LB_BLOCK)
+            if (shZc[e] != 0.0) atomicAdd(a.Z + e, shZc[e]);
+        for (int e = threadIdx.x; e < a.sumPv; e += LB_BLOCK)
+            if (shZc[wz + e] != 0.0) atomicAdd(a.cnt + e, shZc[wz + e]);
+    }
+}
+
+// One wavefront per batch row (four rows per workgroup): the row's working-set entries are dealt to its 64 / S slots,
+// one entry per slot at a time, with the same arithmetic and the same LDS staging of the Z / count increments as
+// lb_k_post_vcs. The slot-0 lanes write the new V row after every slot has read the old one.
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_post_vcs_wave(Post2Args a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSW = 64 / S;
+    extern __shared__ double shZc[];
+    const int lane = threadIdx.x & 63, ws = lane / S, q = lane % S;
+    const int64_t pos = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
+    const int wz = a.sumPv * R;
+    double *Zacc = a.use_lds ? shZc : a.Z;
+    double *Cacc = a.use_lds ? shZc + wz : a.cnt;
+    if (a.use_lds) {
+        for (int e = threadIdx.x; e < wz + a.sumPv; e += LB_BLOCK) shZc[e] = 0.0;
+        __syncthreads();
+    }
+    if (pos < a.nb) {                                  // wavefront-uniform
+        const int64_t i = a.rows[pos];
+        const uint4 un = a.Vstage[pos * S + q];
+        TC xn[CPL], xo[CPL];
+        VT<TV>::unpack(un, xn);
+        VT<TV>::unpack(a.V[i * S + q], xo);
+        const double di = a.d[i];
+        const TC *sig = (const TC *)a.sig;
+        for (int m = 0; m < a.nvcs; m++) {
+            const int Ri = a.lab[m][i];
+            const int64_t ea = a.ptr[m][i], eb = a.ptr[m][i + 1];
+            for (int64_t e = ea + ws; e < eb; e += NSW) {
+                const int Rc = a.col[e];
+                const double sq = (double)sig[e], so = a.stZ[e];
+                const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < CPL; c++) s += (double)xn[c] * YR[c];
+                s = slot_sum<S>(s);
+                double sn = sq;
+                if (a.rho > 0.0 && di > 0.0) {
+                    const double g = (Rc == Ri) ? di * (s - di) : di * s;
+                    sn = sn - a.rho * (a.wv * a.rq[m][Rc] / (di * di)) * g;
+                    if (sn < 0.0) sn = 0.0;
+                }
+                if (sn != 0.0 || so != 0.0) {
+                    double *dst = Zacc + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+#pragma unroll
+                    for (int c = 0; c < CPL; c++) atomicAdd(dst + c, di * (sn * (double)xn[c] - so * (double)xo[c]));
+                }
+                if (q == 0) {
+                    const double dc = (sn > 0.0 ? 1.0 : 0.0) - (so > 0.0 ? 1.0 : 0.0);
+                    if (dc != 0.0) atomicAdd(Cacc + a.Zoff[m] + Rc, dc);
+                    a.stZ[e] = sn;
+                }
+            }
+        }
+        if (ws == 0) a.V[i * S + q] = un;
     }
     if (a.use_lds) {                                     // one global add per workgroup and nonzero entry
         __syncthreads();
@@ -2163,6 +2472,8 @@ struct lb_ctx {
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     bool overlap = true;
     int64_t seg_grid = 0;              // workgroups of the evaluation's hub-segment kernel (0: one per 4 segments)
+    bool step_wave = true;             // sweeps: one wavefront per batch row (lb_k_step_wave, lb_k_post_vcs_wave) or one
+                                       // slot per row (lb_k_step, lb_k_post_vcs)
     std::string err;
     size_t vbytes = 4, cbytes = 4;   // storage bytes of V elements, of TC values
     int heavy_edges = 128, heavy_tris = 128, seg_len = 1024;  // rows above these counts are split into segments
@@ -2717,7 +3028,7 @@ struct Launch {
             if (nb == 0) continue;
             if (c->nlev) {
                 pa.tmask = (const uint8_t *)c->tmask.p + (int64_t)b * c->sumPpair;
-                hipLaunchKernelGGL(lb_k_pair_rows, dim3(c->sumPpair), dim3(64), 0, c->stream, pa);
+                hipLaunchKernelGGL(lb_k_pair_rows<R>, dim3(c->sumPpair), dim3(64), 0, c->stream, pa);
             }
             const int64_t ns = lpos[b + 1] - lpos[b];
             if (ns) {
@@ -2725,13 +3036,20 @@ struct Launch {
                 hipLaunchKernelGGL((lb_k_step_segs<TV, R>), dim3((unsigned)((ns + 3) / 4)), dim3(LB_BLOCK), 0, c->stream, ga);
             }
             sa.rows = order + p0; sa.nb = nb;
-            hipLaunchKernelGGL((lb_k_step<TV, R>), dim3((unsigned)((nb + NSL - 1) / NSL)), dim3(LB_BLOCK), 0, c->stream, sa);
+            if (c->step_wave)
+                hipLaunchKernelGGL((lb_k_step_wave<TV, R>), dim3((unsigned)((nb + 3) / 4)), dim3(LB_BLOCK), 0, c->stream, sa);
+            else
+                hipLaunchKernelGGL((lb_k_step<TV, R>), dim3((unsigned)((nb + NSL - 1) / NSL)), dim3(LB_BLOCK), 0, c->stream, sa);
             qa.rows = order + p0; qa.nb = nb;
-            const unsigned pg = (unsigned)std::min<int64_t>(64, (nb + 127) / 128);
+            const unsigned pg = (unsigned)std::min<int64_t>(64, (nb + LB_POST_ROWS - 1) / LB_POST_ROWS);
             hipLaunchKernelGGL(lb_k_post_acc, dim3(pg), dim3(LB_BLOCK), qa.use_lds ? ldsP : 0, c->stream, qa);
             pb.rows = order + p0; pb.nb = nb;
-            hipLaunchKernelGGL((lb_k_post_vcs<TV, R>), dim3((unsigned)((nb + NSL - 1) / NSL)), dim3(LB_BLOCK),
-                               pb.use_lds ? ldsV : 0, c->stream, pb);
+            if (c->step_wave)
+                hipLaunchKernelGGL((lb_k_post_vcs_wave<TV, R>), dim3((unsigned)((nb + 3) / 4)), dim3(LB_BLOCK),
+                                   pb.use_lds ? ldsV : 0, c->stream, pb);
+            else
+                hipLaunchKernelGGL((lb_k_post_vcs<TV, R>), dim3((unsigned)((nb + NSL - 1) / NSL)), dim3(LB_BLOCK),
+                                   pb.use_lds ? ldsV : 0, c->stream, pb);
         }
         HIPCHK(c, hipGetLastError());
         return 0;
@@ -3169,6 +3487,7 @@ int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
     if (!c || !name) return fail(c, -1, "lb_set_option: bad arguments");
     if (std::strcmp(name, "row_order") == 0) { c->use_order = value != 0; return 0; }
     if (std::strcmp(name, "overlap") == 0) { c->overlap = value != 0; return 0; }
+    if (std::strcmp(name, "step_wave") == 0) { c->step_wave = value != 0; return 0; }
     if (std::strcmp(name, "seg_grid") == 0) {
         if (value < 0 || value > ((int64_t)1 << 30)) return fail(c, -1, "lb_set_option: seg_grid must be >= 0");
         c->seg_grid = value;

@@ -40,13 +40,15 @@ def test_small_sweeps(dtype, tol, rho, nbatch):
     assert np.abs(np.linalg.norm(W, axis=1) - 1).max() <= (1e-12 if dtype == "f64" else 1e-5)
 
 
+@pytest.mark.parametrize("step_wave", [1, 0])
 @pytest.mark.parametrize("dtype,tol", [("f64", 1e-9), ("f32", 5e-4)])
-def test_chung_lu_sweeps_with_hubs(dtype, tol):
-    """Hub rows (segment path) inside batches; weighted/unweighted edges; three sweeps."""
+def test_chung_lu_sweeps_with_hubs(dtype, tol, step_wave):
+    """Hub rows (segment path) inside batches; weighted/unweighted edges; three sweeps; with the row step as one
+    wavefront per batch row (the default) and as one slot per row."""
     inst, V = chung_lu_instance(n=20000, seed=5, k=16)
     rng = np.random.default_rng(6)
     orders = np.stack([rng.permutation(inst.n) for _ in range(3)])
-    V0, W = _run(inst, V, dtype, 16, orders, 16, 0.02, 0.0)
+    V0, W = _run(inst, V, dtype, 16, orders, 16, 0.02, 0.0, options={"step_wave": step_wave})
     Wo = O.sweeps_c(inst, V0, orders, 16, 0.02, damp=0.0)
     assert np.abs(W - Wo).max() <= tol
 
