@@ -150,8 +150,10 @@ int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int
  * row kernel in descending order of their work instead of index order (same results up to summation order);
  * "overlap" (default 1) runs the evaluation's hub-segment kernel on a second stream of the context, concurrently
  * with the cluster-sum and VCS kernels, joined before the row kernel (identical results; lb_time_eval's per-kernel
- * figures always come from the one-stream order); "sweep_heavy" / "sweep_seg" (defaults 32 / 128) are the hub
- * threshold and segment length of the sweeps' own segment tables. */
+ * figures always come from the one-stream order); "seg_grid" (default 0) caps the workgroups of that hub-segment
+ * kernel (0: one per four segments); "sweep_heavy" / "sweep_seg" (defaults 32 / 128) are the hub threshold and
+ * segment length of the sweeps' own segment tables; "step_wave" (default 1) runs the sweeps' row step and VCS post
+ * step with one wavefront per batch row (0: one slot of r/4 lanes per row). */
 int lb_set_option(lb_ctx *ctx, const char *name, int64_t value);
 
 /* Device row work of the candidate bound's Riemannian gradient polish (polish_rgd, est_dual_vcs.py:276-308), with
