@@ -53,6 +53,18 @@ def test_chung_lu_sweeps_with_hubs(dtype, tol, step_wave):
     assert np.abs(W - Wo).max() <= tol
 
 
+@pytest.mark.parametrize("r", [8, 32])
+@pytest.mark.parametrize("step_wave", [1, 0])
+def test_sweeps_other_ranks(r, step_wave):
+    """Ranks 8 and 32 (2 and 8 lanes per row slot, so 32 and 8 slots per wavefront in the wavefront-per-row step)."""
+    inst, V = small_instance(n=70, seed=12, k=r, ntri=60, sig_scale=0.05)
+    rng = np.random.default_rng(13)
+    orders = np.stack([rng.permutation(inst.n) for _ in range(2)])
+    V0, W = _run(inst, V, "f64", r, orders, 6, 0.3, 0.1, options={"step_wave": step_wave})
+    Wo = O.sweeps_c(inst, V0, orders, 6, 0.3, damp=0.1)
+    assert np.abs(W - Wo).max() <= 1e-10
+
+
 def test_segments_forced_small():
     """Tiny hub thresholds for the evaluation's tables and for the sweeps' own (rows above 2 edges / records split
     into 3-item segments), so the segment paths of both run on a small weighted instance."""
