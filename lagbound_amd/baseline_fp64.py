@@ -1,11 +1,25 @@
 """The plain FP64 GPU form of one f+grad evaluation: the same mathematics as the HIP kernels with straightforward
 torch operations (FP64 CSR SpMM, gathers, index_add_). It is the comparison baseline timed beside the HIP path
 (`vs FP64` in the metric) and is not used by the search. Mirrors `augmented` + `CV` (alm_vcs.py:1496-1533,
-1299-1312)."""
+1299-1312). Sums into a few cluster rows (Y per label set, Z per VCS level) are sparse products with the
+cluster-membership matrices rather than index_add_ onto 64 rows, whose atomics all collide."""
 from __future__ import annotations
 
 import numpy as np
 import torch
+
+
+def _csr(crow, col, val, shape, device):
+    return torch.sparse_csr_tensor(torch.as_tensor(crow, dtype=torch.int64), torch.as_tensor(col, dtype=torch.int64),
+                                   torch.as_tensor(val, dtype=torch.float64), shape, device=device)
+
+
+def _members(lab, P, device):
+    """(P x n) CSR with a one in row Q for every member of cluster Q: M @ X sums the rows of X per cluster."""
+    lab = np.asarray(lab, np.int64)
+    crow = np.zeros(P + 1, np.int64)
+    np.cumsum(np.bincount(lab, minlength=P), out=crow[1:])
+    return _csr(crow, np.argsort(lab, kind="stable"), np.ones(len(lab)), (P, len(lab)), device)
 
 
 class PlainFP64:
@@ -23,13 +37,22 @@ class PlainFP64:
         self.pw = float(inst.pairlvl_weight)
         self.wv = float(inst.wv)
         self.levels = [dict(lab=torch.as_tensor(l.lab.astype(np.int64), device=device), P=l.P,
+                            mem=_members(l.lab, l.P, device),
                             sig=torch.as_tensor(l.sig, dtype=dt, device=device),
                             rq=torch.as_tensor(l.rq, dtype=dt, device=device),
                             dsq=torch.as_tensor(l.dsq, dtype=dt, device=device)) for l in inst.levels]
         self.vcs = []
         for v in inst.vcs:
             rows = np.repeat(np.arange(n), np.diff(v.ptr))
+            col = np.asarray(v.col, np.int64)
+            zperm = np.argsort(col, kind="stable")              # entries grouped by cluster: Z = (P x n) @ V
+            zcrow = np.zeros(v.P + 1, np.int64)
+            np.cumsum(np.bincount(col, minlength=v.P), out=zcrow[1:])
             self.vcs.append(dict(lab=torch.as_tensor(v.lab.astype(np.int64), device=device), P=v.P,
+                                 mem=_members(v.lab, v.P, device),
+                                 ptr=torch.as_tensor(np.asarray(v.ptr, np.int64), device=device),
+                                 zperm=torch.as_tensor(zperm, device=device), zcrow=torch.as_tensor(zcrow, device=device),
+                                 zcol=torch.as_tensor(rows[zperm], device=device),
                                  rq=torch.as_tensor(v.rq, dtype=dt, device=device),
                                  rows=torch.as_tensor(rows, device=device),
                                  col=torch.as_tensor(v.col.astype(np.int64), device=device),
@@ -40,6 +63,17 @@ class PlainFP64:
         self.b = torch.as_tensor(TI[:, 1], device=device)
         self.c = torch.as_tensor(TI[:, 2], device=device)
         self.lam = torch.as_tensor(inst.lam, dtype=dt, device=device)
+        # T(pt) V as one sparse product: per triangle the six entries (a,c,+h) (a,b,-h) (b,a,-h) (b,c,-h) (c,a,+h)
+        # (c,b,-h), laid out once by row (index_add_ onto the rows would make every triangle at a hub collide)
+        a_, b_, c_ = TI[:, 0], TI[:, 1], TI[:, 2]
+        r6 = np.concatenate([a_, a_, b_, b_, c_, c_])
+        c6 = np.concatenate([c_, b_, a_, c_, a_, b_])
+        tperm = np.argsort(r6, kind="stable")
+        tcrow = np.zeros(n + 1, np.int64)
+        np.cumsum(np.bincount(r6, minlength=n), out=tcrow[1:])
+        self.tperm = torch.as_tensor(tperm, device=device)
+        self.tcrow = torch.as_tensor(tcrow, device=device)
+        self.tcol = torch.as_tensor(c6[tperm], device=device)
 
     def eval(self, V, rho, rt=None):
         """Returns (f as a 0-d tensor, G)."""
@@ -52,7 +86,7 @@ class PlainFP64:
         f = quad - (z @ z) / self.L + self.C0
         f0 = torch.zeros((), dtype=V.dtype, device=V.device)
         for l in self.levels:
-            Y = torch.zeros((l["P"], V.shape[1]), dtype=V.dtype, device=V.device).index_add_(0, l["lab"], d[:, None] * V)
+            Y = torch.sparse.mm(l["mem"], d[:, None] * V)
             if rho > 0:
                 H = Y @ Y.T - torch.diag(l["dsq"])
                 rr = rho * torch.outer(l["rq"], l["rq"]) * self.pw
@@ -69,10 +103,11 @@ class PlainFP64:
                 f = f - ((pt * pt - self.lam * self.lam) / (2 * rt)).sum()
             else:
                 pt = self.lam
-            h = 0.5 * pt[:, None]
-            G = G.index_add(0, self.a, h * (Vc - Vb)).index_add(0, self.b, -h * (Va + Vc)).index_add(0, self.c, h * (Va - Vb))
+            h = 0.5 * pt
+            vals = torch.cat([h, -h, -h, -h, h, -h])[self.tperm]
+            G = G + torch.sparse.mm(torch.sparse_csr_tensor(self.tcrow, self.tcol, vals, (self.n, self.n), device=V.device), V)
         for v in self.vcs:
-            Y = torch.zeros((v["P"], V.shape[1]), dtype=V.dtype, device=V.device).index_add_(0, v["lab"], d[:, None] * V)
+            Y = torch.sparse.mm(v["mem"], d[:, None] * V)
             rows, col = v["rows"], v["col"]
             Vr = V[rows]
             s = (Vr * Y[col]).sum(1)
@@ -85,8 +120,10 @@ class PlainFP64:
                 f = f - ((pv * pv - v["sig"] * v["sig"]) / (2 * rr)).sum()
             else:
                 pv = v["sig"]
-            Z = torch.zeros_like(Y).index_add_(0, col, (pv * di)[:, None] * Vr)
-            SY = torch.zeros_like(V).index_add_(0, rows, pv[:, None] * Y[col])
+            Zm = torch.sparse_csr_tensor(v["zcrow"], v["zcol"], (pv * di)[v["zperm"]], (v["P"], self.n), device=V.device)
+            Z = torch.sparse.mm(Zm, V)
+            Sm = torch.sparse_csr_tensor(v["ptr"], col, pv, (self.n, v["P"]), device=V.device)
+            SY = torch.sparse.mm(Sm, Y)
             ownv = torch.zeros_like(d).index_add_(0, rows, torch.where(own, pv, torch.zeros_like(pv)))
             G = G + 0.5 * d[:, None] * (SY + Z[v["lab"]]) - (ownv * d * d)[:, None] * V
         if rho <= 0:
