@@ -61,7 +61,7 @@
 #define LB_MAXVCS 4
 #define LB_RED_BLOCKS 512
 #ifndef LB_VCS_BLOCKS
-#define LB_VCS_BLOCKS 1024
+#define LB_VCS_BLOCKS 1536   // VCS pass workgroups: 1536 measured best of 512 / 1024 / 1536 / 2048 on the 1M-row graph
 #endif
 #define LB_BLOCK 256
 
