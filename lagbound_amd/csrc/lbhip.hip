@@ -1208,6 +1208,7 @@ struct StepArgs {
     int64_t heavyE, heavyT;
     const int32_t *hE_rows; const int32_t *hE_seg0; int nhE;
     const int32_t *hT_rows; const int32_t *hT_seg0; int nhT;
+    const int32_t *hE_idx; const int32_t *hT_idx;   // row -> hub index (or -1)
     const void *segout; const double *segkap;
     const int32_t *rows; int64_t nb;            // the rows of this batch
     uint4 *Vstage; double *DD;                  // batch-position indexed
@@ -1701,14 +1702,14 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step_wave(StepArgs a) {
     }
     for (int m = S; m < 64; m <<= 1) kap += __shfl_xor(kap, m);
     if (hubE) {
-        const int k = find_row(a.hE_rows, a.nhE, i);
+        const int k = a.hE_idx[i];
         for (int s = a.hE_seg0[k]; s < a.hE_seg0[k + 1]; s++) {
 #pragma unroll
             for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
         }
     }
     if (hubT) {
-        const int k = find_row(a.hT_rows, a.nhT, i);
+        const int k = a.hT_idx[i];
         for (int s = a.hT_seg0[k]; s < a.hT_seg0[k + 1]; s++) {
 #pragma unroll
             for (int c = 0; c < CPL; c++) g[c] += so[(int64_t)s * R + q * CPL + c];
@@ -2463,6 +2464,7 @@ struct SweepSegs {
     std::vector<int32_t> hE_rows, hE_seg0, hT_rows, hT_seg0, hE_idx, hT_idx;
     std::vector<Seg> segE, segT;
     DBuf hErows, hEseg0, hTrows, hTseg0, segs, segout, segkap;
+    DBuf hEidx, hTidx;                  // row -> its index in hE_rows / hT_rows, or -1 (one load, no search)
 };
 
 struct lb_ctx {
@@ -2711,6 +2713,8 @@ int build_sweep_segs(lb_ctx *c) {
     if (!w.hT_rows.empty() && (rc = upload(c, w.hTrows, w.hT_rows.data(), sizeof(int32_t) * w.hT_rows.size()))) return rc;
     if ((rc = upload(c, w.hEseg0, w.hE_seg0.data(), sizeof(int32_t) * w.hE_seg0.size()))) return rc;
     if ((rc = upload(c, w.hTseg0, w.hT_seg0.data(), sizeof(int32_t) * w.hT_seg0.size()))) return rc;
+    if ((rc = upload(c, w.hEidx, w.hE_idx.data(), sizeof(int32_t) * w.hE_idx.size()))) return rc;
+    if ((rc = upload(c, w.hTidx, w.hT_idx.data(), sizeof(int32_t) * w.hT_idx.size()))) return rc;
     if ((rc = dalloc(c, w.segs, sizeof(Seg) * ns))) return rc;
     if (!all.empty() && (rc = upload(c, w.segs, all.data(), sizeof(Seg) * all.size()))) return rc;
     if ((rc = dalloc(c, w.segout, ns * c->r * c->cbytes))) return rc;
@@ -2989,6 +2993,7 @@ struct Launch {
         sa.heavyE = w.heavyE; sa.heavyT = w.heavyT;
         sa.hE_rows = (const int32_t *)w.hErows.p; sa.hE_seg0 = (const int32_t *)w.hEseg0.p; sa.nhE = (int)w.hE_rows.size();
         sa.hT_rows = (const int32_t *)w.hTrows.p; sa.hT_seg0 = (const int32_t *)w.hTseg0.p; sa.nhT = (int)w.hT_rows.size();
+        sa.hE_idx = (const int32_t *)w.hEidx.p; sa.hT_idx = (const int32_t *)w.hTidx.p;
         sa.segout = w.segout.p; sa.segkap = (const double *)w.segkap.p;
         sa.Vstage = (uint4 *)c->Vstage.p; sa.DD = (double *)c->DDb.p;
         StepSegArgs ga{};
@@ -3462,7 +3467,7 @@ void lb_destroy(lb_ctx *c) {
     if (c->side) hipStreamSynchronize(c->side);
     DBuf *bufs[] = {&c->rowptr, &c->col, &c->w, &c->d, &c->labs, &c->sig, &c->pp, &c->rq, &c->dsq, &c->vlabs, &c->vrq,
                     &c->vptr, &c->vcol, &c->vsig, &c->pv, &c->vrow, &c->pol_V, &c->pol_Vn, &c->pol_R, &c->pol_g,
-                    &c->pol_part, &c->pol_sum, &c->vst_part, &c->b_light, &c->b_heavy, &c->sw.hErows, &c->sw.hEseg0, &c->sw.hTrows, &c->sw.hTseg0,
+                    &c->pol_part, &c->pol_sum, &c->vst_part, &c->b_light, &c->b_heavy, &c->sw.hErows, &c->sw.hEseg0, &c->sw.hTrows, &c->sw.hTseg0, &c->sw.hEidx, &c->sw.hTidx,
                     &c->sw.segs, &c->sw.segout, &c->sw.segkap, &c->TI, &c->lam, &c->pt, &c->iptr, &c->rec, &c->V, &c->G,
                     &c->Y, &c->z, &c->M, &c->pdiag, &c->levtot, &c->levviol, &c->Z, &c->colsum_part, &c->vcs_part,
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
