@@ -456,15 +456,31 @@ def _merge_ws(vl, keep, ni, nR):
     """update_ws / _merge (vcsops.py:439-464): kept old entries and new (i, R) entries, CSR sorted by (i, R).
     Returns (ptr, col, src) with src = index of the old entry or -1 for a new one."""
     n = len(vl.lab)
+    P = np.int64(vl.P)
     rows_old = np.repeat(np.arange(n), np.diff(vl.ptr))
     ko = np.flatnonzero(keep)
-    r = np.r_[rows_old[ko], np.asarray(ni, np.int64)]
-    c = np.r_[vl.col[ko].astype(np.int64), np.asarray(nR, np.int64)]
-    src = np.r_[ko, np.full(len(ni), -1)]
-    o = np.argsort(r * np.int64(vl.P) + c, kind="stable")     # by (i, R): one integer key
-    c, src = c[o], src[o]
+    ro, co = rows_old[ko], vl.col[ko].astype(np.int64)      # kept entries: in (i, R) order as this function writes them
+    rn, cn = np.asarray(ni, np.int64), np.asarray(nR, np.int64)
+    ko_key = ro * P + co
+    if len(ko_key) > 1 and not np.all(ko_key[1:] >= ko_key[:-1]):   # a working set from elsewhere: sort everything
+        o = np.argsort(np.r_[ko_key, rn * P + cn], kind="stable")
+        c, src = np.r_[co, cn][o], np.r_[ko, np.full(len(rn), -1)][o]
+        ptr = np.zeros(n + 1, np.int64)
+        np.cumsum(np.bincount(np.r_[ro, rn], minlength=n), out=ptr[1:])
+        return ptr, c.astype(np.int32), src
+    on = np.argsort(rn * P + cn, kind="stable")               # only the new entries are sorted
+    rn, cn = rn[on], cn[on]
+    # merge by the key i P + R, kept entries first on equal keys (the order of a stable sort of [kept, new])
+    at = np.searchsorted(ko_key, rn * P + cn, side="right") + np.arange(len(rn))
+    tot = len(ro) + len(rn)
+    isold = np.ones(tot, bool)
+    isold[at] = False
+    c = np.empty(tot, np.int64)
+    src = np.empty(tot, np.int64)
+    c[isold], src[isold] = co, ko
+    c[at], src[at] = cn, -1
     ptr = np.zeros(n + 1, np.int64)
-    np.cumsum(np.bincount(r, minlength=n), out=ptr[1:])
+    np.cumsum(np.bincount(np.r_[ro, rn], minlength=n), out=ptr[1:])
     return ptr, c.astype(np.int32), src
 
 

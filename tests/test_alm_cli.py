@@ -76,3 +76,34 @@ def test_merge_ws_from_empty_working_set():
     assert ptr.tolist() == [0, 0, 1, 1, 2, 2]
     assert col.tolist() == [0, 1]
     assert _take_kept(vl2.sig, src, np.float64).tolist() == [0.0, 0.25]
+
+
+def test_merge_ws_matches_full_sort():
+    """The sorted merge equals a stable sort of [kept, new] by (i, R) on random working sets, including new entries
+    that repeat a kept one (kept first) and repeats among the new ones (their given order)."""
+    from lagbound_amd.alm import _merge_ws
+    from lagbound_amd import instances as I
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        n, P = int(rng.integers(1, 60)), int(rng.integers(1, 9))
+        cnt = rng.integers(0, P + 1, n)
+        cols = [rng.choice(P, size=k, replace=False) for k in cnt]
+        if trial % 2 == 0:                      # sorted rows (as the solver writes them); odd trials: unsorted
+            cols = [np.sort(x) for x in cols]
+        ptr0 = np.zeros(n + 1, np.int64)
+        np.cumsum(cnt, out=ptr0[1:])
+        col0 = np.concatenate(cols).astype(np.int32) if cnt.sum() else np.zeros(0, np.int32)
+        vl = I.VCSLevelData(lab=rng.integers(0, P, n), P=P, rq=np.ones(P), ptr=ptr0, col=col0, sig=np.zeros(len(col0)))
+        keep = rng.random(len(col0)) < 0.6
+        m = int(rng.integers(0, 3 * n))
+        ni, nR = rng.integers(0, n, m), rng.integers(0, P, m)
+        ptr, col, src = _merge_ws(vl, keep, ni, nR)
+        rows_old = np.repeat(np.arange(n), np.diff(ptr0))
+        ko = np.flatnonzero(keep)
+        r = np.r_[rows_old[ko], ni]
+        c = np.r_[col0[ko].astype(np.int64), nR]
+        s_ = np.r_[ko, np.full(m, -1)]
+        o = np.argsort(r * P + c, kind="stable")
+        want_ptr = np.zeros(n + 1, np.int64)
+        np.cumsum(np.bincount(r, minlength=n), out=want_ptr[1:])
+        assert np.array_equal(ptr, want_ptr) and np.array_equal(col, c[o]) and np.array_equal(src, s_[o]), trial
