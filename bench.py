@@ -8,7 +8,7 @@ Inputs are resident in HBM before the timed region; the timed region is K back-t
 device synchronisation.
 
 Also reported on the same instance: the HIP path with BF16-stored V and with FP64 V, the plain FP64 GPU form
-(torch FP64 CSR SpMM + gathers/index_add), and the FP64 C/OpenMP oracle on the host cores (cpu_baseline). The
+(torch FP64 CSR SpMM, gathers and sparse products), and the FP64 C/OpenMP oracle on the host cores (cpu_baseline). The
 metric's second half, batched bound latency, comes from a Poisson stream of whole bound requests (`serving`:
 p50 / p99 latency, solves/s, certified count) and from the batched evaluation of 512 uneven graphs in one launch
 per kernel (`batch`).

@@ -27,7 +27,8 @@ def main():
     torch.cuda.synchronize()
     s = (time.perf_counter() - t0) / a.steps
     print(json.dumps({"evals_per_s": round(1.0 / s, 3), "ms_per_eval": round(s * 1e3, 4), "f": float(f),
-                      "form": "torch FP64 CSR SpMM + gathers/index_add", "n": inst.n}), flush=True)
+                      "form": "torch FP64: CSR SpMM, gathers, sparse products for the cluster sums and the triangle scatter",
+                      "n": inst.n}), flush=True)
 
 
 if __name__ == "__main__":

@@ -1,5 +1,5 @@
 """The plain FP64 GPU form of one f+grad evaluation: the same mathematics as the HIP kernels with straightforward
-torch operations (FP64 CSR SpMM, gathers, index_add_). It is the comparison baseline timed beside the HIP path
+torch operations (FP64 CSR SpMM, gathers, sparse products, index_add_ where few rows collide). It is the comparison baseline timed beside the HIP path
 (`vs FP64` in the metric) and is not used by the search. Mirrors `augmented` + `CV` (alm_vcs.py:1496-1533,
 1299-1312). Sums into a few cluster rows (Y per label set, Z per VCS level) are sparse products with the
 cluster-membership matrices rather than index_add_ onto 64 rows, whose atomics all collide."""
