@@ -191,6 +191,10 @@ def main():
         extra = {}
         for dt, sv, sc in (("bf16", 2, 4), ("f64", 8, 8)):
             e2 = Engine(r=r, dtype=dt)
+            if tuned:                                   # the FP32 run's launch shapes, so the variants compare
+                TU.apply(e2, tuned)
+            elif tune:
+                e2.set_tuning(*tune)
             e2.set_instance(inst)
             e2.set_V(V)
             s2 = time_engine(e2, rho, max(20, a.steps // 2), 5)
