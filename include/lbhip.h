@@ -212,6 +212,24 @@ int lb_batch_eval(lb_ctx *ctx, double rho, double rt, double *parts);
 int lb_capture_eval(lb_ctx *ctx, double rho, double rt);
 int lb_replay_eval(lb_ctx *ctx, int count, double *parts);
 
+/* Device eigensolver blocks (the pencil LOBPCG of the candidate bound, est_dual_vcs.py:325-364, with its n-sized
+ * blocks kept on the device; no reference counterpart at this granularity: the reference runs scipy's LOBPCG on
+ * the host). FP64 contexts only. A block is n x r FP64, passed in the caller's numbering (lb_set_row_order applies).
+ * lb_blk_alloc reserves nblk blocks and ndiag n-vectors (diagonals). lb_blk_op: dst = Diag(w) src - C(y) src with
+ * the instance's multipliers (the evaluation at rho = 0; it overwrites the context's V; diag < 0 drops the Diag
+ * term). lb_blk_elem kind 1: dst = Diag(w) src; kind 2: dst = other - Diag(w) src Diag(theta) (theta: r values).
+ * lb_blk_gram: out = [A_a0 .. A_a(na-1)]^T Diag(w) [B_b0 ..] as (na r) x (nb r) row-major (diag < 0: no weights),
+ * summed in a fixed order. lb_blk_comb: dst (+)= sum_j src_j coef_j, coef_j r x r row-major; dst must not be a
+ * source. At most 6 blocks per side. */
+int lb_blk_alloc(lb_ctx *ctx, int nblk, int ndiag);
+int lb_blk_set(lb_ctx *ctx, int k, const double *block);
+int lb_blk_get(lb_ctx *ctx, int k, double *block);
+int lb_blk_set_diag(lb_ctx *ctx, int k, const double *diag);
+int lb_blk_op(lb_ctx *ctx, int dst, int src, int diag);
+int lb_blk_elem(lb_ctx *ctx, int kind, int dst, int src, int other, int diag, const double *theta);
+int lb_blk_gram(lb_ctx *ctx, int na, const int *a, int nb, const int *b, int diag, double *out);
+int lb_blk_comb(lb_ctx *ctx, int dst, int ns, const int *src, const double *coef, int accumulate);
+
 #ifdef __cplusplus
 }
 #endif

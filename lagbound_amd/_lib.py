@@ -19,7 +19,9 @@ EXPORTS = ["lb_create", "lb_destroy", "lb_last_error", "lb_set_graph", "lb_set_l
            "lb_eval_fgrad", "lb_get_G", "lb_get_multipliers", "lb_phr_update", "lb_sweeps", "lb_separate_triangles", "lb_separate_vcs", "lb_set_vcs_age", "lb_get_vcs_state",
            "lb_set_weights", "lb_sync", "lb_time_eval",
            "lb_kernel_name", "lb_set_tuning", "lb_version", "lb_reduce_exact", "lb_set_option", "lb_certify_dense", "lb_batch_set",
-           "lb_batch_set_levels", "lb_batch_eval", "lb_capture_eval", "lb_replay_eval"]
+           "lb_batch_set_levels", "lb_batch_eval", "lb_capture_eval", "lb_replay_eval",
+           "lb_blk_alloc", "lb_blk_set", "lb_blk_get", "lb_blk_set_diag", "lb_blk_op", "lb_blk_elem", "lb_blk_gram",
+           "lb_blk_comb"]
 
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
@@ -86,6 +88,14 @@ def load():
         "lb_replay_eval": ([ctx, C.c_int, _f64p], C.c_int),
         "lb_reduce_exact": ([ctx, C.c_int64, C.c_int64, _i64p, C.POINTER(C.c_uint64), _i64p, _i64p, _i64p, _f64p,
                              _i64p], C.c_int),
+        "lb_blk_alloc": ([ctx, C.c_int, C.c_int], C.c_int),
+        "lb_blk_set": ([ctx, C.c_int, _f64p], C.c_int),
+        "lb_blk_get": ([ctx, C.c_int, _f64p], C.c_int),
+        "lb_blk_set_diag": ([ctx, C.c_int, _f64p], C.c_int),
+        "lb_blk_op": ([ctx, C.c_int, C.c_int, C.c_int], C.c_int),
+        "lb_blk_elem": ([ctx, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _f64p], C.c_int),
+        "lb_blk_gram": ([ctx, C.c_int, _i32p, C.c_int, _i32p, C.c_int, _f64p], C.c_int),
+        "lb_blk_comb": ([ctx, C.c_int, C.c_int, _i32p, _f64p, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -251,10 +251,120 @@ def pencil_lmin(lag, g, s, b, maxiter=400, chunk=20, tol=1e-9, atol=1e-5, eig_ti
     return th, rn, Xe, trace
 
 
+def pencil_lmin_device(lag, g, s, b, maxiter=400, chunk=20, tol=1e-9, atol=1e-5, eig_time=1200.0, X0=None, rng=None,
+                       log=print, tag=""):
+    """pencil_lmin with the n-sized blocks on the device (lb_blk_*): block LOBPCG for the smallest eigenvalues of
+    (Diag(g) - C, Diag(s)) with the Jacobi preconditioner of pencil_lmin. The block is as wide as the context (r
+    columns: the operator costs the same for any width up to r); its Gram matrices and combinations run on the device
+    and only the (3r x 3r) Rayleigh-Ritz problems on the host (the scheme of Knyazev's LOBPCG, as scipy's lobpcg
+    used by est_dual_vcs.py:325-364). Convergence is tested every `chunk` iterations with pencil_lmin's rule. Returns
+    (theta, resid, Xe, trace) like pencil_lmin, with r Ritz pairs (b is only the minimum width)."""
+    import scipy.linalg as sl
+    inst = lag.inst
+    n, L, r = inst.n, inst.L, lag.r
+    e = lag.eng.blk_engine()
+    sums = float(s.sum())
+    diagM = g + inst.d ** 2 / L
+    prec = 1.0 / (np.abs(diagM) + 1e-2 * s)
+    X_, AX_, W_, AW_, P_, AP_, Xn_, AXn_, Pn_, APn_, R_ = range(11)
+    DG, DS, DP, DI = 0, 1, 2, 3                               # diagonals: g, s, preconditioner, 1 / s
+    e.blk_alloc(11, 4)
+    for k, w in ((DG, g), (DS, s), (DP, prec), (DI, 1.0 / s)):
+        e.blk_set_diag(k, np.asarray(w, np.float64))
+    rng = np.random.default_rng(0) if rng is None else rng
+    r0 = 0 if X0 is None else min(X0.shape[1], r)
+    Xh = rng.standard_normal((n, r)) if r0 == 0 else np.c_[X0[:, :r0], rng.standard_normal((n, r - r0))]
+    e.blk_set(X_, Xh)
+    ids = {"X": X_, "AX": AX_, "P": P_, "AP": AP_, "Xn": Xn_, "AXn": AXn_, "Pn": Pn_, "APn": APn_}
+
+    def sym(A):
+        return 0.5 * (A + A.T)
+
+    def bnormalize(src, dst):
+        """dst = src L^-T with L L^T = src^T Diag(s) src; False if the block is rank deficient."""
+        G = sym(e.blk_gram([src], [src], DS))
+        try:
+            Lc = np.linalg.cholesky(G)
+        except np.linalg.LinAlgError:
+            return False
+        e.blk_comb(dst, [src], np.linalg.inv(Lc).T)
+        return True
+
+    if not bnormalize(X_, Xn_):
+        raise np.linalg.LinAlgError("initial block is rank deficient")
+    e.blk_op(AXn_, Xn_, DG)
+    GA = sym(e.blk_gram([Xn_], [AXn_]))
+    th, Q = np.linalg.eigh(GA)
+    e.blk_comb(X_, [Xn_], Q)
+    e.blk_comb(AX_, [AXn_], Q)
+    have_p = False
+    te = time.time()
+    th_prev = np.inf
+    trace = []
+    it = 0
+
+    def residuals():
+        e.blk_elem(2, R_, ids["X"], ids["AX"], DS, th)
+        return np.sqrt(np.maximum(np.diag(e.blk_gram([R_], [R_], DI)), 0.0))
+
+    while True:
+        it += 1
+        X, AX, P, AP = ids["X"], ids["AX"], ids["P"], ids["AP"]
+        e.blk_elem(2, R_, X, AX, DS, th)                      # R = AX - Diag(s) X Theta
+        e.blk_elem(1, W_, R_, -1, DP)                         # W = T R
+        if not bnormalize(W_, R_):
+            break
+        e.blk_op(AW_, R_, DG)
+        W = R_
+        S, AS = ([X, W, P], [AX, AW_, AP]) if have_p else ([X, W], [AX, AW_])
+        GA = sym(e.blk_gram(S, AS))
+        GB = sym(e.blk_gram(S, S, DS))
+        try:
+            vals, C = sl.eigh(GA, GB)
+        except (np.linalg.LinAlgError, ValueError):
+            if not have_p:
+                break
+            S, AS = [X, W], [AX, AW_]                          # restart without the search directions
+            GA, GB = sym(e.blk_gram(S, AS)), sym(e.blk_gram(S, S, DS))
+            vals, C = sl.eigh(GA, GB)
+        C = C[:, :r]
+        th = vals[:r]
+        e.blk_comb(ids["Xn"], S, C)
+        e.blk_comb(ids["AXn"], AS, C)
+        e.blk_comb(ids["Pn"], S[1:], C[r:])
+        e.blk_comb(ids["APn"], AS[1:], C[r:])
+        # W's block is reused as R next time round: the new P must not live there
+        ids["X"], ids["Xn"] = ids["Xn"], ids["X"]
+        ids["AX"], ids["AXn"] = ids["AXn"], ids["AX"]
+        ids["P"], ids["Pn"] = ids["Pn"], ids["P"]
+        ids["AP"], ids["APn"] = ids["APn"], ids["AP"]
+        have_p = True
+        if it % chunk == 0 or it >= maxiter:
+            rn = residuals()
+            trace.append((it, time.time() - te, float(th[0]), float(rn[0])))
+            log(f"  [s={tag}] device lobpcg its {it} theta1 {th[0]:.6e} r1 {rn[0]:.2e}  (shift-units: theta "
+                f"{max(0, -th[0]) * sums / L:.6f} r {rn[0] * sums / L:.2e})")
+            if rn[0] * sums / L < atol and abs(th[0] - th_prev) * sums / L < atol:
+                break
+            if it >= maxiter or time.time() - te > eig_time:
+                break
+            th_prev = th[0]
+    rn = residuals()
+    Xe = e.blk_get(ids["X"])
+    o = np.argsort(th)
+    return th[o], rn[o], Xe[:, o], trace
+
+
+def _device_eig_ok(lag, n):
+    """An FP64 device operator with the block entry points, on a graph large enough for an r-wide block."""
+    eng = getattr(lag, "eng", None)
+    return eng is not None and hasattr(eng, "blk_alloc") and getattr(eng, "dtype", None) == "f64" and n >= 8 * lag.r
+
+
 def estimate(inst, V, lag=None, polish="rgd", rgd_iters=400, polish_time=600.0, ptol=2e-5, scale="vol",
              eig_extra=16, eig_maxiter=400, eig_tol=1e-9, eig_chunk=20, eig_atol=1e-5, eig_time=1200.0, escape=2,
              escape_min=0.002, escape_p=4, escape_ratio=0.3, seed=0, no_w_init=False, device=0, dtype="f64",
-             eig_block=None, device_polish=True, log=print):
+             eig_block=None, device_polish=True, eig_device=True, log=print):
     """Candidate bound at the instance's multipliers (levels' sig, lam, VCS sig): polish, then per scale s the
     pencil eigenvalue, ub = dv + max(0, -(theta_1 - r_1)) sum(s) / L, then up to `escape` rank-escape rounds
     (append the isolated bottom eigenvectors to W, re-polish, re-estimate; the best round is kept) -- est_dual_vcs.py
@@ -262,7 +372,9 @@ def estimate(inst, V, lag=None, polish="rgd", rgd_iters=400, polish_time=600.0, 
     best round's W, g and s (`W`, `g`, `s` keys, for certification). `lag` (anything with .inst/.const/.CX) defaults
     to the device operator. `eig_block` overrides the LOBPCG block width k + eig_extra (a narrower block costs
     less host work per iteration). With the device operator and W as wide as its rank, the polish runs its row work
-    on the device (polish_rgd_device; device_polish=False keeps it on the host). ub is an estimate, not a proof."""
+    on the device (polish_rgd_device; device_polish=False keeps it on the host). With an FP64 device operator the
+    pencil eigenproblem runs as a device-block LOBPCG as wide as the operator (pencil_lmin_device; eig_device=False
+    keeps scipy's LOBPCG on the host). ub is an estimate, not a proof."""
     own = lag is None
     if own:
         lag = DeviceLagrangian(inst, r=V.shape[1], dtype=dtype, device=device)
@@ -290,8 +402,9 @@ def estimate(inst, V, lag=None, polish="rgd", rgd_iters=400, polish_time=600.0, 
                 Uw, Sw, _ = np.linalg.svd(W, full_matrices=False)
                 X0 = Uw[:, :int((Sw > 1e-6 * Sw[0]).sum())]
             te = time.time()
-            th, rn, Xe, trace = pencil_lmin(lag, g, s, b, maxiter=eig_maxiter, chunk=eig_chunk, tol=eig_tol,
-                                            atol=eig_atol, eig_time=eig_time, X0=X0, rng=rng, log=log, tag=sname)
+            eig = pencil_lmin_device if eig_device and _device_eig_ok(lag, n) else pencil_lmin
+            th, rn, Xe, trace = eig(lag, g, s, b, maxiter=eig_maxiter, chunk=eig_chunk, tol=eig_tol, atol=eig_atol,
+                                    eig_time=eig_time, X0=X0, rng=rng, log=log, tag=sname)
             sums = float(s.sum())
             lmin_est = float(th[0] - rn[0])
             shift = max(0.0, -lmin_est) * sums / L

@@ -35,6 +35,7 @@
 // LDS row stride padding (doubles) of per-cluster rows accumulated or read by many slots at once (colsum, the VCS
 // passes): two doubles shift consecutive cluster rows by 16 B, so slots on different clusters spread over the banks
 #define LB_VPAD 2
+#define LB_BLK_MAXS 6      // blocks per Gram side / per combination of the device eigensolver
 #ifndef LB_SEB
 #define LB_SEB 8           // sweep hub segments: edge gathers per slot in flight
 #endif
@@ -2509,6 +2510,9 @@ struct lb_ctx {
     DBuf b_light, b_heavy;            // batch rows by work: one slot per row / one wavefront per row
     int64_t b_nlight = 0, b_nheavy = 0; bool b_lists_ok = false;
     DBuf pol_V, pol_Vn, pol_R, pol_g, pol_part, pol_sum;   // device RGD polish state (n x r FP64, n, partials)
+    // block workspace of the device eigensolver (lb_blk_*): nblk blocks of n x r FP64 and ndiag n-vectors, device
+    // row order; gram partials and combination coefficients
+    DBuf blk, blkdiag, blkpart, blkcoef, blkout; int nblk = 0, ndiag = 0;
     bool pol_ok = false;
     DBuf rowptr, col, w, d, labs, sig, pp, rq, dsq, vlabs, vrq, vptr, vcol, vsig, pv, TI, lam, pt, iptr, rec,
         V, G, Y, z, M, pdiag, levtot, levviol, Z, colsum_part, vcs_part, row_part, seg_part, segs, segout,
@@ -3467,7 +3471,7 @@ void lb_destroy(lb_ctx *c) {
     if (c->side) hipStreamSynchronize(c->side);
     DBuf *bufs[] = {&c->rowptr, &c->col, &c->w, &c->d, &c->labs, &c->sig, &c->pp, &c->rq, &c->dsq, &c->vlabs, &c->vrq,
                     &c->vptr, &c->vcol, &c->vsig, &c->pv, &c->vrow, &c->pol_V, &c->pol_Vn, &c->pol_R, &c->pol_g,
-                    &c->pol_part, &c->pol_sum, &c->vst_part, &c->b_light, &c->b_heavy, &c->sw.hErows, &c->sw.hEseg0, &c->sw.hTrows, &c->sw.hTseg0, &c->sw.hEidx, &c->sw.hTidx,
+                    &c->pol_part, &c->pol_sum, &c->blk, &c->blkdiag, &c->blkpart, &c->blkcoef, &c->blkout, &c->vst_part, &c->b_light, &c->b_heavy, &c->sw.hErows, &c->sw.hEseg0, &c->sw.hTrows, &c->sw.hTseg0, &c->sw.hEidx, &c->sw.hTidx,
                     &c->sw.segs, &c->sw.segout, &c->sw.segkap, &c->TI, &c->lam, &c->pt, &c->iptr, &c->rec, &c->V, &c->G,
                     &c->Y, &c->z, &c->M, &c->pdiag, &c->levtot, &c->levviol, &c->Z, &c->colsum_part, &c->vcs_part,
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
@@ -4091,6 +4095,257 @@ int lb_time_eval(lb_ctx *c, double rho, double rt, int iters, double *ms_per_eva
         for (int k = 0; k <= LB_NKERNELS; k++) hipEventDestroy(ev[k]);
     }
     c->have_eval = true; c->last_rho = rho;
+    return 0;
+}
+
+}  // extern "C"
+
+
+// ================================================================================================ device eigensolver blocks
+// Building blocks of a LOBPCG whose n-sized data stay on the device (the candidate bound's pencil eigenproblem,
+// est_dual_vcs.py:325-364): blocks of n x r FP64 in the context's row order, the operator M X = Diag(g) X - C(y) X
+// through the evaluation at rho = 0 (FP64 contexts), Gram matrices A^T Diag(w) B, block combinations, residuals and
+// diagonal scalings. The small Rayleigh-Ritz problems are solved by the caller.
+namespace {
+
+#define LB_BLK_ROWBLK 128      // row blocks of the Gram kernel
+#define LB_BLK_CH 64           // rows staged per step
+
+struct GramArgs {
+    const double *A[LB_BLK_MAXS]; const double *B[LB_BLK_MAXS]; int na, nb, r; const double *w; int64_t n;
+    double *part;              // per row block: (na r) x (nb r), row-major
+};
+
+// one 16 x 16 tile of the Gram matrix per workgroup (blockIdx.y), over a range of rows (blockIdx.x); the tile's A and
+// B columns of LB_BLK_CH rows are staged in LDS, each thread owns one entry
+__global__ __launch_bounds__(256) void lb_k_blk_gram(GramArgs a) {
+    __shared__ double sa[LB_BLK_CH][16], sb[LB_BLK_CH][16];
+    const int wa = a.na * a.r, wb = a.nb * a.r;
+    const int ntj = (wb + 15) / 16;
+    const int ti = blockIdx.y / ntj, tj = blockIdx.y % ntj;
+    const int ii = threadIdx.x / 16, jj = threadIdx.x % 16;
+    const int64_t per = (a.n + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = (int64_t)blockIdx.x * per, r1 = r0 + per < a.n ? r0 + per : a.n;
+    double acc = 0.0;
+    for (int64_t c0 = r0; c0 < r1; c0 += LB_BLK_CH) {
+        for (int e = threadIdx.x; e < LB_BLK_CH * 16; e += 256) {
+            const int rr = e / 16, cc = e % 16;
+            const int64_t row = c0 + rr;
+            const int ca = ti * 16 + cc, cb = tj * 16 + cc;
+            double va = 0.0, vb = 0.0;
+            if (row < r1 && ca < wa) va = a.A[ca / a.r][row * a.r + ca % a.r];
+            if (row < r1 && cb < wb) {
+                vb = a.B[cb / a.r][row * a.r + cb % a.r];
+                if (a.w) vb *= a.w[row];
+            }
+            sa[rr][cc] = va;
+            sb[rr][cc] = vb;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int rr = 0; rr < LB_BLK_CH; rr++) acc += sa[rr][ii] * sb[rr][jj];
+        __syncthreads();
+    }
+    const int gi = ti * 16 + ii, gj = tj * 16 + jj;
+    if (gi < wa && gj < wb) a.part[(int64_t)blockIdx.x * wa * wb + (int64_t)gi * wb + gj] = acc;
+}
+
+struct CombArgs {
+    const double *S[LB_BLK_MAXS]; int ns, r; int64_t n; const double *coef;   // coef: ns blocks of r x r (row-major)
+    double *dst; int accumulate;
+};
+
+// dst[row, col] (+)= sum_j sum_k S_j[row, k] coef_j[k, col]; the coefficients are staged in LDS
+__global__ __launch_bounds__(256) void lb_k_blk_comb(CombArgs a) {
+    extern __shared__ double sc[];
+    const int nc = a.ns * a.r * a.r;
+    for (int e = threadIdx.x; e < nc; e += 256) sc[e] = a.coef[e];
+    __syncthreads();
+    const int64_t cnt = a.n * a.r;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * 256) {
+        const int64_t row = e / a.r;
+        const int col = (int)(e % a.r);
+        double v = a.accumulate ? a.dst[e] : 0.0;
+        for (int j = 0; j < a.ns; j++) {
+            const double *src = a.S[j] + row * a.r;
+            const double *cf = sc + j * a.r * a.r + col;
+            for (int k = 0; k < a.r; k++) v += src[k] * cf[k * a.r];
+        }
+        a.dst[e] = v;
+    }
+}
+
+struct ThetaArg { double t[32]; };
+
+// kind 0: dst = w o src - G (operator finish); 1: dst = w o src; 2: dst = ax - w o x o theta (columns)
+__global__ void lb_k_blk_elem(int kind, const double *src, const double *G, const double *w, int64_t n, int r,
+                              ThetaArg th, double *dst) {
+    const int64_t cnt = n * r;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = e / r;
+        const double wi = w ? w[row] : 0.0;
+        if (kind == 0) dst[e] = wi * src[e] - G[e];
+        else if (kind == 1) dst[e] = wi * src[e];
+        else dst[e] = G[e] - wi * src[e] * th.t[e % r];
+    }
+}
+
+double *blk_ptr(lb_ctx *c, int k) { return (double *)c->blk.p + (size_t)k * c->n * c->r; }
+double *diag_ptr(lb_ctx *c, int k) { return k < 0 ? nullptr : (double *)c->blkdiag.p + (size_t)k * c->n; }
+
+int blk_check(lb_ctx *c, const char *what) {
+    if (!c || !c->n) return fail(c, -1, std::string(what) + ": no graph");
+    if (c->bG) return fail(c, -1, std::string(what) + ": not available on a batch context");
+    if (!c->nblk) return fail(c, -1, std::string(what) + ": call lb_blk_alloc first");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lb_blk_alloc(lb_ctx *c, int nblk, int ndiag) {
+    drop_capture(c);
+    if (!c || !c->n) return fail(c, -1, "lb_blk_alloc: no graph");
+    if (c->vdtype != LB_F64) return fail(c, -1, "lb_blk_alloc: the block operator needs an FP64 context");
+    if (nblk < 1 || nblk > 64 || ndiag < 0 || ndiag > 16) return fail(c, -1, "lb_blk_alloc: 1..64 blocks, 0..16 diagonals");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t nr = (size_t)c->n * c->r;
+    int rc;
+    if ((rc = dalloc(c, c->blk, sizeof(double) * nr * nblk)) ||
+        (rc = dalloc(c, c->blkdiag, sizeof(double) * (size_t)c->n * std::max(ndiag, 1))))
+        return rc;
+    c->nblk = nblk; c->ndiag = ndiag;
+    return 0;
+}
+
+int lb_blk_set(lb_ctx *c, int k, const double *host) {
+    int rc;
+    if ((rc = blk_check(c, "lb_blk_set"))) return rc;
+    if (k < 0 || k >= c->nblk || !host) return fail(c, -1, "lb_blk_set: bad block or data");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = c->n;
+    const int r = c->r;
+    const size_t bytes = sizeof(double) * (size_t)n * r;
+    const int64_t *ord = c->row_order.empty() ? nullptr : c->row_order.data();
+    void *st = nullptr;
+    if ((rc = host_stage(c, bytes, &st))) return rc;
+    par_rows(n, [&](int64_t a, int64_t b) { rows_to_storage(a, b, r, host, LB_F64, st, ord); });
+    HIPCHK(c, hipMemcpyAsync(blk_ptr(c, k), st, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lb_blk_get(lb_ctx *c, int k, double *host) {
+    int rc;
+    if ((rc = blk_check(c, "lb_blk_get"))) return rc;
+    if (k < 0 || k >= c->nblk || !host) return fail(c, -1, "lb_blk_get: bad block or output");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = c->n;
+    const int r = c->r;
+    const size_t bytes = sizeof(double) * (size_t)n * r;
+    const int64_t *ord = c->row_order.empty() ? nullptr : c->row_order.data();
+    void *st = nullptr;
+    if ((rc = host_stage(c, bytes, &st))) return rc;
+    HIPCHK(c, hipMemcpyAsync(st, blk_ptr(c, k), bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    par_rows(n, [&](int64_t a, int64_t b) { storage_to_rows(a, b, r, st, LB_F64, host, ord); });
+    return 0;
+}
+
+int lb_blk_set_diag(lb_ctx *c, int k, const double *host) {
+    int rc;
+    if ((rc = blk_check(c, "lb_blk_set_diag"))) return rc;
+    if (k < 0 || k >= c->ndiag || !host) return fail(c, -1, "lb_blk_set_diag: bad diagonal or data");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = c->n;
+    std::vector<double> h((size_t)n);
+    if (c->row_order.empty()) std::copy(host, host + n, h.begin());
+    else for (int64_t i = 0; i < n; i++) h[(size_t)i] = host[c->row_order[(size_t)i]];
+    HIPCHK(c, hipMemcpy(diag_ptr(c, k), h.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int lb_blk_op(lb_ctx *c, int dst, int src, int diag) {
+    int rc;
+    if ((rc = blk_check(c, "lb_blk_op"))) return rc;
+    if (dst < 0 || dst >= c->nblk || src < 0 || src >= c->nblk || dst == src || diag >= c->ndiag)
+        return fail(c, -1, "lb_blk_op: bad block or diagonal index");
+    HIPCHK(c, hipSetDevice(c->device));
+    eval_fn f = pick_eval(c->vdtype, c->r);
+    if (!f) return fail(c, -1, "lb_blk_op: unsupported rank / dtype");
+    const int64_t n = c->n;
+    const int r = c->r;
+    HIPCHK(c, hipMemcpyAsync(c->V.p, blk_ptr(c, src), sizeof(double) * (size_t)n * r, hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = f(c, 0.0, 0.0, nullptr))) return rc;
+    c->have_eval = true; c->last_rho = 0.0;
+    ThetaArg th{};
+    hipLaunchKernelGGL(lb_k_blk_elem, dim3(2048), dim3(256), 0, c->stream, 0, (const double *)blk_ptr(c, src),
+                       (const double *)c->G.p, (const double *)diag_ptr(c, diag), n, r, th, blk_ptr(c, dst));
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int lb_blk_elem(lb_ctx *c, int kind, int dst, int src, int other, int diag, const double *theta) {
+    int rc;
+    if ((rc = blk_check(c, "lb_blk_elem"))) return rc;
+    if (kind < 1 || kind > 2 || dst < 0 || dst >= c->nblk || src < 0 || src >= c->nblk || diag < 0 || diag >= c->ndiag ||
+        (kind == 2 && (other < 0 || other >= c->nblk || !theta)))
+        return fail(c, -1, "lb_blk_elem: bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    ThetaArg th{};
+    if (kind == 2) for (int j = 0; j < c->r; j++) th.t[j] = theta[j];
+    hipLaunchKernelGGL(lb_k_blk_elem, dim3(2048), dim3(256), 0, c->stream, kind, (const double *)blk_ptr(c, src),
+                       kind == 2 ? (const double *)blk_ptr(c, other) : nullptr, (const double *)diag_ptr(c, diag), c->n,
+                       c->r, th, blk_ptr(c, dst));
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int lb_blk_gram(lb_ctx *c, int na, const int *a, int nb, const int *b, int diag, double *out) {
+    int rc;
+    if ((rc = blk_check(c, "lb_blk_gram"))) return rc;
+    if (na < 1 || na > LB_BLK_MAXS || nb < 1 || nb > LB_BLK_MAXS || !a || !b || !out || diag >= c->ndiag)
+        return fail(c, -1, "lb_blk_gram: bad arguments");
+    for (int j = 0; j < na; j++) if (a[j] < 0 || a[j] >= c->nblk) return fail(c, -1, "lb_blk_gram: bad block index");
+    for (int j = 0; j < nb; j++) if (b[j] < 0 || b[j] >= c->nblk) return fail(c, -1, "lb_blk_gram: bad block index");
+    HIPCHK(c, hipSetDevice(c->device));
+    GramArgs g{};
+    for (int j = 0; j < na; j++) g.A[j] = blk_ptr(c, a[j]);
+    for (int j = 0; j < nb; j++) g.B[j] = blk_ptr(c, b[j]);
+    g.na = na; g.nb = nb; g.r = c->r; g.w = diag_ptr(c, diag); g.n = c->n;
+    const int wa = na * c->r, wb = nb * c->r, m = wa * wb;
+    if ((rc = dalloc(c, c->blkpart, sizeof(double) * (size_t)LB_BLK_ROWBLK * m)) ||
+        (rc = dalloc(c, c->blkout, sizeof(double) * (size_t)m)))
+        return rc;
+    g.part = (double *)c->blkpart.p;
+    const int tiles = ((wa + 15) / 16) * ((wb + 15) / 16);
+    hipLaunchKernelGGL(lb_k_blk_gram, dim3(LB_BLK_ROWBLK, tiles), dim3(256), 0, c->stream, g);
+    hipLaunchKernelGGL(lb_k_reduce, dim3((m + 63) / 64), dim3(1024), 0, c->stream, (const double *)c->blkpart.p,
+                       LB_BLK_ROWBLK, m, m, (double *)c->blkout.p);
+    HIPCHK(c, hipMemcpyAsync(out, c->blkout.p, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lb_blk_comb(lb_ctx *c, int dst, int ns, const int *src, const double *coef, int accumulate) {
+    int rc;
+    if ((rc = blk_check(c, "lb_blk_comb"))) return rc;
+    if (dst < 0 || dst >= c->nblk || ns < 1 || ns > LB_BLK_MAXS || !src || !coef) return fail(c, -1, "lb_blk_comb: bad arguments");
+    for (int j = 0; j < ns; j++)
+        if (src[j] < 0 || src[j] >= c->nblk || src[j] == dst) return fail(c, -1, "lb_blk_comb: bad source (or the destination)");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int r = c->r;
+    const size_t nc = (size_t)ns * r * r;
+    if ((rc = dalloc(c, c->blkcoef, sizeof(double) * nc))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->blkcoef.p, coef, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream));
+    CombArgs ca{};
+    for (int j = 0; j < ns; j++) ca.S[j] = blk_ptr(c, src[j]);
+    ca.ns = ns; ca.r = r; ca.n = c->n; ca.coef = (const double *)c->blkcoef.p; ca.dst = blk_ptr(c, dst);
+    ca.accumulate = accumulate ? 1 : 0;
+    hipLaunchKernelGGL(lb_k_blk_comb, dim3(2048), dim3(256), sizeof(double) * nc, c->stream, ca);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));     // the coefficients are the caller's memory
     return 0;
 }
 

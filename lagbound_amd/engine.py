@@ -191,6 +191,49 @@ class DeviceEngine:
         self._chk(self.L.lb_get_V(self.ctx, out.ctypes.data_as(C.c_void_p), LB.LB_F64), "lb_get_V")
         return out
 
+    # ------------------------------------------------------------------------------------------------ eigensolver blocks
+    # n x r FP64 blocks and n-vectors kept on the device for the pencil LOBPCG (lb_blk_*; FP64 contexts)
+    def blk_alloc(self, nblk, ndiag):
+        self._chk(self.L.lb_blk_alloc(self.ctx, int(nblk), int(ndiag)), "lb_blk_alloc")
+
+    def blk_set(self, k, X):
+        X = np.ascontiguousarray(X, np.float64)
+        assert X.shape == (self.inst.n, self.r), X.shape
+        self._chk(self.L.lb_blk_set(self.ctx, int(k), ptr(X, _f64p)), "lb_blk_set")
+
+    def blk_get(self, k):
+        out = np.empty((self.inst.n, self.r), np.float64)
+        self._chk(self.L.lb_blk_get(self.ctx, int(k), ptr(out, _f64p)), "lb_blk_get")
+        return out
+
+    def blk_set_diag(self, k, w):
+        w = np.ascontiguousarray(w, np.float64)
+        assert w.shape == (self.inst.n,), w.shape
+        self._chk(self.L.lb_blk_set_diag(self.ctx, int(k), ptr(w, _f64p)), "lb_blk_set_diag")
+
+    def blk_op(self, dst, src, diag):
+        self._chk(self.L.lb_blk_op(self.ctx, int(dst), int(src), int(diag)), "lb_blk_op")
+
+    def blk_elem(self, kind, dst, src, other=-1, diag=-1, theta=None):
+        th = None if theta is None else np.ascontiguousarray(theta, np.float64)
+        self._chk(self.L.lb_blk_elem(self.ctx, int(kind), int(dst), int(src), int(other), int(diag), ptr(th, _f64p)),
+                  "lb_blk_elem")
+
+    def blk_gram(self, a, b, diag=-1):
+        a = np.ascontiguousarray(a, np.int32)
+        b = np.ascontiguousarray(b, np.int32)
+        out = np.empty((len(a) * self.r, len(b) * self.r), np.float64)
+        self._chk(self.L.lb_blk_gram(self.ctx, len(a), ptr(a, _i32p), len(b), ptr(b, _i32p), int(diag), ptr(out, _f64p)),
+                  "lb_blk_gram")
+        return out
+
+    def blk_comb(self, dst, src, coef, accumulate=False):
+        src = np.ascontiguousarray(src, np.int32)
+        coef = np.ascontiguousarray(coef, np.float64)
+        assert coef.shape == (len(src) * self.r, self.r), coef.shape
+        self._chk(self.L.lb_blk_comb(self.ctx, int(dst), len(src), ptr(src, _i32p), ptr(coef, _f64p), int(accumulate)),
+                  "lb_blk_comb")
+
     # ------------------------------------------------------------------------------------------------ compute
     def eval(self, rho: float, rt: float | None = None, fetch: bool = True):
         rt = rho if rt is None else rt

@@ -262,6 +262,10 @@ class Relabeled:
             raise ValueError(f"{what} needs a backend that keeps the row order itself (set_row_order)")
         return self.b
 
+    def blk_engine(self):
+        """The backend itself, for the device eigensolver blocks (their rows cross in the caller's numbering)."""
+        return self._native("the device eigensolver blocks")
+
     def polish_begin(self, V):
         return self._native("polish_begin").polish_begin(V)
 

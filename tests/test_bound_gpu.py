@@ -40,15 +40,79 @@ def test_estimate_device_matches_oracle_and_fp32():
     kw = dict(rgd_iters=40, eig_maxiter=120, log=lambda *a: None)
     print("[bound] estimate f64 device vs oracle", flush=True)
     ref = B.estimate(inst, V, lag=OracleLagrangian(inst, 8), **kw)
-    e64 = B.estimate(inst, V, dtype="f64", **kw)
+    e64 = B.estimate(inst, V, dtype="f64", eig_device=False, **kw)     # the same eigensolver as the reference run
     assert abs(e64["dv"] - ref["dv"]) <= 1e-9 * abs(ref["dv"])
     assert abs(e64["ub"] - ref["ub"]) <= 1e-7 * abs(ref["ub"])
     print("[bound] estimate f32 device", flush=True)
-    e32 = B.estimate(inst, V, dtype="f32", **kw)
+    e32 = B.estimate(inst, V, dtype="f32", eig_device=False, **kw)
     du = abs(e32["ub"] - e64["ub"])
     print(f"[bound] U_f64 {e64['ub']:.9f} U_f32 {e32['ub']:.9f} |dU| {du:.3e}", flush=True)
     assert du <= 1e-4 * abs(e64["ub"])
     cert = B.certify_estimate(inst, e64)
+    assert cert["passed"] and cert["UB"] >= _modularity(inst, lab)
+
+
+def test_device_blocks_match_numpy():
+    """lb_blk_*: Gram matrices (with and without a diagonal weight), combinations, residual / scaling and the operator
+    Diag(g) X - C X against numpy and the oracle's C X, through the degree numbering of the context."""
+    inst, _ = _inst()
+    n, r = inst.n, 8
+    rng = np.random.default_rng(4)
+    X, Y, Z = (rng.standard_normal((n, r)) for _ in range(3))
+    w, gd = rng.random(n) + 0.5, rng.standard_normal(n)
+    lag = B.DeviceLagrangian(inst, r=r, dtype="f64")
+    e = lag.eng.blk_engine()
+    e.blk_alloc(6, 2)
+    e.blk_set_diag(0, gd)
+    e.blk_set_diag(1, w)
+    for k, A in enumerate((X, Y, Z)):
+        e.blk_set(k, A)
+    assert np.array_equal(e.blk_get(1), Y)
+    S = np.c_[X, Y]
+    T = np.c_[Y, Z]
+    tol = 1e-12 * n
+    assert np.abs(e.blk_gram([0, 1], [1, 2]) - S.T @ T).max() <= tol
+    assert np.abs(e.blk_gram([0, 1], [1, 2], 1) - S.T @ (w[:, None] * T)).max() <= tol
+    C = rng.standard_normal((2 * r, r))
+    e.blk_comb(3, [0, 1], C)
+    assert np.abs(e.blk_get(3) - S @ C).max() <= 1e-12 * np.abs(S @ C).max()
+    e.blk_comb(3, [2], C[:r], accumulate=True)
+    assert np.abs(e.blk_get(3) - (S @ C + Z @ C[:r])).max() <= 1e-12 * np.abs(S @ C).max()
+    th = rng.standard_normal(r)
+    e.blk_elem(2, 4, 0, 1, 1, th)
+    assert np.abs(e.blk_get(4) - (Y - w[:, None] * X * th[None, :])).max() <= 1e-12
+    e.blk_elem(1, 4, 2, -1, 1)
+    assert np.abs(e.blk_get(4) - w[:, None] * Z).max() <= 1e-12
+    e.blk_op(5, 0, 0)
+    ref = gd[:, None] * X - OracleLagrangian(inst, r).CX(X)
+    assert np.abs(e.blk_get(5) - ref).max() <= 1e-11 * np.abs(ref).max()
+    lag.close()
+
+
+def test_device_lobpcg_matches_host():
+    """The device-block LOBPCG reaches the host LOBPCG's smallest pencil eigenvalue (both converged to the same
+    absolute tolerance in bound units), and an estimate through it certifies."""
+    inst, lab = _inst()
+    V = np.random.default_rng(2).standard_normal((inst.n, 8))
+    W = V / np.linalg.norm(V, axis=1, keepdims=True)
+    lag = B.DeviceLagrangian(inst, r=8, dtype="f64")
+    dv, g, _ = B.value(lag, W)
+    s = np.maximum(inst.d, 1.0)
+    sums = float(s.sum())
+    kw = dict(maxiter=200, chunk=20, atol=1e-7, log=lambda *a: None)
+    th_h, rn_h, _, _ = B.pencil_lmin(lag, g, s, 8, rng=np.random.default_rng(0), **kw)
+    th_d, rn_d, Xe, tr = B.pencil_lmin_device(lag, g, s, 8, rng=np.random.default_rng(0), **kw)
+    print(f"[bound] lmin host {th_h[0]:.10e} (r {rn_h[0]:.2e}) device {th_d[0]:.10e} (r {rn_d[0]:.2e}) "
+          f"its {tr[-1][0] if tr else 0}", flush=True)
+    assert Xe.shape == (inst.n, 8)
+    assert abs(th_d[0] - th_h[0]) * sums / inst.L <= 1e-6
+    assert rn_d[0] * sums / inst.L <= 1e-5
+    lag.close()
+    est = B.estimate(inst, V, dtype="f64", rgd_iters=40, eig_maxiter=120, log=lambda *a: None)
+    ref = B.estimate(inst, V, dtype="f64", rgd_iters=40, eig_maxiter=120, eig_device=False, log=lambda *a: None)
+    print(f"[bound] ub device-block LOBPCG {est['ub']:.9f}, host LOBPCG {ref['ub']:.9f}", flush=True)
+    assert abs(est["ub"] - ref["ub"]) <= 1e-4 * abs(ref["ub"])
+    cert = B.certify_estimate(inst, est)
     assert cert["passed"] and cert["UB"] >= _modularity(inst, lab)
 
 
