@@ -280,23 +280,20 @@ def pencil_lmin_device(lag, g, s, b, maxiter=400, chunk=20, tol=1e-9, atol=1e-5,
     def sym(A):
         return 0.5 * (A + A.T)
 
-    def bnormalize(src, dst):
-        """dst = src L^-T with L L^T = src^T Diag(s) src; False if the block is rank deficient."""
-        G = sym(e.blk_gram([src], [src], DS))
-        try:
-            Lc = np.linalg.cholesky(G)
-        except np.linalg.LinAlgError:
-            return False
-        e.blk_comb(dst, [src], np.linalg.inv(Lc).T)
-        return True
+    def rr(GA, GB):
+        """Rayleigh-Ritz on span(S) with Gram matrices GA = S^T M S, GB = S^T Diag(s) S: directions in which GB is
+        numerically singular (converged residuals, dependent search directions) are dropped, as soft locking does."""
+        lam, U = np.linalg.eigh(sym(GB))
+        keep = lam > 1e-11 * lam[-1]
+        T = U[:, keep] / np.sqrt(lam[keep])
+        vals, Y = np.linalg.eigh(sym(T.T @ sym(GA) @ T))
+        return vals[:r], (T @ Y)[:, :r]
 
-    if not bnormalize(X_, Xn_):
-        raise np.linalg.LinAlgError("initial block is rank deficient")
-    e.blk_op(AXn_, Xn_, DG)
-    GA = sym(e.blk_gram([Xn_], [AXn_]))
-    th, Q = np.linalg.eigh(GA)
-    e.blk_comb(X_, [Xn_], Q)
-    e.blk_comb(AX_, [AXn_], Q)
+    e.blk_op(AX_, X_, DG)
+    th, C = rr(e.blk_gram([X_], [AX_]), e.blk_gram([X_], [X_], DS))
+    e.blk_comb(Xn_, [X_], C)
+    e.blk_comb(AXn_, [AX_], C)
+    ids["X"], ids["Xn"], ids["AX"], ids["AXn"] = Xn_, X_, AXn_, AX_
     have_p = False
     te = time.time()
     th_prev = np.inf
@@ -311,29 +308,21 @@ def pencil_lmin_device(lag, g, s, b, maxiter=400, chunk=20, tol=1e-9, atol=1e-5,
         it += 1
         X, AX, P, AP = ids["X"], ids["AX"], ids["P"], ids["AP"]
         e.blk_elem(2, R_, X, AX, DS, th)                      # R = AX - Diag(s) X Theta
+        rres = np.sqrt(np.maximum(np.diag(e.blk_gram([R_], [R_], DI)), 0.0))
         e.blk_elem(1, W_, R_, -1, DP)                         # W = T R
-        if not bnormalize(W_, R_):
-            break
+        # unit Diag(s)-norm columns; the columns of converged Ritz pairs (residual below tol) are left out (soft
+        # locking, as scipy's lobpcg), so their rounding noise does not enter the search space
+        dw = np.diag(e.blk_gram([W_], [W_], DS))
+        sc = np.where((dw > 1e-300) & (rres > tol), 1.0 / np.sqrt(np.maximum(dw, 1e-300)), 0.0)
+        e.blk_comb(R_, [W_], np.diag(sc))
         e.blk_op(AW_, R_, DG)
         W = R_
         S, AS = ([X, W, P], [AX, AW_, AP]) if have_p else ([X, W], [AX, AW_])
-        GA = sym(e.blk_gram(S, AS))
-        GB = sym(e.blk_gram(S, S, DS))
-        try:
-            vals, C = sl.eigh(GA, GB)
-        except (np.linalg.LinAlgError, ValueError):
-            if not have_p:
-                break
-            S, AS = [X, W], [AX, AW_]                          # restart without the search directions
-            GA, GB = sym(e.blk_gram(S, AS)), sym(e.blk_gram(S, S, DS))
-            vals, C = sl.eigh(GA, GB)
-        C = C[:, :r]
-        th = vals[:r]
+        th, C = rr(e.blk_gram(S, AS), e.blk_gram(S, S, DS))
         e.blk_comb(ids["Xn"], S, C)
         e.blk_comb(ids["AXn"], AS, C)
         e.blk_comb(ids["Pn"], S[1:], C[r:])
         e.blk_comb(ids["APn"], AS[1:], C[r:])
-        # W's block is reused as R next time round: the new P must not live there
         ids["X"], ids["Xn"] = ids["Xn"], ids["X"]
         ids["AX"], ids["AXn"] = ids["AXn"], ids["AX"]
         ids["P"], ids["Pn"] = ids["Pn"], ids["P"]
