@@ -93,10 +93,11 @@ def nbatch_rule(n):
 
 def footprint_bytes(n, m, k, p: Params):
     """Projected device memory of one request: the solver context (V, G and sweep snapshots in FP32, CSR, the
-    triangle pool at its cap, VCS working set and separation scratch) plus the FP64 estimate context."""
+    triangle pool at its cap, VCS working set and separation scratch) plus the FP64 estimate context with the
+    device eigensolver's eleven n x k blocks and four diagonals."""
     tri_cap = (p.tri_add or 2 * n) * max(1, p.outer // max(1, p.tri_every))
     solver = 6 * n * k * 4 + 2 * m * 12 + tri_cap * (16 * 3 + 16) + 8 * n * 24 + 64 * n
-    est = 3 * n * k * 8 + 2 * m * 12 + tri_cap * (24 * 3 + 16) + 8 * n * 24
+    est = 3 * n * k * 8 + 2 * m * 12 + tri_cap * (24 * 3 + 16) + 8 * n * 24 + 11 * n * k * 8 + 4 * n * 8
     cert = 16 * n * n if n <= p.cert_max_n else 0          # dense M and its magnitudes (FP64)
     return int(1.25 * (solver + est)) + cert + (32 << 20)
 
