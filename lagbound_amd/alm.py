@@ -312,15 +312,7 @@ class Solver:
                     new = np.vstack([new, n1[o1]])
                     val = np.r_[val + 10.0, v1[o1]]           # keep both budgets (wedges first)
             if len(new):
-                kn = (new[:, 0] * n + new[:, 1]) * n + new[:, 2]
-                ex = np.sort((TI[:, 0] * n + TI[:, 1]) * n + TI[:, 2])
-                pos = np.searchsorted(ex, kn)
-                isnew = np.ones(len(kn), bool)
-                ok = pos < len(ex)
-                isnew[ok] = ex[pos[ok]] != kn[ok]
-                new, val, kn = new[isnew], val[isnew], kn[isnew]
-                _, fi = np.unique(kn, return_index=True)
-                new, val = new[fi], val[fi]
+                new, val = _new_triangles(new, val, TI, n)
                 o = np.argsort(-val, kind="stable")[:min(a.tri_add + a.tri1_add, room)]
                 new = new[o]
                 TI = np.vstack([TI, new]) if len(TI) else new
@@ -442,6 +434,21 @@ class Solver:
         self.history = hist
         self.eng.close()
         return hist
+
+
+def _new_triangles(new, val, TI, n):
+    """Candidates that are not in the pool, one per packed key (its first occurrence), in ascending key order
+    (alm_vcs.py:1751-1758: drop existing, dedupe with np.unique). The distinct candidates are found first and then
+    searched in the sorted pool keys: the same set and order as filtering first, with a monotone search."""
+    kn = (new[:, 0] * n + new[:, 1]) * n + new[:, 2]
+    uk, fi = np.unique(kn, return_index=True)
+    ex = np.sort((TI[:, 0] * n + TI[:, 1]) * n + TI[:, 2])
+    pos = np.searchsorted(ex, uk)
+    isnew = np.ones(len(uk), bool)
+    ok = pos < len(ex)
+    isnew[ok] = ex[pos[ok]] != uk[ok]
+    fi = fi[isnew]
+    return new[fi], val[fi]
 
 
 def _take_kept(a, src, dtype):

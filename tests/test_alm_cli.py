@@ -107,3 +107,22 @@ def test_merge_ws_matches_full_sort():
         want_ptr = np.zeros(n + 1, np.int64)
         np.cumsum(np.bincount(r, minlength=n), out=want_ptr[1:])
         assert np.array_equal(ptr, want_ptr) and np.array_equal(col, c[o]) and np.array_equal(src, s_[o]), trial
+
+
+def test_new_triangles_matches_filter_then_unique():
+    """_new_triangles equals dropping pool members first and deduplicating after (the reference's order of steps)
+    on random candidates with repeats and pool hits."""
+    from lagbound_amd.alm import _new_triangles
+    rng = np.random.default_rng(1)
+    for t in range(100):
+        n, m, T = int(rng.integers(3, 40)), int(rng.integers(0, 60)), int(rng.integers(0, 50))
+        new, val, TI = rng.integers(0, n, (m, 3)), rng.random(m), rng.integers(0, n, (T, 3))
+        if m and T:
+            TI[:min(T, m) // 2] = new[:min(T, m) // 2]
+        got_new, got_val = _new_triangles(new, val, TI, n)
+        kn = (new[:, 0] * n + new[:, 1]) * n + new[:, 2]
+        pool = set(((TI[:, 0] * n + TI[:, 1]) * n + TI[:, 2]).tolist())
+        keep = np.array([k not in pool for k in kn.tolist()], bool)
+        k2, v2, n2 = kn[keep], val[keep], new[keep]
+        _, f = np.unique(k2, return_index=True)
+        assert np.array_equal(got_new, n2[f]) and np.array_equal(got_val, v2[f]), t
