@@ -475,7 +475,12 @@ def _merge_ws(vl, keep, ni, nR):
         ptr = np.zeros(n + 1, np.int64)
         np.cumsum(np.bincount(np.r_[ro, rn], minlength=n), out=ptr[1:])
         return ptr, c.astype(np.int32), src
-    on = np.argsort(rn * P + cn, kind="stable")               # only the new entries are sorted
+    kn = rn * P + cn
+    m = len(kn)
+    if m and int(kn.max()) < (1 << 40) and m < (1 << 23):    # only the new entries are sorted: (key, position)
+        on = np.sort((kn << 23) | np.arange(m, dtype=np.int64)) & ((1 << 23) - 1)   # packed, so an unstable sort
+    else:                                                     # gives the stable order
+        on = np.argsort(kn, kind="stable")
     rn, cn = rn[on], cn[on]
     # merge by the key i P + R, kept entries first on equal keys (the order of a stable sort of [kept, new])
     at = np.searchsorted(ko_key, rn * P + cn, side="right") + np.arange(len(rn))
