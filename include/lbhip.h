@@ -6,7 +6,8 @@
  * conventions this header keeps: plain C, 64-bit sizes, caller-owned host arrays borrowed for the call only, an int
  * status (0 ok, < 0 usage / allocation / HIP error, > 0 numerical), no exceptions across the ABI.
  *
- * A context owns one device stream and every device buffer (graph, levels, working sets, triangle pool, V, G,
+ * A context owns two device streams (the second runs the evaluation's hub-segment kernel beside the cluster-sum and
+ * VCS kernels, joined by events) and every device buffer (graph, levels, working sets, triangle pool, V, G,
  * multipliers, scratch). V and the multipliers stay resident on the device between calls. Calls on one context are
  * serialised by the caller (one context per host thread); several contexts may share a device.
  *
@@ -165,7 +166,8 @@ int lb_set_option(lb_ctx *ctx, const char *name, int64_t value);
  *   lb_polish_accept: W <- Wn (the last candidate), new R; out4 = (sum Rr . R, sy, ss, yy), the Barzilai-Borwein
  *     sums sum S.(Y nr), S.(S nr), Y.(Y nr) with S = Wn - W, Y = Rn - R, nr the new row norms of C Wn;
  *   lb_polish_get: W and g of the current iterate (either pointer may be NULL).
- * A trial overwrites the context's V. No reference counterpart beyond the polish itself (its loop is numpy). */
+ * A trial overwrites the context's V. lb_set_graph discards the polish state (the polish calls then fail until the
+ * next lb_polish_begin). No reference counterpart beyond the polish itself (its loop is numpy). */
 int lb_polish_begin(lb_ctx *ctx, const double *W, double *out);
 int lb_polish_trial(lb_ctx *ctx, double alpha, double *gsum);
 int lb_polish_accept(lb_ctx *ctx, double *out4);
@@ -220,7 +222,8 @@ int lb_replay_eval(lb_ctx *ctx, int count, double *parts);
  * term). lb_blk_elem kind 1: dst = Diag(w) src; kind 2: dst = other - Diag(w) src Diag(theta) (theta: r values).
  * lb_blk_gram: out = [A_a0 .. A_a(na-1)]^T Diag(w) [B_b0 ..] as (na r) x (nb r) row-major (diag < 0: no weights),
  * summed in a fixed order. lb_blk_comb: dst (+)= sum_j src_j coef_j, coef_j r x r row-major; dst must not be a
- * source. At most 6 blocks per side. */
+ * source. At most 6 blocks per side. lb_set_graph releases the reservation: the block calls then fail (-1) until the
+ * next lb_blk_alloc. */
 int lb_blk_alloc(lb_ctx *ctx, int nblk, int ndiag);
 int lb_blk_set(lb_ctx *ctx, int k, const double *block);
 int lb_blk_get(lb_ctx *ctx, int k, double *block);

@@ -3535,6 +3535,8 @@ int lb_set_graph(lb_ctx *c, int64_t n, const int64_t *rowptr, const int32_t *col
         if (col[e] < 0 || col[e] >= n) return fail(c, -1, "lb_set_graph: column index out of range");
     c->n = n; c->nnz = nnz; c->L = L; c->C0 = C0; c->weighted = weights != nullptr;
     c->bG = 0; c->blev = false; c->bsumP = 0;
+    // the eigensolver blocks and the polish state are sized by n: a new graph invalidates both
+    c->nblk = 0; c->ndiag = 0; c->pol_ok = false;
     c->row_order.clear();
     c->b_lists_ok = false;
     c->h_rowptr.assign(rowptr, rowptr + n + 1);
@@ -4262,7 +4264,9 @@ int lb_blk_set_diag(lb_ctx *c, int k, const double *host) {
     std::vector<double> h((size_t)n);
     if (c->row_order.empty()) std::copy(host, host + n, h.begin());
     else for (int64_t i = 0; i < n; i++) h[(size_t)i] = host[c->row_order[(size_t)i]];
-    HIPCHK(c, hipMemcpy(diag_ptr(c, k), h.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+    // on the context's stream, after any block work still queued there, and complete before h goes away
+    HIPCHK(c, hipMemcpyAsync(diag_ptr(c, k), h.data(), sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -179,3 +179,42 @@ def test_device_polish_matches_host_polish(dtype, ftol, vtol):
     assert np.abs(Vd - Vh).max() <= vtol
     assert np.abs(np.linalg.norm(Vd, axis=1) - 1).max() <= 1e-12
     assert np.abs(gd - gh).max() <= vtol * max(1.0, np.abs(gh).max())
+
+
+def test_new_graph_invalidates_blocks_and_polish():
+    """lb_set_graph releases the eigensolver block reservation and the polish state, which are sized by n: after a
+    second, larger graph the block and polish calls return -1 (usage error) instead of indexing the old buffers."""
+    import ctypes as C
+    from lagbound_amd import _lib
+    L = _lib.load()
+    ctx = C.c_void_p()
+    assert L.lb_create(C.byref(ctx), 0, _lib.LB_F64, 8) == 0
+    try:
+        f64p = C.POINTER(C.c_double)
+        for k, n in enumerate((200, 900)):
+            inst = I.build_instance(*I.chung_lu(n, 2.1, 6.0, 3 + k))
+            rp = np.ascontiguousarray(inst.rowptr, np.int64)
+            col = np.ascontiguousarray(inst.col, np.int32)
+            d = np.ascontiguousarray(inst.d, np.float64)
+            rc = L.lb_set_graph(ctx, inst.n, rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                col.ctypes.data_as(C.POINTER(C.c_int32)), None, d.ctypes.data_as(f64p), inst.L, 0.0)
+            assert rc == 0, L.lb_last_error(ctx)
+            if k == 0:
+                assert L.lb_blk_alloc(ctx, 3, 1) == 0
+                Z0 = np.zeros((inst.n, 8))
+                assert L.lb_blk_set(ctx, 0, Z0.ctypes.data_as(f64p)) == 0
+                W = np.random.default_rng(0).standard_normal((inst.n, 8))
+                W /= np.linalg.norm(W, axis=1, keepdims=True)
+                out = np.zeros(2)
+                assert L.lb_polish_begin(ctx, W.ctypes.data_as(f64p), out.ctypes.data_as(f64p)) == 0
+                assert L.lb_blk_op(ctx, 1, 0, -1) == 0
+        assert L.lb_blk_op(ctx, 1, 0, -1) == -1
+        assert b"lb_blk_alloc" in L.lb_last_error(ctx)
+        g = np.zeros(1)
+        assert L.lb_polish_trial(ctx, 0.5, g.ctypes.data_as(f64p)) == -1
+        Z1 = np.zeros((inst.n, 8))
+        assert L.lb_blk_alloc(ctx, 3, 1) == 0 and L.lb_blk_set(ctx, 0, Z1.ctypes.data_as(f64p)) == 0
+        assert L.lb_blk_op(ctx, 1, 0, -1) == 0
+        assert L.lb_sync(ctx) == 0
+    finally:
+        L.lb_destroy(ctx)

@@ -62,3 +62,10 @@ def test_process_slots_report_worker_errors():
         serve.Server(p, slots=2).run(reqs, realtime=False)
     with pytest.raises(ValueError):
         serve.Server(p, workers="fiber")
+
+
+def test_thread_slots_refused_with_the_device_library():
+    """Thread slots serve only with a host engine: with the device library as the solver's engine they are refused
+    (their one device run ended in a host segmentation fault of unknown cause)."""
+    with pytest.raises(RuntimeError, match="thread slots"):
+        serve.Server(serve.Params(), workers="thread")

@@ -10,8 +10,9 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.skip(reason="one host segmentation fault inside lb_eval_fgrad while three workers were in device calls "
-                         "(one in its first lb_certify_dense); cause not established, kept off the device until it is "
-                         "(the alone-versus-mixed check runs on the CPU in test_serve.py)")
+                         "(one in its first lb_certify_dense); cause not established, so serve.Server refuses thread "
+                         "slots with the device library (the alone-versus-mixed check of thread slots runs on the CPU "
+                         "in test_serve.py)")
 def test_alone_equals_mixed_on_device():
     reqs = serve.poisson_stream(6, 200.0, 800, 3000, 6, 20, seed=5)
     p = serve.Params(outer=12, tri_every=4, vcs_every=3, rgd_iters=60, eig_maxiter=100, cert_max_n=3000)
