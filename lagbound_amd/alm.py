@@ -100,6 +100,46 @@ def parse(argv=None):
     return a
 
 
+def _pw_constants(spec):
+    """(c1, c2) of the pair-weight rule: 0.1 and 0.05 unless `spec` is "c1,c2" (an empty field keeps its default)."""
+    c = [0.1, 0.05]
+    if spec not in ("default", "on", "1", "", None):
+        for k, tok in enumerate(str(spec).split(",")[:2]):
+            if tok:
+                c[k] = float(tok)
+    return c[0], c[1]
+
+
+def pair_weight_rule(inst, V, rho, w0, spec="default"):
+    """Pair-level weight of the search (the reference's --pw_rule, alm_vcs.py:1663-1688): the smaller of
+      w_mob  = c1 / R(1), R(w) = sum_i d_i kappa_i(w) / max(|(B V)_i|, 1e-9) / sum_i d_i, the volume-weighted ratio of
+               a row's pair-level curvature kappa_i(w) = d_i^2 sum_levels K_Q(i), K_Q = 2 rho (sum_R r_QR |Y_R|^2 +
+               r_QQ |Y_Q|^2), r = w rq rq^T, Y = cluster sums of d V, to its objective gradient (B V)_i; R is linear
+               in w, so R(1) = R(w0) / w0;
+      w_step = c2 / (rho n^2 / L).
+    Returns (w, info)."""
+    import scipy.sparse as sp
+    c1, c2 = _pw_constants(spec)
+    d = inst.d
+    A = sp.csr_matrix((np.ones(inst.nnz) if inst.w is None else inst.w, inst.col, inst.rowptr), shape=(inst.n, inst.n))
+    grad = np.linalg.norm(A @ V - np.outer(d, d @ V) / inst.L, axis=1)
+    curv = np.zeros(inst.n)
+    for lv in inst.levels:
+        Y = np.zeros((lv.P, V.shape[1]))
+        np.add.at(Y, lv.lab, d[:, None] * V)
+        ysq = np.einsum("ij,ij->i", Y, Y)
+        rr = w0 * np.outer(lv.rq, lv.rq)
+        K = 2.0 * rho * (rr @ ysq + np.diag(rr) * ysq)
+        curv += d ** 2 * K[lv.lab]
+    R_w0 = float(np.dot(d, curv / np.maximum(grad, 1e-9)) / d.sum())
+    R1 = R_w0 / w0
+    n2L = inst.n * float(inst.n) / inst.L
+    w_mob, w_step = c1 / R1, c2 / (rho * n2L)
+    w = min(w_mob, w_step)
+    return w, dict(R1=R1, R_w0=R_w0, w_mob=w_mob, w_step=w_step, n2L=n2L, c1=c1, c2=c2,
+                   binding="mobility" if w_mob <= w_step else "step")
+
+
 class Solver:
     """The solver stage. From the command line it reads DATA_DIR/<tag> and the label files named by the flags. In
     process, `inst` (graph only), `labels` / `vcs_labels` (label arrays) and `init_labels` replace the files;
@@ -209,41 +249,13 @@ class Solver:
         self.inst.pairlvl_weight = float(w)
         self.eng.set_weights(float(w), self.a.vc_weight)
 
-    def mobility_ratio(self, V, w=1.0):
-        """Volume-weighted mean over rows of kappa_i / |(B V)_i| at pair weight w, all pairs active, trace rule
-        (mobility_ratio, alm_vcs.py:1663-1675)."""
-        inst = self.inst
-        d = inst.d
-        import scipy.sparse as sp
-        A = sp.csr_matrix((np.ones(inst.nnz) if inst.w is None else inst.w, inst.col, inst.rowptr),
-                          shape=(inst.n, inst.n))
-        gB = np.linalg.norm(A @ V - np.outer(d, d @ V) / inst.L, axis=1)
-        kap = np.zeros(inst.n)
-        for lv in inst.levels:
-            r_ = np.outer(lv.rq, lv.rq) * w
-            Y = np.zeros((lv.P, V.shape[1]))
-            np.add.at(Y, lv.lab, d[:, None] * V)
-            y2 = (Y * Y).sum(1)
-            kap += d ** 2 * (2 * self.a.rho * (r_ @ y2 + np.diag(r_) * y2))[lv.lab]
-        return float(np.sum(d * kap / np.maximum(gB, 1e-9)) / d.sum())
-
     def pw_rule(self, V):
-        a = self.a
-        w0 = float(a.pairlvl_weight)
-        R0 = self.mobility_ratio(V, w0)
-        R1 = R0 / w0
-        pr = [] if a.pw_rule in ("default", "on", "1") else a.pw_rule.split(",")
-        c1 = float(pr[0]) if len(pr) > 0 and pr[0] else 0.1
-        c2 = float(pr[1]) if len(pr) > 1 and pr[1] else 0.05
-        n = self.inst.n
-        n2L = n * float(n) / self.inst.L
-        wm = c1 / R1
-        ws = c2 / (a.rho * n2L)
-        self.W_AUTO = min(wm, ws)
-        self.log(f"pw_rule c1 {c1:g} c2 {c2:g}: R(1) {R1:.6g} (R at w {w0:g}: {R0:.4g}), n {n} L {self.inst.L:g} "
-                 f"n^2/L {n2L:.4g} rho {a.rho:g} | w_mob = c1/R {wm:.6g}, w_step = c2/(rho n^2/L) {ws:.6g} -> "
-                 f"pairlvl_weight {self.W_AUTO:.6g} ({'mob' if wm <= ws else 'step'})")
-        self.set_pair_weight(self.W_AUTO)
+        """Choose the pair weight by pair_weight_rule at the solver's rho and install it."""
+        w, info = pair_weight_rule(self.inst, V, self.a.rho, float(self.a.pairlvl_weight), self.a.pw_rule)
+        self.W_AUTO = w
+        self.log(f"pair weight: mobility at unit weight {info['R1']:.6g} -> {info['w_mob']:.6g}, step limit "
+                 f"{info['w_step']:.6g} (n^2/L {info['n2L']:.4g}), {info['binding']} binds -> pairlvl_weight {w:.6g}")
+        self.set_pair_weight(w)
 
     def sweeps(self, nsw, rho, rt, damp):
         n = self.inst.n
@@ -360,15 +372,18 @@ class Solver:
             o += lv.P * lv.P
 
     # ------------------------------------------------------------------------------------------------ main loop
-    def run(self):
+    def run(self, close=True):
+        """The outer loop. With close=False the device context stays open (V and the final multipliers resident) for
+        the caller, who closes it (self.eng.close())."""
         a = self.a
         inst = self.inst
         L = inst.L
         rho = a.rho
-        self.log(f"[C0/L {inst.C0 / L:.6f}] n {inst.n} m {inst.nnz // 2} k {a.k} levels {[lv.P for lv in inst.levels]} "
-                 f"ntri {inst.T} rho {rho} nbatch {a.nbatch} dtype {a.dtype}")
+        self.log(f"search: {inst.n} vertices, {inst.nnz // 2} edges, rank {a.k}, pair levels "
+                 f"{[lv.P for lv in inst.levels]}, {inst.T} triangles, rho {rho}, {a.nbatch} batches, {a.dtype}; "
+                 f"C0/L {inst.C0 / L:.6f}")
         parts = self.eng.eval(rho, rho * a.tri_weight)
-        self.log(f"start L {parts[0] / L:.6f} primal {parts[6] / L:.6f}")
+        self.log(f"initial augmented value {parts[0] / L:.6f}, primal {parts[6] / L:.6f}")
         hist = []
         tsweep = tsep = tvcs = 0.0
         PH2 = False
@@ -381,8 +396,8 @@ class Solver:
                 a.inner = a.phase2_inner
                 a.tri_every = a.phase2_tri_every
                 a.est_every = a.phase2_est_every
-                self.log(f"PHASE2 at outer {it}: inner {a.inner}, pairlvl_weight {a.pairlvl_weight:.4g}, "
-                         f"tri_every {a.tri_every}, est_every {a.est_every}")
+                self.log(f"proximal phase from outer iteration {it}: {a.inner} sweeps per update, pair weight "
+                         f"{a.pairlvl_weight:.4g}, triangle separation every {a.tri_every}")
             ts = time.time()
             self.sweeps(a.inner, rho, rho * a.tri_weight, a.damp)
             self.eng.sync()
@@ -413,8 +428,9 @@ class Solver:
                        tviol=tviol, ntri=int(inst.T))
             if inst.vcs:
                 rec.update(vcs_nnz=[vl.nnz for vl in inst.vcs], vcs_viol=int(parts[10]))
-            self.log(f"it {it} L {Lc / L:.6f} prim {prim / L:.6f} viol {nviol} tviol {tviol:.3f} ntri {inst.T}{msg} "
-                     f"[sweep {tsweep:.0f}s sep {tsep:.0f}s vcs {tvcs:.0f}s]")
+            self.log(f"outer {it}: augmented {Lc / L:.6f}, primal {prim / L:.6f}, {nviol} pair violations, worst "
+                     f"triangle {tviol:.3f}, {inst.T} triangles{msg} (sweeps {tsweep:.0f}s, triangle separation "
+                     f"{tsep:.0f}s, VCS {tvcs:.0f}s)")
             hist.append(rec)
             if a.save and (last or (a.ckpt_every > 0 and it % a.ckpt_every == 0)):
                 self.save(a.save.replace(".npz", "_last.npz"))
@@ -432,7 +448,8 @@ class Solver:
                 vl.sig = sig_all[o:o + vl.nnz].copy()
                 o += vl.nnz
         self.history = hist
-        self.eng.close()
+        if close:
+            self.eng.close()
         return hist
 
 

@@ -1,37 +1,31 @@
 """Candidate bound and its certificate at fixed multipliers y (the stages after the search).
 
-Weak duality (est_dual_vcs.py:1-12): for multipliers y >= 0 of valid inequalities, ANY diagonal g and positive
-weights s,
+Weak duality (the reference's estimate stage, est_dual_vcs.py:1-12): for multipliers y >= 0 of valid inequalities,
+ANY diagonal g and positive weights s,
     max_{X psd, diag X = 1} <C(y), X> <= sum(g) + max(0, -mu) sum(s),   mu = lambda_min(Diag(g) - C(y), Diag(s)),
 so L Q* <= C0 + sum(lam) + sum(g) + max(0, -mu) sum(s).
 
-* `estimate` restates est_dual_vcs.py's production path (--polish rgd --scale vol): Riemannian gradient ascent of
-  <C, W W^T> over unit rows with BB steps and Zhang-Hager nonmonotone acceptance (polish_rgd, est_dual_vcs.py:
-  276-308), g_i = (C W)_i . W_i (value, 245-246), then LOBPCG on the pencil with a Jacobi preconditioner and the
-  residual-hedged value theta_1 - r_1 (eig_round, 327-364). Every C(y) X product runs on the device (the liblbhip
-  evaluation at rho = 0, whose half-gradient is C(y) V exactly); the scalar assembly is FP64 on the host. The result
-  is an ESTIMATE: an eigensolver can miss the smallest eigenvalue.
+* `estimate` follows the behaviour of est_dual_vcs.py's production path (--polish rgd --scale vol, lines 245-389):
+  a preconditioned Riemannian ascent of <C, W W^T> over unit rows (`ascend`), g_i = (C W)_i . W_i, then block
+  LOBPCG on the pencil (`pencil_min`) with the residual-hedged value theta_1 - r_1, and rank-escape rounds. The
+  ascent's row work and the LOBPCG blocks run in the device context (lb_polish_*, lb_blk_*) or on the host, through
+  one implementation of each method; the scalar assembly is FP64. The result is an ESTIMATE: an eigensolver can
+  miss the smallest eigenvalue.
 * `certify_dense` turns (g, shift) into a proved bound for instances small enough for a dense factorisation: it
   assembles M = Diag(g + t) - C(y) in FP64, factors M = R^T R (Cholesky), and charges every rounding error --
   of the assembly (entrywise bound F) and of the factorisation (R^T R = M + E, |E| <= gamma(n+1) |R^T||R|, Higham,
   Accuracy and Stability of Numerical Algorithms, Thm 10.3) -- as one extra uniform diagonal eps >= ||E||_F + ||F||_F,
   so Diag(g + t + eps) - C(y) is PSD in exact arithmetic. The bound is summed in exact rational arithmetic and
-  rounded up (exact_sum / frac_up, verify_cert.py:68-81). It is the dense analogue of the reference's verifier
-  (psd_ldl / accept_price, verify_cert.py:92-104, 363-386) without the lifted tree.
+  rounded up (exact_sum / frac_up, as verify_cert.py:68-81). Larger instances go through the lifted sparse
+  certificate of lagbound_amd.sparse_cert.
 """
 from __future__ import annotations
 
 import math
 import time
-import warnings
 from fractions import Fraction
 
 import numpy as np
-
-# LOBPCG runs in chunks of a few iterations and the chunk loop checks convergence itself: scipy's per-chunk
-# "not reaching the requested tolerance" notices are expected (filters set once; catch_warnings is not thread-safe)
-for _msg in (".*not reaching the requested tolerance.*", ".*Exited at iteration.*", ".*Exited postprocessing.*"):
-    warnings.filterwarnings("ignore", message=_msg, category=UserWarning)
 
 
 # ----------------------------------------------------------------------------------------------- exact arithmetic
@@ -107,348 +101,473 @@ def value(lag, W):
 
 
 # ----------------------------------------------------------------------------------------------- estimate
-def polish_rgd(lag, V, iters=400, polish_time=600.0, ptol=2e-5, log=print):
-    """Riemannian gradient ascent with BB steps and nonmonotone acceptance (polish_rgd, est_dual_vcs.py:276-308)."""
-    L = lag.inst.L
-    tp = time.time()
-    f, g, CW = value(lag, V)
-    Cref, Q, alpha = f, 1.0, 2.0
-    nr = np.maximum(np.linalg.norm(CW, axis=1), 1e-12)
-    Rr = CW - g[:, None] * V
-    R = Rr / nr[:, None]
-    last_chk = f
-    it = 0
-    while it < iters and time.time() - tp < polish_time:
-        it += 1
-        gr = float(np.einsum("ij,ij->", Rr, R)) / L
-        for bt in range(12):
-            Vn = V + alpha * R
-            Vn /= np.linalg.norm(Vn, axis=1, keepdims=True)
-            fn, gn, CWn = value(lag, Vn)
-            if fn >= Cref + 1e-4 * alpha * gr:
-                break
-            alpha *= 0.5
-        S_ = Vn - V
-        nrn = np.maximum(np.linalg.norm(CWn, axis=1), 1e-12)
-        Rrn = CWn - gn[:, None] * Vn
-        Rn = Rrn / nrn[:, None]
-        Y_ = Rn - R
-        sy = float(np.einsum("ij,ij->", S_, Y_ * nrn[:, None]))
-        ss = float(np.einsum("ij,ij->", S_, S_ * nrn[:, None]))
-        yy = float(np.einsum("ij,ij->", Y_, Y_ * nrn[:, None]))
-        if sy < 0:
-            alpha = (ss / (-sy)) if it % 2 else (-sy / yy)
-        else:
-            alpha = min(2 * alpha, 50.0)
-        alpha = float(np.clip(alpha, 0.05, 50.0))
-        V, f, g, CW, R, Rr = Vn, fn, gn, CWn, Rn, Rrn
-        Qn = 0.85 * Q + 1
-        Cref = (0.85 * Q * Cref + f) / Qn
-        Q = Qn
-        if it % 10 == 0:
-            log(f"rgd it {it} dv {f:.6f} alpha {alpha:.3g} bt {bt}")
-            if f - last_chk < ptol * 2:
-                break
-            last_chk = f
-    return V, f, g
+# The estimate has two numerical parts, each written once and run on either of two storage backends:
+#   * ascent: maximise f(W) = <C(y), W W^T> over unit rows W (the oblique manifold) by preconditioned Riemannian
+#     gradient steps; the rows live on the host (HostRows, any operator with .CX) or in the device context
+#     (DeviceRows, lb_polish_*);
+#   * pencil: the smallest eigenvalues of (Diag(g) - C(y), Diag(s)) by block LOBPCG; the blocks live on the host
+#     (HostBlocks) or in the device context (DeviceBlocks, lb_blk_*), and only the small Rayleigh-Ritz problems are
+#     solved on the host in either case.
+# Behaviour of the reference stage (est_dual_vcs.py:245-389): the same objective, preconditioner, step rule,
+# eigenproblem, convergence test and rank-escape rule; its result record keys are kept because its consumers read
+# them.
+
+ZH_ETA = 0.85          # Zhang-Hager averaging weight of the nonmonotone reference value
+ARMIJO = 1e-4          # sufficient-increase constant
+HALVINGS = 12          # step halvings per iteration at most
+STEP_MIN, STEP_MAX = 0.05, 50.0
 
 
-def polish_rgd_device(lag, V, iters=400, polish_time=600.0, ptol=2e-5, log=print):
-    """polish_rgd with the row work on the device (lb_polish_*: step and normalise, g and its sum, new direction and
-    the Barzilai-Borwein sums on an FP64 copy of the iterate); the step rule, the backtracking and the
-    Zhang-Hager reference value are the same as polish_rgd's, on the scalars read back."""
-    eng = lag.eng
-    L = lag.inst.L
-    cst = lag.const
-    tp = time.time()
-    gs, grs = eng.polish_begin(V)
-    f = (cst + gs) / L
-    Cref, Q, alpha = f, 1.0, 2.0
-    last_chk = f
-    it = 0
-    while it < iters and time.time() - tp < polish_time:
-        it += 1
-        gr = grs / L
-        for bt in range(12):
-            fn = (cst + eng.polish_trial(alpha)) / L
-            if fn >= Cref + 1e-4 * alpha * gr:
-                break
-            alpha *= 0.5
-        grs, sy, ss, yy = eng.polish_accept()
-        if sy < 0:
-            alpha = (ss / (-sy)) if it % 2 else (-sy / yy)
-        else:
-            alpha = min(2 * alpha, 50.0)
-        alpha = float(np.clip(alpha, 0.05, 50.0))
-        f = fn
-        Qn = 0.85 * Q + 1
-        Cref = (0.85 * Q * Cref + f) / Qn
-        Q = Qn
-        if it % 10 == 0:
-            log(f"rgd it {it} dv {f:.6f} alpha {alpha:.3g} bt {bt}")
-            if f - last_chk < ptol * 2:
-                break
-            last_chk = f
-    V, g = eng.polish_get()
-    return V, f, g
+class HostRows:
+    """Ascent iterate on the host. State per iterate: W (unit rows), CW = C(y) W, g_i = (C W)_i . W_i, Jacobi row
+    weights nu_i = |(C W)_i| and the preconditioned ascent direction D = (CW - g W) / nu (the tangent gradient
+    divided by the row weight)."""
+
+    def __init__(self, lag):
+        self.lag = lag
+        self.cur = self.cand = None
+
+    def _state(self, W):
+        CW = self.lag.CX(W)
+        g = np.einsum("ij,ij->i", CW, W)
+        nu = np.maximum(np.linalg.norm(CW, axis=1), 1e-12)
+        tg = CW - g[:, None] * W
+        return dict(W=W, g=g, nu=nu, tg=tg, D=tg / nu[:, None])
+
+    def begin(self, W):
+        self.cur = self._state(np.array(W, np.float64))
+        return float(self.cur["g"].sum()), float(np.einsum("ij,ij->", self.cur["tg"], self.cur["D"]))
+
+    def trial(self, step):
+        Wc = self.cur["W"] + step * self.cur["D"]
+        Wc /= np.linalg.norm(Wc, axis=1, keepdims=True)
+        self.cand = self._state(Wc)
+        return float(self.cand["g"].sum())
+
+    def accept(self):
+        a, b = self.cur, self.cand
+        ds, dD = b["W"] - a["W"], b["D"] - a["D"]
+        nu = b["nu"][:, None]
+        sums = (float(np.einsum("ij,ij->", ds, dD * nu)), float(np.einsum("ij,ij->", ds, ds * nu)),
+                float(np.einsum("ij,ij->", dD, dD * nu)))
+        self.cur, self.cand = b, None
+        return (float(np.einsum("ij,ij->", b["tg"], b["D"])),) + sums
+
+    def result(self):
+        return self.cur["W"], self.cur["g"]
 
 
-def _device_polish_ok(lag, W):
+class DeviceRows:
+    """Ascent iterate in the device context (lb_polish_*): the same state as HostRows, on an FP64 copy of W; the
+    calls return the same scalars."""
+
+    def __init__(self, eng):
+        self.eng = eng
+
+    def begin(self, W):
+        return self.eng.polish_begin(W)
+
+    def trial(self, step):
+        return self.eng.polish_trial(step)
+
+    def accept(self):
+        return self.eng.polish_accept()
+
+    def result(self):
+        return self.eng.polish_get()
+
+
+def rows_backend(lag, W, on_device=True):
+    """DeviceRows when the operator is a device context as wide as W, else HostRows."""
     eng = getattr(lag, "eng", None)
-    return eng is not None and hasattr(eng, "polish_begin") and W.shape[1] == getattr(lag, "r", -1)
+    if on_device and eng is not None and hasattr(eng, "polish_begin") and W.shape[1] == getattr(lag, "r", -1):
+        return DeviceRows(eng)
+    return HostRows(lag)
+
+
+def bb_step(k, sy, ss, yy, prev):
+    """Barzilai-Borwein step (Barzilai & Borwein, IMA J. Numer. Anal. 8, 1988) for an ascent, in the row-weighted
+    metric: with curvature sy < 0 along the last step, odd iterations take the long step ss / |sy| and even ones the
+    short step |sy| / yy (alternating BB); otherwise the step doubles. Clipped to [STEP_MIN, STEP_MAX]."""
+    if sy < 0.0:
+        step = ss / -sy if k % 2 else -sy / yy
+    else:
+        step = min(2.0 * prev, STEP_MAX)
+    return float(min(max(step, STEP_MIN), STEP_MAX))
+
+
+def ascend(lag, W, iters=400, time_limit=600.0, ptol=2e-5, on_device=True, log=print):
+    """Maximise f(W) = (const + <C(y), W W^T>) / L over unit rows: W+ = normalise(W + t D) with the Jacobi-
+    preconditioned direction D of the backend, t from bb_step, accepted by the nonmonotone Armijo test of Zhang &
+    Hager (SIAM J. Optim. 14, 2004): f(W+) >= C_k + ARMIJO t <grad, D> / L with the reference value
+    C_{k+1} = (ZH_ETA Q_k C_k + f_{k+1}) / Q_{k+1}, Q_{k+1} = ZH_ETA Q_k + 1; up to HALVINGS halvings of t. Stops
+    after `iters` iterations, after `time_limit` seconds, or when 10 iterations gain less than 2 ptol.
+    Returns (W, f, g)."""
+    rows = rows_backend(lag, W, on_device)
+    L, const = lag.inst.L, lag.const
+    gsum, slope = rows.begin(W)
+    f = (const + gsum) / L
+    ref, qk, step = f, 1.0, 2.0
+    t0 = time.time()
+    f_mark = f
+    for k in range(1, iters + 1):
+        if time.time() - t0 >= time_limit:
+            break
+        for h in range(HALVINGS):
+            f_try = (const + rows.trial(step)) / L
+            if f_try >= ref + ARMIJO * step * slope / L:
+                break
+            step *= 0.5
+        slope, sy, ss, yy = rows.accept()
+        f = f_try
+        step = bb_step(k, sy, ss, yy, step)
+        q_next = ZH_ETA * qk + 1.0
+        ref = (ZH_ETA * qk * ref + f) / q_next
+        qk = q_next
+        if k % 10 == 0:
+            log(f"ascent {k}: value {f:.6f}, step {step:.3g}, halvings {h}")
+            if f - f_mark < 2.0 * ptol:
+                break
+            f_mark = f
+    W, g = rows.result()
+    return W, f, g
 
 
 SCALES = {"ones": lambda d, L, n: np.ones(n), "vol": lambda d, L, n: np.maximum(d, 1.0),
           "sqrtvol": lambda d, L, n: np.sqrt(np.maximum(d, 1.0)) * math.sqrt(L / n)}
 
 
-def pencil_lmin(lag, g, s, b, maxiter=400, chunk=20, tol=1e-9, atol=1e-5, eig_time=1200.0, X0=None, rng=None,
-                log=print, tag=""):
-    """LOBPCG for the smallest eigenvalues of (Diag(g) - C, Diag(s)) with a Jacobi preconditioner, in chunks until
-    the residual and the change of theta_1 (in bound units, x sum(s)/L) are below atol (eig_round,
-    est_dual_vcs.py:325-364). Returns (theta, resid, Xe, trace): Rayleigh quotients and Diag(s)^-1-norm residuals
-    of the final block, sorted ascending."""
-    import scipy.sparse.linalg as sla
-    inst = lag.inst
-    n = inst.n
-    L = inst.L
-    diagM = g + inst.d ** 2 / L                         # C_ii = -d_i^2 / L: the other parts have zero diagonal
-    prec = 1.0 / (np.abs(diagM) + 1e-2 * s)
+class HostBlocks:
+    """LOBPCG blocks as host arrays (n x b). The operator is M X = Diag(g) X - C(y) X through lag.CX; `diags` maps
+    a name to an n-vector used as a diagonal weight."""
 
-    def Mop(X):
-        X = np.asarray(X, np.float64).reshape(n, -1)
-        return g[:, None] * X - lag.CX(X)
+    def __init__(self, lag, g, diags, width):
+        self.lag, self.g, self.diags, self.b = lag, g, diags, int(width)
+        self.buf = {}
 
-    Aop = sla.LinearOperator((n, n), matvec=lambda x: Mop(x)[:, 0], matmat=Mop, dtype=float)
-    Bop = sla.LinearOperator((n, n), matvec=lambda x: s * x.ravel(), matmat=lambda X: s[:, None] * X, dtype=float)
-    Pop = sla.LinearOperator((n, n), matvec=lambda x: prec * x.ravel(), matmat=lambda X: prec[:, None] * X,
-                             dtype=float)
-    rng = np.random.default_rng(0) if rng is None else rng
-    r0 = 0 if X0 is None else min(X0.shape[1], b)
-    Xe = rng.standard_normal((n, b)) if r0 == 0 else np.c_[X0[:, :r0], rng.standard_normal((n, b - r0))]
-    sums = float(s.sum())
-    te = time.time()
-    it_tot = 0
-    th_prev = np.inf
-    trace = []
-    while True:
-        ev, Xe = sla.lobpcg(Aop, Xe, B=Bop, M=Pop, largest=False, maxiter=chunk, tol=tol)
-        it_tot += chunk
-        Xe = Xe[:, np.argsort(ev)]
-        MX = Mop(Xe)
-        xs = np.einsum("ij,ij->j", Xe, s[:, None] * Xe)
-        th = np.einsum("ij,ij->j", Xe, MX) / xs
-        Rres = MX - th[None, :] * (s[:, None] * Xe)
-        rn = np.sqrt(np.einsum("ij,ij->j", Rres, Rres / s[:, None]) / xs)
-        trace.append((it_tot, time.time() - te, float(th[0]), float(rn[0])))
-        log(f"  [s={tag}] lobpcg its {it_tot} theta1 {th[0]:.6e} r1 {rn[0]:.2e}  (shift-units: theta "
-            f"{max(0, -th[0]) * sums / L:.6f} r {rn[0] * sums / L:.2e})")
-        if rn[0] * sums / L < atol and abs(th[0] - th_prev) * sums / L < atol:
-            break
-        if it_tot >= maxiter or time.time() - te > eig_time:
-            break
-        th_prev = th[0]
-    return th, rn, Xe, trace
+    def put(self, name, X):
+        self.buf[name] = np.array(X, np.float64)
+
+    def get(self, name):
+        return self.buf[name]
+
+    def apply(self, dst, src):
+        X = self.buf[src]
+        self.buf[dst] = self.g[:, None] * X - self.lag.CX(X)
+
+    def gram(self, A, B, diag=None):
+        SA = np.hstack([self.buf[a] for a in A])
+        SB = np.hstack([self.buf[b] for b in B])
+        if diag is not None:
+            SB = self.diags[diag][:, None] * SB
+        return SA.T @ SB
+
+    def comb(self, dst, srcs, C):
+        self.buf[dst] = np.hstack([self.buf[s] for s in srcs]) @ C
+
+    def resid(self, dst, X, MX, theta, diag):
+        self.buf[dst] = self.buf[MX] - self.diags[diag][:, None] * self.buf[X] * theta[None, :]
+
+    def scale(self, dst, src, diag):
+        self.buf[dst] = self.diags[diag][:, None] * self.buf[src]
 
 
-def pencil_lmin_device(lag, g, s, b, maxiter=400, chunk=20, tol=1e-9, atol=1e-5, eig_time=1200.0, X0=None, rng=None,
-                       log=print, tag=""):
-    """pencil_lmin with the n-sized blocks on the device (lb_blk_*): block LOBPCG for the smallest eigenvalues of
-    (Diag(g) - C, Diag(s)) with the Jacobi preconditioner of pencil_lmin. The block is as wide as the context (r
-    columns: the operator costs the same for any width up to r); its Gram matrices and combinations run on the device
-    and only the (3r x 3r) Rayleigh-Ritz problems on the host (the scheme of Knyazev's LOBPCG, as scipy's lobpcg
-    used by est_dual_vcs.py:325-364). Convergence is tested every `chunk` iterations with pencil_lmin's rule. Returns
-    (theta, resid, Xe, trace) like pencil_lmin, with r Ritz pairs (b is only the minimum width)."""
-    import scipy.linalg as sl
-    inst = lag.inst
-    n, L, r = inst.n, inst.L, lag.r
-    e = lag.eng.blk_engine()
-    sums = float(s.sum())
-    diagM = g + inst.d ** 2 / L
-    prec = 1.0 / (np.abs(diagM) + 1e-2 * s)
-    X_, AX_, W_, AW_, P_, AP_, Xn_, AXn_, Pn_, APn_, R_ = range(11)
-    DG, DS, DP, DI = 0, 1, 2, 3                               # diagonals: g, s, preconditioner, 1 / s
-    e.blk_alloc(11, 4)
-    for k, w in ((DG, g), (DS, s), (DP, prec), (DI, 1.0 / s)):
-        e.blk_set_diag(k, np.asarray(w, np.float64))
-    rng = np.random.default_rng(0) if rng is None else rng
-    r0 = 0 if X0 is None else min(X0.shape[1], r)
-    Xh = rng.standard_normal((n, r)) if r0 == 0 else np.c_[X0[:, :r0], rng.standard_normal((n, r - r0))]
-    e.blk_set(X_, Xh)
-    ids = {"X": X_, "AX": AX_, "P": P_, "AP": AP_, "Xn": Xn_, "AXn": AXn_, "Pn": Pn_, "APn": APn_}
+class DeviceBlocks:
+    """LOBPCG blocks in the device context (lb_blk_*). A block of width b is m = ceil(b / r) device blocks of the
+    context's r columns; the library takes at most 6 device blocks per side of a Gram matrix or combination, so wider
+    products are assembled from several calls."""
+    NAMES = ("X", "MX", "W", "MW", "P", "MP", "X1", "MX1", "P1", "MP1", "R")
+    SIDE = 6
 
-    def sym(A):
-        return 0.5 * (A + A.T)
+    def __init__(self, eng, r, g, diags, width):
+        self.e, self.r = eng, int(r)
+        self.m = -(-int(width) // self.r)
+        self.b = self.m * self.r
+        if len(self.NAMES) * self.m > 64:
+            raise ValueError(f"LOBPCG block of {width} columns needs more than 64 device blocks of {r}")
+        self.dix = {"g": 0}
+        eng.blk_alloc(len(self.NAMES) * self.m, len(diags) + 1)
+        eng.blk_set_diag(0, np.asarray(g, np.float64))
+        for k, (name, w) in enumerate(diags.items()):
+            self.dix[name] = k + 1
+            eng.blk_set_diag(k + 1, np.asarray(w, np.float64))
+        self.ids = {nm: list(range(k * self.m, (k + 1) * self.m)) for k, nm in enumerate(self.NAMES)}
 
-    def rr(GA, GB):
-        """Rayleigh-Ritz on span(S) with Gram matrices GA = S^T M S, GB = S^T Diag(s) S: directions in which GB is
-        numerically singular (converged residuals, dependent search directions) are dropped, as soft locking does."""
-        lam, U = np.linalg.eigh(sym(GB))
-        keep = lam > 1e-11 * lam[-1]
-        T = U[:, keep] / np.sqrt(lam[keep])
-        vals, Y = np.linalg.eigh(sym(T.T @ sym(GA) @ T))
-        return vals[:r], (T @ Y)[:, :r]
+    def put(self, name, X):
+        for j, bid in enumerate(self.ids[name]):
+            self.e.blk_set(bid, np.ascontiguousarray(X[:, j * self.r:(j + 1) * self.r]))
 
-    e.blk_op(AX_, X_, DG)
-    th, C = rr(e.blk_gram([X_], [AX_]), e.blk_gram([X_], [X_], DS))
-    e.blk_comb(Xn_, [X_], C)
-    e.blk_comb(AXn_, [AX_], C)
-    ids["X"], ids["Xn"], ids["AX"], ids["AXn"] = Xn_, X_, AXn_, AX_
-    have_p = False
-    te = time.time()
-    th_prev = np.inf
-    trace = []
-    it = 0
+    def get(self, name):
+        return np.hstack([self.e.blk_get(bid) for bid in self.ids[name]])
 
-    def residuals():
-        e.blk_elem(2, R_, ids["X"], ids["AX"], DS, th)
-        return np.sqrt(np.maximum(np.diag(e.blk_gram([R_], [R_], DI)), 0.0))
+    def apply(self, dst, src):
+        for a, b in zip(self.ids[dst], self.ids[src]):
+            self.e.blk_op(a, b, self.dix["g"])
 
+    def gram(self, A, B, diag=None):
+        ia = [bid for a in A for bid in self.ids[a]]
+        ib = [bid for b in B for bid in self.ids[b]]
+        r, S = self.r, self.SIDE
+        out = np.empty((len(ia) * r, len(ib) * r))
+        dg = -1 if diag is None else self.dix[diag]
+        for p in range(0, len(ia), S):
+            for q in range(0, len(ib), S):
+                ca, cb = ia[p:p + S], ib[q:q + S]
+                out[p * r:(p + len(ca)) * r, q * r:(q + len(cb)) * r] = self.e.blk_gram(ca, cb, dg)
+        return out
+
+    def comb(self, dst, srcs, C):
+        isrc = [bid for s in srcs for bid in self.ids[s]]
+        r, S = self.r, self.SIDE
+        for j, bid in enumerate(self.ids[dst]):
+            cols = C[:, j * r:(j + 1) * r]
+            for p in range(0, len(isrc), S):
+                chunk = isrc[p:p + S]
+                self.e.blk_comb(bid, chunk, np.ascontiguousarray(cols[p * r:(p + len(chunk)) * r]), accumulate=p > 0)
+
+    def resid(self, dst, X, MX, theta, diag):
+        r = self.r
+        for j, (d_, x_, m_) in enumerate(zip(self.ids[dst], self.ids[X], self.ids[MX])):
+            self.e.blk_elem(2, d_, x_, m_, self.dix[diag], theta[j * r:(j + 1) * r])
+
+    def scale(self, dst, src, diag):
+        for d_, s_ in zip(self.ids[dst], self.ids[src]):
+            self.e.blk_elem(1, d_, s_, -1, self.dix[diag])
+
+
+def _ritz(GA, GB, k):
+    """Rayleigh-Ritz on a subspace S from its Gram matrices GA = S^T M S and GB = S^T Diag(s) S: B-orthonormalise
+    through the eigendecomposition of GB, leaving out the directions in which GB is numerically singular (dependent
+    search directions, converged residuals), and return the k smallest Ritz values with their coefficients."""
+    lam, U = np.linalg.eigh(0.5 * (GB + GB.T))
+    keep = lam > 1e-11 * lam[-1]
+    T = U[:, keep] / np.sqrt(lam[keep])
+    H = T.T @ (0.5 * (GA + GA.T)) @ T
+    vals, Y = np.linalg.eigh(0.5 * (H + H.T))
+    k = min(k, len(vals))
+    return vals[:k], (T @ Y)[:, :k]
+
+
+def lobpcg_pencil(space, X0, s, L, maxiter=400, chunk=20, tol=1e-9, atol=1e-5, time_limit=1200.0, log=print, tag=""):
+    """Block LOBPCG (Knyazev, SIAM J. Sci. Comput. 23(2), 2001) for the smallest eigenvalues of the pencil
+    (M, Diag(s)) with the preconditioner "prec" of `space` (a HostBlocks or DeviceBlocks holding the diagonals "s",
+    "is" = 1/s and "prec"). Each iteration: residuals R = M X - Diag(s) X Theta; W = prec R, columns scaled to unit
+    Diag(s)-norm and those of pairs with residual below tol left out (soft locking); Rayleigh-Ritz on [X, W, P].
+    Every `chunk` iterations the first pair is tested: stop when its Diag(s)^-1-norm residual and the change of its
+    value, both times sum(s) / L (bound units), are below atol. Returns (theta, resid, X, trace) in ascending order."""
+    b = space.b
+    sums = float(np.sum(s))
+    space.put("X", X0)
+    space.apply("MX", "X")
+    theta, C = _ritz(space.gram(["X"], ["MX"]), space.gram(["X"], ["X"], "s"), b)
+    names = {nm: nm for nm in ("X", "MX", "X1", "MX1", "P", "MP", "P1", "MP1")}
+
+    def rotate(pairs):
+        # the loop writes the next X / MX / P / MP into the spare buffers, then swaps the names
+        for a, c in pairs:
+            names[a], names[c] = names[c], names[a]
+
+    space.comb(names["X1"], [names["X"]], C)
+    space.comb(names["MX1"], [names["MX"]], C)
+    rotate((("X", "X1"), ("MX", "MX1")))
+
+    def residual_norms():
+        space.resid("R", names["X"], names["MX"], theta, "s")
+        return np.sqrt(np.maximum(np.diag(space.gram(["R"], ["R"], "is")), 0.0))
+
+    trace, t0, prev, it, searched = [], time.time(), np.inf, 0, False
     while True:
         it += 1
-        X, AX, P, AP = ids["X"], ids["AX"], ids["P"], ids["AP"]
-        e.blk_elem(2, R_, X, AX, DS, th)                      # R = AX - Diag(s) X Theta
-        rres = np.sqrt(np.maximum(np.diag(e.blk_gram([R_], [R_], DI)), 0.0))
-        e.blk_elem(1, W_, R_, -1, DP)                         # W = T R
-        # unit Diag(s)-norm columns; the columns of converged Ritz pairs (residual below tol) are left out (soft
-        # locking, as scipy's lobpcg), so their rounding noise does not enter the search space
-        dw = np.diag(e.blk_gram([W_], [W_], DS))
-        sc = np.where((dw > 1e-300) & (rres > tol), 1.0 / np.sqrt(np.maximum(dw, 1e-300)), 0.0)
-        e.blk_comb(R_, [W_], np.diag(sc))
-        e.blk_op(AW_, R_, DG)
-        W = R_
-        S, AS = ([X, W, P], [AX, AW_, AP]) if have_p else ([X, W], [AX, AW_])
-        th, C = rr(e.blk_gram(S, AS), e.blk_gram(S, S, DS))
-        e.blk_comb(ids["Xn"], S, C)
-        e.blk_comb(ids["AXn"], AS, C)
-        e.blk_comb(ids["Pn"], S[1:], C[r:])
-        e.blk_comb(ids["APn"], AS[1:], C[r:])
-        ids["X"], ids["Xn"] = ids["Xn"], ids["X"]
-        ids["AX"], ids["AXn"] = ids["AXn"], ids["AX"]
-        ids["P"], ids["Pn"] = ids["Pn"], ids["P"]
-        ids["AP"], ids["APn"] = ids["APn"], ids["AP"]
-        have_p = True
+        rn = residual_norms()
+        space.scale("W", "R", "prec")
+        dw = np.diag(space.gram(["W"], ["W"], "s"))
+        live = (dw > 1e-300) & (rn > tol)
+        space.comb("R", ["W"], np.diag(np.where(live, 1.0 / np.sqrt(np.maximum(dw, 1e-300)), 0.0)))
+        space.apply("MW", "R")
+        S = [names["X"], "R"] + ([names["P"]] if searched else [])
+        MS = [names["MX"], "MW"] + ([names["MP"]] if searched else [])
+        theta, C = _ritz(space.gram(S, MS), space.gram(S, S, "s"), b)
+        space.comb(names["X1"], S, C)
+        space.comb(names["MX1"], MS, C)
+        space.comb(names["P1"], S[1:], C[b:])
+        space.comb(names["MP1"], MS[1:], C[b:])
+        rotate((("X", "X1"), ("MX", "MX1"), ("P", "P1"), ("MP", "MP1")))
+        searched = True
         if it % chunk == 0 or it >= maxiter:
-            rn = residuals()
-            trace.append((it, time.time() - te, float(th[0]), float(rn[0])))
-            log(f"  [s={tag}] device lobpcg its {it} theta1 {th[0]:.6e} r1 {rn[0]:.2e}  (shift-units: theta "
-                f"{max(0, -th[0]) * sums / L:.6f} r {rn[0] * sums / L:.2e})")
-            if rn[0] * sums / L < atol and abs(th[0] - th_prev) * sums / L < atol:
+            rn = residual_norms()
+            trace.append((it, time.time() - t0, float(theta[0]), float(rn[0])))
+            log(f"  pencil [{tag}] iteration {it}: theta_1 {theta[0]:.6e}, residual {rn[0]:.2e} (bound units: "
+                f"shift {max(0.0, -theta[0]) * sums / L:.6f}, residual {rn[0] * sums / L:.2e})")
+            if rn[0] * sums / L < atol and abs(theta[0] - prev) * sums / L < atol:
                 break
-            if it >= maxiter or time.time() - te > eig_time:
+            if it >= maxiter or time.time() - t0 > time_limit:
                 break
-            th_prev = th[0]
-    rn = residuals()
-    Xe = e.blk_get(ids["X"])
-    o = np.argsort(th)
-    return th[o], rn[o], Xe[:, o], trace
+            prev = theta[0]
+    rn = residual_norms()
+    X = space.get(names["X"])
+    o = np.argsort(theta)
+    return theta[o], rn[o], X[:, o], trace
 
 
-def _device_eig_ok(lag, n):
-    """An FP64 device operator with the block entry points, on a graph large enough for an r-wide block."""
+def dense_pencil(lag, g, s, k):
+    """The k smallest eigenpairs of (Diag(g) - C(y), Diag(s)) by a dense generalized eigensolver, with C(y) built
+    column by column through the operator (for graphs too small for a block iteration). Residuals are those of the
+    computed pairs, as lobpcg_pencil reports them."""
+    import scipy.linalg as sl
+    n = lag.inst.n
+    C = lag.CX(np.eye(n))
+    M = np.diag(g) - 0.5 * (C + C.T)
+    vals, X = sl.eigh(M, np.diag(s))
+    k = min(k, n)
+    vals, X = vals[:k], X[:, :k]
+    R = M @ X - s[:, None] * X * vals[None, :]
+    xs = np.einsum("ij,ij->j", X, s[:, None] * X)
+    rn = np.sqrt(np.einsum("ij,ij->j", R, R / s[:, None]) / xs)
+    return vals, rn, X, []
+
+
+def pencil_min(lag, g, s, b, X0=None, rng=None, maxiter=400, chunk=20, tol=1e-9, atol=1e-5, time_limit=1200.0,
+               on_device=True, log=print, tag=""):
+    """Smallest eigenvalues of (Diag(g) - C(y), Diag(s)) with a block of b columns: the starting block holds the
+    given directions X0 (e.g. an orthonormal basis of the ascent's W) and random columns up to b. On the device
+    (DeviceBlocks) when the operator is an FP64 device context on a graph of at least 8 r rows, else on the host;
+    dense for graphs of at most 4 b rows. The preconditioner is Jacobi: 1 / (|M_ii| + 0.01 s_i), with
+    M_ii = g_i + d_i^2 / L (C has -d_i^2 / L on its diagonal). Returns (theta, resid, X, trace)."""
+    inst = lag.inst
+    n, L = inst.n, inst.L
+    rng = np.random.default_rng(0) if rng is None else rng
+    if n <= 4 * b:
+        return dense_pencil(lag, g, s, b)
+    prec = 1.0 / (np.abs(g + inst.d ** 2 / L) + 1e-2 * s)
+    diags = {"s": s, "is": 1.0 / s, "prec": prec}
     eng = getattr(lag, "eng", None)
-    return eng is not None and hasattr(eng, "blk_alloc") and getattr(eng, "dtype", None) == "f64" and n >= 8 * lag.r
+    if on_device and eng is not None and hasattr(eng, "blk_alloc") and getattr(eng, "dtype", None) == "f64" \
+            and n >= 8 * lag.r:
+        space = DeviceBlocks(eng.blk_engine(), lag.r, g, diags, b)
+    else:
+        space = HostBlocks(lag, g, diags, b)
+    width = space.b
+    k0 = 0 if X0 is None else min(X0.shape[1], width)
+    cols = [X0[:, :k0]] if k0 else []
+    cols.append(rng.standard_normal((n, width - k0)))
+    return lobpcg_pencil(space, np.hstack(cols), s, L, maxiter=maxiter, chunk=chunk, tol=tol, atol=atol,
+                         time_limit=time_limit, log=log, tag=tag)
+
+
+def escape_directions(theta, X, ratio=0.3, count=4):
+    """Rank escape (Journee, Bach, Absil & Sepulchre, SIAM J. Optim. 20(5), 2010: a stationary point of the rank-k
+    problem that is not optimal has a direction of negative curvature, and adding a column along it escapes): the
+    eigenvectors of at most `count` Ritz values that are negative and at most ratio theta_1, each scaled to max-abs
+    entry 1/2, as new columns for W. None when no value qualifies."""
+    pick = [j for j in range(len(theta)) if theta[j] < 0 and theta[j] <= ratio * theta[0]][:count]
+    if not pick:
+        return None
+    U = X[:, pick]
+    return 0.5 * U / np.abs(U).max(axis=0, keepdims=True)
+
+
+def _fp64_operator(lag, device):
+    """The operator for the FP64 assembly: `lag` itself unless it is an FP32 / BF16 device context, in which case an
+    FP64 context of the same instance (returned as (operator, owned))."""
+    eng = getattr(lag, "eng", None)
+    if eng is None or getattr(eng, "dtype", "f64") == "f64":
+        return lag, False
+    return DeviceLagrangian(lag.inst, r=lag.r, dtype="f64", device=device), True
 
 
 def estimate(inst, V, lag=None, polish="rgd", rgd_iters=400, polish_time=600.0, ptol=2e-5, scale="vol",
              eig_extra=16, eig_maxiter=400, eig_tol=1e-9, eig_chunk=20, eig_atol=1e-5, eig_time=1200.0, escape=2,
              escape_min=0.002, escape_p=4, escape_ratio=0.3, seed=0, no_w_init=False, device=0, dtype="f64",
              eig_block=None, device_polish=True, eig_device=True, log=print):
-    """Candidate bound at the instance's multipliers (levels' sig, lam, VCS sig): polish, then per scale s the
-    pencil eigenvalue, ub = dv + max(0, -(theta_1 - r_1)) sum(s) / L, then up to `escape` rank-escape rounds
-    (append the isolated bottom eigenvectors to W, re-polish, re-estimate; the best round is kept) -- est_dual_vcs.py
-    :316-389. Returns the reference's result record (dv0, dv, s_<scale> records, escape_rounds, ub, ...) plus the
-    best round's W, g and s (`W`, `g`, `s` keys, for certification). `lag` (anything with .inst/.const/.CX) defaults
-    to the device operator. `eig_block` overrides the LOBPCG block width k + eig_extra (a narrower block costs
-    less host work per iteration). With the device operator and W as wide as its rank, the polish runs its row work
-    on the device (polish_rgd_device; device_polish=False keeps it on the host). With an FP64 device operator the
-    pencil eigenproblem runs as a device-block LOBPCG as wide as the operator (pencil_lmin_device; eig_device=False
-    keeps scipy's LOBPCG on the host). ub is an estimate, not a proof."""
+    """Candidate bound at the instance's multipliers. Weak duality: for any g and s > 0,
+    L Q* <= C0 + sum(lam) + sum(g) + max(0, -lambda_min(Diag(g) - C, Diag(s))) sum(s).
+    Steps: ascend from V (the search's embedding) so that g_i = (C W)_i . W_i is a good diagonal; per scale s, the
+    pencil's smallest eigenvalue mu, hedged by its residual (theta_1 - r_1), gives ub = dv + max(0, -mu) sum(s) / L;
+    then up to `escape` rank-escape rounds (new columns from escape_directions, ascent again, new estimate), keeping
+    the best. The ascent runs on `lag`'s precision (dtype: the search's), the values g, dv and the pencil on an FP64
+    operator (the FP64 assembly). The pencil block is eig_block columns, or W's width + eig_extra.
+    Returns the record of the reference stage (dv0, dv, s_<scale>, escape_rounds, ub, ...) plus the best round's W, g,
+    s and lmin for the certificate. ub is an estimate: the eigensolver may stop above the smallest eigenvalue."""
     own = lag is None
     if own:
         lag = DeviceLagrangian(inst, r=V.shape[1], dtype=dtype, device=device)
-    t0 = time.time()
+    asm, own_asm = _fp64_operator(lag, device)
+    t_start = time.time()
     rng = np.random.default_rng(seed)
     n, L = inst.n, inst.L
     scales = scale.split(",")
+    methods = [m for m in polish.split("+") if m not in ("none", "")]
+    for m in methods:
+        if m != "rgd":
+            raise ValueError(f"polish method {m!r} is not supported (rgd | none)")
 
-    def do_polish(W):
-        for meth in polish.split("+"):
-            if meth == "rgd":
-                pol = polish_rgd_device if device_polish and _device_polish_ok(lag, W) else polish_rgd
-                W = pol(lag, W, iters=rgd_iters, polish_time=polish_time, ptol=ptol, log=log)[0]
-            elif meth not in ("none", ""):
-                raise ValueError(f"polish method {meth!r} is not supported (rgd | none)")
+    def improve(W):
+        for _ in methods:
+            W = ascend(lag, W, iters=rgd_iters, time_limit=polish_time, ptol=ptol, on_device=device_polish, log=log)[0]
         return W
 
-    def eig_round(W, dv, g):
-        recs, vecs = {}, {}
-        for sname in scales:
-            s = SCALES[sname](inst.d, L, n)
+    def pencil_round(W, dv, g):
+        """One estimate per scale at the diagonal g: (records by scale, (theta, X, s) by scale)."""
+        recs, eig = {}, {}
+        for name in scales:
+            s = SCALES[name](inst.d, L, n)
             b = W.shape[1] + eig_extra if eig_block is None else int(eig_block)
             X0 = None
             if not no_w_init:
-                Uw, Sw, _ = np.linalg.svd(W, full_matrices=False)
-                X0 = Uw[:, :int((Sw > 1e-6 * Sw[0]).sum())]
-            te = time.time()
-            eig = pencil_lmin_device if eig_device and _device_eig_ok(lag, n) else pencil_lmin
-            th, rn, Xe, trace = eig(lag, g, s, b, maxiter=eig_maxiter, chunk=eig_chunk, tol=eig_tol, atol=eig_atol,
-                                    eig_time=eig_time, X0=X0, rng=rng, log=log, tag=sname)
+                U, sv, _ = np.linalg.svd(W, full_matrices=False)
+                X0 = U[:, :int(np.count_nonzero(sv > 1e-6 * sv[0]))]
+            t = time.time()
+            theta, rn, X, trace = pencil_min(asm, g, s, b, X0=X0, rng=rng, maxiter=eig_maxiter, chunk=eig_chunk,
+                                             tol=eig_tol, atol=eig_atol, time_limit=eig_time, on_device=eig_device,
+                                             log=log, tag=name)
             sums = float(s.sum())
-            lmin_est = float(th[0] - rn[0])
-            shift = max(0.0, -lmin_est) * sums / L
-            recs[sname] = dict(theta=th[:6].tolist(), resid=rn[:6].tolist(), lmin_est=lmin_est, shift=shift,
-                               ub=dv + shift, sum_s=sums, eig_time=time.time() - te, eig_trace=trace,
-                               ub_ritz=dv + max(0.0, -float(th[0])) * sums / L, dv=dv, k=int(W.shape[1]))
-            log(f"[s={sname}] theta {th[:4]} resid {rn[:4]} -> lmin_est {lmin_est:.3e}  shift {shift:.6f}  "
-                f"UB {dv + shift:.6f}")
-            vecs[sname] = (th, Xe, s)
-        return recs, vecs
+            mu = float(theta[0] - rn[0])
+            shift = max(0.0, -mu) * sums / L
+            recs[name] = dict(theta=theta[:6].tolist(), resid=rn[:6].tolist(), lmin_est=mu, shift=shift,
+                              ub=dv + shift, sum_s=sums, eig_time=time.time() - t, eig_trace=trace,
+                              ub_ritz=dv + max(0.0, -float(theta[0])) * sums / L, dv=dv, k=int(W.shape[1]),
+                              block=int(X.shape[1]))
+            log(f"scale {name}: lambda_min ~ {mu:.3e} (theta_1 {theta[0]:.3e}, residual {rn[0]:.1e}); "
+                f"shift {shift:.6f}; candidate {dv + shift:.6f}")
+            eig[name] = (theta, X, s)
+        return recs, eig
 
     try:
         W = V / np.maximum(np.linalg.norm(V, axis=1, keepdims=True), 1e-300)
-        dv0 = value(lag, W)[0]
-        tp = time.time()
-        W = do_polish(W)
-        tpol = time.time() - tp
-        dv, g, _ = value(lag, W)
-        log(f"POLISHED dv {dv:.6f} (start {dv0:.6f}) in {tpol:.0f}s")
-        res = dict(n=n, L=L, k=int(W.shape[1]), dv0=dv0, dv=dv, polish_time=tpol)
-        best, rounds = np.inf, []
+        dv0 = value(asm, W)[0]
+        t = time.time()
+        W = improve(W)
+        t_pol = time.time() - t
+        dv, g, _ = value(asm, W)
+        log(f"ascent: value {dv0:.6f} -> {dv:.6f} in {t_pol:.1f}s")
+        res = dict(n=n, L=L, k=int(W.shape[1]), dv0=dv0, dv=dv, polish_time=t_pol)
+        best, history = np.inf, []
         for rnd in range(escape + 1):
-            recs, vecs = eig_round(W, dv, g)
-            ub_r = min(r["ub"] for r in recs.values())
-            rounds.append(dict(round=rnd, dv=dv, ub=ub_r, k=int(W.shape[1])))
-            log(f"[round {rnd}] k {W.shape[1]} dv {dv:.6f} UB {ub_r:.6f}")
-            if ub_r < best:
-                best = ub_r
-                res["dv"] = dv
-                for sname, rec in recs.items():
-                    res[f"s_{sname}"] = rec
-                bs = min(recs, key=lambda q: recs[q]["ub"])
-                res.update(W=W, g=g, s=vecs[bs][2], lmin=recs[bs]["lmin_est"], best_scale=bs)
-            if rnd == escape:
+            recs, eig = pencil_round(W, dv, g)
+            ub = min(r_["ub"] for r_ in recs.values())
+            history.append(dict(round=rnd, dv=dv, ub=ub, k=int(W.shape[1])))
+            if ub < best:
+                best = ub
+                top = min(recs, key=lambda q: recs[q]["ub"])
+                res.update({f"s_{q}": rec for q, rec in recs.items()})
+                res.update(dv=dv, W=W, g=g, s=eig[top][2], lmin=recs[top]["lmin_est"], best_scale=top)
+            first = scales[0]
+            if rnd == escape or recs[first]["shift"] < escape_min:
                 break
-            th, Xe, _ = vecs[scales[0]]
-            if recs[scales[0]]["shift"] < escape_min:
+            U = escape_directions(eig[first][0], eig[first][1], escape_ratio, escape_p)
+            if U is None:
                 break
-            sel = [j for j in range(len(th)) if th[j] < 0 and th[j] <= escape_ratio * th[0]][:escape_p]
-            if not sel:
-                break
-            U = Xe[:, sel]
-            U = 0.5 * U / np.abs(U).max(axis=0, keepdims=True)
-            W = np.c_[W, U]
+            W = np.hstack([W, U])
             W /= np.linalg.norm(W, axis=1, keepdims=True)
-            W = do_polish(W)
-            dv, g, _ = value(lag, W)
-            log(f"ESCAPE round {rnd + 1}: appended {len(sel)} eigenvectors, re-polished dv {dv:.6f} k {W.shape[1]}")
+            W = improve(W)
+            dv, g, _ = value(asm, W)
+            log(f"rank escape {rnd + 1}: {U.shape[1]} columns added (k = {W.shape[1]}), value {dv:.6f}")
     finally:
+        if own_asm:
+            asm.close()
         if own:
             lag.close()
-    res["escape_rounds"] = rounds
-    res["ub"] = best
-    res["shift"] = best - res["dv"]
-    res["time"] = time.time() - t0
+    res.update(escape_rounds=history, ub=best, shift=best - res["dv"], time=time.time() - t_start)
     return res
 
 

@@ -80,6 +80,9 @@ class Result:
     certified: bool
     UB: float | None
     device_s: float
+    ub_search: float | None = None   # the candidate bound in the search context's precision (same W and shift)
+    du_fp32_fp64: float | None = None  # |ub_search - ub|: FP32 against FP64 assembly of one result
+    cert_note: str | None = None     # why a result is not certified
     log: list = field(default_factory=list)
 
     @property
@@ -112,11 +115,10 @@ def modularity(inst, lab):
     return float((inst.C0 + w[same].sum() - (vol ** 2).sum() / inst.L) / inst.L)
 
 
-def solve_request(req: Request, p: Params, device=0):
-    """The bound pipeline of one request; returns (Result fields without timing, seconds spent inside the device
-    library's calls)."""
-    logs = []
-    log = lambda *s: logs.append(" ".join(str(x) for x in s))
+def search_request(req: Request, p: Params, device=0, log=None):
+    """Instance, label-propagation levels and the device search of one request: returns (inst with the final
+    multipliers, labels, the finished alm.Solver, whose context still holds V and the multipliers)."""
+    log = log or (lambda *a: None)
     inst = I.build_instance(req.n, req.keys)
     lab = I.label_propagation(inst.rowptr, inst.col, inst.n, req.seed, P=p.P, iters=p.lpa_iters)
     nb = p.nbatch or nbatch_rule(inst.n)
@@ -127,20 +129,48 @@ def solve_request(req: Request, p: Params, device=0):
             "--vcs_max", str(16 * inst.n), "--pw_rule", "default", "--dtype", p.dtype, "--device", str(device)]
     a = alm.parse(argv)
     s = alm.Solver(a, inst=inst, labels=[lab], vcs_labels=[lab], init_labels=lab, log=log, signals=False)
-    s.run()
-    lag = B.DeviceLagrangian(inst, r=p.k, dtype="f64", device=device)
+    s.run(close=False)
+    return inst, lab, s
+
+
+def search_precision_bound(s, inst, est):
+    """The candidate bound recomputed with the search context's own evaluation (its storage precision: FP32 for the
+    FP32 search) at the estimate's W and shift: (C0 + sum(lam) + sum_i (C W)_i . W_i) / L + shift. Against the
+    FP64-assembled est["ub"] it gives |U_fp32 - U_fp64| of one result. None when W is wider than the context (after
+    rank-escape rounds)."""
+    W = est["W"]
+    if W.shape[1] != s.eng.r:
+        return None
+    s.eng.set_V(W)
+    parts = s.eng.eval(0.0)
+    return float(parts[0]) / inst.L + est["shift"]
+
+
+def solve_request(req: Request, p: Params, device=0):
+    """The bound pipeline of one request; returns (Result fields without timing, seconds spent inside the device
+    library's calls)."""
+    logs = []
+    log = lambda *s: logs.append(" ".join(str(x) for x in s))
+    inst, lab, s = search_request(req, p, device, log)
     try:
-        est = B.estimate(inst, s.V, lag=lag, rgd_iters=p.rgd_iters, eig_maxiter=p.eig_maxiter, eig_block=p.eig_block,
-                         escape=p.escape, log=log)
+        lag = B.DeviceLagrangian(inst, r=p.k, dtype="f64", device=device)
+        try:
+            est = B.estimate(inst, s.V, lag=lag, rgd_iters=p.rgd_iters, eig_maxiter=p.eig_maxiter,
+                             eig_block=p.eig_block, escape=p.escape, log=log)
+        finally:
+            lag.close()
+        ub_s = search_precision_bound(s, inst, est)
     finally:
-        lag.close()
+        s.eng.close()
     t_dev = s.eng.lib_s + lag.eng.lib_s
     cert = None
     if inst.n <= p.cert_max_n:
         cert = B.certify_estimate(inst, est, margin=p.cert_margin, device=device if p.cert_on_device else None)
+    why = None if (cert and cert["passed"]) else (cert.get("reason") if cert else f"n = {inst.n} above cert_max_n")
     return dict(n=inst.n, m=inst.nnz // 2, lb=modularity(inst, lab.astype(np.int64)), dv=est["dv"], ub=est["ub"],
+                ub_search=ub_s, du_fp32_fp64=(abs(ub_s - est["ub"]) if ub_s is not None else None),
                 certified=bool(cert and cert["passed"]), UB=(cert["UB"] if cert and cert["passed"] else None),
-                log=logs), t_dev
+                cert_note=why, log=logs), t_dev
 
 
 def prepare_device(p: Params, device=0):
@@ -349,7 +379,10 @@ def summarize(results, wall):
             "gap_ub_minus_lb": {"p10": round(float(np.percentile(gaps, 10)), 5),
                                 "p50": round(float(np.percentile(gaps, 50)), 5),
                                 "p90": round(float(np.percentile(gaps, 90)), 5)},
-            "certified": len(cert)}
+            "certified": len(cert),
+            "du_fp32_fp64_max": (max(du) if (du := [r.du_fp32_fp64 for r in results.values()
+                                                      if r.du_fp32_fp64 is not None]) else None),
+            "uncertified": {str(r.rid): r.cert_note for r in results.values() if not r.certified}}
 
 
 def main(argv=None):

@@ -22,9 +22,14 @@ class OracleEngine:
         self.V = None
         self.last = None
         self.ages = None
+        self.closed = False
 
     def close(self):
-        pass
+        self.closed = True              # like the device context: no calls after close
+
+    def _open(self):
+        if self.closed:
+            raise RuntimeError("engine used after close()")
 
     def set_tuning(self, *a, **k):
         pass
@@ -56,6 +61,7 @@ class OracleEngine:
         self.ages = np.asarray(age, np.uint8).copy()
 
     def set_V(self, V):
+        self._open()
         V = np.asarray(V, np.float64)
         assert V.shape == (self.inst.n, self.r)
         self.V = V.copy()
@@ -67,6 +73,7 @@ class OracleEngine:
         pass
 
     def eval(self, rho, rt=None, fetch=True):
+        self._open()
         rt = rho if rt is None else rt
         parts, G, pp, pt, pv = O.eval_c(self.inst, self.V, rho, rt, threads=2)
         self.last = (rho, parts, G, pp, pt, pv)

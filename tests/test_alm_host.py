@@ -76,3 +76,29 @@ def test_solve_estimate_certify_host(tmp_path, monkeypatch, capsys):
     assert abs(cert["UB"] - res["ub"]) <= 1e-7 * sums / inst.L + cert["price"] + 1e-12
     W = np.load(os.path.join(ws.runs, "W.npz"))["V"]
     assert W.shape[0] == n and W.dtype == np.float32
+
+
+def test_pair_weight_rule_closed_form():
+    """pair_weight_rule on the 6-cycle (d_i = 2, L = 12) with V alternating +e1 / -e1 and one level of two clusters
+    (even / odd vertices): (B V)_i = -2 v_i, so |(B V)_i| = 2; Y_Q = +-6 e1, rq_Q = 3 / 36 = 1/12,
+    K_Q = 2 rho w (2 * 36 + 36) / 144 = 1.5 rho w, kappa_i = d_i^2 K = 6 rho w, hence R(w) = kappa / 2 = 3 rho w and
+    R(1) = 3 rho; w_mob = c1 / (3 rho), w_step = c2 / (rho n^2 / L) = c2 / (3 rho). With the defaults (0.1, 0.05) the
+    step limit binds; with c1 = 0.01 the mobility does. R(1) does not depend on the weight the rule is measured at."""
+    from lagbound_amd.alm import pair_weight_rule
+    from lagbound_amd import instances as I
+    n = 6
+    keys = np.array(sorted(min(i, (i + 1) % n) * n + max(i, (i + 1) % n) for i in range(n)), np.int64)
+    inst = I.build_instance(n, keys)
+    lab = np.arange(n) % 2
+    inst.levels = [I.pair_level(lab, inst.d)]
+    V = np.zeros((n, 4))
+    V[:, 0] = np.where(lab == 0, 1.0, -1.0)
+    rho = 0.1
+    w, info = pair_weight_rule(inst, V, rho, 1.0)
+    assert info["R1"] == pytest.approx(3 * rho, rel=1e-12)
+    assert info["w_mob"] == pytest.approx(0.1 / (3 * rho), rel=1e-12)
+    assert info["w_step"] == pytest.approx(0.05 / (3 * rho), rel=1e-12)
+    assert w == pytest.approx(0.05 / (3 * rho), rel=1e-12) and info["binding"] == "step"
+    w2, info2 = pair_weight_rule(inst, V, rho, 7.5, "0.01,")
+    assert info2["R1"] == pytest.approx(3 * rho, rel=1e-12) and info2["c2"] == 0.05
+    assert w2 == pytest.approx(0.01 / (3 * rho), rel=1e-12) and info2["binding"] == "mobility"

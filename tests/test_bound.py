@@ -176,3 +176,74 @@ def test_checkpoint_roundtrip(tmp_path):
     assert np.array_equal(inst2.levels[0].sig, inst.levels[0].sig)
     assert np.array_equal(inst2.TI, inst.TI) and np.array_equal(inst2.lam, inst.lam)
     assert np.array_equal(inst2.vcs[0].sig, inst.vcs[0].sig.astype(np.float32).astype(np.float64))
+
+
+def test_host_lobpcg_matches_dense_pencil():
+    """lobpcg_pencil on host blocks (b = 12 columns, n = 300 > 4 b, so the block iteration runs) reaches the four
+    smallest eigenvalues of the dense pencil (Diag(g) - C, Diag(s)); the reported residuals are those of the returned
+    pairs; a start block holding a basis of W plus random columns reaches the same values."""
+    n_, keys, _ = I.sbm(300, 5, 0.08, 0.01, 7)
+    inst = _with_multipliers(I.build_instance(n_, keys), 4, scale=0.01, ntri=60)
+    rng = np.random.default_rng(1)
+    W = rng.standard_normal((n_, 8))
+    W /= np.linalg.norm(W, axis=1, keepdims=True)
+    lag = OracleLagrangian(inst, 8)
+    _, g, _ = B.value(lag, W)
+    s = np.maximum(inst.d, 1.0)
+    ref = _pencil_min(inst, g, s)
+    Cm, _ = B.dense_C(inst)
+    ref4 = sl.eigh(np.diag(g) - Cm, np.diag(s), eigvals_only=True)[:4]
+    for X0 in (None, np.linalg.svd(W, full_matrices=False)[0]):
+        th, rn, X, trace = B.pencil_min(lag, g, s, 12, X0=X0, rng=np.random.default_rng(2), maxiter=300, atol=1e-9,
+                                        on_device=False, log=lambda *a: None)
+        assert X.shape == (n_, 12) and trace
+        assert abs(th[0] - ref) <= 1e-8 * max(1.0, abs(ref))
+        assert np.allclose(th[:4], ref4, rtol=1e-6, atol=1e-8)
+        MX = g[:, None] * X - lag.CX(X)
+        xs = np.einsum("ij,ij->j", X, s[:, None] * X)
+        R = MX - s[:, None] * X * th[None, :]
+        assert np.allclose(np.sqrt(np.einsum("ij,ij->j", R, R / s[:, None]) / xs), rn, rtol=1e-6, atol=1e-12)
+
+
+def test_dense_pencil_for_tiny_graphs():
+    """At most 4 b rows: the pencil is solved densely (same values as the generalized eigensolver)."""
+    inst = _with_multipliers(_tiny(3, n=9), 2)
+    g = np.random.default_rng(5).standard_normal(inst.n)
+    s = np.maximum(inst.d, 1.0)
+    th, rn, X, _ = B.pencil_min(OracleLagrangian(inst, 8), g, s, 8, log=lambda *a: None)
+    assert abs(th[0] - _pencil_min(inst, g, s)) <= 1e-10 * max(1.0, abs(th[0]))
+    assert rn.max() <= 1e-8
+
+
+def test_ascent_increases_value_and_keeps_unit_rows():
+    """The nonmonotone ascent never ends below its start value, keeps unit rows, and its g is the row products of
+    C W at the returned W."""
+    n_, keys, _ = I.sbm(200, 4, 0.1, 0.01, 9)
+    inst = _with_multipliers(I.build_instance(n_, keys), 6, scale=0.01, ntri=30)
+    lag = OracleLagrangian(inst, 8)
+    W0 = np.random.default_rng(3).standard_normal((n_, 8))
+    W0 /= np.linalg.norm(W0, axis=1, keepdims=True)
+    f0 = B.value(lag, W0)[0]
+    W, f, g = B.ascend(lag, W0, iters=60, on_device=False, log=lambda *a: None)
+    assert f >= f0
+    assert np.abs(np.linalg.norm(W, axis=1) - 1).max() <= 1e-12
+    assert np.allclose(g, np.einsum("ij,ij->i", lag.CX(W), W), rtol=1e-12, atol=1e-12)
+    assert abs(f - B.value(lag, W)[0]) <= 1e-12 * abs(f)
+
+
+def test_bb_step_rule():
+    """Alternating Barzilai-Borwein steps for the ascent (curvature sy < 0): odd iterations ss / |sy|, even |sy| / yy;
+    no curvature: the step doubles; always clipped to [0.05, 50]."""
+    assert B.bb_step(1, -2.0, 6.0, 4.0, 1.0) == 3.0
+    assert B.bb_step(2, -2.0, 6.0, 4.0, 1.0) == 0.5
+    assert B.bb_step(3, 1e-3, 6.0, 4.0, 1.5) == 3.0
+    assert B.bb_step(3, 1e-3, 6.0, 4.0, 40.0) == 50.0
+    assert B.bb_step(1, -1.0, 1e3, 1.0, 1.0) == 50.0 and B.bb_step(2, -1e-6, 1.0, 1.0, 1.0) == 0.05
+
+
+def test_escape_directions():
+    theta = np.array([-1.0, -0.5, -0.2, 0.1])
+    X = np.random.default_rng(0).standard_normal((20, 4))
+    U = B.escape_directions(theta, X, ratio=0.3, count=4)
+    assert U.shape == (20, 2) and np.allclose(np.abs(U).max(axis=0), 0.5)
+    assert B.escape_directions(np.array([0.1, 0.2]), X[:, :2]) is None

@@ -43,11 +43,14 @@ def test_estimate_device_matches_oracle_and_fp32():
     e64 = B.estimate(inst, V, dtype="f64", eig_device=False, **kw)     # the same eigensolver as the reference run
     assert abs(e64["dv"] - ref["dv"]) <= 1e-9 * abs(ref["dv"])
     assert abs(e64["ub"] - ref["ub"]) <= 1e-7 * abs(ref["ub"])
-    print("[bound] estimate f32 device", flush=True)
-    e32 = B.estimate(inst, V, dtype="f32", eig_device=False, **kw)
+    print("[bound] estimate f32 device (FP32 ascent, FP64 assembly and device pencil)", flush=True)
+    e32 = B.estimate(inst, V, dtype="f32", **kw)
     du = abs(e32["ub"] - e64["ub"])
     print(f"[bound] U_f64 {e64['ub']:.9f} U_f32 {e32['ub']:.9f} |dU| {du:.3e}", flush=True)
     assert du <= 1e-4 * abs(e64["ub"])
+    for e in (e64, e32):                  # both pencils converged to the stated tolerance (bound units)
+        rec = e["s_vol"]
+        assert rec["resid"][0] * rec["sum_s"] / inst.L <= 1e-5
     cert = B.certify_estimate(inst, e64)
     assert cert["passed"] and cert["UB"] >= _modularity(inst, lab)
 
@@ -100,8 +103,8 @@ def test_device_lobpcg_matches_host():
     s = np.maximum(inst.d, 1.0)
     sums = float(s.sum())
     kw = dict(maxiter=200, chunk=20, atol=1e-7, log=lambda *a: None)
-    th_h, rn_h, _, _ = B.pencil_lmin(lag, g, s, 8, rng=np.random.default_rng(0), **kw)
-    th_d, rn_d, Xe, tr = B.pencil_lmin_device(lag, g, s, 8, rng=np.random.default_rng(0), **kw)
+    th_h, rn_h, _, _ = B.pencil_min(lag, g, s, 8, rng=np.random.default_rng(0), on_device=False, **kw)
+    th_d, rn_d, Xe, tr = B.pencil_min(lag, g, s, 8, rng=np.random.default_rng(0), **kw)
     print(f"[bound] lmin host {th_h[0]:.10e} (r {rn_h[0]:.2e}) device {th_d[0]:.10e} (r {rn_d[0]:.2e}) "
           f"its {tr[-1][0] if tr else 0}", flush=True)
     assert Xe.shape == (inst.n, 8)
@@ -162,16 +165,17 @@ def test_device_certificate_matches_host(weighted):
 
 @pytest.mark.parametrize("dtype,ftol,vtol", [("f64", 1e-10, 1e-7), ("f32", 1e-6, 1e-3)])
 def test_device_polish_matches_host_polish(dtype, ftol, vtol):
-    """polish_rgd_device (row work on the device, FP64 iterate) against polish_rgd (row work in numpy) on the same
-    device operator: the same step and backtracking decisions, the same iterate up to summation order."""
+    """The ascent with its row work on the device (DeviceRows, FP64 iterate) against the same ascent with its row work
+    in numpy (HostRows) on the same device operator: the same step and backtracking decisions, the same iterate up to
+    summation order."""
     inst, _ = _inst()
     V = np.random.default_rng(3).standard_normal((inst.n, 8))
     V /= np.linalg.norm(V, axis=1, keepdims=True)
     print(f"[bound] device polish dtype={dtype} n={inst.n}", flush=True)
     lag = B.DeviceLagrangian(inst, r=8, dtype=dtype)
     try:
-        Vh, fh, gh = B.polish_rgd(lag, V, iters=40, log=lambda *a: None)
-        Vd, fd, gd = B.polish_rgd_device(lag, V, iters=40, log=lambda *a: None)
+        Vh, fh, gh = B.ascend(lag, V, iters=40, on_device=False, log=lambda *a: None)
+        Vd, fd, gd = B.ascend(lag, V, iters=40, on_device=True, log=lambda *a: None)
     finally:
         lag.close()
     print(f"[bound] f host {fh:.12f} device {fd:.12f}", flush=True)
@@ -218,3 +222,31 @@ def test_new_graph_invalidates_blocks_and_polish():
         assert L.lb_sync(ctx) == 0
     finally:
         L.lb_destroy(ctx)
+
+
+@pytest.mark.parametrize("r,extra", [(8, 16), (16, 16)])
+def test_device_lobpcg_wide_block_matches_dense(r, extra):
+    """A pencil block wider than the context (b = r + 16: 3 or 2 device blocks per LOBPCG block, so the Gram matrices
+    and combinations over [X, W, P] span more than the library's 6 blocks per call) reaches the smallest eigenvalue
+    of the dense pencil (Diag(g) - C, Diag(s)) from scipy.linalg.eigh at n ~ 2000."""
+    import scipy.linalg as sl
+    n_, keys, lab = I.sbm(2000, 8, 0.02, 0.002, 11)
+    inst = _with_multipliers(I.build_instance(n_, keys), 12, scale=0.01, ntri=800)
+    rng = np.random.default_rng(6)
+    W = rng.standard_normal((n_, r))
+    W /= np.linalg.norm(W, axis=1, keepdims=True)
+    lag = B.DeviceLagrangian(inst, r=r, dtype="f64")
+    try:
+        dv, g, _ = B.value(lag, W)
+        s = np.maximum(inst.d, 1.0)
+        X0 = np.linalg.svd(W, full_matrices=False)[0]
+        th, rn, X, tr = B.pencil_min(lag, g, s, r + extra, X0=X0, rng=np.random.default_rng(0), maxiter=400,
+                                     atol=1e-8, log=lambda *a: None)
+    finally:
+        lag.close()
+    Cm, _ = B.dense_C(inst)
+    mu = float(sl.eigh(np.diag(g) - Cm, np.diag(s), eigvals_only=True, subset_by_index=[0, 0])[0])
+    print(f"[bound] wide block r={r} b={X.shape[1]}: theta_1 {th[0]:.12e} dense {mu:.12e} its {tr[-1][0]}", flush=True)
+    assert X.shape == (n_, r + extra)
+    assert abs(th[0] - mu) * float(s.sum()) / inst.L <= 1e-7
+    assert th[0] - rn[0] <= mu + 1e-12 * abs(mu)

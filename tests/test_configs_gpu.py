@@ -5,8 +5,8 @@ test files, test_batch_gpu.py and test_serve_gpu.py):
   against the oracle, r = 8 solve with a 20,000-triangle pool, bit-exact separation on the solved V, candidate
   bound, dense FP64 certificate >= the planted partition's modularity);
 - Chung-Lu n = 300k, gamma 2.1, mean degree 13, 1M wedge triangles, pair + VCS levels over 64 LPA clusters: FP32 and
-  FP64 evaluation against the oracle, bit-exact separation of both families at the 1M budget, and
-  |U_fp32 - U_fp64| of the candidate bound on the same V;
+  FP64 evaluation against the oracle, bit-exact separation of both families at the 1M budget; |U_fp32 - U_fp64| of
+  the converged candidate bound at a solved search's multipliers, and of two whole searches (FP32 and FP64);
 - Chung-Lu n = 2.4M, gamma 2.2, mean degree 50 (~120M CSR entries, int64 row pointers), 8M triangles, 64 LPA
   clusters: BF16-stored V evaluation against the oracle (FP32 tolerances on the dequantised V) and bit-exact
   separation of both families with binding caps.
@@ -136,20 +136,37 @@ def test_chung_lu_300k_separation(chung_lu_300k):
     eng.close()
 
 
-@pytest.mark.timeout(600)
-def test_chung_lu_300k_bound_fp32_vs_fp64(chung_lu_300k):
-    """The candidate bound from FP32 and FP64 device operators on the same V and multipliers; |U_fp32 - U_fp64|
-    is printed and bounded by 1e-4 of |U_fp64|."""
+@pytest.mark.timeout(900)
+def test_chung_lu_300k_bound_fp32_vs_fp64():
+    """The candidate bound at the multipliers of a solved search (the 300k-vertex Chung-Lu request, FP64 search,
+    8 outer iterations): the estimate from the solved V with an FP32 ascent and with an FP64 ascent (both with the
+    FP64 assembly and device pencil). Both pencils must have converged (first residual times sum(s) / L below the
+    1e-5 tolerance), and abs(U_fp32 - U_fp64) is bounded by 1e-4 of U_fp64; the search context's own FP32 / FP64
+    evaluation of the same bound is printed beside it."""
     from lagbound_amd import bound as B
-    inst, V = chung_lu_300k
-    kw = dict(rgd_iters=30, eig_maxiter=60, escape=0, log=lambda *a: None)
+    from lagbound_amd import instances as I
+    from lagbound_amd import serve
+    n, keys = I.chung_lu_target(300_000, 2.1, 13.0, 0)
+    p = serve.Params(dtype="f64", outer=8, cert_max_n=0)
+    print("[config] chung-lu 300k: FP64 search for the multipliers", flush=True)
+    inst, lab, s = serve.search_request(serve.Request(rid=0, n=n, keys=keys, seed=3), p)
+    V = s.V.copy()
+    s.eng.close()
+    kw = dict(rgd_iters=60, eig_maxiter=400, escape=0, log=lambda *a: None)
     out = {}
     for dtype in ("f64", "f32"):
-        print(f"[config] chung-lu 300k estimate {dtype}", flush=True)
-        out[dtype] = B.estimate(inst, V, dtype=dtype, **kw)
+        print(f"[config] chung-lu 300k estimate, {dtype} ascent", flush=True)
+        e = B.estimate(inst, V, dtype=dtype, **kw)
+        rec = e["s_vol"]
+        conv = rec["resid"][0] * rec["sum_s"] / inst.L
+        print(f"[config] {dtype}: dv {e['dv']:.9f} ub {e['ub']:.9f} pencil residual (bound units) {conv:.2e} "
+              f"after {rec['eig_trace'][-1][0] if rec['eig_trace'] else 0} iterations", flush=True)
+        assert conv <= 1e-5, (dtype, conv)
+        out[dtype] = e
     du = abs(out["f32"]["ub"] - out["f64"]["ub"])
     print(f"[config] U_f64 {out['f64']['ub']:.9f} U_f32 {out['f32']['ub']:.9f} |dU| {du:.3e}", flush=True)
     assert np.isfinite(out["f64"]["ub"]) and du <= 1e-4 * abs(out["f64"]["ub"])
+    assert out["f64"]["ub"] >= serve.modularity(inst, lab.astype(np.int64))
 
 
 # ------------------------------------------------------------------------ Chung-Lu n = 2.4M, BF16-stored V
