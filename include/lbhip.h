@@ -233,6 +233,18 @@ int lb_blk_elem(lb_ctx *ctx, int kind, int dst, int src, int other, int diag, co
 int lb_blk_gram(lb_ctx *ctx, int na, const int *a, int nb, const int *b, int diag, double *out);
 int lb_blk_comb(lb_ctx *ctx, int dst, int ns, const int *src, const double *coef, int accumulate);
 
+/* Gram matrix without any allocation. No counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid hipMalloc). The no-allocation variant of
+ * lb_blk_gram: out = [A_0 .. A_(na-1)]^T Diag(w) [B_0 .. B_(nb-1)] for caller-owned device blocks A_j, B_j (n x r
+ * FP64 each, row-major, r 8 / 16 / 32), w a device n-vector or NULL (no weights), out a caller-owned device array of
+ * (na r) x (nb r) doubles (row-major), summed in the same fixed order as lb_blk_gram. Launched on `stream` (a
+ * hipStream_t of device `device`; NULL: its null stream) without waiting; the calling thread's current device is
+ * left unchanged. The per-row-block partial sums live in a library-owned static device array of 128 x 96 x 96
+ * doubles (9.4 MB, one per device until the process ends), so na r and nb r must not exceed 96. Calls that overlap
+ * on one device race on that array. Returns 0, -1 (bad arguments), -2 (na r or nb r above 96) or -3 (HIP error). */
+int lb_gram_noalloc(int device, void *stream, int64_t n, int r, int na, const double *const *A, int nb,
+                    const double *const *B, const double *w, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,6 +28,7 @@
 #include <thread>
 #include <vector>
 #include <array>
+#include <mutex>
 
 #include "../../include/lbhip.h"
 
@@ -4351,6 +4352,75 @@ int lb_blk_comb(lb_ctx *c, int dst, int ns, const int *src, const double *coef, 
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));     // the coefficients are the caller's memory
     return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------- Gram without allocation
+// lb_gram_noalloc keeps the Gram kernel's per-row-block partial sums in this file-scope device array instead of a
+// context buffer: each device holds its own copy, alive until the process ends. In host code the name is a host-side
+// stand-in, never device memory: the device address comes from hipGetSymbolAddress, looked up once per device with
+// that device current, and only that address is passed to the kernels.
+#define LB_GRAM_MAXW 96                                      // na r and nb r at most
+#define LB_GRAM_SCRATCH_DOUBLES ((size_t)LB_BLK_ROWBLK * LB_GRAM_MAXW * LB_GRAM_MAXW)   // 9.4 MB
+static __device__ __align__(256) double lb_gram_scratch[LB_GRAM_SCRATCH_DOUBLES];
+
+namespace {
+
+#define LB_MAX_DEVICES 64
+std::mutex g_gram_mu;
+double *g_gram_ptr[LB_MAX_DEVICES] = {};
+
+// device address of lb_gram_scratch on the current device `device` (the first call per device looks it up)
+hipError_t gram_scratch(int device, double **out) {
+    std::lock_guard<std::mutex> lk(g_gram_mu);
+    if (!g_gram_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(lb_gram_scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_gram_ptr[device] = (double *)p;
+    }
+    *out = g_gram_ptr[device];
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lb_gram_noalloc(int device, void *stream, int64_t n, int r, int na, const double *const *A, int nb,
+                    const double *const *B, const double *w, double *out) {
+    if (device < 0 || device >= LB_MAX_DEVICES || n < 1 || (r != 8 && r != 16 && r != 32) || na < 1 ||
+        na > LB_BLK_MAXS || nb < 1 || nb > LB_BLK_MAXS || !A || !B || !out)
+        return -1;
+    for (int j = 0; j < na; j++) if (!A[j]) return -1;
+    for (int j = 0; j < nb; j++) if (!B[j]) return -1;
+    if (na * r > LB_GRAM_MAXW || nb * r > LB_GRAM_MAXW) return -2;        // the partials would not fit the array
+    int ndev = 0, prev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device >= ndev) return -3;
+    if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) return -3;
+    int rc = 0;
+    double *part = nullptr;
+    if (gram_scratch(device, &part) != hipSuccess) {
+        rc = -3;
+    } else {
+        GramArgs g{};
+        for (int j = 0; j < na; j++) g.A[j] = A[j];
+        for (int j = 0; j < nb; j++) g.B[j] = B[j];
+        g.na = na; g.nb = nb; g.r = r; g.w = w; g.n = n; g.part = part;
+        const int wa = na * r, wb = nb * r, m = wa * wb;
+        const int tiles = ((wa + 15) / 16) * ((wb + 15) / 16);
+        const hipStream_t st = (hipStream_t)stream;
+        // every row block writes every entry of its partial tile (zero for an empty row range), so the array needs
+        // no clearing between calls
+        hipLaunchKernelGGL(lb_k_blk_gram, dim3(LB_BLK_ROWBLK, tiles), dim3(256), 0, st, g);
+        hipLaunchKernelGGL(lb_k_reduce, dim3((m + 63) / 64), dim3(1024), 0, st, (const double *)part, LB_BLK_ROWBLK,
+                           m, m, out);
+        if (hipGetLastError() != hipSuccess) rc = -3;
+    }
+    if (hipSetDevice(prev) != hipSuccess && rc == 0) rc = -3;
+    return rc;
 }
 
 }  // extern "C"
