@@ -195,6 +195,21 @@ int lb_reduce_exact(lb_ctx *ctx, int64_t n, int64_t m, const int64_t *keys, cons
  * out[1] + ||F||_F). Needs 16 n^2 bytes of free device memory. */
 int lb_certify_dense(lb_ctx *ctx, const double *dg, double *out);
 
+/* Dense core of the lifted sparse certificate on device `device` (the factorization's last, dense block; the
+ * reference factors the whole lifted matrix with CHOLMOD on the host, certify_lifted_vc.py:391-449, cholcert.c:24-76):
+ * forms the Schur complement S = A22 - sum_k cd_k c_k c_k^T (m x m FP64) from the entries of A among the m core rows
+ * (na, lower: ai >= aj) and, per sparse pivot k, the core part c_k of its column of L (CSR cptr nsp+1 / cidx
+ * ascending / cval) with its pivot cd_k, factors S = R R^T and returns out[0] = info (0: factored; k > 0: leading
+ * minor k not positive) and out[1] = sum_j (sum_i |R_ij|)^2, from which the caller prices the factor's rounding
+ * (Higham, Thm 10.3). method 1: rocSOLVER dpotrf; 2: the library's own blocked right-looking Cholesky (64-column
+ * panels, 64-bit offsets); 0: rocSOLVER while m^2 < 2^31, the own kernels above (rocSOLVER's dpotrf_64 reported a
+ * non-positive leading minor at m = 131,287 on a matrix whose leading blocks factor, so it is not used there). Host
+ * arrays in; allocates 8 m^2 bytes of device memory on a stream of its own. Returns 0, -1 (bad arguments), -2 (not
+ * enough free device memory) or -3 (HIP / rocSOLVER error). */
+int lb_cert_core(int device, int64_t m, int64_t na, const int32_t *ai, const int32_t *aj, const double *av,
+                 int64_t nsp, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *cd,
+                 int method, double *out);
+
 /* Batch mode (no reference counterpart: the reference solves one graph per process): the graph set by lb_set_graph
  * is the block-diagonal union of G graphs with contiguous row ranges row_off (G+1, from 0 to n), each with its own
  * L_g and C0_g. Edges and triangles must stay inside one graph (checked). lb_batch_set_levels gives each graph one

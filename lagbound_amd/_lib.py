@@ -21,7 +21,7 @@ EXPORTS = ["lb_create", "lb_destroy", "lb_last_error", "lb_set_graph", "lb_set_l
            "lb_kernel_name", "lb_set_tuning", "lb_version", "lb_reduce_exact", "lb_set_option", "lb_certify_dense", "lb_batch_set",
            "lb_batch_set_levels", "lb_batch_eval", "lb_capture_eval", "lb_replay_eval",
            "lb_blk_alloc", "lb_blk_set", "lb_blk_get", "lb_blk_set_diag", "lb_blk_op", "lb_blk_elem", "lb_blk_gram",
-           "lb_blk_comb", "lb_gram_noalloc"]
+           "lb_blk_comb", "lb_gram_noalloc", "lb_cert_core"]
 
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
@@ -96,6 +96,8 @@ def load():
         "lb_blk_elem": ([ctx, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _f64p], C.c_int),
         "lb_blk_gram": ([ctx, C.c_int, _i32p, C.c_int, _i32p, C.c_int, _f64p], C.c_int),
         "lb_blk_comb": ([ctx, C.c_int, C.c_int, _i32p, _f64p, C.c_int], C.c_int),
+        "lb_cert_core": ([C.c_int, C.c_int64, C.c_int64, _i32p, _i32p, _f64p, C.c_int64, _i64p, _i32p, _f64p, _f64p,
+                          C.c_int, _f64p], C.c_int),
         "lb_gram_noalloc": ([C.c_int, _vp, C.c_int64, C.c_int, C.c_int, C.POINTER(_vp), C.c_int, C.POINTER(_vp),
                              _vp, _vp], C.c_int),
     }

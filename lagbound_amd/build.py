@@ -28,11 +28,28 @@ def _compile(src, out, verbose, libs=()):
         print(" ".join(cmd))
 
 
+CERT_SRC = os.path.join(HERE, "csrc", "lbcert.cpp")
+CERT_OUT = os.path.join(HERE, "liblbcert.so")
+
+
+def _compile_host(src, out, verbose):
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", INCLUDE, "-o", out + ".tmp", src]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"{cxx} failed:\n" + res.stderr[-4000:])
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print(" ".join(cmd))
+
+
 def build(verbose: bool = False) -> str:
-    """Compile every HIP source of the package (liblbhip.so, and the bandwidth probe liblbprobe.so used by the
-    benchmark's traffic calibration); always recompiles."""
+    """Compile every native source of the package: liblbhip.so (HIP kernels + C ABI), the bandwidth probe
+    liblbprobe.so used by the benchmark's traffic calibration, and the host part of the sparse certificate
+    liblbcert.so; always recompiles."""
     _compile(SRC, OUT, verbose, libs=["-ldl"])
     _compile(PROBE_SRC, PROBE_OUT, verbose)
+    _compile_host(CERT_SRC, CERT_OUT, verbose)
     return OUT
 
 

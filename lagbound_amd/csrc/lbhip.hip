@@ -4716,6 +4716,8 @@ struct SolverApi {
     rocblas_status (*destroy)(rocblas_handle) = nullptr;
     rocblas_status (*set_stream)(rocblas_handle, hipStream_t) = nullptr;
     rocblas_status (*potrf)(rocblas_handle, rocblas_fill, rocblas_int, double *, rocblas_int, rocblas_int *) = nullptr;
+    // 64-bit sizes (dense cores of more than 2^31 entries); optional
+    rocblas_status (*potrf64)(rocblas_handle, rocblas_fill, int64_t, double *, int64_t, int64_t *) = nullptr;
 };
 
 SolverApi load_solver_api() {
@@ -4729,6 +4731,7 @@ SolverApi load_solver_api() {
     a.destroy = (rocblas_status(*)(rocblas_handle))dlsym(hb, "rocblas_destroy_handle");
     a.set_stream = (rocblas_status(*)(rocblas_handle, hipStream_t))dlsym(hb, "rocblas_set_stream");
     a.potrf = (rocblas_status(*)(rocblas_handle, rocblas_fill, rocblas_int, double *, rocblas_int, rocblas_int *))dlsym(hs, "rocsolver_dpotrf");
+    a.potrf64 = (rocblas_status(*)(rocblas_handle, rocblas_fill, int64_t, double *, int64_t, int64_t *))dlsym(hs, "rocsolver_dpotrf_64");
     a.ok = a.create && a.destroy && a.set_stream && a.potrf;
     if (!a.ok) a.err = "rocBLAS / rocSOLVER entry points not found";
     return a;
@@ -4905,6 +4908,279 @@ int lb_certify_dense(lb_ctx *c, const double *dg, double *out) {
     cleanup();
 #undef CCHK
 #undef CHIP
+    return 0;
+}
+
+}  // extern "C"
+
+// Dense core of the sparse certificate: S = A22 - sum_k d_k c_k c_k^T (lower triangle, column-major m x m), where c_k
+// is the core part of sparse pivot k's column of L (core rows ascending), then its Cholesky factor in place.
+namespace {
+
+__global__ void lb_k_core_a(const int32_t *ai, const int32_t *aj, const double *av, int64_t na, int64_t m, double *S) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < na; e += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(S + (int64_t)aj[e] * m + ai[e], av[e]);
+}
+
+// one workgroup per pivot column (grid-stride): every pair x >= y of its core entries, t = x (x + 1) / 2 + y
+__global__ __launch_bounds__(256) void lb_k_core_upd(const int64_t *cptr, const int32_t *cidx, const double *cval,
+                                                     const double *cd, int64_t nsp, int64_t m, double *S) {
+    for (int64_t k = blockIdx.x; k < nsp; k += gridDim.x) {
+        const int64_t b = cptr[k], c = cptr[k + 1] - b;
+        if (c == 0) continue;
+        const double d = cd[k];
+        const int64_t npair = c * (c + 1) / 2;
+        for (int64_t t = threadIdx.x; t < npair; t += blockDim.x) {
+            int64_t x = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+            while (x * (x + 1) / 2 > t) x--;
+            while ((x + 1) * (x + 2) / 2 <= t) x++;
+            const int64_t y = t - x * (x + 1) / 2;
+            const int64_t rx = cidx[b + x], ry = cidx[b + y];          // rx >= ry: the lower triangle
+            atomicAdd(S + ry * m + rx, -(d * cval[b + x]) * cval[b + y]);
+        }
+    }
+}
+
+// colsum[j] = sum_{i >= j} |R_ij| of the lower-triangular factor (one workgroup per column, grid-stride)
+__global__ __launch_bounds__(256) void lb_k_core_colsum(const double *S, int64_t m, double *colsum) {
+    __shared__ double red[4];
+    for (int64_t j = blockIdx.x; j < m; j += gridDim.x) {
+        double a = 0.0;
+        for (int64_t i = j + threadIdx.x; i < m; i += 256) a += fabs(S[j * m + i]);
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+        __syncthreads();
+        if (threadIdx.x == 0) colsum[j] = red[0] + red[1] + red[2] + red[3];
+        __syncthreads();
+    }
+}
+
+// Blocked right-looking Cholesky S = R R^T in place (lower, column-major, lda m, 64-bit offsets), for cores whose
+// m^2 entries exceed what the library factorization handles. Per panel of LB_CH columns: the diagonal block in LDS,
+// the rows below solved against it (one thread per row, the row in registers), the trailing lower triangle updated by
+// 64 x 64 tiles. Each entry of R is a - sum of products / pivot, accumulated panel by panel, so the componentwise
+// bound |S - R R^T| <= gamma(m + 1) |R| |R^T| holds as for any Cholesky. info (device, 0 on entry): 1-based index of
+// the first pivot that is not positive (or not finite); every kernel returns at once once it is set.
+#define LB_CH 64
+
+__global__ __launch_bounds__(256) void lb_k_chol_diag(double *S, int64_t m, int64_t k, int64_t *info) {
+    __shared__ double A[LB_CH][LB_CH + 1];
+    __shared__ int bad;
+    if (*info) return;
+    const int b = (int)min<int64_t>(LB_CH, m - k);
+    for (int e = threadIdx.x; e < b * b; e += 256) {
+        const int i = e % b, j = e / b;
+        A[i][j] = i >= j ? S[(k + j) * m + k + i] : 0.0;
+    }
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < b; j++) {
+        if (threadIdx.x == 0) {
+            const double dj = A[j][j];
+            if (!(dj > 0.0) || !isfinite(dj)) { bad = j + 1; A[j][j] = 1.0; }
+            else A[j][j] = sqrt(dj);
+        }
+        __syncthreads();
+        if (bad) break;
+        const double piv = A[j][j];
+        for (int i = j + 1 + threadIdx.x; i < b; i += 256) A[i][j] /= piv;
+        __syncthreads();
+        const int t = b - 1 - j;                          // trailing size
+        for (int e = threadIdx.x; e < t * t; e += 256) {
+            const int i = j + 1 + e % t, l = j + 1 + e / t;
+            if (i >= l) A[i][l] -= A[i][j] * A[l][j];
+        }
+        __syncthreads();
+    }
+    if (bad) { if (threadIdx.x == 0) *info = k + bad; return; }
+    for (int e = threadIdx.x; e < b * b; e += 256) {
+        const int i = e % b, j = e / b;
+        if (i >= j) S[(k + j) * m + k + i] = A[i][j];
+    }
+}
+
+// rows r >= k + LB_CH: x L11^T = a (forward substitution over the panel's LB_CH columns), L11 staged in LDS
+__global__ __launch_bounds__(256) void lb_k_chol_trsm(double *S, int64_t m, int64_t k, const int64_t *info) {
+    __shared__ double Lt[LB_CH][LB_CH + 1];
+    if (*info) return;
+    for (int e = threadIdx.x; e < LB_CH * LB_CH; e += 256) {
+        const int i = e % LB_CH, j = e / LB_CH;
+        Lt[i][j] = i >= j ? S[(k + j) * m + k + i] : 0.0;
+    }
+    __syncthreads();
+    const int64_t r = k + LB_CH + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= m) return;
+    double x[LB_CH];
+#pragma unroll
+    for (int j = 0; j < LB_CH; j++) x[j] = S[(k + j) * m + r];
+#pragma unroll
+    for (int j = 0; j < LB_CH; j++) {
+        double a = x[j];
+#pragma unroll
+        for (int l = 0; l < j; l++) a -= x[l] * Lt[j][l];
+        x[j] = a / Lt[j][j];
+    }
+#pragma unroll
+    for (int j = 0; j < LB_CH; j++) S[(k + j) * m + r] = x[j];
+}
+
+// trailing update C_IJ -= P_I P_J^T on the lower 64 x 64 tiles (J <= I) of rows / columns >= k + LB_CH; P = the
+// panel's solved rows. blockIdx.x = I (I + 1) / 2 + J; each thread owns a 4 x 4 block of the tile.
+__global__ __launch_bounds__(256) void lb_k_chol_syrk(double *S, int64_t m, int64_t k, const int64_t *info) {
+    __shared__ double PI[LB_CH][LB_CH + 1], PJ[LB_CH][LB_CH + 1];
+    if (*info) return;
+    const int64_t t = blockIdx.x;
+    int64_t I = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (I * (I + 1) / 2 > t) I--;
+    while ((I + 1) * (I + 2) / 2 <= t) I++;
+    const int64_t J = t - I * (I + 1) / 2;
+    const int64_t base = k + LB_CH, r0 = base + I * 64, c0 = base + J * 64;
+    for (int e = threadIdx.x; e < 64 * LB_CH; e += 256) {
+        const int i = e % 64, l = e / 64;
+        PI[i][l] = r0 + i < m ? S[(k + l) * m + r0 + i] : 0.0;
+        PJ[i][l] = c0 + i < m ? S[(k + l) * m + c0 + i] : 0.0;
+    }
+    __syncthreads();
+    const int ti = threadIdx.x % 16, tj = threadIdx.x / 16;
+    double acc[4][4] = {};
+    for (int l = 0; l < LB_CH; l++) {
+        double a[4], bj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) { a[u] = PI[ti * 4 + u][l]; bj[u] = PJ[tj * 4 + u][l]; }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[u][v] += a[u] * bj[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int64_t c = c0 + tj * 4 + v;
+        if (c >= m) continue;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t r = r0 + ti * 4 + u;
+            if (r < m && r >= c) S[c * m + r] -= acc[u][v];
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lb_cert_core(int device, int64_t m, int64_t na, const int32_t *ai, const int32_t *aj, const double *av,
+                 int64_t nsp, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *cd,
+                 int method, double *out) {
+    if (m < 1 || na < 0 || nsp < 0 || !out || (na && (!ai || !aj || !av)) || (nsp && (!cptr || !cidx || !cval || !cd)) ||
+        method < 0 || method > 2)
+        return -1;
+    for (int64_t e = 0; e < na; e++)
+        if (ai[e] < 0 || ai[e] >= m || aj[e] < 0 || aj[e] > ai[e]) return -1;
+    if (nsp) {
+        if (cptr[0] != 0) return -1;
+        for (int64_t k = 0; k < nsp; k++) {
+            if (cptr[k + 1] < cptr[k]) return -1;
+            for (int64_t t = cptr[k]; t < cptr[k + 1]; t++)
+                if (cidx[t] < 0 || cidx[t] >= m || (t > cptr[k] && cidx[t] <= cidx[t - 1])) return -1;
+        }
+    }
+    const int64_t nc = nsp ? cptr[nsp] : 0;
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) return -3;
+    // method 0: the library factorization while m^2 < 2^31, the blocked kernels above; 1: library; 2: blocked kernels
+    const bool own = method == 2 || (method == 0 && m * m >= ((int64_t)1 << 31));
+    const SolverApi &api = solver_api();
+    if (!own && (!api.ok || (!api.potrf64 && m * m >= ((int64_t)1 << 31)))) { hipSetDevice(prev); return -3; }
+    size_t fr = 0, tt = 0;
+    if (hipMemGetInfo(&fr, &tt) != hipSuccess) { hipSetDevice(prev); return -3; }
+    const size_t need = sizeof(double) * ((size_t)m * m + m) + 16 * (size_t)(na + nc + nsp) + (256u << 20);
+    if (need > fr) { hipSetDevice(prev); return -2; }
+    hipStream_t s = nullptr;
+    void *S = nullptr, *dai = nullptr, *daj = nullptr, *dav = nullptr, *dcp = nullptr, *dci = nullptr, *dcv = nullptr,
+         *dcd = nullptr, *dcs = nullptr, *dinfo = nullptr;
+    auto cleanup = [&]() {
+        for (void *q : {S, dai, daj, dav, dcp, dci, dcv, dcd, dcs, dinfo}) if (q) hipFree(q);
+        if (s) hipStreamDestroy(s);
+        hipSetDevice(prev);
+    };
+    int rc = 0;
+#define KHIP(x) do { if ((x) != hipSuccess) { cleanup(); return -3; } } while (0)
+    KHIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    KHIP(hipMalloc(&S, sizeof(double) * (size_t)m * m));
+    KHIP(hipMalloc(&dcs, sizeof(double) * (size_t)m));
+    KHIP(hipMalloc(&dinfo, sizeof(int64_t)));
+    KHIP(hipMemsetAsync(S, 0, sizeof(double) * (size_t)m * m, s));
+    if (na) {
+        KHIP(hipMalloc(&dai, sizeof(int32_t) * na)); KHIP(hipMalloc(&daj, sizeof(int32_t) * na));
+        KHIP(hipMalloc(&dav, sizeof(double) * na));
+        KHIP(hipMemcpyAsync(dai, ai, sizeof(int32_t) * na, hipMemcpyHostToDevice, s));
+        KHIP(hipMemcpyAsync(daj, aj, sizeof(int32_t) * na, hipMemcpyHostToDevice, s));
+        KHIP(hipMemcpyAsync(dav, av, sizeof(double) * na, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(lb_k_core_a, dim3(4096), dim3(256), 0, s, (const int32_t *)dai, (const int32_t *)daj,
+                           (const double *)dav, na, m, (double *)S);
+    }
+    if (nsp && nc) {
+        KHIP(hipMalloc(&dcp, sizeof(int64_t) * (nsp + 1))); KHIP(hipMalloc(&dci, sizeof(int32_t) * nc));
+        KHIP(hipMalloc(&dcv, sizeof(double) * nc)); KHIP(hipMalloc(&dcd, sizeof(double) * nsp));
+        KHIP(hipMemcpyAsync(dcp, cptr, sizeof(int64_t) * (nsp + 1), hipMemcpyHostToDevice, s));
+        KHIP(hipMemcpyAsync(dci, cidx, sizeof(int32_t) * nc, hipMemcpyHostToDevice, s));
+        KHIP(hipMemcpyAsync(dcv, cval, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+        KHIP(hipMemcpyAsync(dcd, cd, sizeof(double) * nsp, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(lb_k_core_upd, dim3(8192), dim3(256), 0, s, (const int64_t *)dcp, (const int32_t *)dci,
+                           (const double *)dcv, (const double *)dcd, nsp, m, (double *)S);
+    }
+    KHIP(hipGetLastError());
+    int64_t info = -1;
+    if (own) {
+        KHIP(hipMemsetAsync(dinfo, 0, sizeof(int64_t), s));
+        for (int64_t k = 0; k < m; k += LB_CH) {
+            hipLaunchKernelGGL(lb_k_chol_diag, dim3(1), dim3(256), 0, s, (double *)S, m, k, (int64_t *)dinfo);
+            const int64_t rest = m - k - LB_CH;
+            if (rest > 0) {
+                hipLaunchKernelGGL(lb_k_chol_trsm, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0, s, (double *)S, m,
+                                   k, (const int64_t *)dinfo);
+                const int64_t T = (rest + 63) / 64;
+                hipLaunchKernelGGL(lb_k_chol_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, (double *)S, m,
+                                   k, (const int64_t *)dinfo);
+            }
+            if ((k / LB_CH) % 64 == 63) KHIP(hipStreamSynchronize(s));   // bounded queue depth
+        }
+        KHIP(hipGetLastError());
+        KHIP(hipMemcpyAsync(&info, dinfo, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        KHIP(hipStreamSynchronize(s));
+    } else {
+    rocblas_handle hb = nullptr;
+    if (api.create(&hb) != rocblas_status_success) { cleanup(); return -3; }
+    api.set_stream(hb, s);
+    rocblas_status st;
+    if (api.potrf64) {
+        st = api.potrf64(hb, rocblas_fill_lower, m, (double *)S, m, (int64_t *)dinfo);
+        if (hipMemcpyAsync(&info, dinfo, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess) rc = -3;
+    } else {
+        st = api.potrf(hb, rocblas_fill_lower, (rocblas_int)m, (double *)S, (rocblas_int)m, (rocblas_int *)dinfo);
+        int i32 = -1;
+        if (hipMemcpyAsync(&i32, dinfo, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) rc = -3;
+        if (hipStreamSynchronize(s) != hipSuccess) rc = -3;
+        info = i32;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) rc = -3;
+    api.destroy(hb);
+    if (st != rocblas_status_success || rc) { cleanup(); return -3; }
+    }
+    double sr = 0.0;
+    if (info == 0) {
+        std::vector<double> cs((size_t)m);
+        hipLaunchKernelGGL(lb_k_core_colsum, dim3((unsigned)std::min<int64_t>(m, 8192)), dim3(256), 0, s,
+                           (const double *)S, m, (double *)dcs);
+        KHIP(hipGetLastError());
+        KHIP(hipMemcpyAsync(cs.data(), dcs, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, s));
+        KHIP(hipStreamSynchronize(s));
+        for (double x : cs) sr += x * x;
+    }
+    out[0] = (double)info;
+    out[1] = sr;
+#undef KHIP
+    cleanup();
     return 0;
 }
 

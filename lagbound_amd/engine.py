@@ -408,3 +408,23 @@ class Engine(Relabeled):
     def __init__(self, r: int = 16, dtype: str = "f32", device: int = 0, order: str | None = None):
         make = lambda: DeviceEngine(r=r, dtype=dtype, device=device)  # noqa: E731
         super().__init__(make(), make, order or DEFAULT_ORDER)
+
+
+def cert_core(device, m, ai, aj, av, cptr, cidx, cval, cd, method=0):
+    """The dense core of the lifted sparse certificate on the device (lb_cert_core): returns (info, sum_j (sum_i
+    |R_ij|)^2) of the Cholesky factor R of the core's Schur complement (info 0: factored, k > 0: leading minor k not
+    positive). method 0: rocSOLVER below 2^31 entries, the library's blocked kernels above; 1 / 2: force either."""
+    L = LB.load()
+    ai, aj = np.ascontiguousarray(ai, np.int32), np.ascontiguousarray(aj, np.int32)
+    av, cd, cval = (np.ascontiguousarray(x, np.float64) for x in (av, cd, cval))
+    cptr, cidx = np.ascontiguousarray(cptr, np.int64), np.ascontiguousarray(cidx, np.int32)
+    out = np.zeros(2)
+    rc = L.lb_cert_core(int(device), int(m), len(av), ptr(ai, _i32p), ptr(aj, _i32p), ptr(av, _f64p), len(cd),
+                        ptr(cptr, _i64p), ptr(cidx, _i32p), ptr(cval, _f64p), ptr(cd, _f64p), int(method),
+                        ptr(out, _f64p))
+    if rc == -2:
+        raise LBError(f"lb_cert_core: not enough free device memory for a dense core of {m} rows "
+                      f"({8 * m * m / 2 ** 30:.1f} GiB)")
+    if rc:
+        raise LBError(f"lb_cert_core: status {rc}")
+    return int(out[0]), float(out[1])

@@ -362,10 +362,12 @@ def eval_dense(inst, V, rho, rt=None):
 
 
 # ----------------------------------------------------------------------------------------------- numpy sweeps
-def sweeps_numpy(inst, V, orders, nbatch, rho, rt=None, damp=0.0, kap2=1.0):
+def sweeps_numpy(inst, V, orders, nbatch, rho, rt=None, damp=0.0, kap2=1.0, hook=None):
     """Vectorised restatement of the batched row steps with per-batch snapshot semantics (sweeps, alm_vcs.py:1360-1436;
     batch_step6 255-351; pair_all 378-415; kq_level 417-428; vcs_state / vcs_pre / vcs_post vcsops.py:127-213,
-    308-323)."""
+    308-323). `hook(V, z, Ys, vst)`, if given, sees the running aggregates after every batch (z, the pair levels'
+    cluster sums, and per VCS level its cluster sums "Y", stored multipliers "st", "Z" and "cnt"), for tests that
+    recompute them from V."""
     rt = rho if rt is None else rt
     V = np.array(V, np.float64, copy=True)
     n, k = V.shape
@@ -485,6 +487,8 @@ def sweeps_numpy(inst, V, orders, nbatch, rho, rt=None, damp=0.0, kap2=1.0):
                 np.add.at(s_["Z"], s_["R"][sel], d[er][:, None] * (sn[:, None] * V[er] - so[:, None] * vold))
                 np.add.at(s_["cnt"], s_["R"][sel], (sn > 0).astype(float) - (so > 0).astype(float))
                 s_["st"][sel] = sn
+            if hook is not None:
+                hook(V, z, Ys, vst)
     return V
 
 

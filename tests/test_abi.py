@@ -51,3 +51,15 @@ def test_no_cpu_fallback():
         return
     e.close()
     pytest.skip("a HIP device is present")
+
+
+def test_certificate_library_exports_every_symbol():
+    """liblbcert.so (the host part of the sparse certificate) exports every entry point include/lbcert.h declares."""
+    from lagbound_amd import sparse_cert
+    src = open(os.path.join(ROOT, "include", "lbcert.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(lbc_[a-z_A-Z0-9]+)\s*\(", src)))
+    assert names == ["lbc_core", "lbc_factor", "lbc_free", "lbc_info", "lbc_md_order", "lbc_order"]
+    lib = ctypes.CDLL(sparse_cert.CERTLIB)
+    for name in names:
+        assert hasattr(lib, name), name

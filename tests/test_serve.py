@@ -23,8 +23,9 @@ class _Lag(OracleLagrangian):
 
 
 def _params():
+    # requests above 250 vertices take the lifted sparse certificate (host core), the others the dense one
     return serve.Params(k=8, outer=6, tri_every=3, vcs_every=2, rgd_iters=20, eig_maxiter=40, cert_max_n=400,
-                        cert_on_device=False)
+                        cert_dense_max_n=250, cert_on_device=False)
 
 
 def test_footprint_and_rule():
@@ -49,7 +50,7 @@ def test_alone_equals_mixed_and_admission(cpu_engine):
             assert res.certified == alone.certified
         assert alone.certified and alone.UB >= alone.lb
     summ = serve.summarize(mixed, wall=1.0)
-    for key in ("latency_p50_s", "latency_p99_s", "solves_per_s", "gap_ub_minus_lb", "certified"):
+    for key in ("latency_p50_s", "latency_p99_s", "solves_per_s", "certified_gap_UB_minus_lb", "certified"):
         assert key in summ
 
 

@@ -85,3 +85,45 @@ def test_first_step_from_first_principles(rho):
     q = g + 0.5 * kappa * V[i]
     pred = q / np.linalg.norm(q)
     assert np.abs(W[i] - pred).max() < 2e-6, (W[i], pred)
+
+
+@pytest.mark.parametrize("rho", [0.0, 0.3])
+def test_aggregates_match_V_after_every_batch(rho):
+    """The incremental bookkeeping between batches, checked from scratch after EVERY batch of the restatement: z = d^T V,
+    each pair level's cluster sums Y = W V, each VCS level's cluster sums, and its Z_R = sum over working-set entries
+    of st d_i V_i and cnt_R = #(st > 0) with the stored multipliers st; the stored multiplier of every entry whose row
+    moved in the batch equals its effective value at the new V and Y (vcs_post, vcsops.py:159-213). A dropped increment,
+    a wrong index or a stale st breaks one of them."""
+    inst, V = small_instance(n=45, seed=4, k=8, sig_scale=0.05)
+    rng = np.random.default_rng(2)
+    orders = np.stack([rng.permutation(inst.n) for _ in range(2)])
+    d = inst.d
+    seen = {"batches": 0, "Vprev": np.array(V, np.float64)}
+
+    def W(lab, P):
+        M = np.zeros((P, inst.n))
+        M[lab, np.arange(inst.n)] = d
+        return M
+
+    def hook(Vc, z, Ys, vst):
+        seen["batches"] += 1
+        assert np.allclose(z, d @ Vc, atol=1e-11)
+        for lv, Y in zip(inst.levels, Ys):
+            assert np.allclose(Y, W(lv.lab, lv.P) @ Vc, atol=1e-11)
+        moved = np.flatnonzero(np.any(Vc != seen["Vprev"], axis=1))
+        for vl, s_ in zip(inst.vcs, vst):
+            assert np.allclose(s_["Y"], W(vl.lab, vl.P) @ Vc, atol=1e-11)
+            rows, R, st = s_["rows"], s_["R"], s_["st"]
+            Z = np.zeros((vl.P, Vc.shape[1]))
+            np.add.at(Z, R, (st * d[rows])[:, None] * Vc[rows])
+            assert np.allclose(s_["Z"], Z, atol=1e-11)
+            assert np.array_equal(s_["cnt"], np.bincount(R, weights=(st > 0).astype(float), minlength=vl.P))
+            sel = np.isin(rows, moved)
+            if sel.any():
+                fresh = O._vcs_eff(vl, Vc, s_["Y"], d, rows[sel], R[sel], s_["own"][sel], rho, inst.wv,
+                                   sig=vl.sig[sel])
+                assert np.allclose(st[sel], fresh, atol=1e-12)
+        seen["Vprev"] = np.array(Vc, copy=True)
+
+    O.sweeps_numpy(inst, V, orders, 5, rho, damp=0.1, hook=hook)
+    assert seen["batches"] == 2 * 5
