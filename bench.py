@@ -10,7 +10,7 @@ device synchronisation.
 Also reported on the same instance: the HIP path with BF16-stored V and with FP64 V, the plain FP64 GPU form
 (torch FP64 CSR SpMM, gathers and sparse products), and the FP64 C/OpenMP oracle on the host cores (cpu_baseline). The
 metric's second half, batched bound latency, comes from a Poisson stream of whole bound requests (`serving`:
-p50 / p99 latency, solves/s, certified count) and from the batched evaluation of 512 uneven graphs in one launch
+p50 / p99 latency, solves/s, every result certified by the FP64 certificate) and from the batched evaluation of 512 uneven graphs in one launch
 per kernel (`batch`).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-extra] [--no-serve] [--no-batch]
@@ -182,7 +182,10 @@ def main():
                    "launch_shapes": tuned if tuned else ({"tuning": a.tuning} if a.tuning else "library defaults")},
         "roofline": {"bound": "hbm", "kernel": "one f+grad evaluation (lb_k_colsum .. lb_k_finalize chain)",
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "alg_bytes": B, "ms_per_eval_events": round(ms_ev, 5),
+                     "traffic": traffic,
+                     "traffic_source": "profiles/eval_metric_f32_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE "
+                                       "per evaluation, scaled by the copy probe's factors (not measured in this run)",
+                     "alg_bytes": B, "ms_per_eval_events": round(ms_ev, 5),
                      "dominant_kernel": dom, "kernel_ms": {k: round(v, 5) for k, v in kms.items()},
                      "kernel_ms_note": "per-kernel HIP events with the chain on one stream; the timed evaluation runs "
                                        "lb_k_segs on a second stream beside lb_k_colsum .. lb_k_reduce(Z)"},
