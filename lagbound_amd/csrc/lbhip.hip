@@ -4956,9 +4956,10 @@ __global__ __launch_bounds__(256) void lb_k_core_colsum(const double *S, int64_t
 }
 
 // Blocked right-looking Cholesky S = R R^T in place (lower, column-major, lda m, 64-bit offsets), for cores whose
-// m^2 entries exceed what the library factorization handles. Per panel of LB_CH columns: the diagonal block in LDS,
-// the rows below solved against it (one thread per row, the row in registers), the trailing lower triangle updated by
-// 64 x 64 tiles. Each entry of R is a - sum of products / pivot, accumulated panel by panel, so the componentwise
+// m^2 entries exceed what the library factorization handles. Panels of 2 LB_CH columns, each factored as two halves of
+// LB_CH: the diagonal block in LDS, the rows below solved against it (one thread per row, the row in registers); the
+// second half is first updated by the first; then the trailing lower triangle takes one rank-2 LB_CH update on
+// 128 x 128 tiles, so that it is read and written once per 128 columns. Each entry of R is a - sum of products / pivot, accumulated panel by panel, so the componentwise
 // bound |S - R R^T| <= gamma(m + 1) |R| |R^T| holds as for any Cholesky. info (device, 0 on entry): 1-based index of
 // the first pivot that is not positive (or not finite); every kernel returns at once once it is set.
 #define LB_CH 64
@@ -5024,41 +5025,58 @@ __global__ __launch_bounds__(256) void lb_k_chol_trsm(double *S, int64_t m, int6
     for (int j = 0; j < LB_CH; j++) S[(k + j) * m + r] = x[j];
 }
 
-// trailing update C_IJ -= P_I P_J^T on the lower 64 x 64 tiles (J <= I) of rows / columns >= k + LB_CH; P = the
-// panel's solved rows. blockIdx.x = I (I + 1) / 2 + J; each thread owns a 4 x 4 block of the tile.
-__global__ __launch_bounds__(256) void lb_k_chol_syrk(double *S, int64_t m, int64_t k, const int64_t *info) {
-    __shared__ double PI[LB_CH][LB_CH + 1], PJ[LB_CH][LB_CH + 1];
+// The rank-K update C -= P_r P_c^T on 128 x 128 tiles, P = columns [kb, kb + K) (the solved panel): rows from rb,
+// columns from cb up to ce (exclusive), lower triangle only (r >= c). tri = 1: tiles (I, J <= I) of the square
+// trailing block, blockIdx.x = I (I + 1) / 2 + J; tri = 0: one column of tiles (J = 0), blockIdx.x = I. 256 threads,
+// each owning an 8 x 8 block (rows ti + 16 u, columns tj + 16 v, so that a wavefront's LDS reads are broadcasts or
+// consecutive); the panel columns are staged in LDS in chunks of LB_CK.
+#define LB_CK 16
+__global__ __launch_bounds__(256) void lb_k_chol_update(double *S, int64_t m, int64_t rb, int64_t cb, int64_t ce,
+                                                        int64_t kb, int K, int tri, const int64_t *info) {
+    __shared__ double PA[LB_CK][128 + 2], PB[LB_CK][128 + 2];
     if (*info) return;
     const int64_t t = blockIdx.x;
-    int64_t I = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-    while (I * (I + 1) / 2 > t) I--;
-    while ((I + 1) * (I + 2) / 2 <= t) I++;
-    const int64_t J = t - I * (I + 1) / 2;
-    const int64_t base = k + LB_CH, r0 = base + I * 64, c0 = base + J * 64;
-    for (int e = threadIdx.x; e < 64 * LB_CH; e += 256) {
-        const int i = e % 64, l = e / 64;
-        PI[i][l] = r0 + i < m ? S[(k + l) * m + r0 + i] : 0.0;
-        PJ[i][l] = c0 + i < m ? S[(k + l) * m + c0 + i] : 0.0;
+    int64_t I = t, J = 0;
+    if (tri) {
+        I = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while (I * (I + 1) / 2 > t) I--;
+        while ((I + 1) * (I + 2) / 2 <= t) I++;
+        J = t - I * (I + 1) / 2;
     }
-    __syncthreads();
+    const int64_t r0 = rb + I * 128, c0 = cb + J * 128;
     const int ti = threadIdx.x % 16, tj = threadIdx.x / 16;
-    double acc[4][4] = {};
-    for (int l = 0; l < LB_CH; l++) {
-        double a[4], bj[4];
+    double acc[8][8];
 #pragma unroll
-        for (int u = 0; u < 4; u++) { a[u] = PI[ti * 4 + u][l]; bj[u] = PJ[tj * 4 + u][l]; }
+    for (int u = 0; u < 8; u++)
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int v = 0; v < 8; v++) acc[u][v] = 0.0;
+    for (int kc = 0; kc < K; kc += LB_CK) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < LB_CK * 128; e += 256) {
+            const int i = e % 128, l = e / 128;
+            const int64_t col = (kb + kc + l) * m;
+            PA[l][i] = r0 + i < m ? S[col + r0 + i] : 0.0;
+            PB[l][i] = c0 + i < ce ? S[col + c0 + i] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int l = 0; l < LB_CK; l++) {
+            double a[8], b[8];
 #pragma unroll
-            for (int v = 0; v < 4; v++) acc[u][v] += a[u] * bj[v];
+            for (int u = 0; u < 8; u++) { a[u] = PA[l][ti + 16 * u]; b[u] = PB[l][tj + 16 * u]; }
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+#pragma unroll
+                for (int v = 0; v < 8; v++) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+        }
     }
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int64_t c = c0 + tj * 4 + v;
-        if (c >= m) continue;
+    for (int v = 0; v < 8; v++) {
+        const int64_t c = c0 + tj + 16 * v;
+        if (c >= ce) continue;
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int64_t r = r0 + ti * 4 + u;
+        for (int u = 0; u < 8; u++) {
+            const int64_t r = r0 + ti + 16 * u;
             if (r < m && r >= c) S[c * m + r] -= acc[u][v];
         }
     }
@@ -5133,17 +5151,33 @@ int lb_cert_core(int device, int64_t m, int64_t na, const int32_t *ai, const int
     int64_t info = -1;
     if (own) {
         KHIP(hipMemsetAsync(dinfo, 0, sizeof(int64_t), s));
-        for (int64_t k = 0; k < m; k += LB_CH) {
-            hipLaunchKernelGGL(lb_k_chol_diag, dim3(1), dim3(256), 0, s, (double *)S, m, k, (int64_t *)dinfo);
+        // panels of 2 LB_CH columns: the first half (diagonal block, rows below), its rank-LB_CH update of the
+        // second half's columns, the second half, then one rank-2 LB_CH update of the trailing lower triangle
+        double *Sd = (double *)S;
+        int64_t *inf = (int64_t *)dinfo;
+        auto half = [&](int64_t k) {
+            hipLaunchKernelGGL(lb_k_chol_diag, dim3(1), dim3(256), 0, s, Sd, m, k, inf);
             const int64_t rest = m - k - LB_CH;
-            if (rest > 0) {
-                hipLaunchKernelGGL(lb_k_chol_trsm, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0, s, (double *)S, m,
-                                   k, (const int64_t *)dinfo);
-                const int64_t T = (rest + 63) / 64;
-                hipLaunchKernelGGL(lb_k_chol_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, (double *)S, m,
-                                   k, (const int64_t *)dinfo);
+            if (rest > 0)
+                hipLaunchKernelGGL(lb_k_chol_trsm, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0, s, Sd, m, k,
+                                   (const int64_t *)inf);
+        };
+        for (int64_t k = 0; k < m; k += 2 * LB_CH) {
+            half(k);
+            const int64_t k2 = k + LB_CH;
+            if (k2 < m) {
+                const int64_t ce = std::min<int64_t>(m, k2 + LB_CH);
+                hipLaunchKernelGGL(lb_k_chol_update, dim3((unsigned)((m - k2 + 127) / 128)), dim3(256), 0, s, Sd, m,
+                                   k2, k2, ce, k, LB_CH, 0, (const int64_t *)inf);
+                half(k2);
             }
-            if ((k / LB_CH) % 64 == 63) KHIP(hipStreamSynchronize(s));   // bounded queue depth
+            const int64_t base = k + 2 * LB_CH, rest = m - base;
+            if (rest > 0) {
+                const int64_t T = (rest + 127) / 128;
+                hipLaunchKernelGGL(lb_k_chol_update, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, Sd, m, base,
+                                   base, m, k, 2 * LB_CH, 1, (const int64_t *)inf);
+            }
+            if ((k / (2 * LB_CH)) % 32 == 31) KHIP(hipStreamSynchronize(s));   // bounded queue depth
         }
         KHIP(hipGetLastError());
         KHIP(hipMemcpyAsync(&info, dinfo, sizeof(int64_t), hipMemcpyDeviceToHost, s));
