@@ -30,6 +30,8 @@ def _compile(src, out, verbose, libs=()):
 
 CERT_SRC = os.path.join(HERE, "csrc", "lbcert.cpp")
 CERT_OUT = os.path.join(HERE, "liblbcert.so")
+VERIFY_SRC = os.path.join(HERE, "csrc", "lbverify.cpp")
+VERIFY_OUT = os.path.join(HERE, "liblbverify.so")
 
 
 def _compile_host(src, out, verbose):
@@ -45,11 +47,12 @@ def _compile_host(src, out, verbose):
 
 def build(verbose: bool = False) -> str:
     """Compile every native source of the package: liblbhip.so (HIP kernels + C ABI), the bandwidth probe
-    liblbprobe.so used by the benchmark's traffic calibration, and the host part of the sparse certificate
-    liblbcert.so; always recompiles."""
+    liblbprobe.so used by the benchmark's traffic calibration, the host part of the sparse certificate liblbcert.so
+    and the independent verifier's factorization liblbverify.so; always recompiles."""
     _compile(SRC, OUT, verbose, libs=["-ldl"])
     _compile(PROBE_SRC, PROBE_OUT, verbose)
     _compile_host(CERT_SRC, CERT_OUT, verbose)
+    _compile_host(VERIFY_SRC, VERIFY_OUT, verbose)
     return OUT
 
 

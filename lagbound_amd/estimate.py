@@ -4,7 +4,10 @@ device (lagbound_amd.bound).
 
 It reads the instance directory DATA_DIR/<tag>/ and the checkpoint (alm_vcs.py:1576-1589 keys), runs polish (rgd),
 the pencil eigenvalue per scale and the rank-escape rounds, and writes the reference's result JSON (--out) and
-W (--saveW). `--certify 1` adds the dense FP64 certificate (small instances only: it factors an n x n matrix).
+W (--saveW). `--certify 1` adds the dense FP64 certificate (small instances only: it factors an n x n matrix);
+`--certify 2` the lifted sparse FP64 certificate (lagbound_amd.sparse_cert, its dense core on the device), and
+`--dump_cert FILE.npz` writes its record for the independent verifier (python -m lagbound_amd.verify), like the
+reference's certifier option of the same name.
 Checkpoints with terms the device evaluation does not carry (connectivity multipliers mu / om, VC levels, nested
 levels) are refused.
 """
@@ -50,7 +53,9 @@ def parse(argv=None):
     p.add_argument("--device", type=int, default=0)
     p.add_argument("--dtype", default="f64", choices=["f64", "f32", "bf16"])
     p.add_argument("--rank", type=int, default=16, choices=[8, 16, 32], help="device block width of C(y) X")
-    p.add_argument("--certify", type=int, default=0, help="1: dense FP64 certificate at the best round")
+    p.add_argument("--certify", type=int, default=0,
+                   help="1: dense FP64 certificate at the best round; 2: lifted sparse FP64 certificate")
+    p.add_argument("--dump_cert", default="", help="with --certify 2: write the certificate record (.npz)")
     p.add_argument("--certify_margin", type=float, default=1e-7)
     return p.parse_args(argv)
 
@@ -98,7 +103,15 @@ def main(argv=None):
     res.update(tag=a.tag, ckpt=a.ckpt, lb=a.lb, dtype=a.dtype)
     if np.isfinite(a.lb):
         res["gap"] = (res["ub"] - a.lb) / a.lb
-    if a.certify:
+    if a.certify == 2:
+        from . import sparse_cert as SC
+        est = dict(res, g=g, s=s)
+        cert = SC.certify_estimate_sparse(inst, est, margin=a.certify_margin, device=a.device, keep=True, log=log)
+        if cert["passed"] and a.dump_cert:
+            SC.write_certificate(a.dump_cert, inst, cert)
+        cert.pop("proof", None)
+        res["certificate"] = cert
+    elif a.certify:
         cert = B.certify_dense(inst, g, (max(0.0, -res["lmin"]) + a.certify_margin) * s)
         res["certificate"] = cert
         log(f"CERTIFICATE {'passed' if cert['passed'] else 'FAILED'}" +
@@ -107,7 +120,7 @@ def main(argv=None):
         (f"  gap {res['gap'] * 100:.2f}%" if np.isfinite(a.lb) else "") + f"  total {time.time() - t0:.0f}s")
     if a.out:
         with open(a.out, "w") as fh:
-            json.dump(res, fh, indent=1)
+            json.dump(res, fh, indent=1, default=float)
     if a.saveW:
         np.savez(a.saveW, V=W.astype(np.float32), g=g)
     return 0

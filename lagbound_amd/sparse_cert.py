@@ -389,9 +389,10 @@ def factor_price(Ml, device=None, dense_deg=1000, core_deg=512, max_core=170_000
 
 # ------------------------------------------------------------------------------------------------ certificate
 def certify_sparse(inst, g, shift, sa=0.01, mu_c=1e4, device=None, key=None, dense_deg=1000, core_deg=512,
-                   max_core=170_000, log=None):
+                   max_core=170_000, log=None, keep=False):
     """Prove L Q* <= C0 + sum(lam) + sum(g) + sum(sdiag) + price with the lifted matrix at sdiag = shift w (w = max(d,
-    1) on the vertices, sa on the auxiliary nodes). Returns dict(passed, UB, ...)."""
+    1) on the vertices, sa on the auxiliary nodes). Returns dict(passed, UB, ...); keep=True also returns the lifted
+    structure, g and sdiag under "proof" (for certificate_record)."""
     check_multipliers(inst)
     t0 = time.time()
     if key is None:
@@ -405,11 +406,15 @@ def certify_sparse(inst, g, shift, sa=0.01, mu_c=1e4, device=None, key=None, den
                 seconds=round(time.time() - t0, 2))
     if not ok:
         return dict(passed=False, reason=info.get("failed", "factorization failed"), info=info)
+    proof = dict(lifted=lf, g=np.asarray(g, np.float64), sdiag=sdiag) if keep else None
     total = (Fraction(inst.C0) + (exact_sum(inst.lam) if inst.T else Fraction(0)) + exact_sum(g) + exact_sum(sdiag)
              + Fraction(p_asm) + Fraction(p_fact))
     UB = frac_up(total / Fraction(inst.L))
-    return dict(passed=True, UB=UB, price=(p_asm + p_fact) / inst.L, price_asm=p_asm / inst.L,
-                price_fact=p_fact / inst.L, shift_part=float(exact_sum(sdiag)) / inst.L, info=info)
+    res = dict(passed=True, UB=UB, price=(p_asm + p_fact) / inst.L, price_asm=p_asm / inst.L,
+               price_fact=p_fact / inst.L, shift_part=float(exact_sum(sdiag)) / inst.L, info=info)
+    if keep:
+        res["proof"] = proof
+    return res
 
 
 def certify_estimate_sparse(inst, est, margin=1e-7, device=None, tries=8, log=None, **kw):
@@ -435,3 +440,41 @@ def certify_estimate_sparse(inst, est, margin=1e-7, device=None, tries=8, log=No
         factor *= 2.0
         mg = max(mg * 10.0, 1e-6 * max(base, 1e-3))
     return res
+
+
+# ------------------------------------------------------------------------------------------------ certificate record
+def certificate_record(inst, res):
+    """Everything an independent verifier needs, as plain arrays (the reference's --dump_cert,
+    verify_cert.py:130-177): the graph, volumes, L, C0, the multipliers, g, sdiag, the link stiffness, the lifted
+    trees (link parent, volume, children, child coefficients, root), the cluster -> node maps of every level and VCS
+    level, and the certified UB. `res` is a passed certify_sparse result made with keep=True."""
+    if not res.get("passed") or "proof" not in res:
+        raise ValueError("certificate_record needs a passed certify_sparse(..., keep=True) result")
+    pf = res["proof"]
+    lf = pf["lifted"]
+    rec = dict(n=np.int64(inst.n), N=np.int64(lf.N), L=np.float64(inst.L), C0=np.float64(inst.C0),
+               d=np.asarray(inst.d, np.float64), rowptr=np.asarray(inst.rowptr, np.int64),
+               col=np.asarray(inst.col, np.int64),
+               w=np.zeros(0) if inst.w is None else np.asarray(inst.w, np.float64),
+               TI=np.asarray(inst.TI, np.int64).reshape(-1, 3) if inst.T else np.zeros((0, 3), np.int64),
+               lam=np.asarray(inst.lam, np.float64) if inst.T else np.zeros(0),
+               g=pf["g"], sdiag=np.asarray(pf["sdiag"], np.float64), mu_c=np.float64(lf.mu_c),
+               link_p=lf.link_p, link_vol=lf.link_vol, link_ch=lf.link_ch, link_cf=lf.link_cf,
+               root=np.int64(lf.root), UB=np.float64(res["UB"]),
+               nlev=np.int64(len(inst.levels)), nvcs=np.int64(len(inst.vcs)))
+    for k, (lv, (node, coef)) in enumerate(zip(inst.levels, lf.lv_nodes)):
+        rec[f"lv{k}_lab"] = np.asarray(lv.lab, np.int64)
+        rec[f"lv{k}_sig"] = np.asarray(lv.sig, np.float64)
+        rec[f"lv{k}_node"], rec[f"lv{k}_coef"] = np.asarray(node, np.int64), np.asarray(coef, np.float64)
+    for k, (vl, (node, coef)) in enumerate(zip(inst.vcs, lf.vc_nodes)):
+        rec[f"vc{k}_lab"] = np.asarray(vl.lab, np.int64)
+        rec[f"vc{k}_ptr"] = np.asarray(vl.ptr, np.int64)
+        rec[f"vc{k}_col"] = np.asarray(vl.col, np.int64)
+        rec[f"vc{k}_sig"] = np.asarray(vl.sig, np.float64)
+        rec[f"vc{k}_node"], rec[f"vc{k}_coef"] = np.asarray(node, np.int64), np.asarray(coef, np.float64)
+    return rec
+
+
+def write_certificate(path, inst, res):
+    """certificate_record written as an .npz (plain arrays, loadable without pickles)."""
+    np.savez(path, **certificate_record(inst, res))

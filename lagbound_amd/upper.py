@@ -1,13 +1,14 @@
 """Upper bound of the maximum modularity of a small graph, end to end in one process, in the stage order of the
 reference's `_sdp_upper_bound` (src/lagbound/api.py:195-298): exact reduction -> aggregate levels -> main solver
-phase -> proximal phase resumed from the main checkpoint -> candidate bound -> FP64 certificate.
+phase -> proximal phase resumed from the main checkpoint -> candidate bound -> lifted sparse FP64 certificate ->
+independent verification (the reference's `verify=True`, api.py:280-298).
 
 Differences from the reference's run, all on the input side of the search:
 - levels come from seeded greedy local moving at the resolutions `levels` on the reduced graph (the reference runs
   Leiden at those resolutions on the original graph and restricts the labels; igraph/leidenalg are not part of
   this build);
-- the certificate is the dense FP64 Cholesky certificate of lagbound_amd.bound (the reference's lifted sparse LDL
-  needs CHOLMOD), so this entry point is for graphs of up to a few thousand vertices.
+- the search runs in one process (the reference launches its stages as subprocesses), so this entry point is for
+  graphs of up to a few thousand vertices; larger graphs go through lagbound_amd.pipeline.
 """
 from __future__ import annotations
 
@@ -21,6 +22,8 @@ from . import alm
 from . import bound as B
 from . import instances as I
 from . import reduce as RD
+from . import sparse_cert as SC
+from . import verify as VF
 
 
 def _flags(tag, n, k, nbatch, outer, seed, extra):
@@ -31,8 +34,10 @@ def _flags(tag, n, k, nbatch, outer, seed, extra):
 
 def upper_bound_small(n, keys, k=8, outer=150, prox_outer=40, levels=(16.0, 64.0), seed=0, device=0, dtype="f32",
                       rgd_iters=400, eig_maxiter=400, margin=1e-7, log=None):
-    """Returns dict(value, status, estimate, reduced_n, stages...). status: "certified" (dense FP64 certificate
-    passed), "solved_by_reduction" (no edge between groups: the value is the exact optimum), or "estimate"."""
+    """Returns dict(value, status, estimate, reduced_n, stages...). status: "verified" (the lifted sparse FP64
+    certificate passed and the independent verifier, with the reduction check against the original graph, accepted
+    it), "certified" (the certificate passed, the verifier did not), "solved_by_reduction" (no edge between groups:
+    the value is the exact optimum), or "estimate"."""
     log = log or (lambda *a: None)
     keys = np.asarray(keys, np.int64)
     m = len(keys)
@@ -72,10 +77,12 @@ def upper_bound_small(n, keys, k=8, outer=150, prox_outer=40, levels=(16.0, 64.0
         s2 = alm.Solver(a2, inst=inst2, labels=labs, vcs_labels=[labs[-1]], log=log, signals=False)
         s2.run()
     est = B.estimate(inst2, s2.V, rgd_iters=rgd_iters, eig_maxiter=eig_maxiter, device=device, log=log)
-    cert = B.certify_estimate(inst2, est, margin=margin)
+    cert = SC.certify_estimate_sparse(inst2, est, margin=margin, device=device, keep=True)
     out.update(estimate=est["ub"], dv=est["dv"], pair_weight=w, ntri=int(inst2.T))
     if cert["passed"]:
-        out.update(value=cert["UB"], status="certified", price=cert["price"])
+        ver = VF.verify(SC.certificate_record(inst2, cert), orig=(n, keys, red["group_of"], None))
+        out.update(value=cert["UB"], status="verified" if ver["passed"] else "certified", price=cert["price"],
+                   verified_UB=ver.get("UB"), verifier=ver.get("fail", "passed"))
     else:
         out.update(value=est["ub"], status="estimate", reason=cert.get("reason"))
     return out

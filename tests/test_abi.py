@@ -63,3 +63,15 @@ def test_certificate_library_exports_every_symbol():
     lib = ctypes.CDLL(sparse_cert.CERTLIB)
     for name in names:
         assert hasattr(lib, name), name
+
+
+def test_verifier_library_exports_every_symbol():
+    """liblbverify.so (the independent verifier's factorization) exports every entry point include/lbverify.h
+    declares."""
+    from lagbound_amd import verify
+    src = open(os.path.join(ROOT, "include", "lbverify.h")).read()
+    names = sorted(set(re.findall(r"\b(lbv_[a-z_A-Z0-9]+)\s*\(", src)))
+    assert names == ["lbv_ldl"]
+    lib = ctypes.CDLL(verify.VERIFYLIB)
+    for name in names:
+        assert hasattr(lib, name), name

@@ -101,13 +101,23 @@ def test_sbm_2000_fp64_chain(tmp_path):
     got = eng.separate_triangles(0, thresh=0.3, maxh=100, vtol=1e-3, cap=cap)
     _same_selection(got, O.separate_c(inst, Vs, 0, thresh=0.3, maxh=100, vtol=1e-3, cap=cap), "sbm solved V")
     eng.close()
-    print("[config] sbm estimate + certificate", flush=True)
-    res = ws.estimate("c1est", ck, q, polish_time=120, eig_time=120, extra_args=["--certify", "1"])
+    print("[config] sbm estimate + lifted sparse certificate", flush=True)
+    dump = os.path.join(ws.runs, "c1cert.npz")
+    res = ws.estimate("c1est", ck, q, polish_time=120, eig_time=120, extra_args=["--certify", "2", "--dump_cert", dump])
     cert = res["certificate"]
     print(f"[config] sbm ub {res['ub']:.6f} certified {cert.get('UB')} passed {cert['passed']}", flush=True)
     assert cert["passed"], cert
     assert cert["UB"] >= q
     assert cert["UB"] <= 1.0
+    # the independent verifier: the record against the reduced instance files, and the reduction from the original
+    from lagbound_amd import verify as Vf
+    ver = Vf.verify(dump, Dz)
+    red = Vf.check_reduction_dirs(D, Dz, np.load(planted))
+    print(f"[config] sbm verifier: {ver.get('UB')} passed {ver['passed']} ({ver.get('fail', '')}); reduction {red}",
+          flush=True)
+    assert ver["passed"], ver
+    assert abs(ver["UB"] - cert["UB"]) <= 1e-6 and ver["UB"] >= q
+    assert red["bookkeeping"] == "ok" and red["identity"] == "exact"
 
 
 # --------------------------------------------------------------------------- Chung-Lu n = 300k, FP32 and FP64

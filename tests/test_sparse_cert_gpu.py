@@ -3,6 +3,7 @@
 - the device core equals the host core (same verdict, same rounding sum to 1e-9) on a core of a few thousand rows;
 - on a solved 3000-vertex Chung-Lu request the sparse certificate (device core) and the dense FP64 certificate both
   pass at the same vertex shift, and their bounds differ by the auxiliary nodes' shift up to the two rounding prices;
+  the independent verifier accepts the sparse certificate's record;
 - the 300k-vertex Chung-Lu request (FP32 search, 8 outer iterations, FP64 estimate) is certified, with the time, the
   dense core's size and its device memory printed.
 """
@@ -15,6 +16,7 @@ from lagbound_amd import bound as B
 from lagbound_amd import instances as I
 from lagbound_amd import serve
 from lagbound_amd import sparse_cert as S
+from lagbound_amd import verify as Vf
 
 pytestmark = pytest.mark.gpu
 
@@ -59,8 +61,11 @@ def _solved(n, dtype="f32", outer=8, seed=3):
 
 def test_sparse_matches_dense_on_a_solved_request():
     inst, lab, est = _solved(3000)
-    sp_ = S.certify_estimate_sparse(inst, est, device=0, log=lambda *a: print("[cert]", *a, flush=True))
+    sp_ = S.certify_estimate_sparse(inst, est, device=0, keep=True, log=lambda *a: print("[cert]", *a, flush=True))
     assert sp_["passed"], sp_
+    ver = Vf.verify(S.certificate_record(inst, sp_), log=lambda *a: print("[verify]", *a, flush=True))
+    assert ver["passed"], ver
+    assert abs(ver["UB"] - sp_["UB"]) <= sp_["price"] + ver["price_asm"] + ver["price_psd"] + 1e-12
     shift = sp_["info"]["shift"]
     dn = B.certify_dense_device(inst, est["g"], shift * est["s"])
     assert dn["passed"], dn

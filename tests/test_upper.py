@@ -1,5 +1,6 @@
-"""The whole upper-bound chain (reduction -> levels -> main -> proximal -> estimate -> dense certificate) on the
-reference's own known answers, on the CPU operators (device engine replaced by the FP64 oracle):
+"""The whole upper-bound chain (reduction -> levels -> main -> proximal -> estimate -> lifted sparse certificate ->
+independent verifier) on the reference's own known answers, on the CPU operators (device engine replaced by the FP64
+oracle):
 Zachary's karate club, optimum 0.4197896120973044 (tests/test_bounds.py:10), Les Miserables, optimum 72259/129032
 with the reference's bar U < 0.58 (tests/test_bounds.py:78-82), and the graphs the reductions solve exactly,
 K12 -> 0 and K5 + K6 -> 0.48 (tests/test_bounds.py:69-75)."""
@@ -42,7 +43,7 @@ def test_solved_by_reduction(cpu_chain, g, q):
 
 def test_karate_certified_at_the_optimum(cpu_chain):
     res = upper.upper_bound_small(*nx_keys(nx.karate_club_graph()), outer=400, prox_outer=150)
-    assert res["status"] == "certified", res
+    assert res["status"] == "verified", res
     assert KARATE_OPT <= res["value"] <= KARATE_OPT + 2e-3, res
 
 
@@ -50,5 +51,5 @@ def test_les_miserables_reduces_to_46_and_certifies_below_058(cpu_chain):
     """tests/test_bounds.py:78-82: Les Miserables reduces to 46 nodes; 72259/129032 <= U < 0.58."""
     res = upper.upper_bound_small(*nx_keys(nx.les_miserables_graph()), outer=400, prox_outer=150)
     assert res["reduced_n"] == 46
-    assert res["status"] == "certified", res
+    assert res["status"] == "verified", res
     assert 72259 / 129032 <= res["value"] < 0.58, res
