@@ -201,9 +201,10 @@ int lb_certify_dense(lb_ctx *ctx, const double *dg, double *out);
  * (na, lower: ai >= aj) and, per sparse pivot k, the core part c_k of its column of L (CSR cptr nsp+1 / cidx
  * ascending / cval) with its pivot cd_k, factors S = R R^T and returns out[0] = info (0: factored; k > 0: leading
  * minor k not positive) and out[1] = sum_j (sum_i |R_ij|)^2, from which the caller prices the factor's rounding
- * (Higham, Thm 10.3). method 1: rocSOLVER dpotrf; 2: the library's own blocked right-looking Cholesky (64-column
- * panels, 64-bit offsets); 0: rocSOLVER while m^2 < 2^31, the own kernels above (rocSOLVER's dpotrf_64 reported a
- * non-positive leading minor at m = 131,287 on a matrix whose leading blocks factor, so it is not used there). Host
+ * (Higham, Thm 10.3). method 1: rocSOLVER dpotrf; 2: the library's own blocked right-looking Cholesky (128-column
+ * panels, vector FMA trailing updates, 64-bit offsets); 3: the same with the trailing updates on the FP64 matrix
+ * cores; 0: rocSOLVER while m^2 < 2^31, method 2 above (rocSOLVER's dpotrf_64 reported a non-positive leading minor at
+ * m = 131,287 on a matrix whose leading blocks factor, so it is not used there). Host
  * arrays in; allocates 8 m^2 bytes of device memory on a stream of its own. Returns 0, -1 (bad arguments), -2 (not
  * enough free device memory) or -3 (HIP / rocSOLVER error). */
 int lb_cert_core(int device, int64_t m, int64_t na, const int32_t *ai, const int32_t *aj, const double *av,

@@ -5050,13 +5050,24 @@ __global__ __launch_bounds__(256) void lb_k_chol_update(double *S, int64_t m, in
     for (int u = 0; u < 8; u++)
 #pragma unroll
         for (int v = 0; v < 8; v++) acc[u][v] = 0.0;
+    // staging: thread x loads row x % 128 of columns x / 128 + 2 q of both panels from clamped (in-bounds) addresses,
+    // so the loads carry no branches and are in flight together; rows past the matrix are zeroed after the load
+    const int si = threadIdx.x & 127, sh = threadIdx.x >> 7;
+    const bool ina = r0 + si < m, inb = c0 + si < ce;
+    const int64_t ia = ina ? r0 + si : m - 1, ib = inb ? c0 + si : ce - 1;
     for (int kc = 0; kc < K; kc += LB_CK) {
+        double xa[LB_CK / 2], xb[LB_CK / 2];
+#pragma unroll
+        for (int q = 0; q < LB_CK / 2; q++) {
+            const double *col = S + (kb + kc + sh + 2 * q) * m;
+            xa[q] = col[ia];
+            xb[q] = col[ib];
+        }
         __syncthreads();
-        for (int e = threadIdx.x; e < LB_CK * 128; e += 256) {
-            const int i = e % 128, l = e / 128;
-            const int64_t col = (kb + kc + l) * m;
-            PA[l][i] = r0 + i < m ? S[col + r0 + i] : 0.0;
-            PB[l][i] = c0 + i < ce ? S[col + c0 + i] : 0.0;
+#pragma unroll
+        for (int q = 0; q < LB_CK / 2; q++) {
+            PA[sh + 2 * q][si] = ina ? xa[q] : 0.0;
+            PB[sh + 2 * q][si] = inb ? xb[q] : 0.0;
         }
         __syncthreads();
 #pragma unroll 4
@@ -5070,14 +5081,118 @@ __global__ __launch_bounds__(256) void lb_k_chol_update(double *S, int64_t m, in
                 for (int v = 0; v < 8; v++) acc[u][v] = fma(a[u], b[v], acc[u][v]);
         }
     }
+    // write-back, two columns (16 entries) at a time: the loads from clamped addresses first, then the masked stores
 #pragma unroll
-    for (int v = 0; v < 8; v++) {
-        const int64_t c = c0 + tj + 16 * v;
-        if (c >= ce) continue;
+    for (int v = 0; v < 8; v += 2) {
+        double old[2][8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int64_t r = r0 + ti + 16 * u;
-            if (r < m && r >= c) S[c * m + r] -= acc[u][v];
+        for (int h = 0; h < 2; h++) {
+            const int64_t c = min(c0 + tj + 16 * (v + h), ce - 1);
+#pragma unroll
+            for (int u = 0; u < 8; u++) old[h][u] = S[c * m + min(r0 + ti + 16 * u, m - 1)];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int64_t c = c0 + tj + 16 * (v + h);
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int64_t r = r0 + ti + 16 * u;
+                if (c < ce && r < m && r >= c) S[c * m + r] = old[h][u] - acc[u][v + h];
+            }
+        }
+    }
+}
+
+// The same rank-K update on the FP64 matrix cores (v_mfma_f64_16x16x4f64; lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16], and register v of the result is D[4 v + l / 16][l % 16], as measured on gfx950). The tile is
+// computed transposed, D = P_c P_r^T (D rows = columns of C), so that the 16 lanes of a result row hold 16
+// consecutive rows of one column of S and every store is a contiguous 128-byte run. Wave w of the workgroup owns the
+// 64 x 64 quarter (rows 64 (w % 2), columns 64 (w / 2)) as 4 x 4 blocks of 16 x 16; per step of 4 panel columns each
+// lane reads 8 operands from LDS for 16 matrix instructions. Same tiles, masks and arguments as lb_k_chol_update.
+typedef double lb_d4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void lb_k_chol_update_mfma(double *S, int64_t m, int64_t rb, int64_t cb,
+                                                             int64_t ce, int64_t kb, int K, int tri,
+                                                             const int64_t *info) {
+    __shared__ double PA[LB_CK][128 + 2], PB[LB_CK][128 + 2];
+    if (*info) return;
+    const int64_t t = blockIdx.x;
+    int64_t I = t, J = 0;
+    if (tri) {
+        I = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while (I * (I + 1) / 2 > t) I--;
+        while ((I + 1) * (I + 2) / 2 <= t) I++;
+        J = t - I * (I + 1) / 2;
+    }
+    const int64_t r0 = rb + I * 128, c0 = cb + J * 128;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w & 1) * 64, wc = (w >> 1) * 64;       // the wave's quarter: rows wr.., columns wc..
+    const int li = lane & 15, lk = lane >> 4;
+    lb_d4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = (lb_d4){0.0, 0.0, 0.0, 0.0};
+    // the next chunk of panel columns is loaded into registers while the current one is multiplied: thread x stages
+    // row x % 128 of columns x / 128 + 2 q (q < LB_CK / 2) of both panels
+    // (rows past the matrix read a clamped, in-bounds address and are zeroed after the load, so the loads carry no
+    // branches and are all in flight together)
+    const int si = threadIdx.x & 127, sh = threadIdx.x >> 7;
+    const bool ina = r0 + si < m, inb = c0 + si < ce;
+    const int64_t ia = ina ? r0 + si : m - 1, ib = inb ? c0 + si : ce - 1;
+    double xa[LB_CK / 2], xb[LB_CK / 2];
+    auto fetch = [&](int kc) {
+#pragma unroll
+        for (int q = 0; q < LB_CK / 2; q++) {
+            const double *col = S + (kb + kc + sh + 2 * q) * m;
+            xa[q] = col[ia];
+            xb[q] = col[ib];
+        }
+#pragma unroll
+        for (int q = 0; q < LB_CK / 2; q++) {
+            xa[q] = ina ? xa[q] : 0.0;
+            xb[q] = inb ? xb[q] : 0.0;
+        }
+    };
+    fetch(0);
+    for (int kc = 0; kc < K; kc += LB_CK) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < LB_CK / 2; q++) { PA[sh + 2 * q][si] = xa[q]; PB[sh + 2 * q][si] = xb[q]; }
+        __syncthreads();
+        if (kc + LB_CK < K) fetch(kc + LB_CK);
+#pragma unroll
+        for (int ks = 0; ks < LB_CK; ks += 4) {
+            double opc[4], opr[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                opc[a] = PB[ks + lk][wc + 16 * a + li];     // A operand: D row (a column of C) x panel column
+                opr[a] = PA[ks + lk][wr + 16 * a + li];     // B operand: panel column x D column (a row of C)
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(opc[a], opr[b], acc[a][b], 0, 0, 0);
+        }
+    }
+    // write-back, 16 entries at a time: all 16 loads (clamped addresses) in flight, then the masked stores
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        double old[4][4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int64_t c = min(c0 + wc + 16 * a + 4 * v + lk, ce - 1);
+#pragma unroll
+            for (int b = 0; b < 4; b++) old[v][b] = S[c * m + min(r0 + wr + 16 * b + li, m - 1)];
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int64_t c = c0 + wc + 16 * a + 4 * v + lk;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int64_t r = r0 + wr + 16 * b + li;
+                if (c < ce && r < m && r >= c) S[c * m + r] = old[v][b] - acc[a][b][v];
+            }
         }
     }
 }
@@ -5090,7 +5205,7 @@ int lb_cert_core(int device, int64_t m, int64_t na, const int32_t *ai, const int
                  int64_t nsp, const int64_t *cptr, const int32_t *cidx, const double *cval, const double *cd,
                  int method, double *out) {
     if (m < 1 || na < 0 || nsp < 0 || !out || (na && (!ai || !aj || !av)) || (nsp && (!cptr || !cidx || !cval || !cd)) ||
-        method < 0 || method > 2)
+        method < 0 || method > 3)
         return -1;
     for (int64_t e = 0; e < na; e++)
         if (ai[e] < 0 || ai[e] >= m || aj[e] < 0 || aj[e] > ai[e]) return -1;
@@ -5106,7 +5221,9 @@ int lb_cert_core(int device, int64_t m, int64_t na, const int32_t *ai, const int
     int prev = 0;
     if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) return -3;
     // method 0: the library factorization while m^2 < 2^31, the blocked kernels above; 1: library; 2: blocked kernels
-    const bool own = method == 2 || (method == 0 && m * m >= ((int64_t)1 << 31));
+    // (vector FMA trailing updates); 3: blocked kernels with the trailing updates on the matrix cores (one wave per
+    // SIMD at 128 accumulator registers: measured slower than method 2 at m = 40,000)
+    const bool own = method >= 2 || (method == 0 && m * m >= ((int64_t)1 << 31));
     const SolverApi &api = solver_api();
     if (!own && (!api.ok || (!api.potrf64 && m * m >= ((int64_t)1 << 31)))) { hipSetDevice(prev); return -3; }
     size_t fr = 0, tt = 0;
@@ -5155,6 +5272,7 @@ int lb_cert_core(int device, int64_t m, int64_t na, const int32_t *ai, const int
         // second half's columns, the second half, then one rank-2 LB_CH update of the trailing lower triangle
         double *Sd = (double *)S;
         int64_t *inf = (int64_t *)dinfo;
+        auto upd = method == 3 ? lb_k_chol_update_mfma : lb_k_chol_update;
         auto half = [&](int64_t k) {
             hipLaunchKernelGGL(lb_k_chol_diag, dim3(1), dim3(256), 0, s, Sd, m, k, inf);
             const int64_t rest = m - k - LB_CH;
@@ -5167,15 +5285,15 @@ int lb_cert_core(int device, int64_t m, int64_t na, const int32_t *ai, const int
             const int64_t k2 = k + LB_CH;
             if (k2 < m) {
                 const int64_t ce = std::min<int64_t>(m, k2 + LB_CH);
-                hipLaunchKernelGGL(lb_k_chol_update, dim3((unsigned)((m - k2 + 127) / 128)), dim3(256), 0, s, Sd, m,
-                                   k2, k2, ce, k, LB_CH, 0, (const int64_t *)inf);
+                hipLaunchKernelGGL(upd, dim3((unsigned)((m - k2 + 127) / 128)), dim3(256), 0, s, Sd, m, k2, k2, ce,
+                                   k, LB_CH, 0, (const int64_t *)inf);
                 half(k2);
             }
             const int64_t base = k + 2 * LB_CH, rest = m - base;
             if (rest > 0) {
                 const int64_t T = (rest + 127) / 128;
-                hipLaunchKernelGGL(lb_k_chol_update, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, Sd, m, base,
-                                   base, m, k, 2 * LB_CH, 1, (const int64_t *)inf);
+                hipLaunchKernelGGL(upd, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, Sd, m, base, base, m, k,
+                                   2 * LB_CH, 1, (const int64_t *)inf);
             }
             if ((k / (2 * LB_CH)) % 32 == 31) KHIP(hipStreamSynchronize(s));   // bounded queue depth
         }

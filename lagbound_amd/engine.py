@@ -413,7 +413,8 @@ class Engine(Relabeled):
 def cert_core(device, m, ai, aj, av, cptr, cidx, cval, cd, method=0):
     """The dense core of the lifted sparse certificate on the device (lb_cert_core): returns (info, sum_j (sum_i
     |R_ij|)^2) of the Cholesky factor R of the core's Schur complement (info 0: factored, k > 0: leading minor k not
-    positive). method 0: rocSOLVER below 2^31 entries, the library's blocked kernels above; 1 / 2: force either."""
+    positive). method 0: rocSOLVER below 2^31 entries, the library's blocked kernels above; 1: rocSOLVER; 2: the
+    blocked kernels (vector trailing updates); 3: the blocked kernels with matrix-core trailing updates."""
     L = LB.load()
     ai, aj = np.ascontiguousarray(ai, np.int32), np.ascontiguousarray(aj, np.int32)
     av, cd, cval = (np.ascontiguousarray(x, np.float64) for x in (av, cd, cval))

@@ -2,8 +2,8 @@
 and device memory free up, and each runs the whole bound pipeline with its V and multipliers resident on the device:
 instance build -> seeded label-propagation levels -> device ALM search (pair level, VCS level, triangle and VCS
 separation, PHR updates; lagbound_amd.alm.Solver in process) -> candidate bound (device C(y)X products, FP64
-assembly; lagbound_amd.bound.estimate) -> dense FP64 certificate (assembly and Cholesky on the device) when the
-graph fits it.
+assembly; lagbound_amd.bound.estimate) -> FP64 certificate: the dense one (assembly and Cholesky on the device) up to
+3000 vertices, the lifted sparse one (lagbound_amd.sparse_cert: host LDL^T, dense core on the device) above.
 
 Concurrency: one worker per slot, each with its own device contexts (own HIP stream), so the work of requests that
 are in flight at the same time overlaps on the GPU. By default a worker is a process of its own (spawned and warmed
@@ -85,6 +85,7 @@ class Result:
     ub_search: float | None = None   # the candidate bound in the search context's precision (same W and shift)
     du_fp32_fp64: float | None = None  # |ub_search - ub|: FP32 against FP64 assembly of one result
     cert_note: str | None = None     # why a result is not certified
+    stages: dict = field(default_factory=dict)   # seconds per stage (search, estimate, certificate) and core rows
     log: list = field(default_factory=list)
 
     @property
@@ -157,7 +158,9 @@ def solve_request(req: Request, p: Params, device=0):
     library's calls)."""
     logs = []
     log = lambda *s: logs.append(" ".join(str(x) for x in s))
+    t0 = time.perf_counter()
     inst, lab, s = search_request(req, p, device, log)
+    t1 = time.perf_counter()
     try:
         lag = B.DeviceLagrangian(inst, r=p.k, dtype="f64", device=device)
         try:
@@ -169,6 +172,7 @@ def solve_request(req: Request, p: Params, device=0):
     finally:
         s.eng.close()
     t_dev = s.eng.lib_s + lag.eng.lib_s
+    t2 = time.perf_counter()
     cert = None
     dev = device if p.cert_on_device else None
     if inst.n <= p.cert_dense_max_n and inst.n <= p.cert_max_n:
@@ -178,7 +182,9 @@ def solve_request(req: Request, p: Params, device=0):
         cert = SC.certify_estimate_sparse(inst, est, margin=p.cert_margin, device=dev, max_core=p.cert_max_core,
                                           log=log)
     why = None if (cert and cert["passed"]) else (cert.get("reason") if cert else f"n = {inst.n} above cert_max_n")
-    return dict(n=inst.n, m=inst.nnz // 2, lb=modularity(inst, lab.astype(np.int64)), dv=est["dv"], ub=est["ub"],
+    stages = dict(search_s=round(t1 - t0, 3), estimate_s=round(t2 - t1, 3), cert_s=round(time.perf_counter() - t2, 3),
+                  cert_core=int(cert.get("info", {}).get("core", 0)) if cert else 0)
+    return dict(n=inst.n, m=inst.nnz // 2, stages=stages, lb=modularity(inst, lab.astype(np.int64)), dv=est["dv"], ub=est["ub"],
                 ub_search=ub_s, du_fp32_fp64=(abs(ub_s - est["ub"]) if ub_s is not None else None),
                 certified=bool(cert and cert["passed"]), UB=(cert["UB"] if cert and cert["passed"] else None),
                 cert_note=why, log=logs), t_dev
@@ -429,7 +435,7 @@ def main(argv=None):
 
     def progress(r):
         print(f"[serve] rid {r.rid} n {r.n} m {r.m} latency {r.latency:.3f}s service {r.finish - r.start:.3f}s "
-              f"lb {r.lb:.5f} ub {r.ub:.5f} certified {r.certified}", file=sys.stderr, flush=True)
+              f"lb {r.lb:.5f} ub {r.ub:.5f} certified {r.certified} stages {r.stages}", file=sys.stderr, flush=True)
 
     srv = Server(p, slots=a.slots, budget_bytes=a.budget_gb * 2 ** 30, progress=progress, workers=a.workers)
     res = srv.run(reqs)

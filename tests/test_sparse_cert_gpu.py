@@ -33,19 +33,20 @@ def test_device_core_matches_host_core():
     lf = S.Lifted(inst, g, key=S.vertex_key(inst))
     Ml = lf.assemble(5.0 * lf.w)[0]
     out = {}
-    for dev, meth in ((None, 0), (0, 1), (0, 2)):     # host; device with rocSOLVER; device with the blocked kernels
+    # host; device with rocSOLVER; device with the blocked kernels (vector, then matrix-core trailing updates)
+    for dev, meth in ((None, 0), (0, 1), (0, 2), (0, 3)):
         ok, price, info = S.factor_price(Ml, device=dev, core_deg=256, core_method=meth)
         print(f"[cert] core on {'host' if dev is None else 'device'} (method {meth}): ok {ok} core {info['core']} "
               f"p_core {info.get('p_core')} ({info.get('core_s')} s)", flush=True)
         out[meth] = (ok, price, info)
     assert out[0][2]["core"] > 1000
-    assert out[0][0] and out[1][0] and out[2][0]
-    for meth in (1, 2):
+    assert all(out[k][0] for k in out)
+    for meth in (1, 2, 3):
         assert out[meth][2]["p_core"] == pytest.approx(out[0][2]["p_core"], rel=1e-9)
         assert out[meth][1] == pytest.approx(out[0][1], rel=1e-9)
-    # and below the PSD threshold both refuse (a shift of 0 leaves -A - T)
+    # and below the PSD threshold every method refuses (a shift of 0 leaves -A - T)
     bad = lf.assemble(0.0 * lf.w)[0]
-    for meth in (1, 2):
+    for meth in (1, 2, 3):
         assert not S.factor_price(bad, device=0, core_deg=256, core_method=meth)[0]
 
 
