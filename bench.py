@@ -10,10 +10,12 @@ device synchronisation.
 Also reported on the same instance: the HIP path with BF16-stored V and with FP64 V, the plain FP64 GPU form
 (torch FP64 CSR SpMM, gathers and sparse products), and the FP64 C/OpenMP oracle on the host cores (cpu_baseline). The
 metric's second half, batched bound latency, comes from a Poisson stream of whole bound requests (`serving`:
-p50 / p99 latency, solves/s, every result certified by the FP64 certificate) and from the batched evaluation of 512 uneven graphs in one launch
-per kernel (`batch`).
+p50 / p99 latency, solves/s, every result certified by the FP64 certificate), from the batched evaluation of 512
+uneven graphs in one launch per kernel (`batch`), and from the batched search of 512 such graphs with every graph's
+FP64 certificate (`batch_search`).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-extra] [--no-serve] [--no-batch]
+       [--no-batch-search]
 """
 from __future__ import annotations
 
@@ -109,6 +111,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="use the library's default launch shapes")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream (batched bound latency)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched evaluation")
+    ap.add_argument("--no-batch-search", action="store_true",
+                    help="skip the 512-graph batched search with per-graph certificates")
     ap.add_argument("--serve-count", type=int, default=16)
     ap.add_argument("--no-large", action="store_true", help="skip the n=2.4M BF16 evaluation")
     ap.add_argument("--only-large", action="store_true",
@@ -295,6 +299,14 @@ def main():
         if "ms_per_batch_eval" in bt:
             log(f"batch: {bt['ms_per_batch_eval']} ms per evaluation of {bt['graphs']} graphs "
                 f"({bt['graph_evals_per_s']} graph evaluations/s)")
+    if not a.no_batch_search:
+        # the batched search over 512 graphs of the same distribution (captured iterations of evaluation +
+        # per-graph step, PHR updates, per-graph triangle budgets), then every graph's FP64 certificate
+        out["batch_search"] = _module_json(["lagbound_amd.batch", "--count", "512", "--search", "1"], 1200)
+        bs = out["batch_search"]
+        if "search_s" in bs:
+            log(f"batched search: {bs['search_s']} s ({bs['ms_per_captured_iteration']} ms per captured iteration of "
+                f"{bs['graphs']} graphs), certificates {bs['finish_s']} s, certified {bs['certified']}")
     if a.calibrate:
         import ctypes
         lp = ctypes.CDLL(os.path.join(ROOT, "lagbound_amd", "liblbprobe.so"))

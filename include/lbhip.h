@@ -216,12 +216,26 @@ int lb_cert_core(int device, int64_t m, int64_t na, const int32_t *ai, const int
  * L_g and C0_g. Edges and triangles must stay inside one graph (checked). lb_batch_set_levels gives each graph one
  * pair level: lab (n, cluster ids local to the row's graph), P (G), sig (concatenated P_g x P_g), rq and dsq
  * (concatenated P_g). lb_batch_eval evaluates every graph in one launch of each kernel and writes parts (G x
- * LB_NPARTS, the layout of lb_eval_fgrad per graph) and G = C(y)V for all rows (lb_get_G). VCS levels, sweeps and
- * separation act on single-graph contexts only. */
+ * LB_NPARTS, the layout of lb_eval_fgrad per graph) and G = C(y)V for all rows (lb_get_G). Triangle separation of
+ * the wedge family (fam 0) on a batch context selects over the union (a wedge stays inside its graph); VCS levels,
+ * sweeps and the one-edge family act on single-graph contexts only. */
 int lb_batch_set(lb_ctx *ctx, int G, const int64_t *row_off, const double *L, const double *C0);
 int lb_batch_set_levels(lb_ctx *ctx, const int32_t *lab, const int32_t *P, const double *sig, const double *rq,
                         const double *dsq, double pairlvl_weight);
 int lb_batch_eval(lb_ctx *ctx, double rho, double rt, double *parts);
+/* Batched search (no reference counterpart; the reference's inner solver is its sweeps, alm_vcs.py:1360-1436, one
+ * graph per process): `iters` iterations of [lb_batch_eval at (rho, rt); one Riemannian ascent step of every graph
+ * on the unit rows with its own Barzilai-Borwein step, clamped to [amin, amax] / rmax_g, a0 / rmax_g on its first
+ * step, rmax_g the graph's largest row gradient norm (so amax bounds the turn of any row)], recorded once
+ * into a HIP graph on the context's stream and replayed (re-recorded when rho, rt, a0, amin or amax change, or after
+ * any lb_set_* call). reset = 1 forgets every graph's previous iterate first (after a multiplier update). state
+ * (4 per graph, may be NULL): step, have-previous flag, <s,s>, <s,y>. FP32 or FP64 storage. The last evaluation
+ * precedes the last step: evaluate again before lb_phr_update. */
+int lb_batch_rgd(lb_ctx *ctx, int iters, double rho, double rt, double a0, double amin, double amax, int reset,
+                 double *state);
+/* The batch pair-level multipliers (concatenated P_g x P_g, the layout of lb_batch_set_levels); lb_phr_update
+ * updates them from the last rho > 0 batch evaluation. */
+int lb_batch_get_sig(lb_ctx *ctx, double *sig);
 
 /* Captured evaluation (no reference counterpart): record the evaluation chain (lb_eval_fgrad, or lb_batch_eval on
  * a batch context) at fixed rho, rt into a HIP graph on the context's stream, then replay it `count` times with one

@@ -19,7 +19,7 @@ EXPORTS = ["lb_create", "lb_destroy", "lb_last_error", "lb_set_graph", "lb_set_l
            "lb_eval_fgrad", "lb_get_G", "lb_get_multipliers", "lb_phr_update", "lb_sweeps", "lb_separate_triangles", "lb_separate_vcs", "lb_set_vcs_age", "lb_get_vcs_state",
            "lb_set_weights", "lb_sync", "lb_time_eval",
            "lb_kernel_name", "lb_set_tuning", "lb_version", "lb_reduce_exact", "lb_set_option", "lb_certify_dense", "lb_batch_set",
-           "lb_batch_set_levels", "lb_batch_eval", "lb_capture_eval", "lb_replay_eval",
+           "lb_batch_set_levels", "lb_batch_eval", "lb_batch_rgd", "lb_batch_get_sig", "lb_capture_eval", "lb_replay_eval",
            "lb_blk_alloc", "lb_blk_set", "lb_blk_get", "lb_blk_set_diag", "lb_blk_op", "lb_blk_elem", "lb_blk_gram",
            "lb_blk_comb", "lb_gram_noalloc", "lb_cert_core"]
 
@@ -84,6 +84,9 @@ def load():
         "lb_batch_set": ([ctx, C.c_int, _i64p, _f64p, _f64p], C.c_int),
         "lb_batch_set_levels": ([ctx, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double], C.c_int),
         "lb_batch_eval": ([ctx, C.c_double, C.c_double, _f64p], C.c_int),
+        "lb_batch_rgd": ([ctx, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _f64p],
+                         C.c_int),
+        "lb_batch_get_sig": ([ctx, _f64p], C.c_int),
         "lb_capture_eval": ([ctx, C.c_double, C.c_double], C.c_int),
         "lb_replay_eval": ([ctx, C.c_int, _f64p], C.c_int),
         "lb_reduce_exact": ([ctx, C.c_int64, C.c_int64, _i64p, C.POINTER(C.c_uint64), _i64p, _i64p, _i64p, _f64p,

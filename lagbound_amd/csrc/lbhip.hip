@@ -2529,6 +2529,10 @@ struct lb_ctx {
     double bw = 1.0;
     void *gexec = nullptr; double g_rho = 0.0;   // captured evaluation (hipGraphExec_t)
     bool b_lamsum_ok = false;                    // batch: per-graph lam sums up to date
+    int64_t b_nsig = 0;                          // batch: pair-level multipliers (sum_g P_g^2)
+    DBuf b_R, b_Vp, b_Rp, b_rgst;                // batched search: gradient, previous iterate / gradient, per-graph step
+    void *rgd_exec = nullptr;                    // captured search iteration (hipGraphExec_t) and its parameters
+    double rgd_key[5] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -2543,6 +2547,10 @@ void drop_capture(lb_ctx *c) {
     if (c && c->gexec) {
         hipGraphExecDestroy((hipGraphExec_t)c->gexec);
         c->gexec = nullptr;
+    }
+    if (c && c->rgd_exec) {
+        hipGraphExecDestroy((hipGraphExec_t)c->rgd_exec);
+        c->rgd_exec = nullptr;
     }
 }
 
@@ -3480,11 +3488,11 @@ void lb_destroy(lb_ctx *c) {
                     &c->Vstage, &c->DDb, &c->segkap, &c->seglist, &c->vslack, &c->vage, &c->vvol, &c->rorder,
                     &c->b_off, &c->b_gid, &c->b_L, &c->b_C0, &c->b_Poff, &c->b_soff, &c->b_cg, &c->b_lab, &c->b_sig,
                     &c->b_pp, &c->b_rq, &c->b_dsq, &c->b_zY, &c->b_M, &c->b_pdiag, &c->b_ltot, &c->b_lviol, &c->b_rowv,
-                    &c->b_parts, &c->b_lamsum, &c->b_tptr, &c->b_tidx};
+                    &c->b_parts, &c->b_lamsum, &c->b_tptr, &c->b_tidx, &c->b_R, &c->b_Vp, &c->b_Rp, &c->b_rgst};
     for (DBuf *b : bufs) dfree(*b);
     if (c->hstage) hipHostFree(c->hstage);
     c->hstage = nullptr; c->hstage_bytes = 0;
-    if (c->gexec) hipGraphExecDestroy((hipGraphExec_t)c->gexec);
+    drop_capture(c);
     if (c->fork_ev) hipEventDestroy(c->fork_ev);
     if (c->join_ev) hipEventDestroy(c->join_ev);
     if (c->side) hipStreamDestroy(c->side);
@@ -3920,6 +3928,8 @@ int lb_phr_update(lb_ctx *c) {
     if (!(c->last_rho > 0.0)) return fail(c, -1, "lb_phr_update: the last evaluation had rho = 0");
     HIPCHK(c, hipSetDevice(c->device));
     if (c->nlev) HIPCHK(c, hipMemcpyAsync(c->sig.p, c->pp.p, sizeof(double) * c->soff.back(), hipMemcpyDeviceToDevice, c->stream));
+    if (c->bG && c->blev && c->b_nsig)           // batch mode: every graph's pair level
+        HIPCHK(c, hipMemcpyAsync(c->b_sig.p, c->b_pp.p, sizeof(double) * c->b_nsig, hipMemcpyDeviceToDevice, c->stream));
     if (c->nvcs) {
         int64_t tot = 0;
         for (int64_t x : c->vnnz) tot += x;
@@ -4008,7 +4018,8 @@ int lb_separate_triangles(lb_ctx *c, int fam, double thresh, int64_t maxh, int64
                           double vtol, int64_t cap, int nchunk, int nb, int64_t *TI_out, double *val_out,
                           int64_t *count, double *vmin) {
     drop_capture(c);
-    if (c && c->bG) return fail(c, -1, "lb_separate_triangles: not available on a batch context");
+    // on a batch context only the wedge family: its candidates (two edges at a centre) stay inside one graph
+    if (c && c->bG && fam != 0) return fail(c, -1, "lb_separate_triangles: only fam 0 on a batch context");
     if (!c || !c->n) return fail(c, -1, "lb_separate_triangles: no graph");
     if ((fam != 0 && fam != 1) || maxh < 1 || maxk < 1 || hub2 < 0 || cap2 < 0 || cap < 0 || nchunk < 1 || nb < 1 ||
         !(vtol > 0.0) || !(vtol < 2.0) || !TI_out || !val_out || !count || !vmin)
@@ -5836,6 +5847,7 @@ int lb_batch_set_levels(lb_ctx *c, const int32_t *lab, const int32_t *P, const d
     if ((rc = upload(c, c->b_cg, cg.data(), sizeof(int32_t) * cg.size()))) return rc;
     if ((rc = upload(c, c->b_sig, sig, sizeof(double) * soff[G]))) return rc;
     if ((rc = dalloc(c, c->b_pp, sizeof(double) * soff[G]))) return rc;
+    c->b_nsig = soff[G];
     if ((rc = upload(c, c->b_rq, rq, sizeof(double) * sumP))) return rc;
     if ((rc = upload(c, c->b_dsq, dsq, sizeof(double) * sumP))) return rc;
     if ((rc = dalloc(c, c->b_zY, sizeof(double) * ((size_t)G + sumP) * c->r))) return rc;
@@ -5847,15 +5859,10 @@ int lb_batch_set_levels(lb_ctx *c, const int32_t *lab, const int32_t *P, const d
     return 0;
 }
 
-int lb_batch_eval(lb_ctx *c, double rho, double rt, double *parts) {
-    if (!c || !c->bG) return fail(c, -1, "lb_batch_eval: not in batch mode (lb_batch_set)");
-    if (!(rho >= 0.0) || (rho > 0.0 && c->T && !(rt > 0.0))) return fail(c, -1, "lb_batch_eval: need rho >= 0 and rt > 0 when rho > 0");
-    HIPCHK(c, hipSetDevice(c->device));
-    beval_fn f = pick_beval(c->vdtype, c->r);
-    if (!f) return fail(c, -1, "lb_batch_eval: unsupported rank / dtype");
-    if (c->T && !c->b_tptr.p) return fail(c, -1, "lb_batch_eval: triangles were set before lb_batch_set");
-    int rcl;
-    if ((rcl = ensure_batch_lists(c))) return rcl;
+}  // extern "C"
+
+// the batch kernels' arguments for the context's current buffers
+static BArgs batch_args(lb_ctx *c) {
     BArgs a{};
     a.V = (const uint4 *)c->V.p; a.n = c->n; a.rowptr = (const int64_t *)c->rowptr.p; a.col = (const int32_t *)c->col.p;
     a.w = c->weighted ? c->w.p : nullptr; a.d = (const double *)c->d.p;
@@ -5870,10 +5877,177 @@ int lb_batch_eval(lb_ctx *c, double rho, double rt, double *parts) {
     a.iptr = (const int64_t *)c->iptr.p; a.rec = c->rec.p; a.pt = c->pt.p; a.T = c->T; a.TI = (const int32_t *)c->TI.p;
     a.lam = c->lam.p;
     a.Gout = c->G.p; a.rowv = (double *)c->b_rowv.p; a.parts = (double *)c->b_parts.p; a.lamsum = (double *)c->b_lamsum.p;
-            a.tptr = (const int64_t *)c->b_tptr.p; a.tidx = (const int32_t *)c->b_tidx.p;
+    a.tptr = (const int64_t *)c->b_tptr.p; a.tidx = (const int32_t *)c->b_tidx.p;
     a.light = (const int32_t *)c->b_light.p; a.nlight = c->b_nlight;
     a.heavy = (const int32_t *)c->b_heavy.p; a.nheavy = c->b_nheavy;
-    int rc = f(c, rho, rt, a);
+    return a;
+}
+
+// ------------------------------------------------------------------------------------------------ batched search
+// One Riemannian ascent step of every graph of a batch on the oblique manifold (unit rows), after an evaluation
+// wrote G = C(y)V: the tangent gradient R_i = G_i - (G_i . V_i) V_i, a per-graph Barzilai-Borwein step
+// alpha_g = <s, s> / |<s, y>| (s = V - V_prev, y = R - R_prev over the graph's rows; Barzilai & Borwein, IMA J.
+// Numer. Anal. 8, 1988), clamped to [amin, amax] (a0 on a graph's first step), and the retraction
+// V_i <- (V_i + alpha_g R_i) / |V_i + alpha_g R_i|. The step is scaled by the graph's largest row gradient
+// rmax_g = max_i |R_i|: a graph's first step is a0 / rmax_g, and every step is clamped to [amin, amax] / rmax_g, so
+// that no row turns by much more than amax radians whatever the graph's degrees. One workgroup per graph: its sums
+// are fixed-order block reductions, so a graph's step does not depend on the other graphs. st (4 per graph): alpha,
+// have-previous flag, <s,s>, <s,y>.
+struct BRgdArgs {
+    void *V; const void *Gm; void *Rb; void *Vp; void *Rp;
+    const int64_t *boff; double *st; double a0, amin, amax;
+};
+
+template <typename T, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_brgd(BRgdArgs a) {
+    __shared__ double red[LB_BLOCK / 64];
+    __shared__ double alpha_s;
+    const int g = blockIdx.x;
+    const int64_t r0 = a.boff[g], r1 = a.boff[g + 1];
+    double *st = a.st + 4 * (int64_t)g;
+    const bool prev = st[1] > 0.5;
+    T *V = (T *)a.V, *Rb = (T *)a.Rb, *Vp = (T *)a.Vp, *Rp = (T *)a.Rp;
+    const T *Gm = (const T *)a.Gm;
+    double ss = 0.0, sy = 0.0, rmax = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += LB_BLOCK) {
+        T v[R], gr[R];
+#pragma unroll
+        for (int c = 0; c < R; c++) { v[c] = V[i * R + c]; gr[c] = Gm[i * R + c]; }
+        double gv = 0.0, rn = 0.0;
+#pragma unroll
+        for (int c = 0; c < R; c++) gv += (double)gr[c] * (double)v[c];
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            const T rr = (T)((double)gr[c] - gv * (double)v[c]);
+            rn += (double)rr * (double)rr;
+            if (prev) {
+                const double sd = (double)v[c] - (double)Vp[i * R + c], yd = (double)rr - (double)Rp[i * R + c];
+                ss += sd * sd;
+                sy += sd * yd;
+            }
+            Rb[i * R + c] = rr;
+        }
+        rmax = fmax(rmax, sqrt(rn));
+    }
+    ss = block_sum(ss, red);
+    __syncthreads();
+    sy = block_sum(sy, red);
+    __syncthreads();
+    rmax = -block_min(-rmax, red);
+    if (threadIdx.x == 0) {
+        const double sc = rmax > 0.0 ? 1.0 / rmax : 0.0;
+        double al = prev ? st[0] : a.a0 * sc;
+        if (prev && fabs(sy) > 0.0) al = ss / fabs(sy);
+        al = fmin(fmax(al, a.amin * sc), a.amax * sc);
+        st[0] = al; st[1] = 1.0; st[2] = ss; st[3] = sy;
+        alpha_s = al;
+    }
+    __syncthreads();
+    const double al = alpha_s;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += LB_BLOCK) {   // the rows this thread wrote R for above
+        double w[R], nrm = 0.0;
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            const T v = V[i * R + c], rr = Rb[i * R + c];
+            Vp[i * R + c] = v;
+            Rp[i * R + c] = rr;
+            w[c] = (double)v + al * (double)rr;
+            nrm += w[c] * w[c];
+        }
+        const double inv = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+#pragma unroll
+        for (int c = 0; c < R; c++) V[i * R + c] = (T)(w[c] * inv);
+    }
+}
+
+template <typename T, int R> void brgd_launch(lb_ctx *c, const BRgdArgs &a) {
+    hipLaunchKernelGGL((lb_k_brgd<T, R>), dim3(c->bG), dim3(LB_BLOCK), 0, c->stream, a);
+}
+
+// one search iteration on the context's stream: the batch evaluation at (rho, rt), then the step
+static int brgd_iteration(lb_ctx *c, beval_fn f, double rho, double rt, const BRgdArgs &ra) {
+    int rc = f(c, rho, rt, batch_args(c));
+    if (rc) return rc;
+    const bool dbl = c->vdtype == LB_F64;
+    switch (c->r) {
+        case 8: dbl ? brgd_launch<double, 8>(c, ra) : brgd_launch<float, 8>(c, ra); break;
+        case 16: dbl ? brgd_launch<double, 16>(c, ra) : brgd_launch<float, 16>(c, ra); break;
+        default: dbl ? brgd_launch<double, 32>(c, ra) : brgd_launch<float, 32>(c, ra); break;
+    }
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+extern "C" {
+
+int lb_batch_rgd(lb_ctx *c, int iters, double rho, double rt, double a0, double amin, double amax, int reset,
+                 double *state) {
+    if (!c || !c->bG) return fail(c, -1, "lb_batch_rgd: not in batch mode (lb_batch_set)");
+    if (c->vdtype != LB_F32 && c->vdtype != LB_F64) return fail(c, -1, "lb_batch_rgd: FP32 or FP64 storage only");
+    if (iters < 0 || !(a0 > 0.0) || !(amin > 0.0) || !(amax >= amin) || !(rho >= 0.0) ||
+        (rho > 0.0 && c->T && !(rt > 0.0)))
+        return fail(c, -1, "lb_batch_rgd: bad arguments");
+    if (c->T && !c->b_tptr.p) return fail(c, -1, "lb_batch_rgd: triangles were set before lb_batch_set");
+    beval_fn f = pick_beval(c->vdtype, c->r);
+    if (!f) return fail(c, -1, "lb_batch_rgd: unsupported rank");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    const size_t vb = (size_t)c->n * c->r * c->cbytes;
+    if ((rc = dalloc(c, c->b_R, vb)) || (rc = dalloc(c, c->b_Vp, vb)) || (rc = dalloc(c, c->b_Rp, vb)) ||
+        (rc = dalloc(c, c->b_rgst, sizeof(double) * 4 * (size_t)c->bG)))
+        return rc;
+    if ((rc = ensure_batch_lists(c))) return rc;
+    if (reset) hipLaunchKernelGGL(lb_k_fill, dim3(64), dim3(256), 0, c->stream, (double *)c->b_rgst.p,
+                                  (int64_t)4 * c->bG, 0.0);
+    BRgdArgs ra{};
+    ra.V = c->V.p; ra.Gm = c->G.p; ra.Rb = c->b_R.p; ra.Vp = c->b_Vp.p; ra.Rp = c->b_Rp.p;
+    ra.boff = (const int64_t *)c->b_off.p; ra.st = (double *)c->b_rgst.p; ra.a0 = a0; ra.amin = amin; ra.amax = amax;
+    const double key[5] = {rho, rt, a0, amin, amax};
+    if (c->rgd_exec && std::memcmp(key, c->rgd_key, sizeof(key)) != 0) {
+        hipGraphExecDestroy((hipGraphExec_t)c->rgd_exec);
+        c->rgd_exec = nullptr;
+    }
+    if (!c->rgd_exec && iters > 0) {             // record one iteration; replays launch it as one graph
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = brgd_iteration(c, f, rho, rt, ra);
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &gr);
+        if (rc) { if (gr) hipGraphDestroy(gr); return rc; }
+        if (e != hipSuccess) return fail(c, -3, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        if (e != hipSuccess) return fail(c, -3, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        c->rgd_exec = ge;
+        std::memcpy(c->rgd_key, key, sizeof(key));
+    }
+    for (int k = 0; k < iters; k++) HIPCHK(c, hipGraphLaunch((hipGraphExec_t)c->rgd_exec, c->stream));
+    if (state) HIPCHK(c, hipMemcpyAsync(state, c->b_rgst.p, sizeof(double) * 4 * c->bG, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_eval = false;                        // V moved after the last evaluation
+    return 0;
+}
+
+int lb_batch_get_sig(lb_ctx *c, double *sig) {
+    if (!c || !c->bG || !c->blev) return fail(c, -1, "lb_batch_get_sig: no batch pair levels");
+    if (!sig) return fail(c, -1, "lb_batch_get_sig: bad arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(sig, c->b_sig.p, sizeof(double) * c->b_nsig, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lb_batch_eval(lb_ctx *c, double rho, double rt, double *parts) {
+    if (!c || !c->bG) return fail(c, -1, "lb_batch_eval: not in batch mode (lb_batch_set)");
+    if (!(rho >= 0.0) || (rho > 0.0 && c->T && !(rt > 0.0))) return fail(c, -1, "lb_batch_eval: need rho >= 0 and rt > 0 when rho > 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    beval_fn f = pick_beval(c->vdtype, c->r);
+    if (!f) return fail(c, -1, "lb_batch_eval: unsupported rank / dtype");
+    if (c->T && !c->b_tptr.p) return fail(c, -1, "lb_batch_eval: triangles were set before lb_batch_set");
+    int rcl;
+    if ((rcl = ensure_batch_lists(c))) return rcl;
+    int rc = f(c, rho, rt, batch_args(c));
     if (rc) return rc;
     if (parts) {
         HIPCHK(c, hipMemcpyAsync(parts, c->b_parts.p, sizeof(double) * LB_NPARTS * (size_t)c->bG, hipMemcpyDeviceToHost, c->stream));
@@ -5906,26 +6080,7 @@ int lb_capture_eval(lb_ctx *c, double rho, double rt) {
         beval_fn f = pick_beval(c->vdtype, c->r);
         if (!f) rc = fail(c, -1, "lb_capture_eval: unsupported rank / dtype");
         else if (c->T && !c->b_tptr.p) rc = fail(c, -1, "lb_capture_eval: triangles were set before lb_batch_set");
-        else {
-            BArgs a{};
-            a.V = (const uint4 *)c->V.p; a.n = c->n; a.rowptr = (const int64_t *)c->rowptr.p; a.col = (const int32_t *)c->col.p;
-            a.w = c->weighted ? c->w.p : nullptr; a.d = (const double *)c->d.p;
-            a.gid = (const int32_t *)c->b_gid.p; a.boff = (const int64_t *)c->b_off.p; a.G = c->bG;
-            a.Lg = (const double *)c->b_L.p; a.C0g = (const double *)c->b_C0.p;
-            a.lev = c->blev ? 1 : 0; a.lab = (const int32_t *)c->b_lab.p; a.Poff = (const int32_t *)c->b_Poff.p;
-            a.soff = (const int64_t *)c->b_soff.p; a.cg = (const int32_t *)c->b_cg.p; a.sumP = c->bsumP;
-            a.sig = (const double *)c->b_sig.p; a.pp = (double *)c->b_pp.p; a.rq = (const double *)c->b_rq.p;
-            a.dsq = (const double *)c->b_dsq.p; a.wpair = c->bw;
-            a.zY = (double *)c->b_zY.p; a.M = (double *)c->b_M.p; a.pdiag = (double *)c->b_pdiag.p;
-            a.ltot = (double *)c->b_ltot.p; a.lviol = (double *)c->b_lviol.p;
-            a.iptr = (const int64_t *)c->iptr.p; a.rec = c->rec.p; a.pt = c->pt.p; a.T = c->T; a.TI = (const int32_t *)c->TI.p;
-            a.lam = c->lam.p;
-            a.Gout = c->G.p; a.rowv = (double *)c->b_rowv.p; a.parts = (double *)c->b_parts.p; a.lamsum = (double *)c->b_lamsum.p;
-            a.tptr = (const int64_t *)c->b_tptr.p; a.tidx = (const int32_t *)c->b_tidx.p;
-            a.light = (const int32_t *)c->b_light.p; a.nlight = c->b_nlight;
-            a.heavy = (const int32_t *)c->b_heavy.p; a.nheavy = c->b_nheavy;
-            rc = f(c, rho, rt, a);
-        }
+        else rc = f(c, rho, rt, batch_args(c));
     } else {
         eval_fn f = pick_eval(c->vdtype, c->r);
         if (!f) rc = fail(c, -1, "lb_capture_eval: unsupported rank / dtype");

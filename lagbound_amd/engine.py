@@ -342,6 +342,21 @@ class DeviceEngine:
         self._chk(self.L.lb_batch_eval(self.ctx, float(rho), float(rt), ptr(parts, _f64p)), "lb_batch_eval")
         return parts
 
+    def batch_rgd(self, iters, rho, rt=None, a0=0.3, amin=1e-4, amax=1.0, reset=False):
+        """`iters` captured iterations of [batch evaluation, per-graph Riemannian Barzilai-Borwein ascent step];
+        returns the per-graph state (G x 4: step, have-previous, <s,s>, <s,y>)."""
+        rt = rho if rt is None else rt
+        st = np.zeros((self.batch_G, 4))
+        self._chk(self.L.lb_batch_rgd(self.ctx, int(iters), float(rho), float(rt), float(a0), float(amin),
+                                      float(amax), int(bool(reset)), ptr(st, _f64p)), "lb_batch_rgd")
+        return st
+
+    def batch_get_sig(self, nsig):
+        """The batch pair-level multipliers (concatenated P_g x P_g, nsig values)."""
+        sig = np.zeros(max(1, int(nsig)))
+        self._chk(self.L.lb_batch_get_sig(self.ctx, ptr(sig, _f64p)), "lb_batch_get_sig")
+        return sig[:int(nsig)]
+
     def capture_eval(self, rho, rt=None):
         """Record the evaluation chain at (rho, rt) into a HIP graph (single graph or batch)."""
         rt = rho if rt is None else rt
