@@ -154,7 +154,10 @@ int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int
  * figures always come from the one-stream order); "seg_grid" (default 0) caps the workgroups of that hub-segment
  * kernel (0: one per four segments); "sweep_heavy" / "sweep_seg" (defaults 32 / 128) are the hub threshold and
  * segment length of the sweeps' own segment tables; "step_wave" (default 1) runs the sweeps' row step and VCS post
- * step with one wavefront per batch row (0: one slot of r/4 lanes per row). */
+ * step with one wavefront per batch row (0: one slot of r/4 lanes per row); batch contexts: "batch_light" (default
+ * 128) is the most edges and triangle records a row may have to run as one slot of the light-row kernel (longer rows
+ * take one wavefront each), "batch_light_grid" / "batch_heavy_grid" (defaults 8192) cap the workgroups of those two
+ * kernels (lagbound_amd.tune.autotune_batch picks all three). */
 int lb_set_option(lb_ctx *ctx, const char *name, int64_t value);
 
 /* Device row work of the candidate bound's Riemannian gradient polish (polish_rgd, est_dual_vcs.py:276-308), with

@@ -236,7 +236,9 @@ class BatchSearch:
         return added
 
     def run(self, outer=24, inner=100, rho=0.02, tri_weight=1.0, tri_every=4, tri_per_n=10, thresh=0.3, maxh=100,
-            a0=0.3, amin=1e-4, amax=0.3):
+            a0=0.3, amin=1e-4, amax=0.3, autotune=False):
+        """The outer loop; autotune=True picks the batch kernels' launch shapes (lagbound_amd.tune.autotune_batch)
+        once the first triangle pool is in place."""
         import time
         rt = rho * tri_weight
         for it in range(outer):
@@ -251,6 +253,10 @@ class BatchSearch:
             added = 0
             if tri_every and it % tri_every == 0 and it < outer - 1:
                 added = self._separate(tri_per_n, thresh, maxh)
+                if autotune and it == 0:
+                    from . import tune
+                    self.tuning, _ = tune.autotune_batch(self.be, rho, reps=5)
+                    self.log(f"autotuned batch shapes {self.tuning}")
             t2 = time.perf_counter()
             rec = dict(it=it, inner_s=round(t1 - t0, 4), other_s=round(t2 - t1, 4), T=int(len(self.TI)), added=added,
                        aug_mean=float(np.mean(parts[:, 0] / self.p.L)), step_med=float(np.median(st[:, 0])))
@@ -339,7 +345,7 @@ def search_main(a, log):
     bs = BatchSearch(insts, r=a.r, dtype="f32", log=log)
     t_setup = time.perf_counter() - t
     t = time.perf_counter()
-    hist = bs.run(outer=a.outer, inner=a.inner, rho=a.rho)
+    hist = bs.run(outer=a.outer, inner=a.inner, rho=a.rho, autotune=bool(a.autotune))
     t_search = time.perf_counter() - t
     pairs = bs.results()
     p = bs.p
@@ -353,6 +359,7 @@ def search_main(a, log):
     gap = UB - lb
     it_s = float(np.mean([h["inner_s"] for h in hist])) / a.inner
     out = {"graphs": p.G, "n": p.n, "nnz": int(len(p.col)), "r": a.r, "rho": a.rho, "outer": a.outer,
+           "tuning": getattr(bs, "tuning", None),
            "inner": a.inner, "T_final": int(hist[-1]["T"]), "generate_s": round(t_gen, 2),
            "setup_s": round(t_setup, 2), "search_s": round(t_search, 3),
            "ms_per_captured_iteration": round(1e3 * it_s, 4),
@@ -385,6 +392,7 @@ def main(argv=None):
     ap.add_argument("--outer", type=int, default=24)
     ap.add_argument("--inner", type=int, default=100)
     ap.add_argument("--workers", type=int, default=8, help="processes for the per-graph estimate and certificate")
+    ap.add_argument("--autotune", type=int, default=1, help="1: pick the batch kernels' launch shapes first")
     a = ap.parse_args(argv)
     log = lambda *s: print("[batch]", *s, file=sys.stderr, flush=True)
     if a.search:
@@ -400,6 +408,16 @@ def main(argv=None):
     V /= np.linalg.norm(V, axis=1, keepdims=True)
     be = BatchEngine(p, r=a.r, dtype="f32")
     be.set_V(V)
+    tuning = None
+    if a.autotune:
+        from . import tune
+        t = time.perf_counter()
+        best, trials = tune.autotune_batch(be, a.rho, log=log)
+        tuning = {"best": best, "ms_best": round(min(ms for _, ms in trials), 4),
+                  "ms_default": round(next(ms for c, ms in trials if c == dict(batch_light=128, batch_light_grid=8192,
+                                                                            batch_heavy_grid=8192)), 4),
+                  "candidates": len(trials), "seconds": round(time.perf_counter() - t, 1)}
+        log(f"autotuned batch shapes {best}: {tuning['ms_best']} ms (library defaults {tuning['ms_default']} ms)")
     for _ in range(3):
         be.eval(a.rho, fetch=False)
     be.sync()
@@ -422,6 +440,8 @@ def main(argv=None):
            "ms_per_batch_eval_captured": round(ms_replay, 4),
            "alg_bytes": int(p.n * a.r * 8 + len(p.col) * 4 + p.n * 24 + len(p.TI) * 16)}
     out["hbm_GBps"] = round(out["alg_bytes"] / (ms_batch * 1e-3) / 1e9, 1)
+    if tuning:
+        out["tuning"] = tuning
     log(f"batch: {ms_batch:.3f} ms per evaluation of all {p.G} graphs; captured replays {ms_replay:.3f} ms")
     if a.single:
         from .engine import Engine

@@ -55,6 +55,11 @@
 #ifndef LB_BEB
 #define LB_BEB 8
 #endif
+// batch mode: rows with at most this many edges and triangle records run one slot per row (default of the
+// "batch_light" option)
+#ifndef LB_BLIGHT
+#define LB_BLIGHT 128
+#endif
 #ifndef LB_BRB
 #define LB_BRB 4
 #endif
@@ -2530,6 +2535,7 @@ struct lb_ctx {
     void *gexec = nullptr; double g_rho = 0.0;   // captured evaluation (hipGraphExec_t)
     bool b_lamsum_ok = false;                    // batch: per-graph lam sums up to date
     int64_t b_nsig = 0;                          // batch: pair-level multipliers (sum_g P_g^2)
+    int64_t b_light_max = LB_BLIGHT, b_lgrid = 8192, b_hgrid = 8192;   // batch launch shapes (lb_set_option)
     DBuf b_R, b_Vp, b_Rp, b_rgst;                // batched search: gradient, previous iterate / gradient, per-graph step
     void *rgd_exec = nullptr;                    // captured search iteration (hipGraphExec_t) and its parameters
     double rgd_key[5] = {0, 0, 0, 0, 0};
@@ -3511,6 +3517,14 @@ int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
     if (std::strcmp(name, "seg_grid") == 0) {
         if (value < 0 || value > ((int64_t)1 << 30)) return fail(c, -1, "lb_set_option: seg_grid must be >= 0");
         c->seg_grid = value;
+        return 0;
+    }
+    if (std::strcmp(name, "batch_light") == 0 || std::strcmp(name, "batch_light_grid") == 0 ||
+        std::strcmp(name, "batch_heavy_grid") == 0) {
+        if (value < 1 || value > ((int64_t)1 << 30)) return fail(c, -1, "lb_set_option: batch shapes must be positive");
+        if (name[6] == 'l' && name[11] == '\0') { c->b_light_max = value; c->b_lists_ok = false; }
+        else if (name[6] == 'l') c->b_lgrid = value;
+        else c->b_hgrid = value;
         return 0;
     }
     if (std::strcmp(name, "sweep_heavy") == 0 || std::strcmp(name, "sweep_seg") == 0) {
@@ -5371,10 +5385,6 @@ struct BArgs {
     const int32_t *heavy; int64_t nheavy;       // the others: one wavefront per row
 };
 
-// rows with at most this many edges and triangle records run one slot per row in a batch
-#ifndef LB_BLIGHT
-#define LB_BLIGHT 128
-#endif
 
 // z_g and the cluster sums of graph g = blockIdx.x (Y in LDS when it fits, else global atomics)
 template <typename TV, int R>
@@ -5731,11 +5741,11 @@ struct BLaunch {
         if (c->blev && c->bsumP) hipLaunchKernelGGL(lb_k_blevels<R>, dim3(c->bsumP), dim3(64), 0, s, a);
         constexpr int NSLb = LB_BLOCK / (R / VT<TV>::CPL);   // slots per workgroup of the light-row kernel
         if (a.nlight) {
-            const int64_t lblk = std::min<int64_t>((a.nlight + NSLb - 1) / NSLb, 8192);
+            const int64_t lblk = std::min<int64_t>((a.nlight + NSLb - 1) / NSLb, c->b_lgrid);
             hipLaunchKernelGGL((lb_k_brows_slot<TV, R>), dim3((unsigned)lblk), dim3(LB_BLOCK), 0, s, a);
         }
         if (a.nheavy) {
-            const int64_t nblk = std::min<int64_t>((a.nheavy + 3) / 4, 8192);
+            const int64_t nblk = std::min<int64_t>((a.nheavy + 3) / 4, c->b_hgrid);
             hipLaunchKernelGGL((lb_k_brows<TV, R>), dim3((unsigned)nblk), dim3(LB_BLOCK), 0, s, a);
         }
         hipLaunchKernelGGL(lb_k_bfinal, dim3(c->bG), dim3(LB_BLOCK), 0, s, a, R);
@@ -5755,7 +5765,7 @@ int ensure_batch_lists(lb_ctx *c) {
     for (int64_t i = 0; i < n; i++) {
         const int64_t de = c->h_rowptr[i + 1] - c->h_rowptr[i];
         const int64_t dt = has_t ? c->h_iptr[i + 1] - c->h_iptr[i] : 0;
-        (de <= LB_BLIGHT && dt <= LB_BLIGHT ? L : H).push_back((int32_t)i);
+        (de <= c->b_light_max && dt <= c->b_light_max ? L : H).push_back((int32_t)i);
     }
     int rc;
     if ((rc = dalloc(c, c->b_light, sizeof(int32_t) * std::max<size_t>(L.size(), 1))) ||

@@ -1,8 +1,10 @@
-"""Autotuner of the single-graph evaluation's launch shapes: hub thresholds (rows with more edges / triangle records
-than the threshold are split into wavefront segments), segment length, row-kernel grid and the row order switch.
-Each candidate is uploaded to a fresh context, warmed up and timed with device events (lb_time_eval); the fastest
-is returned and can be applied to any engine with `apply`. No reference counterpart (the reference's parallel
-shapes are numba prange and fixed chunk sizes)."""
+"""Autotuner of the evaluation's launch shapes. Single graph: hub thresholds (rows with more edges / triangle records
+than the threshold are split into wavefront segments), segment length, row-kernel grid and the row order switch;
+each candidate is uploaded to a fresh context, warmed up and timed with device events (lb_time_eval). Batch of
+graphs (autotune_batch): the light-row threshold of the batch row kernels (rows up to it run one slot each, longer
+rows one wavefront each) and the grid caps of both kernels; each candidate is set on one batch context and timed as
+captured replays of the batch evaluation. The fastest is returned and applied with `apply` / `apply_batch`. No
+reference counterpart (the reference's parallel shapes are numba prange and fixed chunk sizes)."""
 from __future__ import annotations
 
 import itertools
@@ -54,4 +56,46 @@ def autotune(inst, V, rho=0.02, r=None, dtype="f32", iters=20, grid=None, device
         if log:
             log(f"tune {cfg}: {ms:.4f} ms")
     best = min(results, key=lambda x: x[1])[0]
+    return best, results
+
+
+BATCH_GRID = {
+    "batch_light": [32, 64, 128, 256, 512],
+    "batch_light_grid": [2048, 8192, 32768],
+    "batch_heavy_grid": [1024, 8192],
+}
+
+
+def batch_candidates(grid=None):
+    g = dict(BATCH_GRID, **(grid or {}))
+    for lt, lg, hg in itertools.product(g["batch_light"], g["batch_light_grid"], g["batch_heavy_grid"]):
+        yield dict(batch_light=lt, batch_light_grid=lg, batch_heavy_grid=hg)
+
+
+def apply_batch(eng: Engine, cfg):
+    """Set a batch configuration on an engine holding a batch (takes effect at its next evaluation)."""
+    for k, v in cfg.items():
+        eng.set_option(k, int(v))
+
+
+def autotune_batch(be, rho=0.02, reps=20, grid=None, log=None):
+    """Time every batch candidate on a BatchEngine `be` (its current V and multipliers) as `reps` captured replays of
+    the batch evaluation (host clock around a synchronized run); leaves the fastest applied. Returns (best config,
+    [(config, ms per evaluation)])."""
+    import time
+    results = []
+    for cfg in batch_candidates(grid):
+        apply_batch(be.eng, cfg)
+        be.eng.capture_eval(rho)
+        be.eng.replay_eval(3, fetch=False)
+        be.sync()
+        t = time.perf_counter()
+        be.eng.replay_eval(reps, fetch=False)
+        be.sync()
+        ms = 1e3 * (time.perf_counter() - t) / reps
+        results.append((cfg, ms))
+        if log:
+            log(f"tune batch {cfg}: {ms:.4f} ms")
+    best = min(results, key=lambda x: x[1])[0]
+    apply_batch(be.eng, best)
     return best, results

@@ -110,3 +110,35 @@ def test_c4_batch_512_graphs():
                 assert np.abs(pt_all[t0:t1] - pto).max() <= 1e-5 * rho + 1e-6 * max(1.0, np.abs(pto).max()), g
         print(f"[c4] rho={rho}: all {p.G} graphs match; worst G rel err {worst:.2e}", flush=True)
     be.close()
+
+
+def test_batch_autotuner_shapes_keep_results():
+    """lagbound_amd.tune.autotune_batch times every batch launch-shape candidate; under the chosen shapes, and under
+    the extremes of the light-row threshold (every row one slot / every row one wavefront), every graph's parts and
+    G match the oracle as in the default shapes (FP32, 1e-5)."""
+    from lagbound_amd import tune
+    insts = make_batch(12, seed=2)
+    p = Packed(insts)
+    rng = np.random.default_rng(6)
+    V = rng.standard_normal((p.n, 8))
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    be = BatchEngine(p, r=8, dtype="f32")
+    try:
+        be.set_V(V)
+        best, trials = tune.autotune_batch(be, 0.02, reps=5, log=lambda *a: print("[batch]", *a, flush=True))
+        assert len(trials) == len(list(tune.batch_candidates()))
+        Vs = V.astype(np.float32).astype(np.float64)
+        for cfg in (best, dict(batch_light=1 << 30, batch_light_grid=64, batch_heavy_grid=64),
+                    dict(batch_light=1, batch_light_grid=8192, batch_heavy_grid=7)):
+            tune.apply_batch(be.eng, cfg)
+            parts = be.eval(0.02)
+            G = be.get_G()
+            for g, x in enumerate(insts):
+                sl = p.rows(g)
+                po, Go, _, _, _ = O.eval_c(x, Vs[sl], 0.02, threads=4)
+                scale = max(abs(po[1]), abs(po[2]), abs(po[6]), 1.0)
+                for k, name in KEYS:
+                    assert abs(parts[g, k] - po[k]) <= 1e-5 * scale + 1e-12, (cfg, g, name)
+                assert np.linalg.norm(G[sl] - Go) <= 1e-5 * max(np.linalg.norm(Go), 1e-300), (cfg, g)
+    finally:
+        be.close()
