@@ -223,6 +223,11 @@ def main():
             pf = json.loads(line[-1])
             extra["plain_fp64_gpu"] = pf
             out["speedup_vs_plain_fp64_gpu"] = round(value * pf["ms_per_eval"] * 1e-3, 2)
+            # beside it, the like-for-like ratios: the FP64 HIP path against the plain FP64 form, and the FP32
+            # headline against the FP64 HIP path
+            if "f64" in extra:
+                out["speedup_f64_hip_vs_plain_fp64_gpu"] = round(pf["ms_per_eval"] / extra["f64"]["ms_per_eval"], 2)
+                out["speedup_f32_vs_f64_hip"] = round(value * extra["f64"]["ms_per_eval"] * 1e-3, 3)
             log(f"plain fp64 gpu: {pf['ms_per_eval']:.3f} ms/eval")
         except Exception as exc:  # the comparison leg is reported, never silently replaced
             extra["plain_fp64_gpu"] = {"error": repr(exc)[:300]}

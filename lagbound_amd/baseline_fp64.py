@@ -1,8 +1,10 @@
 """The plain FP64 GPU form of one f+grad evaluation: the same mathematics as the HIP kernels with straightforward
-torch operations (FP64 CSR SpMM, gathers, sparse products, index_add_ where few rows collide). It is the comparison baseline timed beside the HIP path
-(`vs FP64` in the metric) and is not used by the search. Mirrors `augmented` + `CV` (alm_vcs.py:1496-1533,
-1299-1312). Sums into a few cluster rows (Y per label set, Z per VCS level) are sparse products with the
-cluster-membership matrices rather than index_add_ onto 64 rows, whose atomics all collide."""
+torch operations (FP64 CSR SpMM, gathers, sparse products, index_add_ where few rows collide). It is the comparison
+baseline timed beside the HIP path (`vs FP64` in the metric) and is not used by the search. Mirrors `augmented` +
+`CV` (alm_vcs.py:1496-1533, 1299-1312). Sums into a few cluster rows (Y per label set, Z per VCS level) are sparse
+products with the cluster-membership matrices (measured faster than index_add_ onto 64 rows, whose atomics all
+collide, and than segment sums). Every sparse matrix (A, the memberships, T(pt) with its six entries per triangle,
+the VCS products) is built once with 32-bit indices; a call writes only the values that depend on V."""
 from __future__ import annotations
 
 import numpy as np
@@ -10,7 +12,9 @@ import torch
 
 
 def _csr(crow, col, val, shape, device):
-    return torch.sparse_csr_tensor(torch.as_tensor(crow, dtype=torch.int64), torch.as_tensor(col, dtype=torch.int64),
+    """FP64 CSR with 32-bit indices (64-bit only when the entries need them)."""
+    it = torch.int32 if int(np.asarray(crow)[-1]) < 2 ** 31 else torch.int64
+    return torch.sparse_csr_tensor(torch.as_tensor(np.asarray(crow), dtype=it), torch.as_tensor(np.asarray(col), dtype=it),
                                    torch.as_tensor(val, dtype=torch.float64), shape, device=device)
 
 
@@ -29,8 +33,7 @@ class PlainFP64:
         n = inst.n
         self.n = n
         data = np.ones(inst.nnz) if inst.w is None else inst.w
-        self.A = torch.sparse_csr_tensor(torch.as_tensor(inst.rowptr), torch.as_tensor(inst.col.astype(np.int64)),
-                                         torch.as_tensor(data, dtype=dt), (n, n), device=device)
+        self.A = _csr(inst.rowptr, inst.col, data, (n, n), device)
         self.d = torch.as_tensor(inst.d, dtype=dt, device=device)
         self.L = float(inst.L)
         self.C0 = float(inst.C0)
@@ -48,11 +51,11 @@ class PlainFP64:
             zperm = np.argsort(col, kind="stable")              # entries grouped by cluster: Z = (P x n) @ V
             zcrow = np.zeros(v.P + 1, np.int64)
             np.cumsum(np.bincount(col, minlength=v.P), out=zcrow[1:])
-            self.vcs.append(dict(lab=torch.as_tensor(v.lab.astype(np.int64), device=device), P=v.P,
+            self.vcs.append(dict(Zm=_csr(zcrow, rows[zperm], np.zeros(len(col)), (v.P, n), device),
+                                 Sm=_csr(v.ptr, v.col, np.zeros(len(col)), (n, v.P), device),
+                                 lab=torch.as_tensor(v.lab.astype(np.int64), device=device), P=v.P,
                                  mem=_members(v.lab, v.P, device),
-                                 ptr=torch.as_tensor(np.asarray(v.ptr, np.int64), device=device),
-                                 zperm=torch.as_tensor(zperm, device=device), zcrow=torch.as_tensor(zcrow, device=device),
-                                 zcol=torch.as_tensor(rows[zperm], device=device),
+                                 zperm=torch.as_tensor(zperm, device=device),
                                  rq=torch.as_tensor(v.rq, dtype=dt, device=device),
                                  rows=torch.as_tensor(rows, device=device),
                                  col=torch.as_tensor(v.col.astype(np.int64), device=device),
@@ -72,15 +75,14 @@ class PlainFP64:
         tcrow = np.zeros(n + 1, np.int64)
         np.cumsum(np.bincount(r6, minlength=n), out=tcrow[1:])
         self.tperm = torch.as_tensor(tperm, device=device)
-        self.tcrow = torch.as_tensor(tcrow, device=device)
-        self.tcol = torch.as_tensor(c6[tperm], device=device)
+        self.Tm = _csr(tcrow, c6[tperm], np.zeros(6 * len(TI)), (n, n), device)
 
     def eval(self, V, rho, rt=None):
         """Returns (f as a 0-d tensor, G)."""
         rt = rho if rt is None else rt
         d = self.d
         AV = torch.sparse.mm(self.A, V)
-        z = d @ V
+        z = (d[:, None] * V).sum(0)                 # a column reduction (as d @ V it runs as a one-tile GEMM)
         G = AV - torch.outer(d, z) / self.L
         quad = (AV * V).sum()
         f = quad - (z @ z) / self.L + self.C0
@@ -104,8 +106,8 @@ class PlainFP64:
             else:
                 pt = self.lam
             h = 0.5 * pt
-            vals = torch.cat([h, -h, -h, -h, h, -h])[self.tperm]
-            G = G + torch.sparse.mm(torch.sparse_csr_tensor(self.tcrow, self.tcol, vals, (self.n, self.n), device=V.device), V)
+            self.Tm.values().copy_(torch.cat([h, -h, -h, -h, h, -h])[self.tperm])
+            G = G + torch.sparse.mm(self.Tm, V)
         for v in self.vcs:
             Y = torch.sparse.mm(v["mem"], d[:, None] * V)
             rows, col = v["rows"], v["col"]
@@ -120,10 +122,10 @@ class PlainFP64:
                 f = f - ((pv * pv - v["sig"] * v["sig"]) / (2 * rr)).sum()
             else:
                 pv = v["sig"]
-            Zm = torch.sparse_csr_tensor(v["zcrow"], v["zcol"], (pv * di)[v["zperm"]], (v["P"], self.n), device=V.device)
-            Z = torch.sparse.mm(Zm, V)
-            Sm = torch.sparse_csr_tensor(v["ptr"], col, pv, (self.n, v["P"]), device=V.device)
-            SY = torch.sparse.mm(Sm, Y)
+            v["Zm"].values().copy_((pv * di)[v["zperm"]])
+            Z = torch.sparse.mm(v["Zm"], V)
+            v["Sm"].values().copy_(pv)
+            SY = torch.sparse.mm(v["Sm"], Y)
             ownv = torch.zeros_like(d).index_add_(0, rows, torch.where(own, pv, torch.zeros_like(pv)))
             G = G + 0.5 * d[:, None] * (SY + Z[v["lab"]]) - (ownv * d * d)[:, None] * V
         if rho <= 0:
