@@ -582,6 +582,222 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_vcs_acc(VcsArgs a) {
     for (int e = threadIdx.x; e < width; e += LB_BLOCK) out[2 + e] = shZ[(e / R) * RS + e % R];
 }
 
+// ------------------------------------------------------------------------------------------------ lb_k_vcs_mf
+// The VCS pass in one sweep over the rows, with the Z sums on the FP64 matrix cores. A wavefront takes tiles of 16
+// consecutive rows:
+//   1. the tile's V rows and d (as FP64) and the tile's working-set bounds go to the wavefront's part of LDS;
+//   2. one lane per working-set entry (i, R) of the tile: g = d_i (V_i . Y_R - [R = R_i] d_i) against LDS-staged Y,
+//      the PHR multiplier pv, its PHR value, the violation count and the slack flag (vcsops.py:266-288), and pv
+//      written into a dense 16 x SP block of the tile (row of the tile x cluster; zero where a row has no entry;
+//      every (i, R) is one entry, so the writes never collide);
+//   3. Z^T (R x SP) += (diag(d) V)_tile^T (R x 16) P_tile (16 x SP) with v_mfma_f64_16x16x4f64 over the tile's four
+//      4-row steps (vcs_apply's Z = sum over entries of pv_iR d_i V_i, vcsops.py:326-330).
+// V, the entries and their outputs stream through once; there are no atomics. The wavefronts' Z^T tiles are summed
+// in a fixed order into the workgroup's partial row (2 + sumPv R doubles, as lb_k_vcs_z writes), which lb_k_reduce
+// sums over the workgroups in a fixed order. Used when sumPv <= 128 and ceil(R / 16) ceil(sumPv / 16) <= 8 (the
+// accumulators stay in registers). MFMA lane layout as in lb_k_chol_update_mfma: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; register v of the result is D[4 v + l / 16][l % 16].
+typedef double lb_vd4 __attribute__((ext_vector_type(4)));
+#define LB_VMF_WAVES 4
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// LDS bytes of lb_k_vcs_mf: Y and rq of the VCS clusters, then per wavefront the multiplier tile (compute type),
+// the V tile and d (FP64), the bounds and the labels; at the end the FP64 Z^T tiles of all wavefronts reuse the
+// space from the multiplier tiles on, which must therefore hold NW x 16 MB x SP doubles.
+template <typename TC> inline size_t vcs_mf_lds(int sumPv, int R) {
+    const size_t NB = (sumPv + 15) / 16, SP = NB * 16, MB = (R + 15) / 16, NW = LB_VMF_WAVES;
+    const size_t head = sizeof(double) * ((size_t)sumPv * (R + LB_VPAD) + sumPv);
+    const size_t tiles = sizeof(TC) * (NW * 16 * SP + NW * 16 * R) + sizeof(double) * NW * 16 +
+                         sizeof(int64_t) * NW * LB_MAXVCS * 17 + sizeof(int32_t) * NW * LB_MAXVCS * 16;
+    const size_t zt = sizeof(double) * NW * 16 * MB * SP;
+    return head + std::max(tiles, zt);
+}
+template <typename TV, int R, int NACC>
+__device__ __forceinline__ void vcs_mf_body(const VcsArgs &a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL, MB = (R + 15) / 16, RS = R + LB_VPAD, NW = LB_VMF_WAVES;
+    constexpr int RPL = 64 / S;                                 // tile rows per wavefront-wide load of V
+    constexpr int NLD = (16 + RPL - 1) / RPL;                   // loads per lane for the 16 rows of a tile
+    extern __shared__ double smem[];
+    __shared__ double red[NW];
+    const int NB = (a.sumPv + 15) / 16, SP = NB * 16;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double *shY = smem;                                         // sumPv x RS: the VCS cluster sums
+    double *shq = shY + (int64_t)a.sumPv * RS;                  // sumPv: rq of every VCS cluster
+    TC *dense = reinterpret_cast<TC *>(shq + a.sumPv);          // NW x 16 x SP: the tiles' multipliers
+    TC *tV = dense + NW * 16 * SP;                              // NW x 16 x R: the tile's V rows
+    double *td = reinterpret_cast<double *>(tV + NW * 16 * R);  // NW x 16: the tile's d (8-byte aligned: R even)
+    int64_t *tp = reinterpret_cast<int64_t *>(td + NW * 16);    // NW x LB_MAXVCS x 17: working-set bounds
+    int32_t *tl = reinterpret_cast<int32_t *>(tp + NW * LB_MAXVCS * 17);   // NW x LB_MAXVCS x 16: labels
+    const double *Yg = a.Y + (int64_t)a.Yoff[0] * R;
+    for (int e = threadIdx.x; e < a.sumPv * R; e += 64 * NW) shY[(e / R) * RS + e % R] = Yg[e];
+    for (int m = 0; m < a.nvcs; m++)
+        for (int e = threadIdx.x; e < a.Zoff[m + 1] - a.Zoff[m]; e += 64 * NW) shq[a.Zoff[m] + e] = a.rq[m][e];
+    TC *myd = dense + w * 16 * SP;
+    TC *mV = tV + w * 16 * R;
+    double *mdd = td + w * 16;
+    int64_t *mp = tp + w * LB_MAXVCS * 17;
+    int32_t *ml = tl + w * LB_MAXVCS * 16;
+    for (int e = lane; e < 16 * SP; e += 64) myd[e] = (TC)0;
+    __syncthreads();
+    const TC *sig = (const TC *)a.sig;
+    TC *pv = (TC *)a.pv;
+    double tot = 0.0, viol = 0.0;
+    lb_vd4 acc[NACC];
+#pragma unroll
+    for (int k = 0; k < NACC; k++) acc[k] = (lb_vd4){0.0, 0.0, 0.0, 0.0};
+    const int64_t ntile = (a.n + 15) / 16;
+    // the tile data (V rows, d, working-set bounds, labels) is loaded one tile ahead into registers
+    uint4 pV[NLD];
+    double pd = 0.0;
+    int64_t pp0 = 0, pp1 = 0;
+    int32_t pl = 0;
+    auto fetch = [&](int64_t t) {
+        const int64_t r0 = t * 16;
+#pragma unroll
+        for (int j = 0; j < NLD; j++) {
+            const int k = j * RPL + lane / S;
+            const int64_t row = r0 + k < a.n ? r0 + k : a.n - 1;   // rows past n: a valid row, zeroed when stored
+            pV[j] = a.V[row * S + lane % S];
+        }
+        pd = lane < 16 && r0 + lane < a.n ? a.d[r0 + lane] : 0.0;
+        // lanes 0..16 x levels: bounds; lanes 0..15 x levels: labels (one level per 17 / 16 lanes, up to 3 levels
+        // in one pass; more levels take the loop below)
+        const int mb_ = lane / 17, kb_ = lane % 17;
+        pp0 = mb_ < a.nvcs && mb_ < 3 ? a.ptr[mb_][min(r0 + kb_, a.n)] : 0;
+        pp1 = 0;
+        const int ml_ = lane / 16, kl_ = lane % 16;
+        pl = ml_ < a.nvcs && r0 + kl_ < a.n ? a.lab[ml_][r0 + kl_] : 0;
+    };
+    auto store = [&](int64_t t) {
+        const int64_t r0 = t * 16;
+#pragma unroll
+        for (int j = 0; j < NLD; j++) {
+            const int k = j * RPL + lane / S, q = lane % S;
+            if (k < 16) {
+                TC x[CPL];
+                VT<TV>::unpack(pV[j], x);
+#pragma unroll
+                for (int c = 0; c < CPL; c++) mV[k * R + q * CPL + c] = r0 + k < a.n ? x[c] : (TC)0;
+            }
+        }
+        if (lane < 16) mdd[lane] = pd;
+        const int mb_ = lane / 17, kb_ = lane % 17;
+        if (mb_ < a.nvcs && mb_ < 3) mp[mb_ * 17 + kb_] = pp0;
+        for (int e = 51 + lane; e < a.nvcs * 17; e += 64) mp[e] = a.ptr[e / 17][min(r0 + e % 17, a.n)];
+        const int ml_ = lane / 16, kl_ = lane % 16;
+        if (ml_ < a.nvcs) ml[ml_ * 16 + kl_] = pl;
+    };
+    int64_t t = (int64_t)blockIdx.x * NW + w;
+    const int64_t tstride = (int64_t)gridDim.x * NW;
+    if (t < ntile) fetch(t);
+    for (; t < ntile; t += tstride) {
+        const int64_t r0 = t * 16;
+        // 1. this tile's data to LDS; the next tile's loads go out now and land while this one is computed
+        store(t);
+        if (t + tstride < ntile) fetch(t + tstride);
+        wave_lds_sync();
+        // 2. the entries' multipliers, one lane each, into the dense tile
+        for (int m = 0; m < a.nvcs; m++) {
+            const int64_t E0 = mp[m * 17], E1 = mp[m * 17 + 16];
+            for (int64_t e = E0 + lane; e < E1; e += 64) {
+                const int Rc = a.col[e];
+                const double sg = (double)sig[e];
+                int k = 0;                                  // the entry's row: mp[k] <= e < mp[k + 1]
+#pragma unroll
+                for (int h = 8; h >= 1; h >>= 1) k += mp[m * 17 + k + h] <= e ? h : 0;
+                const int Ri = ml[m * 16 + k];
+                const double di = mdd[k];
+                const double *YR = shY + (int64_t)(a.Zoff[m] + Rc) * RS;
+                double s = 0.0;
+#pragma unroll 4
+                for (int c = 0; c < R; c++) s += (double)mV[k * R + c] * YR[c];
+                const double g = (Rc == Ri) ? di * (s - di) : di * s;
+                double p = sg;
+                if (a.rho > 0.0 && di > 0.0) {
+                    const double rr = a.rho * (a.wv * shq[a.Zoff[m] + Rc] / (di * di));
+                    p = sg - rr * g;
+                    if (p < 0.0) p = 0.0;
+                    tot += (p * p - sg * sg) / (2 * rr);
+                }
+                const TC pc = (TC)p;  // the stored multiplier is what Z and G use
+                pv[e] = pc;
+                if (g < -1e-9) viol += 1.0;
+                a.slack[e] = (p == 0.0 && g > 0.0) ? 1 : 0;   // inactive with slack (vcsops.py:285)
+                myd[k * SP + a.Zoff[m] + Rc] = pc;
+            }
+        }
+        wave_lds_sync();
+        // 3. Z^T += (D V)_tile^T P_tile, four 4-row steps
+        const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+        for (int st = 0; st < 4; st++) {
+            const int k = 4 * st + lk;
+            double av[MB];
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++) {
+                const int c = 16 * mb + li;
+                av[mb] = c < R ? mdd[k] * (double)mV[k * R + c] : 0.0;
+            }
+            for (int b = 0; b < NB; b++) {
+                const double bv = (double)myd[k * SP + 16 * b + li];
+#pragma unroll
+                for (int mb = 0; mb < MB; mb++) {
+                    const int slot = b * MB + mb;
+                    // slot < NB MB <= NACC: the accumulators are indexed with compile-time constants
+#pragma unroll
+                    for (int u = 0; u < NACC; u++)
+                        if (u == slot) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mb], bv, acc[u], 0, 0, 0);
+                }
+            }
+        }
+        wave_lds_sync();
+        for (int e = lane; e < 16 * SP; e += 64) myd[e] = (TC)0;
+        (void)r0;
+    }
+    // the wavefronts' Z^T tiles go to LDS (over the multiplier tiles and the V tiles, which are no longer needed)
+    // and are summed over the waves in a fixed order
+    __syncthreads();
+    double *zt = reinterpret_cast<double *>(dense);             // NW x (16 MB) x SP doubles fit in that space
+    double *myz = zt + (int64_t)w * 16 * MB * SP;
+#pragma unroll
+    for (int u = 0; u < NACC; u++) {
+        if (u >= NB * MB) break;
+        const int b = u / MB, mb = u % MB;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int c = 16 * mb + 4 * v + (lane >> 4);
+            myz[c * SP + 16 * b + (lane & 15)] = acc[u][v];
+        }
+    }
+    tot = block_sum(tot, red);
+    __syncthreads();
+    viol = block_sum(viol, red);
+    __syncthreads();
+    const int width = a.sumPv * R;
+    double *out = a.part + (int64_t)blockIdx.x * (2 + width);
+    if (threadIdx.x == 0) { out[0] = tot; out[1] = viol; }
+    for (int e = threadIdx.x; e < width; e += 64 * NW) {
+        const int cl = e / R, c = e % R;
+        double z = 0.0;
+        for (int k = 0; k < NW; k++) z += zt[(int64_t)k * 16 * MB * SP + c * SP + cl];
+        out[2 + e] = z;
+    }
+}
+
+// four accumulator tiles (R <= 16 and sumPv <= 64): 128 registers, four wavefronts per SIMD without spills
+template <typename TV, int R>
+__global__ __launch_bounds__(64 * LB_VMF_WAVES) __attribute__((amdgpu_waves_per_eu(4))) void lb_k_vcs_mf4(VcsArgs a) {
+    vcs_mf_body<TV, R, 4>(a);
+}
+template <typename TV, int R>
+__global__ __launch_bounds__(64 * LB_VMF_WAVES) void lb_k_vcs_mf8(VcsArgs a) {
+    vcs_mf_body<TV, R, 8>(a);
+}
+
 // ------------------------------------------------------------------------------------------------ triangle record
 // One incidence record of vertex i (x = its V chunk, xu / xw = the chunks of the two other vertices): tau of the
 // triangle in the canonical (a, b, c) order (the same arithmetic for all three owners, as batch_step6 does,
@@ -2527,6 +2743,12 @@ struct lb_ctx {
     std::vector<double> h_d;                // vertex volumes (host copy)
     DBuf rorder;                            // row processing order of lb_k_rows (descending work)
     bool use_order = true;
+    // VCS pass with LDS-resident sums: 1 lb_k_vcs_mf (one pass, Z on the matrix cores), 0 lb_k_vcs_pv + lb_k_vcs_acc;
+    // -1 picks by storage type (measured on the 1M-row graph: one pass for FP32 and FP64 V, two passes for BF16 V)
+    int vcs_kernel = -1;
+    // workgroups of the VCS pass, at most LB_VCS_BLOCKS (the partials' allocation); 0 picks by kernel (measured:
+    // 512 for the one-pass kernel, LB_VCS_BLOCKS for the others)
+    int64_t vcs_grid = 0;
     // batch mode: the graph is the block-diagonal union of bG graphs with contiguous row ranges
     int bG = 0; std::vector<int64_t> hboff; std::vector<int> bP; int bsumP = 0; bool blev = false;
     DBuf b_off, b_gid, b_L, b_C0, b_Poff, b_soff, b_cg, b_lab, b_sig, b_pp, b_rq, b_dsq, b_zY, b_M, b_pdiag, b_ltot,
@@ -2845,6 +3067,7 @@ struct Launch {
         const size_t ldsZ = (sizeof(double) + (LB_BLOCK / 64) * sizeof(TC)) * (size_t)c->sumPv * R;
         const int vcs_lds = ldsZ <= 48 * 1024;
         const int vcs_width = 2 + (vcs_lds ? c->sumPv * R : 0);
+        int64_t vgrid = LB_VCS_BLOCKS;
         if (c->nvcs) {
             VcsArgs va{};
             va.V = (const uint4 *)c->V.p; va.n = n; va.d = (const double *)c->d.p; va.nvcs = c->nvcs;
@@ -2866,12 +3089,22 @@ struct Launch {
             va.zacc = vcs_lds ? 0 : 1;
             const size_t ldsY = sizeof(double) * (size_t)c->sumPv * R;
             if (!vcs_lds) hipLaunchKernelGGL(lb_k_fill, dim3(256), dim3(256), 0, c->stream, (double *)c->Z.p, (int64_t)c->sumPv * R, 0.0);
-            if (vcs_lds) {
+            const int NB = (c->sumPv + 15) / 16, MB = (R + 15) / 16;
+            const size_t ldsMf = vcs_mf_lds<TC>(c->sumPv, R);
+            const bool mf = vcs_lds && (c->vcs_kernel == 1 || (c->vcs_kernel < 0 && c->vdtype != LB_BF16)) &&
+                            c->sumPv <= 128 && NB * MB <= 8 && ldsMf <= 64 * 1024;
+            vgrid = c->vcs_grid > 0 ? c->vcs_grid : (mf ? 512 : LB_VCS_BLOCKS);
+            if (mf) {
+                if (NB * MB <= 4)
+                    hipLaunchKernelGGL((lb_k_vcs_mf4<TV, R>), dim3(vgrid), dim3(64 * LB_VMF_WAVES), ldsMf, c->stream, va);
+                else
+                    hipLaunchKernelGGL((lb_k_vcs_mf8<TV, R>), dim3(vgrid), dim3(64 * LB_VMF_WAVES), ldsMf, c->stream, va);
+            } else if (vcs_lds) {
                 const size_t ldsYp = sizeof(double) * (size_t)c->sumPv * (R + LB_VPAD);
-                hipLaunchKernelGGL((lb_k_vcs_pv<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), ldsYp, c->stream, va);
-                hipLaunchKernelGGL((lb_k_vcs_acc<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), ldsYp, c->stream, va);
+                hipLaunchKernelGGL((lb_k_vcs_pv<TV, R>), dim3(vgrid), dim3(LB_BLOCK), ldsYp, c->stream, va);
+                hipLaunchKernelGGL((lb_k_vcs_acc<TV, R>), dim3(vgrid), dim3(LB_BLOCK), ldsYp, c->stream, va);
             } else {
-                hipLaunchKernelGGL((lb_k_vcs_z<TV, R>), dim3(LB_VCS_BLOCKS), dim3(LB_BLOCK), 0, c->stream, va);
+                hipLaunchKernelGGL((lb_k_vcs_z<TV, R>), dim3(vgrid), dim3(LB_BLOCK), 0, c->stream, va);
             }
         }
         mark();
@@ -2879,7 +3112,7 @@ struct Launch {
             const int width = c->sumPv * R;
             // Z entries sit after the two scalars of each block's partial row
             hipLaunchKernelGGL(lb_k_reduce, dim3((width + 63) / 64), dim3(1024), 0, c->stream,
-                               (const double *)c->vcs_part.p + 2, LB_VCS_BLOCKS, width, vcs_width, (double *)c->Z.p);
+                               (const double *)c->vcs_part.p + 2, (int)vgrid, width, vcs_width, (double *)c->Z.p);
         }
         mark();
         // --- hub segments (joined here when they ran on the side stream)
@@ -2920,7 +3153,7 @@ struct Launch {
         FinArgs fa{};
         fa.R = R; fa.L = c->L; fa.C0 = c->C0; fa.z = zp; fa.lamsum = (const double *)c->lamsum.p;
         fa.lev_tot = (const double *)c->levtot.p; fa.lev_viol = (const double *)c->levviol.p; fa.sumPpair = c->sumPpair;
-        fa.vcs_part = (const double *)c->vcs_part.p; fa.nvcs_blk = c->nvcs ? LB_VCS_BLOCKS : 0; fa.vcs_width = vcs_width;
+        fa.vcs_part = (const double *)c->vcs_part.p; fa.nvcs_blk = c->nvcs ? (int)vgrid : 0; fa.vcs_width = vcs_width;
         fa.row_part = (const double *)c->row_part.p; fa.nrow_blk = (int)nrow_blk;
         fa.seg_part = (const double *)c->seg_part.p; fa.nseg_blk = nseg ? nseg_blk : 0;
         fa.rho = rho; fa.parts = (double *)c->parts.p;
@@ -3513,6 +3746,16 @@ int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
     if (!c || !name) return fail(c, -1, "lb_set_option: bad arguments");
     if (std::strcmp(name, "row_order") == 0) { c->use_order = value != 0; return 0; }
     if (std::strcmp(name, "overlap") == 0) { c->overlap = value != 0; return 0; }
+    if (std::strcmp(name, "vcs_grid") == 0) {
+        if (value < 0 || value > LB_VCS_BLOCKS) return fail(c, -1, "lb_set_option: vcs_grid must be in [0, 1536]");
+        c->vcs_grid = value;
+        return 0;
+    }
+    if (std::strcmp(name, "vcs_kernel") == 0) {
+        if (value < -1 || value > 1) return fail(c, -1, "lb_set_option: vcs_kernel is -1 (by storage type), 0 or 1");
+        c->vcs_kernel = (int)value;
+        return 0;
+    }
     if (std::strcmp(name, "step_wave") == 0) { c->step_wave = value != 0; return 0; }
     if (std::strcmp(name, "seg_grid") == 0) {
         if (value < 0 || value > ((int64_t)1 << 30)) return fail(c, -1, "lb_set_option: seg_grid must be >= 0");

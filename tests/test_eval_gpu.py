@@ -216,3 +216,34 @@ def test_metric_instance_parity():
     inst, V = I.metric_instance()
     eng = _engine(inst, V, "f32", 16)
     _check(eng, inst, 0.02, "f32", "metric instance")
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f64"])
+@pytest.mark.parametrize("r,P", [(16, 64), (8, 16), (32, 32)])
+def test_vcs_pass_forms_agree(dtype, r, P):
+    """The one-pass VCS kernel (multipliers per entry, Z on the FP64 matrix cores) and the two-pass form (multipliers,
+    then Z by LDS adds) give the same multipliers and the same evaluation: pv to 1e-6 relative to |pv| + |sig| (the
+    dot V_i . Y_R is summed in another order, so a stored FP32 multiplier can move by one rounding), f and G to 1e-9
+    relative (FP32 / BF16 V: G is FP32) and both match the oracle; with either workgroup count."""
+    inst, V = chung_lu_instance(n=6000, seed=11, k=r, P=P)
+    eng = _engine(inst, V, dtype, r)
+    sig = np.concatenate([v.sig for v in inst.vcs])
+    out = {}
+    for kern, grid in ((0, 0), (1, 0), (1, 1536), (0, 512)):
+        eng.set_option("vcs_kernel", kern)
+        eng.set_option("vcs_grid", grid)
+        parts = np.array(eng.eval(0.02))
+        out[(kern, grid)] = (parts, eng.get_G(), eng.get_multipliers()[2])
+    p0, G0, pv0 = out[(0, 0)]
+    for key, (p1, G1, pv1) in out.items():
+        assert np.max(np.abs(pv1 - pv0) / (np.abs(pv0) + np.abs(sig) + 1e-30)) <= 1e-6, key
+        gt = 1e-9 if dtype == "f64" else 2e-6
+        assert np.linalg.norm(G1 - G0) <= gt * np.linalg.norm(G0), key
+        assert abs(p1[0] - p0[0]) <= 1e-9 * abs(p0[0]) + 1e-9, key
+    po, Go, ppo, pto, pvo = O.eval_c(inst, eng.get_V(), 0.02)
+    floor = sum(_penalty_floor(inst, 0.02, ppo, pto, pvo).values())
+    tol = TOL[dtype]
+    for key, (p1, G1, _) in out.items():
+        assert abs(p1[0] - po[0]) <= tol * max(1.0, abs(po[0])) + floor, key
+        assert np.linalg.norm(G1 - Go) <= tol * np.linalg.norm(Go), key
+    eng.close()
