@@ -16,7 +16,7 @@ DEFAULT_GRID = {
     "heavy": [64, 128, 256, 512],
     "seg_len": [256, 512, 1024],
     "row_grid": [2048, 4096],
-    "row_order": [1],
+    "row_order": [0, 1],
 }
 
 

@@ -99,3 +99,37 @@ def test_sweeps_then_eval_consistent():
     po, Go, _, _, _ = O.eval_c(inst, eng.get_V(), 0.02)
     assert abs(parts[0] - po[0]) <= 1e-10 * max(1.0, abs(po[0]))
     eng.close()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_captured_sweeps_equal_eager(dtype):
+    """A sweep's per-batch launches run as a captured graph (option sweep_graph 1) or eagerly (the default): the same
+    kernels with the same arguments, so V after the sweeps agrees to the run-to-run rounding of the FP64 LDS adds in
+    the aggregate increments (1e-12 absolute; FP32 V is the same bit for bit in practice). Three calls per engine: the
+    second replays the graph captured by the first (new orders, so new hub-segment lists through the device copy of
+    their bounds), the third changes rho and must capture again; the third call is also checked against the oracle
+    from the same start."""
+    from lagbound_amd.engine import Engine
+    inst, V = chung_lu_instance(n=6000, seed=5, k=16, P=16)
+    rng = np.random.default_rng(9)
+    calls = [(np.stack([rng.permutation(inst.n) for _ in range(2)]), 0.3), (np.stack([rng.permutation(inst.n)]), 0.3),
+             (np.stack([rng.permutation(inst.n)]), 0.5)]
+    out = {}
+    for graph in (1, 0):
+        eng = Engine(r=16, dtype=dtype)
+        eng.set_option("sweep_graph", graph)
+        eng.set_instance(inst)
+        eng.set_V(V)
+        V0 = eng.get_V()
+        Ws = []
+        for orders, rho in calls:
+            eng.sweeps(orders, 16, rho, damp=0.1)
+            Ws.append(eng.get_V())
+        out[graph] = (V0, Ws)
+        eng.close()
+    for a, b in zip(out[1][1], out[0][1]):
+        assert np.abs(a - b).max() <= (1e-12 if dtype == "f64" else 1e-6)
+    Ws = out[1][1]
+    Wo = O.sweeps_c(inst, Ws[-2], calls[-1][0], 16, calls[-1][1], damp=0.1)
+    tol = 1e-10 if dtype == "f64" else 2e-4
+    assert np.abs(Ws[-1] - Wo).max() <= tol
