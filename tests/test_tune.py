@@ -22,4 +22,5 @@ def test_candidates_and_apply():
     assert e.calls[0] == ("tuning", (cs[0]["heavy_edges"], cs[0]["heavy_tris"], cs[0]["seg_len"], cs[0]["row_grid"]))
     assert e.calls[1] == ("option", "row_order", cs[0]["row_order"])
     small = list(tune.candidates({"heavy": [128], "seg_len": [256], "row_grid": [2048, 4096]}))
-    assert len(small) == 2
+    assert len(small) == 2 * len(tune.DEFAULT_GRID["row_order"])
+    assert {c["row_order"] for c in small} == {0, 1}
