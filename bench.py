@@ -96,6 +96,62 @@ def large_graph(tuned, rho, steps):
     return out
 
 
+def summarize_pmc(fetch_dir, write_dir, hitmiss_dir=None, workload="", tuning="", commands=()):
+    """Per-evaluation HBM traffic of the f+grad chain from rocprofv3 counter runs of this script with --calibrate:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) summed over the chain's kernels and averaged over the evaluations
+    (one lb_k_finalize per evaluation), each scaled by the factor that makes the copy probe of the same run
+    (lbp_copy_kernel, 256 MiB read + 256 MiB written per launch) read / write its known bytes; optional TCC hit /
+    miss counts per kernel. Returns the dict stored as profiles/eval_metric_f32_traffic.json."""
+    import csv
+    import glob
+    import collections
+
+    def load(d, name):
+        f = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))
+        if not f:
+            raise FileNotFoundError(f"no counter_collection.csv under {d}")
+        per = collections.defaultdict(lambda: collections.defaultdict(float))
+        for row in csv.DictReader(open(f[0])):
+            if not row["Counter_Name"].startswith(name):
+                continue
+            per[row["Kernel_Name"]][row["Dispatch_Id"]] += float(row["Counter_Value"])
+        return per
+
+    fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    evals = max(len(v) for k, v in fetch.items() if k.startswith("lb_k_finalize"))
+    import numpy as np
+    probe = lambda per: np.mean([x for k, v in per.items() if k.startswith("lbp_copy_kernel") for x in v.values()])  # noqa: E731
+    kib = (256 << 20) / 1024.0
+    ff, wf = kib / probe(fetch), kib / probe(write)
+    hm = {}
+    if hitmiss_dir:
+        hit, miss = load(hitmiss_dir, "TCC_HIT"), load(hitmiss_dir, "TCC_MISS")
+        hm = {k: (sum(hit[k].values()) / evals, sum(miss.get(k, {}).values()) / evals) for k in hit}
+    per_kernel, tf, tw = {}, 0.0, 0.0
+    for k in fetch:
+        # the chain's kernels only: each runs at least once per evaluation
+        if not (k.startswith("lb_k_") or k.startswith("void lb_k_")) or len(fetch[k]) < evals:
+            continue
+        name = k.replace("void ", "").split("(")[0]
+        fk, wk = sum(fetch[k].values()) / evals, sum(write.get(k, {}).values()) / evals
+        e = {"fetch_KiB_raw": round(fk, 1), "write_KiB_raw": round(wk, 1), "fetch_bytes": int(fk * 1024 * ff),
+             "write_bytes": int(wk * 1024 * wf)}
+        if k in hm:
+            h_, m_ = hm[k]
+            e.update(tcc_hit=int(h_), tcc_miss=int(m_), l2_hit_rate=round(h_ / max(h_ + m_, 1.0), 3))
+        per_kernel[name] = e
+        tf += e["fetch_bytes"]
+        tw += e["write_bytes"]
+    return {"workload": workload, "tuning": tuning, "bytes_per_eval": int(tf + tw), "fetch_bytes_per_eval": int(tf),
+            "write_bytes_per_eval": int(tw), "calibration": {
+                "probe": "lbp_copy_kernel, 268435456 bytes read + 268435456 bytes written per launch",
+                "fetch_KiB_per_launch": round(float(probe(fetch)), 1), "write_KiB_per_launch": round(float(probe(write)), 1),
+                "fetch_factor": round(ff, 4), "write_factor": round(wf, 4)},
+            "evals_profiled": evals, "per_kernel": per_kernel, "commands": list(commands),
+            "note": "FETCH_SIZE / WRITE_SIZE summed over the kernels of one evaluation, scaled by the factors that make "
+                    "the copy probe read and write its known byte counts."}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +173,11 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the n=2.4M BF16 evaluation")
     ap.add_argument("--only-large", action="store_true",
                     help="run only the n=2.4M evaluation (with --tuning if given) and print its JSON object")
+    ap.add_argument("--summarize-pmc", nargs="+", metavar="DIR", default=None,
+                    help="no GPU work: write the traffic summary of rocprofv3 counter runs (FETCH_SIZE dir, WRITE_SIZE "
+                         "dir, optional TCC_HIT/TCC_MISS dir) to --pmc-out")
+    ap.add_argument("--pmc-out", default=os.path.join(ROOT, "profiles", "eval_metric_f32_traffic.json"))
+    ap.add_argument("--pmc-note", action="append", default=[], help="a command line recorded in the summary")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain copy kernel of known size (for scaling hardware byte counters)")
     a = ap.parse_args()
@@ -124,6 +185,16 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if a.summarize_pmc:
+        dirs = a.summarize_pmc + [None] * (3 - len(a.summarize_pmc))
+        res = summarize_pmc(dirs[0], dirs[1], dirs[2], tuning=a.tuning, commands=a.pmc_note,
+                            workload="f+grad eval, Chung-Lu gamma=2.1 mean-degree~13 (metric instance M), r=16, FP32 V, "
+                                     "rho=0.02, vertices numbered by descending degree on the device")
+        with open(a.pmc_out, "w") as f:
+            json.dump(res, f, indent=1)
+        log(f"traffic {res['bytes_per_eval'] / 1e9:.3f} GB per evaluation over {res['evals_profiled']} evaluations "
+            f"-> {a.pmc_out}")
+        return
     if a.only_large:
         tl = None
         if a.tuning:

@@ -400,7 +400,14 @@ def summarize(results, wall):
             "certified": len(cert),
             "du_fp32_fp64_max": (max(du) if (du := [r.du_fp32_fp64 for r in results.values()
                                                       if r.du_fp32_fp64 is not None]) else None),
-            "uncertified": {str(r.rid): r.cert_note for r in results.values() if not r.certified}}
+            "uncertified": {str(r.rid): r.cert_note for r in results.values() if not r.certified},
+            # where a request's service time goes: search, candidate bound, FP64 certificate (p50 / p99 over the
+            # requests), and the latency a request would have without its certificate stage
+            "stage_s": {k: {"p50": round(float(np.percentile(v, 50)), 4), "p99": round(float(np.percentile(v, 99)), 4)}
+                        for k in ("search_s", "estimate_s", "cert_s")
+                        if len(v := np.array([r.stages.get(k, 0.0) for r in results.values()]))},
+            "latency_before_cert_p99_s": round(float(np.percentile(
+                lat - np.array([r.stages.get("cert_s", 0.0) for r in results.values()]), 99)), 4)}
 
 
 def main(argv=None):
