@@ -2750,7 +2750,11 @@ struct lb_ctx {
         order, tmask, Kq, stZ, Zs, cnts, Vstage, DDb, segkap, seglist, vslack, vage, vvol;
     std::vector<double> h_d;                // vertex volumes (host copy)
     DBuf rorder;                            // row processing order of lb_k_rows (descending work)
-    bool use_order = true;
+    // option row_order 1: lb_k_rows takes the rows by descending work (rorder); 0 (default): in index order. With
+    // the vertices numbered by degree (the Engine's default) index order is also by work, and a wavefront's rows,
+    // their CSR and record ranges share cache lines: on the 1M-row graph 0.598 against 0.618 ms per evaluation,
+    // lb_k_rows hitting L2 on 51% of its requests against 40%
+    bool use_order = false;
     // VCS pass with LDS-resident sums: 1 lb_k_vcs_mf (one pass, Z on the matrix cores), 0 lb_k_vcs_pv + lb_k_vcs_acc;
     // -1 picks by storage type (measured on the 1M-row graph: one pass for FP32 and FP64 V, two passes for BF16 V)
     int vcs_kernel = -1;
