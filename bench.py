@@ -152,6 +152,44 @@ def summarize_pmc(fetch_dir, write_dir, hitmiss_dir=None, workload="", tuning=""
                     "the copy probe read and write its known byte counts."}
 
 
+def gather_probe(n):
+    """The evaluation's row gathers on the metric instance with nothing around them (liblbprobe lbp_gather): the
+    neighbour rows of every edge in CSR order and the two other rows of every triangle record in incidence order,
+    under the degree numbering the engine uses; FP32 rows (64 B, S = 4 lanes) and BF16-sized rows (32 B, S = 2), from
+    global memory only or with the highest-degree rows staged in LDS (as many as a workgroup's 160 KB hold)."""
+    import ctypes
+    import numpy as np
+    from lagbound_amd import instances as I
+    from lagbound_amd.vertex_order import VertexOrder
+    inst, _ = I.metric_instance(n=n)
+    vo = VertexOrder(inst.rowptr)
+    g, _ = vo.instance(inst)
+    TI = np.asarray(g.TI, np.int64)
+    own = np.concatenate([TI[:, 0], TI[:, 1], TI[:, 2]])
+    u = np.concatenate([TI[:, 1], TI[:, 0], TI[:, 0]])
+    w = np.concatenate([TI[:, 2], TI[:, 2], TI[:, 1]])
+    o = np.argsort(own, kind="stable")
+    rec = np.stack([u[o], w[o]], axis=1).ravel()
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(g.col, np.int64), rec]), np.int32)
+    lp = ctypes.CDLL(os.path.join(ROOT, "lagbound_amd", "liblbprobe.so"))
+    lp.lbp_gather.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    cnt = np.bincount(idx, minlength=g.n)
+    res = {"gathers": int(len(idx)), "n": int(g.n), "runs": []}
+    for S in (4, 2):
+        H = (160 * 1024) // (S * 16) - 16
+        for h, grid in ((0, 1024), (0, 256), (H, 256)):
+            for eb in (4, 8):
+                ms = ctypes.c_double()
+                rc = lp.lbp_gather(idx.ctypes.data, len(idx), g.n, S, eb, grid, h, 5, ctypes.byref(ms))
+                r_ = {"row_bytes": 16 * S, "lds_rows": h, "lds_share": round(float(cnt[:h].sum() / len(idx)), 3),
+                      "eb": eb, "workgroups": grid, "rc": rc, "us": round(ms.value * 1e3, 1),
+                      "rows_per_s": round(len(idx) / (ms.value * 1e-3), -6) if rc == 0 else None}
+                res["runs"].append(r_)
+                log(f"gather probe: {r_}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +216,9 @@ def main():
                          "dir, optional TCC_HIT/TCC_MISS dir) to --pmc-out")
     ap.add_argument("--pmc-out", default=os.path.join(ROOT, "profiles", "eval_metric_f32_traffic.json"))
     ap.add_argument("--pmc-note", action="append", default=[], help="a command line recorded in the summary")
+    ap.add_argument("--gather-probe", action="store_true",
+                    help="only the row-gather probe on the metric instance (measurement of the gather floor and of "
+                         "LDS staging of the hub rows); prints its JSON object")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain copy kernel of known size (for scaling hardware byte counters)")
     a = ap.parse_args()
@@ -194,6 +235,9 @@ def main():
             json.dump(res, f, indent=1)
         log(f"traffic {res['bytes_per_eval'] / 1e9:.3f} GB per evaluation over {res['evals_profiled']} evaluations "
             f"-> {a.pmc_out}")
+        return
+    if a.gather_probe:
+        print(json.dumps({"workload": "row gathers of the metric instance", **gather_probe(a.n)}), flush=True)
         return
     if a.only_large:
         tl = None
