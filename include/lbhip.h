@@ -147,8 +147,14 @@ const char *lb_kernel_name(int k);
  * 13) and 2.4M rows (mean degree 50). No reference
  * counterpart (the reference's parallel shapes are numba prange and chunk sizes, alm_vcs.py:1420-1423). */
 int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int row_grid);
-/* Named switches of the kernels (no reference counterpart): "row_order" (default 1) processes the rows of the f+grad
- * row kernel in descending order of their work instead of index order (same results up to summation order);
+/* Named switches of the kernels (no reference counterpart): "row_order" (default 0) 1 processes the rows of the f+grad
+ * row kernel in descending order of their work instead of index order (same results up to summation order; on a
+ * graph numbered by degree index order is faster);
+ * "vcs_kernel" (default -1: by storage type, 1 for FP32 / FP64 V and 0 for BF16 V) 1 runs the evaluation's VCS pass
+ * as one sweep with the Z sums on the FP64 matrix cores, 0 as two passes (multipliers, then Z by LDS adds), same
+ * multipliers and G up to summation order; "vcs_grid" (default 0: 512 for the one-pass kernel, 1536 otherwise, at
+ * most 1536) its workgroups; "sweep_graph" (default 0) 1 captures a sweep's per-batch launches as one HIP graph and
+ * replays it while the launch arguments are unchanged (same kernels and results as the eager launches);
  * "overlap" (default 1) runs the evaluation's hub-segment kernel on a second stream of the context, concurrently
  * with the cluster-sum and VCS kernels, joined before the row kernel (identical results; lb_time_eval's per-kernel
  * figures always come from the one-stream order); "seg_grid" (default 0) caps the workgroups of that hub-segment
