@@ -187,6 +187,16 @@ def gather_probe(n):
                       "rows_per_s": round(len(idx) / (ms.value * 1e-3), -6) if rc == 0 else None}
                 res["runs"].append(r_)
                 log(f"gather probe: {r_}")
+    # reference points: uniform indices over rows that fit L2 (2 MB), the Infinity Cache (64 MB) and neither (256 MB)
+    rng = np.random.default_rng(0)
+    for nrows in (32768, 1 << 20, 1 << 22):
+        uni = np.ascontiguousarray(rng.integers(0, nrows, len(idx)), np.int32)
+        ms = ctypes.c_double()
+        rc = lp.lbp_gather(uni.ctypes.data, len(uni), nrows, 4, 4, 1024, 0, 5, ctypes.byref(ms))
+        r_ = {"uniform_rows": nrows, "row_bytes": 64, "eb": 4, "workgroups": 1024, "rc": rc, "us": round(ms.value * 1e3, 1),
+              "rows_per_s": round(len(uni) / (ms.value * 1e-3), -6) if rc == 0 else None}
+        res["runs"].append(r_)
+        log(f"gather probe: {r_}")
     return res
 
 

@@ -224,17 +224,19 @@ def test_vcs_pass_forms_agree(dtype, r, P):
     """The one-pass VCS kernel (multipliers per entry, Z on the FP64 matrix cores) and the two-pass form (multipliers,
     then Z by LDS adds) give the same multipliers and the same evaluation: pv to 1e-6 relative to |pv| + |sig| (the
     dot V_i . Y_R is summed in another order, so a stored FP32 multiplier can move by one rounding), f and G to 1e-9
-    relative (FP32 / BF16 V: G is FP32) and both match the oracle; with either workgroup count."""
+    relative (FP32 / BF16 V: G is FP32) and both match the oracle; with either workgroup count and tiles of 16 or
+    32 rows."""
     inst, V = chung_lu_instance(n=6000, seed=11, k=r, P=P)
     eng = _engine(inst, V, dtype, r)
     sig = np.concatenate([v.sig for v in inst.vcs])
     out = {}
-    for kern, grid in ((0, 0), (1, 0), (1, 1536), (0, 512)):
+    for kern, grid, tile in ((0, 0, 16), (1, 0, 16), (1, 1536, 16), (0, 512, 16), (1, 0, 32)):
         eng.set_option("vcs_kernel", kern)
         eng.set_option("vcs_grid", grid)
+        eng.set_option("vcs_tile", tile)
         parts = np.array(eng.eval(0.02))
-        out[(kern, grid)] = (parts, eng.get_G(), eng.get_multipliers()[2])
-    p0, G0, pv0 = out[(0, 0)]
+        out[(kern, grid, tile)] = (parts, eng.get_G(), eng.get_multipliers()[2])
+    p0, G0, pv0 = out[(0, 0, 16)]
     for key, (p1, G1, pv1) in out.items():
         assert np.max(np.abs(pv1 - pv0) / (np.abs(pv0) + np.abs(sig) + 1e-30)) <= 1e-6, key
         gt = 1e-9 if dtype == "f64" else 2e-6
