@@ -153,7 +153,8 @@ int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int
  * "vcs_kernel" (default -1: by storage type, 1 for FP32 / FP64 V and 0 for BF16 V) 1 runs the evaluation's VCS pass
  * as one sweep with the Z sums on the FP64 matrix cores, 0 as two passes (multipliers, then Z by LDS adds), same
  * multipliers and G up to summation order; "vcs_grid" (default 0: 512 for the one-pass kernel, 1536 otherwise, at
- * most 1536) its workgroups; "sweep_graph" (default 0) 1 captures a sweep's per-batch launches as one HIP graph and
+ * most 1536) its workgroups; "vcs_tile" (default 32, else 16) the rows per wavefront tile of the one-pass kernel (32
+ * only where its LDS fits); "sweep_graph" (default 0) 1 captures a sweep's per-batch launches as one HIP graph and
  * replays it while the launch arguments are unchanged (same kernels and results as the eager launches);
  * "overlap" (default 1) runs the evaluation's hub-segment kernel on a second stream of the context, concurrently
  * with the cluster-sum and VCS kernels, joined before the row kernel (identical results; lb_time_eval's per-kernel

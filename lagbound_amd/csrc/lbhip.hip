@@ -607,41 +607,43 @@ __device__ inline void wave_lds_sync() {
 // LDS bytes of lb_k_vcs_mf: Y and rq of the VCS clusters, then per wavefront the multiplier tile (compute type),
 // the V tile and d (FP64), the bounds and the labels; at the end the FP64 Z^T tiles of all wavefronts reuse the
 // space from the multiplier tiles on, which must therefore hold NW x 16 MB x SP doubles.
-template <typename TC> inline size_t vcs_mf_lds(int sumPv, int R) {
+template <typename TC> inline size_t vcs_mf_lds(int sumPv, int R, int TR) {
     const size_t NB = (sumPv + 15) / 16, SP = NB * 16, MB = (R + 15) / 16, NW = LB_VMF_WAVES;
     const size_t head = sizeof(double) * ((size_t)sumPv * (R + LB_VPAD) + sumPv);
-    const size_t tiles = sizeof(TC) * (NW * 16 * SP + NW * 16 * R) + sizeof(double) * NW * 16 +
-                         sizeof(int64_t) * NW * LB_MAXVCS * 17 + sizeof(int32_t) * NW * LB_MAXVCS * 16;
+    const size_t tiles = sizeof(TC) * (NW * TR * SP + NW * TR * R) + sizeof(double) * NW * TR +
+                         sizeof(int64_t) * NW * LB_MAXVCS * (TR + 1) + sizeof(int32_t) * NW * LB_MAXVCS * TR;
     const size_t zt = sizeof(double) * NW * 16 * MB * SP;
     return head + std::max(tiles, zt);
 }
-template <typename TV, int R, int NACC>
+template <typename TV, int R, int NACC, int TR>
 __device__ __forceinline__ void vcs_mf_body(const VcsArgs &a) {
     typedef typename VT<TV>::TC TC;
     constexpr int CPL = VT<TV>::CPL, S = R / CPL, MB = (R + 15) / 16, RS = R + LB_VPAD, NW = LB_VMF_WAVES;
     constexpr int RPL = 64 / S;                                 // tile rows per wavefront-wide load of V
-    constexpr int NLD = (16 + RPL - 1) / RPL;                   // loads per lane for the 16 rows of a tile
+    constexpr int NLD = (TR + RPL - 1) / RPL;                   // loads per lane for the TR rows of a tile
+    constexpr int TB = TR + 1;                                  // working-set bounds per level and tile
+    constexpr int KB = 64 / TB, KL = 64 / TR;                   // levels whose bounds / labels one load covers
     extern __shared__ double smem[];
     __shared__ double red[NW];
     const int NB = (a.sumPv + 15) / 16, SP = NB * 16;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     double *shY = smem;                                         // sumPv x RS: the VCS cluster sums
     double *shq = shY + (int64_t)a.sumPv * RS;                  // sumPv: rq of every VCS cluster
-    TC *dense = reinterpret_cast<TC *>(shq + a.sumPv);          // NW x 16 x SP: the tiles' multipliers
-    TC *tV = dense + NW * 16 * SP;                              // NW x 16 x R: the tile's V rows
-    double *td = reinterpret_cast<double *>(tV + NW * 16 * R);  // NW x 16: the tile's d (8-byte aligned: R even)
-    int64_t *tp = reinterpret_cast<int64_t *>(td + NW * 16);    // NW x LB_MAXVCS x 17: working-set bounds
-    int32_t *tl = reinterpret_cast<int32_t *>(tp + NW * LB_MAXVCS * 17);   // NW x LB_MAXVCS x 16: labels
+    TC *dense = reinterpret_cast<TC *>(shq + a.sumPv);          // NW x TR x SP: the tiles' multipliers
+    TC *tV = dense + NW * TR * SP;                              // NW x TR x R: the tile's V rows
+    double *td = reinterpret_cast<double *>(tV + NW * TR * R);  // NW x TR: the tile's d (8-byte aligned: R even)
+    int64_t *tp = reinterpret_cast<int64_t *>(td + NW * TR);    // NW x LB_MAXVCS x TB: working-set bounds
+    int32_t *tl = reinterpret_cast<int32_t *>(tp + NW * LB_MAXVCS * TB);   // NW x LB_MAXVCS x TR: labels
     const double *Yg = a.Y + (int64_t)a.Yoff[0] * R;
     for (int e = threadIdx.x; e < a.sumPv * R; e += 64 * NW) shY[(e / R) * RS + e % R] = Yg[e];
     for (int m = 0; m < a.nvcs; m++)
         for (int e = threadIdx.x; e < a.Zoff[m + 1] - a.Zoff[m]; e += 64 * NW) shq[a.Zoff[m] + e] = a.rq[m][e];
-    TC *myd = dense + w * 16 * SP;
-    TC *mV = tV + w * 16 * R;
-    double *mdd = td + w * 16;
-    int64_t *mp = tp + w * LB_MAXVCS * 17;
-    int32_t *ml = tl + w * LB_MAXVCS * 16;
-    for (int e = lane; e < 16 * SP; e += 64) myd[e] = (TC)0;
+    TC *myd = dense + w * TR * SP;
+    TC *mV = tV + w * TR * R;
+    double *mdd = td + w * TR;
+    int64_t *mp = tp + w * LB_MAXVCS * TB;
+    int32_t *ml = tl + w * LB_MAXVCS * TR;
+    for (int e = lane; e < TR * SP; e += 64) myd[e] = (TC)0;
     __syncthreads();
     const TC *sig = (const TC *)a.sig;
     TC *pv = (TC *)a.pv;
@@ -649,67 +651,69 @@ __device__ __forceinline__ void vcs_mf_body(const VcsArgs &a) {
     lb_vd4 acc[NACC];
 #pragma unroll
     for (int k = 0; k < NACC; k++) acc[k] = (lb_vd4){0.0, 0.0, 0.0, 0.0};
-    const int64_t ntile = (a.n + 15) / 16;
+    const int64_t ntile = (a.n + TR - 1) / TR;
     // the tile data (V rows, d, working-set bounds, labels) is loaded one tile ahead into registers
     uint4 pV[NLD];
     double pd = 0.0;
     int64_t pp0 = 0, pp1 = 0;
     int32_t pl = 0;
     auto fetch = [&](int64_t t) {
-        const int64_t r0 = t * 16;
+        const int64_t r0 = t * TR;
 #pragma unroll
         for (int j = 0; j < NLD; j++) {
             const int k = j * RPL + lane / S;
             const int64_t row = r0 + k < a.n ? r0 + k : a.n - 1;   // rows past n: a valid row, zeroed when stored
             pV[j] = a.V[row * S + lane % S];
         }
-        pd = lane < 16 && r0 + lane < a.n ? a.d[r0 + lane] : 0.0;
-        // lanes 0..16 x levels: bounds; lanes 0..15 x levels: labels (one level per 17 / 16 lanes, up to 3 levels
-        // in one pass; more levels take the loop below)
-        const int mb_ = lane / 17, kb_ = lane % 17;
-        pp0 = mb_ < a.nvcs && mb_ < 3 ? a.ptr[mb_][min(r0 + kb_, a.n)] : 0;
+        pd = lane < TR && r0 + lane < a.n ? a.d[r0 + lane] : 0.0;
+        // bounds: TB lanes per level for the first KB levels; labels: TR lanes per level for the first KL levels;
+        // further levels are loaded when the tile is stored
+        const int mb_ = lane / TB, kb_ = lane % TB;
+        pp0 = mb_ < a.nvcs && mb_ < KB ? a.ptr[mb_][min(r0 + kb_, a.n)] : 0;
         pp1 = 0;
-        const int ml_ = lane / 16, kl_ = lane % 16;
-        pl = ml_ < a.nvcs && r0 + kl_ < a.n ? a.lab[ml_][r0 + kl_] : 0;
+        const int ml_ = lane / TR, kl_ = lane % TR;
+        pl = ml_ < a.nvcs && ml_ < KL && r0 + kl_ < a.n ? a.lab[ml_][r0 + kl_] : 0;
     };
     auto store = [&](int64_t t) {
-        const int64_t r0 = t * 16;
+        const int64_t r0 = t * TR;
 #pragma unroll
         for (int j = 0; j < NLD; j++) {
             const int k = j * RPL + lane / S, q = lane % S;
-            if (k < 16) {
+            if (k < TR) {
                 TC x[CPL];
                 VT<TV>::unpack(pV[j], x);
 #pragma unroll
                 for (int c = 0; c < CPL; c++) mV[k * R + q * CPL + c] = r0 + k < a.n ? x[c] : (TC)0;
             }
         }
-        if (lane < 16) mdd[lane] = pd;
-        const int mb_ = lane / 17, kb_ = lane % 17;
-        if (mb_ < a.nvcs && mb_ < 3) mp[mb_ * 17 + kb_] = pp0;
-        for (int e = 51 + lane; e < a.nvcs * 17; e += 64) mp[e] = a.ptr[e / 17][min(r0 + e % 17, a.n)];
-        const int ml_ = lane / 16, kl_ = lane % 16;
-        if (ml_ < a.nvcs) ml[ml_ * 16 + kl_] = pl;
+        if (lane < TR) mdd[lane] = pd;
+        const int mb_ = lane / TB, kb_ = lane % TB;
+        if (mb_ < a.nvcs && mb_ < KB) mp[mb_ * TB + kb_] = pp0;
+        for (int e = KB * TB + lane; e < a.nvcs * TB; e += 64) mp[e] = a.ptr[e / TB][min(r0 + e % TB, a.n)];
+        const int ml_ = lane / TR, kl_ = lane % TR;
+        if (ml_ < a.nvcs && ml_ < KL) ml[ml_ * TR + kl_] = pl;
+        for (int e = KL * TR + lane; e < a.nvcs * TR; e += 64)
+            ml[e] = r0 + e % TR < a.n ? a.lab[e / TR][r0 + e % TR] : 0;
     };
     int64_t t = (int64_t)blockIdx.x * NW + w;
     const int64_t tstride = (int64_t)gridDim.x * NW;
     if (t < ntile) fetch(t);
     for (; t < ntile; t += tstride) {
-        const int64_t r0 = t * 16;
+        const int64_t r0 = t * TR;
         // 1. this tile's data to LDS; the next tile's loads go out now and land while this one is computed
         store(t);
         if (t + tstride < ntile) fetch(t + tstride);
         wave_lds_sync();
         // 2. the entries' multipliers, one lane each, into the dense tile
         for (int m = 0; m < a.nvcs; m++) {
-            const int64_t E0 = mp[m * 17], E1 = mp[m * 17 + 16];
+            const int64_t E0 = mp[m * TB], E1 = mp[m * TB + TR];
             for (int64_t e = E0 + lane; e < E1; e += 64) {
                 const int Rc = a.col[e];
                 const double sg = (double)sig[e];
                 int k = 0;                                  // the entry's row: mp[k] <= e < mp[k + 1]
 #pragma unroll
-                for (int h = 8; h >= 1; h >>= 1) k += mp[m * 17 + k + h] <= e ? h : 0;
-                const int Ri = ml[m * 16 + k];
+                for (int h = TR / 2; h >= 1; h >>= 1) k += mp[m * TB + k + h] <= e ? h : 0;
+                const int Ri = ml[m * TR + k];
                 const double di = mdd[k];
                 const double *YR = shY + (int64_t)(a.Zoff[m] + Rc) * RS;
                 double s = 0.0;
@@ -731,10 +735,10 @@ __device__ __forceinline__ void vcs_mf_body(const VcsArgs &a) {
             }
         }
         wave_lds_sync();
-        // 3. Z^T += (D V)_tile^T P_tile, four 4-row steps
+        // 3. Z^T += (D V)_tile^T P_tile, TR / 4 steps of 4 rows
         const int li = lane & 15, lk = lane >> 4;
-#pragma unroll
-        for (int st = 0; st < 4; st++) {
+#pragma unroll 4
+        for (int st = 0; st < TR / 4; st++) {
             const int k = 4 * st + lk;
             double av[MB];
 #pragma unroll
@@ -755,7 +759,7 @@ __device__ __forceinline__ void vcs_mf_body(const VcsArgs &a) {
             }
         }
         wave_lds_sync();
-        for (int e = lane; e < 16 * SP; e += 64) myd[e] = (TC)0;
+        for (int e = lane; e < TR * SP; e += 64) myd[e] = (TC)0;
         (void)r0;
     }
     // the wavefronts' Z^T tiles go to LDS (over the multiplier tiles and the V tiles, which are no longer needed)
@@ -789,13 +793,13 @@ __device__ __forceinline__ void vcs_mf_body(const VcsArgs &a) {
 }
 
 // four accumulator tiles (R <= 16 and sumPv <= 64): 128 registers, four wavefronts per SIMD without spills
-template <typename TV, int R>
+template <typename TV, int R, int TR>
 __global__ __launch_bounds__(64 * LB_VMF_WAVES) __attribute__((amdgpu_waves_per_eu(4))) void lb_k_vcs_mf4(VcsArgs a) {
-    vcs_mf_body<TV, R, 4>(a);
+    vcs_mf_body<TV, R, 4, TR>(a);
 }
-template <typename TV, int R>
+template <typename TV, int R, int TR>
 __global__ __launch_bounds__(64 * LB_VMF_WAVES) void lb_k_vcs_mf8(VcsArgs a) {
-    vcs_mf_body<TV, R, 8>(a);
+    vcs_mf_body<TV, R, 8, TR>(a);
 }
 
 // ------------------------------------------------------------------------------------------------ triangle record
@@ -2761,6 +2765,9 @@ struct lb_ctx {
     // workgroups of the VCS pass, at most LB_VCS_BLOCKS (the partials' allocation); 0 picks by kernel (measured:
     // 512 for the one-pass kernel, LB_VCS_BLOCKS for the others)
     int64_t vcs_grid = 0;
+    // rows per wavefront tile of the one-pass VCS kernel: 32 when its LDS fits 64 KB (FP32 / BF16 V at P <= 64), else
+    // 16; measured on the 1M-row graph: 140 against 150 us
+    int vcs_tile = 32;
     // batch mode: the graph is the block-diagonal union of bG graphs with contiguous row ranges
     int bG = 0; std::vector<int64_t> hboff; std::vector<int> bP; int bsumP = 0; bool blev = false;
     DBuf b_off, b_gid, b_L, b_C0, b_Poff, b_soff, b_cg, b_lab, b_sig, b_pp, b_rq, b_dsq, b_zY, b_M, b_pdiag, b_ltot,
@@ -3108,15 +3115,21 @@ struct Launch {
             const size_t ldsY = sizeof(double) * (size_t)c->sumPv * R;
             if (!vcs_lds) hipLaunchKernelGGL(lb_k_fill, dim3(256), dim3(256), 0, c->stream, (double *)c->Z.p, (int64_t)c->sumPv * R, 0.0);
             const int NB = (c->sumPv + 15) / 16, MB = (R + 15) / 16;
-            const size_t ldsMf = vcs_mf_lds<TC>(c->sumPv, R);
+            // tiles of 32 rows when they fit the LDS budget (twice the work per wavefront between waits), else 16
+            const int TRw = (c->vcs_tile == 32 && vcs_mf_lds<TC>(c->sumPv, R, 32) <= 64 * 1024) ? 32 : 16;
+            const size_t ldsMf = vcs_mf_lds<TC>(c->sumPv, R, TRw);
             const bool mf = vcs_lds && (c->vcs_kernel == 1 || (c->vcs_kernel < 0 && c->vdtype != LB_BF16)) &&
                             c->sumPv <= 128 && NB * MB <= 8 && ldsMf <= 64 * 1024;
             vgrid = c->vcs_grid > 0 ? c->vcs_grid : (mf ? 512 : LB_VCS_BLOCKS);
             if (mf) {
-                if (NB * MB <= 4)
-                    hipLaunchKernelGGL((lb_k_vcs_mf4<TV, R>), dim3(vgrid), dim3(64 * LB_VMF_WAVES), ldsMf, c->stream, va);
-                else
-                    hipLaunchKernelGGL((lb_k_vcs_mf8<TV, R>), dim3(vgrid), dim3(64 * LB_VMF_WAVES), ldsMf, c->stream, va);
+                const dim3 gv((unsigned)vgrid), bv(64 * LB_VMF_WAVES);
+                if (NB * MB <= 4) {
+                    if (TRw == 32) hipLaunchKernelGGL((lb_k_vcs_mf4<TV, R, 32>), gv, bv, ldsMf, c->stream, va);
+                    else hipLaunchKernelGGL((lb_k_vcs_mf4<TV, R, 16>), gv, bv, ldsMf, c->stream, va);
+                } else {
+                    if (TRw == 32) hipLaunchKernelGGL((lb_k_vcs_mf8<TV, R, 32>), gv, bv, ldsMf, c->stream, va);
+                    else hipLaunchKernelGGL((lb_k_vcs_mf8<TV, R, 16>), gv, bv, ldsMf, c->stream, va);
+                }
             } else if (vcs_lds) {
                 const size_t ldsYp = sizeof(double) * (size_t)c->sumPv * (R + LB_VPAD);
                 hipLaunchKernelGGL((lb_k_vcs_pv<TV, R>), dim3(vgrid), dim3(LB_BLOCK), ldsYp, c->stream, va);
@@ -3804,6 +3817,11 @@ int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
     if (std::strcmp(name, "row_order") == 0) { c->use_order = value != 0; return 0; }
     if (std::strcmp(name, "overlap") == 0) { c->overlap = value != 0; return 0; }
     if (std::strcmp(name, "sweep_graph") == 0) { c->sweep_graph = value != 0; return 0; }
+    if (std::strcmp(name, "vcs_tile") == 0) {
+        if (value != 16 && value != 32) return fail(c, -1, "lb_set_option: vcs_tile is 16 or 32");
+        c->vcs_tile = (int)value;
+        return 0;
+    }
     if (std::strcmp(name, "vcs_grid") == 0) {
         if (value < 0 || value > LB_VCS_BLOCKS) return fail(c, -1, "lb_set_option: vcs_grid must be in [0, 1536]");
         c->vcs_grid = value;
