@@ -961,7 +961,7 @@ __device__ inline int find_row(const int32_t *rows, int cnt, int64_t i) {
 // arrays are staged in LDS. Neighbour gathers are issued 8 at a time and triangle records 4 at a time (all row loads
 // of a group in flight before any is used), so each slot keeps several independent gathers outstanding.
 template <typename TV, int R>
-__global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
+__device__ __forceinline__ void rows_body(const RowArgs &a) {
     typedef typename VT<TV>::TC TC;
     constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSL = LB_BLOCK / S;
     extern __shared__ double sm[];
@@ -1132,6 +1132,17 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
         double *o = a.part + 4 * (int64_t)blockIdx.x;
         o[0] = quad; o[1] = gv; o[2] = tot; o[3] = gmin;
     }
+}
+
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_rows(RowArgs a) {
+    rows_body<TV, R>(a);
+}
+// BF16 V: eight columns per lane take 169 registers, one step over three wavefronts per SIMD; capped at 168 (a
+// 12-byte spill) the row kernel runs 271 against 331 us on the 1M-row graph. FP64 V loses from the same cap.
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void lb_k_rows_w3(RowArgs a) {
+    rows_body<TV, R>(a);
 }
 
 // ------------------------------------------------------------------------------------------------ lb_k_finalize
@@ -3177,7 +3188,10 @@ struct Launch {
             const size_t stage_bytes = sizeof(TC) * ((size_t)R + (size_t)c->sumPpair * R + c->sumPpair + 2 * (size_t)c->sumPv * R);
             ra.stage = stage_bytes <= 48 * 1024;
             ra.order = c->use_order ? (const int32_t *)c->rorder.p : nullptr;
-            hipLaunchKernelGGL((lb_k_rows<TV, R>), dim3((unsigned)nrow_blk), dim3(LB_BLOCK), ra.stage ? stage_bytes : 0, c->stream, ra);
+            if (c->vdtype == LB_BF16)
+                hipLaunchKernelGGL((lb_k_rows_w3<TV, R>), dim3((unsigned)nrow_blk), dim3(LB_BLOCK), ra.stage ? stage_bytes : 0, c->stream, ra);
+            else
+                hipLaunchKernelGGL((lb_k_rows<TV, R>), dim3((unsigned)nrow_blk), dim3(LB_BLOCK), ra.stage ? stage_bytes : 0, c->stream, ra);
         }
         mark();
         // --- scalars
