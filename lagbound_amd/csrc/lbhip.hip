@@ -5584,7 +5584,10 @@ int lb_cert_core(int device, int64_t m, int64_t na, const int32_t *ai, const int
     // method 0: the library factorization while m^2 < 2^31, the blocked kernels above; 1: library; 2: blocked kernels
     // (vector FMA trailing updates); 3: blocked kernels with the trailing updates on the matrix cores (one wave per
     // SIMD at 128 accumulator registers). At m = 90,000: library 6.5 s, method 2 7.2 s, method 3 9.1 s, the same
-    // factor sum; the library's 64-bit dpotrf is still not used by default above 2^31 entries (see lbhip.h)
+    // factor sum; the library's 64-bit dpotrf is still not used by default above 2^31 entries (see lbhip.h). The
+    // blocked kernels' panels with rocBLAS dsyrk_64 as the trailing update ran 22-25% faster at m = 90,000-120,000,
+    // but at m = 131,287 reported a non-positive leading minor (850) on a core that method 2 factors -- the size at
+    // which the library's dpotrf_64 fails too -- so the trailing updates stay in this library's own kernels.
     const bool own = method >= 2 || (method == 0 && m * m >= ((int64_t)1 << 31));
     const SolverApi &api = solver_api();
     if (!own && (!api.ok || (!api.potrf64 && m * m >= ((int64_t)1 << 31)))) { hipSetDevice(prev); return -3; }
