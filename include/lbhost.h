@@ -1,0 +1,34 @@
+// liblbhost: host bookkeeping of the search between device calls (C ABI). The reference does these steps with numpy
+// inside its main loop (alm_vcs.py:1735-1758 for the triangle pool, vcsops.py:439-464 `update_ws` / `_merge` for the
+// VCS working sets); lagbound_amd.alm calls this library for them. Plain C, 64-bit sizes, caller-owned arrays.
+#ifndef LBHOST_H
+#define LBHOST_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Working-set merge (update_ws, vcsops.py:439-464): the kept entries of a CSR working set (n rows, entries ptr/col,
+// keep[e] != 0) and m new entries (ni[k], nR[k]) as one CSR sorted by the key i P + R, in the order of a stable sort
+// of [kept entries in storage order, new entries in the given order]. Outputs ptr_out (n + 1), col_out and src_out
+// (tot = kept + m entries each): src_out[e] = the old entry index of a kept entry, -1 for a new one. Returns tot, or
+// -1 on bad arguments (a row or cluster out of range).
+int64_t lbh_ws_merge(int64_t n, int64_t P, const int64_t *ptr, const int32_t *col, const uint8_t *keep, int64_t m,
+                     const int64_t *ni, const int64_t *nR, int64_t *ptr_out, int32_t *col_out, int64_t *src_out);
+
+// out[e] = a[src[e]] for src[e] >= 0, 0 for src[e] < 0 (FP64 and byte values). Returns 0, or -1 if src reaches past
+// na.
+int lbh_take_f64(int64_t cnt, const int64_t *src, const double *a, int64_t na, double *out);
+int lbh_take_u8(int64_t cnt, const int64_t *src, const uint8_t *a, int64_t na, uint8_t *out);
+
+// Triangle candidates not in the pool (alm_vcs.py:1751-1758): the packed key (a n + b) n + c of every candidate
+// (rows of cand, m x 3) and of every pool triangle (rows of pool, T x 3), in int64 arithmetic with its wrap-around;
+// writes to idx_out the index of the first occurrence of every distinct candidate key absent from the pool, in
+// ascending key order. Returns the number written.
+int64_t lbh_new_triangles(int64_t n, int64_t m, const int64_t *cand, int64_t T, const int64_t *pool, int64_t *idx_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

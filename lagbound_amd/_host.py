@@ -1,0 +1,82 @@
+"""ctypes binding of liblbhost.so (include/lbhost.h): the search's host bookkeeping between device calls (working-set
+merges, triangle-pool filtering). Raises if the library has not been built."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liblbhost.so")
+EXPORTS = ["lbh_ws_merge", "lbh_take_f64", "lbh_take_u8", "lbh_new_triangles"]
+
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+_f64p = C.POINTER(C.c_double)
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is missing: build it with `python -m lagbound_amd.build`")
+        L = C.CDLL(LIBPATH)
+        L.lbh_ws_merge.argtypes = [C.c_int64, C.c_int64, _i64p, _i32p, _u8p, C.c_int64, _i64p, _i64p, _i64p, _i32p,
+                                   _i64p]
+        L.lbh_ws_merge.restype = C.c_int64
+        L.lbh_take_f64.argtypes = [C.c_int64, _i64p, _f64p, C.c_int64, _f64p]
+        L.lbh_take_u8.argtypes = [C.c_int64, _i64p, _u8p, C.c_int64, _u8p]
+        L.lbh_new_triangles.argtypes = [C.c_int64, C.c_int64, _i64p, C.c_int64, _i64p, _i64p]
+        L.lbh_new_triangles.restype = C.c_int64
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def ws_merge(n, P, ptr, col, keep, ni, nR):
+    """(ptr, col, src) of the merged working set (lbh_ws_merge)."""
+    ptr = np.ascontiguousarray(ptr, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    keep = np.ascontiguousarray(keep, np.uint8)
+    ni = np.ascontiguousarray(ni, np.int64)
+    nR = np.ascontiguousarray(nR, np.int64)
+    tot = int(keep.sum()) + len(ni)
+    po = np.empty(n + 1, np.int64)
+    co = np.empty(tot, np.int32)
+    so = np.empty(tot, np.int64)
+    w = load().lbh_ws_merge(int(n), int(P), _p(ptr, _i64p), _p(col, _i32p), _p(keep, _u8p), len(ni), _p(ni, _i64p),
+                            _p(nR, _i64p), _p(po, _i64p), _p(co, _i32p), _p(so, _i64p))
+    if w != tot:
+        raise ValueError("lbh_ws_merge: a row or cluster out of range")
+    return po, co, so
+
+
+def take(a, src, dtype):
+    """out[j] = a[src[j]] for src[j] >= 0, 0 for a new entry (lbh_take_f64 / lbh_take_u8)."""
+    src = np.ascontiguousarray(src, np.int64)
+    a = np.ascontiguousarray(a, dtype)
+    out = np.empty(len(src), dtype)
+    if dtype == np.uint8:
+        rc = load().lbh_take_u8(len(src), _p(src, _i64p), _p(a, _u8p), len(a), _p(out, _u8p))
+    else:
+        rc = load().lbh_take_f64(len(src), _p(src, _i64p), _p(a, _f64p), len(a), _p(out, _f64p))
+    if rc:
+        raise ValueError("lbh_take: an index past the source")
+    return out
+
+
+def new_triangles(n, cand, pool):
+    """Indices of the first occurrence of every distinct candidate triangle absent from the pool, ascending key."""
+    cand = np.ascontiguousarray(cand, np.int64).reshape(-1, 3)
+    pool = np.ascontiguousarray(pool, np.int64).reshape(-1, 3)
+    out = np.empty(len(cand), np.int64)
+    w = load().lbh_new_triangles(int(n), len(cand), _p(cand, _i64p), len(pool), _p(pool, _i64p), _p(out, _i64p))
+    if w < 0:
+        raise ValueError("lbh_new_triangles: bad arguments")
+    return out[:w]

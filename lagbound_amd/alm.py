@@ -27,6 +27,7 @@ import time
 
 import numpy as np
 
+from . import _host as H
 from . import instances as I
 from .engine import Engine
 
@@ -455,62 +456,23 @@ class Solver:
 
 def _new_triangles(new, val, TI, n):
     """Candidates that are not in the pool, one per packed key (its first occurrence), in ascending key order
-    (alm_vcs.py:1751-1758: drop existing, dedupe with np.unique). The distinct candidates are found first and then
-    searched in the sorted pool keys: the same set and order as filtering first, with a monotone search."""
-    kn = (new[:, 0] * n + new[:, 1]) * n + new[:, 2]
-    uk, fi = np.unique(kn, return_index=True)
-    ex = np.sort((TI[:, 0] * n + TI[:, 1]) * n + TI[:, 2])
-    pos = np.searchsorted(ex, uk)
-    isnew = np.ones(len(uk), bool)
-    ok = pos < len(ex)
-    isnew[ok] = ex[pos[ok]] != uk[ok]
-    fi = fi[isnew]
+    (alm_vcs.py:1751-1758: drop existing, dedupe with np.unique), through liblbhost (lbh_new_triangles: the distinct
+    candidates found by a sort and checked against the sorted pool keys in one merge pass; the same set and order as
+    filtering first)."""
+    fi = H.new_triangles(n, new, TI)
     return new[fi], val[fi]
 
 
 def _take_kept(a, src, dtype):
     """out[j] = a[src[j]] where src[j] >= 0 (a kept entry), 0 for a new entry; `a` may be empty."""
-    out = np.zeros(len(src), dtype)
-    k = src >= 0
-    out[k] = a[src[k]]
-    return out
+    return H.take(a, src, dtype)
 
 
 def _merge_ws(vl, keep, ni, nR):
-    """update_ws / _merge (vcsops.py:439-464): kept old entries and new (i, R) entries, CSR sorted by (i, R).
-    Returns (ptr, col, src) with src = index of the old entry or -1 for a new one."""
-    n = len(vl.lab)
-    P = np.int64(vl.P)
-    rows_old = np.repeat(np.arange(n), np.diff(vl.ptr))
-    ko = np.flatnonzero(keep)
-    ro, co = rows_old[ko], vl.col[ko].astype(np.int64)      # kept entries: in (i, R) order as this function writes them
-    rn, cn = np.asarray(ni, np.int64), np.asarray(nR, np.int64)
-    ko_key = ro * P + co
-    if len(ko_key) > 1 and not np.all(ko_key[1:] >= ko_key[:-1]):   # a working set from elsewhere: sort everything
-        o = np.argsort(np.r_[ko_key, rn * P + cn], kind="stable")
-        c, src = np.r_[co, cn][o], np.r_[ko, np.full(len(rn), -1)][o]
-        ptr = np.zeros(n + 1, np.int64)
-        np.cumsum(np.bincount(np.r_[ro, rn], minlength=n), out=ptr[1:])
-        return ptr, c.astype(np.int32), src
-    kn = rn * P + cn
-    m = len(kn)
-    if m and int(kn.max()) < (1 << 40) and m < (1 << 23):    # only the new entries are sorted: (key, position)
-        on = np.sort((kn << 23) | np.arange(m, dtype=np.int64)) & ((1 << 23) - 1)   # packed, so an unstable sort
-    else:                                                     # gives the stable order
-        on = np.argsort(kn, kind="stable")
-    rn, cn = rn[on], cn[on]
-    # merge by the key i P + R, kept entries first on equal keys (the order of a stable sort of [kept, new])
-    at = np.searchsorted(ko_key, rn * P + cn, side="right") + np.arange(len(rn))
-    tot = len(ro) + len(rn)
-    isold = np.ones(tot, bool)
-    isold[at] = False
-    c = np.empty(tot, np.int64)
-    src = np.empty(tot, np.int64)
-    c[isold], src[isold] = co, ko
-    c[at], src[at] = cn, -1
-    ptr = np.zeros(n + 1, np.int64)
-    np.cumsum(np.bincount(np.r_[ro, rn], minlength=n), out=ptr[1:])
-    return ptr, c.astype(np.int32), src
+    """update_ws / _merge (vcsops.py:439-464): kept old entries and new (i, R) entries, CSR sorted by (i, R), in the
+    order of a stable sort of [kept, new] (liblbhost lbh_ws_merge). Returns (ptr, col, src) with src = index of the
+    old entry or -1 for a new one."""
+    return H.ws_merge(len(vl.lab), vl.P, vl.ptr, vl.col, keep, ni, nR)
 
 
 def load_instance(D):

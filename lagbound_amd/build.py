@@ -32,6 +32,8 @@ CERT_SRC = os.path.join(HERE, "csrc", "lbcert.cpp")
 CERT_OUT = os.path.join(HERE, "liblbcert.so")
 VERIFY_SRC = os.path.join(HERE, "csrc", "lbverify.cpp")
 VERIFY_OUT = os.path.join(HERE, "liblbverify.so")
+HOST_SRC = os.path.join(HERE, "csrc", "lbhost.cpp")
+HOST_OUT = os.path.join(HERE, "liblbhost.so")
 
 
 def _compile_host(src, out, verbose):
@@ -47,12 +49,14 @@ def _compile_host(src, out, verbose):
 
 def build(verbose: bool = False) -> str:
     """Compile every native source of the package: liblbhip.so (HIP kernels + C ABI), the bandwidth probe
-    liblbprobe.so used by the benchmark's traffic calibration, the host part of the sparse certificate liblbcert.so
-    and the independent verifier's factorization liblbverify.so; always recompiles."""
+    liblbprobe.so used by the benchmark's traffic calibration, the host part of the sparse certificate liblbcert.so,
+    the independent verifier's factorization liblbverify.so and the search's host bookkeeping liblbhost.so; always
+    recompiles."""
     _compile(SRC, OUT, verbose, libs=["-ldl"])
     _compile(PROBE_SRC, PROBE_OUT, verbose)
     _compile_host(CERT_SRC, CERT_OUT, verbose)
     _compile_host(VERIFY_SRC, VERIFY_OUT, verbose)
+    _compile_host(HOST_SRC, HOST_OUT, verbose)
     return OUT
 
 

@@ -1,0 +1,105 @@
+// liblbhost: host bookkeeping of the search (include/lbhost.h). Each function has one pass over its inputs plus
+// sorts of the new items only, in place of numpy's chains of temporaries.
+#include "../../include/lbhost.h"
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+extern "C" {
+
+int64_t lbh_ws_merge(int64_t n, int64_t P, const int64_t *ptr, const int32_t *col, const uint8_t *keep, int64_t m,
+                     const int64_t *ni, const int64_t *nR, int64_t *ptr_out, int32_t *col_out, int64_t *src_out) {
+    if (n < 0 || P < 1 || m < 0 || !ptr || !ptr_out || (m && (!ni || !nR))) return -1;
+    for (int64_t k = 0; k < m; k++)
+        if (ni[k] < 0 || ni[k] >= n || nR[k] < 0 || nR[k] >= P) return -1;
+    // per row: kept count, then the new entries bucketed by row in their given order (a counting sort: stable)
+    std::vector<int64_t> cnt_new((size_t)n + 1, 0);
+    for (int64_t k = 0; k < m; k++) cnt_new[(size_t)ni[k] + 1]++;
+    for (int64_t i = 0; i < n; i++) cnt_new[(size_t)i + 1] += cnt_new[(size_t)i];
+    std::vector<int64_t> newby((size_t)m);
+    {
+        std::vector<int64_t> at(cnt_new.begin(), cnt_new.end() - 1);
+        for (int64_t k = 0; k < m; k++) newby[(size_t)at[(size_t)ni[k]]++] = k;
+    }
+    ptr_out[0] = 0;
+    for (int64_t i = 0; i < n; i++) {
+        int64_t kept = 0;
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) kept += keep[e] ? 1 : 0;
+        ptr_out[i + 1] = ptr_out[i] + kept + (cnt_new[(size_t)i + 1] - cnt_new[(size_t)i]);
+    }
+    // per row: [kept in storage order, new in given order], stably sorted by R, in place in the output (an insertion
+    // sort: rows hold a few entries each, and the kept ones are already in order)
+    for (int64_t i = 0; i < n; i++) {
+        int64_t o = ptr_out[i];
+        const int64_t o0 = o;
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; e++)
+            if (keep[e]) {
+                if (col[e] < 0 || col[e] >= P) return -1;
+                col_out[o] = col[e];
+                src_out[o++] = e;
+            }
+        for (int64_t q = cnt_new[(size_t)i]; q < cnt_new[(size_t)i + 1]; q++) {
+            const int64_t k = newby[(size_t)q];
+            col_out[o] = (int32_t)nR[k];
+            src_out[o++] = -1;
+        }
+        for (int64_t a = o0 + 1; a < o; a++) {
+            const int32_t c = col_out[a];
+            const int64_t sr = src_out[a];
+            int64_t b = a;
+            while (b > o0 && col_out[b - 1] > c) {      // strictly greater: equal keys keep their order (stable)
+                col_out[b] = col_out[b - 1];
+                src_out[b] = src_out[b - 1];
+                b--;
+            }
+            col_out[b] = c;
+            src_out[b] = sr;
+        }
+    }
+    return ptr_out[n];
+}
+
+int lbh_take_f64(int64_t cnt, const int64_t *src, const double *a, int64_t na, double *out) {
+    for (int64_t e = 0; e < cnt; e++) {
+        if (src[e] >= na) return -1;
+        out[e] = src[e] >= 0 ? a[src[e]] : 0.0;
+    }
+    return 0;
+}
+
+int lbh_take_u8(int64_t cnt, const int64_t *src, const uint8_t *a, int64_t na, uint8_t *out) {
+    for (int64_t e = 0; e < cnt; e++) {
+        if (src[e] >= na) return -1;
+        out[e] = src[e] >= 0 ? a[src[e]] : 0;
+    }
+    return 0;
+}
+
+int64_t lbh_new_triangles(int64_t n, int64_t m, const int64_t *cand, int64_t T, const int64_t *pool, int64_t *idx_out) {
+    if (n < 0 || m < 0 || T < 0) return -1;
+    // the packed key in int64 arithmetic with its wrap-around (computed in uint64, read back as int64)
+    auto key = [n](const int64_t *t) -> int64_t {
+        const uint64_t un = (uint64_t)n;
+        return (int64_t)(((uint64_t)t[0] * un + (uint64_t)t[1]) * un + (uint64_t)t[2]);
+    };
+    std::vector<std::pair<int64_t, int64_t>> kc((size_t)m);
+    for (int64_t k = 0; k < m; k++) kc[(size_t)k] = {key(cand + 3 * k), k};
+    std::sort(kc.begin(), kc.end());                 // by key, then index: the first occurrence leads each run
+    std::vector<int64_t> kp((size_t)T);
+    for (int64_t t = 0; t < T; t++) kp[(size_t)t] = key(pool + 3 * t);
+    std::sort(kp.begin(), kp.end());
+    int64_t w = 0;
+    size_t p = 0;
+    for (size_t a = 0; a < kc.size();) {
+        size_t b = a + 1;
+        while (b < kc.size() && kc[b].first == kc[a].first) b++;
+        while (p < kp.size() && kp[p] < kc[a].first) p++;   // both ascending: one merge pass
+        if (!(p < kp.size() && kp[p] == kc[a].first)) idx_out[w++] = kc[a].second;
+        a = b;
+    }
+    return w;
+}
+
+}  // extern "C"

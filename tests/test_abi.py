@@ -75,3 +75,16 @@ def test_verifier_library_exports_every_symbol():
     lib = ctypes.CDLL(verify.VERIFYLIB)
     for name in names:
         assert hasattr(lib, name), name
+
+
+def test_host_bookkeeping_library_exports_every_symbol():
+    """liblbhost.so (the search's host bookkeeping) exports every entry point include/lbhost.h declares, and the
+    binding lists the same names."""
+    from lagbound_amd import _host
+    src = open(os.path.join(ROOT, "include", "lbhost.h")).read()
+    src = re.sub(r"//[^\n]*", "", src)
+    names = sorted(set(re.findall(r"\b(lbh_[a-z_A-Z0-9]+)\s*\(", src)))
+    assert names == sorted(_host.EXPORTS)
+    lib = ctypes.CDLL(_host.LIBPATH)
+    for name in names:
+        assert hasattr(lib, name), name

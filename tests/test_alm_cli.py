@@ -126,3 +126,23 @@ def test_new_triangles_matches_filter_then_unique():
         k2, v2, n2 = kn[keep], val[keep], new[keep]
         _, f = np.unique(k2, return_index=True)
         assert np.array_equal(got_new, n2[f]) and np.array_equal(got_val, v2[f]), t
+
+
+def test_new_triangles_keeps_the_int64_wraparound():
+    """Above n = 2^21 the packed key (a n + b) n + c overflows int64; the pool filter keeps numpy's wrapped keys (the
+    reference's key, alm_vcs.py:1751-1752), so its set and order equal a numpy restatement with the same keys."""
+    from lagbound_amd.alm import _new_triangles
+    rng = np.random.default_rng(5)
+    n = 3_000_000
+    new = rng.integers(0, n, (500, 3))
+    new[100:150] = new[:50]                              # repeats
+    TI = rng.integers(0, n, (300, 3))
+    TI[:40] = new[200:240]                              # pool hits
+    val = rng.random(500)
+    got_new, got_val = _new_triangles(new, val, TI, n)
+    with np.errstate(over="ignore"):
+        kn = (new[:, 0] * n + new[:, 1]) * n + new[:, 2]
+        kp = (TI[:, 0] * n + TI[:, 1]) * n + TI[:, 2]
+    uk, fi = np.unique(kn, return_index=True)
+    keep = ~np.isin(uk, kp)
+    assert np.array_equal(got_new, new[fi[keep]]) and np.array_equal(got_val, val[fi[keep]])
