@@ -73,6 +73,7 @@
 #define LB_VCS_BLOCKS 1536   // VCS pass workgroups: 1536 measured best of 512 / 1024 / 1536 / 2048 on the 1M-row graph
 #endif
 #define LB_BLOCK 256
+#define LB_ZPART 32        // fused sweeps: copies of z the rows of a batch add into (a power of two)
 
 typedef uint16_t bf16raw;
 
@@ -1303,8 +1304,14 @@ struct PairRowArgs {
 
 // pair_all for the clusters the batch touches: pp row from the current Y, M_Q = sum_R pp_QR Y_R, pdiag_Q = pp_QQ
 template <int R>
+__device__ inline void pair_row_one(const PairRowArgs &a, int Qg, int lane);
+template <int R>
 __global__ __launch_bounds__(64) void lb_k_pair_rows(PairRowArgs a) {
-    const int Qg = blockIdx.x;
+    pair_row_one<R>(a, (int)blockIdx.x, (int)threadIdx.x);
+}
+// one wavefront per cluster (lane = the wavefront's lane)
+template <int R>
+__device__ inline void pair_row_one(const PairRowArgs &a, int Qg, int lane) {
     if (!a.tmask[Qg]) return;
     int l = 0;
     while (l + 1 < a.nlev && Qg >= a.Poff[l + 1]) l++;
@@ -1315,7 +1322,7 @@ __global__ __launch_bounds__(64) void lb_k_pair_rows(PairRowArgs a) {
 #pragma unroll
     for (int c = 0; c < R; c++) Macc[c] = 0.0;
     double pq = 0.0;
-    for (int Rr = threadIdx.x; Rr < P; Rr += 64) {
+    for (int Rr = lane; Rr < P; Rr += 64) {
         const double *YR = a.Y + (int64_t)(a.Poff[l] + Rr) * R;
         double p = sig[Rr];
         if (a.rho > 0.0) {
@@ -1335,7 +1342,7 @@ __global__ __launch_bounds__(64) void lb_k_pair_rows(PairRowArgs a) {
     pq = wave_sum(pq);
 #pragma unroll
     for (int c = 0; c < R; c++) Macc[c] = wave_sum(Macc[c]);
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
 #pragma unroll
         for (int c = 0; c < R; c++) a.M[(int64_t)Qg * R + c] = Macc[c];
         a.pdiag[Qg] = pq;
@@ -1450,6 +1457,11 @@ struct StepArgs {
     const void *segout; const double *segkap;
     const int32_t *rows; int64_t nb;            // the rows of this batch
     uint4 *Vstage; double *DD;                  // batch-position indexed
+    // fused sweeps (lb_k_step_fused): zYn = [z | Y] after this batch, a copy of [z | Y] before it into which every
+    // row adds its increment (FP64 atomics; DD is then not written); per hub row h (edge hubs by their hE index, the
+    // other triangle hubs at nhE + their hT index): arrival count, batch position, own terms g[R] and kap
+    double *zYn; int32_t *hubcnt; int64_t *hubpos; double *hubown;
+    double *zpart;     // z increments spread over LB_ZPART copies of R doubles (by batch position), folded into z later
 };
 
 // one triangle record of the row step: tau (canonical order), q, pt = lam - rt (1 - tau); if pt > 0 the record adds
@@ -1783,6 +1795,23 @@ __device__ inline void step_finish_wave(const StepArgs &a, const typename VT<TV>
     a.Vstage[pos * S + q] = u;
     TC xs[CPL];
     VT<TV>::unpack(u, xs);
+    if (a.zYn) {
+        // the increment straight into the next [z | Y]: the row's cluster of every pair and VCS level, and z through
+        // one of LB_ZPART copies (every row of a batch adds to z: one address would serialise them)
+        double *zy = a.zYn;
+        double *zp = a.zpart + (int64_t)(pos & (LB_ZPART - 1)) * R + q * CPL;
+        double *yl[LB_MAXLEV + LB_MAXVCS];
+        int ns = 0;
+        for (int l = 0; l < a.nlev; l++) yl[ns++] = zy + R + (int64_t)(a.Poff[l] + a.lab[l][i]) * R + q * CPL;
+        for (int m = 0; m < a.nvcs; m++) yl[ns++] = zy + R + (int64_t)(a.Yoff[m] + a.vlab[m][i]) * R + q * CPL;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+            const double dv = di * ((double)xs[c] - (double)x[c]);
+            atomicAdd(zp + c, dv);
+            for (int t = 0; t < ns; t++) atomicAdd(yl[t] + c, dv);
+        }
+        return;
+    }
     double *dd = a.DD + pos * R + q * CPL;
 #pragma unroll
     for (int c = 0; c < CPL; c++) dd[c] = di * ((double)xs[c] - (double)x[c]);
@@ -1873,22 +1902,18 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step(StepArgs a) {
 // row's edges and triangle records are dealt to its 64 / S slots, so all their gathers are in flight at once, and the
 // slot partials are summed by a butterfly over the slots. Every slot then holds the same g and curvature and runs
 // the rest of the step (the slot-0 lanes write). Hub rows take their segment sums as in lb_k_step.
+// The row's own terms of the wavefront-per-row step: its edges unless it is an edge hub, its triangle records unless
+// it is a triangle hub, summed over the slots (every slot ends with the totals in g and kap).
 template <typename TV, int R>
-__global__ __launch_bounds__(LB_BLOCK) void lb_k_step_wave(StepArgs a) {
+__device__ inline void step_row_own(const StepArgs &a, int64_t i, int q, int ws, const typename VT<TV>::TC *x,
+                                    typename VT<TV>::TC *g, double &kap, bool &hubE, bool &hubT) {
     typedef typename VT<TV>::TC TC;
     constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSW = 64 / S;
-    const int lane = threadIdx.x & 63, ws = lane / S, q = lane % S;
-    const int64_t pos = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
-    if (pos >= a.nb) return;                           // wavefront-uniform
-    const int64_t i = a.rows[pos];
-    TC x[CPL], g[CPL];
-    VT<TV>::unpack(a.V[i * S + q], x);
 #pragma unroll
     for (int c = 0; c < CPL; c++) g[c] = 0;
-    double kap = 0.0;
-    const TC *so = (const TC *)a.segout;
+    kap = 0.0;
     const int64_t e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
-    const bool hubE = e1 - e0 > a.heavyE;
+    hubE = e1 - e0 > a.heavyE;
     if (!hubE) {
         // two edges per slot in flight; items past the row read row i with weight 0 (an exact zero)
         const TC *wt = (const TC *)a.w;
@@ -1915,7 +1940,7 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step_wave(StepArgs a) {
         }
     }
     const int64_t r0 = a.iptr[i], r1 = a.iptr[i + 1];
-    const bool hubT = r1 - r0 > a.heavyT;
+    hubT = r1 - r0 > a.heavyT;
     if (!hubT) {
         // two records per slot in flight (their four rows gathered together)
         const TriRec<TC> *rec = (const TriRec<TC> *)a.rec;
@@ -1947,6 +1972,15 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step_wave(StepArgs a) {
         g[c] = t;
     }
     for (int m = S; m < 64; m <<= 1) kap += __shfl_xor(kap, m);
+}
+
+// a hub row's segment sums, added in segment order: edge segments, then triangle segments
+template <typename TV, int R>
+__device__ inline void step_add_segs(const StepArgs &a, int64_t i, int q, bool hubE, bool hubT, typename VT<TV>::TC *g,
+                                     double &kap) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL;
+    const TC *so = (const TC *)a.segout;
     if (hubE) {
         const int k = a.hE_idx[i];
         for (int s = a.hE_seg0[k]; s < a.hE_seg0[k + 1]; s++) {
@@ -1962,7 +1996,110 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step_wave(StepArgs a) {
             kap += a.segkap[s];
         }
     }
+}
+
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_step_wave(StepArgs a) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL;
+    const int lane = threadIdx.x & 63, ws = lane / S, q = lane % S;
+    const int64_t pos = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
+    if (pos >= a.nb) return;                           // wavefront-uniform
+    const int64_t i = a.rows[pos];
+    TC x[CPL], g[CPL];
+    VT<TV>::unpack(a.V[i * S + q], x);
+    double kap;
+    bool hubE, hubT;
+    step_row_own<TV, R>(a, i, q, ws, x, g, kap, hubE, hubT);
+    step_add_segs<TV, R>(a, i, q, hubE, hubT, g, kap);
     step_finish_wave<TV, R>(a, x, i, pos, q, ws, g, kap);
+}
+
+// Fused sweeps, hub rows: the row's own wavefront and the wavefronts that computed its segments arrive with the
+// number of parts they hold (a release fence -- an L2 write-back, no invalidation -- then one atomic add); the last
+// to arrive takes an acquire fence, adds the parts in the fixed order of lb_k_step_wave -- own terms, edge segments,
+// triangle segments -- finishes the row, and clears the count for the next batch. No wavefront waits for another, so
+// the kernel needs no co-residency.
+template <typename TV, int R>
+__device__ inline void hub_arrive(const StepArgs &a, int64_t i, int parts, int lane, int q, int ws) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL;
+    const int ke = a.hE_idx[i], kt = a.hT_idx[i];
+    const int h = ke >= 0 ? ke : a.nhE + kt;
+    const int total = 1 + (ke >= 0 ? a.hE_seg0[ke + 1] - a.hE_seg0[ke] : 0) +
+                      (kt >= 0 ? a.hT_seg0[kt + 1] - a.hT_seg0[kt] : 0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    int last = 0;
+    if (lane == 0) last = atomicAdd(a.hubcnt + h, parts) + parts == total ? 1 : 0;
+    last = __shfl(last, 0);
+    if (!last) return;                                   // wavefront-uniform
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int64_t pos = a.hubpos[h];
+    const double *ow = a.hubown + (int64_t)h * (R + 1);
+    TC x[CPL], g[CPL];
+    VT<TV>::unpack(a.V[i * S + q], x);
+#pragma unroll
+    for (int c = 0; c < CPL; c++) g[c] = (TC)ow[q * CPL + c];
+    double kap = ow[R];
+    step_add_segs<TV, R>(a, i, q, ke >= 0, kt >= 0, g, kap);
+    if (lane == 0) a.hubcnt[h] = 0;
+    step_finish_wave<TV, R>(a, x, i, pos, q, ws, g, kap);
+}
+
+// One launch per batch for the row steps and the hub segments: workgroups [0, nrow_blk) take four batch rows each
+// (one wavefront per row, as lb_k_step_wave), the rest stride over the batch's hub segments (as lb_k_step_segs).
+// Light rows finish at once; a hub row's own wavefront stores its own terms and position, and every part arrives
+// through hub_arrive. Row increments go into a.zYn (no DD, no separate aggregate pass).
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_step_fused(StepArgs a, StepSegArgs ga, int nrow_blk) {
+    typedef typename VT<TV>::TC TC;
+    constexpr int CPL = VT<TV>::CPL, S = R / CPL;
+    const int lane = threadIdx.x & 63, ws = lane / S, q = lane % S;
+    if ((int)blockIdx.x < nrow_blk) {
+        const int64_t pos = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
+        if (pos >= a.nb) return;                       // wavefront-uniform
+        const int64_t i = a.rows[pos];
+        TC x[CPL], g[CPL];
+        VT<TV>::unpack(a.V[i * S + q], x);
+        double kap;
+        bool hubE, hubT;
+        step_row_own<TV, R>(a, i, q, ws, x, g, kap, hubE, hubT);
+        if (!hubE && !hubT) {
+            step_finish_wave<TV, R>(a, x, i, pos, q, ws, g, kap);
+            return;
+        }
+        const int ke = a.hE_idx[i];
+        const int h = ke >= 0 ? ke : a.nhE + a.hT_idx[i];
+        double *ow = a.hubown + (int64_t)h * (R + 1);
+        if (ws == 0) {
+#pragma unroll
+            for (int c = 0; c < CPL; c++) ow[q * CPL + c] = (double)g[c];
+        }
+        if (lane == 0) { ow[R] = kap; a.hubpos[h] = pos; }
+        hub_arrive<TV, R>(a, i, 1, lane, q, ws);
+        return;
+    }
+    // each segment wavefront takes a contiguous share of the batch's list (a hub's segments are adjacent there) and
+    // arrives once per hub row it worked for
+    const int64_t l0 = ga.lpos[ga.b], l1 = ga.lpos[ga.b + 1];
+    const int64_t nsw = (int64_t)(gridDim.x - nrow_blk) * (LB_BLOCK / 64);
+    const int64_t per = (l1 - l0 + nsw - 1) / nsw;
+    const int64_t wv = (int64_t)(blockIdx.x - nrow_blk) * (LB_BLOCK / 64) + (threadIdx.x >> 6);
+    const int64_t lb = l0 + wv * per, le = lb + per < l1 ? lb + per : l1;
+    int64_t row = -1;
+    int parts = 0;
+    for (int64_t li = lb; li < le; li++) {
+        const int32_t sid = ga.list[li];
+        const int64_t rs = ga.segs[sid].row;
+        if (rs != row && parts) {
+            hub_arrive<TV, R>(a, row, parts, lane, q, ws);
+            parts = 0;
+        }
+        row = rs;
+        step_seg_one<TV, R>(ga, sid);
+        parts++;
+    }
+    if (parts) hub_arrive<TV, R>(a, row, parts, lane, q, ws);
 }
 
 struct PostArgs {
@@ -2100,12 +2237,19 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_vcs(Post2Args a) {
 // one entry per slot at a time, with the same arithmetic and the same LDS staging of the Z / count increments as
 // lb_k_post_vcs. The slot-0 lanes write the new V row after every slot has read the old one.
 template <typename TV, int R>
+__device__ inline void post_vcs_wave_blk(const Post2Args &a, int blk);
+template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_vcs_wave(Post2Args a) {
+    post_vcs_wave_blk<TV, R>(a, (int)blockIdx.x);
+}
+// workgroup blk of lb_k_post_vcs_wave (every thread of the workgroup calls it: it synchronises the workgroup)
+template <typename TV, int R>
+__device__ inline void post_vcs_wave_blk(const Post2Args &a, int blk) {
     typedef typename VT<TV>::TC TC;
     constexpr int CPL = VT<TV>::CPL, S = R / CPL, NSW = 64 / S;
     extern __shared__ double shZc[];
     const int lane = threadIdx.x & 63, ws = lane / S, q = lane % S;
-    const int64_t pos = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
+    const int64_t pos = (int64_t)blk * (LB_BLOCK / 64) + (threadIdx.x >> 6);
     const int wz = a.sumPv * R;
     double *Zacc = a.use_lds ? shZc : a.Z;
     double *Cacc = a.use_lds ? shZc + wz : a.cnt;
@@ -2158,6 +2302,40 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_vcs_wave(Post2Args a) {
             if (shZc[e] != 0.0) atomicAdd(a.Z + e, shZc[e]);
         for (int e = threadIdx.x; e < a.sumPv; e += LB_BLOCK)
             if (shZc[wz + e] != 0.0) atomicAdd(a.cnt + e, shZc[wz + e]);
+    }
+}
+
+// Fused sweeps, the launch before a batch's step: workgroups [0, npost) run lb_k_post_vcs_wave for the previous batch
+// (its rows' new values into V, their VCS multipliers, Z and counts at the Y after that batch), the next npair take
+// the pair-level rows of the clusters this batch touches (lb_k_pair_rows, one wavefront per cluster, from the same
+// Y), the rest complete z of that [z | Y] (zsrc) with the previous batch's LB_ZPART partial copies, clear those, and
+// copy [z | Y] (cw doubles) to zdst, the buffer this batch's steps add into (none: zdst null). The parts touch
+// disjoint memory: the first two read Y only.
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_sweep_pre(Post2Args pb, int npost, PairRowArgs pa, int npair, int nclu,
+                                                           double *zsrc, double *zdst, int64_t cw, double *zpart) {
+    const int blk = (int)blockIdx.x;
+    if (blk < npost) {
+        post_vcs_wave_blk<TV, R>(pb, blk);
+        return;
+    }
+    if (blk < npost + npair) {
+        const int Qg = (blk - npost) * (LB_BLOCK / 64) + (int)(threadIdx.x >> 6);
+        if (Qg < nclu) pair_row_one<R>(pa, Qg, (int)(threadIdx.x & 63));
+        return;
+    }
+    const int64_t nthr = (int64_t)(gridDim.x - npost - npair) * LB_BLOCK;
+    const int64_t ce = zdst ? cw : R;
+    for (int64_t e = (int64_t)(blk - npost - npair) * LB_BLOCK + threadIdx.x; e < ce; e += nthr) {
+        double v = zsrc[e];
+        if (e < R) {
+            for (int k = 0; k < LB_ZPART; k++) {
+                v += zpart[k * R + e];
+                zpart[k * R + e] = 0.0;
+            }
+            zsrc[e] = v;
+        }
+        if (zdst) zdst[e] = v;
     }
 }
 
@@ -2796,6 +2974,11 @@ struct lb_ctx {
     // since a batch is bound by its five kernels' own durations (about 33 of its 36 us), not by the launches.
     void *sw_exec = nullptr; std::vector<unsigned char> sw_key; bool sweep_graph = false;
     DBuf lposd;                                  // device copy of the sweep's per-batch segment-list bounds
+    // option "sweep_fused" 1 (default, with step_wave): two launches per batch (lb_k_sweep_pre, lb_k_step_fused) in
+    // place of five; the batches' [z | Y] alternate between Y and Yalt; hub rows meet through hubcnt / hubpos / hubown
+    bool sweep_fused = true;
+    DBuf Yalt, hubcnt, hubpos, hubown, zpart;
+    int64_t sweep_seg_grid = 0;                  // workgroups of the sweeps' hub segments (0: 16)
     double rgd_key[5] = {0, 0, 0, 0, 0};
 };
 
@@ -3299,7 +3482,8 @@ struct Launch {
         ga.list = (const int32_t *)c->seglist.p; ga.lpos = (const int64_t *)c->lposd.p;
         const int64_t nsegs = (int64_t)(w.segE.size() + w.segT.size());
         // hub segments per batch vary from sweep to sweep: a fixed grid strides over the batch's list
-        const unsigned gsegs = (unsigned)std::max<int64_t>(1, std::min<int64_t>(16, (nsegs + 3) / 4));
+        const unsigned gsegs = (unsigned)std::max<int64_t>(1, std::min<int64_t>(c->sweep_seg_grid > 0 ? c->sweep_seg_grid : 16,
+                                                                                (nsegs + 3) / 4));
         PairRowArgs pa{};
         pa.nlev = c->nlev; pa.R = R;
         for (int l = 0; l <= c->nlev; l++) { pa.Poff[l] = Poff[l]; pa.soff[l] = c->soff[l]; }
@@ -3330,6 +3514,11 @@ struct Launch {
         const size_t ldsV = sizeof(double) * (size_t)c->sumPv * (R + 1);
         pb.use_lds = c->sumPv > 0 && ldsV <= 48 * 1024;
         const int32_t *order = (const int32_t *)c->order.p;
+        const bool fused = c->sweep_fused && c->step_wave;
+        if (fused) {
+            sa.hubcnt = (int32_t *)c->hubcnt.p; sa.hubpos = (int64_t *)c->hubpos.p; sa.hubown = (double *)c->hubown.p;
+            sa.zpart = (double *)c->zpart.p;
+        }
         // the key of a captured sweep: every argument block before the per-batch fields are set, the batch bounds,
         // the kernel forms and the hub-segment grid
         std::vector<unsigned char> key;
@@ -3339,9 +3528,10 @@ struct Launch {
         };
         put(&pa, sizeof(pa)); put(&ga, sizeof(ga)); put(&sa, sizeof(sa)); put(&qa, sizeof(qa)); put(&pb, sizeof(pb));
         put(bpos.data(), sizeof(int64_t) * bpos.size());
-        const int flags[4] = {c->step_wave ? 1 : 0, (int)gsegs, (int)c->sumPpair, c->nlev};
+        const int flags[5] = {c->step_wave ? 1 : 0, (int)gsegs, (int)c->sumPpair, c->nlev, fused ? 1 : 0};
         put(flags, sizeof(flags));
         put(&c->stream, sizeof(c->stream)); put(&c->tmask.p, sizeof(void *)); put(&order, sizeof(order));
+        put(&c->Y.p, sizeof(void *)); put(&c->Yalt.p, sizeof(void *));
         auto batches = [&](PairRowArgs pa, StepSegArgs ga, StepArgs sa, PostArgs qa, Post2Args pb) {
         for (int b = 0; b < nbatch; b++) {
             const int64_t p0 = bpos[b], nb = bpos[b + 1] - bpos[b];
@@ -3371,15 +3561,50 @@ struct Launch {
                                    pb.use_lds ? ldsV : 0, c->stream, pb);
         }
         };
+        // fused: per batch b, lb_k_sweep_pre (the previous batch's post-step, this batch's pair rows, [z | Y] copied
+        // to the buffer this batch adds into), then lb_k_step_fused; the batches' [z | Y] alternate between Y and Yalt
+        auto batches_fused = [&](PairRowArgs pa, StepSegArgs ga, StepArgs sa, Post2Args pb) {
+            const int64_t cw = R + (int64_t)sumPall * R;
+            double *zb[2] = {(double *)c->Y.p, (double *)c->Yalt.p};
+            int cur = 0;
+            int64_t pprev = 0, nbprev = 0;
+            const unsigned ncopy = (unsigned)std::max<int64_t>(1, std::min<int64_t>(16, (cw + 4 * LB_BLOCK - 1) / (4 * LB_BLOCK)));
+            const int npair = c->nlev ? (c->sumPpair + LB_BLOCK / 64 - 1) / (LB_BLOCK / 64) : 0;
+            const size_t ldsPre = pb.use_lds ? ldsV : 0;
+            auto pre = [&](int npost, int np, const uint8_t *tm, double *dst) {
+                pb.rows = order + pprev; pb.nb = nbprev; pb.Y = zb[cur] + R;
+                pa.Y = zb[cur] + R; pa.tmask = tm;
+                hipLaunchKernelGGL((lb_k_sweep_pre<TV, R>), dim3((unsigned)(npost + np) + ncopy), dim3(LB_BLOCK), ldsPre,
+                                   c->stream, pb, npost, pa, np, c->sumPpair, zb[cur], dst, cw, (double *)c->zpart.p);
+            };
+            for (int b = 0; b < nbatch; b++) {
+                const int64_t p0 = bpos[b], nb = bpos[b + 1] - bpos[b];
+                if (nb == 0) continue;
+                pre(nbprev ? (int)((nbprev + 3) / 4) : 0, npair,
+                    c->nlev ? (const uint8_t *)c->tmask.p + (int64_t)b * c->sumPpair : nullptr, zb[cur ^ 1]);
+                sa.z = zb[cur]; sa.Y = zb[cur] + R; sa.zYn = zb[cur ^ 1]; sa.rows = order + p0; sa.nb = nb;
+                ga.b = b;
+                const int nrb = (int)((nb + 3) / 4);
+                hipLaunchKernelGGL((lb_k_step_fused<TV, R>), dim3((unsigned)nrb + (nsegs ? gsegs : 0u)), dim3(LB_BLOCK), 0,
+                                   c->stream, sa, ga, nrb);
+                pprev = p0; nbprev = nb; cur ^= 1;
+            }
+            // the last batch's post-step, its z completed, and its [z | Y] back into Y
+            if (nbprev) pre((int)((nbprev + 3) / 4), 0, nullptr, cur ? zb[0] : nullptr);
+        };
+        auto run = [&]() {
+            if (fused) batches_fused(pa, ga, sa, pb);
+            else batches(pa, ga, sa, qa, pb);
+        };
         if (!c->sweep_graph) {
-            batches(pa, ga, sa, qa, pb);
+            run();
             HIPCHK(c, hipGetLastError());
             return 0;
         }
         if (!c->sw_exec || key != c->sw_key) {
             if (c->sw_exec) { hipGraphExecDestroy((hipGraphExec_t)c->sw_exec); c->sw_exec = nullptr; }
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            batches(pa, ga, sa, qa, pb);
+            run();
             hipGraph_t g = nullptr;
             const hipError_t le = hipGetLastError();
             hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -3806,6 +4031,7 @@ void lb_destroy(lb_ctx *c) {
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
                     &c->hTseg0, &c->parts, &c->lamsum, &c->order, &c->tmask, &c->Kq, &c->stZ, &c->Zs, &c->cnts,
                     &c->Vstage, &c->DDb, &c->segkap, &c->seglist, &c->vslack, &c->vage, &c->vvol, &c->rorder,
+                    &c->Yalt, &c->hubcnt, &c->hubpos, &c->hubown, &c->zpart,
                     &c->b_off, &c->b_gid, &c->b_L, &c->b_C0, &c->b_Poff, &c->b_soff, &c->b_cg, &c->b_lab, &c->b_sig,
                     &c->b_pp, &c->b_rq, &c->b_dsq, &c->b_zY, &c->b_M, &c->b_pdiag, &c->b_ltot, &c->b_lviol, &c->b_rowv,
                     &c->b_parts, &c->b_lamsum, &c->b_tptr, &c->b_tidx, &c->b_R, &c->b_Vp, &c->b_Rp, &c->b_rgst};
@@ -3831,6 +4057,12 @@ int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
     if (std::strcmp(name, "row_order") == 0) { c->use_order = value != 0; return 0; }
     if (std::strcmp(name, "overlap") == 0) { c->overlap = value != 0; return 0; }
     if (std::strcmp(name, "sweep_graph") == 0) { c->sweep_graph = value != 0; return 0; }
+    if (std::strcmp(name, "sweep_fused") == 0) { c->sweep_fused = value != 0; return 0; }
+    if (std::strcmp(name, "sweep_seg_grid") == 0) {
+        if (value < 0 || value > 4096) return fail(c, -1, "lb_set_option: sweep_seg_grid is 0..4096");
+        c->sweep_seg_grid = value;
+        return 0;
+    }
     if (std::strcmp(name, "vcs_tile") == 0) {
         if (value != 16 && value != 32) return fail(c, -1, "lb_set_option: vcs_tile is 16 or 32");
         c->vcs_tile = (int)value;
@@ -4322,6 +4554,18 @@ int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap
     if ((rc = dalloc(c, c->vst_part, sizeof(double) * (size_t)LB_RED_BLOCKS * std::max(c->sumPv * (R + 1), 1)))) return rc;
     if ((rc = dalloc(c, c->Vstage, (size_t)maxb * S * 16))) return rc;
     if ((rc = dalloc(c, c->DDb, sizeof(double) * (size_t)maxb * R))) return rc;
+    {
+        // fused sweeps: the alternate [z | Y] and the hub-row meeting state (counts start at zero; each batch's last
+        // arrival clears its row's count)
+        const size_t nhub = std::max<size_t>(1, c->sw.hE_rows.size() + c->sw.hT_rows.size());
+        if ((rc = dalloc(c, c->Yalt, sizeof(double) * (size_t)(R + (c->sumPpair + c->sumPv) * R)))) return rc;
+        if ((rc = dalloc(c, c->hubcnt, sizeof(int32_t) * nhub))) return rc;
+        if ((rc = dalloc(c, c->hubpos, sizeof(int64_t) * nhub))) return rc;
+        if ((rc = dalloc(c, c->hubown, sizeof(double) * nhub * (size_t)(R + 1)))) return rc;
+        HIPCHK(c, hipMemsetAsync(c->hubcnt.p, 0, sizeof(int32_t) * nhub, c->stream));
+        if ((rc = dalloc(c, c->zpart, sizeof(double) * LB_ZPART * (size_t)R))) return rc;
+        HIPCHK(c, hipMemsetAsync(c->zpart.p, 0, sizeof(double) * LB_ZPART * (size_t)R, c->stream));
+    }
     std::vector<int32_t> o32(n);
     std::vector<uint8_t> seen(n);
     std::vector<int64_t> bpos(nbatch + 1), lpos(nbatch + 1);

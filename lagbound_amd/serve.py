@@ -430,6 +430,9 @@ def main(argv=None):
     ap.add_argument("--budget_gb", type=float, default=192.0)
     ap.add_argument("--outer", type=int, default=24)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--inline", action="store_true",
+                    help="solve the requests one after another in this process (no slots, no arrival times): one "
+                         "JSON line per request; the form to run under a profiler")
     a = ap.parse_args(argv)
     t = time.perf_counter()
     reqs = poisson_stream(a.count, a.rate, a.n_lo, a.n_hi, a.kbar_lo, a.kbar_hi, a.seed)
@@ -439,6 +442,16 @@ def main(argv=None):
         p.blas_threads = a.blas_threads
     if a.hw_queues is not None:
         p.hw_queues = a.hw_queues
+
+    if a.inline:
+        prepare_device(p)
+        for req in reqs:
+            t0 = time.perf_counter()
+            out, t_dev = solve_request(req, p)
+            out.pop("log", None)
+            out.update(rid=req.rid, service_s=round(time.perf_counter() - t0, 3), device_lib_s=round(t_dev, 3))
+            print(json.dumps(out), flush=True)
+        return 0
 
     def progress(r):
         print(f"[serve] rid {r.rid} n {r.n} m {r.m} latency {r.latency:.3f}s service {r.finish - r.start:.3f}s "

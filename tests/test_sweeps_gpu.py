@@ -133,3 +133,41 @@ def test_captured_sweeps_equal_eager(dtype):
     Wo = O.sweeps_c(inst, Ws[-2], calls[-1][0], 16, calls[-1][1], damp=0.1)
     tol = 1e-10 if dtype == "f64" else 2e-4
     assert np.abs(Ws[-1] - Wo).max() <= tol
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32", "bf16"])
+def test_fused_sweeps_equal_five_kernel_form(dtype):
+    """Option sweep_fused 1 (the default: two launches per batch, the rows adding their increments to the cluster sums
+    themselves, hub rows finished by whichever of their parts arrives last) against sweep_fused 0 (five launches: pair
+    rows, hub segments, row steps, aggregate increments, VCS post step). Same row arithmetic; only the FP64 adds into
+    the sums run in another order, so V agrees to 1e-12 (FP64 V) or to a unit of the storage precision where such a
+    difference moves a rounding (FP32 2e-6, BF16 3e-2). Small hub thresholds (rows above 8 edges or records split
+    into 5-item segments, so most rows are hubs with several segments of both kinds), three sweeps, pair and VCS
+    levels; the fused form is also captured as a graph and replayed (second call), and its last call is checked
+    against the oracle."""
+    from lagbound_amd.engine import Engine
+    inst, V = chung_lu_instance(n=5000, seed=21, k=16, P=16)
+    rng = np.random.default_rng(22)
+    calls = [np.stack([rng.permutation(inst.n) for _ in range(2)]), np.stack([rng.permutation(inst.n)])]
+    out = {}
+    for fused, graph in ((1, 0), (0, 0), (1, 1)):
+        eng = Engine(r=16, dtype=dtype)
+        for k, v in {"sweep_fused": fused, "sweep_graph": graph, "sweep_heavy": 8, "sweep_seg": 5}.items():
+            eng.set_option(k, v)
+        eng.set_instance(inst)
+        eng.set_V(V)
+        Ws = [eng.get_V()]
+        print(f"[sweeps] fused={fused} graph={graph} dtype={dtype} n={inst.n}", flush=True)
+        for orders in calls:
+            eng.sweeps(orders, 12, 0.05, damp=0.1)
+            Ws.append(eng.get_V())
+        out[(fused, graph)] = Ws
+        eng.close()
+    tol = {"f64": 1e-12, "f32": 2e-6, "bf16": 3e-2}[dtype]
+    for key in ((1, 1), (0, 0)):
+        for a, b in zip(out[(1, 0)][1:], out[key][1:]):
+            assert np.abs(a - b).max() <= tol, key
+    Ws = out[(1, 0)]
+    Wo = O.sweeps_c(inst, Ws[1], calls[1], 12, 0.05, damp=0.1)
+    otol = {"f64": 1e-10, "f32": 2e-4, "bf16": 5e-2}[dtype]
+    assert np.abs(Ws[2] - Wo).max() <= otol
