@@ -86,6 +86,8 @@ def parse(argv=None):
     # device options (no reference counterpart)
     p.add_argument("--dtype", default="f32", help="V storage on the device: f32 | bf16 | f64")
     p.add_argument("--device", type=int, default=0)
+    p.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                   help="a kernel switch of the device library (lb_set_option), e.g. sweep_fused=0")
     a, unknown = p.parse_known_args(argv)
     if unknown:
         raise SystemExit(f"unsupported flags for the device solver: {' '.join(unknown)}")
@@ -98,6 +100,10 @@ def parse(argv=None):
         raise SystemExit("--k must be 8, 16 or 32 on the device")
     if a.vcs_score not in ("pen", "mean", "mass"):
         raise SystemExit("bad --vcs_score")
+    for o in a.option:
+        name, _, val = o.partition("=")
+        if not name or not val.lstrip("-").isdigit():
+            raise SystemExit(f"--option {o!r}: expected NAME=INTEGER")
     return a
 
 
@@ -221,6 +227,9 @@ class Solver:
             raise SystemExit(f"V has {V.shape[1]} columns, --k {a.k}")
         inst.TI, inst.lam = TI, lam
         self.eng = Engine(r=a.k, dtype=a.dtype, device=a.device)
+        for o in a.option:
+            name, _, val = o.partition("=")
+            self.eng.set_option(name, int(val))
         self.eng.set_instance(inst)
         self.eng.set_V(V)
         self._upload_ages()

@@ -48,6 +48,7 @@ class Request:
 class Params:
     k: int = 16
     dtype: str = "f32"
+    options: tuple = ()              # kernel switches of the search's device library ("NAME=VALUE", lb_set_option)
     outer: int = 24
     nbatch: int = 0                  # 0: the reference's rule (nbatch_rule)
     tri_every: int = 4
@@ -134,6 +135,8 @@ def search_request(req: Request, p: Params, device=0, log=None):
             "--tri_add", str(p.tri_add or 2 * inst.n), "--tri1_add", "0", "--tri_every", str(p.tri_every),
             "--tri_max", "16000000", "--vcs_every", str(p.vcs_every), "--vcs_add", str(4 * inst.n),
             "--vcs_max", str(16 * inst.n), "--pw_rule", "default", "--dtype", p.dtype, "--device", str(device)]
+    for o in p.options:
+        argv += ["--option", o]
     a = alm.parse(argv)
     s = alm.Solver(a, inst=inst, labels=[lab], vcs_labels=[lab], init_labels=lab, log=log, signals=False)
     s.run(close=False)
@@ -430,6 +433,8 @@ def main(argv=None):
     ap.add_argument("--budget_gb", type=float, default=192.0)
     ap.add_argument("--outer", type=int, default=24)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                    help="a kernel switch of the search's device library (lb_set_option), e.g. sweep_fused=0")
     ap.add_argument("--inline", action="store_true",
                     help="solve the requests one after another in this process (no slots, no arrival times): one "
                          "JSON line per request; the form to run under a profiler")
@@ -437,7 +442,7 @@ def main(argv=None):
     t = time.perf_counter()
     reqs = poisson_stream(a.count, a.rate, a.n_lo, a.n_hi, a.kbar_lo, a.kbar_hi, a.seed)
     gen = time.perf_counter() - t
-    p = Params(outer=a.outer)
+    p = Params(outer=a.outer, options=tuple(a.option))
     if a.blas_threads:
         p.blas_threads = a.blas_threads
     if a.hw_queues is not None:

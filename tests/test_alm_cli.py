@@ -20,10 +20,16 @@ def test_production_flags_parse():
 
 
 @pytest.mark.parametrize("bad", [["--mom", "0.5"], ["--est_every", "10"], ["--ne_split", "1"], ["--k", "12"],
-                                 ["--nbr_cut", "1"]])
+                                 ["--nbr_cut", "1"], ["--option", "sweep_fused"], ["--option", "=1"],
+                                 ["--option", "sweep_fused=yes"]])
 def test_unsupported_flags_rejected(bad):
     with pytest.raises(SystemExit):
         alm.parse(["--tag", "g", "--save", "x.npz", "--est_every", "0", "--k", "16"] + bad)
+
+
+def test_device_options_parse():
+    a = alm.parse(["--tag", "g", "--est_every", "0", "--k", "16", "--option", "sweep_fused=0", "--option", "vcs_grid=-1"])
+    assert a.option == ["sweep_fused=0", "vcs_grid=-1"]
 
 
 def test_load_instance_reduced(tmp_path):
