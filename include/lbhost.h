@@ -22,6 +22,12 @@ int64_t lbh_ws_merge(int64_t n, int64_t P, const int64_t *ptr, const int32_t *co
 int lbh_take_f64(int64_t cnt, const int64_t *src, const double *a, int64_t na, double *out);
 int lbh_take_u8(int64_t cnt, const int64_t *src, const uint8_t *a, int64_t na, uint8_t *out);
 
+// Stable argsorts (the permutation numpy's argsort(kind="stable") returns): of keys in [0, kmax) by counting (-1 if
+// a key is out of range), and of any int64 keys by a radix sort. The search's set-up uses them for the degree order,
+// the CSR transpose of the relabelled graph and the CSR of the edge keys (instances.py, vertex_order.py).
+int lbh_argsort_bounded(int64_t m, const int64_t *keys, int64_t kmax, int64_t *perm_out);
+int lbh_argsort_i64(int64_t m, const int64_t *keys, int64_t *perm_out);
+
 // Triangle candidates not in the pool (alm_vcs.py:1751-1758): the packed key (a n + b) n + c of every candidate
 // (rows of cand, m x 3) and of every pool triangle (rows of pool, T x 3), in int64 arithmetic with its wrap-around;
 // writes to idx_out the index of the first occurrence of every distinct candidate key absent from the pool, in

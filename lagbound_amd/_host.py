@@ -1,5 +1,5 @@
 """ctypes binding of liblbhost.so (include/lbhost.h): the search's host bookkeeping between device calls (working-set
-merges, triangle-pool filtering). Raises if the library has not been built."""
+merges, triangle-pool filtering) and the stable argsorts of its set-up. Raises if the library has not been built."""
 from __future__ import annotations
 
 import ctypes as C
@@ -9,7 +9,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "liblbhost.so")
-EXPORTS = ["lbh_ws_merge", "lbh_take_f64", "lbh_take_u8", "lbh_new_triangles"]
+EXPORTS = ["lbh_ws_merge", "lbh_take_f64", "lbh_take_u8", "lbh_new_triangles", "lbh_argsort_bounded",
+           "lbh_argsort_i64"]
 
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
@@ -31,6 +32,8 @@ def load():
         L.lbh_take_u8.argtypes = [C.c_int64, _i64p, _u8p, C.c_int64, _u8p]
         L.lbh_new_triangles.argtypes = [C.c_int64, C.c_int64, _i64p, C.c_int64, _i64p, _i64p]
         L.lbh_new_triangles.restype = C.c_int64
+        L.lbh_argsort_bounded.argtypes = [C.c_int64, _i64p, C.c_int64, _i64p]
+        L.lbh_argsort_i64.argtypes = [C.c_int64, _i64p, _i64p]
         _lib = L
     return _lib
 
@@ -80,3 +83,21 @@ def new_triangles(n, cand, pool):
     if w < 0:
         raise ValueError("lbh_new_triangles: bad arguments")
     return out[:w]
+
+
+def argsort_bounded(keys, kmax):
+    """np.argsort(keys, kind="stable") for integer keys in [0, kmax), by a counting sort (lbh_argsort_bounded)."""
+    keys = np.ascontiguousarray(keys, np.int64)
+    out = np.empty(len(keys), np.int64)
+    if load().lbh_argsort_bounded(len(keys), _p(keys, _i64p), int(max(kmax, 1)), _p(out, _i64p)):
+        raise ValueError("lbh_argsort_bounded: a key outside [0, kmax)")
+    return out
+
+
+def argsort_i64(keys):
+    """np.argsort(keys, kind="stable") for int64 keys, by a radix sort (lbh_argsort_i64)."""
+    keys = np.ascontiguousarray(keys, np.int64)
+    out = np.empty(len(keys), np.int64)
+    if load().lbh_argsort_i64(len(keys), _p(keys, _i64p), _p(out, _i64p)):
+        raise ValueError("lbh_argsort_i64: bad arguments")
+    return out

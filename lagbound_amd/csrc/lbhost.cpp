@@ -7,6 +7,46 @@
 #include <cstring>
 #include <vector>
 
+namespace {
+
+// Stable LSD radix sort of 64-bit keys (11-bit digits; a pass is skipped when every key has the same digit there),
+// carrying an int64 payload when one is given. Signed order: the keys are taken with their sign bit flipped.
+void radix_sort(std::vector<uint64_t> &k, std::vector<int64_t> *v) {
+    const size_t n = k.size();
+    if (n < 2) return;
+    constexpr int B = 11, NB = 1 << B, PASSES = (64 + B - 1) / B;
+    std::vector<uint64_t> hist((size_t)PASSES * NB, 0);
+    for (size_t t = 0; t < n; t++) {
+        const uint64_t x = k[t];
+        for (int p = 0; p < PASSES; p++) hist[(size_t)p * NB + ((x >> (p * B)) & (NB - 1))]++;
+    }
+    std::vector<uint64_t> k2(n);
+    std::vector<int64_t> v2(v ? n : 0);
+    for (int p = 0; p < PASSES; p++) {
+        uint64_t *h = hist.data() + (size_t)p * NB;
+        bool trivial = false;
+        for (int b = 0; b < NB; b++)
+            if (h[b] == n) { trivial = true; break; }
+        if (trivial) continue;
+        uint64_t run = 0;
+        for (int b = 0; b < NB; b++) { const uint64_t c = h[b]; h[b] = run; run += c; }
+        const int sh = p * B;
+        if (v) {
+            for (size_t t = 0; t < n; t++) {
+                const uint64_t o = h[(k[t] >> sh) & (NB - 1)]++;
+                k2[o] = k[t];
+                v2[o] = (*v)[t];
+            }
+            v->swap(v2);
+        } else {
+            for (size_t t = 0; t < n; t++) k2[h[(k[t] >> sh) & (NB - 1)]++] = k[t];
+        }
+        k.swap(k2);
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 int64_t lbh_ws_merge(int64_t n, int64_t P, const int64_t *ptr, const int32_t *col, const uint8_t *keep, int64_t m,
@@ -77,26 +117,51 @@ int lbh_take_u8(int64_t cnt, const int64_t *src, const uint8_t *a, int64_t na, u
     return 0;
 }
 
+int lbh_argsort_bounded(int64_t m, const int64_t *keys, int64_t kmax, int64_t *perm_out) {
+    if (m < 0 || kmax < 1 || (m && (!keys || !perm_out))) return -1;
+    std::vector<int64_t> at((size_t)kmax + 1, 0);
+    for (int64_t t = 0; t < m; t++) {
+        if (keys[t] < 0 || keys[t] >= kmax) return -1;
+        at[(size_t)keys[t] + 1]++;
+    }
+    for (int64_t k = 0; k < kmax; k++) at[(size_t)k + 1] += at[(size_t)k];
+    for (int64_t t = 0; t < m; t++) perm_out[at[(size_t)keys[t]]++] = t;
+    return 0;
+}
+
+int lbh_argsort_i64(int64_t m, const int64_t *keys, int64_t *perm_out) {
+    if (m < 0 || (m && (!keys || !perm_out))) return -1;
+    const uint64_t flip = (uint64_t)1 << 63;
+    std::vector<uint64_t> k((size_t)m);
+    std::vector<int64_t> v((size_t)m);
+    for (int64_t t = 0; t < m; t++) { k[(size_t)t] = (uint64_t)keys[t] ^ flip; v[(size_t)t] = t; }
+    radix_sort(k, &v);
+    std::memcpy(perm_out, v.data(), sizeof(int64_t) * (size_t)m);
+    return 0;
+}
+
 int64_t lbh_new_triangles(int64_t n, int64_t m, const int64_t *cand, int64_t T, const int64_t *pool, int64_t *idx_out) {
     if (n < 0 || m < 0 || T < 0) return -1;
     // the packed key in int64 arithmetic with its wrap-around (computed in uint64, read back as int64)
-    auto key = [n](const int64_t *t) -> int64_t {
-        const uint64_t un = (uint64_t)n;
-        return (int64_t)(((uint64_t)t[0] * un + (uint64_t)t[1]) * un + (uint64_t)t[2]);
+    // (sign bit flipped, so that unsigned order is the int64 order)
+    const uint64_t un = (uint64_t)n, flip = (uint64_t)1 << 63;
+    auto key = [un, flip](const int64_t *t) -> uint64_t {
+        return (((uint64_t)t[0] * un + (uint64_t)t[1]) * un + (uint64_t)t[2]) ^ flip;
     };
-    std::vector<std::pair<int64_t, int64_t>> kc((size_t)m);
-    for (int64_t k = 0; k < m; k++) kc[(size_t)k] = {key(cand + 3 * k), k};
-    std::sort(kc.begin(), kc.end());                 // by key, then index: the first occurrence leads each run
-    std::vector<int64_t> kp((size_t)T);
+    std::vector<uint64_t> kc((size_t)m);
+    std::vector<int64_t> ic((size_t)m);
+    for (int64_t k = 0; k < m; k++) { kc[(size_t)k] = key(cand + 3 * k); ic[(size_t)k] = k; }
+    radix_sort(kc, &ic);                             // stable: the first occurrence leads each run of equal keys
+    std::vector<uint64_t> kp((size_t)T);
     for (int64_t t = 0; t < T; t++) kp[(size_t)t] = key(pool + 3 * t);
-    std::sort(kp.begin(), kp.end());
+    radix_sort(kp, nullptr);
     int64_t w = 0;
     size_t p = 0;
     for (size_t a = 0; a < kc.size();) {
         size_t b = a + 1;
-        while (b < kc.size() && kc[b].first == kc[a].first) b++;
-        while (p < kp.size() && kp[p] < kc[a].first) p++;   // both ascending: one merge pass
-        if (!(p < kp.size() && kp[p] == kc[a].first)) idx_out[w++] = kc[a].second;
+        while (b < kc.size() && kc[b] == kc[a]) b++;
+        while (p < kp.size() && kp[p] < kc[a]) p++;   // both ascending: one merge pass
+        if (!(p < kp.size() && kp[p] == kc[a])) idx_out[w++] = ic[a];
         a = b;
     }
     return w;

@@ -18,6 +18,8 @@ from __future__ import annotations
 import dataclasses
 import numpy as np
 
+from . import _host as H
+
 
 @dataclasses.dataclass
 class PairLevelData:
@@ -96,12 +98,12 @@ def csr_from_keys(n: int, keys: np.ndarray, weights: np.ndarray | None = None):
     v = keys % n
     # keys sorted (u, v): entries (v, u) first, then (u, v); a stable sort by row alone then leaves every row's
     # columns ascending (below-diagonal ones from the first half, above-diagonal ones from the second)
-    korder = np.argsort(keys, kind="stable")
+    korder = H.argsort_i64(keys)
     u = u[korder]
     v = v[korder]
     r = np.r_[v, u]
     c = np.r_[u, v]
-    order = np.argsort(r, kind="stable")
+    order = H.argsort_bounded(r, n)
     c = c[order]
     rowptr = np.zeros(n + 1, np.int64)
     rowptr[1:] = np.cumsum(np.bincount(r, minlength=n))

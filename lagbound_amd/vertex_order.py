@@ -27,6 +27,7 @@ import copy
 
 import numpy as np
 
+from . import _host as H
 from . import instances as I
 
 
@@ -34,7 +35,8 @@ class VertexOrder:
     def __init__(self, rowptr: np.ndarray):
         deg = np.diff(np.asarray(rowptr, np.int64))
         self.n = len(deg)
-        self.order = np.argsort(-deg, kind="stable").astype(np.int64)
+        dmax = int(deg.max()) if len(deg) else 0
+        self.order = H.argsort_bounded(dmax - deg, dmax + 1)       # argsort(-deg, kind="stable")
         self.new_of = np.empty(self.n, np.int64)
         self.new_of[self.order] = np.arange(self.n, dtype=np.int64)
 
@@ -72,7 +74,7 @@ class VertexOrder:
         c, the rows k ascending -- row c of B^T = P A P^T (A is symmetric)."""
         rowptr, idx, cnt = self._row_gather(inst.rowptr)
         bc = self.new_of[np.asarray(inst.col, np.int64)[idx]]
-        t = np.argsort(bc, kind="stable")
+        t = H.argsort_bounded(bc, self.n)
         col = np.repeat(np.arange(self.n, dtype=np.int64), cnt)[t].astype(np.int32)
         w = None if inst.w is None else np.asarray(inst.w, np.float64)[idx][t]
         return rowptr, col, w, np.asarray(inst.d, np.float64)[self.order]

@@ -152,3 +152,21 @@ def test_new_triangles_keeps_the_int64_wraparound():
     uk, fi = np.unique(kn, return_index=True)
     keep = ~np.isin(uk, kp)
     assert np.array_equal(got_new, new[fi[keep]]) and np.array_equal(got_val, val[fi[keep]])
+
+
+def test_host_argsorts_equal_numpy_stable():
+    """liblbhost's stable argsorts (counting sort of bounded keys, radix sort of int64 keys) return numpy's
+    argsort(kind="stable") permutation, ties in index order, including empty input, one key, the int64 extremes and
+    heavy duplication; a key outside [0, kmax) is an error."""
+    from lagbound_amd import _host as H
+    rng = np.random.default_rng(5)
+    for m, kmax in ((0, 5), (1, 1), (1000, 7), (50000, 40000)):
+        k = rng.integers(0, kmax, m)
+        assert np.array_equal(H.argsort_bounded(k, kmax), np.argsort(k, kind="stable"))
+    for k in (np.zeros(0, np.int64), rng.integers(-2 ** 62, 2 ** 62, 20000), rng.integers(0, 10, 20000),
+              np.array([5, -3, 5, 2 ** 63 - 1, -2 ** 63, 0], np.int64)):
+        assert np.array_equal(H.argsort_i64(k), np.argsort(k, kind="stable"))
+    with pytest.raises(ValueError):
+        H.argsort_bounded(np.array([3]), 3)
+    with pytest.raises(ValueError):
+        H.argsort_bounded(np.array([-1]), 3)
