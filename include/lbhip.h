@@ -156,9 +156,10 @@ int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int
  * most 1536) its workgroups; "vcs_tile" (default 32, else 16) the rows per wavefront tile of the one-pass kernel (32
  * only where its LDS fits); "sweep_graph" (default 0) 1 captures a sweep's per-batch launches as one HIP graph and
  * replays it while the launch arguments are unchanged (same kernels and results as the eager launches);
- * "sweep_fused" (default 1, with "step_wave" 1) runs a sweep batch as two launches -- the previous batch's VCS post
+ * "sweep_fused" (default 0; with "step_wave" 1) 1 runs a sweep batch as two launches -- the previous batch's VCS post
  * step with this batch's pair rows, then the row steps with the hub segments, each row adding its increment to the
- * cluster sums itself -- in place of five (same arithmetic; the sums' FP64 adds in another order); "sweep_seg_grid"
+ * cluster sums itself -- in place of five (same arithmetic; the sums' FP64 adds in another order); off by default
+ * since one serving run with it ended in a GPU memory fault of unestablished cause; "sweep_seg_grid"
  * (default 0: 64) caps the workgroups that compute the sweeps' hub segments;
  * "overlap" (default 1) runs the evaluation's hub-segment kernel on a second stream of the context, concurrently
  * with the cluster-sum and VCS kernels, joined before the row kernel (identical results; lb_time_eval's per-kernel

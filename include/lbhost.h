@@ -28,6 +28,16 @@ int lbh_take_u8(int64_t cnt, const int64_t *src, const uint8_t *a, int64_t na, u
 int lbh_argsort_bounded(int64_t m, const int64_t *keys, int64_t kmax, int64_t *perm_out);
 int lbh_argsort_i64(int64_t m, const int64_t *keys, int64_t *perm_out);
 
+// Rows of a CSR (n rows, ptr) taken in `order` (a permutation of 0..n-1), each row's entries kept in their order:
+// the new row pointers (nptr_out, n + 1) and, for each entry of the new layout, its entry index in the old one
+// (idx_out, ptr[n] entries). Returns the entry count, or -1 on bad arguments.
+int64_t lbh_row_gather(int64_t n, const int64_t *ptr, const int64_t *order, int64_t *nptr_out, int64_t *idx_out);
+
+// Order of VCS separation candidates given by descending score sc (m entries): perm_out = the permutation that sorts
+// every run of equal scores by (vi, vR, position) ascending and keeps the runs in place -- np.lexsort((vR, vi, -sc)) for
+// descending sc. Returns 0, 1 if sc is not descending (nothing written: sort in full), -1 on bad arguments.
+int lbh_tie_order(int64_t m, const double *sc, const int64_t *vi, const int64_t *vR, int64_t *perm_out);
+
 // Triangle candidates not in the pool (alm_vcs.py:1751-1758): the packed key (a n + b) n + c of every candidate
 // (rows of cand, m x 3) and of every pool triangle (rows of pool, T x 3), in int64 arithmetic with its wrap-around;
 // writes to idx_out the index of the first occurrence of every distinct candidate key absent from the pool, in

@@ -10,7 +10,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "liblbhost.so")
 EXPORTS = ["lbh_ws_merge", "lbh_take_f64", "lbh_take_u8", "lbh_new_triangles", "lbh_argsort_bounded",
-           "lbh_argsort_i64"]
+           "lbh_argsort_i64", "lbh_tie_order", "lbh_row_gather"]
 
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
@@ -34,6 +34,9 @@ def load():
         L.lbh_new_triangles.restype = C.c_int64
         L.lbh_argsort_bounded.argtypes = [C.c_int64, _i64p, C.c_int64, _i64p]
         L.lbh_argsort_i64.argtypes = [C.c_int64, _i64p, _i64p]
+        L.lbh_tie_order.argtypes = [C.c_int64, _f64p, _i64p, _i64p, _i64p]
+        L.lbh_row_gather.argtypes = [C.c_int64, _i64p, _i64p, _i64p, _i64p]
+        L.lbh_row_gather.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -101,3 +104,31 @@ def argsort_i64(keys):
     if load().lbh_argsort_i64(len(keys), _p(keys, _i64p), _p(out, _i64p)):
         raise ValueError("lbh_argsort_i64: bad arguments")
     return out
+
+
+def tie_order(sc, i, R):
+    """np.lexsort((R, i, -sc)) for candidates already in descending score order (lbh_tie_order: only the runs of
+    equal scores are sorted); the full lexsort if sc is not descending."""
+    sc = np.ascontiguousarray(sc, np.float64)
+    i = np.ascontiguousarray(i, np.int64)
+    R = np.ascontiguousarray(R, np.int64)
+    out = np.empty(len(sc), np.int64)
+    rc = load().lbh_tie_order(len(sc), _p(sc, _f64p), _p(i, _i64p), _p(R, _i64p), _p(out, _i64p))
+    if rc < 0:
+        raise ValueError("lbh_tie_order: bad arguments")
+    return out if rc == 0 else np.lexsort((R, i, -sc))
+
+
+def row_gather(ptr, order):
+    """(nptr, idx): the rows of a CSR taken in `order`, entries kept in row order (lbh_row_gather)."""
+    ptr = np.ascontiguousarray(ptr, np.int64)
+    order = np.ascontiguousarray(order, np.int64)
+    n = len(order)
+    if len(ptr) != n + 1:
+        raise ValueError("row_gather: ptr must have len(order) + 1 entries")
+    nptr = np.empty(n + 1, np.int64)
+    idx = np.empty(int(ptr[-1] - ptr[0]) if n else 0, np.int64)
+    w = load().lbh_row_gather(n, _p(ptr, _i64p), _p(order, _i64p), _p(nptr, _i64p), _p(idx, _i64p))
+    if w < 0:
+        raise ValueError("lbh_row_gather: bad arguments")
+    return nptr, idx[:w]

@@ -87,7 +87,7 @@ def parse(argv=None):
     p.add_argument("--dtype", default="f32", help="V storage on the device: f32 | bf16 | f64")
     p.add_argument("--device", type=int, default=0)
     p.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
-                   help="a kernel switch of the device library (lb_set_option), e.g. sweep_fused=0")
+                   help="a kernel switch of the device library (lb_set_option), e.g. sweep_graph=1")
     a, unknown = p.parse_known_args(argv)
     if unknown:
         raise SystemExit(f"unsupported flags for the device solver: {' '.join(unknown)}")

@@ -2974,9 +2974,11 @@ struct lb_ctx {
     // since a batch is bound by its five kernels' own durations (about 33 of its 36 us), not by the launches.
     void *sw_exec = nullptr; std::vector<unsigned char> sw_key; bool sweep_graph = false;
     DBuf lposd;                                  // device copy of the sweep's per-batch segment-list bounds
-    // option "sweep_fused" 1 (default, with step_wave): two launches per batch (lb_k_sweep_pre, lb_k_step_fused) in
-    // place of five; the batches' [z | Y] alternate between Y and Yalt; hub rows meet through hubcnt / hubpos / hubown
-    bool sweep_fused = true;
+    // option "sweep_fused" 1 (with step_wave): two launches per batch (lb_k_sweep_pre, lb_k_step_fused) in place of
+    // five; the batches' [z | Y] alternate between Y and Yalt; hub rows meet through hubcnt / hubpos / hubown. Off by
+    // default: one run of a 16-request serving stream ended in a GPU memory fault inside a sweep of this form (after
+    // two clean runs of the same stream) and its cause is not established
+    bool sweep_fused = false;
     DBuf Yalt, hubcnt, hubpos, hubown, zpart;
     // workgroups of the sweeps' hub segments (0: 64); on a 150k-vertex request's search 0.326 s of sweeps at 64
     // against 0.335 s at 16

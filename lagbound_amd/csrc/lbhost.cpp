@@ -140,6 +140,40 @@ int lbh_argsort_i64(int64_t m, const int64_t *keys, int64_t *perm_out) {
     return 0;
 }
 
+int64_t lbh_row_gather(int64_t n, const int64_t *ptr, const int64_t *order, int64_t *nptr_out, int64_t *idx_out) {
+    if (n < 0 || !ptr || (n && (!order || !nptr_out))) return -1;
+    nptr_out[0] = 0;
+    for (int64_t k = 0; k < n; k++) {
+        const int64_t i = order[k];
+        if (i < 0 || i >= n || ptr[i + 1] < ptr[i]) return -1;
+        nptr_out[k + 1] = nptr_out[k] + (ptr[i + 1] - ptr[i]);
+    }
+    if (nptr_out[n] && !idx_out) return -1;
+    int64_t o = 0;
+    for (int64_t k = 0; k < n; k++) {
+        const int64_t i = order[k];
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) idx_out[o++] = e;
+    }
+    return o;
+}
+
+int lbh_tie_order(int64_t m, const double *sc, const int64_t *vi, const int64_t *vR, int64_t *perm_out) {
+    if (m < 0 || (m && (!sc || !vi || !vR || !perm_out))) return -1;
+    for (int64_t t = 0; t + 1 < m; t++)
+        if (!(sc[t] >= sc[t + 1])) return 1;         // not descending (or a NaN): the caller sorts in full
+    for (int64_t t = 0; t < m; t++) perm_out[t] = t;
+    for (int64_t a = 0; a < m;) {
+        int64_t b = a + 1;
+        while (b < m && sc[b] == sc[a]) b++;
+        if (b - a > 1)
+            std::sort(perm_out + a, perm_out + b, [vi, vR](int64_t x, int64_t y) {
+                return vi[x] != vi[y] ? vi[x] < vi[y] : (vR[x] != vR[y] ? vR[x] < vR[y] : x < y);
+            });
+        a = b;
+    }
+    return 0;
+}
+
 int64_t lbh_new_triangles(int64_t n, int64_t m, const int64_t *cand, int64_t T, const int64_t *pool, int64_t *idx_out) {
     if (n < 0 || m < 0 || T < 0) return -1;
     // the packed key in int64 arithmetic with its wrap-around (computed in uint64, read back as int64)

@@ -434,7 +434,7 @@ def main(argv=None):
     ap.add_argument("--outer", type=int, default=24)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
-                    help="a kernel switch of the search's device library (lb_set_option), e.g. sweep_fused=0")
+                    help="a kernel switch of the search's device library (lb_set_option), e.g. sweep_graph=1")
     ap.add_argument("--inline", action="store_true",
                     help="solve the requests one after another in this process (no slots, no arrival times): one "
                          "JSON line per request; the form to run under a profiler")

@@ -61,12 +61,8 @@ class VertexOrder:
     def _row_gather(self, ptr: np.ndarray):
         """For a row-indexed CSR (ptr), the new ptr and, for each entry of the new layout, its entry index in the
         old layout (rows taken in `order`, entries of a row kept in their order)."""
-        ptr = np.asarray(ptr, np.int64)
-        cnt = np.diff(ptr)[self.order]
-        nptr = np.zeros(self.n + 1, np.int64)
-        np.cumsum(cnt, out=nptr[1:])
-        idx = np.repeat(ptr[self.order] - nptr[:-1], cnt) + np.arange(nptr[-1], dtype=np.int64)
-        return nptr, idx, cnt
+        nptr, idx = H.row_gather(ptr, self.order)
+        return nptr, idx, np.diff(nptr)
 
     def graph(self, inst):
         """(rowptr, col int32, w or None, d) of P A P^T with rows sorted by column. Row k of the row-permuted matrix
@@ -256,7 +252,7 @@ class Relabeled:
         i = self.vo.ids_out(i)
         if not np.any(sc[1:] == sc[:-1]):            # no tied scores: the order (score descending) is final
             return i, R, sc, st
-        o = np.lexsort((R, i, -sc))
+        o = H.tie_order(sc, i, R)                  # np.lexsort((R, i, -sc)), sorting only the tied runs
         return i[o], R[o], sc[o], st
 
     def _native(self, what):

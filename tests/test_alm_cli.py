@@ -170,3 +170,40 @@ def test_host_argsorts_equal_numpy_stable():
         H.argsort_bounded(np.array([3]), 3)
     with pytest.raises(ValueError):
         H.argsort_bounded(np.array([-1]), 3)
+
+
+def test_tie_order_equals_lexsort():
+    """Candidates in descending score order with runs of tied scores: liblbhost's run-local sort gives
+    np.lexsort((R, i, -sc)); an order that is not descending falls back to the full lexsort."""
+    from lagbound_amd import _host as H
+    rng = np.random.default_rng(8)
+    for m in (0, 1, 2, 5000):
+        sc = -np.sort(-rng.integers(0, 40, m).astype(np.float64))       # many ties
+        i = rng.integers(0, 300, m)
+        R = rng.integers(0, 64, m)
+        assert np.array_equal(H.tie_order(sc, i, R), np.lexsort((R, i, -sc)))
+    sc = np.array([3.0, 1.0, 2.0, 2.0])
+    i = np.array([4, 3, 2, 1])
+    R = np.zeros(4, np.int64)
+    assert np.array_equal(H.tie_order(sc, i, R), np.lexsort((R, i, -sc)))
+
+
+def test_row_gather_equals_numpy():
+    """liblbhost's row gather (rows of a CSR taken in a permutation order) against its numpy statement, with empty
+    rows, a non-zero first pointer and no rows at all; a bad order is an error."""
+    from lagbound_amd import _host as H
+    rng = np.random.default_rng(12)
+    for n, base in ((0, 0), (1, 3), (500, 7)):
+        cnt = rng.integers(0, 6, n)
+        ptr = np.full(n + 1, base, np.int64)
+        np.cumsum(cnt, out=ptr[1:])
+        ptr[1:] += base
+        order = rng.permutation(n)
+        c2 = np.diff(ptr)[order]
+        nptr = np.zeros(n + 1, np.int64)
+        np.cumsum(c2, out=nptr[1:])
+        idx = np.repeat(ptr[order] - nptr[:-1], c2) + np.arange(nptr[-1], dtype=np.int64)
+        a, b = H.row_gather(ptr, order)
+        assert np.array_equal(a, nptr) and np.array_equal(b, idx)
+    with pytest.raises(ValueError):
+        H.row_gather(np.array([0, 1, 2], np.int64), np.array([0, 2], np.int64))

@@ -135,6 +135,8 @@ def test_captured_sweeps_equal_eager(dtype):
     assert np.abs(Ws[-1] - Wo).max() <= tol
 
 
+@pytest.mark.skip(reason="the fused sweep form (option sweep_fused 1) is switched off: one run of the 16-request "
+                         "serving stream with it ended in a GPU memory fault inside a sweep, cause not established")
 @pytest.mark.parametrize("dtype", ["f64", "f32", "bf16"])
 def test_fused_sweeps_equal_five_kernel_form(dtype):
     """Option sweep_fused 1 (the default: two launches per batch, the rows adding their increments to the cluster sums
