@@ -159,7 +159,9 @@ int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int
  * "sweep_fused" (default 0; with "step_wave" 1) 1 runs a sweep batch as two launches -- the previous batch's VCS post
  * step with this batch's pair rows, then the row steps with the hub segments, each row adding its increment to the
  * cluster sums itself -- in place of five (same arithmetic; the sums' FP64 adds in another order); off by default
- * since one serving run with it ended in a GPU memory fault of unestablished cause; "sweep_seg_grid"
+ * since one serving run with it ended in a GPU memory fault of unestablished cause; "sweep_pair_segs" (default 1)
+ * runs a batch's pair-level rows and hub segments in one launch (0: two launches; same kernels' arithmetic);
+ * "sweep_seg_grid"
  * (default 0: 64) caps the workgroups that compute the sweeps' hub segments;
  * "overlap" (default 1) runs the evaluation's hub-segment kernel on a second stream of the context, concurrently
  * with the cluster-sum and VCS kernels, joined before the row kernel (identical results; lb_time_eval's per-kernel

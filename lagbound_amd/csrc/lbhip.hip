@@ -1522,6 +1522,21 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step_segs(StepSegArgs a) {
          li += (int64_t)gridDim.x * (LB_BLOCK / 64))
         step_seg_one<TV, R>(a, a.list[li]);
 }
+// A batch's pair-level rows and its hub segments in one launch: the two read disjoint inputs and write disjoint
+// outputs (M, pdiag; segout, segkap). Workgroups [0, npair) take four clusters each (one wavefront per cluster, as
+// lb_k_pair_rows), the rest stride over the batch's hub segments (as lb_k_step_segs).
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_pair_segs(PairRowArgs pa, int npair, int nclu, StepSegArgs a) {
+    if ((int)blockIdx.x < npair) {
+        const int Qg = (int)blockIdx.x * (LB_BLOCK / 64) + (int)(threadIdx.x >> 6);
+        if (Qg < nclu) pair_row_one<R>(pa, Qg, (int)(threadIdx.x & 63));
+        return;
+    }
+    const int64_t l0 = a.lpos[a.b], l1 = a.lpos[a.b + 1];
+    const int64_t nsw = (int64_t)(gridDim.x - npair) * (LB_BLOCK / 64);
+    for (int64_t li = l0 + (int64_t)(blockIdx.x - npair) * (LB_BLOCK / 64) + (threadIdx.x >> 6); li < l1; li += nsw)
+        step_seg_one<TV, R>(a, a.list[li]);
+}
 template <typename TV, int R>
 __device__ inline void step_seg_one(const StepSegArgs &a, int32_t sid) {
     typedef typename VT<TV>::TC TC;
@@ -2983,6 +2998,9 @@ struct lb_ctx {
     // workgroups of the sweeps' hub segments (0: 64); on a 150k-vertex request's search 0.326 s of sweeps at 64
     // against 0.335 s at 16
     int64_t sweep_seg_grid = 0;
+    // option "sweep_pair_segs" 1 (default): a batch's pair-level rows and hub segments share one launch
+    // (lb_k_pair_segs) in place of lb_k_pair_rows + lb_k_step_segs
+    bool sweep_pair_segs = true;
     double rgd_key[5] = {0, 0, 0, 0, 0};
 };
 
@@ -3532,7 +3550,8 @@ struct Launch {
         };
         put(&pa, sizeof(pa)); put(&ga, sizeof(ga)); put(&sa, sizeof(sa)); put(&qa, sizeof(qa)); put(&pb, sizeof(pb));
         put(bpos.data(), sizeof(int64_t) * bpos.size());
-        const int flags[5] = {c->step_wave ? 1 : 0, (int)gsegs, (int)c->sumPpair, c->nlev, fused ? 1 : 0};
+        const int flags[6] = {c->step_wave ? 1 : 0, (int)gsegs, (int)c->sumPpair, c->nlev, fused ? 1 : 0,
+                              c->sweep_pair_segs ? 1 : 0};
         put(flags, sizeof(flags));
         put(&c->stream, sizeof(c->stream)); put(&c->tmask.p, sizeof(void *)); put(&order, sizeof(order));
         put(&c->Y.p, sizeof(void *)); put(&c->Yalt.p, sizeof(void *));
@@ -3540,13 +3559,15 @@ struct Launch {
         for (int b = 0; b < nbatch; b++) {
             const int64_t p0 = bpos[b], nb = bpos[b + 1] - bpos[b];
             if (nb == 0) continue;
-            if (c->nlev) {
-                pa.tmask = (const uint8_t *)c->tmask.p + (int64_t)b * c->sumPpair;
-                hipLaunchKernelGGL(lb_k_pair_rows<R>, dim3(c->sumPpair), dim3(64), 0, c->stream, pa);
-            }
-            if (nsegs) {
-                ga.b = b;
-                hipLaunchKernelGGL((lb_k_step_segs<TV, R>), dim3(gsegs), dim3(LB_BLOCK), 0, c->stream, ga);
+            if (c->nlev) pa.tmask = (const uint8_t *)c->tmask.p + (int64_t)b * c->sumPpair;
+            ga.b = b;
+            if (c->sweep_pair_segs && c->nlev && nsegs) {
+                const int npair = (c->sumPpair + LB_BLOCK / 64 - 1) / (LB_BLOCK / 64);
+                hipLaunchKernelGGL((lb_k_pair_segs<TV, R>), dim3((unsigned)npair + gsegs), dim3(LB_BLOCK), 0, c->stream,
+                                   pa, npair, c->sumPpair, ga);
+            } else {
+                if (c->nlev) hipLaunchKernelGGL(lb_k_pair_rows<R>, dim3(c->sumPpair), dim3(64), 0, c->stream, pa);
+                if (nsegs) hipLaunchKernelGGL((lb_k_step_segs<TV, R>), dim3(gsegs), dim3(LB_BLOCK), 0, c->stream, ga);
             }
             sa.rows = order + p0; sa.nb = nb;
             if (c->step_wave)
@@ -4062,6 +4083,7 @@ int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
     if (std::strcmp(name, "overlap") == 0) { c->overlap = value != 0; return 0; }
     if (std::strcmp(name, "sweep_graph") == 0) { c->sweep_graph = value != 0; return 0; }
     if (std::strcmp(name, "sweep_fused") == 0) { c->sweep_fused = value != 0; return 0; }
+    if (std::strcmp(name, "sweep_pair_segs") == 0) { c->sweep_pair_segs = value != 0; return 0; }
     if (std::strcmp(name, "sweep_seg_grid") == 0) {
         if (value < 0 || value > 4096) return fail(c, -1, "lb_set_option: sweep_seg_grid is 0..4096");
         c->sweep_seg_grid = value;

@@ -173,3 +173,34 @@ def test_fused_sweeps_equal_five_kernel_form(dtype):
     Wo = O.sweeps_c(inst, Ws[1], calls[1], 12, 0.05, damp=0.1)
     otol = {"f64": 1e-10, "f32": 2e-4, "bf16": 5e-2}[dtype]
     assert np.abs(Ws[2] - Wo).max() <= otol
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_pair_segs_launch_equals_separate(dtype):
+    """Option sweep_pair_segs 1 (the default: a batch's pair-level rows and hub segments in one launch) against 0
+    (lb_k_pair_rows and lb_k_step_segs launched apart): the same per-wavefront arithmetic, so V agrees to the
+    run-to-run rounding of the FP64 aggregate adds (1e-12 on FP64 V, 1e-6 on FP32 V). Small hub thresholds (most rows
+    are hubs with several segments of both kinds), two calls of sweeps; the merged form is also checked against the
+    oracle."""
+    from lagbound_amd.engine import Engine
+    inst, V = chung_lu_instance(n=5000, seed=31, k=16, P=16)
+    rng = np.random.default_rng(32)
+    calls = [np.stack([rng.permutation(inst.n) for _ in range(2)]), np.stack([rng.permutation(inst.n)])]
+    out = {}
+    for merged in (1, 0):
+        eng = Engine(r=16, dtype=dtype)
+        for k, v in {"sweep_pair_segs": merged, "sweep_heavy": 8, "sweep_seg": 5}.items():
+            eng.set_option(k, v)
+        eng.set_instance(inst)
+        eng.set_V(V)
+        Ws = [eng.get_V()]
+        print(f"[sweeps] pair_segs={merged} dtype={dtype} n={inst.n}", flush=True)
+        for orders in calls:
+            eng.sweeps(orders, 12, 0.05, damp=0.1)
+            Ws.append(eng.get_V())
+        out[merged] = Ws
+        eng.close()
+    for a, b in zip(out[1][1:], out[0][1:]):
+        assert np.abs(a - b).max() <= (1e-12 if dtype == "f64" else 1e-6)
+    Wo = O.sweeps_c(inst, out[1][1], calls[1], 12, 0.05, damp=0.1)
+    assert np.abs(out[1][2] - Wo).max() <= (1e-10 if dtype == "f64" else 2e-4)
