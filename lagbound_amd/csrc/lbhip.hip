@@ -74,6 +74,9 @@
 #endif
 #define LB_BLOCK 256
 #define LB_ZPART 32        // fused sweeps: copies of z the rows of a batch add into (a power of two)
+// merged post step (lb_k_post_both) only for batches of at most this many rows: each of its workgroups sums the whole
+// batch's increments per VCS cluster in LDS (measured: 1.3 us per batch saved at 76 rows, 3.3 us lost at 282)
+#define LB_POST_MERGE_ROWS 128
 
 typedef uint16_t bf16raw;
 
@@ -1524,18 +1527,27 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step_segs(StepSegArgs a) {
 }
 // A batch's pair-level rows and its hub segments in one launch: the two read disjoint inputs and write disjoint
 // outputs (M, pdiag; segout, segkap). Workgroups [0, npair) take four clusters each (one wavefront per cluster, as
-// lb_k_pair_rows), the rest stride over the batch's hub segments (as lb_k_step_segs).
+// lb_k_pair_rows), the next nsegb stride over the batch's hub segments (as lb_k_step_segs), any others copy cw
+// doubles from csrc to cdst (a snapshot no other part touches).
 template <typename TV, int R>
-__global__ __launch_bounds__(LB_BLOCK) void lb_k_pair_segs(PairRowArgs pa, int npair, int nclu, StepSegArgs a) {
-    if ((int)blockIdx.x < npair) {
-        const int Qg = (int)blockIdx.x * (LB_BLOCK / 64) + (int)(threadIdx.x >> 6);
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_pair_segs(PairRowArgs pa, int npair, int nclu, StepSegArgs a, int nsegb,
+                                                           const double *csrc, double *cdst, int64_t cw) {
+    const int blk = (int)blockIdx.x;
+    if (blk < npair) {
+        const int Qg = blk * (LB_BLOCK / 64) + (int)(threadIdx.x >> 6);
         if (Qg < nclu) pair_row_one<R>(pa, Qg, (int)(threadIdx.x & 63));
         return;
     }
-    const int64_t l0 = a.lpos[a.b], l1 = a.lpos[a.b + 1];
-    const int64_t nsw = (int64_t)(gridDim.x - npair) * (LB_BLOCK / 64);
-    for (int64_t li = l0 + (int64_t)(blockIdx.x - npair) * (LB_BLOCK / 64) + (threadIdx.x >> 6); li < l1; li += nsw)
-        step_seg_one<TV, R>(a, a.list[li]);
+    if (blk < npair + nsegb) {
+        const int64_t l0 = a.lpos[a.b], l1 = a.lpos[a.b + 1];
+        const int64_t nsw = (int64_t)nsegb * (LB_BLOCK / 64);
+        for (int64_t li = l0 + (int64_t)(blk - npair) * (LB_BLOCK / 64) + (threadIdx.x >> 6); li < l1; li += nsw)
+            step_seg_one<TV, R>(a, a.list[li]);
+        return;
+    }
+    // the rest: copy cw doubles (the VCS cluster rows at the batch start, for lb_k_post_both)
+    const int64_t nthr = (int64_t)(gridDim.x - npair - nsegb) * LB_BLOCK;
+    for (int64_t e = (int64_t)(blk - npair - nsegb) * LB_BLOCK + threadIdx.x; e < cw; e += nthr) cdst[e] = csrc[e];
 }
 template <typename TV, int R>
 __device__ inline void step_seg_one(const StepSegArgs &a, int32_t sid) {
@@ -2124,7 +2136,12 @@ struct PostArgs {
 };
 
 // z += sum DD, Y[Q] += DD for every label set (per-block LDS partials, then one atomic per block and entry)
+__device__ inline void post_acc_blk(const PostArgs &a, int blk, int nblk);
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_acc(PostArgs a) {
+    post_acc_blk(a, (int)blockIdx.x, (int)gridDim.x);
+}
+// workgroup blk of nblk of lb_k_post_acc (every thread of the workgroup calls it: it synchronises the workgroup)
+__device__ inline void post_acc_blk(const PostArgs &a, int blk, int nblk) {
     extern __shared__ double sh[];
     const int width = a.R + a.sumP * a.R;
     if (a.use_lds) {
@@ -2132,8 +2149,8 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_acc(PostArgs a) {
         __syncthreads();
     }
     const int c = threadIdx.x % a.R, rs = threadIdx.x / a.R, nrs = LB_BLOCK / a.R;
-    const int64_t per = (a.nb + gridDim.x - 1) / gridDim.x;
-    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = b0 + per < a.nb ? b0 + per : a.nb;
+    const int64_t per = (a.nb + nblk - 1) / nblk;
+    const int64_t b0 = (int64_t)blk * per, b1 = b0 + per < a.nb ? b0 + per : a.nb;
     double zc = 0.0;
     for (int64_t p = b0 + rs; p < b1; p += nrs) {
         const double x = a.DD[p * a.R + c];
@@ -2159,6 +2176,10 @@ struct Post2Args {
     const int64_t *ptr[LB_MAXVCS]; const int32_t *col; const void *sig; const double *Y; double wv, rho;
     double *stZ; double *Z; double *cnt;
     int sumPv; int use_lds;     // Z / cnt increments gathered per workgroup in LDS first (sumPv (R + 1) doubles)
+    // dd set (lb_k_post_both): Y holds the VCS cluster rows at the batch start (Yoff relative to it) and every
+    // workgroup adds the batch's increments dd (batch-position indexed, R each) per VCS cluster in LDS (sumPv R more
+    // doubles), so the step reads the cluster rows after the batch without waiting for the aggregate pass
+    const double *dd;
 };
 
 // vcs_post: new effective multipliers SN of the moved rows (new v_i, updated Y), Z_R += d_i (SN v_new - stZ v_old),
@@ -2268,8 +2289,19 @@ __device__ inline void post_vcs_wave_blk(const Post2Args &a, int blk) {
     const int wz = a.sumPv * R;
     double *Zacc = a.use_lds ? shZc : a.Z;
     double *Cacc = a.use_lds ? shZc + wz : a.cnt;
+    double *dY = shZc + wz + a.sumPv;                   // with dd: the batch's increments of the VCS cluster rows
     if (a.use_lds) {
-        for (int e = threadIdx.x; e < wz + a.sumPv; e += LB_BLOCK) shZc[e] = 0.0;
+        for (int e = threadIdx.x; e < wz + a.sumPv + (a.dd ? wz : 0); e += LB_BLOCK) shZc[e] = 0.0;
+        __syncthreads();
+    }
+    if (a.dd) {
+        for (int64_t t = threadIdx.x; t < a.nb * R; t += LB_BLOCK) {
+            const int64_t p = t / R;
+            const int c = (int)(t - p * R);
+            const double x = a.dd[t];
+            const int32_t i = a.rows[p];
+            for (int m = 0; m < a.nvcs; m++) atomicAdd(dY + (int64_t)(a.Zoff[m] + a.lab[m][i]) * R + c, x);
+        }
         __syncthreads();
     }
     if (pos < a.nb) {                                  // wavefront-uniform
@@ -2287,9 +2319,10 @@ __device__ inline void post_vcs_wave_blk(const Post2Args &a, int blk) {
                 const int Rc = a.col[e];
                 const double sq = (double)sig[e], so = a.stZ[e];
                 const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
+                const double *dR = dY + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
                 double s = 0.0;
 #pragma unroll
-                for (int c = 0; c < CPL; c++) s += (double)xn[c] * YR[c];
+                for (int c = 0; c < CPL; c++) s += (double)xn[c] * (a.dd ? YR[c] + dR[c] : YR[c]);
                 s = slot_sum<S>(s);
                 double sn = sq;
                 if (a.rho > 0.0 && di > 0.0) {
@@ -2352,6 +2385,16 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_sweep_pre(Post2Args pb, int npo
         }
         if (zdst) zdst[e] = v;
     }
+}
+
+// A batch's aggregate increments and its VCS post step in one launch: workgroups [0, npa) run lb_k_post_acc (z and Y
+// += the batch's increments, FP64 atomics), the rest lb_k_post_vcs_wave with pb.dd set (the VCS cluster rows after the
+// batch formed in LDS from the batch-start snapshot and the same increments). The two write disjoint memory and the
+// second reads none of what the first writes.
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_post_both(PostArgs qa, int npa, Post2Args pb) {
+    if ((int)blockIdx.x < npa) post_acc_blk(qa, (int)blockIdx.x, npa);
+    else post_vcs_wave_blk<TV, R>(pb, (int)blockIdx.x - npa);
 }
 
 // ------------------------------------------------------------------------------------------------ triangle separation
@@ -3001,6 +3044,12 @@ struct lb_ctx {
     // option "sweep_pair_segs" 1 (default): a batch's pair-level rows and hub segments share one launch
     // (lb_k_pair_segs) in place of lb_k_pair_rows + lb_k_step_segs
     bool sweep_pair_segs = true;
+    // option "sweep_post_merge" 1 (default, with sweep_pair_segs and step_wave): a batch's aggregate increments and
+    // VCS post step in one launch (lb_k_post_both), the VCS cluster rows at the batch start copied to Ysnap by the
+    // batch's first launch; used for batches of at most LB_POST_MERGE_ROWS rows where the post step's LDS
+    // (sumPv (2R + 1) doubles) fits 48 KB
+    bool sweep_post_merge = true;
+    DBuf Ysnap;
     double rgd_key[5] = {0, 0, 0, 0, 0};
 };
 
@@ -3550,21 +3599,39 @@ struct Launch {
         };
         put(&pa, sizeof(pa)); put(&ga, sizeof(ga)); put(&sa, sizeof(sa)); put(&qa, sizeof(qa)); put(&pb, sizeof(pb));
         put(bpos.data(), sizeof(int64_t) * bpos.size());
-        const int flags[6] = {c->step_wave ? 1 : 0, (int)gsegs, (int)c->sumPpair, c->nlev, fused ? 1 : 0,
-                              c->sweep_pair_segs ? 1 : 0};
+        const int flags[7] = {c->step_wave ? 1 : 0, (int)gsegs, (int)c->sumPpair, c->nlev, fused ? 1 : 0,
+                              c->sweep_pair_segs ? 1 : 0, c->sweep_post_merge ? 1 : 0};
         put(flags, sizeof(flags));
         put(&c->stream, sizeof(c->stream)); put(&c->tmask.p, sizeof(void *)); put(&order, sizeof(order));
-        put(&c->Y.p, sizeof(void *)); put(&c->Yalt.p, sizeof(void *));
+        put(&c->Y.p, sizeof(void *)); put(&c->Yalt.p, sizeof(void *)); put(&c->Ysnap.p, sizeof(void *));
+        put(&c->DDb.p, sizeof(void *));
+        // merged post step: the VCS cluster rows at each batch start copied to Ysnap by the batch's first launch
+        const size_t ldsV2 = ldsV + sizeof(double) * (size_t)c->sumPv * R;
+        const bool pmerge = c->sweep_post_merge && c->sweep_pair_segs && c->step_wave && c->sumPv > 0 && pb.use_lds &&
+                            ldsV2 <= 48 * 1024;
+        const int64_t cwv = (int64_t)c->sumPv * R;
+        const unsigned ncopy = pmerge ? (unsigned)std::max<int64_t>(1, std::min<int64_t>(8, (cwv + 1023) / 1024)) : 0u;
+        Post2Args pbm = pb;
+        if (pmerge) {
+            pbm.Y = (const double *)c->Ysnap.p;
+            for (int m = 0; m < c->nvcs; m++) pbm.Yoff[m] = Zoff[m];
+            pbm.dd = (const double *)c->DDb.p;
+        }
+        const size_t ldsBoth = std::max(qa.use_lds ? ldsP : (size_t)0, ldsV2);
         auto batches = [&](PairRowArgs pa, StepSegArgs ga, StepArgs sa, PostArgs qa, Post2Args pb) {
         for (int b = 0; b < nbatch; b++) {
             const int64_t p0 = bpos[b], nb = bpos[b + 1] - bpos[b];
             if (nb == 0) continue;
             if (c->nlev) pa.tmask = (const uint8_t *)c->tmask.p + (int64_t)b * c->sumPpair;
             ga.b = b;
-            if (c->sweep_pair_segs && c->nlev && nsegs) {
-                const int npair = (c->sumPpair + LB_BLOCK / 64 - 1) / (LB_BLOCK / 64);
-                hipLaunchKernelGGL((lb_k_pair_segs<TV, R>), dim3((unsigned)npair + gsegs), dim3(LB_BLOCK), 0, c->stream,
-                                   pa, npair, c->sumPpair, ga);
+            const bool pm = pmerge && nb <= LB_POST_MERGE_ROWS;
+            if (c->sweep_pair_segs && ((c->nlev && nsegs) || pm)) {
+                const int npair = c->nlev ? (c->sumPpair + LB_BLOCK / 64 - 1) / (LB_BLOCK / 64) : 0;
+                const int nsegb = nsegs ? (int)gsegs : 0;
+                hipLaunchKernelGGL((lb_k_pair_segs<TV, R>), dim3((unsigned)(npair + nsegb) + (pm ? ncopy : 0u)), dim3(LB_BLOCK),
+                                   0, c->stream, pa, npair, c->sumPpair, ga, nsegb,
+                                   (const double *)(Yp + (int64_t)c->sumPpair * R), pm ? (double *)c->Ysnap.p : nullptr,
+                                   pm ? cwv : (int64_t)0);
             } else {
                 if (c->nlev) hipLaunchKernelGGL(lb_k_pair_rows<R>, dim3(c->sumPpair), dim3(64), 0, c->stream, pa);
                 if (nsegs) hipLaunchKernelGGL((lb_k_step_segs<TV, R>), dim3(gsegs), dim3(LB_BLOCK), 0, c->stream, ga);
@@ -3576,6 +3643,13 @@ struct Launch {
                 hipLaunchKernelGGL((lb_k_step<TV, R>), dim3((unsigned)((nb + NSL - 1) / NSL)), dim3(LB_BLOCK), 0, c->stream, sa);
             qa.rows = order + p0; qa.nb = nb;
             const unsigned pg = (unsigned)std::min<int64_t>(64, (nb + LB_POST_ROWS - 1) / LB_POST_ROWS);
+            if (pm) {
+                Post2Args pbb = pbm;
+                pbb.rows = order + p0; pbb.nb = nb;
+                hipLaunchKernelGGL((lb_k_post_both<TV, R>), dim3(pg + (unsigned)((nb + 3) / 4)), dim3(LB_BLOCK), ldsBoth,
+                                   c->stream, qa, (int)pg, pbb);
+                continue;
+            }
             hipLaunchKernelGGL(lb_k_post_acc, dim3(pg), dim3(LB_BLOCK), qa.use_lds ? ldsP : 0, c->stream, qa);
             pb.rows = order + p0; pb.nb = nb;
             if (c->step_wave)
@@ -4056,7 +4130,7 @@ void lb_destroy(lb_ctx *c) {
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
                     &c->hTseg0, &c->parts, &c->lamsum, &c->order, &c->tmask, &c->Kq, &c->stZ, &c->Zs, &c->cnts,
                     &c->Vstage, &c->DDb, &c->segkap, &c->seglist, &c->vslack, &c->vage, &c->vvol, &c->rorder,
-                    &c->Yalt, &c->hubcnt, &c->hubpos, &c->hubown, &c->zpart,
+                    &c->Yalt, &c->hubcnt, &c->hubpos, &c->hubown, &c->zpart, &c->Ysnap,
                     &c->b_off, &c->b_gid, &c->b_L, &c->b_C0, &c->b_Poff, &c->b_soff, &c->b_cg, &c->b_lab, &c->b_sig,
                     &c->b_pp, &c->b_rq, &c->b_dsq, &c->b_zY, &c->b_M, &c->b_pdiag, &c->b_ltot, &c->b_lviol, &c->b_rowv,
                     &c->b_parts, &c->b_lamsum, &c->b_tptr, &c->b_tidx, &c->b_R, &c->b_Vp, &c->b_Rp, &c->b_rgst};
@@ -4084,6 +4158,7 @@ int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
     if (std::strcmp(name, "sweep_graph") == 0) { c->sweep_graph = value != 0; return 0; }
     if (std::strcmp(name, "sweep_fused") == 0) { c->sweep_fused = value != 0; return 0; }
     if (std::strcmp(name, "sweep_pair_segs") == 0) { c->sweep_pair_segs = value != 0; return 0; }
+    if (std::strcmp(name, "sweep_post_merge") == 0) { c->sweep_post_merge = value != 0; return 0; }
     if (std::strcmp(name, "sweep_seg_grid") == 0) {
         if (value < 0 || value > 4096) return fail(c, -1, "lb_set_option: sweep_seg_grid is 0..4096");
         c->sweep_seg_grid = value;
@@ -4590,6 +4665,7 @@ int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap
         if ((rc = dalloc(c, c->hubown, sizeof(double) * nhub * (size_t)(R + 1)))) return rc;
         HIPCHK(c, hipMemsetAsync(c->hubcnt.p, 0, sizeof(int32_t) * nhub, c->stream));
         if ((rc = dalloc(c, c->zpart, sizeof(double) * LB_ZPART * (size_t)R))) return rc;
+        if ((rc = dalloc(c, c->Ysnap, sizeof(double) * (size_t)std::max(c->sumPv, 1) * R))) return rc;
         HIPCHK(c, hipMemsetAsync(c->zpart.p, 0, sizeof(double) * LB_ZPART * (size_t)R, c->stream));
     }
     std::vector<int32_t> o32(n);

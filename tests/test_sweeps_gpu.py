@@ -175,13 +175,16 @@ def test_fused_sweeps_equal_five_kernel_form(dtype):
     assert np.abs(Ws[2] - Wo).max() <= otol
 
 
+@pytest.mark.parametrize("nbatch", [12, 64])
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-def test_pair_segs_launch_equals_separate(dtype):
-    """Option sweep_pair_segs 1 (the default: a batch's pair-level rows and hub segments in one launch) against 0
-    (lb_k_pair_rows and lb_k_step_segs launched apart): the same per-wavefront arithmetic, so V agrees to the
-    run-to-run rounding of the FP64 aggregate adds (1e-12 on FP64 V, 1e-6 on FP32 V). Small hub thresholds (most rows
-    are hubs with several segments of both kinds), two calls of sweeps; the merged form is also checked against the
-    oracle."""
+def test_pair_segs_launch_equals_separate(dtype, nbatch):
+    """Options sweep_pair_segs and sweep_post_merge 1 (the defaults: three launches per batch -- pair-level rows, hub
+    segments and the VCS cluster-row snapshot; the row steps; aggregate increments with the VCS post step, which forms
+    the cluster rows after the batch from the snapshot and the batch's increments) against 0 (five launches): the same
+    per-wavefront arithmetic, so V agrees to the run-to-run rounding of the FP64 aggregate adds (1e-12 on FP64 V,
+    1e-6 on FP32 V). Small hub thresholds (most rows are hubs with several segments of both kinds), two calls of
+    sweeps at 12 batches (417 rows: the post step stays in two launches) and 64 (78 rows: one launch); the merged
+    form is also checked against the oracle."""
     from lagbound_amd.engine import Engine
     inst, V = chung_lu_instance(n=5000, seed=31, k=16, P=16)
     rng = np.random.default_rng(32)
@@ -189,18 +192,18 @@ def test_pair_segs_launch_equals_separate(dtype):
     out = {}
     for merged in (1, 0):
         eng = Engine(r=16, dtype=dtype)
-        for k, v in {"sweep_pair_segs": merged, "sweep_heavy": 8, "sweep_seg": 5}.items():
+        for k, v in {"sweep_pair_segs": merged, "sweep_post_merge": merged, "sweep_heavy": 8, "sweep_seg": 5}.items():
             eng.set_option(k, v)
         eng.set_instance(inst)
         eng.set_V(V)
         Ws = [eng.get_V()]
-        print(f"[sweeps] pair_segs={merged} dtype={dtype} n={inst.n}", flush=True)
+        print(f"[sweeps] pair_segs={merged} dtype={dtype} n={inst.n} nbatch={nbatch}", flush=True)
         for orders in calls:
-            eng.sweeps(orders, 12, 0.05, damp=0.1)
+            eng.sweeps(orders, nbatch, 0.05, damp=0.1)
             Ws.append(eng.get_V())
         out[merged] = Ws
         eng.close()
     for a, b in zip(out[1][1:], out[0][1:]):
         assert np.abs(a - b).max() <= (1e-12 if dtype == "f64" else 1e-6)
-    Wo = O.sweeps_c(inst, out[1][1], calls[1], 12, 0.05, damp=0.1)
+    Wo = O.sweeps_c(inst, out[1][1], calls[1], nbatch, 0.05, damp=0.1)
     assert np.abs(out[1][2] - Wo).max() <= (1e-10 if dtype == "f64" else 2e-4)
