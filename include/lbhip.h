@@ -161,7 +161,7 @@ int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int
  * cluster sums itself -- in place of five (same arithmetic; the sums' FP64 adds in another order); off by default
  * since one serving run with it ended in a GPU memory fault of unestablished cause; "sweep_pair_segs" (default 1)
  * runs a batch's pair-level rows and hub segments in one launch (0: two launches; same kernels' arithmetic);
- * "sweep_post_merge" (default 1) runs the aggregate increments and the VCS post step of batches of at most 128 rows
+ * "sweep_post_merge" (default 1) runs the aggregate increments and the VCS post step of batches of at most 256 rows
  * in one launch, the post step forming the VCS cluster rows after the batch from a snapshot and the batch's
  * increments (0: two launches);
  * "sweep_seg_grid"

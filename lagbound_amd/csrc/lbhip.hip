@@ -75,8 +75,10 @@
 #define LB_BLOCK 256
 #define LB_ZPART 32        // fused sweeps: copies of z the rows of a batch add into (a power of two)
 // merged post step (lb_k_post_both) only for batches of at most this many rows: each of its workgroups sums the whole
-// batch's increments per VCS cluster in LDS (measured: 1.3 us per batch saved at 76 rows, 3.3 us lost at 282)
-#define LB_POST_MERGE_ROWS 128
+// batch's increments per VCS cluster in LDS, one batch row per thread, so larger batches take a second pass
+// (measured on a 40k-vertex request, 76 rows: 0.304 s of sweeps against 0.321 s in two post launches; on a
+// 150k-vertex request, 282 rows: 0.400 s against 0.389 s)
+#define LB_POST_MERGE_ROWS LB_BLOCK
 
 typedef uint16_t bf16raw;
 
@@ -2295,12 +2297,17 @@ __device__ inline void post_vcs_wave_blk(const Post2Args &a, int blk) {
         __syncthreads();
     }
     if (a.dd) {
-        for (int64_t t = threadIdx.x; t < a.nb * R; t += LB_BLOCK) {
-            const int64_t p = t / R;
-            const int c = (int)(t - p * R);
-            const double x = a.dd[t];
+        // one batch row per thread: its cluster per VCS level, then its R increments added there
+        for (int64_t p = threadIdx.x; p < a.nb; p += LB_BLOCK) {
             const int32_t i = a.rows[p];
-            for (int m = 0; m < a.nvcs; m++) atomicAdd(dY + (int64_t)(a.Zoff[m] + a.lab[m][i]) * R + c, x);
+            double xv[R];
+#pragma unroll
+            for (int c = 0; c < R; c++) xv[c] = a.dd[p * R + c];
+            for (int m = 0; m < a.nvcs; m++) {
+                double *dst = dY + (int64_t)(a.Zoff[m] + a.lab[m][i]) * R;
+#pragma unroll
+                for (int c = 0; c < R; c++) atomicAdd(dst + c, xv[c]);
+            }
         }
         __syncthreads();
     }
@@ -3046,7 +3053,7 @@ struct lb_ctx {
     bool sweep_pair_segs = true;
     // option "sweep_post_merge" 1 (default, with sweep_pair_segs and step_wave): a batch's aggregate increments and
     // VCS post step in one launch (lb_k_post_both), the VCS cluster rows at the batch start copied to Ysnap by the
-    // batch's first launch; used for batches of at most LB_POST_MERGE_ROWS rows where the post step's LDS
+    // batch's first launch; used for batches of at most LB_POST_MERGE_ROWS (256) rows where the post step's LDS
     // (sumPv (2R + 1) doubles) fits 48 KB
     bool sweep_post_merge = true;
     DBuf Ysnap;
