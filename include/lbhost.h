@@ -33,6 +33,14 @@ int lbh_argsort_i64(int64_t m, const int64_t *keys, int64_t *perm_out);
 // (idx_out, ptr[n] entries). Returns the entry count, or -1 on bad arguments.
 int64_t lbh_row_gather(int64_t n, const int64_t *ptr, const int64_t *order, int64_t *nptr_out, int64_t *idx_out);
 
+// COO triplets (r, c, v; m of them, indices in [0, N)) as a CSR with rows and, inside a row, columns ascending and
+// duplicates combined in input order: indptr_out (N + 1), indices_out and, per stored entry, the sum of its values,
+// the sum of their absolute values and their count (each output may be null; v null: every value 1). Outputs hold
+// up to m entries. Returns the number of stored entries, or -1 on bad arguments. Used for the lifted certificate's
+// assembly (with the counts and absolute sums of its rounding price), its symmetric copy and the vertex graph.
+int64_t lbh_coo_sum(int64_t N, int64_t m, const int64_t *r, const int64_t *c, const double *v, int64_t *indptr_out,
+                    int64_t *indices_out, double *sum_out, double *abs_out, double *cnt_out);
+
 // Order of VCS separation candidates given by descending score sc (m entries): perm_out = the permutation that sorts
 // every run of equal scores by (vi, vR, position) ascending and keeps the runs in place -- np.lexsort((vR, vi, -sc)) for
 // descending sc. Returns 0, 1 if sc is not descending (nothing written: sort in full), -1 on bad arguments.

@@ -10,7 +10,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "liblbhost.so")
 EXPORTS = ["lbh_ws_merge", "lbh_take_f64", "lbh_take_u8", "lbh_new_triangles", "lbh_argsort_bounded",
-           "lbh_argsort_i64", "lbh_tie_order", "lbh_row_gather"]
+           "lbh_argsort_i64", "lbh_tie_order", "lbh_row_gather", "lbh_coo_sum"]
 
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
@@ -37,6 +37,8 @@ def load():
         L.lbh_tie_order.argtypes = [C.c_int64, _f64p, _i64p, _i64p, _i64p]
         L.lbh_row_gather.argtypes = [C.c_int64, _i64p, _i64p, _i64p, _i64p]
         L.lbh_row_gather.restype = C.c_int64
+        L.lbh_coo_sum.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p, _f64p, _f64p]
+        L.lbh_coo_sum.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -132,3 +134,26 @@ def row_gather(ptr, order):
     if w < 0:
         raise ValueError("lbh_row_gather: bad arguments")
     return nptr, idx[:w]
+
+
+def coo_sum(N, r, c, v=None, sums=True, abs_sums=False, counts=False):
+    """(indptr, indices, sum, abs_sum, count) of COO triplets as a CSR with sorted rows and columns, duplicates combined
+    in input order (lbh_coo_sum); the value arrays not asked for are None."""
+    r = np.ascontiguousarray(r, np.int64)
+    c = np.ascontiguousarray(c, np.int64)
+    m = len(r)
+    if len(c) != m or (v is not None and len(v) != m):
+        raise ValueError("coo_sum: r, c and v must have one length")
+    vp = None if v is None else np.ascontiguousarray(v, np.float64)
+    indptr = np.empty(int(N) + 1, np.int64)
+    idx = np.empty(m, np.int64)
+    sv = np.empty(m) if sums else None
+    sa = np.empty(m) if abs_sums else None
+    sn = np.empty(m) if counts else None
+    w = load().lbh_coo_sum(int(N), m, _p(r, _i64p), _p(c, _i64p), None if vp is None else _p(vp, _f64p),
+                           _p(indptr, _i64p), _p(idx, _i64p), None if sv is None else _p(sv, _f64p),
+                           None if sa is None else _p(sa, _f64p), None if sn is None else _p(sn, _f64p))
+    if w < 0:
+        raise ValueError("lbh_coo_sum: an index out of range")
+    cut = (lambda a: None if a is None else a[:w])
+    return indptr, idx[:w], cut(sv), cut(sa), cut(sn)

@@ -3,6 +3,7 @@
 #include "../../include/lbhost.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -155,6 +156,55 @@ int64_t lbh_row_gather(int64_t n, const int64_t *ptr, const int64_t *order, int6
         for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) idx_out[o++] = e;
     }
     return o;
+}
+
+int64_t lbh_coo_sum(int64_t N, int64_t m, const int64_t *r, const int64_t *c, const double *v, int64_t *indptr_out,
+                    int64_t *indices_out, double *sum_out, double *abs_out, double *cnt_out) {
+    if (N < 0 || m < 0 || !indptr_out || (m && (!r || !c || !indices_out))) return -1;
+    std::vector<int64_t> at((size_t)N + 1, 0);
+    for (int64_t t = 0; t < m; t++) {
+        if (r[t] < 0 || r[t] >= N || c[t] < 0 || c[t] >= N) return -1;
+        at[(size_t)r[t] + 1]++;
+    }
+    for (int64_t i = 0; i < N; i++) at[(size_t)i + 1] += at[(size_t)i];
+    // (column, input position) of every entry, bucketed by row in input order
+    std::vector<std::pair<int64_t, int64_t>> cp((size_t)m);
+    {
+        std::vector<int64_t> fill(at.begin(), at.end() - 1);
+        for (int64_t t = 0; t < m; t++) cp[(size_t)fill[(size_t)r[t]]++] = {c[t], t};
+    }
+    int64_t w = 0;
+    indptr_out[0] = 0;
+    for (int64_t i = 0; i < N; i++) {
+        std::pair<int64_t, int64_t> *b = cp.data() + at[(size_t)i], *e = cp.data() + at[(size_t)i + 1];
+        if (e - b <= 32) {                                // short rows: insertion sort, equal columns keep their order
+            for (std::pair<int64_t, int64_t> *q = b + 1; q < e; q++) {
+                const std::pair<int64_t, int64_t> x = *q;
+                std::pair<int64_t, int64_t> *y = q;
+                while (y > b && (y - 1)->first > x.first) { *y = *(y - 1); y--; }
+                *y = x;
+            }
+        } else {
+            std::sort(b, e);                              // by column, then input position
+        }
+        for (std::pair<int64_t, int64_t> *q = b; q < e;) {
+            const int64_t col = q->first;
+            double sv = 0.0, sa = 0.0, sn = 0.0;
+            for (; q < e && q->first == col; q++) {       // duplicates summed in input order
+                const double x = v ? v[q->second] : 1.0;
+                sv += x;
+                sa += std::fabs(x);
+                sn += 1.0;
+            }
+            indices_out[w] = col;
+            if (sum_out) sum_out[w] = sv;
+            if (abs_out) abs_out[w] = sa;
+            if (cnt_out) cnt_out[w] = sn;
+            w++;
+        }
+        indptr_out[i + 1] = w;
+    }
+    return w;
 }
 
 int lbh_tie_order(int64_t m, const double *sc, const int64_t *vi, const int64_t *vR, int64_t *perm_out) {

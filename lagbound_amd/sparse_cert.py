@@ -31,6 +31,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from .bound import check_multipliers, exact_sum, frac_up, gamma
+from . import _host as HL
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CERTLIB = os.path.join(HERE, "liblbcert.so")
@@ -237,34 +238,29 @@ class Lifted:
         if not np.isfinite(v).all():
             raise ValueError("non-finite term in the lifted matrix")
         it = np.int32 if N < 2 ** 31 - 1 else np.int64
-        r, c = r.astype(it), c.astype(it)
-        M = sp.csr_matrix((v, (r, c)), shape=(N, N))
-        M.sum_duplicates()
-        M.sort_indices()
-        Ab = sp.csr_matrix((np.abs(v), (r, c)), shape=(N, N))
-        Ab.sum_duplicates()
-        Ab.sort_indices()
-        H = sp.csr_matrix((np.ones(len(v)), (r, c)), shape=(N, N))
-        H.sum_duplicates()
-        H.sort_indices()
-        if not (np.array_equal(M.indptr, Ab.indptr) and np.array_equal(M.indices, Ab.indices)):
-            raise RuntimeError("assembly patterns differ")
-        rows = np.repeat(np.arange(N), np.diff(M.indptr))
-        mult = np.where(rows == M.indices, 1.0, 2.0)
-        hmax = float(H.data.max())
-        gam = (H.data + 5.0) * U64 / (1.0 - (H.data + 5.0) * U64)
-        price = float(np.sum(mult * gam * Ab.data)) * (1.0 + 4.0 * len(Ab.data) * U64) * 1.0001
+        # one pass (liblbhost): the entries' sums, the sums of their terms' absolute values and their term counts
+        indptr, indices, sm, sab, cnt = HL.coo_sum(N, r, c, v, abs_sums=True, counts=True)
+        M = sp.csr_matrix((sm, indices.astype(it), indptr.astype(it)), shape=(N, N))
+        rows = np.repeat(np.arange(N), np.diff(indptr))
+        mult = np.where(rows == indices, 1.0, 2.0)
+        hmax = float(cnt.max())
+        gam = (cnt + 5.0) * U64 / (1.0 - (cnt + 5.0) * U64)
+        price = float(np.sum(mult * gam * sab)) * (1.0 + 4.0 * len(sab) * U64) * 1.0001
         return M, price, hmax
 
 
 def full_symmetric(Ml):
-    """The full symmetric CSR (both triangles and the diagonal) of a lower-triangle CSR, without arithmetic."""
+    """The full symmetric CSR (both triangles and the diagonal) of a lower-triangle CSR, without arithmetic (every
+    position is stored once, so the combine step of liblbhost's conversion adds nothing to a value)."""
     Mc = Ml.tocoo()
     off = Mc.row != Mc.col
-    F = sp.csr_matrix((np.r_[Mc.data, Mc.data[off]], (np.r_[Mc.row, Mc.col[off]], np.r_[Mc.col, Mc.row[off]])),
-                      shape=Ml.shape)
-    F.sort_indices()
-    return F
+    N = Ml.shape[0]
+    indptr, indices, sm, _, _ = HL.coo_sum(N, np.r_[Mc.row, Mc.col[off]], np.r_[Mc.col, Mc.row[off]],
+                                           np.r_[Mc.data, Mc.data[off]])
+    if len(indices) != len(Mc.data) + int(off.sum()):
+        raise ValueError("full_symmetric: a position stored twice in the lower triangle")
+    it = np.int32 if N < 2 ** 31 - 1 else np.int64
+    return sp.csr_matrix((sm, indices.astype(it), indptr.astype(it)), shape=Ml.shape)
 
 
 def vertex_key(inst, dense_deg=1000, core_deg=256):
@@ -279,13 +275,11 @@ def vertex_key(inst, dense_deg=1000, core_deg=256):
         r += [TI[:, 0], TI[:, 2]]
         c += [TI[:, 2], TI[:, 0]]
     r, c = np.concatenate(r), np.concatenate(c)
-    G = sp.csr_matrix((np.ones(len(r), np.int8), (r, c)), shape=(n, n))
-    G = (G + G.T).tocsr()
-    G.sort_indices()
+    # the symmetric pattern of the pairs, rows and columns sorted (liblbhost: one pass, no values)
+    ptr, idx64, _, _, _ = HL.coo_sum(n, np.r_[r, c], np.r_[c, r], sums=False)
     order = np.zeros(n, np.int64)
     ns = np.zeros(1, np.int64)
-    ptr = G.indptr.astype(np.int64)
-    idx = G.indices.astype(np.int32)
+    idx = idx64.astype(np.int32)
     rc = _cert_lib().lbc_md_order(n, _p(ptr, _i64p), _p(idx, _i32p), dense_deg, core_deg, _p(order, _i64p),
                                   _p(ns, _i64p))
     if rc:

@@ -207,3 +207,27 @@ def test_row_gather_equals_numpy():
         assert np.array_equal(a, nptr) and np.array_equal(b, idx)
     with pytest.raises(ValueError):
         H.row_gather(np.array([0, 1, 2], np.int64), np.array([0, 2], np.int64))
+
+
+def test_coo_sum_equals_scipy():
+    """liblbhost's COO -> CSR conversion with duplicates combined against scipy's (sum_duplicates + sort_indices):
+    the same pattern, sums equal up to the order of the adds, absolute sums and counts, and pattern-only input;
+    an index out of range is an error."""
+    import scipy.sparse as sp
+    from lagbound_amd import _host as H
+    rng = np.random.default_rng(14)
+    for N, m in ((1, 0), (7, 40), (300, 5000)):
+        r = rng.integers(0, N, m)
+        c = rng.integers(0, N, m)
+        v = rng.standard_normal(m)
+        ip, ix, s, a, n = H.coo_sum(N, r, c, v, abs_sums=True, counts=True)
+        for data, got in ((v, s), (np.abs(v), a), (np.ones(m), n)):
+            M = sp.csr_matrix((data, (r, c)), shape=(N, N))
+            M.sum_duplicates()
+            M.sort_indices()
+            assert np.array_equal(ip, M.indptr) and np.array_equal(ix, M.indices)
+            assert np.allclose(got, M.data, rtol=1e-13, atol=1e-13)
+        ip2, ix2, none, _, _ = H.coo_sum(N, r, c, sums=False)
+        assert none is None and np.array_equal(ip2, ip) and np.array_equal(ix2, ix)
+    with pytest.raises(ValueError):
+        H.coo_sum(3, np.array([0, 3]), np.array([0, 1]))
