@@ -30,7 +30,8 @@ int lbh_argsort_i64(int64_t m, const int64_t *keys, int64_t *perm_out);
 
 // Rows of a CSR (n rows, ptr) taken in `order` (a permutation of 0..n-1), each row's entries kept in their order:
 // the new row pointers (nptr_out, n + 1) and, for each entry of the new layout, its entry index in the old one
-// (idx_out, ptr[n] entries). Returns the entry count, or -1 on bad arguments.
+// (idx_out, ptr[n] - ptr[0] entries). Returns the entry count, or -1 on bad arguments (an order entry out of range,
+// or a total that differs from ptr[n] - ptr[0], as a repeated row makes it; idx_out is then not written).
 int64_t lbh_row_gather(int64_t n, const int64_t *ptr, const int64_t *order, int64_t *nptr_out, int64_t *idx_out);
 
 // COO triplets (r, c, v; m of them, indices in [0, N)) as a CSR with rows and, inside a row, columns ascending and

@@ -149,7 +149,8 @@ int64_t lbh_row_gather(int64_t n, const int64_t *ptr, const int64_t *order, int6
         if (i < 0 || i >= n || ptr[i + 1] < ptr[i]) return -1;
         nptr_out[k + 1] = nptr_out[k] + (ptr[i + 1] - ptr[i]);
     }
-    if (nptr_out[n] && !idx_out) return -1;
+    // a repeated row (order not a permutation) would write past the caller's ptr[n] - ptr[0] entries
+    if (nptr_out[n] != ptr[n] - ptr[0] || (nptr_out[n] && !idx_out)) return -1;
     int64_t o = 0;
     for (int64_t k = 0; k < n; k++) {
         const int64_t i = order[k];

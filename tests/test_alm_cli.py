@@ -207,6 +207,8 @@ def test_row_gather_equals_numpy():
         assert np.array_equal(a, nptr) and np.array_equal(b, idx)
     with pytest.raises(ValueError):
         H.row_gather(np.array([0, 1, 2], np.int64), np.array([0, 2], np.int64))
+    with pytest.raises(ValueError):                      # a repeated row: more entries than the CSR holds
+        H.row_gather(np.array([0, 1, 3], np.int64), np.array([1, 1], np.int64))
 
 
 def test_coo_sum_equals_scipy():
