@@ -3035,8 +3035,9 @@ struct lb_ctx {
     void *rgd_exec = nullptr;                    // captured search iteration (hipGraphExec_t) and its parameters
     // option "sweep_graph" 1: a sweep's per-batch launches captured as one graph (hipGraphExec_t), replayed while the
     // launch arguments (the key: every argument block, the batch bounds and grids) are unchanged. Off by default: on
-    // a 150k-vertex request (512 batches per sweep, 24 sweeps) the replayed sweeps took 0.49 s against 0.44 s eager,
-    // since a batch is bound by its five kernels' own durations (about 33 of its 36 us), not by the launches.
+    // a 150k-vertex request (512 batches per sweep, 24 sweeps) the replayed sweeps took 0.49 s against 0.44 s eager
+    // (measured with five launches per batch), since a batch is bound by its kernels' own durations (about 33 of its
+    // 36 us), not by the launches.
     void *sw_exec = nullptr; std::vector<unsigned char> sw_key; bool sweep_graph = false;
     DBuf lposd;                                  // device copy of the sweep's per-batch segment-list bounds
     // option "sweep_fused" 1 (with step_wave): two launches per batch (lb_k_sweep_pre, lb_k_step_fused) in place of
