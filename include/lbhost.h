@@ -42,6 +42,12 @@ int64_t lbh_row_gather(int64_t n, const int64_t *ptr, const int64_t *order, int6
 int64_t lbh_coo_sum(int64_t N, int64_t m, const int64_t *r, const int64_t *c, const double *v, int64_t *indptr_out,
                     int64_t *indices_out, double *sum_out, double *abs_out, double *cnt_out);
 
+// The full symmetric CSR of a lower-triangular CSR (N rows; columns of a row strictly ascending and at most the row
+// index), with no arithmetic: rows and columns ascending, every off-diagonal value stored twice. Outputs hold
+// 2 ptr[N] entries at most. Returns the entry count, or -1 if the input is not such a lower triangle.
+int64_t lbh_sym_from_lower(int64_t N, const int64_t *ptr, const int64_t *idx, const double *val, int64_t *ptr_out,
+                           int64_t *idx_out, double *val_out);
+
 // Order of VCS separation candidates given by descending score sc (m entries): perm_out = the permutation that sorts
 // every run of equal scores by (vi, vR, position) ascending and keeps the runs in place -- np.lexsort((vR, vi, -sc)) for
 // descending sc. Returns 0, 1 if sc is not descending (nothing written: sort in full), -1 on bad arguments.

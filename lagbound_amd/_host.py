@@ -10,7 +10,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "liblbhost.so")
 EXPORTS = ["lbh_ws_merge", "lbh_take_f64", "lbh_take_u8", "lbh_new_triangles", "lbh_argsort_bounded",
-           "lbh_argsort_i64", "lbh_tie_order", "lbh_row_gather", "lbh_coo_sum"]
+           "lbh_argsort_i64", "lbh_tie_order", "lbh_row_gather", "lbh_coo_sum", "lbh_sym_from_lower"]
 
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
@@ -39,6 +39,8 @@ def load():
         L.lbh_row_gather.restype = C.c_int64
         L.lbh_coo_sum.argtypes = [C.c_int64, C.c_int64, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p, _f64p, _f64p]
         L.lbh_coo_sum.restype = C.c_int64
+        L.lbh_sym_from_lower.argtypes = [C.c_int64, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p]
+        L.lbh_sym_from_lower.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -157,3 +159,20 @@ def coo_sum(N, r, c, v=None, sums=True, abs_sums=False, counts=False):
         raise ValueError("lbh_coo_sum: an index out of range")
     cut = (lambda a: None if a is None else a[:w])
     return indptr, idx[:w], cut(sv), cut(sa), cut(sn)
+
+
+def sym_from_lower(N, indptr, indices, data):
+    """(indptr, indices, data) of the full symmetric CSR of a lower-triangular CSR with sorted rows, by a bucket
+    transpose (lbh_sym_from_lower)."""
+    ptr = np.ascontiguousarray(indptr, np.int64)
+    idx = np.ascontiguousarray(indices, np.int64)
+    val = np.ascontiguousarray(data, np.float64)
+    nnz = int(ptr[-1]) if len(ptr) else 0
+    po = np.empty(int(N) + 1, np.int64)
+    io = np.empty(2 * nnz, np.int64)
+    vo = np.empty(2 * nnz)
+    w = load().lbh_sym_from_lower(int(N), _p(ptr, _i64p), _p(idx, _i64p), _p(val, _f64p), _p(po, _i64p),
+                                  _p(io, _i64p), _p(vo, _f64p))
+    if w < 0:
+        raise ValueError("lbh_sym_from_lower: not a lower-triangular CSR with ascending columns")
+    return po, io[:w], vo[:w]

@@ -208,6 +208,39 @@ int64_t lbh_coo_sum(int64_t N, int64_t m, const int64_t *r, const int64_t *c, co
     return w;
 }
 
+int64_t lbh_sym_from_lower(int64_t N, const int64_t *ptr, const int64_t *idx, const double *val, int64_t *ptr_out,
+                           int64_t *idx_out, double *val_out) {
+    if (N < 0 || !ptr || !ptr_out) return -1;
+    // row k of the result: row k of the lower triangle (columns <= k, ascending), then the entries (i, k) with i > k
+    // in ascending i, which a pass over the rows in order drops into bucket k already sorted
+    std::vector<int64_t> up((size_t)N + 1, 0);
+    for (int64_t i = 0; i < N; i++)
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+            const int64_t j = idx[e];
+            if (j < 0 || j > i) return -1;
+            if (e > ptr[i] && idx[e - 1] >= j) return -1;  // columns strictly ascending
+            if (j < i) up[(size_t)j + 1]++;
+        }
+    ptr_out[0] = 0;
+    for (int64_t k = 0; k < N; k++) ptr_out[k + 1] = ptr_out[k] + (ptr[k + 1] - ptr[k]) + up[(size_t)k + 1];
+    std::vector<int64_t> fill((size_t)N);
+    for (int64_t k = 0; k < N; k++) {
+        int64_t o = ptr_out[k];
+        for (int64_t e = ptr[k]; e < ptr[k + 1]; e++) { idx_out[o] = idx[e]; val_out[o] = val[e]; o++; }
+        fill[(size_t)k] = o;
+    }
+    for (int64_t i = 0; i < N; i++)
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+            const int64_t j = idx[e];
+            if (j < i) {
+                const int64_t o = fill[(size_t)j]++;
+                idx_out[o] = i;
+                val_out[o] = val[e];
+            }
+        }
+    return ptr_out[N];
+}
+
 int lbh_tie_order(int64_t m, const double *sc, const int64_t *vi, const int64_t *vR, int64_t *perm_out) {
     if (m < 0 || (m && (!sc || !vi || !vR || !perm_out))) return -1;
     for (int64_t t = 0; t + 1 < m; t++)

@@ -250,17 +250,12 @@ class Lifted:
 
 
 def full_symmetric(Ml):
-    """The full symmetric CSR (both triangles and the diagonal) of a lower-triangle CSR, without arithmetic (every
-    position is stored once, so the combine step of liblbhost's conversion adds nothing to a value)."""
-    Mc = Ml.tocoo()
-    off = Mc.row != Mc.col
+    """The full symmetric CSR (both triangles and the diagonal) of a lower-triangle CSR with sorted rows, without
+    arithmetic (a bucket transpose in liblbhost: no sort)."""
     N = Ml.shape[0]
-    indptr, indices, sm, _, _ = HL.coo_sum(N, np.r_[Mc.row, Mc.col[off]], np.r_[Mc.col, Mc.row[off]],
-                                           np.r_[Mc.data, Mc.data[off]])
-    if len(indices) != len(Mc.data) + int(off.sum()):
-        raise ValueError("full_symmetric: a position stored twice in the lower triangle")
+    indptr, indices, data = HL.sym_from_lower(N, Ml.indptr, Ml.indices, Ml.data)
     it = np.int32 if N < 2 ** 31 - 1 else np.int64
-    return sp.csr_matrix((sm, indices.astype(it), indptr.astype(it)), shape=Ml.shape)
+    return sp.csr_matrix((data, indices.astype(it), indptr.astype(it)), shape=Ml.shape)
 
 
 def vertex_key(inst, dense_deg=1000, core_deg=256):

@@ -233,3 +233,22 @@ def test_coo_sum_equals_scipy():
         assert none is None and np.array_equal(ip2, ip) and np.array_equal(ix2, ix)
     with pytest.raises(ValueError):
         H.coo_sum(3, np.array([0, 3]), np.array([0, 1]))
+
+
+def test_sym_from_lower_equals_scipy():
+    """liblbhost's full symmetric CSR of a lower triangle (bucket transpose, no sort) against scipy's construction of
+    the same matrix; an entry above the diagonal or unsorted columns are errors."""
+    import scipy.sparse as sp
+    from lagbound_amd import _host as H
+    for N, dens in ((1, 1.0), (40, 0.2), (500, 0.02)):
+        A = sp.random(N, N, density=dens, random_state=2, format="csr")
+        Lo = sp.tril(A + A.T + sp.eye(N), format="csr")
+        Lo.sort_indices()
+        ip, ix, v = H.sym_from_lower(N, Lo.indptr, Lo.indices, Lo.data)
+        F = (Lo + sp.tril(Lo, -1).T).tocsr()
+        F.sort_indices()
+        assert np.array_equal(ip, F.indptr) and np.array_equal(ix, F.indices) and np.array_equal(v, F.data)
+    with pytest.raises(ValueError):
+        H.sym_from_lower(2, np.array([0, 1, 2]), np.array([1, 1]), np.ones(2))      # (0, 1) is above the diagonal
+    with pytest.raises(ValueError):
+        H.sym_from_lower(2, np.array([0, 0, 2]), np.array([1, 0]), np.ones(2))      # row 1 columns not ascending
