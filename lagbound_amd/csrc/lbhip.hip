@@ -1305,35 +1305,45 @@ struct PairRowArgs {
     const double *Y; const double *sig; const double *rq; const double *dsq; double w, rho;
     const uint8_t *tmask;    // touched clusters of this batch (sumP bytes)
     double *M; double *pdiag;
+    // dd set (lb_k_post_next): Y holds the pair cluster rows before the previous batch, whose increments dd (dnb
+    // rows drows, R each, labels lab) each workgroup adds per cluster in LDS (sumP R doubles) first
+    const double *dd; const int32_t *drows; int64_t dnb; const int32_t *lab[LB_MAXLEV];
 };
 
 // pair_all for the clusters the batch touches: pp row from the current Y, M_Q = sum_R pp_QR Y_R, pdiag_Q = pp_QQ
 template <int R>
-__device__ inline void pair_row_one(const PairRowArgs &a, int Qg, int lane);
+__device__ inline void pair_row_one(const PairRowArgs &a, int Qg, int lane, const double *dP = nullptr);
 template <int R>
 __global__ __launch_bounds__(64) void lb_k_pair_rows(PairRowArgs a) {
     pair_row_one<R>(a, (int)blockIdx.x, (int)threadIdx.x);
 }
-// one wavefront per cluster (lane = the wavefront's lane)
+// one wavefront per cluster (lane = the wavefront's lane); dP (optional): increments added to every Y row read
 template <int R>
-__device__ inline void pair_row_one(const PairRowArgs &a, int Qg, int lane) {
+__device__ inline void pair_row_one(const PairRowArgs &a, int Qg, int lane, const double *dP) {
     if (!a.tmask[Qg]) return;
     int l = 0;
     while (l + 1 < a.nlev && Qg >= a.Poff[l + 1]) l++;
     const int P = a.Poff[l + 1] - a.Poff[l], Q = Qg - a.Poff[l];
     const double *YQ = a.Y + (int64_t)Qg * R;
+    double yq[R];
+#pragma unroll
+    for (int c = 0; c < R; c++) yq[c] = dP ? YQ[c] + dP[(int64_t)Qg * R + c] : YQ[c];
     const double *sig = a.sig + a.soff[l] + (int64_t)Q * P;
     double Macc[R];
 #pragma unroll
     for (int c = 0; c < R; c++) Macc[c] = 0.0;
     double pq = 0.0;
     for (int Rr = lane; Rr < P; Rr += 64) {
-        const double *YR = a.Y + (int64_t)(a.Poff[l] + Rr) * R;
+        const int64_t Rg = a.Poff[l] + Rr;
+        const double *YR = a.Y + Rg * R;
+        double yr[R];
+#pragma unroll
+        for (int c = 0; c < R; c++) yr[c] = dP ? YR[c] + dP[Rg * R + c] : YR[c];
         double p = sig[Rr];
         if (a.rho > 0.0) {
             double h = 0.0;
 #pragma unroll
-            for (int c = 0; c < R; c++) h += YQ[c] * YR[c];
+            for (int c = 0; c < R; c++) h += yq[c] * yr[c];
             if (Rr == Q) h -= a.dsq[Qg];
             p = p - (a.rho * ((a.rq[Qg] * a.rq[a.Poff[l] + Rr]) * a.w)) * h;
             if (p < 0.0) p = 0.0;
@@ -1341,7 +1351,7 @@ __device__ inline void pair_row_one(const PairRowArgs &a, int Qg, int lane) {
         if (Rr == Q) pq = p;
         if (p != 0.0) {
 #pragma unroll
-            for (int c = 0; c < R; c++) Macc[c] += p * YR[c];
+            for (int c = 0; c < R; c++) Macc[c] += p * yr[c];
         }
     }
     pq = wave_sum(pq);
@@ -1352,6 +1362,35 @@ __device__ inline void pair_row_one(const PairRowArgs &a, int Qg, int lane) {
         for (int c = 0; c < R; c++) a.M[(int64_t)Qg * R + c] = Macc[c];
         a.pdiag[Qg] = pq;
     }
+}
+
+// Workgroup blk of the pair rows (four clusters, one wavefront each); with a.dd the previous batch's increments are
+// first summed per pair cluster in LDS (one batch row per thread) and added to every Y row read. Every thread of the
+// workgroup calls it (it synchronises the workgroup).
+template <int R>
+__device__ inline void pair_rows_blk(const PairRowArgs &a, int blk, int nclu) {
+    extern __shared__ double shP[];
+    const double *dP = nullptr;
+    if (a.dd) {
+        const int wp = a.Poff[a.nlev] * R;
+        for (int e = threadIdx.x; e < wp; e += LB_BLOCK) shP[e] = 0.0;
+        __syncthreads();
+        for (int64_t p = threadIdx.x; p < a.dnb; p += LB_BLOCK) {
+            const int32_t i = a.drows[p];
+            double xv[R];
+#pragma unroll
+            for (int c = 0; c < R; c++) xv[c] = a.dd[p * R + c];
+            for (int l = 0; l < a.nlev; l++) {
+                double *dst = shP + (int64_t)(a.Poff[l] + a.lab[l][i]) * R;
+#pragma unroll
+                for (int c = 0; c < R; c++) atomicAdd(dst + c, xv[c]);
+            }
+        }
+        __syncthreads();
+        dP = shP;
+    }
+    const int Qg = blk * (LB_BLOCK / 64) + (int)(threadIdx.x >> 6);
+    if (Qg < nclu) pair_row_one<R>(a, Qg, (int)(threadIdx.x & 63), dP);
 }
 
 // batch -> touched clusters: tmask[b * sumP + Q] = 1 (b = batch of position p of the sweep's order)
@@ -1467,6 +1506,8 @@ struct StepArgs {
     // other triangle hubs at nhE + their hT index): arrival count, batch position, own terms g[R] and kap
     double *zYn; int32_t *hubcnt; int64_t *hubpos; double *hubown;
     double *zpart;     // z increments spread over LB_ZPART copies of R doubles (by batch position), folded into z later
+    // posb set (lb_k_step_copy): each row stepped records posb[i] = tag, posp[i] = its batch position
+    int32_t *posb; int32_t *posp; int tag;
 };
 
 // one triangle record of the row step: tau (canonical order), q, pt = lam - rt (1 - tau); if pt > 0 the record adds
@@ -1513,6 +1554,9 @@ struct StepSegArgs {
     const uint4 *V; const int32_t *col; const void *w; const void *rec; const Seg *segs;
     const int32_t *list; const int64_t *lpos; int b;   // segment ids of batch b's hub rows: list[lpos[b] .. lpos[b + 1])
     double rt; void *segout; double *segkap;
+    // posb set (lb_k_post_next): a row j with posb[j] == tag was stepped in the previous batch and its new value is
+    // read from Vstage[posp[j]] (V of such rows is being rewritten by the same launch)
+    const int32_t *posb; const int32_t *posp; const uint4 *Vstage; int tag;
 };
 
 // wavefront per hub segment: partial sum of A V (edge segments) or of the triangle records' step terms
@@ -1577,6 +1621,14 @@ __device__ inline void step_seg_one(const StepSegArgs &a, int32_t sid) {
             uint4 u[LB_SEB];
 #pragma unroll
             for (int t = 0; t < LB_SEB; t++) u[t] = a.V[jj[t] * S + q];
+            if (a.posb) {
+                int32_t tb[LB_SEB];
+#pragma unroll
+                for (int t = 0; t < LB_SEB; t++) tb[t] = a.posb[jj[t]];
+#pragma unroll
+                for (int t = 0; t < LB_SEB; t++)
+                    if (tb[t] == a.tag) u[t] = a.Vstage[(int64_t)a.posp[jj[t]] * S + q];
+            }
 #pragma unroll
             for (int t = 0; t < LB_SEB; t++) {
                 TC xj[CPL];
@@ -1600,6 +1652,16 @@ __device__ inline void step_seg_one(const StepSegArgs &a, int32_t sid) {
             uint4 uu[LB_SRB], uw[LB_SRB];
 #pragma unroll
             for (int t = 0; t < LB_SRB; t++) { uu[t] = a.V[(int64_t)rq[t].u * S + q]; uw[t] = a.V[(int64_t)rq[t].w * S + q]; }
+            if (a.posb) {
+                int32_t tu[LB_SRB], tw[LB_SRB];
+#pragma unroll
+                for (int t = 0; t < LB_SRB; t++) { tu[t] = a.posb[rq[t].u]; tw[t] = a.posb[rq[t].w]; }
+#pragma unroll
+                for (int t = 0; t < LB_SRB; t++) {
+                    if (tu[t] == a.tag) uu[t] = a.Vstage[(int64_t)a.posp[rq[t].u] * S + q];
+                    if (tw[t] == a.tag) uw[t] = a.Vstage[(int64_t)a.posp[rq[t].w] * S + q];
+                }
+            }
 #pragma unroll
             for (int t = 0; t < LB_SRB; t++) {
                 if (k0 + (int64_t)t * NSW >= s.e) break;      // slot-uniform
@@ -1841,6 +1903,7 @@ __device__ inline void step_finish_wave(const StepArgs &a, const typename VT<TV>
         }
         return;
     }
+    if (a.posb && q == 0) { a.posb[i] = a.tag; a.posp[i] = (int32_t)pos; }
     double *dd = a.DD + pos * R + q * CPL;
 #pragma unroll
     for (int c = 0; c < CPL; c++) dd[c] = di * ((double)xs[c] - (double)x[c]);
@@ -2028,11 +2091,18 @@ __device__ inline void step_add_segs(const StepArgs &a, int64_t i, int q, bool h
 }
 
 template <typename TV, int R>
+__device__ inline void step_wave_blk(const StepArgs &a, int blk);
+template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_step_wave(StepArgs a) {
+    step_wave_blk<TV, R>(a, (int)blockIdx.x);
+}
+// workgroup blk of lb_k_step_wave (four batch rows, one wavefront each; no workgroup synchronisation)
+template <typename TV, int R>
+__device__ inline void step_wave_blk(const StepArgs &a, int blk) {
     typedef typename VT<TV>::TC TC;
     constexpr int CPL = VT<TV>::CPL, S = R / CPL;
     const int lane = threadIdx.x & 63, ws = lane / S, q = lane % S;
-    const int64_t pos = (int64_t)blockIdx.x * (LB_BLOCK / 64) + (threadIdx.x >> 6);
+    const int64_t pos = (int64_t)blk * (LB_BLOCK / 64) + (threadIdx.x >> 6);
     if (pos >= a.nb) return;                           // wavefront-uniform
     const int64_t i = a.rows[pos];
     TC x[CPL], g[CPL];
@@ -2402,6 +2472,40 @@ template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_both(PostArgs qa, int npa, Post2Args pb) {
     if ((int)blockIdx.x < npa) post_acc_blk(qa, (int)blockIdx.x, npa);
     else post_vcs_wave_blk<TV, R>(pb, (int)blockIdx.x - npa);
+}
+
+// Two launches per sweep batch (option sweep_two). lb_k_step_copy: workgroups [0, nrb) step the batch's rows
+// (lb_k_step_wave, each row also recording its batch position under the batch's tag), the rest copy [Y] (cw doubles)
+// to a snapshot. lb_k_post_next: the batch's aggregate increments (lb_k_post_acc), its VCS post step and the next
+// batch's pair rows, both reading the snapshot plus the batch's increments summed in LDS, and the next batch's hub
+// segments, which read a row stepped in this batch from Vstage (its V row is being rewritten by the same launch).
+// The parts of each launch write disjoint memory, and none reads what another part of the same launch writes.
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_step_copy(StepArgs sa, int nrb, const double *src, double *dst,
+                                                           int64_t cw) {
+    const int blk = (int)blockIdx.x;
+    if (blk < nrb) {
+        step_wave_blk<TV, R>(sa, blk);
+        return;
+    }
+    const int64_t nthr = (int64_t)(gridDim.x - nrb) * LB_BLOCK;
+    for (int64_t e = (int64_t)(blk - nrb) * LB_BLOCK + threadIdx.x; e < cw; e += nthr) dst[e] = src[e];
+}
+
+template <typename TV, int R>
+__global__ __launch_bounds__(LB_BLOCK) void lb_k_post_next(PostArgs qa, int npa, Post2Args pb, int npv, PairRowArgs pa,
+                                                           int npair, int nclu, StepSegArgs ga, int nsegb) {
+    int blk = (int)blockIdx.x;
+    if (blk < npa) { post_acc_blk(qa, blk, npa); return; }
+    blk -= npa;
+    if (blk < npv) { post_vcs_wave_blk<TV, R>(pb, blk); return; }
+    blk -= npv;
+    if (blk < npair) { pair_rows_blk<R>(pa, blk, nclu); return; }
+    blk -= npair;
+    const int64_t l0 = ga.lpos[ga.b], l1 = ga.lpos[ga.b + 1];
+    const int64_t nsw = (int64_t)nsegb * (LB_BLOCK / 64);
+    for (int64_t li = l0 + (int64_t)blk * (LB_BLOCK / 64) + (threadIdx.x >> 6); li < l1; li += nsw)
+        step_seg_one<TV, R>(ga, ga.list[li]);
 }
 
 // ------------------------------------------------------------------------------------------------ triangle separation
@@ -3058,6 +3162,12 @@ struct lb_ctx {
     // (sumPv (2R + 1) doubles) fits 48 KB
     bool sweep_post_merge = true;
     DBuf Ysnap;
+    // option "sweep_two" 1 (default, with step_wave, sweep_pair_segs and sweep_post_merge, eager sweeps): two launches
+    // per batch (lb_k_step_copy, lb_k_post_next); posb / posp: each row's last batch tag and position (tags unique
+    // within one lb_sweeps call, posb reset to -1 at its start)
+    bool sweep_two = true;
+    DBuf posb, posp;
+    int sw_tag = 0;
     double rgd_key[5] = {0, 0, 0, 0, 0};
 };
 
@@ -3566,6 +3676,7 @@ struct Launch {
         PairRowArgs pa{};
         pa.nlev = c->nlev; pa.R = R;
         for (int l = 0; l <= c->nlev; l++) { pa.Poff[l] = Poff[l]; pa.soff[l] = c->soff[l]; }
+        for (int l = 0; l < c->nlev; l++) pa.lab[l] = (const int32_t *)c->labs.p + (int64_t)l * n;
         pa.Y = Yp; pa.sig = (const double *)c->sig.p; pa.rq = (const double *)c->rq.p; pa.dsq = (const double *)c->dsq.p;
         pa.w = c->pw; pa.rho = rho; pa.M = (double *)c->M.p; pa.pdiag = (double *)c->pdiag.p;
         PostArgs qa{};
@@ -3699,8 +3810,67 @@ struct Launch {
             // the last batch's post-step, its z completed, and its [z | Y] back into Y
             if (nbprev) pre((int)((nbprev + 3) / 4), 0, nullptr, cur ? zb[0] : nullptr);
         };
+        // two launches per batch (lb_k_step_copy, lb_k_post_next): eager sweeps only (the batch tags are host values)
+        const size_t ldsPair = sizeof(double) * (size_t)c->sumPpair * R;
+        const bool two = !fused && c->sweep_two && c->step_wave && c->sweep_pair_segs && c->sweep_post_merge &&
+                         !c->sweep_graph && (c->sumPv == 0 || (pb.use_lds && ldsV2 <= 48 * 1024)) &&
+                         ldsPair <= 48 * 1024;
+        auto batches_two = [&](PairRowArgs pa, StepSegArgs ga, StepArgs sa, PostArgs qa, Post2Args pb) {
+            std::vector<int> nz;
+            for (int b = 0; b < nbatch; b++) if (bpos[b + 1] > bpos[b]) nz.push_back(b);
+            if (nz.empty()) return;
+            const int npair = c->nlev ? (c->sumPpair + LB_BLOCK / 64 - 1) / (LB_BLOCK / 64) : 0;
+            const int nsegb = nsegs ? (int)gsegs : 0;
+            const int64_t cw = (int64_t)(c->sumPpair + c->sumPv) * R;
+            const unsigned ncp = (unsigned)std::max<int64_t>(1, std::min<int64_t>(8, (cw + 1023) / 1024));
+            const size_t ldsNext = std::max(std::max(qa.use_lds ? ldsP : (size_t)0, c->sumPv ? ldsV2 : (size_t)0),
+                                            c->nlev ? ldsPair : (size_t)0);
+            double *ysnap = (double *)c->Ysnap.p;
+            // the first batch's pair rows and hub segments, from the aggregates and V as they are
+            if (npair + nsegb > 0) {
+                if (c->nlev) pa.tmask = (const uint8_t *)c->tmask.p + (int64_t)nz[0] * c->sumPpair;
+                ga.b = nz[0];
+                hipLaunchKernelGGL((lb_k_pair_segs<TV, R>), dim3((unsigned)(npair + nsegb)), dim3(LB_BLOCK), 0, c->stream,
+                                   pa, npair, c->sumPpair, ga, nsegb, (const double *)nullptr, (double *)nullptr, (int64_t)0);
+            }
+            for (size_t k = 0; k < nz.size(); k++) {
+                const int b = nz[k];
+                const int64_t p0 = bpos[b], nb = bpos[b + 1] - bpos[b];
+                const int tag = c->sw_tag++;
+                StepArgs st = sa;
+                st.rows = order + p0; st.nb = nb;
+                st.posb = (int32_t *)c->posb.p; st.posp = (int32_t *)c->posp.p; st.tag = tag;
+                const int nrb = (int)((nb + 3) / 4);
+                hipLaunchKernelGGL((lb_k_step_copy<TV, R>), dim3((unsigned)nrb + ncp), dim3(LB_BLOCK), 0, c->stream, st, nrb,
+                                   Yp, ysnap, cw);
+                PostArgs qb = qa;
+                qb.rows = order + p0; qb.nb = nb;
+                const unsigned pg = (unsigned)std::min<int64_t>(64, (nb + LB_POST_ROWS - 1) / LB_POST_ROWS);
+                Post2Args pv = pb;
+                pv.rows = order + p0; pv.nb = nb;
+                if (c->sumPv) {
+                    pv.Y = ysnap + (int64_t)c->sumPpair * R;
+                    for (int m = 0; m < c->nvcs; m++) pv.Yoff[m] = Zoff[m];
+                    pv.dd = (const double *)c->DDb.p;
+                }
+                const bool next = k + 1 < nz.size();
+                PairRowArgs pn = pa;
+                StepSegArgs gn = ga;
+                if (next) {
+                    if (c->nlev) pn.tmask = (const uint8_t *)c->tmask.p + (int64_t)nz[k + 1] * c->sumPpair;
+                    pn.Y = ysnap; pn.dd = (const double *)c->DDb.p; pn.drows = order + p0; pn.dnb = nb;
+                    gn.b = nz[k + 1];
+                    gn.posb = (const int32_t *)c->posb.p; gn.posp = (const int32_t *)c->posp.p;
+                    gn.Vstage = (const uint4 *)c->Vstage.p; gn.tag = tag;
+                }
+                const int np2 = next ? npair : 0, ns2 = next ? nsegb : 0;
+                hipLaunchKernelGGL((lb_k_post_next<TV, R>), dim3(pg + (unsigned)nrb + (unsigned)(np2 + ns2)), dim3(LB_BLOCK),
+                                   ldsNext, c->stream, qb, (int)pg, pv, nrb, pn, np2, c->sumPpair, gn, ns2);
+            }
+        };
         auto run = [&]() {
             if (fused) batches_fused(pa, ga, sa, pb);
+            else if (two) batches_two(pa, ga, sa, qa, pb);
             else batches(pa, ga, sa, qa, pb);
         };
         if (!c->sweep_graph) {
@@ -4138,7 +4308,7 @@ void lb_destroy(lb_ctx *c) {
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
                     &c->hTseg0, &c->parts, &c->lamsum, &c->order, &c->tmask, &c->Kq, &c->stZ, &c->Zs, &c->cnts,
                     &c->Vstage, &c->DDb, &c->segkap, &c->seglist, &c->vslack, &c->vage, &c->vvol, &c->rorder,
-                    &c->Yalt, &c->hubcnt, &c->hubpos, &c->hubown, &c->zpart, &c->Ysnap,
+                    &c->Yalt, &c->hubcnt, &c->hubpos, &c->hubown, &c->zpart, &c->Ysnap, &c->posb, &c->posp,
                     &c->b_off, &c->b_gid, &c->b_L, &c->b_C0, &c->b_Poff, &c->b_soff, &c->b_cg, &c->b_lab, &c->b_sig,
                     &c->b_pp, &c->b_rq, &c->b_dsq, &c->b_zY, &c->b_M, &c->b_pdiag, &c->b_ltot, &c->b_lviol, &c->b_rowv,
                     &c->b_parts, &c->b_lamsum, &c->b_tptr, &c->b_tidx, &c->b_R, &c->b_Vp, &c->b_Rp, &c->b_rgst};
@@ -4167,6 +4337,7 @@ int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
     if (std::strcmp(name, "sweep_fused") == 0) { c->sweep_fused = value != 0; return 0; }
     if (std::strcmp(name, "sweep_pair_segs") == 0) { c->sweep_pair_segs = value != 0; return 0; }
     if (std::strcmp(name, "sweep_post_merge") == 0) { c->sweep_post_merge = value != 0; return 0; }
+    if (std::strcmp(name, "sweep_two") == 0) { c->sweep_two = value != 0; return 0; }
     if (std::strcmp(name, "sweep_seg_grid") == 0) {
         if (value < 0 || value > 4096) return fail(c, -1, "lb_set_option: sweep_seg_grid is 0..4096");
         c->sweep_seg_grid = value;
@@ -4673,7 +4844,11 @@ int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap
         if ((rc = dalloc(c, c->hubown, sizeof(double) * nhub * (size_t)(R + 1)))) return rc;
         HIPCHK(c, hipMemsetAsync(c->hubcnt.p, 0, sizeof(int32_t) * nhub, c->stream));
         if ((rc = dalloc(c, c->zpart, sizeof(double) * LB_ZPART * (size_t)R))) return rc;
-        if ((rc = dalloc(c, c->Ysnap, sizeof(double) * (size_t)std::max(c->sumPv, 1) * R))) return rc;
+        if ((rc = dalloc(c, c->Ysnap, sizeof(double) * (size_t)std::max(c->sumPpair + c->sumPv, 1) * R))) return rc;
+        if ((rc = dalloc(c, c->posb, sizeof(int32_t) * (size_t)n))) return rc;
+        if ((rc = dalloc(c, c->posp, sizeof(int32_t) * (size_t)n))) return rc;
+        HIPCHK(c, hipMemsetAsync(c->posb.p, 0xFF, sizeof(int32_t) * (size_t)n, c->stream));   // -1: no batch yet
+        c->sw_tag = 0;
         HIPCHK(c, hipMemsetAsync(c->zpart.p, 0, sizeof(double) * LB_ZPART * (size_t)R, c->stream));
     }
     std::vector<int32_t> o32(n);

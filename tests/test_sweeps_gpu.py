@@ -177,33 +177,37 @@ def test_fused_sweeps_equal_five_kernel_form(dtype):
 
 @pytest.mark.parametrize("nbatch", [12, 64])
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-def test_pair_segs_launch_equals_separate(dtype, nbatch):
-    """Options sweep_pair_segs and sweep_post_merge 1 (the defaults: three launches per batch -- pair-level rows, hub
-    segments and the VCS cluster-row snapshot; the row steps; aggregate increments with the VCS post step, which forms
-    the cluster rows after the batch from the snapshot and the batch's increments) against 0 (five launches): the same
-    per-wavefront arithmetic, so V agrees to the run-to-run rounding of the FP64 aggregate adds (1e-12 on FP64 V,
-    1e-6 on FP32 V). Small hub thresholds (most rows are hubs with several segments of both kinds), two calls of
-    sweeps at 12 batches (417 rows: the post step stays in two launches) and 64 (78 rows: one launch); the merged
-    form is also checked against the oracle."""
+def test_sweep_launch_forms_agree(dtype, nbatch):
+    """The sweep batch in two launches (the default: row steps with a snapshot of Y; then the aggregate increments,
+    the VCS post step, the next batch's pair rows and hub segments, the last two reading the snapshot plus the batch's
+    increments and, for a row just stepped, its new value from the staging rows), in three or four (option sweep_two
+    0: pair rows with hub segments, row steps, then the aggregate increments with the VCS post step in one launch for
+    batches of at most 256 rows) and in five (pair rows, hub segments, row steps, aggregate increments, VCS post step
+    apart). Same per-wavefront arithmetic, so V agrees to the run-to-run rounding of the FP64 aggregate adds (1e-12
+    on FP64 V, 1e-6 on FP32 V). Small hub thresholds (most rows are hubs with several segments of both kinds), two
+    calls of sweeps at 12 batches (417 rows) and 64 (78 rows); the default form is also checked against the oracle."""
     from lagbound_amd.engine import Engine
     inst, V = chung_lu_instance(n=5000, seed=31, k=16, P=16)
     rng = np.random.default_rng(32)
     calls = [np.stack([rng.permutation(inst.n) for _ in range(2)]), np.stack([rng.permutation(inst.n)])]
+    forms = {"two": {}, "three": {"sweep_two": 0},
+             "five": {"sweep_two": 0, "sweep_pair_segs": 0, "sweep_post_merge": 0}}
     out = {}
-    for merged in (1, 0):
+    for name, opts in forms.items():
         eng = Engine(r=16, dtype=dtype)
-        for k, v in {"sweep_pair_segs": merged, "sweep_post_merge": merged, "sweep_heavy": 8, "sweep_seg": 5}.items():
+        for k, v in dict(opts, sweep_heavy=8, sweep_seg=5).items():
             eng.set_option(k, v)
         eng.set_instance(inst)
         eng.set_V(V)
         Ws = [eng.get_V()]
-        print(f"[sweeps] pair_segs={merged} dtype={dtype} n={inst.n} nbatch={nbatch}", flush=True)
+        print(f"[sweeps] form={name} dtype={dtype} n={inst.n} nbatch={nbatch}", flush=True)
         for orders in calls:
             eng.sweeps(orders, nbatch, 0.05, damp=0.1)
             Ws.append(eng.get_V())
-        out[merged] = Ws
+        out[name] = Ws
         eng.close()
-    for a, b in zip(out[1][1:], out[0][1:]):
-        assert np.abs(a - b).max() <= (1e-12 if dtype == "f64" else 1e-6)
-    Wo = O.sweeps_c(inst, out[1][1], calls[1], nbatch, 0.05, damp=0.1)
-    assert np.abs(out[1][2] - Wo).max() <= (1e-10 if dtype == "f64" else 2e-4)
+    for name in ("two", "three"):
+        for a, b in zip(out[name][1:], out["five"][1:]):
+            assert np.abs(a - b).max() <= (1e-12 if dtype == "f64" else 1e-6), name
+    Wo = O.sweeps_c(inst, out["two"][1], calls[1], nbatch, 0.05, damp=0.1)
+    assert np.abs(out["two"][2] - Wo).max() <= (1e-10 if dtype == "f64" else 2e-4)
