@@ -163,7 +163,10 @@ int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int
  * runs a batch's pair-level rows and hub segments in one launch (0: two launches; same kernels' arithmetic);
  * "sweep_post_merge" (default 1) runs the aggregate increments and the VCS post step of batches of at most 256 rows
  * in one launch, the post step forming the VCS cluster rows after the batch from a snapshot and the batch's
- * increments (0: two launches);
+ * increments (0: two launches); "sweep_two" (default 1, eager sweeps with the three switches above on) runs a batch
+ * as two launches: its row steps (and a snapshot of Y), then its aggregate increments and VCS post step with the
+ * next batch's pair rows and hub segments, which read the snapshot plus the batch's increments and, for a row just
+ * stepped, its staged new value (0: the forms above);
  * "sweep_seg_grid"
  * (default 0: 64) caps the workgroups that compute the sweeps' hub segments;
  * "overlap" (default 1) runs the evaluation's hub-segment kernel on a second stream of the context, concurrently
