@@ -166,7 +166,9 @@ int lb_set_tuning(lb_ctx *ctx, int heavy_edges, int heavy_tris, int seg_len, int
  * increments (0: two launches); "sweep_two" (default 1, eager sweeps with the three switches above on) runs a batch
  * as two launches: its row steps (and a snapshot of Y), then its aggregate increments and VCS post step with the
  * next batch's pair rows and hub segments, which read the snapshot plus the batch's increments and, for a row just
- * stepped, its staged new value (0: the forms above);
+ * stepped, its staged new value (0: the forms above); "sweep_delta" (default 1; batches above 128 rows in that form)
+ * has the rows add their increments to a per-batch delta buffer with FP64 atomics, read by the next launch, in place
+ * of each of its workgroups summing the batch in LDS (0: always the LDS sums);
  * "sweep_seg_grid"
  * (default 0: 64) caps the workgroups that compute the sweeps' hub segments;
  * "overlap" (default 1) runs the evaluation's hub-segment kernel on a second stream of the context, concurrently

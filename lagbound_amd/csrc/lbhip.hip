@@ -1308,6 +1308,7 @@ struct PairRowArgs {
     // dd set (lb_k_post_next): Y holds the pair cluster rows before the previous batch, whose increments dd (dnb
     // rows drows, R each, labels lab) each workgroup adds per cluster in LDS (sumP R doubles) first
     const double *dd; const int32_t *drows; int64_t dnb; const int32_t *lab[LB_MAXLEV];
+    const double *dPg;          // set (sweep_delta): the previous batch's pair cluster-row increments, read with Y
 };
 
 // pair_all for the clusters the batch touches: pp row from the current Y, M_Q = sum_R pp_QR Y_R, pdiag_Q = pp_QQ
@@ -1389,6 +1390,7 @@ __device__ inline void pair_rows_blk(const PairRowArgs &a, int blk, int nclu) {
         __syncthreads();
         dP = shP;
     }
+    if (a.dPg) dP = a.dPg;
     const int Qg = blk * (LB_BLOCK / 64) + (int)(threadIdx.x >> 6);
     if (Qg < nclu) pair_row_one<R>(a, Qg, (int)(threadIdx.x & 63), dP);
 }
@@ -1508,6 +1510,9 @@ struct StepArgs {
     double *zpart;     // z increments spread over LB_ZPART copies of R doubles (by batch position), folded into z later
     // posb set (lb_k_step_copy): each row stepped records posb[i] = tag, posp[i] = its batch position
     int32_t *posb; int32_t *posp; int tag;
+    // dnext set: the rows add their increments to [LB_ZPART copies of z | Y] deltas of this batch (FP64 atomics) in
+    // place of writing DD
+    double *dnext;
 };
 
 // one triangle record of the row step: tau (canonical order), q, pt = lam - rt (1 - tau); if pt > 0 the record adds
@@ -1904,6 +1909,21 @@ __device__ inline void step_finish_wave(const StepArgs &a, const typename VT<TV>
         return;
     }
     if (a.posb && q == 0) { a.posb[i] = a.tag; a.posp[i] = (int32_t)pos; }
+    if (a.dnext) {
+        double *zp = a.dnext + (int64_t)(pos & (LB_ZPART - 1)) * R + q * CPL;
+        double *yb = a.dnext + LB_ZPART * R;
+        double *yl[LB_MAXLEV + LB_MAXVCS];
+        int ns = 0;
+        for (int l = 0; l < a.nlev; l++) yl[ns++] = yb + (int64_t)(a.Poff[l] + a.lab[l][i]) * R + q * CPL;
+        for (int m = 0; m < a.nvcs; m++) yl[ns++] = yb + (int64_t)(a.Yoff[m] + a.vlab[m][i]) * R + q * CPL;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+            const double dv = di * ((double)xs[c] - (double)x[c]);
+            atomicAdd(zp + c, dv);
+            for (int t = 0; t < ns; t++) atomicAdd(yl[t] + c, dv);
+        }
+        return;
+    }
     double *dd = a.DD + pos * R + q * CPL;
 #pragma unroll
     for (int c = 0; c < CPL; c++) dd[c] = di * ((double)xs[c] - (double)x[c]);
@@ -2252,6 +2272,7 @@ struct Post2Args {
     // workgroup adds the batch's increments dd (batch-position indexed, R each) per VCS cluster in LDS (sumPv R more
     // doubles), so the step reads the cluster rows after the batch without waiting for the aggregate pass
     const double *dd;
+    const double *dYg;          // set (sweep_delta): the batch's VCS cluster-row increments, read with Y (Zoff-indexed)
 };
 
 // vcs_post: new effective multipliers SN of the moved rows (new v_i, updated Y), Z_R += d_i (SN v_new - stZ v_old),
@@ -2396,10 +2417,11 @@ __device__ inline void post_vcs_wave_blk(const Post2Args &a, int blk) {
                 const int Rc = a.col[e];
                 const double sq = (double)sig[e], so = a.stZ[e];
                 const double *YR = a.Y + (int64_t)(a.Yoff[m] + Rc) * R + q * CPL;
-                const double *dR = dY + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+                const double *dR = (a.dYg ? a.dYg : dY) + (int64_t)(a.Zoff[m] + Rc) * R + q * CPL;
+                const bool add = a.dd || a.dYg;
                 double s = 0.0;
 #pragma unroll
-                for (int c = 0; c < CPL; c++) s += (double)xn[c] * (a.dd ? YR[c] + dR[c] : YR[c]);
+                for (int c = 0; c < CPL; c++) s += (double)xn[c] * (add ? YR[c] + dR[c] : YR[c]);
                 s = slot_sum<S>(s);
                 double sn = sq;
                 if (a.rho > 0.0 && di > 0.0) {
@@ -2492,12 +2514,33 @@ __global__ __launch_bounds__(LB_BLOCK) void lb_k_step_copy(StepArgs sa, int nrb,
     for (int64_t e = (int64_t)(blk - nrb) * LB_BLOCK + threadIdx.x; e < cw; e += nthr) dst[e] = src[e];
 }
 
+// sweep_delta: [z | Y] += the batch's deltas (z from its LB_ZPART copies), and the other delta buffer cleared for the
+// next batch
+struct ApplyArgs { double *zY; const double *D; double *Dz; int64_t wY; };
+__device__ inline void apply_blk(const ApplyArgs &a, int R, int blk, int nblk) {
+    const int64_t wd = (int64_t)LB_ZPART * R + a.wY;
+    for (int64_t e = (int64_t)blk * LB_BLOCK + threadIdx.x; e < wd; e += (int64_t)nblk * LB_BLOCK) {
+        if (e < R) {
+            double v = 0.0;
+            for (int k = 0; k < LB_ZPART; k++) v += a.D[(int64_t)k * R + e];
+            a.zY[e] += v;
+        } else if (e >= (int64_t)LB_ZPART * R) {
+            const int64_t y = e - (int64_t)LB_ZPART * R;
+            a.zY[R + y] += a.D[e];
+        }
+        a.Dz[e] = 0.0;
+    }
+}
+
 template <typename TV, int R>
 __global__ __launch_bounds__(LB_BLOCK) void lb_k_post_next(PostArgs qa, int npa, Post2Args pb, int npv, PairRowArgs pa,
-                                                           int npair, int nclu, StepSegArgs ga, int nsegb) {
+                                                           int npair, int nclu, StepSegArgs ga, int nsegb,
+                                                           ApplyArgs ap, int nap) {
     int blk = (int)blockIdx.x;
     if (blk < npa) { post_acc_blk(qa, blk, npa); return; }
     blk -= npa;
+    if (blk < nap) { apply_blk(ap, R, blk, nap); return; }
+    blk -= nap;
     if (blk < npv) { post_vcs_wave_blk<TV, R>(pb, blk); return; }
     blk -= npv;
     if (blk < npair) { pair_rows_blk<R>(pa, blk, nclu); return; }
@@ -3168,6 +3211,11 @@ struct lb_ctx {
     bool sweep_two = true;
     DBuf posb, posp;
     int sw_tag = 0;
+    // option "sweep_delta" 1 (default, with sweep_two, for batches above 128 rows): the rows add their increments to a
+    // per-batch delta buffer (two, alternating by tag) with FP64 atomics; the next launch reads snapshot + delta and
+    // applies the delta to [z | Y]
+    bool sweep_delta = true;
+    DBuf dbuf;
     double rgd_key[5] = {0, 0, 0, 0, 0};
 };
 
@@ -3833,13 +3881,22 @@ struct Launch {
                 hipLaunchKernelGGL((lb_k_pair_segs<TV, R>), dim3((unsigned)(npair + nsegb)), dim3(LB_BLOCK), 0, c->stream,
                                    pa, npair, c->sumPpair, ga, nsegb, (const double *)nullptr, (double *)nullptr, (int64_t)0);
             }
+            // per-batch delta buffers for batches above 128 rows (every workgroup of the next launch would otherwise
+            // sum the whole batch in LDS: 150k-vertex request, 282 rows, 0.330 against 0.352 s of sweeps; 40k, 76
+            // rows, 0.257 against 0.250 s); one choice per call, as the buffers are cleared batch to batch
+            const bool dl = c->sweep_delta && n / nbatch > 128;
+            const int64_t wY = (int64_t)(c->sumPpair + c->sumPv) * R, wd = (int64_t)LB_ZPART * R + wY;
+            const unsigned nap = dl ? (unsigned)std::max<int64_t>(1, std::min<int64_t>(16, (wd + 1023) / 1024)) : 0u;
             for (size_t k = 0; k < nz.size(); k++) {
                 const int b = nz[k];
                 const int64_t p0 = bpos[b], nb = bpos[b + 1] - bpos[b];
                 const int tag = c->sw_tag++;
+                double *Dcur = (double *)c->dbuf.p + (int64_t)(tag & 1) * wd;
+                double *Doth = (double *)c->dbuf.p + (int64_t)((tag + 1) & 1) * wd;
                 StepArgs st = sa;
                 st.rows = order + p0; st.nb = nb;
                 st.posb = (int32_t *)c->posb.p; st.posp = (int32_t *)c->posp.p; st.tag = tag;
+                if (dl) st.dnext = Dcur;
                 const int nrb = (int)((nb + 3) / 4);
                 hipLaunchKernelGGL((lb_k_step_copy<TV, R>), dim3((unsigned)nrb + ncp), dim3(LB_BLOCK), 0, c->stream, st, nrb,
                                    Yp, ysnap, cw);
@@ -3851,21 +3908,28 @@ struct Launch {
                 if (c->sumPv) {
                     pv.Y = ysnap + (int64_t)c->sumPpair * R;
                     for (int m = 0; m < c->nvcs; m++) pv.Yoff[m] = Zoff[m];
-                    pv.dd = (const double *)c->DDb.p;
+                    if (dl) pv.dYg = Dcur + (int64_t)LB_ZPART * R + (int64_t)c->sumPpair * R;
+                    else pv.dd = (const double *)c->DDb.p;
                 }
                 const bool next = k + 1 < nz.size();
                 PairRowArgs pn = pa;
                 StepSegArgs gn = ga;
                 if (next) {
                     if (c->nlev) pn.tmask = (const uint8_t *)c->tmask.p + (int64_t)nz[k + 1] * c->sumPpair;
-                    pn.Y = ysnap; pn.dd = (const double *)c->DDb.p; pn.drows = order + p0; pn.dnb = nb;
+                    pn.Y = ysnap;
+                    if (dl) pn.dPg = Dcur + (int64_t)LB_ZPART * R;
+                    else { pn.dd = (const double *)c->DDb.p; pn.drows = order + p0; pn.dnb = nb; }
                     gn.b = nz[k + 1];
                     gn.posb = (const int32_t *)c->posb.p; gn.posp = (const int32_t *)c->posp.p;
                     gn.Vstage = (const uint4 *)c->Vstage.p; gn.tag = tag;
                 }
                 const int np2 = next ? npair : 0, ns2 = next ? nsegb : 0;
-                hipLaunchKernelGGL((lb_k_post_next<TV, R>), dim3(pg + (unsigned)nrb + (unsigned)(np2 + ns2)), dim3(LB_BLOCK),
-                                   ldsNext, c->stream, qb, (int)pg, pv, nrb, pn, np2, c->sumPpair, gn, ns2);
+                ApplyArgs ap{};
+                ap.zY = (double *)c->Y.p; ap.D = Dcur; ap.Dz = Doth; ap.wY = wY;
+                const unsigned npa = dl ? 0u : pg;
+                hipLaunchKernelGGL((lb_k_post_next<TV, R>), dim3(npa + nap + (unsigned)nrb + (unsigned)(np2 + ns2)),
+                                   dim3(LB_BLOCK), dl ? ((c->sumPv && pb.use_lds) ? ldsV : (size_t)0) : ldsNext, c->stream, qb, (int)npa,
+                                   pv, nrb, pn, np2, c->sumPpair, gn, ns2, ap, (int)nap);
             }
         };
         auto run = [&]() {
@@ -4308,7 +4372,7 @@ void lb_destroy(lb_ctx *c) {
                     &c->row_part, &c->seg_part, &c->segs, &c->segout, &c->hErows, &c->hEseg0, &c->hTrows,
                     &c->hTseg0, &c->parts, &c->lamsum, &c->order, &c->tmask, &c->Kq, &c->stZ, &c->Zs, &c->cnts,
                     &c->Vstage, &c->DDb, &c->segkap, &c->seglist, &c->vslack, &c->vage, &c->vvol, &c->rorder,
-                    &c->Yalt, &c->hubcnt, &c->hubpos, &c->hubown, &c->zpart, &c->Ysnap, &c->posb, &c->posp,
+                    &c->Yalt, &c->hubcnt, &c->hubpos, &c->hubown, &c->zpart, &c->Ysnap, &c->posb, &c->posp, &c->dbuf,
                     &c->b_off, &c->b_gid, &c->b_L, &c->b_C0, &c->b_Poff, &c->b_soff, &c->b_cg, &c->b_lab, &c->b_sig,
                     &c->b_pp, &c->b_rq, &c->b_dsq, &c->b_zY, &c->b_M, &c->b_pdiag, &c->b_ltot, &c->b_lviol, &c->b_rowv,
                     &c->b_parts, &c->b_lamsum, &c->b_tptr, &c->b_tidx, &c->b_R, &c->b_Vp, &c->b_Rp, &c->b_rgst};
@@ -4338,6 +4402,7 @@ int lb_set_option(lb_ctx *c, const char *name, int64_t value) {
     if (std::strcmp(name, "sweep_pair_segs") == 0) { c->sweep_pair_segs = value != 0; return 0; }
     if (std::strcmp(name, "sweep_post_merge") == 0) { c->sweep_post_merge = value != 0; return 0; }
     if (std::strcmp(name, "sweep_two") == 0) { c->sweep_two = value != 0; return 0; }
+    if (std::strcmp(name, "sweep_delta") == 0) { c->sweep_delta = value != 0; return 0; }
     if (std::strcmp(name, "sweep_seg_grid") == 0) {
         if (value < 0 || value > 4096) return fail(c, -1, "lb_set_option: sweep_seg_grid is 0..4096");
         c->sweep_seg_grid = value;
@@ -4849,6 +4914,9 @@ int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap
         if ((rc = dalloc(c, c->posp, sizeof(int32_t) * (size_t)n))) return rc;
         HIPCHK(c, hipMemsetAsync(c->posb.p, 0xFF, sizeof(int32_t) * (size_t)n, c->stream));   // -1: no batch yet
         c->sw_tag = 0;
+        const size_t wd = (size_t)LB_ZPART * R + (size_t)(c->sumPpair + c->sumPv) * R;
+        if ((rc = dalloc(c, c->dbuf, sizeof(double) * 2 * wd))) return rc;
+        HIPCHK(c, hipMemsetAsync(c->dbuf.p, 0, sizeof(double) * 2 * wd, c->stream));
         HIPCHK(c, hipMemsetAsync(c->zpart.p, 0, sizeof(double) * LB_ZPART * (size_t)R, c->stream));
     }
     std::vector<int32_t> o32(n);

@@ -178,9 +178,11 @@ def test_fused_sweeps_equal_five_kernel_form(dtype):
 @pytest.mark.parametrize("nbatch", [12, 64])
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_sweep_launch_forms_agree(dtype, nbatch):
-    """The sweep batch in two launches (the default: row steps with a snapshot of Y; then the aggregate increments,
-    the VCS post step, the next batch's pair rows and hub segments, the last two reading the snapshot plus the batch's
-    increments and, for a row just stepped, its new value from the staging rows), in three or four (option sweep_two
+    """The sweep batch in two launches (the default: row steps adding their increments to a per-batch delta buffer,
+    with a snapshot of Y; then the deltas applied to the aggregates, the VCS post step, the next batch's pair rows and
+    hub segments, the last two reading the snapshot plus the deltas and, for a row just stepped, its new value from the
+    staging rows -- above 128 rows per batch; below, and with option sweep_delta 0, each workgroup sums the batch's
+    increments in its own LDS instead), in three or four (option sweep_two
     0: pair rows with hub segments, row steps, then the aggregate increments with the VCS post step in one launch for
     batches of at most 256 rows) and in five (pair rows, hub segments, row steps, aggregate increments, VCS post step
     apart). Same per-wavefront arithmetic, so V agrees to the run-to-run rounding of the FP64 aggregate adds (1e-12
@@ -190,7 +192,7 @@ def test_sweep_launch_forms_agree(dtype, nbatch):
     inst, V = chung_lu_instance(n=5000, seed=31, k=16, P=16)
     rng = np.random.default_rng(32)
     calls = [np.stack([rng.permutation(inst.n) for _ in range(2)]), np.stack([rng.permutation(inst.n)])]
-    forms = {"two": {}, "three": {"sweep_two": 0},
+    forms = {"two": {}, "two_lds": {"sweep_delta": 0}, "three": {"sweep_two": 0},
              "five": {"sweep_two": 0, "sweep_pair_segs": 0, "sweep_post_merge": 0}}
     out = {}
     for name, opts in forms.items():
@@ -206,7 +208,7 @@ def test_sweep_launch_forms_agree(dtype, nbatch):
             Ws.append(eng.get_V())
         out[name] = Ws
         eng.close()
-    for name in ("two", "three"):
+    for name in ("two", "two_lds", "three"):
         for a, b in zip(out[name][1:], out["five"][1:]):
             assert np.abs(a - b).max() <= (1e-12 if dtype == "f64" else 1e-6), name
     Wo = O.sweeps_c(inst, out["two"][1], calls[1], nbatch, 0.05, damp=0.1)
