@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -4928,6 +4929,11 @@ int lb_sweeps(lb_ctx *c, int nsw, double rho, double rt, double damp, double kap
     for (int b = 0; b <= nbatch; b++) bpos[b] = (int64_t)b * q + std::min<int64_t>(b, rem);
     for (int sw = 0; sw < nsw; sw++) {
         const int64_t *ord = orders + (int64_t)sw * n;
+        if (c->sw_tag > INT32_MAX - nbatch - 1) {        // batch tags about to wrap: clear the row map and the delta
+            HIPCHK(c, hipMemsetAsync(c->posb.p, 0xFF, sizeof(int32_t) * (size_t)n, c->stream));   // buffers (whose
+            HIPCHK(c, hipMemsetAsync(c->dbuf.p, 0, c->dbuf.bytes, c->stream));                     // parity restarts)
+            c->sw_tag = 0;
+        }
         std::fill(seen.begin(), seen.end(), 0);
         for (int64_t p = 0; p < n; p++) {
             const int64_t i = ord[p];
